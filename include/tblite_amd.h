@@ -1,0 +1,224 @@
+/*
+ * tblite_amd.h -- C ABI of the MI355X GFN2-xTB singlepoint engine.
+ *
+ * The engine replaces the reference's device entry point
+ *
+ *   __device__ void xtb_singlepoint(const structure_type& mol,
+ *       const xtb_calculator& calc, wavefunction_type& wfn, const float accuracy,
+ *       float energy, float (&gradient)[MAX_NAT][3], float (&sigma)[3][3],
+ *       const int verbosity)                                   (src/xtb.cu:188-197)
+ *
+ * and the data types it takes: structure_type (src/structure.h:5-15),
+ * xtb_calculator (src/xtb.h:105-126) and wavefunction_type
+ * (src/wavefunction/type.h:81-99).  Those are fixed-size FP32 PODs living on one
+ * GPU thread; here they are opaque library-owned handles holding FP64 data of
+ * any size, and the call is made from the host.  The wavefunction keeps
+ * tblite's array layout (Fortran column-major, first index fastest):
+ *   qsh(nsh), qat(nat), dpat(3,nat), qpat(6,nat), emo(nao), focc(nao),
+ *   density(nao,nao), coeff(nao,nao).
+ * Coordinates are bohr, xyz(3,nat) with the three components of an atom
+ * contiguous, as in tblite's mol%xyz.
+ *
+ * All entry points return TBLITE_AMD_OK (0) or a negative error code, except
+ * constructors, which return NULL on failure; the message of the last failure
+ * on a context is available from tblite_amd_get_error().  No C++ exception
+ * crosses this boundary.  A context owns one HIP stream on one device; handles
+ * made from a context must not be used from several host threads at once.
+ */
+#ifndef TBLITE_AMD_H
+#define TBLITE_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- codes */
+#define TBLITE_AMD_OK 0
+#define TBLITE_AMD_ERR_ARGUMENT -1      /* invalid argument (null, size, element) */
+#define TBLITE_AMD_ERR_HIP -2           /* HIP runtime failure / no device / no kernel */
+#define TBLITE_AMD_ERR_NOT_CONVERGED -3 /* SCF reached max_iter */
+#define TBLITE_AMD_ERR_CHOLESKY -4      /* overlap matrix not positive definite */
+#define TBLITE_AMD_ERR_EIGEN -5         /* eigensolver did not converge */
+#define TBLITE_AMD_ERR_NAN -6           /* non-finite number in the SCF */
+#define TBLITE_AMD_ERR_MIXER -7         /* Broyden update singular */
+#define TBLITE_AMD_ERR_TOO_LARGE -8     /* molecule beyond the size the engine supports */
+
+/* per-molecule status bits written by the batched entry points */
+#define TBLITE_AMD_STATUS_CONVERGED 0
+#define TBLITE_AMD_STATUS_NOT_CONVERGED 1
+#define TBLITE_AMD_STATUS_CHOLESKY 2
+#define TBLITE_AMD_STATUS_EIGEN 4
+#define TBLITE_AMD_STATUS_NAN 8
+#define TBLITE_AMD_STATUS_MIXER 16
+#define TBLITE_AMD_STATUS_TOO_LARGE 32
+
+/* ------------------------------------------------------- parameter data */
+#define TBLITE_AMD_MAX_SHELL 3
+#define TBLITE_AMD_MAX_ELEM 16
+
+/* Raw GFN2 element record: the per-element entries of tblite's parameter file
+ * (the reference only declares the containers built from them:
+ * tb_hamiltonian src/h0.h:8-26, tb_repulsion src/xtb.h:70-90, tb_coulomb and
+ * damped_multipole src/coulomb.h:98-223, gfn_ncoord_type src/coulomb.h:90-95). */
+typedef struct {
+  int z;                                  /* atomic number */
+  int nsh;                                /* number of shells */
+  int ang[TBLITE_AMD_MAX_SHELL];          /* angular momentum of each shell */
+  int pqn[TBLITE_AMD_MAX_SHELL];          /* principal quantum number */
+  int ngauss[TBLITE_AMD_MAX_SHELL];       /* STO-nG expansion length */
+  double levels[TBLITE_AMD_MAX_SHELL];    /* atomic levels / eV */
+  double slater[TBLITE_AMD_MAX_SHELL];    /* Slater exponents / bohr^-1 */
+  double refocc[TBLITE_AMD_MAX_SHELL];    /* reference occupations */
+  double shpoly[TBLITE_AMD_MAX_SHELL];    /* polynomial enhancement of H0 */
+  double kcn[TBLITE_AMD_MAX_SHELL];       /* CN shift of the levels / eV */
+  double lgam[TBLITE_AMD_MAX_SHELL];      /* shell scaling of the hardness */
+  double gam;                             /* atomic hardness / Eh */
+  double gam3;                            /* third-order hubbard derivative / Eh */
+  double zeff;                            /* repulsion effective charge */
+  double arep;                            /* repulsion exponent */
+  double en;                              /* Pauling electronegativity */
+  double dkernel;                         /* on-site dipole XC kernel */
+  double qkernel;                         /* on-site quadrupole XC kernel */
+  double mprad;                           /* multipole damping base radius / bohr */
+  double mpvcn;                           /* multipole valence CN */
+  double rcov;                            /* covalent radius for the CN / bohr */
+  double atomic_rad;                      /* atomic radius for shpoly / bohr */
+} tblite_amd_element;
+
+/* Global GFN2 parameters plus the element table. */
+typedef struct {
+  double kshell[3][3];       /* H0 shell-pair scaling k_ll' */
+  double enscale;            /* electronegativity scaling k_EN */
+  double wexp;               /* exponent of the Slater-exponent term */
+  double kpol;               /* scaling for polarisation shells */
+  double rep_kexp;           /* repulsion distance exponent, heavy pairs */
+  double rep_kexp_light;     /* repulsion distance exponent, H/He pairs */
+  double rep_rexp;           /* repulsion 1/r exponent */
+  double rep_cutoff;         /* repulsion real-space cutoff / bohr */
+  double gexp;               /* ES2 kernel exponent */
+  double thirdorder_shell[3];/* ES3 shell scaling for s, p, d */
+  double dmp3, dmp5;         /* AES2 damping exponents */
+  double mp_shift;           /* AES2 radius shift */
+  double mp_kexp;            /* AES2 radius steepness */
+  double mp_rmax;            /* AES2 maximum radius / bohr */
+  double cn_ka, cn_kb;       /* CN counting-function steepness */
+  double cn_rshift;          /* CN second counting-function offset / bohr */
+  double cn_cutoff;          /* CN real-space cutoff / bohr */
+  int nelem;
+  tblite_amd_element elem[TBLITE_AMD_MAX_ELEM];
+} tblite_amd_param;
+
+/* Fill *param with the built-in GFN2 data for H, C, N, O and S. */
+int tblite_amd_param_default(tblite_amd_param* param);
+
+/* ------------------------------------------------------------- handles */
+typedef struct tblite_amd_context_s* tblite_amd_context;
+typedef struct tblite_amd_structure_s* tblite_amd_structure;
+typedef struct tblite_amd_calculator_s* tblite_amd_calculator;
+typedef struct tblite_amd_wavefunction_s* tblite_amd_wavefunction;
+
+/* Context: one device, one stream, last-error message.  device < 0 selects
+ * the current device.  No counterpart in the reference (it has no error
+ * channel, src/utils/gpu.h:6-14 aborts). */
+tblite_amd_context tblite_amd_new_context(int device);
+void tblite_amd_delete_context(tblite_amd_context* ctx);
+int tblite_amd_get_error(tblite_amd_context ctx, char* buffer, int buffer_size);
+
+/* Structure: replaces structure_type (src/structure.h:5-15). num(nat) atomic
+ * numbers, xyz(3,nat) in bohr. */
+tblite_amd_structure tblite_amd_new_structure(tblite_amd_context ctx, int nat, const int* num,
+                                              const double* xyz, double charge, int uhf);
+void tblite_amd_delete_structure(tblite_amd_structure* mol);
+int tblite_amd_get_structure_dims(tblite_amd_structure mol, int* nat, int* nsh, int* nao);
+
+/* Calculator: replaces xtb_calculator (src/xtb.h:105-126) together with its
+ * never-defined constructor new_hamiltonian (src/h0.h:80-86).  param may be
+ * NULL for the built-in GFN2 data. Defaults: accuracy 1.0, max_iter 250,
+ * mixer damping 0.4, electronic temperature 300 K. */
+tblite_amd_calculator tblite_amd_new_gfn2_calculator(tblite_amd_context ctx,
+                                                     const tblite_amd_param* param);
+void tblite_amd_delete_calculator(tblite_amd_calculator* calc);
+int tblite_amd_set_calculator_accuracy(tblite_amd_calculator calc, double accuracy);
+int tblite_amd_set_calculator_max_iter(tblite_amd_calculator calc, int max_iter);
+int tblite_amd_set_calculator_mixer_damping(tblite_amd_calculator calc, double damping);
+int tblite_amd_set_calculator_temperature(tblite_amd_calculator calc, double kelvin);
+
+/* Wavefunction: replaces wavefunction_type (src/wavefunction/type.h:81-99);
+ * sized from the structure on first use by a singlepoint. */
+tblite_amd_wavefunction tblite_amd_new_wavefunction(tblite_amd_context ctx);
+void tblite_amd_delete_wavefunction(tblite_amd_wavefunction* wfn);
+/* Copy one tblite wavefunction array to host memory; name is one of
+ * "qsh", "qat", "dpat", "qpat", "emo", "focc", "density", "coeff", "n0sh",
+ * "n0at".  *count receives the number of doubles; out may be NULL to query. */
+int tblite_amd_get_wavefunction_array(tblite_amd_wavefunction wfn, const char* name, double* out,
+                                      int* count);
+
+/* Singlepoint: replaces xtb_singlepoint (src/xtb.cu:188-434).  Energy in Eh;
+ * gradient/sigma are not computed (the reference never fills them either).
+ * accuracy <= 0 uses the calculator's accuracy.  iterations may be NULL. */
+int tblite_amd_xtb_singlepoint(tblite_amd_context ctx, tblite_amd_structure mol,
+                               tblite_amd_calculator calc, tblite_amd_wavefunction wfn,
+                               double accuracy, double* energy, int* iterations, int verbosity);
+
+/* Batched singlepoint over nmol independent molecules in one pass of the
+ * engine: the data-parallel form of xtb_singlepoint that the reference states
+ * as its goal (README.md:21) and runs as <<<1,1>>> (src/xtb.cu:495-497).
+ * wfns may be NULL, or an array of nmol handles (entries may be NULL).
+ * energies, iterations and status are host arrays of length nmol (iterations
+ * and status may be NULL).  Returns OK when every molecule converged,
+ * TBLITE_AMD_ERR_NOT_CONVERGED when any status is non-zero. */
+int tblite_amd_xtb_singlepoint_batch(tblite_amd_context ctx, int nmol,
+                                     const tblite_amd_structure* mols,
+                                     tblite_amd_calculator calc, tblite_amd_wavefunction* wfns,
+                                     double accuracy, double* energies, int* iterations,
+                                     int* status);
+
+/* Flat batched singlepoint: molecules given as one atom list.  mol_nat(nmol)
+ * atom counts; num(total) atomic numbers; xyz(3,total) bohr; charge(nmol) and
+ * uhf(nmol) may be NULL (neutral, closed shell).  All pointers are host
+ * pointers.  The engine keeps its device workspace between calls and grows it
+ * only when a batch needs more. */
+int tblite_amd_singlepoint_flat(tblite_amd_context ctx, tblite_amd_calculator calc, int nmol,
+                                const int* mol_nat, const int* num, const double* xyz,
+                                const double* charge, const int* uhf, double accuracy,
+                                double* energies, int* iterations, int* status);
+
+/* ------------------------------------------------- stage-level access
+ * Intermediate results of the last flat/batched call, for stage-by-stage
+ * parity checks.  All copy to host memory; imol indexes the last batch.
+ * Each names the reference routine whose output it returns. */
+
+/* Atom/shell/AO counts of molecule imol. */
+int tblite_amd_batch_dims(tblite_amd_context ctx, int imol, int* nat, int* nsh, int* nao);
+/* Coordination numbers, ncoord_get_cn (src/coulomb.cu:334-347): cn(nat). */
+int tblite_amd_batch_get_cn(tblite_amd_context ctx, int imol, double* cn);
+/* Neighbour list, generate (src/adjlist.cu:3-75): inl(nat), nnl(nat) and
+ * nlat(nnz) (0-based atom index); returns nnz, or < 0 on error.  nlat may be
+ * NULL to query nnz. */
+int tblite_amd_batch_get_adjlist(tblite_amd_context ctx, int imol, int* inl, int* nnl, int* nlat,
+                                 int capacity);
+/* Integrals, get_hamiltonian (src/h0.cu:292-502): overlap(nao,nao),
+ * dpint(3,nao,nao), qpint(6,nao,nao), hamiltonian(nao,nao) in tblite's
+ * layout; any pointer may be NULL. */
+int tblite_amd_batch_get_integrals(tblite_amd_context ctx, int imol, double* overlap,
+                                   double* dpint, double* qpint, double* hamiltonian);
+/* Self energies, get_selfenergy (src/h0.cu:125-199): selfenergy(nsh). */
+int tblite_amd_batch_get_selfenergy(tblite_amd_context ctx, int imol, double* selfenergy);
+/* Charges and multipoles after the last SCF step: qsh(nsh), qat(nat),
+ * dpat(3,nat), qpat(6,nat), emo(nao), focc(nao); any pointer may be NULL. */
+int tblite_amd_batch_get_charges(tblite_amd_context ctx, int imol, double* qsh, double* qat,
+                                 double* dpat, double* qpat, double* emo, double* focc);
+/* Energy decomposition: erep, eelec (band + entropy + ES2 + AES2 + ES3). */
+int tblite_amd_batch_get_energy_parts(tblite_amd_context ctx, int imol, double* erep,
+                                      double* eelec);
+
+/* Milliseconds spent in each engine phase during the last batched call:
+ * times[0] setup (CN, repulsion, Coulomb matrices), [1] neighbour list,
+ * [2] integrals/H0, [3] SCF, [4] total.  n = number of entries wanted. */
+int tblite_amd_last_timings(tblite_amd_context ctx, double* times, int n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TBLITE_AMD_H */

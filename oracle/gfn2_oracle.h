@@ -1,0 +1,94 @@
+/*
+ * TEST INFRASTRUCTURE -- not part of the engine.
+ *
+ * CPU FP64 restatement of tblite's GFN2-xTB energy singlepoint, following the
+ * upstream Fortran that the reference embeds in its comments (file:line
+ * citations on each function in gfn2_oracle.c).  Only tests/, the smoke check
+ * and the CPU-baseline leg of bench.py load it, and only as the checker.
+ */
+#ifndef GFN2_ORACLE_H
+#define GFN2_ORACLE_H
+
+#include "tblite_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct or_calc_s or_calc;
+
+/* Build the per-element basis and parameter tables from raw parameters. */
+or_calc* or_new_calc(const tblite_amd_param* par);
+void or_free_calc(or_calc* calc);
+
+/* nsh/nao of a molecule; returns 0 or -1 for an unknown element. */
+int or_dims(const or_calc* calc, int nat, const int* num, int* nsh, int* nao);
+
+typedef struct {
+  double energy;  /* total energy */
+  double erep;    /* repulsion */
+  double eelec;   /* electronic: band + entropy + ES2 + AES2 + ES3 */
+  int iterations;
+  int converged;
+  int status;     /* TBLITE_AMD_STATUS_* bits */
+  /* optional caller-owned outputs (may be NULL) */
+  double* qsh;    /* nsh */
+  double* qat;    /* nat */
+  double* dpat;   /* 3*nat */
+  double* qpat;   /* 6*nat */
+  double* emo;    /* nao */
+  double* focc;   /* nao */
+} or_result;
+
+/* Full singlepoint (xtb_singlepoint, src/xtb.cu:188-434). kt in Eh. */
+int or_singlepoint(const or_calc* calc, int nat, const int* num, const double* xyz, double charge,
+                   int uhf, double accuracy, double kt, int max_iter, double damping,
+                   or_result* res);
+
+/* Independent molecules over an OpenMP pool of nthreads (0: all cores).
+ * Molecule m has mol_nat[m] atoms starting at atom offset sum(mol_nat[:m]). */
+int or_singlepoint_many(const or_calc* calc, int nmol, const int* mol_nat, const int* num,
+                        const double* xyz, double accuracy, double kt, int max_iter,
+                        double damping, int nthreads, double* energies, int* iterations,
+                        int* status);
+
+/* ---- stage functions, exported for stage-level checks ---- */
+void or_multipole_3d(const double rpj[3], const double rpi[3], double aj, double ai,
+                     const int lj[3], const int li[3], const double* s1d, double* s3d,
+                     double d3d[3], double q3d[6]);
+void or_overlap_1d_table(double eab, int lmax, double* s1d);
+/* cart -> spherical for one shell pair; cart(mlao_j, mlao_i), sphr(msao_j, msao_i) */
+void or_transform0(int lj, int li, const double* cart, double* sphr);
+/* Contracted multipole block: S(msao_j, msao_i), D(3, msao_j, msao_i),
+ * Q(6, msao_j, msao_i) (Fortran order). Species index into the calculator,
+ * shell index within the species. */
+int or_multipole_cgto(const or_calc* calc, int zj, int jsh, int zi, int ish, double r2,
+                      const double vec[3], double intcut, double* S, double* D, double* Q);
+int or_ncoord(const or_calc* calc, int nat, const int* num, const double* xyz, double* cn);
+int or_repulsion(const or_calc* calc, int nat, const int* num, const double* xyz, double* erep);
+double or_get_cutoff(const or_calc* calc, int nat, const int* num, double accuracy);
+/* Half neighbour list; returns nnz (or -(needed) when capacity is too small). */
+int or_adjlist(int nat, const double* xyz, double cutoff, int* inl, int* nnl, int* nlat,
+               int capacity);
+int or_selfenergy(const or_calc* calc, int nat, const int* num, const double* cn, double* se);
+int or_hamiltonian(const or_calc* calc, int nat, const int* num, const double* xyz,
+                   double accuracy, double* S, double* D, double* Q, double* H);
+int or_coulomb_matrices(const or_calc* calc, int nat, const int* num, const double* xyz,
+                        double* amat, double* amat_sd, double* amat_dd, double* amat_sq,
+                        double* mrad);
+/* Generalized eigenproblem F C = S C e (dsygvd itype 1, ascending e). */
+int or_sygvd(int n, const double* F, const double* S, double* e, double* C);
+void or_fermi_filling(double nel, double kt, int n, const double* emo, int* homo, double* focc,
+                      double* e_fermi);
+double or_entropy(int n, const double* focc, double kt);
+/* Electrostatic energy and potential for given charges / multipoles:
+ * e(nat) atom-resolved energies; vsh(nsh), vat(nat), vdp(3,nat), vqp(6,nat). */
+int or_coulomb_eval(const or_calc* calc, int nat, const int* num, const double* xyz,
+                    const double* qsh, const double* dpat, const double* qpat, double* e,
+                    double* vsh, double* vat, double* vdp, double* vqp);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,269 @@
+"""TEST INFRASTRUCTURE -- ctypes binding of the CPU oracle (gfn2_oracle.c).
+
+Only tests/, the smoke check and bench.py's CPU-baseline leg use this, and only
+as the checker.  Builds the oracle with its Makefile when the library is
+missing (gcc is available wherever the tests run).
+"""
+import ctypes as C
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB = os.path.join(HERE, "libgfn2_oracle.so")
+sys.path.insert(0, os.path.join(ROOT, "tblite-cuda_amd"))
+from tblite_amd import Param  # noqa: E402  (struct layout of include/tblite_amd.h)
+
+KB_EH = 3.166808578545117e-06
+
+
+def build(force=False):
+    if force or not os.path.exists(LIB):
+        subprocess.run(["make", "-s", "-C", HERE] + (["-B"] if force else []), check=True)
+    return LIB
+
+
+class Result(C.Structure):
+    _fields_ = [
+        ("energy", C.c_double), ("erep", C.c_double), ("eelec", C.c_double),
+        ("iterations", C.c_int), ("converged", C.c_int), ("status", C.c_int),
+        ("qsh", C.POINTER(C.c_double)), ("qat", C.POINTER(C.c_double)),
+        ("dpat", C.POINTER(C.c_double)), ("qpat", C.POINTER(C.c_double)),
+        ("emo", C.POINTER(C.c_double)), ("focc", C.POINTER(C.c_double)),
+    ]
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int)) if a is not None else None
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = C.CDLL(LIB)
+        L.or_new_calc.restype = C.c_void_p
+        L.or_new_calc.argtypes = [C.POINTER(Param)]
+        L.or_free_calc.argtypes = [C.c_void_p]
+        L.or_dims.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.or_singlepoint.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                     C.c_double, C.c_int, C.c_double, C.c_double, C.c_int, C.c_double,
+                                     C.POINTER(Result)]
+        L.or_singlepoint_many.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                          C.POINTER(C.c_double), C.c_double, C.c_double, C.c_int,
+                                          C.c_double, C.c_int, C.POINTER(C.c_double),
+                                          C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.or_multipole_3d.argtypes = [C.POINTER(C.c_double)] * 2 + [C.c_double] * 2 + \
+            [C.POINTER(C.c_int)] * 2 + [C.POINTER(C.c_double)] * 4
+        L.or_transform0.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.or_multipole_cgto.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                        C.POINTER(C.c_double), C.c_double] + [C.POINTER(C.c_double)] * 3
+        L.or_ncoord.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                C.POINTER(C.c_double)]
+        L.or_repulsion.argtypes = L.or_ncoord.argtypes
+        L.or_get_cutoff.restype = C.c_double
+        L.or_get_cutoff.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.c_double]
+        L.or_adjlist.argtypes = [C.c_int, C.POINTER(C.c_double), C.c_double, C.POINTER(C.c_int),
+                                 C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int]
+        L.or_selfenergy.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double)]
+        L.or_hamiltonian.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                     C.c_double] + [C.POINTER(C.c_double)] * 4
+        L.or_coulomb_matrices.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int),
+                                          C.POINTER(C.c_double)] + [C.POINTER(C.c_double)] * 5
+        L.or_sygvd.argtypes = [C.c_int] + [C.POINTER(C.c_double)] * 4
+        L.or_fermi_filling.argtypes = [C.c_double, C.c_double, C.c_int, C.POINTER(C.c_double),
+                                       C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.or_entropy.restype = C.c_double
+        L.or_entropy.argtypes = [C.c_int, C.POINTER(C.c_double), C.c_double]
+        L.or_coulomb_eval.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)] + \
+            [C.POINTER(C.c_double)] * 8
+        _lib = L
+    return _lib
+
+
+class Oracle:
+    def __init__(self, param=None):
+        self.L = lib()
+        self.calc = self.L.or_new_calc(C.byref(param) if param is not None else None)
+        if not self.calc:
+            raise RuntimeError("oracle calculator construction failed")
+
+    def __del__(self):
+        try:
+            self.L.or_free_calc(self.calc)
+        except Exception:
+            pass
+
+    @staticmethod
+    def _mol(num, xyz):
+        return (np.ascontiguousarray(num, dtype=np.int32),
+                np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3))
+
+    def dims(self, num):
+        num = np.ascontiguousarray(num, dtype=np.int32)
+        nsh, nao = C.c_int(), C.c_int()
+        if self.L.or_dims(self.calc, len(num), _ip(num), C.byref(nsh), C.byref(nao)):
+            raise ValueError("unknown element")
+        return len(num), nsh.value, nao.value
+
+    def singlepoint(self, num, xyz, charge=0.0, uhf=0, accuracy=1.0, temperature=300.0,
+                    max_iter=250, damping=0.4):
+        num, xyz = self._mol(num, xyz)
+        nat, nsh, nao = self.dims(num)
+        out = dict(qsh=np.zeros(nsh), qat=np.zeros(nat), dpat=np.zeros((nat, 3)),
+                   qpat=np.zeros((nat, 6)), emo=np.zeros(nao), focc=np.zeros(nao))
+        r = Result()
+        for k in out:
+            setattr(r, k, _dp(out[k]))
+        rc = self.L.or_singlepoint(self.calc, nat, _ip(num), _dp(xyz), charge, uhf, accuracy,
+                                   temperature * KB_EH, max_iter, damping, C.byref(r))
+        out.update(energy=r.energy, erep=r.erep, eelec=r.eelec, iterations=r.iterations,
+                   converged=bool(r.converged), status=r.status, rc=rc)
+        return out
+
+    def singlepoint_many(self, mols, accuracy=1.0, temperature=300.0, max_iter=250,
+                         damping=0.4, nthreads=0):
+        mol_nat = np.array([len(m[0]) for m in mols], dtype=np.int32)
+        num = np.concatenate([m[0] for m in mols]).astype(np.int32)
+        xyz = np.ascontiguousarray(np.concatenate([m[1] for m in mols]), dtype=np.float64)
+        n = len(mols)
+        e = np.zeros(n)
+        it = np.zeros(n, dtype=np.int32)
+        st = np.zeros(n, dtype=np.int32)
+        self.L.or_singlepoint_many(self.calc, n, _ip(mol_nat), _ip(num), _dp(xyz), accuracy,
+                                   temperature * KB_EH, max_iter, damping, nthreads, _dp(e),
+                                   _ip(it), _ip(st))
+        return e, it, st
+
+    def cn(self, num, xyz):
+        num, xyz = self._mol(num, xyz)
+        out = np.zeros(len(num))
+        self.L.or_ncoord(self.calc, len(num), _ip(num), _dp(xyz), _dp(out))
+        return out
+
+    def repulsion(self, num, xyz):
+        num, xyz = self._mol(num, xyz)
+        out = np.zeros(len(num))
+        self.L.or_repulsion(self.calc, len(num), _ip(num), _dp(xyz), _dp(out))
+        return out
+
+    def cutoff(self, num, accuracy=1.0):
+        num = np.ascontiguousarray(num, dtype=np.int32)
+        return self.L.or_get_cutoff(self.calc, len(num), _ip(num), accuracy)
+
+    def adjlist(self, xyz, cutoff):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        nat = len(xyz)
+        inl = np.zeros(nat, dtype=np.int32)
+        nnl = np.zeros(nat, dtype=np.int32)
+        cap = nat * (nat + 1) // 2 + 1
+        nlat = np.zeros(cap, dtype=np.int32)
+        nnz = self.L.or_adjlist(nat, _dp(xyz), cutoff, _ip(inl), _ip(nnl), _ip(nlat), cap)
+        return inl, nnl, nlat[:nnz]
+
+    def selfenergy(self, num, cn):
+        num = np.ascontiguousarray(num, dtype=np.int32)
+        _, nsh, _ = self.dims(num)
+        out = np.zeros(nsh)
+        self.L.or_selfenergy(self.calc, len(num), _ip(num), _dp(np.ascontiguousarray(cn, dtype=np.float64)), _dp(out))
+        return out
+
+    def hamiltonian(self, num, xyz, accuracy=1.0):
+        num, xyz = self._mol(num, xyz)
+        _, _, nao = self.dims(num)
+        S = np.zeros((nao, nao))
+        D = np.zeros((nao, nao, 3))
+        Q = np.zeros((nao, nao, 6))
+        H = np.zeros((nao, nao))
+        self.L.or_hamiltonian(self.calc, len(num), _ip(num), _dp(xyz), accuracy, _dp(S), _dp(D), _dp(Q), _dp(H))
+        return S, D, Q, H
+
+    def coulomb_matrices(self, num, xyz):
+        num, xyz = self._mol(num, xyz)
+        nat, nsh, _ = self.dims(num)
+        amat = np.zeros((nsh, nsh))
+        sd = np.zeros((nat, nat, 3))
+        dd = np.zeros((nat, 3, nat, 3))
+        sq = np.zeros((nat, nat, 6))
+        mrad = np.zeros(nat)
+        self.L.or_coulomb_matrices(self.calc, nat, _ip(num), _dp(xyz), _dp(amat), _dp(sd), _dp(dd), _dp(sq), _dp(mrad))
+        return amat, sd, dd, sq, mrad
+
+    def coulomb_eval(self, num, xyz, qsh, dpat, qpat):
+        num, xyz = self._mol(num, xyz)
+        nat, nsh, _ = self.dims(num)
+        e = np.zeros(nat)
+        vsh = np.zeros(nsh)
+        vat = np.zeros(nat)
+        vdp = np.zeros((nat, 3))
+        vqp = np.zeros((nat, 6))
+        f = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        self.L.or_coulomb_eval(self.calc, nat, _ip(num), _dp(xyz), _dp(f(qsh)), _dp(f(dpat)), _dp(f(qpat)),
+                               _dp(e), _dp(vsh), _dp(vat), _dp(vdp), _dp(vqp))
+        return e, vsh, vat, vdp, vqp
+
+    def multipole_cgto(self, zj, jsh, zi, ish, vec, intcut=25.0):
+        vec = np.ascontiguousarray(vec, dtype=np.float64)
+        S = np.zeros(25)
+        D = np.zeros(75)
+        Q = np.zeros(150)
+        self.L.or_multipole_cgto(self.calc, zj, jsh, zi, ish, float(vec @ vec), _dp(vec), intcut,
+                                 _dp(S), _dp(D), _dp(Q))
+        return S, D, Q
+
+
+def sygvd(F, S):
+    F = np.asfortranarray(F, dtype=np.float64)
+    S = np.asfortranarray(S, dtype=np.float64)
+    n = F.shape[0]
+    w = np.zeros(n)
+    Cm = np.zeros((n, n), order="F")
+    rc = lib().or_sygvd(n, F.ctypes.data_as(C.POINTER(C.c_double)), S.ctypes.data_as(C.POINTER(C.c_double)),
+                        _dp(w), Cm.ctypes.data_as(C.POINTER(C.c_double)))
+    return rc, w, Cm
+
+
+def multipole_3d(rpj, rpi, aj, ai, lj, li, s1d):
+    f = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    g = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    s = np.zeros(1)
+    d = np.zeros(3)
+    q = np.zeros(6)
+    s1 = np.zeros(9)
+    s1[:len(s1d)] = s1d
+    lib().or_multipole_3d(_dp(f(rpj)), _dp(f(rpi)), aj, ai, _ip(g(lj)), _ip(g(li)), _dp(s1), _dp(s), _dp(d), _dp(q))
+    return s[0], d, q
+
+
+def transform0(lj, li, cart):
+    cart = np.asfortranarray(cart, dtype=np.float64)
+    sj, si = 2 * lj + 1, 2 * li + 1
+    sph = np.zeros((sj, si), order="F")
+    lib().or_transform0(lj, li, cart.ctypes.data_as(C.POINTER(C.c_double)),
+                        sph.ctypes.data_as(C.POINTER(C.c_double)))
+    return sph
+
+
+def fermi_filling(nel, kt, emo):
+    emo = np.ascontiguousarray(emo, dtype=np.float64)
+    occ = np.zeros(len(emo))
+    homo = C.c_int()
+    ef = C.c_double()
+    lib().or_fermi_filling(nel, kt, len(emo), _dp(emo), C.byref(homo), _dp(occ), C.byref(ef))
+    return homo.value, occ, ef.value
+
+
+def entropy(occ, kt):
+    occ = np.ascontiguousarray(occ, dtype=np.float64)
+    return lib().or_entropy(len(occ), _dp(occ), kt)

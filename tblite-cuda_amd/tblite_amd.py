@@ -1,0 +1,253 @@
+"""ctypes binding of the engine's C ABI (include/tblite_amd.h).
+
+Loads libtblite_amd.so built in this directory (see build.py).  The binding
+raises if the library is missing: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBNAME = os.path.join(HERE, "libtblite_amd.so")
+
+MAX_SHELL = 3
+MAX_ELEM = 16
+
+STATUS = {
+    0: "converged",
+    1: "not converged",
+    2: "cholesky",
+    4: "eigen",
+    8: "nan",
+    16: "mixer",
+    32: "too large",
+}
+
+
+class Element(C.Structure):
+    _fields_ = [
+        ("z", C.c_int), ("nsh", C.c_int),
+        ("ang", C.c_int * MAX_SHELL), ("pqn", C.c_int * MAX_SHELL), ("ngauss", C.c_int * MAX_SHELL),
+        ("levels", C.c_double * MAX_SHELL), ("slater", C.c_double * MAX_SHELL),
+        ("refocc", C.c_double * MAX_SHELL), ("shpoly", C.c_double * MAX_SHELL),
+        ("kcn", C.c_double * MAX_SHELL), ("lgam", C.c_double * MAX_SHELL),
+        ("gam", C.c_double), ("gam3", C.c_double), ("zeff", C.c_double), ("arep", C.c_double),
+        ("en", C.c_double), ("dkernel", C.c_double), ("qkernel", C.c_double),
+        ("mprad", C.c_double), ("mpvcn", C.c_double), ("rcov", C.c_double),
+        ("atomic_rad", C.c_double),
+    ]
+
+
+class Param(C.Structure):
+    _fields_ = [
+        ("kshell", (C.c_double * 3) * 3), ("enscale", C.c_double), ("wexp", C.c_double),
+        ("kpol", C.c_double), ("rep_kexp", C.c_double), ("rep_kexp_light", C.c_double),
+        ("rep_rexp", C.c_double), ("rep_cutoff", C.c_double), ("gexp", C.c_double),
+        ("thirdorder_shell", C.c_double * 3), ("dmp3", C.c_double), ("dmp5", C.c_double),
+        ("mp_shift", C.c_double), ("mp_kexp", C.c_double), ("mp_rmax", C.c_double),
+        ("cn_ka", C.c_double), ("cn_kb", C.c_double), ("cn_rshift", C.c_double),
+        ("cn_cutoff", C.c_double), ("nelem", C.c_int), ("elem", Element * MAX_ELEM),
+    ]
+
+
+_P = C.c_void_p
+_I = C.c_int
+_D = C.c_double
+_IP = C.POINTER(C.c_int)
+_DP = C.POINTER(C.c_double)
+
+_PROTOS = {
+    "tblite_amd_param_default": (_I, [C.POINTER(Param)]),
+    "tblite_amd_new_context": (_P, [_I]),
+    "tblite_amd_delete_context": (None, [C.POINTER(_P)]),
+    "tblite_amd_get_error": (_I, [_P, C.c_char_p, _I]),
+    "tblite_amd_new_structure": (_P, [_P, _I, _IP, _DP, _D, _I]),
+    "tblite_amd_delete_structure": (None, [C.POINTER(_P)]),
+    "tblite_amd_get_structure_dims": (_I, [_P, _IP, _IP, _IP]),
+    "tblite_amd_new_gfn2_calculator": (_P, [_P, C.POINTER(Param)]),
+    "tblite_amd_delete_calculator": (None, [C.POINTER(_P)]),
+    "tblite_amd_set_calculator_accuracy": (_I, [_P, _D]),
+    "tblite_amd_set_calculator_max_iter": (_I, [_P, _I]),
+    "tblite_amd_set_calculator_mixer_damping": (_I, [_P, _D]),
+    "tblite_amd_set_calculator_temperature": (_I, [_P, _D]),
+    "tblite_amd_new_wavefunction": (_P, [_P]),
+    "tblite_amd_delete_wavefunction": (None, [C.POINTER(_P)]),
+    "tblite_amd_get_wavefunction_array": (_I, [_P, C.c_char_p, _DP, _IP]),
+    "tblite_amd_xtb_singlepoint": (_I, [_P, _P, _P, _P, _D, _DP, _IP, _I]),
+    "tblite_amd_xtb_singlepoint_batch": (_I, [_P, _I, C.POINTER(_P), _P, C.POINTER(_P), _D, _DP, _IP, _IP]),
+    "tblite_amd_singlepoint_flat": (_I, [_P, _P, _I, _IP, _IP, _DP, _DP, _IP, _D, _DP, _IP, _IP]),
+    "tblite_amd_batch_dims": (_I, [_P, _I, _IP, _IP, _IP]),
+    "tblite_amd_batch_get_cn": (_I, [_P, _I, _DP]),
+    "tblite_amd_batch_get_adjlist": (_I, [_P, _I, _IP, _IP, _IP, _I]),
+    "tblite_amd_batch_get_integrals": (_I, [_P, _I, _DP, _DP, _DP, _DP]),
+    "tblite_amd_batch_get_selfenergy": (_I, [_P, _I, _DP]),
+    "tblite_amd_batch_get_charges": (_I, [_P, _I, _DP, _DP, _DP, _DP, _DP, _DP]),
+    "tblite_amd_batch_get_energy_parts": (_I, [_P, _I, _DP, _DP]),
+    "tblite_amd_last_timings": (_I, [_P, _DP, _I]),
+}
+
+EXPORTED = sorted(_PROTOS)
+
+_lib = None
+
+
+def load(path=None):
+    """Load and type the engine library; raises OSError if it is not built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIBNAME
+    if not os.path.exists(p):
+        raise OSError(f"{p} not built: run `python __graft_entry__.py build` (no CPU fallback)")
+    lib = C.CDLL(p)
+    for name, (res, args) in _PROTOS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _ip(a):
+    return a.ctypes.data_as(_IP) if a is not None else None
+
+
+def _dp(a):
+    return a.ctypes.data_as(_DP) if a is not None else None
+
+
+def default_param():
+    lib = load()
+    p = Param()
+    lib.tblite_amd_param_default(C.byref(p))
+    return p
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+class Engine:
+    """One device context with one GFN2 calculator."""
+
+    def __init__(self, device=0, param=None, accuracy=1.0, max_iter=250, damping=0.4,
+                 temperature=300.0):
+        self.lib = load()
+        self.ctx = self.lib.tblite_amd_new_context(device)
+        if not self.ctx:
+            raise EngineError("could not create a tblite_amd context (no HIP device?)")
+        self.calc = self.lib.tblite_amd_new_gfn2_calculator(
+            self.ctx, C.byref(param) if param is not None else None)
+        if not self.calc:
+            raise EngineError(self.error())
+        self._check(self.lib.tblite_amd_set_calculator_accuracy(self.calc, accuracy))
+        self._check(self.lib.tblite_amd_set_calculator_max_iter(self.calc, max_iter))
+        self._check(self.lib.tblite_amd_set_calculator_mixer_damping(self.calc, damping))
+        self._check(self.lib.tblite_amd_set_calculator_temperature(self.calc, temperature))
+        self.accuracy = accuracy
+
+    def error(self):
+        buf = C.create_string_buffer(1024)
+        self.lib.tblite_amd_get_error(self.ctx, buf, 1024)
+        return buf.value.decode(errors="replace")
+
+    def _check(self, rc, allow=()):
+        if rc != 0 and rc not in allow:
+            raise EngineError(f"tblite_amd error {rc}: {self.error()}")
+        return rc
+
+    def close(self):
+        if getattr(self, "calc", None):
+            h = C.c_void_p(self.calc)
+            self.lib.tblite_amd_delete_calculator(C.byref(h))
+            self.calc = None
+        if getattr(self, "ctx", None):
+            h = C.c_void_p(self.ctx)
+            self.lib.tblite_amd_delete_context(C.byref(h))
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def singlepoint_flat(self, mol_nat, num, xyz, charge=None, uhf=None, accuracy=None):
+        mol_nat = np.ascontiguousarray(mol_nat, dtype=np.int32)
+        num = np.ascontiguousarray(num, dtype=np.int32)
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        nmol = len(mol_nat)
+        energies = np.zeros(nmol)
+        iters = np.zeros(nmol, dtype=np.int32)
+        status = np.zeros(nmol, dtype=np.int32)
+        ch = None if charge is None else np.ascontiguousarray(charge, dtype=np.float64)
+        uh = None if uhf is None else np.ascontiguousarray(uhf, dtype=np.int32)
+        rc = self.lib.tblite_amd_singlepoint_flat(
+            self.ctx, self.calc, nmol, _ip(mol_nat), _ip(num), _dp(xyz), _dp(ch), _ip(uh),
+            self.accuracy if accuracy is None else accuracy, _dp(energies), _ip(iters), _ip(status))
+        self._check(rc, allow=(-3,))
+        return energies, iters, status
+
+    def dims(self, imol):
+        nat, nsh, nao = C.c_int(), C.c_int(), C.c_int()
+        self._check(self.lib.tblite_amd_batch_dims(self.ctx, imol, C.byref(nat), C.byref(nsh), C.byref(nao)))
+        return nat.value, nsh.value, nao.value
+
+    def cn(self, imol):
+        nat, _, _ = self.dims(imol)
+        out = np.zeros(nat)
+        self._check(self.lib.tblite_amd_batch_get_cn(self.ctx, imol, _dp(out)))
+        return out
+
+    def selfenergy(self, imol):
+        _, nsh, _ = self.dims(imol)
+        out = np.zeros(nsh)
+        self._check(self.lib.tblite_amd_batch_get_selfenergy(self.ctx, imol, _dp(out)))
+        return out
+
+    def adjlist(self, imol):
+        nat, _, _ = self.dims(imol)
+        inl = np.zeros(nat, dtype=np.int32)
+        nnl = np.zeros(nat, dtype=np.int32)
+        nnz = self.lib.tblite_amd_batch_get_adjlist(self.ctx, imol, _ip(inl), _ip(nnl), None, 0)
+        if nnz < 0:
+            raise EngineError(self.error())
+        nlat = np.zeros(max(nnz, 1), dtype=np.int32)
+        nnz = self._check_nonneg(self.lib.tblite_amd_batch_get_adjlist(
+            self.ctx, imol, _ip(inl), _ip(nnl), _ip(nlat), len(nlat)))
+        return inl, nnl, nlat[:nnz]
+
+    def _check_nonneg(self, rc):
+        if rc < 0:
+            raise EngineError(f"tblite_amd error {rc}: {self.error()}")
+        return rc
+
+    def integrals(self, imol):
+        _, _, nao = self.dims(imol)
+        S = np.zeros((nao, nao))
+        D = np.zeros((nao, nao, 3))
+        Q = np.zeros((nao, nao, 6))
+        H = np.zeros((nao, nao))
+        self._check(self.lib.tblite_amd_batch_get_integrals(self.ctx, imol, _dp(S), _dp(D), _dp(Q), _dp(H)))
+        return S, D, Q, H
+
+    def charges(self, imol):
+        nat, nsh, nao = self.dims(imol)
+        out = dict(qsh=np.zeros(nsh), qat=np.zeros(nat), dpat=np.zeros((nat, 3)),
+                   qpat=np.zeros((nat, 6)), emo=np.zeros(nao), focc=np.zeros(nao))
+        self._check(self.lib.tblite_amd_batch_get_charges(
+            self.ctx, imol, _dp(out["qsh"]), _dp(out["qat"]), _dp(out["dpat"]), _dp(out["qpat"]),
+            _dp(out["emo"]), _dp(out["focc"])))
+        return out
+
+    def energy_parts(self, imol):
+        erep, eelec = C.c_double(), C.c_double()
+        self._check(self.lib.tblite_amd_batch_get_energy_parts(self.ctx, imol, C.byref(erep), C.byref(eelec)))
+        return erep.value, eelec.value
+
+    def timings(self):
+        t = np.zeros(5)
+        self._check(self.lib.tblite_amd_last_timings(self.ctx, _dp(t), 5))
+        return t

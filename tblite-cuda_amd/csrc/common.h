@@ -1,0 +1,197 @@
+// Device-side data layout shared by the engine's kernels and its host code.
+//
+// A batch is a set of independent molecules.  Everything per atom, shell and
+// atomic orbital lives in flat arrays indexed by a global position; each
+// molecule knows its offsets.  Matrices are stored per molecule, n x n row
+// blocks, component-planar for the multipole integrals:
+//   S[moff + i*n + j]            overlap(j, i)        (tblite layout, j fastest)
+//   H[moff + i*n + j]            hamiltonian(j, i)
+//   D[3*moff + k*n*n + i*n + j]  dpint(k, j, i)  operator centred on atom(i)
+//   Q[6*moff + k*n*n + i*n + j]  qpint(k, j, i)
+// with quadrupole components ordered xx, xy, yy, xz, yz, zz (tblite order).
+#pragma once
+
+#include <stdint.h>
+
+#define TA_MAX_ELEM 16
+#define TA_MAX_SHELL 3
+#define TA_MAXG 6
+
+struct DevElem {
+  int z, nsh;
+  int ang[TA_MAX_SHELL];
+  int nprim[TA_MAX_SHELL];
+  double alpha[TA_MAX_SHELL][TA_MAXG];
+  double coeff[TA_MAX_SHELL][TA_MAXG];
+  double selfenergy[TA_MAX_SHELL];  // Eh
+  double kcn[TA_MAX_SHELL];         // Eh
+  double hardness[TA_MAX_SHELL];    // Eh, gam * lgam
+  double hubbard_derivs[TA_MAX_SHELL];
+  double refocc[TA_MAX_SHELL];
+  double shpoly[TA_MAX_SHELL];
+  double rcov, atomic_rad, zeff, arep, dkernel, qkernel, mprad, mpvcn, en;
+};
+
+struct DevParams {
+  int nelem;
+  DevElem el[TA_MAX_ELEM];
+  // hscale[ei][ej][ish][jsh]
+  double hscale[TA_MAX_ELEM][TA_MAX_ELEM][TA_MAX_SHELL][TA_MAX_SHELL];
+  double rep_kexp, rep_kexp_light, rep_rexp, rep_cutoff;
+  double gexp;
+  double dmp3, dmp5, mp_shift, mp_kexp, mp_rmax;
+  double cn_ka, cn_kb, cn_rshift, cn_cutoff;
+};
+
+// Pointers into one batch's device buffers (passed by value to kernels).
+struct BatchDev {
+  int nmol;
+  int max_iter;
+  int mix_memory;   // Broyden memory (= max_iter)
+  double damping;
+  double kt;
+  double accuracy;
+  double intcut;    // primitive-pair screening exponent
+  const DevParams* par;
+
+  // per molecule
+  const int* mol_atom_off;
+  const int* mol_nat;
+  const int* mol_sh_off;
+  const int* mol_nsh;
+  const int* mol_ao_off;
+  const int* mol_nao;
+  const int64_t* mol_mat_off;     // sum of previous n^2
+  const int64_t* mol_nl_off;      // sum of previous nat(nat+1)/2
+  const int64_t* mol_gam_off;     // sum of previous nsh^2
+  const int64_t* mol_pair_off;    // sum of previous nat^2
+  const int64_t* mol_mix_off;     // sum of previous ndim
+  const int64_t* mol_mixmem_off;  // sum of previous ndim * mix_memory
+  const int64_t* mol_mixa_off;    // sum of previous mix_memory^2 (per molecule constant)
+  const double* mol_charge;
+  const int* mol_uhf;
+  const double* mol_cutoff;       // neighbour-list cutoff (depends on species present)
+
+  // per atom
+  const int* at_el;               // element slot
+  const int* at_ish;              // local offset of the atom's first shell
+  const int* at_iao;              // local offset of the atom's first AO
+  const double* xyz;              // 3 per atom, bohr
+  double* cn;
+  double* mrad;
+  double* erep;                   // atom-resolved repulsion energy
+
+  // per shell
+  const int* sh_at;               // local atom index
+  const int* sh_ish;              // shell index within the element
+  const int* sh_iao;              // local AO offset
+  double* selfenergy;
+  double* n0sh;
+
+  // per AO
+  const int* ao_at;               // local atom index
+  const int* ao_sh;               // local shell index
+
+  // neighbour list (half list, jat <= iat, 0-based local atom index)
+  int* nl_inl;                    // per atom, local offset within the molecule's list
+  int* nl_nnl;                    // per atom
+  int* nl_jat;                    // per list entry
+  int* mol_nnz;                   // per molecule
+
+  // integrals
+  double* S;
+  double* H;
+  double* D;
+  double* Q;
+
+  // Coulomb matrices
+  double* gam;                    // nsh x nsh (ES2)
+  double* aes_sd;                 // [3][nat][nat]: sd_k(i,j) = amat_sd(k, j, i)
+  double* aes_dd;                 // [3][3][nat][nat]: dd_kl(i,j) = amat_dd(k, j, l, i)
+  double* aes_sq;                 // [6][nat][nat]
+
+  // SCF results / wavefunction
+  double* qsh;
+  double* qat;
+  double* dpat;                   // 3 per atom
+  double* qpat;                   // 6 per atom
+  double* emo;                    // per AO (sorted)
+  double* focc;                   // per AO
+  double* coeff;                  // optional n x n per molecule (may be null)
+
+  // Broyden mixer state
+  double* mix_qin;
+  double* mix_dq;
+  double* mix_dqlast;
+  double* mix_qlast;
+  double* mix_df;
+  double* mix_u;
+  double* mix_a;                  // mix_memory^2 per molecule: <df_i, df_j>
+  double* mix_beta;               // mix_memory * (mix_memory + 1) per molecule: solve scratch
+  double* mix_omega;              // mix_memory per molecule
+  double* mix_c;                  // mix_memory per molecule
+
+  // per-molecule scalars
+  double* energy;
+  double* eelec;
+  double* erep_tot;
+  int* iterations;
+  int* status;
+
+  // global-memory matrices for molecules too large for LDS (ld x ld, three per molecule)
+  double* big_a;
+  double* big_c;
+  double* big_t;
+  const int64_t* mol_big_off;     // offset into big_a / big_c / big_t, -1 when LDS-resident
+};
+
+// LDS layout of the SCF kernel, in doubles (integer arrays in units of ints
+// from the start of the region).  ld is the padded AO dimension of the
+// bucket (multiple of 4); the two ld x ld matrices are only in LDS when
+// mats is true.
+struct ScfLds {
+  int a, c;           // matrix offsets (doubles)
+  int red, scal;      // reduction scratch (64) and broadcast scalars (16)
+  int ev, focc, emo, fs;
+  int rc, rs, rt;     // rotation cos, sin, tan (ld/2 each)
+  int work;           // phase-shared vectors
+  int iint;           // start of the int region, in ints
+  int rank, inv, occ, rp, rq;  // int arrays (ints, relative to the LDS base)
+  int total;          // doubles
+};
+
+__host__ __device__ inline ScfLds scf_lds_layout(int ld, int natmax, int nshmax, bool mats)
+{
+  ScfLds L;
+  int off = 0;
+  L.a = off;
+  L.c = off + ld * ld;
+  if (mats) off += 2 * ld * ld;
+  L.red = off; off += 64;
+  L.scal = off; off += 16;
+  L.ev = off; off += ld;
+  L.focc = off; off += ld;
+  L.emo = off; off += ld;
+  L.fs = off; off += ld;
+  L.rc = off; off += ld / 2;
+  L.rs = off; off += ld / 2;
+  L.rt = off; off += ld / 2;
+  int w1 = ld + nshmax + 10 * natmax, w2 = 11 * ld;
+  L.work = off; off += (w1 > w2 ? w1 : w2);
+  L.iint = 2 * off;
+  L.rank = L.iint;
+  L.inv = L.rank + ld;
+  L.occ = L.inv + ld;
+  L.rp = L.occ + ld;
+  L.rq = L.rp + ld / 2;
+  off += (4 * ld + 1) / 2 + 1;
+  L.total = off;
+  return L;
+}
+
+#define TA_STATUS_NOT_CONVERGED 1
+#define TA_STATUS_CHOLESKY 2
+#define TA_STATUS_EIGEN 4
+#define TA_STATUS_NAN 8
+#define TA_STATUS_MIXER 16
+#define TA_STATUS_TOO_LARGE 32

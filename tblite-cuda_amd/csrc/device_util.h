@@ -1,0 +1,75 @@
+// Small device helpers shared by the kernels: wave64 / workgroup reductions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#define TA_WAVE 64
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, TA_WAVE);
+  return v;
+}
+
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, TA_WAVE));
+  return v;
+}
+
+// Workgroup sum; red must hold blockDim.x / 64 doubles.  Every thread gets
+// the result.  Contains two barriers.
+__device__ __forceinline__ double block_sum(double v, double* red)
+{
+  const int lane = threadIdx.x & (TA_WAVE - 1);
+  const int wid = threadIdx.x / TA_WAVE;
+  const int nw = blockDim.x / TA_WAVE;
+  v = wave_sum(v);
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double r = 0.0;
+  for (int w = 0; w < nw; ++w) r += red[w];
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ double block_max(double v, double* red)
+{
+  const int lane = threadIdx.x & (TA_WAVE - 1);
+  const int wid = threadIdx.x / TA_WAVE;
+  const int nw = blockDim.x / TA_WAVE;
+  v = wave_max(v);
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double r = red[0];
+  for (int w = 1; w < nw; ++w) r = fmax(r, red[w]);
+  __syncthreads();
+  return r;
+}
+
+// Exclusive prefix sum of ints over the workgroup (red: blockDim.x/64 ints);
+// returns the thread's exclusive prefix and writes the total to *total.
+__device__ __forceinline__ int block_exclusive_scan(int v, int* red, int* total)
+{
+  const int lane = threadIdx.x & (TA_WAVE - 1);
+  const int wid = threadIdx.x / TA_WAVE;
+  const int nw = blockDim.x / TA_WAVE;
+  int incl = v;
+#pragma unroll
+  for (int off = 1; off < TA_WAVE; off <<= 1) {
+    int t = __shfl_up(incl, off, TA_WAVE);
+    if (lane >= off) incl += t;
+  }
+  if (lane == TA_WAVE - 1) red[wid] = incl;
+  __syncthreads();
+  int base = 0, tot = 0;
+  for (int w = 0; w < nw; ++w) {
+    if (w < wid) base += red[w];
+    tot += red[w];
+  }
+  __syncthreads();
+  *total = tot;
+  return base + incl - v;
+}

@@ -1,0 +1,1182 @@
+// Host side of the engine: calculator construction from parameters, batch
+// planning (per-molecule offsets into flat device buffers), the device
+// workspace, kernel launches, and the C ABI declared in include/tblite_amd.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../params/gfn2_default.h"
+#include "../params/sto_ng_table.h"
+#include "common.h"
+#include "tblite_amd.h"
+
+// kernels (setup.hip / scf.hip)
+__global__ void k_fill(double* p, int64_t n, double v);
+__global__ void k_setup(BatchDev b);
+__global__ void k_adjlist(BatchDev b);
+__global__ void k_integrals(BatchDev b, const int* work_mol, const int* work_kind,
+                            const int* work_start);
+template <int NT, int MAXT, bool MATS_LDS>
+__global__ void k_scf(BatchDev b, const int* mols, int ldmax, int natmax, int nshmax);
+
+namespace {
+
+constexpr int kIntegralChunk = 128;
+constexpr int kLdsMaxLd = 96;
+constexpr int kNumStreams = 4;
+
+// ------------------------------------------------------------ calculator
+struct Calculator {
+  tblite_amd_param param;
+  DevParams dev;
+  double min_alpha_el[TA_MAX_ELEM];
+  int zmap[120];
+  double accuracy = 1.0;
+  int max_iter = 250;
+  double damping = 0.4;
+  double etemp = 300.0;
+  DevParams* d_par = nullptr;
+};
+
+int sto_shell_index(int n, int l)
+{
+  if (n == 1 && l == 0) return 0;
+  if (n == 2 && l == 0) return 1;
+  if (n == 2 && l == 1) return 2;
+  if (n == 3 && l == 0) return 3;
+  if (n == 3 && l == 1) return 4;
+  if (n == 3 && l == 2) return 5;
+  return -1;
+}
+
+// Build the device parameter block: STO-nG contraction of every shell
+// (exponents alpha * zeta * zeta, primitive norms (2a/pi)^(3/4) (4a)^(l/2) /
+// sqrt((2l-1)!!)), levels and CN shifts in Eh, shell hardness, third-order
+// derivatives, and the H0 scaling table hscale (zeta term, EN term,
+// shell-pair factor) of tblite's GFN2 specification.
+bool build_calculator(Calculator& c, std::string& err)
+{
+  const tblite_amd_param& p = c.param;
+  if (p.nelem < 1 || p.nelem > TA_MAX_ELEM) {
+    err = "parameter set has an invalid element count";
+    return false;
+  }
+  std::memset(&c.dev, 0, sizeof(c.dev));
+  for (int z = 0; z < 120; ++z) c.zmap[z] = -1;
+  c.dev.nelem = p.nelem;
+  const double dfact[3] = {1.0, 1.0, 3.0};
+  for (int e = 0; e < p.nelem; ++e) {
+    const tblite_amd_element& el = p.elem[e];
+    DevElem& d = c.dev.el[e];
+    if (el.z <= 0 || el.z >= 120 || el.nsh < 1 || el.nsh > TA_MAX_SHELL) {
+      err = "invalid element record in parameter set";
+      return false;
+    }
+    c.zmap[el.z] = e;
+    d.z = el.z;
+    d.nsh = el.nsh;
+    c.min_alpha_el[e] = INFINITY;
+    for (int s = 0; s < el.nsh; ++s) {
+      const int l = el.ang[s];
+      const int ng = el.ngauss[s];
+      const int sh = sto_shell_index(el.pqn[s], l);
+      if (sh < 0 || ng < 1 || ng > STO_NG_MAXG || l < 0 || l > 2) {
+        err = "unsupported shell (n, l, ngauss) in parameter set";
+        return false;
+      }
+      d.ang[s] = l;
+      d.nprim[s] = ng;
+      const double zeta = el.slater[s];
+      for (int i = 0; i < ng; ++i) {
+        const double a = sto_ng_alpha[sh][ng - 1][i] * zeta * zeta;
+        if (!(a > 0.0)) {
+          err = "STO-nG table has no fit for this shell";
+          return false;
+        }
+        d.alpha[s][i] = a;
+        d.coeff[s][i] = sto_ng_coeff[sh][ng - 1][i] * std::pow(2.0 * a / M_PI, 0.75) *
+                        std::pow(4.0 * a, 0.5 * l) / std::sqrt(dfact[l]);
+        c.min_alpha_el[e] = std::min(c.min_alpha_el[e], a);
+      }
+      d.selfenergy[s] = el.levels[s] / TBLITE_AMD_AUTOEV;
+      d.kcn[s] = el.kcn[s] / TBLITE_AMD_AUTOEV;
+      d.hardness[s] = el.gam * el.lgam[s];
+      d.hubbard_derivs[s] = el.gam3 * p.thirdorder_shell[l];
+      d.refocc[s] = el.refocc[s];
+      d.shpoly[s] = el.shpoly[s];
+    }
+    d.rcov = el.rcov;
+    d.atomic_rad = el.atomic_rad;
+    d.zeff = el.zeff;
+    d.arep = el.arep;
+    d.dkernel = el.dkernel;
+    d.qkernel = el.qkernel;
+    d.mprad = el.mprad;
+    d.mpvcn = el.mpvcn;
+    d.en = el.en;
+  }
+  for (int ei = 0; ei < p.nelem; ++ei)
+    for (int ej = 0; ej < p.nelem; ++ej) {
+      const tblite_amd_element& a = p.elem[ei];
+      const tblite_amd_element& b = p.elem[ej];
+      const double den = (a.en - b.en) * (a.en - b.en);
+      for (int is = 0; is < a.nsh; ++is)
+        for (int js = 0; js < b.nsh; ++js) {
+          const double zi = a.slater[is], zj = b.slater[js];
+          const double zij = std::pow(2.0 * std::sqrt(zi * zj) / (zi + zj), p.wexp);
+          c.dev.hscale[ei][ej][is][js] =
+              zij * (1.0 + p.enscale * den) * p.kshell[b.ang[js]][a.ang[is]];
+        }
+    }
+  c.dev.rep_kexp = p.rep_kexp;
+  c.dev.rep_kexp_light = p.rep_kexp_light;
+  c.dev.rep_rexp = p.rep_rexp;
+  c.dev.rep_cutoff = p.rep_cutoff;
+  c.dev.gexp = p.gexp;
+  c.dev.dmp3 = p.dmp3;
+  c.dev.dmp5 = p.dmp5;
+  c.dev.mp_shift = p.mp_shift;
+  c.dev.mp_kexp = p.mp_kexp;
+  c.dev.mp_rmax = p.mp_rmax;
+  c.dev.cn_ka = p.cn_ka;
+  c.dev.cn_kb = p.cn_kb;
+  c.dev.cn_rshift = p.cn_rshift;
+  c.dev.cn_cutoff = p.cn_cutoff;
+  return true;
+}
+
+// integral_cutoff / get_cutoff, src/basis/type.cu:4-64 (text)
+double integral_cutoff(double acc)
+{
+  auto clip = [](double v, double lo, double hi) { return std::fmin(std::fmax(v, lo), hi); };
+  const double min_intcut = 5.0, max_intcut = 25.0, max_acc = 1.0e-4, min_acc = 1.0e+3;
+  return clip(max_intcut - 10.0 * std::log10(clip(acc, min_acc, max_acc)), min_intcut, max_intcut);
+}
+
+double get_cutoff(double min_alpha, double acc)
+{
+  return std::fmin(std::sqrt(2.0 * integral_cutoff(acc) / min_alpha), 40.0);
+}
+
+// -------------------------------------------------------------- workspace
+struct Arena {
+  char* base = nullptr;
+  size_t cap = 0;
+  size_t used = 0;
+};
+
+template <class T>
+T* carve(Arena& a, size_t count)
+{
+  size_t bytes = ((count * sizeof(T)) + 255) & ~size_t(255);
+  T* p = reinterpret_cast<T*>(a.base + a.used);
+  a.used += bytes;
+  return p;
+}
+
+// Host-side plan of one batch
+struct Plan {
+  int nmol = 0;
+  int mem = 0;
+  std::vector<int> mol_atom_off, mol_nat, mol_sh_off, mol_nsh, mol_ao_off, mol_nao, mol_uhf;
+  std::vector<int64_t> mol_mat_off, mol_nl_off, mol_gam_off, mol_pair_off, mol_mix_off,
+      mol_mixmem_off, mol_mixa_off, mol_big_off;
+  std::vector<double> mol_charge, mol_cutoff;
+  std::vector<int> at_el, at_ish, at_iao;
+  std::vector<int> sh_at, sh_ish, sh_iao;
+  std::vector<int> ao_at, ao_sh;
+  std::vector<int> work_mol, work_kind, work_start;
+  struct Bucket {
+    int nt, maxt;
+    bool lds;
+    int ldmax, natmax, nshmax;
+    std::vector<int> mols;
+  };
+  std::vector<Bucket> buckets;
+  int64_t tot_atoms = 0, tot_sh = 0, tot_ao = 0, tot_mat = 0, tot_nl = 0, tot_gam = 0,
+          tot_pair = 0, tot_mix = 0, tot_mixmem = 0, tot_big = 0;
+  std::vector<int> too_large;
+};
+
+struct DeviceBatch {
+  BatchDev b;
+  int* d_work_mol = nullptr;
+  int* d_work_kind = nullptr;
+  int* d_work_start = nullptr;
+  std::vector<int*> d_bucket_mols;
+  size_t bytes = 0;
+};
+
+}  // namespace
+
+// ------------------------------------------------------------- handles
+struct tblite_amd_context_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipStream_t aux[kNumStreams] = {};
+  hipEvent_t ev[8] = {};
+  hipEvent_t aux_done[kNumStreams] = {};
+  std::string error;
+  Arena arena;
+  Plan plan;
+  DeviceBatch batch;
+  bool have_batch = false;
+  double timings[5] = {0, 0, 0, 0, 0};
+  size_t lds_limit = 0;
+};
+
+struct tblite_amd_structure_s {
+  tblite_amd_context ctx;
+  int nat;
+  std::vector<int> num;
+  std::vector<double> xyz;
+  double charge;
+  int uhf;
+};
+
+struct tblite_amd_calculator_s {
+  tblite_amd_context ctx;
+  Calculator c;
+};
+
+struct tblite_amd_wavefunction_s {
+  tblite_amd_context ctx;
+  int nat = 0, nsh = 0, nao = 0;
+  std::vector<double> qsh, qat, dpat, qpat, emo, focc, n0sh, n0at, coeff, density;
+};
+
+namespace {
+
+void set_error(tblite_amd_context ctx, const char* fmt, ...)
+{
+  if (!ctx) return;
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  ctx->error = buf;
+}
+
+#define HIP_TRY(ctx, call)                                                               \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess) {                                                              \
+      set_error(ctx, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__,    \
+                __LINE__);                                                               \
+      return TBLITE_AMD_ERR_HIP;                                                         \
+    }                                                                                    \
+  } while (0)
+
+int ensure_params_on_device(tblite_amd_context ctx, Calculator& c)
+{
+  if (c.d_par) return TBLITE_AMD_OK;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  HIP_TRY(ctx, hipMalloc(&c.d_par, sizeof(DevParams)));
+  HIP_TRY(ctx, hipMemcpy(c.d_par, &c.dev, sizeof(DevParams), hipMemcpyHostToDevice));
+  return TBLITE_AMD_OK;
+}
+
+// Build the host plan for a batch.  Returns OK or ERR_ARGUMENT.
+int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* mol_nat,
+              const int* num, const double* charge, const int* uhf, double acc, Plan& pl)
+{
+  pl = Plan();
+  pl.nmol = nmol;
+  pl.mem = std::max(1, c.max_iter);
+  pl.mol_atom_off.resize(nmol);
+  pl.mol_nat.resize(nmol);
+  pl.mol_sh_off.resize(nmol);
+  pl.mol_nsh.resize(nmol);
+  pl.mol_ao_off.resize(nmol);
+  pl.mol_nao.resize(nmol);
+  pl.mol_uhf.resize(nmol);
+  pl.mol_mat_off.resize(nmol);
+  pl.mol_nl_off.resize(nmol);
+  pl.mol_gam_off.resize(nmol);
+  pl.mol_pair_off.resize(nmol);
+  pl.mol_mix_off.resize(nmol);
+  pl.mol_mixmem_off.resize(nmol);
+  pl.mol_mixa_off.resize(nmol);
+  pl.mol_big_off.resize(nmol);
+  pl.mol_charge.resize(nmol);
+  pl.mol_cutoff.resize(nmol);
+  int64_t atom = 0, shell = 0, ao = 0, mat = 0, nl = 0, gam = 0, pair = 0, mix = 0, mixmem = 0,
+          big = 0;
+  for (int m = 0; m < nmol; ++m) {
+    const int nat = mol_nat[m];
+    if (nat < 1) {
+      set_error(ctx, "molecule %d has %d atoms", m, nat);
+      return TBLITE_AMD_ERR_ARGUMENT;
+    }
+    pl.mol_atom_off[m] = (int)atom;
+    pl.mol_nat[m] = nat;
+    pl.mol_sh_off[m] = (int)shell;
+    pl.mol_ao_off[m] = (int)ao;
+    int nsh = 0, nao = 0;
+    double min_alpha = INFINITY;
+    for (int i = 0; i < nat; ++i) {
+      const int z = num[atom + i];
+      const int e = (z > 0 && z < 120) ? c.zmap[z] : -1;
+      if (e < 0) {
+        set_error(ctx, "molecule %d atom %d: element Z=%d has no parameters", m, i, z);
+        return TBLITE_AMD_ERR_ARGUMENT;
+      }
+      const DevElem& el = c.dev.el[e];
+      pl.at_el.push_back(e);
+      pl.at_ish.push_back(nsh);
+      pl.at_iao.push_back(nao);
+      min_alpha = std::min(min_alpha, c.min_alpha_el[e]);
+      for (int s = 0; s < el.nsh; ++s) {
+        pl.sh_at.push_back(i);
+        pl.sh_ish.push_back(s);
+        pl.sh_iao.push_back(nao);
+        const int na = 2 * el.ang[s] + 1;
+        for (int k = 0; k < na; ++k) {
+          pl.ao_at.push_back(i);
+          pl.ao_sh.push_back(nsh);
+        }
+        nao += na;
+        ++nsh;
+      }
+    }
+    pl.mol_nsh[m] = nsh;
+    pl.mol_nao[m] = nao;
+    pl.mol_uhf[m] = uhf ? uhf[m] : 0;
+    pl.mol_charge[m] = charge ? charge[m] : 0.0;
+    pl.mol_cutoff[m] = get_cutoff(min_alpha, acc);
+    pl.mol_mat_off[m] = mat;
+    pl.mol_nl_off[m] = nl;
+    pl.mol_gam_off[m] = gam;
+    pl.mol_pair_off[m] = pair;
+    pl.mol_mix_off[m] = mix;
+    pl.mol_mixmem_off[m] = mixmem;
+    pl.mol_mixa_off[m] = (int64_t)m * pl.mem * pl.mem;
+    const int ld = (nao + 3) & ~3;
+    const int ndim = nsh + 9 * nat;
+    if (ld > kLdsMaxLd) {
+      pl.mol_big_off[m] = big;
+      big += (int64_t)ld * ld;
+    } else {
+      pl.mol_big_off[m] = -1;
+    }
+    atom += nat;
+    shell += nsh;
+    ao += nao;
+    mat += (int64_t)nao * nao;
+    nl += (int64_t)nat * (nat + 1) / 2;
+    gam += (int64_t)nsh * nsh;
+    pair += (int64_t)nat * nat;
+    mix += ndim;
+    mixmem += (int64_t)ndim * pl.mem;
+    // integral work: neighbour chunks over the list capacity, on-site chunks
+    const int64_t cap = (int64_t)nat * (nat + 1) / 2;
+    for (int64_t s = 0; s < cap; s += kIntegralChunk) {
+      pl.work_mol.push_back(m);
+      pl.work_kind.push_back(0);
+      pl.work_start.push_back((int)s);
+    }
+    for (int s = 0; s < nat; s += kIntegralChunk) {
+      pl.work_mol.push_back(m);
+      pl.work_kind.push_back(1);
+      pl.work_start.push_back(s);
+    }
+  }
+  pl.tot_atoms = atom;
+  pl.tot_sh = shell;
+  pl.tot_ao = ao;
+  pl.tot_mat = mat;
+  pl.tot_nl = nl;
+  pl.tot_gam = gam;
+  pl.tot_pair = pair;
+  pl.tot_mix = mix;
+  pl.tot_mixmem = mixmem;
+  pl.tot_big = big;
+
+  // size buckets over the padded AO dimension; largest molecules first
+  const int bounds[7] = {16, 32, 48, 64, 80, 96, 1 << 30};
+  std::vector<int> order(nmol);
+  for (int m = 0; m < nmol; ++m) order[m] = m;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return pl.mol_nao[a] > pl.mol_nao[b]; });
+  for (int k = 6; k >= 0; --k) {
+    Plan::Bucket bk;
+    const int lo = (k == 0) ? 0 : bounds[k - 1];
+    const int hi = bounds[k];
+    bk.ldmax = 0;
+    bk.natmax = 0;
+    bk.nshmax = 0;
+    for (int m : order) {
+      const int ld = (pl.mol_nao[m] + 3) & ~3;
+      if (ld > lo && ld <= hi) {
+        bk.mols.push_back(m);
+        bk.ldmax = std::max(bk.ldmax, ld);
+        bk.natmax = std::max(bk.natmax, pl.mol_nat[m]);
+        bk.nshmax = std::max(bk.nshmax, pl.mol_nsh[m]);
+      }
+    }
+    if (bk.mols.empty()) continue;
+    if (hi <= 32) {
+      bk.nt = 128;
+      bk.maxt = 1;
+      bk.lds = true;
+    } else if (hi <= 64) {
+      bk.nt = 256;
+      bk.maxt = 1;
+      bk.lds = true;
+    } else if (hi <= kLdsMaxLd) {
+      bk.nt = 512;
+      bk.maxt = 2;
+      bk.lds = true;
+    } else {
+      bk.nt = 512;
+      bk.maxt = 1;
+      bk.lds = false;
+    }
+    pl.buckets.push_back(bk);
+  }
+  return TBLITE_AMD_OK;
+}
+
+template <class T>
+void host_pack(std::vector<char>& buf, std::vector<std::pair<size_t, size_t>>& segs,
+               const std::vector<T>& v)
+{
+  size_t off = (buf.size() + 255) & ~size_t(255);
+  buf.resize(off + v.size() * sizeof(T) + 1);
+  if (!v.empty()) std::memcpy(buf.data() + off, v.data(), v.size() * sizeof(T));
+  segs.push_back({off, v.size() * sizeof(T)});
+}
+
+// Allocate (or reuse) the workspace for a plan and upload the plan arrays.
+int upload_plan(tblite_amd_context ctx, Calculator& c, const Plan& pl, DeviceBatch& db)
+{
+  const int nmol = pl.nmol, mem = pl.mem;
+  // packed host image of all integer / offset arrays
+  std::vector<char> img;
+  std::vector<std::pair<size_t, size_t>> seg;
+  host_pack(img, seg, pl.mol_atom_off);   // 0
+  host_pack(img, seg, pl.mol_nat);        // 1
+  host_pack(img, seg, pl.mol_sh_off);     // 2
+  host_pack(img, seg, pl.mol_nsh);        // 3
+  host_pack(img, seg, pl.mol_ao_off);     // 4
+  host_pack(img, seg, pl.mol_nao);        // 5
+  host_pack(img, seg, pl.mol_mat_off);    // 6
+  host_pack(img, seg, pl.mol_nl_off);     // 7
+  host_pack(img, seg, pl.mol_gam_off);    // 8
+  host_pack(img, seg, pl.mol_pair_off);   // 9
+  host_pack(img, seg, pl.mol_mix_off);    // 10
+  host_pack(img, seg, pl.mol_mixmem_off); // 11
+  host_pack(img, seg, pl.mol_mixa_off);   // 12
+  host_pack(img, seg, pl.mol_charge);     // 13
+  host_pack(img, seg, pl.mol_uhf);        // 14
+  host_pack(img, seg, pl.mol_cutoff);     // 15
+  host_pack(img, seg, pl.at_el);          // 16
+  host_pack(img, seg, pl.at_ish);         // 17
+  host_pack(img, seg, pl.at_iao);         // 18
+  host_pack(img, seg, pl.sh_at);          // 19
+  host_pack(img, seg, pl.sh_ish);         // 20
+  host_pack(img, seg, pl.sh_iao);         // 21
+  host_pack(img, seg, pl.ao_at);          // 22
+  host_pack(img, seg, pl.ao_sh);          // 23
+  host_pack(img, seg, pl.work_mol);       // 24
+  host_pack(img, seg, pl.work_kind);      // 25
+  host_pack(img, seg, pl.work_start);     // 26
+  host_pack(img, seg, pl.mol_big_off);    // 27
+  const size_t nfixed = seg.size();
+  for (const auto& bk : pl.buckets) host_pack(img, seg, bk.mols);
+
+  const int64_t A = pl.tot_atoms, Sh = pl.tot_sh, AO = pl.tot_ao;
+  auto dbytes = [](int64_t n) { return ((size_t)n * 8 + 255) & ~size_t(255); };
+  auto ibytes = [](int64_t n) { return ((size_t)n * 4 + 255) & ~size_t(255); };
+  size_t need = ((img.size() + 255) & ~size_t(255));
+  need += dbytes(3 * A) + 3 * dbytes(A);                 // xyz, cn, mrad, erep
+  need += 2 * dbytes(Sh);                                // selfenergy, n0sh
+  need += 3 * ibytes(A) + ibytes(pl.tot_nl) + ibytes(nmol); // inl, nnl, jat, nnz (+pad)
+  need += 11 * dbytes(pl.tot_mat);                       // S, H, D, Q
+  need += dbytes(pl.tot_gam) + 18 * dbytes(pl.tot_pair); // gam, sd, dd, sq
+  need += dbytes(Sh) + dbytes(A) + dbytes(3 * A) + dbytes(6 * A) + 2 * dbytes(AO); // wfn
+  need += 4 * dbytes(pl.tot_mix) + 2 * dbytes(pl.tot_mixmem);
+  need += dbytes((int64_t)nmol * mem * mem) + dbytes((int64_t)nmol * mem * (mem + 1)) +
+          2 * dbytes((int64_t)nmol * mem);
+  need += 3 * dbytes(nmol) + 2 * ibytes(nmol);
+  need += 3 * dbytes(std::max<int64_t>(pl.tot_big, 1));
+  need += 64 * 256;
+
+  Arena& ar = ctx->arena;
+  if (ar.cap < need) {
+    if (ar.base) {
+      HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+      HIP_TRY(ctx, hipFree(ar.base));
+      ar.base = nullptr;
+      ar.cap = 0;
+    }
+    size_t cap = need + need / 4;
+    HIP_TRY(ctx, hipMalloc(&ar.base, cap));
+    ar.cap = cap;
+  }
+  ar.used = 0;
+  char* d_img = carve<char>(ar, img.size());
+  HIP_TRY(ctx, hipMemcpyAsync(d_img, img.data(), img.size(), hipMemcpyHostToDevice, ctx->stream));
+  auto I = [&](size_t k) { return reinterpret_cast<int*>(d_img + seg[k].first); };
+  auto L = [&](size_t k) { return reinterpret_cast<int64_t*>(d_img + seg[k].first); };
+  auto Dd = [&](size_t k) { return reinterpret_cast<double*>(d_img + seg[k].first); };
+
+  BatchDev& b = db.b;
+  std::memset(&b, 0, sizeof(b));
+  b.nmol = nmol;
+  b.max_iter = c.max_iter;
+  b.mix_memory = mem;
+  b.damping = c.damping;
+  b.kt = c.etemp * TBLITE_AMD_KB_EH;
+  b.accuracy = 0.0; // set by the caller
+  b.par = c.d_par;
+  b.mol_atom_off = I(0);
+  b.mol_nat = I(1);
+  b.mol_sh_off = I(2);
+  b.mol_nsh = I(3);
+  b.mol_ao_off = I(4);
+  b.mol_nao = I(5);
+  b.mol_mat_off = L(6);
+  b.mol_nl_off = L(7);
+  b.mol_gam_off = L(8);
+  b.mol_pair_off = L(9);
+  b.mol_mix_off = L(10);
+  b.mol_mixmem_off = L(11);
+  b.mol_mixa_off = L(12);
+  b.mol_charge = Dd(13);
+  b.mol_uhf = I(14);
+  b.mol_cutoff = Dd(15);
+  b.at_el = I(16);
+  b.at_ish = I(17);
+  b.at_iao = I(18);
+  b.sh_at = I(19);
+  b.sh_ish = I(20);
+  b.sh_iao = I(21);
+  b.ao_at = I(22);
+  b.ao_sh = I(23);
+  db.d_work_mol = I(24);
+  db.d_work_kind = I(25);
+  db.d_work_start = I(26);
+  b.mol_big_off = L(27);
+  db.d_bucket_mols.clear();
+  for (size_t k = 0; k < pl.buckets.size(); ++k) db.d_bucket_mols.push_back(I(nfixed + k));
+
+  b.xyz = carve<double>(ar, 3 * A);
+  b.cn = carve<double>(ar, A);
+  b.mrad = carve<double>(ar, A);
+  b.erep = carve<double>(ar, A);
+  b.selfenergy = carve<double>(ar, Sh);
+  b.n0sh = carve<double>(ar, Sh);
+  b.nl_inl = carve<int>(ar, A);
+  b.nl_nnl = carve<int>(ar, A);
+  b.nl_jat = carve<int>(ar, std::max<int64_t>(pl.tot_nl, 1));
+  b.mol_nnz = carve<int>(ar, nmol);
+  b.S = carve<double>(ar, pl.tot_mat);
+  b.H = carve<double>(ar, pl.tot_mat);
+  b.D = carve<double>(ar, 3 * pl.tot_mat);
+  b.Q = carve<double>(ar, 6 * pl.tot_mat);
+  b.gam = carve<double>(ar, pl.tot_gam);
+  b.aes_sd = carve<double>(ar, 3 * pl.tot_pair);
+  b.aes_dd = carve<double>(ar, 9 * pl.tot_pair);
+  b.aes_sq = carve<double>(ar, 6 * pl.tot_pair);
+  b.qsh = carve<double>(ar, Sh);
+  b.qat = carve<double>(ar, A);
+  b.dpat = carve<double>(ar, 3 * A);
+  b.qpat = carve<double>(ar, 6 * A);
+  b.emo = carve<double>(ar, AO);
+  b.focc = carve<double>(ar, AO);
+  b.coeff = nullptr;
+  b.mix_qin = carve<double>(ar, pl.tot_mix);
+  b.mix_dq = carve<double>(ar, pl.tot_mix);
+  b.mix_dqlast = carve<double>(ar, pl.tot_mix);
+  b.mix_qlast = carve<double>(ar, pl.tot_mix);
+  b.mix_df = carve<double>(ar, pl.tot_mixmem);
+  b.mix_u = carve<double>(ar, pl.tot_mixmem);
+  b.mix_a = carve<double>(ar, (int64_t)nmol * mem * mem);
+  b.mix_beta = carve<double>(ar, (int64_t)nmol * mem * (mem + 1));
+  b.mix_omega = carve<double>(ar, (int64_t)nmol * mem);
+  b.mix_c = carve<double>(ar, (int64_t)nmol * mem);
+  b.energy = carve<double>(ar, nmol);
+  b.eelec = carve<double>(ar, nmol);
+  b.erep_tot = carve<double>(ar, nmol);
+  b.iterations = carve<int>(ar, nmol);
+  b.status = carve<int>(ar, nmol);
+  b.big_a = carve<double>(ar, std::max<int64_t>(pl.tot_big, 1));
+  b.big_c = carve<double>(ar, std::max<int64_t>(pl.tot_big, 1));
+  b.big_t = carve<double>(ar, std::max<int64_t>(pl.tot_big, 1));
+  b.intcut = 0.0;
+  db.bytes = ar.used;
+  if (ar.used > ar.cap) {
+    set_error(ctx, "internal: workspace estimate too small (%zu > %zu)", ar.used, ar.cap);
+    return TBLITE_AMD_ERR_HIP;
+  }
+  return TBLITE_AMD_OK;
+}
+
+template <int NT, int MAXT, bool LDS>
+int launch_scf(tblite_amd_context ctx, hipStream_t st, const BatchDev& b, const int* mols,
+               int count, int ldmax, int natmax, int nshmax)
+{
+  const ScfLds L = scf_lds_layout(ldmax, natmax, nshmax, LDS);
+  const size_t bytes = (size_t)L.total * sizeof(double);
+  if (bytes > ctx->lds_limit) {
+    set_error(ctx, "SCF bucket needs %zu bytes of LDS (limit %zu)", bytes, ctx->lds_limit);
+    return TBLITE_AMD_ERR_TOO_LARGE;
+  }
+  HIP_TRY(ctx, hipFuncSetAttribute((const void*)k_scf<NT, MAXT, LDS>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  hipLaunchKernelGGL((k_scf<NT, MAXT, LDS>), dim3(count), dim3(NT), bytes, st, b, mols, ldmax,
+                     natmax, nshmax);
+  HIP_TRY(ctx, hipGetLastError());
+  return TBLITE_AMD_OK;
+}
+
+// Run the whole pipeline for the planned batch with coordinates xyz (host).
+int run_batch(tblite_amd_context ctx, Calculator& c, const double* xyz, double acc)
+{
+  Plan& pl = ctx->plan;
+  DeviceBatch& db = ctx->batch;
+  BatchDev& b = db.b;
+  b.accuracy = acc;
+  b.intcut = integral_cutoff(acc);
+  hipStream_t st = ctx->stream;
+  HIP_TRY(ctx, hipMemcpyAsync((void*)b.xyz, xyz, sizeof(double) * 3 * pl.tot_atoms,
+                              hipMemcpyHostToDevice, st));
+  HIP_TRY(ctx, hipEventRecord(ctx->ev[0], st));
+  // zero the integral matrices: pairs beyond the cutoff keep zero blocks
+  {
+    const int64_t n = 11 * pl.tot_mat;
+    int grid = (int)std::min<int64_t>((n + 255) / 256, 65536);
+    if (n > 0) hipLaunchKernelGGL(k_fill, dim3(grid), dim3(256), 0, st, b.S, pl.tot_mat, 0.0);
+    if (n > 0) hipLaunchKernelGGL(k_fill, dim3(grid), dim3(256), 0, st, b.H, pl.tot_mat, 0.0);
+    if (n > 0) hipLaunchKernelGGL(k_fill, dim3(grid), dim3(256), 0, st, b.D, 3 * pl.tot_mat, 0.0);
+    if (n > 0) hipLaunchKernelGGL(k_fill, dim3(grid), dim3(256), 0, st, b.Q, 6 * pl.tot_mat, 0.0);
+  }
+  hipLaunchKernelGGL(k_setup, dim3(pl.nmol), dim3(256), 0, st, b);
+  HIP_TRY(ctx, hipGetLastError());
+  HIP_TRY(ctx, hipEventRecord(ctx->ev[1], st));
+  hipLaunchKernelGGL(k_adjlist, dim3(pl.nmol), dim3(256), 0, st, b);
+  HIP_TRY(ctx, hipGetLastError());
+  HIP_TRY(ctx, hipEventRecord(ctx->ev[2], st));
+  const int nwork = (int)pl.work_mol.size();
+  if (nwork > 0)
+    hipLaunchKernelGGL(k_integrals, dim3(nwork), dim3(kIntegralChunk), 0, st, b, db.d_work_mol,
+                       db.d_work_kind, db.d_work_start);
+  HIP_TRY(ctx, hipGetLastError());
+  HIP_TRY(ctx, hipEventRecord(ctx->ev[3], st));
+  // SCF buckets on the auxiliary streams, largest bucket first
+  for (int s = 0; s < kNumStreams; ++s) HIP_TRY(ctx, hipStreamWaitEvent(ctx->aux[s], ctx->ev[3], 0));
+  for (size_t k = 0; k < pl.buckets.size(); ++k) {
+    const Plan::Bucket& bk = pl.buckets[k];
+    hipStream_t as = ctx->aux[k % kNumStreams];
+    const int* mols = db.d_bucket_mols[k];
+    const int cnt = (int)bk.mols.size();
+    int rc;
+    if (!bk.lds)
+      rc = launch_scf<512, 1, false>(ctx, as, b, mols, cnt, bk.ldmax, bk.natmax, bk.nshmax);
+    else if (bk.nt == 128)
+      rc = launch_scf<128, 1, true>(ctx, as, b, mols, cnt, bk.ldmax, bk.natmax, bk.nshmax);
+    else if (bk.nt == 256)
+      rc = launch_scf<256, 1, true>(ctx, as, b, mols, cnt, bk.ldmax, bk.natmax, bk.nshmax);
+    else
+      rc = launch_scf<512, 2, true>(ctx, as, b, mols, cnt, bk.ldmax, bk.natmax, bk.nshmax);
+    if (rc) return rc;
+  }
+  for (int s = 0; s < kNumStreams; ++s) {
+    HIP_TRY(ctx, hipEventRecord(ctx->aux_done[s], ctx->aux[s]));
+    HIP_TRY(ctx, hipStreamWaitEvent(st, ctx->aux_done[s], 0));
+  }
+  HIP_TRY(ctx, hipEventRecord(ctx->ev[4], st));
+  HIP_TRY(ctx, hipStreamSynchronize(st));
+  float t01, t12, t23, t34, t04;
+  HIP_TRY(ctx, hipEventElapsedTime(&t01, ctx->ev[0], ctx->ev[1]));
+  HIP_TRY(ctx, hipEventElapsedTime(&t12, ctx->ev[1], ctx->ev[2]));
+  HIP_TRY(ctx, hipEventElapsedTime(&t23, ctx->ev[2], ctx->ev[3]));
+  HIP_TRY(ctx, hipEventElapsedTime(&t34, ctx->ev[3], ctx->ev[4]));
+  HIP_TRY(ctx, hipEventElapsedTime(&t04, ctx->ev[0], ctx->ev[4]));
+  ctx->timings[0] = t01;
+  ctx->timings[1] = t12;
+  ctx->timings[2] = t23;
+  ctx->timings[3] = t34;
+  ctx->timings[4] = t04;
+  return TBLITE_AMD_OK;
+}
+
+int flat_singlepoint(tblite_amd_context ctx, Calculator& c, int nmol, const int* mol_nat,
+                     const int* num, const double* xyz, const double* charge, const int* uhf,
+                     double acc, double* energies, int* iterations, int* status)
+{
+  if (!ctx) return TBLITE_AMD_ERR_ARGUMENT;
+  if (nmol < 1 || !mol_nat || !num || !xyz || !energies) {
+    set_error(ctx, "invalid arguments to batched singlepoint");
+    return TBLITE_AMD_ERR_ARGUMENT;
+  }
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  int rc = ensure_params_on_device(ctx, c);
+  if (rc) return rc;
+  ctx->have_batch = false;
+  rc = make_plan(ctx, c, nmol, mol_nat, num, charge, uhf, acc, ctx->plan);
+  if (rc) return rc;
+  rc = upload_plan(ctx, c, ctx->plan, ctx->batch);
+  if (rc) return rc;
+  rc = run_batch(ctx, c, xyz, acc);
+  if (rc) return rc;
+  ctx->have_batch = true;
+  const BatchDev& b = ctx->batch.b;
+  std::vector<int> st(nmol), it(nmol);
+  HIP_TRY(ctx, hipMemcpy(energies, b.energy, sizeof(double) * nmol, hipMemcpyDeviceToHost));
+  HIP_TRY(ctx, hipMemcpy(st.data(), b.status, sizeof(int) * nmol, hipMemcpyDeviceToHost));
+  HIP_TRY(ctx, hipMemcpy(it.data(), b.iterations, sizeof(int) * nmol, hipMemcpyDeviceToHost));
+  if (status) std::memcpy(status, st.data(), sizeof(int) * nmol);
+  if (iterations) std::memcpy(iterations, it.data(), sizeof(int) * nmol);
+  for (int m = 0; m < nmol; ++m)
+    if (st[m] != 0) {
+      set_error(ctx, "molecule %d finished with status %d after %d iterations", m, st[m], it[m]);
+      return TBLITE_AMD_ERR_NOT_CONVERGED;
+    }
+  return TBLITE_AMD_OK;
+}
+
+int check_batch(tblite_amd_context ctx, int imol)
+{
+  if (!ctx) return TBLITE_AMD_ERR_ARGUMENT;
+  if (!ctx->have_batch) {
+    set_error(ctx, "no batch has been run on this context");
+    return TBLITE_AMD_ERR_ARGUMENT;
+  }
+  if (imol < 0 || imol >= ctx->plan.nmol) {
+    set_error(ctx, "molecule index %d out of range", imol);
+    return TBLITE_AMD_ERR_ARGUMENT;
+  }
+  return TBLITE_AMD_OK;
+}
+
+}  // namespace
+
+// ================================================================= C ABI
+extern "C" {
+
+int tblite_amd_param_default(tblite_amd_param* param)
+{
+  if (!param) return TBLITE_AMD_ERR_ARGUMENT;
+  tblite_amd_fill_gfn2_default(param);
+  return TBLITE_AMD_OK;
+}
+
+tblite_amd_context tblite_amd_new_context(int device)
+{
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count < 1) return nullptr;
+  if (device < 0 && hipGetDevice(&device) != hipSuccess) return nullptr;
+  if (device >= count) return nullptr;
+  if (hipSetDevice(device) != hipSuccess) return nullptr;
+  tblite_amd_context ctx = new (std::nothrow) tblite_amd_context_s();
+  if (!ctx) return nullptr;
+  ctx->device = device;
+  bool ok = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) == hipSuccess;
+  for (int s = 0; s < kNumStreams && ok; ++s)
+    ok = hipStreamCreateWithFlags(&ctx->aux[s], hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&ctx->aux_done[s], hipEventDisableTiming) == hipSuccess;
+  for (int e = 0; e < 8 && ok; ++e) ok = hipEventCreate(&ctx->ev[e]) == hipSuccess;
+  int lds = 0;
+  if (ok) ok = hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess;
+  ctx->lds_limit = (size_t)lds;
+  if (!ok) {
+    tblite_amd_delete_context(&ctx);
+    return nullptr;
+  }
+  return ctx;
+}
+
+void tblite_amd_delete_context(tblite_amd_context* pctx)
+{
+  if (!pctx || !*pctx) return;
+  tblite_amd_context ctx = *pctx;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (int s = 0; s < kNumStreams; ++s) {
+    if (ctx->aux[s]) {
+      (void)hipStreamSynchronize(ctx->aux[s]);
+      (void)hipStreamDestroy(ctx->aux[s]);
+    }
+    if (ctx->aux_done[s]) (void)hipEventDestroy(ctx->aux_done[s]);
+  }
+  for (int e = 0; e < 8; ++e)
+    if (ctx->ev[e]) (void)hipEventDestroy(ctx->ev[e]);
+  if (ctx->arena.base) (void)hipFree(ctx->arena.base);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  *pctx = nullptr;
+}
+
+int tblite_amd_get_error(tblite_amd_context ctx, char* buffer, int buffer_size)
+{
+  if (!ctx || !buffer || buffer_size < 1) return TBLITE_AMD_ERR_ARGUMENT;
+  std::snprintf(buffer, (size_t)buffer_size, "%s", ctx->error.c_str());
+  return TBLITE_AMD_OK;
+}
+
+tblite_amd_structure tblite_amd_new_structure(tblite_amd_context ctx, int nat, const int* num,
+                                              const double* xyz, double charge, int uhf)
+{
+  if (!ctx) return nullptr;
+  if (nat < 1 || !num || !xyz) {
+    set_error(ctx, "invalid structure arguments");
+    return nullptr;
+  }
+  for (int i = 0; i < 3 * nat; ++i)
+    if (!std::isfinite(xyz[i])) {
+      set_error(ctx, "non-finite coordinate in structure");
+      return nullptr;
+    }
+  tblite_amd_structure s = new (std::nothrow) tblite_amd_structure_s();
+  if (!s) return nullptr;
+  s->ctx = ctx;
+  s->nat = nat;
+  s->num.assign(num, num + nat);
+  s->xyz.assign(xyz, xyz + 3 * nat);
+  s->charge = charge;
+  s->uhf = uhf;
+  return s;
+}
+
+void tblite_amd_delete_structure(tblite_amd_structure* mol)
+{
+  if (!mol || !*mol) return;
+  delete *mol;
+  *mol = nullptr;
+}
+
+int tblite_amd_get_structure_dims(tblite_amd_structure mol, int* nat, int* nsh, int* nao)
+{
+  if (!mol) return TBLITE_AMD_ERR_ARGUMENT;
+  tblite_amd_param p;
+  tblite_amd_fill_gfn2_default(&p);
+  int ns = 0, na = 0;
+  for (int i = 0; i < mol->nat; ++i) {
+    int e = -1;
+    for (int k = 0; k < p.nelem; ++k)
+      if (p.elem[k].z == mol->num[i]) e = k;
+    if (e < 0) return TBLITE_AMD_ERR_ARGUMENT;
+    ns += p.elem[e].nsh;
+    for (int s = 0; s < p.elem[e].nsh; ++s) na += 2 * p.elem[e].ang[s] + 1;
+  }
+  if (nat) *nat = mol->nat;
+  if (nsh) *nsh = ns;
+  if (nao) *nao = na;
+  return TBLITE_AMD_OK;
+}
+
+tblite_amd_calculator tblite_amd_new_gfn2_calculator(tblite_amd_context ctx,
+                                                     const tblite_amd_param* param)
+{
+  if (!ctx) return nullptr;
+  tblite_amd_calculator c = new (std::nothrow) tblite_amd_calculator_s();
+  if (!c) return nullptr;
+  c->ctx = ctx;
+  if (param) c->c.param = *param;
+  else tblite_amd_fill_gfn2_default(&c->c.param);
+  std::string err;
+  if (!build_calculator(c->c, err)) {
+    set_error(ctx, "%s", err.c_str());
+    delete c;
+    return nullptr;
+  }
+  return c;
+}
+
+void tblite_amd_delete_calculator(tblite_amd_calculator* calc)
+{
+  if (!calc || !*calc) return;
+  if ((*calc)->c.d_par) (void)hipFree((*calc)->c.d_par);
+  delete *calc;
+  *calc = nullptr;
+}
+
+int tblite_amd_set_calculator_accuracy(tblite_amd_calculator calc, double accuracy)
+{
+  if (!calc || !(accuracy > 0.0)) return TBLITE_AMD_ERR_ARGUMENT;
+  calc->c.accuracy = accuracy;
+  return TBLITE_AMD_OK;
+}
+
+int tblite_amd_set_calculator_max_iter(tblite_amd_calculator calc, int max_iter)
+{
+  if (!calc || max_iter < 1) return TBLITE_AMD_ERR_ARGUMENT;
+  calc->c.max_iter = max_iter;
+  return TBLITE_AMD_OK;
+}
+
+int tblite_amd_set_calculator_mixer_damping(tblite_amd_calculator calc, double damping)
+{
+  if (!calc || !(damping > 0.0) || damping > 1.0) return TBLITE_AMD_ERR_ARGUMENT;
+  calc->c.damping = damping;
+  return TBLITE_AMD_OK;
+}
+
+int tblite_amd_set_calculator_temperature(tblite_amd_calculator calc, double kelvin)
+{
+  if (!calc || !(kelvin > 0.0)) return TBLITE_AMD_ERR_ARGUMENT;
+  calc->c.etemp = kelvin;
+  return TBLITE_AMD_OK;
+}
+
+tblite_amd_wavefunction tblite_amd_new_wavefunction(tblite_amd_context ctx)
+{
+  if (!ctx) return nullptr;
+  tblite_amd_wavefunction w = new (std::nothrow) tblite_amd_wavefunction_s();
+  if (w) w->ctx = ctx;
+  return w;
+}
+
+void tblite_amd_delete_wavefunction(tblite_amd_wavefunction* wfn)
+{
+  if (!wfn || !*wfn) return;
+  delete *wfn;
+  *wfn = nullptr;
+}
+
+int tblite_amd_get_wavefunction_array(tblite_amd_wavefunction wfn, const char* name, double* out,
+                                      int* count)
+{
+  if (!wfn || !name || !count) return TBLITE_AMD_ERR_ARGUMENT;
+  const std::vector<double>* v = nullptr;
+  const std::string n(name);
+  if (n == "qsh") v = &wfn->qsh;
+  else if (n == "qat") v = &wfn->qat;
+  else if (n == "dpat") v = &wfn->dpat;
+  else if (n == "qpat") v = &wfn->qpat;
+  else if (n == "emo") v = &wfn->emo;
+  else if (n == "focc") v = &wfn->focc;
+  else if (n == "n0sh") v = &wfn->n0sh;
+  else if (n == "n0at") v = &wfn->n0at;
+  else if (n == "coeff") v = &wfn->coeff;
+  else if (n == "density") v = &wfn->density;
+  else return TBLITE_AMD_ERR_ARGUMENT;
+  *count = (int)v->size();
+  if (out && !v->empty()) std::memcpy(out, v->data(), sizeof(double) * v->size());
+  return TBLITE_AMD_OK;
+}
+
+// Fill a wavefunction handle from molecule imol of the last batch.
+static int fill_wavefunction(tblite_amd_context ctx, int imol, tblite_amd_wavefunction w)
+{
+  const Plan& pl = ctx->plan;
+  const BatchDev& b = ctx->batch.b;
+  const int nat = pl.mol_nat[imol], nsh = pl.mol_nsh[imol], nao = pl.mol_nao[imol];
+  const int a0 = pl.mol_atom_off[imol], s0 = pl.mol_sh_off[imol], o0 = pl.mol_ao_off[imol];
+  w->nat = nat;
+  w->nsh = nsh;
+  w->nao = nao;
+  w->qsh.resize(nsh);
+  w->qat.resize(nat);
+  w->dpat.resize(3 * nat);
+  w->qpat.resize(6 * nat);
+  w->emo.resize(nao);
+  w->focc.resize(nao);
+  w->n0sh.resize(nsh);
+  w->n0at.assign(nat, 0.0);
+  HIP_TRY(ctx, hipMemcpy(w->qsh.data(), b.qsh + s0, 8 * nsh, hipMemcpyDeviceToHost));
+  HIP_TRY(ctx, hipMemcpy(w->qat.data(), b.qat + a0, 8 * nat, hipMemcpyDeviceToHost));
+  HIP_TRY(ctx, hipMemcpy(w->dpat.data(), b.dpat + 3 * a0, 8 * 3 * nat, hipMemcpyDeviceToHost));
+  HIP_TRY(ctx, hipMemcpy(w->qpat.data(), b.qpat + 6 * a0, 8 * 6 * nat, hipMemcpyDeviceToHost));
+  HIP_TRY(ctx, hipMemcpy(w->emo.data(), b.emo + o0, 8 * nao, hipMemcpyDeviceToHost));
+  HIP_TRY(ctx, hipMemcpy(w->focc.data(), b.focc + o0, 8 * nao, hipMemcpyDeviceToHost));
+  HIP_TRY(ctx, hipMemcpy(w->n0sh.data(), b.n0sh + s0, 8 * nsh, hipMemcpyDeviceToHost));
+  for (int s = 0; s < nsh; ++s) w->n0at[pl.sh_at[s0 + s]] += w->n0sh[s];
+  return TBLITE_AMD_OK;
+}
+
+int tblite_amd_xtb_singlepoint(tblite_amd_context ctx, tblite_amd_structure mol,
+                               tblite_amd_calculator calc, tblite_amd_wavefunction wfn,
+                               double accuracy, double* energy, int* iterations, int verbosity)
+{
+  if (!ctx || !mol || !calc || !energy) return TBLITE_AMD_ERR_ARGUMENT;
+  const double acc = accuracy > 0.0 ? accuracy : calc->c.accuracy;
+  int st = 0, it = 0;
+  int rc = flat_singlepoint(ctx, calc->c, 1, &mol->nat, mol->num.data(), mol->xyz.data(),
+                            &mol->charge, &mol->uhf, acc, energy, &it, &st);
+  if (iterations) *iterations = it;
+  if (ctx->have_batch && wfn) {
+    int rc2 = fill_wavefunction(ctx, 0, wfn);
+    if (rc2) return rc2;
+  }
+  if (verbosity > 0 && ctx->have_batch)
+    std::printf("tblite_amd: E = %.12f Eh after %d iterations (status %d)\n", *energy, it, st);
+  if (rc == TBLITE_AMD_ERR_NOT_CONVERGED) {
+    if (st & TA_STATUS_CHOLESKY) return TBLITE_AMD_ERR_CHOLESKY;
+    if (st & TA_STATUS_EIGEN) return TBLITE_AMD_ERR_EIGEN;
+    if (st & TA_STATUS_NAN) return TBLITE_AMD_ERR_NAN;
+    if (st & TA_STATUS_MIXER) return TBLITE_AMD_ERR_MIXER;
+  }
+  return rc;
+}
+
+int tblite_amd_xtb_singlepoint_batch(tblite_amd_context ctx, int nmol,
+                                     const tblite_amd_structure* mols,
+                                     tblite_amd_calculator calc, tblite_amd_wavefunction* wfns,
+                                     double accuracy, double* energies, int* iterations,
+                                     int* status)
+{
+  if (!ctx || nmol < 1 || !mols || !calc || !energies) return TBLITE_AMD_ERR_ARGUMENT;
+  std::vector<int> mol_nat(nmol), num, uhf(nmol);
+  std::vector<double> xyz, charge(nmol);
+  for (int m = 0; m < nmol; ++m) {
+    if (!mols[m]) {
+      set_error(ctx, "structure %d is NULL", m);
+      return TBLITE_AMD_ERR_ARGUMENT;
+    }
+    mol_nat[m] = mols[m]->nat;
+    num.insert(num.end(), mols[m]->num.begin(), mols[m]->num.end());
+    xyz.insert(xyz.end(), mols[m]->xyz.begin(), mols[m]->xyz.end());
+    charge[m] = mols[m]->charge;
+    uhf[m] = mols[m]->uhf;
+  }
+  const double acc = accuracy > 0.0 ? accuracy : calc->c.accuracy;
+  int rc = flat_singlepoint(ctx, calc->c, nmol, mol_nat.data(), num.data(), xyz.data(),
+                            charge.data(), uhf.data(), acc, energies, iterations, status);
+  if (ctx->have_batch && wfns)
+    for (int m = 0; m < nmol; ++m)
+      if (wfns[m]) {
+        int rc2 = fill_wavefunction(ctx, m, wfns[m]);
+        if (rc2) return rc2;
+      }
+  return rc;
+}
+
+int tblite_amd_singlepoint_flat(tblite_amd_context ctx, tblite_amd_calculator calc, int nmol,
+                                const int* mol_nat, const int* num, const double* xyz,
+                                const double* charge, const int* uhf, double accuracy,
+                                double* energies, int* iterations, int* status)
+{
+  if (!ctx || !calc) return TBLITE_AMD_ERR_ARGUMENT;
+  const double acc = accuracy > 0.0 ? accuracy : calc->c.accuracy;
+  return flat_singlepoint(ctx, calc->c, nmol, mol_nat, num, xyz, charge, uhf, acc, energies,
+                          iterations, status);
+}
+
+int tblite_amd_batch_dims(tblite_amd_context ctx, int imol, int* nat, int* nsh, int* nao)
+{
+  int rc = check_batch(ctx, imol);
+  if (rc) return rc;
+  if (nat) *nat = ctx->plan.mol_nat[imol];
+  if (nsh) *nsh = ctx->plan.mol_nsh[imol];
+  if (nao) *nao = ctx->plan.mol_nao[imol];
+  return TBLITE_AMD_OK;
+}
+
+int tblite_amd_batch_get_cn(tblite_amd_context ctx, int imol, double* cn)
+{
+  int rc = check_batch(ctx, imol);
+  if (rc) return rc;
+  if (!cn) return TBLITE_AMD_ERR_ARGUMENT;
+  HIP_TRY(ctx, hipMemcpy(cn, ctx->batch.b.cn + ctx->plan.mol_atom_off[imol],
+                         8 * ctx->plan.mol_nat[imol], hipMemcpyDeviceToHost));
+  return TBLITE_AMD_OK;
+}
+
+int tblite_amd_batch_get_selfenergy(tblite_amd_context ctx, int imol, double* se)
+{
+  int rc = check_batch(ctx, imol);
+  if (rc) return rc;
+  if (!se) return TBLITE_AMD_ERR_ARGUMENT;
+  HIP_TRY(ctx, hipMemcpy(se, ctx->batch.b.selfenergy + ctx->plan.mol_sh_off[imol],
+                         8 * ctx->plan.mol_nsh[imol], hipMemcpyDeviceToHost));
+  return TBLITE_AMD_OK;
+}
+
+int tblite_amd_batch_get_adjlist(tblite_amd_context ctx, int imol, int* inl, int* nnl, int* nlat,
+                                 int capacity)
+{
+  int rc = check_batch(ctx, imol);
+  if (rc) return rc;
+  const BatchDev& b = ctx->batch.b;
+  const int nat = ctx->plan.mol_nat[imol], a0 = ctx->plan.mol_atom_off[imol];
+  int nnz = 0;
+  HIP_TRY(ctx, hipMemcpy(&nnz, b.mol_nnz + imol, sizeof(int), hipMemcpyDeviceToHost));
+  if (inl) HIP_TRY(ctx, hipMemcpy(inl, b.nl_inl + a0, 4 * nat, hipMemcpyDeviceToHost));
+  if (nnl) HIP_TRY(ctx, hipMemcpy(nnl, b.nl_nnl + a0, 4 * nat, hipMemcpyDeviceToHost));
+  if (nlat) {
+    if (capacity < nnz) {
+      set_error(ctx, "neighbour list needs %d entries, capacity %d", nnz, capacity);
+      return TBLITE_AMD_ERR_ARGUMENT;
+    }
+    if (nnz > 0)
+      HIP_TRY(ctx, hipMemcpy(nlat, b.nl_jat + ctx->plan.mol_nl_off[imol], 4 * (size_t)nnz,
+                             hipMemcpyDeviceToHost));
+  }
+  return nnz;
+}
+
+int tblite_amd_batch_get_integrals(tblite_amd_context ctx, int imol, double* overlap,
+                                   double* dpint, double* qpint, double* hamiltonian)
+{
+  int rc = check_batch(ctx, imol);
+  if (rc) return rc;
+  const BatchDev& b = ctx->batch.b;
+  const int n = ctx->plan.mol_nao[imol];
+  const int64_t off = ctx->plan.mol_mat_off[imol];
+  const size_t n2 = (size_t)n * n;
+  if (overlap) HIP_TRY(ctx, hipMemcpy(overlap, b.S + off, 8 * n2, hipMemcpyDeviceToHost));
+  if (hamiltonian) HIP_TRY(ctx, hipMemcpy(hamiltonian, b.H + off, 8 * n2, hipMemcpyDeviceToHost));
+  std::vector<double> tmp;
+  if (dpint) {
+    tmp.resize(3 * n2);
+    HIP_TRY(ctx, hipMemcpy(tmp.data(), b.D + 3 * off, 8 * 3 * n2, hipMemcpyDeviceToHost));
+    for (size_t e = 0; e < n2; ++e)
+      for (int k = 0; k < 3; ++k) dpint[3 * e + k] = tmp[k * n2 + e];
+  }
+  if (qpint) {
+    tmp.resize(6 * n2);
+    HIP_TRY(ctx, hipMemcpy(tmp.data(), b.Q + 6 * off, 8 * 6 * n2, hipMemcpyDeviceToHost));
+    for (size_t e = 0; e < n2; ++e)
+      for (int k = 0; k < 6; ++k) qpint[6 * e + k] = tmp[k * n2 + e];
+  }
+  return TBLITE_AMD_OK;
+}
+
+int tblite_amd_batch_get_charges(tblite_amd_context ctx, int imol, double* qsh, double* qat,
+                                 double* dpat, double* qpat, double* emo, double* focc)
+{
+  int rc = check_batch(ctx, imol);
+  if (rc) return rc;
+  tblite_amd_wavefunction_s w;
+  w.ctx = ctx;
+  rc = fill_wavefunction(ctx, imol, &w);
+  if (rc) return rc;
+  if (qsh) std::memcpy(qsh, w.qsh.data(), 8 * w.qsh.size());
+  if (qat) std::memcpy(qat, w.qat.data(), 8 * w.qat.size());
+  if (dpat) std::memcpy(dpat, w.dpat.data(), 8 * w.dpat.size());
+  if (qpat) std::memcpy(qpat, w.qpat.data(), 8 * w.qpat.size());
+  if (emo) std::memcpy(emo, w.emo.data(), 8 * w.emo.size());
+  if (focc) std::memcpy(focc, w.focc.data(), 8 * w.focc.size());
+  return TBLITE_AMD_OK;
+}
+
+int tblite_amd_batch_get_energy_parts(tblite_amd_context ctx, int imol, double* erep,
+                                      double* eelec)
+{
+  int rc = check_batch(ctx, imol);
+  if (rc) return rc;
+  const BatchDev& b = ctx->batch.b;
+  if (erep) HIP_TRY(ctx, hipMemcpy(erep, b.erep_tot + imol, 8, hipMemcpyDeviceToHost));
+  if (eelec) HIP_TRY(ctx, hipMemcpy(eelec, b.eelec + imol, 8, hipMemcpyDeviceToHost));
+  return TBLITE_AMD_OK;
+}
+
+int tblite_amd_last_timings(tblite_amd_context ctx, double* times, int n)
+{
+  if (!ctx || !times) return TBLITE_AMD_ERR_ARGUMENT;
+  for (int i = 0; i < n && i < 5; ++i) times[i] = ctx->timings[i];
+  return TBLITE_AMD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,572 @@
+// Per-molecule setup kernels: coordination numbers, repulsion, self energies,
+// Coulomb interaction matrices, the neighbour list, and the overlap /
+// multipole / H0 integrals.
+//
+// Launch shapes: k_setup and k_adjlist run one workgroup per molecule (the
+// work is O(nat^2) per molecule); k_integrals runs over a host-built work
+// list of (molecule, chunk) items so that large molecules spread over many
+// workgroups.
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "common.h"
+#include "device_util.h"
+
+#define SETUP_THREADS 256
+
+// ---------------------------------------------------------------- k_fill
+__global__ void k_fill(double* __restrict__ p, int64_t n, double v)
+{
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+// --------------------------------------------------------------- k_setup
+// ncoord_dexp (src/coulomb.cu:173-245 text), get_repulsion_derivs energy
+// (src/xtb.cu:62-91 text), get_mrad (src/coulomb.cu:387-397 text),
+// get_selfenergy (src/h0.cu:137-170 text), get_occupation (src/h0.cu:61-91),
+// get_amat_0d (src/coulomb.cu:15-47 text), get_multipole_matrix_0d
+// (src/coulomb.cu:439-467 text).
+__device__ __forceinline__ double exp_count(double k, double r, double r0)
+{
+  return 1.0 / (1.0 + exp(-k * (r0 / r - 1.0)));
+}
+
+__global__ __launch_bounds__(SETUP_THREADS) void k_setup(BatchDev b)
+{
+  const int m = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int a0 = b.mol_atom_off[m], nat = b.mol_nat[m];
+  const int s0 = b.mol_sh_off[m], nsh = b.mol_nsh[m];
+  const double* __restrict__ xyz = b.xyz + 3 * (int64_t)a0;
+  const int* __restrict__ el = b.at_el + a0;
+  const DevParams& P = *b.par;
+
+  const double cn_cut2 = P.cn_cutoff * P.cn_cutoff;
+  const double rep_cut2 = P.rep_cutoff * P.rep_cutoff;
+  for (int i = tid; i < nat; i += blockDim.x) {
+    const DevElem& ei = P.el[el[i]];
+    const double xi = xyz[3 * i], yi = xyz[3 * i + 1], zi = xyz[3 * i + 2];
+    double cn = 0.0, er = 0.0;
+    for (int j = 0; j < nat; ++j) {
+      if (j == i) continue;
+      const DevElem& ej = P.el[el[j]];
+      double dx = xi - xyz[3 * j], dy = yi - xyz[3 * j + 1], dz = zi - xyz[3 * j + 2];
+      double r2 = dx * dx + dy * dy + dz * dz;
+      if (r2 < 1.0e-12) continue;
+      double r1 = sqrt(r2);
+      if (r2 <= cn_cut2) {
+        double rc = ei.rcov + ej.rcov;
+        cn += exp_count(P.cn_ka, r1, rc) * exp_count(P.cn_kb, r1, rc + P.cn_rshift);
+      }
+      if (r2 <= rep_cut2) {
+        double alpha = sqrt(ei.arep * ej.arep);
+        double zeff = ei.zeff * ej.zeff;
+        double kexp = (ei.z <= 2 && ej.z <= 2) ? P.rep_kexp_light : P.rep_kexp;
+        double r1k = pow(r1, kexp);
+        double r1r = pow(r1, P.rep_rexp);
+        er += 0.5 * zeff * exp(-alpha * r1k) / r1r;
+      }
+    }
+    b.cn[a0 + i] = cn;
+    b.erep[a0 + i] = er;
+    double arg = cn - ei.mpvcn - P.mp_shift;
+    double t1 = exp(-P.mp_kexp * arg);
+    b.mrad[a0 + i] = ei.mprad + (P.mp_rmax - ei.mprad) / (1.0 + t1);
+  }
+  __syncthreads();
+
+  for (int s = tid; s < nsh; s += blockDim.x) {
+    int at = b.sh_at[s0 + s];
+    const DevElem& e = P.el[el[at]];
+    int ish = b.sh_ish[s0 + s];
+    b.selfenergy[s0 + s] = e.selfenergy[ish] - e.kcn[ish] * b.cn[a0 + at];
+    b.n0sh[s0 + s] = e.refocc[ish];
+  }
+
+  // ES2 interaction matrix, nsh x nsh, arithmetic mean of shell hardness
+  double* __restrict__ gam = b.gam + b.mol_gam_off[m];
+  const double g = P.gexp;
+  for (int idx = tid; idx < nsh * nsh; idx += blockDim.x) {
+    int i = idx / nsh, j = idx - i * nsh;
+    int ai = b.sh_at[s0 + i], aj = b.sh_at[s0 + j];
+    const DevElem& ei = P.el[el[ai]];
+    const DevElem& ej = P.el[el[aj]];
+    double eta = 0.5 * (ei.hardness[b.sh_ish[s0 + i]] + ej.hardness[b.sh_ish[s0 + j]]);
+    double v;
+    if (ai == aj) {
+      v = eta;
+    } else {
+      double dx = xyz[3 * aj] - xyz[3 * ai], dy = xyz[3 * aj + 1] - xyz[3 * ai + 1];
+      double dz = xyz[3 * aj + 2] - xyz[3 * ai + 2];
+      double r1 = sqrt(dx * dx + dy * dy + dz * dz);
+      v = 1.0 / pow(pow(r1, g) + pow(eta, -g), 1.0 / g);
+    }
+    gam[idx] = v;
+  }
+
+  // AES2 matrices, planar: sd[k][i][j] = amat_sd(k, j, i), etc.
+  const int64_t n2 = (int64_t)nat * nat;
+  double* __restrict__ sd = b.aes_sd + 3 * b.mol_pair_off[m];
+  double* __restrict__ dd = b.aes_dd + 9 * b.mol_pair_off[m];
+  double* __restrict__ sq = b.aes_sq + 6 * b.mol_pair_off[m];
+  const double* __restrict__ mrad = b.mrad + a0;
+  for (int64_t idx = tid; idx < n2; idx += blockDim.x) {
+    int i = (int)(idx / nat), j = (int)(idx - (int64_t)i * nat);
+    double v[3] = {0.0, 0.0, 0.0};
+    double g3 = 0.0, g5 = 0.0, f3 = 0.0, f5 = 0.0;
+    if (i != j) {
+      for (int k = 0; k < 3; ++k) v[k] = xyz[3 * i + k] - xyz[3 * j + k];
+      double r1 = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+      double g1 = 1.0 / r1;
+      g3 = g1 * g1 * g1;
+      g5 = g3 * g1 * g1;
+      double rr = 0.5 * (mrad[j] + mrad[i]) * g1;
+      f3 = 1.0 / (1.0 + 6.0 * pow(rr, P.dmp3));
+      f5 = 1.0 / (1.0 + 6.0 * pow(rr, P.dmp5));
+    }
+    for (int k = 0; k < 3; ++k) sd[k * n2 + idx] = v[k] * g3 * f3;
+    for (int k = 0; k < 3; ++k)
+      for (int l = 0; l < 3; ++l)
+        dd[(k * 3 + l) * n2 + idx] =
+            (i != j) ? ((k == l ? g3 * f5 : 0.0) - v[k] * v[l] * 3.0 * g5 * f5) : 0.0;
+    double gf = g5 * f5;
+    sq[0 * n2 + idx] = v[0] * v[0] * gf;
+    sq[1 * n2 + idx] = 2.0 * v[0] * v[1] * gf;
+    sq[2 * n2 + idx] = v[1] * v[1] * gf;
+    sq[3 * n2 + idx] = 2.0 * v[0] * v[2] * gf;
+    sq[4 * n2 + idx] = 2.0 * v[1] * v[2] * gf;
+    sq[5 * n2 + idx] = v[2] * v[2] * gf;
+  }
+}
+
+// ------------------------------------------------------------- k_adjlist
+// generate (src/adjlist.cu:15-44 text): half list, jat <= iat, skip
+// r2 < epsilon or r2 > cutoff^2.  Integer output, bit-exact: the squared
+// distance is formed as ((dx*dx + dy*dy) + dz*dz) with no contraction, the
+// same operation sequence as the CPU restatement.
+__device__ __forceinline__ double pair_r2_exact(const double* __restrict__ xyz, int i, int j)
+{
+#pragma clang fp contract(off)
+  double dx = xyz[3 * i] - xyz[3 * j];
+  double dy = xyz[3 * i + 1] - xyz[3 * j + 1];
+  double dz = xyz[3 * i + 2] - xyz[3 * j + 2];
+  double xx = dx * dx;
+  double yy = dy * dy;
+  double zz = dz * dz;
+  double s = xx + yy;
+  return s + zz;
+}
+
+__global__ __launch_bounds__(SETUP_THREADS) void k_adjlist(BatchDev b)
+{
+  __shared__ int red[SETUP_THREADS / TA_WAVE];
+  const int m = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int a0 = b.mol_atom_off[m], nat = b.mol_nat[m];
+  const double* __restrict__ xyz = b.xyz + 3 * (int64_t)a0;
+  const double cutoff = b.mol_cutoff[m];
+  const double cutoff2 = cutoff * cutoff;
+  const double eps = 2.220446049250313e-16;
+  int* __restrict__ jat_out = b.nl_jat + b.mol_nl_off[m];
+  int base = 0;
+  for (int chunk = 0; chunk < nat; chunk += blockDim.x) {
+    const int i = chunk + tid;
+    int cnt = 0;
+    if (i < nat)
+      for (int j = 0; j <= i; ++j) {
+        double r2 = pair_r2_exact(xyz, i, j);
+        if (!(r2 < eps || r2 > cutoff2)) ++cnt;
+      }
+    int tot;
+    int excl = block_exclusive_scan(cnt, red, &tot);
+    if (i < nat) {
+      b.nl_inl[a0 + i] = base + excl;
+      b.nl_nnl[a0 + i] = cnt;
+      int pos = base + excl;
+      for (int j = 0; j <= i; ++j) {
+        double r2 = pair_r2_exact(xyz, i, j);
+        if (!(r2 < eps || r2 > cutoff2)) jat_out[pos++] = j;
+      }
+    }
+    base += tot;
+  }
+  if (tid == 0) b.mol_nnz[m] = base;
+}
+
+// ----------------------------------------------------------- integrals
+// Overlap, dipole and quadrupole integrals over contracted Cartesian
+// Gaussians (multipole_cgto, src/integral/multipole.h:34-120 text; 1D moment
+// recursion multipole_3d src/integral/multipole.cu:411-457 text), the
+// Cartesian -> spherical transform (src/integral/trafo.h:6-108) and the
+// operator shift to the other centre (shift_operator, src/h0.cu:251-289).
+//
+// For each primitive pair the 1D factors are tabulated once per axis and
+// power pair, then combined for every Cartesian component pair.
+
+template <int L>
+struct Cart {
+  static constexpr int n = (L + 1) * (L + 2) / 2;
+  static constexpr int s = 2 * L + 1;
+};
+
+// Cartesian exponents (x, y, z), tblite order: s; p = y, z, x;
+// d = xx, yy, zz, xy, xz, yz
+__device__ __forceinline__ void cart_pow(int l, int c, int& px, int& py, int& pz)
+{
+  if (l == 0) {
+    px = py = pz = 0;
+  } else if (l == 1) {
+    px = (c == 2);
+    py = (c == 0);
+    pz = (c == 1);
+  } else {
+    const int tx[6] = {2, 0, 0, 1, 1, 0}, ty[6] = {0, 2, 0, 1, 0, 1}, tz[6] = {0, 0, 2, 0, 1, 1};
+    px = tx[c];
+    py = ty[c];
+    pz = tz[c];
+  }
+}
+
+// spherical component a of a Cartesian vector v (d order z2, xz, yz, x2-y2, xy)
+template <int L>
+__device__ __forceinline__ double to_sph(const double* v, int a)
+{
+  if constexpr (L < 2) {
+    return v[a];
+  } else {
+    const double s3 = 1.7320508075688772;
+    switch (a) {
+      case 0: return v[2] - 0.5 * (v[0] + v[1]);
+      case 1: return s3 * v[4];
+      case 2: return s3 * v[5];
+      case 3: return 0.5 * s3 * (v[0] - v[1]);
+      default: return s3 * v[3];
+    }
+  }
+}
+
+// Block for shell ish of element ei (center i) and shell jsh of element ej
+// (center j), vec = R_i - R_j.  Outputs in tblite order ij = jao + SJ*iao:
+// s[ij], d[3*ij + k], q[6*ij + k] (operator on center i, traceless).
+template <int LI, int LJ>
+__device__ void multipole_block(const DevElem& ej, int jsh, const DevElem& ei, int ish,
+                                double r2, const double vec[3], double intcut, double* s,
+                                double* d, double* q)
+{
+  constexpr int MI = Cart<LI>::n, MJ = Cart<LJ>::n, SI = Cart<LI>::s, SJ = Cart<LJ>::s;
+  constexpr int LL = LI + LJ;
+  double s3[MI * MJ], d3[MI * MJ * 3], q3[MI * MJ * 6];
+#pragma unroll
+  for (int x = 0; x < MI * MJ; ++x) s3[x] = 0.0;
+#pragma unroll
+  for (int x = 0; x < MI * MJ * 3; ++x) d3[x] = 0.0;
+#pragma unroll
+  for (int x = 0; x < MI * MJ * 6; ++x) q3[x] = 0.0;
+  const double sqrtpi3 = 5.568327996831707845; // pi^(3/2)
+  const double dfac[5] = {1.0, 1.0, 3.0, 15.0, 105.0};
+  const int npi = ei.nprim[ish], npj = ej.nprim[jsh];
+  for (int ip = 0; ip < npi; ++ip) {
+    const double ai = ei.alpha[ish][ip], ci = ei.coeff[ish][ip];
+    for (int jp = 0; jp < npj; ++jp) {
+      const double aj = ej.alpha[jsh][jp];
+      const double eab = ai + aj, oab = 1.0 / eab;
+      const double est = ai * aj * r2 * oab;
+      if (est > intcut) continue;
+      const double cc = ci * ej.coeff[jsh][jp] * exp(-est) * sqrtpi3 * oab * sqrt(oab);
+      double s1d[LL + 3];
+      {
+        double h = 0.5 * oab, pw = 1.0;
+#pragma unroll
+        for (int l = 0; l <= LL + 2; ++l) {
+          if (l & 1) {
+            s1d[l] = 0.0;
+          } else {
+            s1d[l] = pw * dfac[l / 2];
+            pw *= h;
+          }
+        }
+      }
+      // v1[axis][pi][pj][moment]
+      double v1[3][LI + 1][LJ + 1][3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const double rpi = -vec[k] * aj * oab;
+        const double rpj = vec[k] * ai * oab;
+#pragma unroll
+        for (int pi = 0; pi <= LI; ++pi)
+#pragma unroll
+          for (int pj = 0; pj <= LJ; ++pj) {
+            // (t + rpi)^pi (t + rpj)^pj expanded in powers of t = x - P
+            double vv[LL + 1];
+#pragma unroll
+            for (int l = 0; l <= LL; ++l) vv[l] = 0.0;
+            double ci_[LI + 1], cj_[LJ + 1];
+            {
+              const double bin[3][3] = {{1, 0, 0}, {1, 1, 0}, {1, 2, 1}};
+#pragma unroll
+              for (int a = 0; a <= LI; ++a) {
+                double pw = 1.0;
+                for (int e = 0; e < pi - a; ++e) pw *= rpi;
+                ci_[a] = (a <= pi) ? bin[pi][a] * pw : 0.0;
+              }
+#pragma unroll
+              for (int a = 0; a <= LJ; ++a) {
+                double pw = 1.0;
+                for (int e = 0; e < pj - a; ++e) pw *= rpj;
+                cj_[a] = (a <= pj) ? bin[pj][a] * pw : 0.0;
+              }
+            }
+#pragma unroll
+            for (int a = 0; a <= LI; ++a)
+#pragma unroll
+              for (int c = 0; c <= LJ; ++c) vv[a + c] += ci_[a] * cj_[c];
+            double e0 = 0.0, e1 = 0.0, e2 = 0.0;
+#pragma unroll
+            for (int l = 0; l <= LL; ++l) {
+              e0 += s1d[l] * vv[l];
+              e1 += (s1d[l + 1] + rpi * s1d[l]) * vv[l];
+              e2 += (s1d[l + 2] + 2.0 * rpi * s1d[l + 1] + rpi * rpi * s1d[l]) * vv[l];
+            }
+            v1[k][pi][pj][0] = e0;
+            v1[k][pi][pj][1] = e1;
+            v1[k][pi][pj][2] = e2;
+          }
+      }
+#pragma unroll
+      for (int mi = 0; mi < MI; ++mi) {
+        int ix, iy, iz;
+        cart_pow(LI, mi, ix, iy, iz);
+#pragma unroll
+        for (int mj = 0; mj < MJ; ++mj) {
+          int jx, jy, jz;
+          cart_pow(LJ, mj, jx, jy, jz);
+          const double* X = v1[0][ix][jx];
+          const double* Y = v1[1][iy][jy];
+          const double* Z = v1[2][iz][jz];
+          const int ij = mj + MJ * mi;
+          s3[ij] += cc * X[0] * Y[0] * Z[0];
+          d3[3 * ij + 0] += cc * X[1] * Y[0] * Z[0];
+          d3[3 * ij + 1] += cc * X[0] * Y[1] * Z[0];
+          d3[3 * ij + 2] += cc * X[0] * Y[0] * Z[1];
+          q3[6 * ij + 0] += cc * X[2] * Y[0] * Z[0];
+          q3[6 * ij + 1] += cc * X[1] * Y[1] * Z[0];
+          q3[6 * ij + 2] += cc * X[0] * Y[2] * Z[0];
+          q3[6 * ij + 3] += cc * X[1] * Y[0] * Z[1];
+          q3[6 * ij + 4] += cc * X[0] * Y[1] * Z[1];
+          q3[6 * ij + 5] += cc * X[0] * Y[0] * Z[2];
+        }
+      }
+    }
+  }
+  // Cartesian -> spherical on both indices, then remove the quadrupole trace
+#pragma unroll
+  for (int comp = 0; comp < 10; ++comp) {
+    // half transform over j for every Cartesian i
+    double half[MI][SJ];
+#pragma unroll
+    for (int mi = 0; mi < MI; ++mi) {
+      double row[MJ];
+#pragma unroll
+      for (int mj = 0; mj < MJ; ++mj) {
+        const int ij = mj + MJ * mi;
+        row[mj] = comp == 0 ? s3[ij] : (comp < 4 ? d3[3 * ij + comp - 1] : q3[6 * ij + comp - 4]);
+      }
+#pragma unroll
+      for (int a = 0; a < SJ; ++a) half[mi][a] = to_sph<LJ>(row, a);
+    }
+#pragma unroll
+    for (int a = 0; a < SJ; ++a) {
+      double col[MI];
+#pragma unroll
+      for (int mi = 0; mi < MI; ++mi) col[mi] = half[mi][a];
+#pragma unroll
+      for (int bb = 0; bb < SI; ++bb) {
+        const double val = to_sph<LI>(col, bb);
+        const int ij = a + SJ * bb;
+        if (comp == 0) s[ij] = val;
+        else if (comp < 4) d[3 * ij + comp - 1] = val;
+        else q[6 * ij + comp - 4] = val;
+      }
+    }
+  }
+#pragma unroll
+  for (int ij = 0; ij < SI * SJ; ++ij) {
+    double* qq = q + 6 * ij;
+    const double tr = 0.5 * (qq[0] + qq[2] + qq[5]);
+    qq[0] = 1.5 * qq[0] - tr;
+    qq[1] = 1.5 * qq[1];
+    qq[2] = 1.5 * qq[2] - tr;
+    qq[3] = 1.5 * qq[3];
+    qq[4] = 1.5 * qq[4];
+    qq[5] = 1.5 * qq[5] - tr;
+  }
+}
+
+__device__ void shell_block(int li, int lj, const DevElem& ej, int jsh, const DevElem& ei, int ish,
+                            double r2, const double vec[3], double intcut, double* s, double* d,
+                            double* q)
+{
+  switch (li * 3 + lj) {
+    case 0: multipole_block<0, 0>(ej, jsh, ei, ish, r2, vec, intcut, s, d, q); break;
+    case 1: multipole_block<0, 1>(ej, jsh, ei, ish, r2, vec, intcut, s, d, q); break;
+    case 2: multipole_block<0, 2>(ej, jsh, ei, ish, r2, vec, intcut, s, d, q); break;
+    case 3: multipole_block<1, 0>(ej, jsh, ei, ish, r2, vec, intcut, s, d, q); break;
+    case 4: multipole_block<1, 1>(ej, jsh, ei, ish, r2, vec, intcut, s, d, q); break;
+    case 5: multipole_block<1, 2>(ej, jsh, ei, ish, r2, vec, intcut, s, d, q); break;
+    case 6: multipole_block<2, 0>(ej, jsh, ei, ish, r2, vec, intcut, s, d, q); break;
+    case 7: multipole_block<2, 1>(ej, jsh, ei, ish, r2, vec, intcut, s, d, q); break;
+    default: multipole_block<2, 2>(ej, jsh, ei, ish, r2, vec, intcut, s, d, q); break;
+  }
+}
+
+// Work item kinds for k_integrals
+#define WORK_PAIRS 0
+#define WORK_ONSITE 1
+
+// get_hamiltonian (src/h0.cu:306-477 text) plus the on-site blocks.  One
+// thread per atom pair (neighbour-list entry) or per atom (on-site); each
+// thread writes a disjoint set of matrix elements, so no atomics.
+__global__ __launch_bounds__(128) void k_integrals(BatchDev b, const int* __restrict__ work_mol,
+                                                 const int* __restrict__ work_kind,
+                                                 const int* __restrict__ work_start)
+{
+  const int w = blockIdx.x;
+  const int m = work_mol[w];
+  const int kind = work_kind[w];
+  const int item = work_start[w] + threadIdx.x;
+  const int a0 = b.mol_atom_off[m], nat = b.mol_nat[m];
+  const int s0 = b.mol_sh_off[m];
+  const int n = b.mol_nao[m];
+  const int64_t n2 = (int64_t)n * n;
+  const double* __restrict__ xyz = b.xyz + 3 * (int64_t)a0;
+  const int* __restrict__ el = b.at_el + a0;
+  const DevParams& P = *b.par;
+  double* __restrict__ S = b.S + b.mol_mat_off[m];
+  double* __restrict__ H = b.H + b.mol_mat_off[m];
+  double* __restrict__ D = b.D + 3 * b.mol_mat_off[m];
+  double* __restrict__ Q = b.Q + 6 * b.mol_mat_off[m];
+  const double* __restrict__ se = b.selfenergy + s0;
+  const double intcut = b.intcut;
+
+  int iat, jat;
+  if (kind == WORK_PAIRS) {
+    if (item >= b.mol_nnz[m]) return;
+    // locate the owning atom: largest iat with inl(iat) <= item
+    const int* inl = b.nl_inl + a0;
+    int lo = 0, hi = nat - 1;
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (inl[mid] <= item) lo = mid;
+      else hi = mid - 1;
+    }
+    // skip atoms with empty lists that share the same offset
+    while (lo < nat - 1 && inl[lo + 1] <= item) ++lo;
+    iat = lo;
+    jat = b.nl_jat[b.mol_nl_off[m] + item];
+  } else {
+    if (item >= nat) return;
+    iat = jat = item;
+  }
+  const int ei_ = el[iat], ej_ = el[jat];
+  const DevElem& ei = P.el[ei_];
+  const DevElem& ej = P.el[ej_];
+  // first shell / AO of each atom
+  int is = 0, ii0 = 0;
+  {
+    // shells of this molecule are ordered by atom; scan the atom's first shell
+    // through the shell->atom map (nsh per atom is at most 3)
+    const int* sh_at = b.sh_at + s0;
+    int lo = 0, hi = b.mol_nsh[m] - 1;
+    while (lo < hi) {
+      int mid = (lo + hi) >> 1;
+      if (sh_at[mid] < iat) lo = mid + 1;
+      else hi = mid;
+    }
+    is = lo;
+    ii0 = b.sh_iao[s0 + is];
+  }
+  int js = is, jj0 = ii0;
+  if (jat != iat) {
+    const int* sh_at = b.sh_at + s0;
+    int lo = 0, hi = b.mol_nsh[m] - 1;
+    while (lo < hi) {
+      int mid = (lo + hi) >> 1;
+      if (sh_at[mid] < jat) lo = mid + 1;
+      else hi = mid;
+    }
+    js = lo;
+    jj0 = b.sh_iao[s0 + js];
+  }
+  double vec[3];
+  for (int k = 0; k < 3; ++k) vec[k] = xyz[3 * iat + k] - xyz[3 * jat + k];
+  const double r2 = vec[0] * vec[0] + vec[1] * vec[1] + vec[2] * vec[2];
+  const double rr = sqrt(sqrt(r2) / (ej.atomic_rad + ei.atomic_rad));
+
+  double st[25], dt[75], qt[150];
+  for (int ish = 0; ish < ei.nsh; ++ish) {
+    const int li = ei.ang[ish];
+    const int ii = b.sh_iao[s0 + is + ish];
+    for (int jsh = 0; jsh < ej.nsh; ++jsh) {
+      const int lj = ej.ang[jsh];
+      const int jj = b.sh_iao[s0 + js + jsh];
+      shell_block(li, lj, ej, jsh, ei, ish, r2, vec, intcut, st, dt, qt);
+      const int si = 2 * li + 1, sj = 2 * lj + 1;
+      if (iat != jat) {
+        const double shp = (1.0 + ei.shpoly[ish] * rr) * (1.0 + ej.shpoly[jsh] * rr);
+        const double hij =
+            0.5 * (se[is + ish] + se[js + jsh]) * P.hscale[ei_][ej_][ish][jsh] * shp;
+        for (int iao = 0; iao < si; ++iao)
+          for (int jao = 0; jao < sj; ++jao) {
+            const int ij = jao + sj * iao;
+            const double sv = st[ij];
+            const double* di = dt + 3 * ij;
+            const double* qi = qt + 6 * ij;
+            // operator moved to center j: r - R_j = (r - R_i) + vec
+            double dj[3], qj[6];
+            dj[0] = di[0] + vec[0] * sv;
+            dj[1] = di[1] + vec[1] * sv;
+            dj[2] = di[2] + vec[2] * sv;
+            double t0 = 2 * vec[0] * di[0] + vec[0] * vec[0] * sv;
+            double t2 = 2 * vec[1] * di[1] + vec[1] * vec[1] * sv;
+            double t5 = 2 * vec[2] * di[2] + vec[2] * vec[2] * sv;
+            double t1 = vec[0] * di[1] + vec[1] * di[0] + vec[0] * vec[1] * sv;
+            double t3 = vec[0] * di[2] + vec[2] * di[0] + vec[0] * vec[2] * sv;
+            double t4 = vec[1] * di[2] + vec[2] * di[1] + vec[1] * vec[2] * sv;
+            double tr = 0.5 * (t0 + t2 + t5);
+            qj[0] = qi[0] + 1.5 * t0 - tr;
+            qj[1] = qi[1] + 1.5 * t1;
+            qj[2] = qi[2] + 1.5 * t2 - tr;
+            qj[3] = qi[3] + 1.5 * t3;
+            qj[4] = qi[4] + 1.5 * t4;
+            qj[5] = qi[5] + 1.5 * t5 - tr;
+            const int64_t a_ij = (int64_t)(ii + iao) * n + (jj + jao); // fortran (jj+jao, ii+iao)
+            const int64_t a_ji = (int64_t)(jj + jao) * n + (ii + iao); // fortran (ii+iao, jj+jao)
+            S[a_ij] = sv;
+            S[a_ji] = sv;
+            H[a_ij] = sv * hij;
+            H[a_ji] = sv * hij;
+            for (int k = 0; k < 3; ++k) {
+              D[k * n2 + a_ij] = di[k];
+              D[k * n2 + a_ji] = dj[k];
+            }
+            for (int k = 0; k < 6; ++k) {
+              Q[k * n2 + a_ij] = qi[k];
+              Q[k * n2 + a_ji] = qj[k];
+            }
+          }
+      } else {
+        for (int iao = 0; iao < si; ++iao)
+          for (int jao = 0; jao < sj; ++jao) {
+            const int ij = jao + sj * iao;
+            const int64_t a_ij = (int64_t)(ii + iao) * n + (jj + jao);
+            S[a_ij] = st[ij];
+            H[a_ij] = (ish == jsh && iao == jao) ? se[is + ish] : 0.0;
+            for (int k = 0; k < 3; ++k) D[k * n2 + a_ij] = dt[3 * ij + k];
+            for (int k = 0; k < 6; ++k) Q[k * n2 + a_ij] = qt[6 * ij + k];
+          }
+      }
+    }
+  }
+}

@@ -1,0 +1,202 @@
+#!/usr/bin/env python
+"""Batched GFN2-xTB singlepoint throughput on one MI355X.
+
+A step is one full singlepoint pass (CN, repulsion, Coulomb matrices,
+neighbour list, integrals/H0, SCF to convergence) over the whole batch, with
+the coordinates already resident in HBM; the batch is the workload's seeded
+molecules (default: 4096 (H2O)n clusters, n ~ U{2..16}).  Prints one JSON
+line.  Beside the GPU number it reports:
+  roofline      FP64 work of the SCF (the dominant kernel) / its measured time
+  cpu_baseline  the CPU oracle (FP64 restatement, OpenMP over molecules) on a
+                bounded sample of the same molecules, timed on this host
+  per_molecule  the plain one-molecule-per-call path on a sample
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "tblite-cuda_amd"))
+
+import tblite_amd  # noqa: E402
+import workloads  # noqa: E402
+
+PEAK_FP64 = 78.6e12  # MI355X dense FP64 (matrix and vector), FLOP/s
+
+WORKLOADS = {
+    "config1": ("4096 seeded (H2O)n clusters, n ~ U{2..16}, liquid density", workloads.config1),
+    "config0": ("one seeded (H2O)8 cluster", workloads.config0),
+    "config3": ("2048 seeded tree-grown C/N/O/S organics, 4..60 heavy atoms", workloads.config3),
+}
+
+
+def scf_flops(nao, nocc, iters):
+    """Algorithmic FP64 flops of the SCF: per iteration (16/3) n^3 + 2 n^2 o + 40 n^2
+    (reduction, tridiagonal-equivalent eigensolve, back-transform, density, Fock,
+    Mulliken), plus n^3/3 once for the factorisation of S."""
+    n = nao.astype(np.float64)
+    o = nocc.astype(np.float64)
+    per_iter = (16.0 / 3.0) * n**3 + 2.0 * n**2 * o + 40.0 * n**2
+    return float(np.sum(iters * per_iter + n**3 / 3.0))
+
+
+def cpu_baseline(mols, seconds, threads):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    orc = oracle.Oracle()
+    # calibrate on a small sample, then run a sample sized to ~`seconds`
+    probe = mols[: min(len(mols), 4 * threads)]
+    t = time.perf_counter()
+    orc.singlepoint_many(probe, nthreads=threads)
+    dt = max(time.perf_counter() - t, 1e-3)
+    count = int(min(len(mols), max(len(probe), len(probe) * seconds / dt)))
+    sample = mols[:count]
+    t = time.perf_counter()
+    e, it, st = orc.singlepoint_many(sample, nthreads=threads)
+    dt = time.perf_counter() - t
+    return sample, e, {
+        "value": round(count / dt, 3),
+        "unit": "singlepoints/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"first {count} molecules of the same workload, CPU oracle (FP64 C restatement), "
+                  f"OpenMP over molecules, {dt:.1f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="config1", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--per-molecule-sample", type=int, default=64)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); molecules are independent, "
+              "so a multi-GPU run is a split of the batch with no exchange", file=sys.stderr)
+        sys.exit(2)
+
+    desc, gen = WORKLOADS[args.workload]
+    t0 = time.perf_counter()
+    mols = gen()
+    t_gen = time.perf_counter() - t0
+    mol_nat, num, xyz = workloads.flatten(mols)
+    nmol = len(mols)
+    print(f"[bench] {args.workload}: {nmol} molecules, {len(num)} atoms (generated in {t_gen:.1f} s)",
+          file=sys.stderr, flush=True)
+
+    eng = tblite_amd.Engine(device=0)
+    t0 = time.perf_counter()
+    eng.prepare(mol_nat, num)
+    eng.set_geometry(xyz)
+    t_prep = time.perf_counter() - t0
+
+    for _ in range(args.warmup):
+        eng.run()
+    step_ms = []
+    phase = []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        eng.run()
+        step_ms.append((time.perf_counter() - ts) * 1e3)
+        phase.append(eng.timings())
+    t_total = time.perf_counter() - t_start
+    e, it, st = eng.results()
+    phase = np.array(phase)
+    ms_per_step = t_total / args.steps * 1e3
+    value = nmol / (t_total / args.steps)
+    print(f"[bench] {ms_per_step:.2f} ms/step, {value:.1f} singlepoints/s; phases (ms): "
+          f"setup {phase[:, 0].mean():.3f} adjlist {phase[:, 1].mean():.3f} "
+          f"integrals {phase[:, 2].mean():.3f} scf {phase[:, 3].mean():.3f}",
+          file=sys.stderr, flush=True)
+
+    # algorithmic FP64 work of the SCF kernels
+    nao = np.array([eng.dims(m)[2] for m in range(nmol)])
+    nel = np.array([sum({1: 1, 6: 4, 7: 5, 8: 6, 16: 6}[int(z)] for z in m[0]) for m in mols])
+    nocc = (nel + 1) // 2
+    flops = scf_flops(nao, nocc, it.astype(np.float64))
+    scf_s = phase[:, 3].mean() / 1e3
+    achieved = flops / scf_s
+    roofline = {
+        "bound": "mfma",
+        "achieved": round(achieved / 1e12, 4),
+        "peak": PEAK_FP64 / 1e12,
+        "unit": "TFLOP/s",
+        "frac": round(achieved / PEAK_FP64, 5),
+        "traffic": None,
+        "kernel": "k_scf (all size buckets, HIP events around the SCF phase)",
+        "flops_per_step": flops,
+    }
+
+    # plain one-molecule-per-call path on a sample
+    k = min(args.per_molecule_sample, nmol)
+    t = time.perf_counter()
+    for m in range(k):
+        eng.singlepoint(mols[m][0], mols[m][1])
+    per_mol = k / (time.perf_counter() - t)
+
+    extra = {
+        "prepare_s": round(t_prep, 3),
+        "step_ms_p50": round(float(np.percentile(step_ms, 50)), 3),
+        "step_ms_p99": round(float(np.percentile(step_ms, 99)), 3),
+        "phase_ms": {name: round(float(phase[:, i].mean()), 3)
+                     for i, name in enumerate(["setup", "adjlist", "integrals", "scf", "total"])},
+        "scf_iterations_mean": round(float(it.mean()), 3),
+        "status_nonzero": int(np.count_nonzero(st)),
+        "per_molecule_path": {"value": round(per_mol, 2), "unit": "singlepoints/s",
+                              "sample": k, "speedup_batched": round(value / per_mol, 2)},
+    }
+
+    cpu = None
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+        sample, e_cpu, cpu = cpu_baseline(mols, args.cpu_seconds, threads)
+        d = np.abs(e[: len(sample)] - e_cpu)
+        extra["parity_sample"] = {"molecules": len(sample), "max_abs_dE": float(d.max())}
+        cpu["cpu_model"] = _cpu_model()
+
+    line = {
+        "metric": "singlepoints/s",
+        "value": round(value, 2),
+        "unit": "singlepoints/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {desc}", "molecules": nmol,
+                   "atoms": int(len(num)), "accuracy": 1.0, "etemp_K": 300.0,
+                   "max_iter": 250, "mixer_damping": 0.4},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(line))
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+if __name__ == "__main__":
+    main()

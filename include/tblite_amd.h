@@ -183,6 +183,24 @@ int tblite_amd_singlepoint_flat(tblite_amd_context ctx, tblite_amd_calculator ca
                                 const double* charge, const int* uhf, double accuracy,
                                 double* energies, int* iterations, int* status);
 
+/* Device-resident batches.  prepare() plans a batch from its composition
+ * (atom counts, elements, charges, spins) and allocates the device workspace;
+ * set_geometry() loads coordinates (host or device pointer, xyz(3,total)
+ * bohr); run() executes the whole singlepoint pipeline on the resident
+ * coordinates without host traffic; results() copies energies, iteration
+ * counts and status words out.  The same prepared batch can be run again
+ * with new coordinates (geometry optimisation, dynamics, benchmarking).
+ * The flat and batched entry points replace the prepared batch. */
+int tblite_amd_batch_prepare(tblite_amd_context ctx, tblite_amd_calculator calc, int nmol,
+                             const int* mol_nat, const int* num, const double* charge,
+                             const int* uhf, double accuracy);
+int tblite_amd_batch_set_geometry(tblite_amd_context ctx, const double* xyz, int on_device);
+/* Device address of the resident coordinate buffer of the prepared batch. */
+const double* tblite_amd_batch_geometry_device(tblite_amd_context ctx);
+int tblite_amd_batch_run(tblite_amd_context ctx);
+int tblite_amd_batch_results(tblite_amd_context ctx, double* energies, int* iterations,
+                             int* status);
+
 /* ------------------------------------------------- stage-level access
  * Intermediate results of the last flat/batched call, for stage-by-stage
  * parity checks.  All copy to host memory; imol indexes the last batch.
@@ -216,6 +234,13 @@ int tblite_amd_batch_get_energy_parts(tblite_amd_context ctx, int imol, double* 
  * times[0] setup (CN, repulsion, Coulomb matrices), [1] neighbour list,
  * [2] integrals/H0, [3] SCF, [4] total.  n = number of entries wanted. */
 int tblite_amd_last_timings(tblite_amd_context ctx, double* times, int n);
+
+/* Per-molecule SCF phase clocks of the last batch (only when the context was
+ * created with TBLITE_AMD_PROFILE=1 in the environment): out receives
+ * nmol x 12 values, wall-clock ticks at 100 MHz for init, mixing, potential,
+ * Fock, transform, Jacobi, occupation, density, Mulliken, energy, then the
+ * Jacobi sweep count and the iteration count.  Returns 12 or < 0. */
+int tblite_amd_last_profile(tblite_amd_context ctx, long long* out);
 
 #ifdef __cplusplus
 }

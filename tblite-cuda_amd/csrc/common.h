@@ -143,7 +143,25 @@ struct BatchDev {
   double* big_c;
   double* big_t;
   const int64_t* mol_big_off;     // offset into big_a / big_c / big_t, -1 when LDS-resident
+
+  // optional per-molecule phase clocks of the SCF kernel (TA_PROF_SLOTS each), may be null
+  long long* prof;
 };
+
+// SCF phase clock slots (wall clock ticks at 100 MHz; last two are counts)
+#define TA_PROF_INIT 0
+#define TA_PROF_MIX 1
+#define TA_PROF_POT 2
+#define TA_PROF_FOCK 3
+#define TA_PROF_TRANSFORM 4
+#define TA_PROF_JACOBI 5
+#define TA_PROF_OCC 6
+#define TA_PROF_DENSITY 7
+#define TA_PROF_MULLIKEN 8
+#define TA_PROF_ENERGY 9
+#define TA_PROF_SWEEPS 10
+#define TA_PROF_ITERS 11
+#define TA_PROF_SLOTS 12
 
 // LDS layout of the SCF kernel, in doubles (integer arrays in units of ints
 // from the start of the region).  ld is the padded AO dimension of the
@@ -173,18 +191,17 @@ __host__ __device__ inline ScfLds scf_lds_layout(int ld, int natmax, int nshmax,
   L.focc = off; off += ld;
   L.emo = off; off += ld;
   L.fs = off; off += ld;
-  L.rc = off; off += ld / 2;
-  L.rs = off; off += ld / 2;
-  L.rt = off; off += ld / 2;
+  L.rc = L.rs = L.rt = off;
+  // work: potentials (ld + nsh + 10 nat), Jacobi records (7 ld, double2
+  // aligned), Mulliken partial sums (11 ld)
   int w1 = ld + nshmax + 10 * natmax, w2 = 11 * ld;
   L.work = off; off += (w1 > w2 ? w1 : w2);
   L.iint = 2 * off;
   L.rank = L.iint;
   L.inv = L.rank + ld;
   L.occ = L.inv + ld;
-  L.rp = L.occ + ld;
-  L.rq = L.rp + ld / 2;
-  off += (4 * ld + 1) / 2 + 1;
+  L.rp = L.rq = L.occ + ld;
+  off += (3 * ld + 1) / 2 + 1;
   L.total = off;
   return L;
 }

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -23,7 +24,7 @@ __global__ void k_setup(BatchDev b);
 __global__ void k_adjlist(BatchDev b);
 __global__ void k_integrals(BatchDev b, const int* work_mol, const int* work_kind,
                             const int* work_start);
-template <int NT, int MAXT, bool MATS_LDS>
+template <int NT, int MAXT, int MAXB, int MAXC, bool MATS_LDS>
 __global__ void k_scf(BatchDev b, const int* mols, int ldmax, int natmax, int nshmax);
 
 namespace {
@@ -230,6 +231,9 @@ struct tblite_amd_context_s {
   bool have_batch = false;
   double timings[5] = {0, 0, 0, 0, 0};
   size_t lds_limit = 0;
+  tblite_amd_calculator prepared = nullptr;
+  double prepared_acc = 1.0;
+  bool profile = false;
 };
 
 struct tblite_amd_structure_s {
@@ -508,6 +512,7 @@ int upload_plan(tblite_amd_context ctx, Calculator& c, const Plan& pl, DeviceBat
           2 * dbytes((int64_t)nmol * mem);
   need += 3 * dbytes(nmol) + 2 * ibytes(nmol);
   need += 3 * dbytes(std::max<int64_t>(pl.tot_big, 1));
+  need += dbytes((int64_t)nmol * TA_PROF_SLOTS);
   need += 64 * 256;
 
   Arena& ar = ctx->arena;
@@ -613,6 +618,7 @@ int upload_plan(tblite_amd_context ctx, Calculator& c, const Plan& pl, DeviceBat
   b.big_c = carve<double>(ar, std::max<int64_t>(pl.tot_big, 1));
   b.big_t = carve<double>(ar, std::max<int64_t>(pl.tot_big, 1));
   b.intcut = 0.0;
+  b.prof = ctx->profile ? carve<long long>(ar, (int64_t)nmol * TA_PROF_SLOTS) : nullptr;
   db.bytes = ar.used;
   if (ar.used > ar.cap) {
     set_error(ctx, "internal: workspace estimate too small (%zu > %zu)", ar.used, ar.cap);
@@ -621,7 +627,7 @@ int upload_plan(tblite_amd_context ctx, Calculator& c, const Plan& pl, DeviceBat
   return TBLITE_AMD_OK;
 }
 
-template <int NT, int MAXT, bool LDS>
+template <int NT, int MAXT, int MAXB, int MAXC, bool LDS>
 int launch_scf(tblite_amd_context ctx, hipStream_t st, const BatchDev& b, const int* mols,
                int count, int ldmax, int natmax, int nshmax)
 {
@@ -631,15 +637,17 @@ int launch_scf(tblite_amd_context ctx, hipStream_t st, const BatchDev& b, const 
     set_error(ctx, "SCF bucket needs %zu bytes of LDS (limit %zu)", bytes, ctx->lds_limit);
     return TBLITE_AMD_ERR_TOO_LARGE;
   }
-  HIP_TRY(ctx, hipFuncSetAttribute((const void*)k_scf<NT, MAXT, LDS>,
+  HIP_TRY(ctx, hipFuncSetAttribute((const void*)k_scf<NT, MAXT, MAXB, MAXC, LDS>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-  hipLaunchKernelGGL((k_scf<NT, MAXT, LDS>), dim3(count), dim3(NT), bytes, st, b, mols, ldmax,
-                     natmax, nshmax);
+  hipLaunchKernelGGL((k_scf<NT, MAXT, MAXB, MAXC, LDS>), dim3(count), dim3(NT), bytes, st, b,
+                     mols, ldmax, natmax, nshmax);
   HIP_TRY(ctx, hipGetLastError());
   return TBLITE_AMD_OK;
 }
 
-// Run the whole pipeline for the planned batch with coordinates xyz (host).
+// Run the whole pipeline for the planned batch.  xyz (host) is uploaded
+// first when given; with xyz == nullptr the coordinates already resident in
+// the batch buffer are used.
 int run_batch(tblite_amd_context ctx, Calculator& c, const double* xyz, double acc)
 {
   Plan& pl = ctx->plan;
@@ -648,8 +656,9 @@ int run_batch(tblite_amd_context ctx, Calculator& c, const double* xyz, double a
   b.accuracy = acc;
   b.intcut = integral_cutoff(acc);
   hipStream_t st = ctx->stream;
-  HIP_TRY(ctx, hipMemcpyAsync((void*)b.xyz, xyz, sizeof(double) * 3 * pl.tot_atoms,
-                              hipMemcpyHostToDevice, st));
+  if (xyz)
+    HIP_TRY(ctx, hipMemcpyAsync((void*)b.xyz, xyz, sizeof(double) * 3 * pl.tot_atoms,
+                                hipMemcpyHostToDevice, st));
   HIP_TRY(ctx, hipEventRecord(ctx->ev[0], st));
   // zero the integral matrices: pairs beyond the cutoff keep zero blocks
   {
@@ -680,14 +689,17 @@ int run_batch(tblite_amd_context ctx, Calculator& c, const double* xyz, double a
     const int* mols = db.d_bucket_mols[k];
     const int cnt = (int)bk.mols.size();
     int rc;
+    // per-worker-thread Jacobi items (MAXB blocks of A, MAXC rows of CT) over the
+    // NT - 64 worker threads: ld <= 32: 120 / 512 over 64; ld <= 64: 496 / 2048
+    // over 192; ld <= 96: 1128 / 4608 over 448
     if (!bk.lds)
-      rc = launch_scf<512, 1, false>(ctx, as, b, mols, cnt, bk.ldmax, bk.natmax, bk.nshmax);
+      rc = launch_scf<512, 1, 0, 0, false>(ctx, as, b, mols, cnt, bk.ldmax, bk.natmax, bk.nshmax);
     else if (bk.nt == 128)
-      rc = launch_scf<128, 1, true>(ctx, as, b, mols, cnt, bk.ldmax, bk.natmax, bk.nshmax);
+      rc = launch_scf<128, 1, 2, 8, true>(ctx, as, b, mols, cnt, bk.ldmax, bk.natmax, bk.nshmax);
     else if (bk.nt == 256)
-      rc = launch_scf<256, 1, true>(ctx, as, b, mols, cnt, bk.ldmax, bk.natmax, bk.nshmax);
+      rc = launch_scf<256, 1, 3, 11, true>(ctx, as, b, mols, cnt, bk.ldmax, bk.natmax, bk.nshmax);
     else
-      rc = launch_scf<512, 2, true>(ctx, as, b, mols, cnt, bk.ldmax, bk.natmax, bk.nshmax);
+      rc = launch_scf<512, 2, 3, 11, true>(ctx, as, b, mols, cnt, bk.ldmax, bk.natmax, bk.nshmax);
     if (rc) return rc;
   }
   for (int s = 0; s < kNumStreams; ++s) {
@@ -710,6 +722,9 @@ int run_batch(tblite_amd_context ctx, Calculator& c, const double* xyz, double a
   return TBLITE_AMD_OK;
 }
 
+int copy_results(tblite_amd_context ctx, int nmol, double* energies, int* iterations,
+                 int* status);
+
 int flat_singlepoint(tblite_amd_context ctx, Calculator& c, int nmol, const int* mol_nat,
                      const int* num, const double* xyz, const double* charge, const int* uhf,
                      double acc, double* energies, int* iterations, int* status)
@@ -723,6 +738,7 @@ int flat_singlepoint(tblite_amd_context ctx, Calculator& c, int nmol, const int*
   int rc = ensure_params_on_device(ctx, c);
   if (rc) return rc;
   ctx->have_batch = false;
+  ctx->prepared = nullptr;
   rc = make_plan(ctx, c, nmol, mol_nat, num, charge, uhf, acc, ctx->plan);
   if (rc) return rc;
   rc = upload_plan(ctx, c, ctx->plan, ctx->batch);
@@ -730,9 +746,16 @@ int flat_singlepoint(tblite_amd_context ctx, Calculator& c, int nmol, const int*
   rc = run_batch(ctx, c, xyz, acc);
   if (rc) return rc;
   ctx->have_batch = true;
+  return copy_results(ctx, nmol, energies, iterations, status);
+}
+
+int copy_results(tblite_amd_context ctx, int nmol, double* energies, int* iterations,
+                 int* status)
+{
   const BatchDev& b = ctx->batch.b;
   std::vector<int> st(nmol), it(nmol);
-  HIP_TRY(ctx, hipMemcpy(energies, b.energy, sizeof(double) * nmol, hipMemcpyDeviceToHost));
+  if (energies)
+    HIP_TRY(ctx, hipMemcpy(energies, b.energy, sizeof(double) * nmol, hipMemcpyDeviceToHost));
   HIP_TRY(ctx, hipMemcpy(st.data(), b.status, sizeof(int) * nmol, hipMemcpyDeviceToHost));
   HIP_TRY(ctx, hipMemcpy(it.data(), b.iterations, sizeof(int) * nmol, hipMemcpyDeviceToHost));
   if (status) std::memcpy(status, st.data(), sizeof(int) * nmol);
@@ -789,6 +812,8 @@ tblite_amd_context tblite_amd_new_context(int device)
   int lds = 0;
   if (ok) ok = hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess;
   ctx->lds_limit = (size_t)lds;
+  const char* pe = std::getenv("TBLITE_AMD_PROFILE");
+  ctx->profile = pe && pe[0] == '1';
   if (!ok) {
     tblite_amd_delete_context(&ctx);
     return nullptr;
@@ -1063,6 +1088,74 @@ int tblite_amd_singlepoint_flat(tblite_amd_context ctx, tblite_amd_calculator ca
                           iterations, status);
 }
 
+int tblite_amd_batch_prepare(tblite_amd_context ctx, tblite_amd_calculator calc, int nmol,
+                             const int* mol_nat, const int* num, const double* charge,
+                             const int* uhf, double accuracy)
+{
+  if (!ctx || !calc) return TBLITE_AMD_ERR_ARGUMENT;
+  if (nmol < 1 || !mol_nat || !num) {
+    set_error(ctx, "invalid arguments to batch_prepare");
+    return TBLITE_AMD_ERR_ARGUMENT;
+  }
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  int rc = ensure_params_on_device(ctx, calc->c);
+  if (rc) return rc;
+  ctx->have_batch = false;
+  ctx->prepared = nullptr;
+  const double acc = accuracy > 0.0 ? accuracy : calc->c.accuracy;
+  rc = make_plan(ctx, calc->c, nmol, mol_nat, num, charge, uhf, acc, ctx->plan);
+  if (rc) return rc;
+  rc = upload_plan(ctx, calc->c, ctx->plan, ctx->batch);
+  if (rc) return rc;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->prepared = calc;
+  ctx->prepared_acc = acc;
+  return TBLITE_AMD_OK;
+}
+
+int tblite_amd_batch_set_geometry(tblite_amd_context ctx, const double* xyz, int on_device)
+{
+  if (!ctx || !xyz) return TBLITE_AMD_ERR_ARGUMENT;
+  if (!ctx->prepared) {
+    set_error(ctx, "no prepared batch on this context");
+    return TBLITE_AMD_ERR_ARGUMENT;
+  }
+  HIP_TRY(ctx, hipMemcpy((void*)ctx->batch.b.xyz, xyz, sizeof(double) * 3 * ctx->plan.tot_atoms,
+                         on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+  return TBLITE_AMD_OK;
+}
+
+const double* tblite_amd_batch_geometry_device(tblite_amd_context ctx)
+{
+  if (!ctx || !ctx->prepared) return nullptr;
+  return ctx->batch.b.xyz;
+}
+
+int tblite_amd_batch_run(tblite_amd_context ctx)
+{
+  if (!ctx) return TBLITE_AMD_ERR_ARGUMENT;
+  if (!ctx->prepared) {
+    set_error(ctx, "no prepared batch on this context");
+    return TBLITE_AMD_ERR_ARGUMENT;
+  }
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  int rc = run_batch(ctx, ctx->prepared->c, nullptr, ctx->prepared_acc);
+  if (rc) return rc;
+  ctx->have_batch = true;
+  return TBLITE_AMD_OK;
+}
+
+int tblite_amd_batch_results(tblite_amd_context ctx, double* energies, int* iterations,
+                             int* status)
+{
+  if (!ctx) return TBLITE_AMD_ERR_ARGUMENT;
+  if (!ctx->have_batch) {
+    set_error(ctx, "no batch has been run on this context");
+    return TBLITE_AMD_ERR_ARGUMENT;
+  }
+  return copy_results(ctx, ctx->plan.nmol, energies, iterations, status);
+}
+
 int tblite_amd_batch_dims(tblite_amd_context ctx, int imol, int* nat, int* nsh, int* nao)
 {
   int rc = check_batch(ctx, imol);
@@ -1170,6 +1263,18 @@ int tblite_amd_batch_get_energy_parts(tblite_amd_context ctx, int imol, double* 
   if (erep) HIP_TRY(ctx, hipMemcpy(erep, b.erep_tot + imol, 8, hipMemcpyDeviceToHost));
   if (eelec) HIP_TRY(ctx, hipMemcpy(eelec, b.eelec + imol, 8, hipMemcpyDeviceToHost));
   return TBLITE_AMD_OK;
+}
+
+int tblite_amd_last_profile(tblite_amd_context ctx, long long* out)
+{
+  if (!ctx || !out) return TBLITE_AMD_ERR_ARGUMENT;
+  if (!ctx->have_batch || !ctx->batch.b.prof) {
+    set_error(ctx, "no profile: set TBLITE_AMD_PROFILE=1 before creating the context");
+    return TBLITE_AMD_ERR_ARGUMENT;
+  }
+  HIP_TRY(ctx, hipMemcpy(out, ctx->batch.b.prof,
+                         sizeof(long long) * ctx->plan.nmol * TA_PROF_SLOTS, hipMemcpyDeviceToHost));
+  return TA_PROF_SLOTS;
 }
 
 int tblite_amd_last_timings(tblite_amd_context ctx, double* times, int n)

@@ -28,7 +28,18 @@
 #include "device_util.h"
 
 #define JACOBI_MAX_SWEEPS 40
-#define JACOBI_TOL 1.0e-11
+
+// A (Fock / working / density matrix) is stored skewed: element (i, j) at
+// i*ld + (i + j) mod ld.  Rows and columns then both spread over the LDS
+// banks (a plain row stride of ld = 0 mod 64 banks would serialise column
+// accesses).  CT (MO coefficients, one MO per row) is stored plainly.
+__device__ __forceinline__ int sk(int i, int j, int ld)
+{
+  int c = i + j;
+  c = c >= ld ? c - ld : c;
+  return i * ld + c;
+}
+#define JACOBI_TOL 1.0e-8
 
 struct MolView {
   int m, n, ld, nat, nsh, a0, s0, o0, ndim, mem;
@@ -55,144 +66,68 @@ struct MolView {
 };
 
 // ------------------------------------------------------------- GEMM tiles
-// out = X * Y on ld x ld row-major matrices, 4x4 register tiles.  When HOLD
-// is set the output aliases an input: all tiles of a thread are kept in
-// registers until every thread has finished reading.
-template <int NT, int MAXT, bool HOLD>
+// out = X * Y (TRANS: out = (X * Y)^T) on ld x ld matrices with 4x4 register
+// tiles; X plain row-major, Y and out skewed.  Threads of a wave share a row
+// tile, so X is read as a broadcast.  When HOLD is set the output aliases an
+// input: a thread's tiles stay in registers until every thread has read.
+template <int NT, int MAXT, bool HOLD, bool TRANS>
 __device__ void mm_nn(const double* X, const double* Y, double* out, int ld)
 {
   const int nb = ld >> 2;
   const int ntile = nb * nb;
+  auto tile_mm = [&](int tile, double* acc) {
+    const int r0 = (tile / nb) * 4, c0 = (tile % nb) * 4;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc[e] = 0.0;
+    for (int k = 0; k < ld; ++k) {
+      double x[4], y[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) x[u] = X[(r0 + u) * ld + k];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) y[v] = Y[sk(k, c0 + v, ld)];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u * 4 + v] += x[u] * y[v];
+    }
+  };
+  auto tile_store = [&](int tile, const double* acc) {
+    const int r0 = (tile / nb) * 4, c0 = (tile % nb) * 4;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        if constexpr (TRANS) out[sk(c0 + v, r0 + u, ld)] = acc[u * 4 + v];
+        else out[sk(r0 + u, c0 + v, ld)] = acc[u * 4 + v];
+      }
+  };
   if constexpr (HOLD) {
     double acc[MAXT][16];
 #pragma unroll
     for (int t = 0; t < MAXT; ++t) {
       const int tile = threadIdx.x + t * NT;
-#pragma unroll
-      for (int e = 0; e < 16; ++e) acc[t][e] = 0.0;
-      if (tile < ntile) {
-        const int r0 = (tile / nb) * 4, c0 = (tile % nb) * 4;
-        for (int k = 0; k < ld; ++k) {
-          double x[4], y[4];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) x[u] = X[(r0 + u) * ld + k];
-#pragma unroll
-          for (int v = 0; v < 4; ++v) y[v] = Y[k * ld + c0 + v];
-#pragma unroll
-          for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) acc[t][u * 4 + v] += x[u] * y[v];
-        }
-      }
+      if (tile < ntile) tile_mm(tile, acc[t]);
     }
     __syncthreads();
 #pragma unroll
     for (int t = 0; t < MAXT; ++t) {
       const int tile = threadIdx.x + t * NT;
-      if (tile < ntile) {
-        const int r0 = (tile / nb) * 4, c0 = (tile % nb) * 4;
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) out[(r0 + u) * ld + c0 + v] = acc[t][u * 4 + v];
-      }
+      if (tile < ntile) tile_store(tile, acc[t]);
     }
   } else {
     for (int tile = threadIdx.x; tile < ntile; tile += NT) {
-      const int r0 = (tile / nb) * 4, c0 = (tile % nb) * 4;
       double acc[16];
-#pragma unroll
-      for (int e = 0; e < 16; ++e) acc[e] = 0.0;
-      for (int k = 0; k < ld; ++k) {
-        double x[4], y[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) x[u] = X[(r0 + u) * ld + k];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) y[v] = Y[k * ld + c0 + v];
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) acc[u * 4 + v] += x[u] * y[v];
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) out[(r0 + u) * ld + c0 + v] = acc[u * 4 + v];
+      tile_mm(tile, acc);
+      tile_store(tile, acc);
     }
   }
   __syncthreads();
 }
 
-// out = X^T * Y
-template <int NT, int MAXT, bool HOLD>
-__device__ void mm_tn(const double* X, const double* Y, double* out, int ld)
-{
-  const int nb = ld >> 2;
-  const int ntile = nb * nb;
-  if constexpr (HOLD) {
-    double acc[MAXT][16];
-#pragma unroll
-    for (int t = 0; t < MAXT; ++t) {
-      const int tile = threadIdx.x + t * NT;
-#pragma unroll
-      for (int e = 0; e < 16; ++e) acc[t][e] = 0.0;
-      if (tile < ntile) {
-        const int r0 = (tile / nb) * 4, c0 = (tile % nb) * 4;
-        for (int k = 0; k < ld; ++k) {
-          double x[4], y[4];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) x[u] = X[k * ld + r0 + u];
-#pragma unroll
-          for (int v = 0; v < 4; ++v) y[v] = Y[k * ld + c0 + v];
-#pragma unroll
-          for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) acc[t][u * 4 + v] += x[u] * y[v];
-        }
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int t = 0; t < MAXT; ++t) {
-      const int tile = threadIdx.x + t * NT;
-      if (tile < ntile) {
-        const int r0 = (tile / nb) * 4, c0 = (tile % nb) * 4;
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) out[(r0 + u) * ld + c0 + v] = acc[t][u * 4 + v];
-      }
-    }
-  } else {
-    for (int tile = threadIdx.x; tile < ntile; tile += NT) {
-      const int r0 = (tile / nb) * 4, c0 = (tile % nb) * 4;
-      double acc[16];
-#pragma unroll
-      for (int e = 0; e < 16; ++e) acc[e] = 0.0;
-      for (int k = 0; k < ld; ++k) {
-        double x[4], y[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) x[u] = X[k * ld + r0 + u];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) y[v] = Y[k * ld + c0 + v];
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) acc[u * 4 + v] += x[u] * y[v];
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) out[(r0 + u) * ld + c0 + v] = acc[u * 4 + v];
-    }
-  }
-  __syncthreads();
-}
-
-// P = sum_t C[:, occ_t] f_t C[:, occ_t]^T over the nocc occupied MO slots
+// P = sum_t f_t CT[occ_t, :]^T CT[occ_t, :] over the occupied MO rows
 // (get_density_matrix, src/wavefunction/type.cu:17-35 text)
 template <int NT>
-__device__ void density(const double* C, const double* f, const int* occ, int nocc, double* P,
+__device__ void density(const double* CT, const double* f, const int* occ, int nocc, double* P,
                         int ld)
 {
   const int nb = ld >> 2;
@@ -204,11 +139,12 @@ __device__ void density(const double* C, const double* f, const int* occ, int no
     for (int t = 0; t < nocc; ++t) {
       const int k = occ[t];
       const double fk = f[k];
+      const double* row = CT + k * ld;
       double x[4], y[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) x[u] = C[(r0 + u) * ld + k] * fk;
+      for (int u = 0; u < 4; ++u) x[u] = row[r0 + u] * fk;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) y[v] = C[(c0 + v) * ld + k];
+      for (int v = 0; v < 4; ++v) y[v] = row[c0 + v];
 #pragma unroll
       for (int u = 0; u < 4; ++u)
 #pragma unroll
@@ -217,7 +153,7 @@ __device__ void density(const double* C, const double* f, const int* occ, int no
 #pragma unroll
     for (int u = 0; u < 4; ++u)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) P[(r0 + u) * ld + c0 + v] = acc[u * 4 + v];
+      for (int v = 0; v < 4; ++v) P[sk(r0 + u, c0 + v, ld)] = acc[u * 4 + v];
   }
   __syncthreads();
 }
@@ -230,33 +166,32 @@ __device__ bool orthonormalizer(double* A, double* C, int n, int ld, double* sca
 {
   for (int j = 0; j < n; ++j) {
     if (threadIdx.x == 0) {
-      const double d = A[j * ld + j];
+      const double d = A[sk(j, j, ld)];
       scal[0] = (d > 0.0) ? sqrt(d) : -1.0;
     }
     __syncthreads();
     const double d = scal[0];
     if (!(d > 0.0)) return false;
-    for (int i = j + 1 + threadIdx.x; i < n; i += NT) A[i * ld + j] /= d;
-    if (threadIdx.x == 0) A[j * ld + j] = d;
+    for (int i = j + 1 + threadIdx.x; i < n; i += NT) A[sk(i, j, ld)] /= d;
+    if (threadIdx.x == 0) A[sk(j, j, ld)] = d;
     __syncthreads();
     // trailing update of the lower triangle
     const int m = n - j - 1;
     for (int idx = threadIdx.x; idx < m * m; idx += NT) {
       const int r = j + 1 + idx / m, c = j + 1 + idx % m;
-      if (c <= r) A[r * ld + c] -= A[r * ld + j] * A[c * ld + j];
+      if (c <= r) A[sk(r, c, ld)] -= A[sk(r, j, ld)] * A[sk(c, j, ld)];
     }
     __syncthreads();
   }
-  // row j of C = column j of L^-1 (forward substitution, one thread per column)
-  for (int j = threadIdx.x; j < ld; j += NT) {
-    double* y = C + j * ld;
-    for (int i = 0; i < ld; ++i) y[i] = 0.0;
-    if (j < n) {
-      for (int i = j; i < n; ++i) {
-        double s = (i == j) ? 1.0 : 0.0;
-        for (int k = j; k < i; ++k) s -= A[i * ld + k] * y[k];
-        y[i] = s / A[i * ld + i];
-      }
+  // CT = L^-1 (row j = MO j of the start basis C = L^-T); one thread per
+  // column j of L^-1, forward substitution
+  for (int idx = threadIdx.x; idx < ld * ld; idx += NT) C[idx] = 0.0;
+  __syncthreads();
+  for (int j = threadIdx.x; j < n; j += NT) {
+    for (int i = j; i < n; ++i) {
+      double s = (i == j) ? 1.0 : 0.0;
+      for (int k = j; k < i; ++k) s -= A[sk(i, k, ld)] * C[k * ld + j];
+      C[i * ld + j] = s / A[sk(i, i, ld)];
     }
   }
   __syncthreads();
@@ -264,86 +199,213 @@ __device__ bool orthonormalizer(double* A, double* C, int n, int ld, double* sca
 }
 
 // ---------------------------------------------------------------- Jacobi
-// Cyclic parallel Jacobi with round-robin pairing: each round applies ld/2
-// disjoint plane rotations; A is updated in 2x2 blocks (row pair, column
-// pair) and the rotations are accumulated into the columns of C.
+// Cyclic parallel Jacobi with round-robin pairing on the symmetric matrix A
+// (upper triangle kept during the sweeps), rotations accumulated into the
+// rows of CT (the MO coefficients, one MO per row).  Each round applies ld/2
+// disjoint plane rotations, one per pair slot k = (position k, position
+// ld-1-k).
+//
+// One barrier per round.  Under round-robin pairing every pair of the next
+// round joins one element of slot k-1 with one of slot k+1 (slot 0 with slot
+// 1, the last slot with the one before), so the 2x2 block of A holding that
+// pair's new off-diagonal element is known in advance.  Wave 0 owns these
+// "owner" blocks and the diagonal blocks: it updates them and computes the
+// next round's rotations from the rotation records of the current round
+// (which carry the pre-rotation diagonal elements).  The other waves update
+// the remaining blocks of A and the rows of CT.  Records are double buffered.
+//
+// Work items of the worker waves are fixed for the whole SCF (slots do not
+// move between rounds, only the indices they hold) and precomputed once.
+// With MAXB == 0 they are decoded on the fly (global-memory path for large
+// molecules).
+//
+// Convergence: a sweep whose largest rotated off-diagonal element is below
+// JACOBI_TOL leaves off-diagonals of order JACOBI_TOL^2 / gap (quadratic
+// convergence), so the solve stops after that sweep.
+struct JRot {
+  int2* pq;      // [2][h] indices (p, q) held by the slot
+  double2* cs;   // [2][h] cos, sin
+  double2* ta;   // [2][h] tan, a_pq before the rotation
+  double2* dg;   // [2][h] a_pp, a_qq before the rotation
+};
+
+template <int MAXB, int MAXC>
+struct JPlan {
+  int bkl[MAXB > 0 ? MAXB : 1];  // worker block items (k << 16 | l), -1 for none
+  int crk[MAXC > 0 ? MAXC : 1];  // worker CT items (k << 16 | ao), -1 for none
+};
+
+__device__ __forceinline__ bool jacobi_owner_block(int k, int l, int h)
+{
+  return (k == 0 && l == 1) || (l == k + 2 && k + 1 <= h - 2) || (k == h - 2 && l == h - 1);
+}
+
+template <int NT, int MAXB, int MAXC>
+__device__ void jacobi_plan(JPlan<MAXB, MAXC>& pl, int ld)
+{
+  if constexpr (MAXB > 0) {
+    const int h = ld >> 1;
+    const int nw = NT - 64, w = (int)threadIdx.x - 64;
+#pragma unroll
+    for (int j = 0; j < MAXB; ++j) pl.bkl[j] = -1;
+#pragma unroll
+    for (int j = 0; j < MAXC; ++j) pl.crk[j] = -1;
+    if (w < 0) return;
+    const int nblk = h * (h - 1) / 2;
+#pragma unroll
+    for (int j = 0; j < MAXB; ++j) {
+      const int bi = w + j * nw;
+      if (bi >= nblk) continue;
+      int k = 0, rem = bi;
+      while (rem >= h - 1 - k) {
+        rem -= h - 1 - k;
+        ++k;
+      }
+      const int l = k + 1 + rem;
+      if (!jacobi_owner_block(k, l, h)) pl.bkl[j] = (k << 16) | l;
+    }
+#pragma unroll
+    for (int j = 0; j < MAXC; ++j) {
+      const int ci = w + j * nw;
+      if (ci < h * ld) pl.crk[j] = ((ci / ld) << 16) | (ci % ld);
+    }
+  }
+}
+
 __device__ __forceinline__ int rr_player(int pos, int r, int ld)
 {
   return pos == 0 ? 0 : 1 + ((pos - 1 + r) % (ld - 1));
 }
 
-template <int NT>
-__device__ bool jacobi(double* A, double* C, int ld, double* rc, double* rs, double* rt, int* rp,
-                       int* rq, double* red, int* sweeps_out)
+__device__ __forceinline__ void make_rotation(const JRot& R, int buf, int h, int slot, int p,
+                                              int q, double app, double aqq, double apq)
+{
+  double c = 1.0, sn = 0.0, t = 0.0;
+  if (apq != 0.0) {
+    // t = 2 apq sgn(d) / (|d| + sqrt(d^2 + 4 apq^2)), the smaller rotation angle
+    const double d = aqq - app;
+    const double a2 = 2.0 * apq;
+    t = a2 / (fabs(d) + sqrt(d * d + a2 * a2));
+    if (d < 0.0) t = -t;
+    c = rsqrt(t * t + 1.0);
+    sn = t * c;
+  }
+  const int o = buf * h + slot;
+  R.pq[o] = make_int2(p, q);
+  R.cs[o] = make_double2(c, sn);
+  R.ta[o] = make_double2(t, apq);
+  R.dg[o] = make_double2(app, aqq);
+}
+
+// upper-triangle element (i != j)
+__device__ __forceinline__ double& aup(double* A, int ld, int i, int j)
+{
+  return i < j ? A[sk(i, j, ld)] : A[sk(j, i, ld)];
+}
+
+template <int NT, int MAXB, int MAXC>
+__device__ bool jacobi(double* A, double* CT, int ld, const JRot& R, const JPlan<MAXB, MAXC>& pl,
+                       double* red, int* sweeps_out)
 {
   const int h = ld >> 1;
-  int sweep = 0;
+  const int lane = threadIdx.x & 63;
+  const bool owner_wave = threadIdx.x < 64;
+  // rotations of the first round
+  if (owner_wave)
+    for (int k = lane; k < h; k += 64) {
+      const int p = rr_player(k, 0, ld), q = rr_player(ld - 1 - k, 0, ld);
+      make_rotation(R, 0, h, k, p, q, A[sk(p, p, ld)], A[sk(q, q, ld)], aup(A, ld, p, q));
+    }
+  __syncthreads();
+
+  int sweep = 0, round = 0;
   bool ok = false;
   for (; sweep < JACOBI_MAX_SWEEPS; ++sweep) {
     double offmax = 0.0;
-    for (int r = 0; r < ld - 1; ++r) {
-      for (int k = threadIdx.x; k < h; k += NT) {
-        int p = rr_player(k, r, ld), q = rr_player(ld - 1 - k, r, ld);
-        if (p > q) {
-          const int t = p;
-          p = q;
-          q = t;
-        }
-        const double apq = A[p * ld + q];
-        double c = 1.0, s = 0.0, t = 0.0;
-        offmax = fmax(offmax, fabs(apq));
-        if (apq != 0.0) {
-          const double app = A[p * ld + p], aqq = A[q * ld + q];
-          const double theta = (aqq - app) / (2.0 * apq);
-          if (fabs(theta) > 1.0e150) {
-            t = 0.5 / theta;
-          } else {
-            t = 1.0 / (fabs(theta) + sqrt(theta * theta + 1.0));
-            if (theta < 0.0) t = -t;
+    for (int r = 0; r < ld - 1; ++r, ++round) {
+      const int co = (round & 1) * h, nxt = (round & 1) ^ 1;
+      auto block = [&](int k, int l, double& z00, double& z01, double& z10, double& z11,
+                       int2& pk, int2& pl_) {
+        pk = R.pq[co + k];
+        pl_ = R.pq[co + l];
+        const double2 ck = R.cs[co + k], cl = R.cs[co + l];
+        const double x00 = aup(A, ld, pk.x, pl_.x), x01 = aup(A, ld, pk.x, pl_.y);
+        const double x10 = aup(A, ld, pk.y, pl_.x), x11 = aup(A, ld, pk.y, pl_.y);
+        const double y00 = ck.x * x00 - ck.y * x10, y01 = ck.x * x01 - ck.y * x11;
+        const double y10 = ck.y * x00 + ck.x * x10, y11 = ck.y * x01 + ck.x * x11;
+        z00 = cl.x * y00 - cl.y * y01;
+        z01 = cl.y * y00 + cl.x * y01;
+        z10 = cl.x * y10 - cl.y * y11;
+        z11 = cl.y * y10 + cl.x * y11;
+        aup(A, ld, pk.x, pl_.x) = z00;
+        aup(A, ld, pk.x, pl_.y) = z01;
+        aup(A, ld, pk.y, pl_.x) = z10;
+        aup(A, ld, pk.y, pl_.y) = z11;
+      };
+      if (owner_wave) {
+        // diagonal element after this round of the first (e = 0) or second index of slot k
+        auto diag_after = [&](int k, int e) {
+          const double2 ta = R.ta[co + k], dg = R.dg[co + k];
+          return e == 0 ? dg.x - ta.x * ta.y : dg.y + ta.x * ta.y;
+        };
+        for (int kn = lane; kn < h; kn += 64) {
+          {
+            const int2 pq = R.pq[co + kn];
+            const double2 ta = R.ta[co + kn], dg = R.dg[co + kn];
+            offmax = fmax(offmax, fabs(ta.y));
+            A[sk(pq.x, pq.x, ld)] = dg.x - ta.x * ta.y;
+            A[sk(pq.y, pq.y, ld)] = dg.y + ta.x * ta.y;
+            aup(A, ld, pq.x, pq.y) = 0.0;
           }
-          c = 1.0 / sqrt(t * t + 1.0);
-          s = t * c;
-        }
-        rc[k] = c;
-        rs[k] = s;
-        rt[k] = t;
-        rp[k] = p;
-        rq[k] = q;
-      }
-      __syncthreads();
-      const int nblk = h * h;
-      const int total = nblk + ld * h;
-      for (int it = threadIdx.x; it < total; it += NT) {
-        if (it < nblk) {
-          const int k = it / h, l = it - (it / h) * h;
-          const int pk = rp[k], qk = rq[k];
-          if (k == l) {
-            const double t = rt[k];
-            const double apq = A[pk * ld + qk];
-            A[pk * ld + pk] -= t * apq;
-            A[qk * ld + qk] += t * apq;
-            A[pk * ld + qk] = 0.0;
-            A[qk * ld + pk] = 0.0;
-          } else {
-            const int pl = rp[l], ql = rq[l];
-            const double ck = rc[k], sk = rs[k], cl = rc[l], sl = rs[l];
-            const double x00 = A[pk * ld + pl], x01 = A[pk * ld + ql];
-            const double x10 = A[qk * ld + pl], x11 = A[qk * ld + ql];
-            const double y00 = ck * x00 - sk * x10, y01 = ck * x01 - sk * x11;
-            const double y10 = sk * x00 + ck * x10, y11 = sk * x01 + ck * x11;
-            A[pk * ld + pl] = cl * y00 - sl * y01;
-            A[pk * ld + ql] = sl * y00 + cl * y01;
-            A[qk * ld + pl] = cl * y10 - sl * y11;
-            A[qk * ld + ql] = sl * y10 + cl * y11;
+          double z00, z01, z10, z11;
+          int2 pk, pl_;
+          if (kn == 0) {
+            block(0, 1, z00, z01, z10, z11, pk, pl_);
+            make_rotation(R, nxt, h, 0, pk.x, pl_.x, diag_after(0, 0), diag_after(1, 0), z00);
+            if (h == 2)
+              make_rotation(R, nxt, h, 1, pl_.y, pk.y, diag_after(1, 1), diag_after(0, 1), z11);
+          } else if (kn <= h - 2) {
+            block(kn - 1, kn + 1, z00, z01, z10, z11, pk, pl_);
+            make_rotation(R, nxt, h, kn, pl_.x, pk.y, diag_after(kn + 1, 0), diag_after(kn - 1, 1),
+                          z10);
+          } else if (h > 2) {
+            block(h - 2, h - 1, z00, z01, z10, z11, pk, pl_);
+            make_rotation(R, nxt, h, h - 1, pl_.y, pk.y, diag_after(h - 1, 1), diag_after(h - 2, 1),
+                          z11);
           }
+        }
+      } else {
+        auto crow = [&](int k, int ao) {
+          const int2 pq = R.pq[co + k];
+          const double2 cs = R.cs[co + k];
+          double* rp = CT + pq.x * ld + ao;
+          double* rq = CT + pq.y * ld + ao;
+          const double u = *rp, v = *rq;
+          *rp = cs.x * u - cs.y * v;
+          *rq = cs.y * u + cs.x * v;
+        };
+        double z00, z01, z10, z11;
+        int2 pk, pl_;
+        if constexpr (MAXB > 0) {
+#pragma unroll
+          for (int j = 0; j < MAXB; ++j)
+            if (pl.bkl[j] >= 0) block(pl.bkl[j] >> 16, pl.bkl[j] & 0xffff, z00, z01, z10, z11, pk, pl_);
+#pragma unroll
+          for (int j = 0; j < MAXC; ++j)
+            if (pl.crk[j] >= 0) crow(pl.crk[j] >> 16, pl.crk[j] & 0xffff);
         } else {
-          const int it2 = it - nblk;
-          const int row = it2 / h, k = it2 - (it2 / h) * h;
-          const int pk = rp[k], qk = rq[k];
-          const double ck = rc[k], sk = rs[k];
-          const double u = C[row * ld + pk], v = C[row * ld + qk];
-          C[row * ld + pk] = ck * u - sk * v;
-          C[row * ld + qk] = sk * u + ck * v;
+          const int nw = NT - 64, w = (int)threadIdx.x - 64;
+          const int nblk = h * (h - 1) / 2;
+          for (int bi = w; bi < nblk; bi += nw) {
+            int k = 0, rem = bi;
+            while (rem >= h - 1 - k) {
+              rem -= h - 1 - k;
+              ++k;
+            }
+            const int l = k + 1 + rem;
+            if (!jacobi_owner_block(k, l, h)) block(k, l, z00, z01, z10, z11, pk, pl_);
+          }
+          for (int ci = w; ci < h * ld; ci += nw) crow(ci / ld, ci % ld);
         }
       }
       __syncthreads();
@@ -405,8 +467,30 @@ __device__ double fermi_wave(double nel, double kt, int n, const double* emo, do
   return wave_sum(s) * kt;
 }
 
+// Phase clock: thread 0 accumulates wall-clock ticks per phase when the
+// batch carries a profile buffer.
+struct PhaseClock {
+  long long acc[TA_PROF_SLOTS];
+  long long last;
+  bool on;
+  __device__ void start(bool enable)
+  {
+    on = enable && threadIdx.x == 0;
+    for (int i = 0; i < TA_PROF_SLOTS; ++i) acc[i] = 0;
+    if (on) last = wall_clock64();
+  }
+  __device__ void lap(int slot)
+  {
+    if (on) {
+      const long long t = wall_clock64();
+      acc[slot] += t - last;
+      last = t;
+    }
+  }
+};
+
 // ---------------------------------------------------------------- kernel
-template <int NT, int MAXT, bool MATS_LDS>
+template <int NT, int MAXT, int MAXB, int MAXC, bool MATS_LDS>
 __global__ __launch_bounds__(NT) void k_scf(BatchDev b, const int* __restrict__ mols, int ldmax,
                                             int natmax, int nshmax)
 {
@@ -473,15 +557,20 @@ __global__ __launch_bounds__(NT) void k_scf(BatchDev b, const int* __restrict__ 
   double* focc_u = smem + L.focc;
   double* emo_s = smem + L.emo;
   double* fs = smem + L.fs;
-  double* rc = smem + L.rc;
-  double* rs = smem + L.rs;
-  double* rt = smem + L.rt;
+  JPlan<MAXB, MAXC> jplan;
+  jacobi_plan<NT, MAXB, MAXC>(jplan, ld);
+  // Jacobi rotation records live in the work region (free between the Fock
+  // build and the Mulliken analysis): 7 ld doubles
+  JRot jr;
+  jr.cs = reinterpret_cast<double2*>(smem + L.work);
+  jr.ta = jr.cs + ld;
+  jr.dg = jr.ta + ld;
+  jr.pq = reinterpret_cast<int2*>(jr.dg + ld);
   double* work = smem + L.work;
   int* rank = ismem + L.rank;
   int* inv = ismem + L.inv;
   int* occl = ismem + L.occ;
-  int* rp = ismem + L.rp;
-  int* rq = ismem + L.rq;
+
 
   double* qin = b.mix_qin + b.mol_mix_off[m];
   double* dq = b.mix_dq + b.mol_mix_off[m];
@@ -498,6 +587,8 @@ __global__ __launch_bounds__(NT) void k_scf(BatchDev b, const int* __restrict__ 
   const double damp = b.damping;
 
   int status = 0;
+  PhaseClock clk;
+  clk.start(b.prof != nullptr);
 
   // --- occupation numbers, src/xtb.cu:290-313 and src/h0.cu:61-91 (text)
   double nref = 0.0;
@@ -527,10 +618,11 @@ __global__ __launch_bounds__(NT) void k_scf(BatchDev b, const int* __restrict__ 
   // --- orthonormal start basis C = L^-T
   for (int idx = tid; idx < ld * ld; idx += NT) {
     const int i = idx / ld, j = idx - (idx / ld) * ld;
-    A[idx] = (i < n && j < n) ? v.S[(int64_t)i * n + j] : 0.0;
+    A[sk(i, j, ld)] = (i < n && j < n) ? v.S[(int64_t)i * n + j] : 0.0;
   }
   __syncthreads();
   if (!orthonormalizer<NT>(A, C, n, ld, scal)) status |= TA_STATUS_CHOLESKY;
+  clk.lap(TA_PROF_INIT);
 
   const double econv = 1.0e-6 * b.accuracy, pconv = 2.0e-5 * b.accuracy;
   const double kt = b.kt;
@@ -686,6 +778,7 @@ __global__ __launch_bounds__(NT) void k_scf(BatchDev b, const int* __restrict__ 
       __syncthreads();
     }
     ++iter;
+    clk.lap(TA_PROF_MIX);
 
     // ------------------------------------------------ potential
     double* vao = work;
@@ -693,52 +786,58 @@ __global__ __launch_bounds__(NT) void k_scf(BatchDev b, const int* __restrict__ 
     double* vat = vsh + nshmax;
     double* vdp = vat + natmax;
     double* vqp = vdp + 3 * natmax;
-    for (int s = tid; s < nsh; s += NT) {
+    // ES2 + ES3 shell potential: one wave per shell, lanes over partner shells
+    for (int sh = wid; sh < nsh; sh += NW) {
+      const double* g = v.gam + (int64_t)sh * nsh;
       double acc = 0.0;
-      for (int j = 0; j < nsh; ++j) acc += v.gam[(int64_t)j * nsh + s] * v.qsh[j];
-      const int at = v.sh_at[s];
-      const double q = v.qsh[s];
-      acc += q * q * P.el[v.el[at]].hubbard_derivs[v.sh_ish[s]];
-      vsh[s] = acc;
-    }
-    for (int a = tid; a < nat; a += NT) {
-      const DevElem& e = P.el[v.el[a]];
-      double va = 0.0, d0 = 0.0, d1 = 0.0, d2 = 0.0;
-      double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0, q4 = 0.0, q5 = 0.0;
-      for (int i = 0; i < nat; ++i) {
-        const int64_t ia = (int64_t)i * nat + a;
-        const double qi = v.qat[i];
-        const double s0_ = v.sd[ia], s1_ = v.sd[p2 + ia], s2_ = v.sd[2 * p2 + ia];
-        // amat_sd(k, i, a) = -amat_sd(k, a, i)
-        va -= s0_ * v.dpat[3 * i] + s1_ * v.dpat[3 * i + 1] + s2_ * v.dpat[3 * i + 2];
-        d0 += s0_ * qi;
-        d1 += s1_ * qi;
-        d2 += s2_ * qi;
-        const double dpi0 = v.dpat[3 * i], dpi1 = v.dpat[3 * i + 1], dpi2 = v.dpat[3 * i + 2];
-        d0 += v.dd[0 * p2 + ia] * dpi0 + v.dd[1 * p2 + ia] * dpi1 + v.dd[2 * p2 + ia] * dpi2;
-        d1 += v.dd[3 * p2 + ia] * dpi0 + v.dd[4 * p2 + ia] * dpi1 + v.dd[5 * p2 + ia] * dpi2;
-        d2 += v.dd[6 * p2 + ia] * dpi0 + v.dd[7 * p2 + ia] * dpi1 + v.dd[8 * p2 + ia] * dpi2;
-        const double t0 = v.sq[ia], t1 = v.sq[p2 + ia], t2 = v.sq[2 * p2 + ia];
-        const double t3 = v.sq[3 * p2 + ia], t4 = v.sq[4 * p2 + ia], t5 = v.sq[5 * p2 + ia];
-        va += t0 * v.qpat[6 * i] + t1 * v.qpat[6 * i + 1] + t2 * v.qpat[6 * i + 2] +
-              t3 * v.qpat[6 * i + 3] + t4 * v.qpat[6 * i + 4] + t5 * v.qpat[6 * i + 5];
-        q0 += t0 * qi;
-        q1 += t1 * qi;
-        q2 += t2 * qi;
-        q3 += t3 * qi;
-        q4 += t4 * qi;
-        q5 += t5 * qi;
+      for (int j = lane; j < nsh; j += 64) acc += g[j] * v.qsh[j];
+      acc = wave_sum(acc);
+      if (lane == 0) {
+        const double q = v.qsh[sh];
+        vsh[sh] = acc + q * q * P.el[v.el[v.sh_at[sh]]].hubbard_derivs[v.sh_ish[sh]];
       }
-      vat[a] = va;
-      vdp[3 * a + 0] = d0 + 2.0 * e.dkernel * v.dpat[3 * a + 0];
-      vdp[3 * a + 1] = d1 + 2.0 * e.dkernel * v.dpat[3 * a + 1];
-      vdp[3 * a + 2] = d2 + 2.0 * e.dkernel * v.dpat[3 * a + 2];
-      vqp[6 * a + 0] = q0 + 2.0 * e.qkernel * v.qpat[6 * a + 0];
-      vqp[6 * a + 1] = q1 + 4.0 * e.qkernel * v.qpat[6 * a + 1];
-      vqp[6 * a + 2] = q2 + 2.0 * e.qkernel * v.qpat[6 * a + 2];
-      vqp[6 * a + 3] = q3 + 4.0 * e.qkernel * v.qpat[6 * a + 3];
-      vqp[6 * a + 4] = q4 + 4.0 * e.qkernel * v.qpat[6 * a + 4];
-      vqp[6 * a + 5] = q5 + 2.0 * e.qkernel * v.qpat[6 * a + 5];
+    }
+    // AES2 atomic potentials: one wave per atom a, lanes over partners i, reading
+    // row a of every interaction plane (amat_sd is antisymmetric in the atom
+    // pair, amat_dd and amat_sq symmetric)
+    for (int a = wid; a < nat; a += NW) {
+      double va = 0.0, d[3] = {0.0, 0.0, 0.0}, q[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      const int64_t row = (int64_t)a * nat;
+      for (int i = lane; i < nat; i += 64) {
+        const int64_t ai = row + i;
+        const double qi = v.qat[i];
+        const double dpi[3] = {v.dpat[3 * i], v.dpat[3 * i + 1], v.dpat[3 * i + 2]};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const double sdk = v.sd[k * p2 + ai];  // amat_sd(k, i, a) = -amat_sd(k, a, i)
+          va += sdk * dpi[k];
+          d[k] -= sdk * qi;
+#pragma unroll
+          for (int l = 0; l < 3; ++l) d[k] += v.dd[(3 * l + k) * p2 + ai] * dpi[l];
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+          const double sqk = v.sq[k * p2 + ai];
+          va += sqk * v.qpat[6 * i + k];
+          q[k] += sqk * qi;
+        }
+      }
+      va = wave_sum(va);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) d[k] = wave_sum(d[k]);
+#pragma unroll
+      for (int k = 0; k < 6; ++k) q[k] = wave_sum(q[k]);
+      if (lane == 0) {
+        const DevElem& e = P.el[v.el[a]];
+        vat[a] = va;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) vdp[3 * a + k] = d[k] + 2.0 * e.dkernel * v.dpat[3 * a + k];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+          const double sc = (k == 1 || k == 3 || k == 4) ? 2.0 : 1.0;
+          vqp[6 * a + k] = q[k] + 2.0 * e.qkernel * v.qpat[6 * a + k] * sc;
+        }
+      }
     }
     __syncthreads();
     for (int i = tid; i < n; i += NT) vao[i] = vsh[v.ao_sh[i]] + vat[v.ao_at[i]];
@@ -746,6 +845,7 @@ __global__ __launch_bounds__(NT) void k_scf(BatchDev b, const int* __restrict__ 
     if (iter == 1)
       for (int i = tid; i < ndim; i += NT) qin[i] = 0.0;
     __syncthreads();
+    clk.lap(TA_PROF_POT);
 
     // ------------------------------------------------ Fock matrix
     // A1[i][j] = (H_ij - S_ij (v_i + v_j)/2)/2 - (D(.,j,i).vdp(at i) + Q(.,j,i).vqp(at i))/2
@@ -762,35 +862,41 @@ __global__ __launch_bounds__(NT) void k_scf(BatchDev b, const int* __restrict__ 
         f -= 0.5 * (v.Q[e] * vq[0] + v.Q[n2 + e] * vq[1] + v.Q[2 * n2 + e] * vq[2] +
                     v.Q[3 * n2 + e] * vq[3] + v.Q[4 * n2 + e] * vq[4] + v.Q[5 * n2 + e] * vq[5]);
       }
-      A[idx] = f;
+      A[sk(i, j, ld)] = f;
     }
     __syncthreads();
     for (int idx = tid; idx < ld * ld; idx += NT) {
       const int i = idx / ld, j = idx - (idx / ld) * ld;
       if (i < j) {
-        const double t = A[i * ld + j] + A[j * ld + i];
-        A[i * ld + j] = t;
-        A[j * ld + i] = t;
+        const double t = A[sk(i, j, ld)] + A[sk(j, i, ld)];
+        A[sk(i, j, ld)] = t;
+        A[sk(j, i, ld)] = t;
       } else if (i == j) {
-        A[idx] *= 2.0;
+        A[sk(i, i, ld)] *= 2.0;
       }
     }
     __syncthreads();
 
+    clk.lap(TA_PROF_FOCK);
     // ------------------------------------------------ eigensolve
+    // A <- C^T F C = CT F CT^T: U = CT F stored transposed, then CT U^T
     if constexpr (MATS_LDS) {
-      mm_nn<NT, MAXT, true>(A, C, A, ld);   // A <- F C
-      mm_tn<NT, MAXT, true>(C, A, A, ld);   // A <- C^T F C
+      mm_nn<NT, MAXT, true, true>(C, A, A, ld);
+      mm_nn<NT, MAXT, true, false>(C, A, A, ld);
     } else {
-      mm_nn<NT, 1, false>(A, C, T, ld);
-      mm_tn<NT, 1, false>(C, T, A, ld);
+      mm_nn<NT, 1, false, true>(C, A, T, ld);
+      mm_nn<NT, 1, false, false>(C, T, A, ld);
     }
+    clk.lap(TA_PROF_TRANSFORM);
     int sweeps = 0;
-    if (!jacobi<NT>(A, C, ld, rc, rs, rt, rp, rq, red, &sweeps)) {
+    const bool jok = jacobi<NT, MAXB, MAXC>(A, C, ld, jr, jplan, red, &sweeps);
+    clk.lap(TA_PROF_JACOBI);
+    clk.acc[TA_PROF_SWEEPS] += sweeps;
+    if (!jok) {
       status |= TA_STATUS_EIGEN;
       break;
     }
-    for (int k = tid; k < n; k += NT) ev[k] = A[k * ld + k];
+    for (int k = tid; k < n; k += NT) ev[k] = A[sk(k, k, ld)];
     __syncthreads();
     for (int k = tid; k < n; k += NT) {
       const double e = ev[k];
@@ -827,9 +933,11 @@ __global__ __launch_bounds__(NT) void k_scf(BatchDev b, const int* __restrict__ 
     }
     __syncthreads();
     last_nocc = (int)scal[4];
+    clk.lap(TA_PROF_OCC);
 
     // ------------------------------------------------ density -> A
     density<NT>(C, focc_u, occl, last_nocc, A, ld);
+    clk.lap(TA_PROF_DENSITY);
 
     // ------------------------------------------------ Mulliken + band energy
     double* part = work; // part[c * ld + i], c = 0 (PS), 1..3 (PD), 4..9 (PQ), 10 (PH)
@@ -839,7 +947,7 @@ __global__ __launch_bounds__(NT) void k_scf(BatchDev b, const int* __restrict__ 
       for (int c = 0; c < 11; ++c) acc[c] = 0.0;
       const int64_t row = (int64_t)i * n;
       for (int j = lane; j < n; j += 64) {
-        const double p = A[i * ld + j];
+        const double p = A[sk(i, j, ld)];
         const int64_t e = row + j;
         acc[0] += p * v.S[e];
         acc[1] += p * v.D[e];
@@ -901,36 +1009,47 @@ __global__ __launch_bounds__(NT) void k_scf(BatchDev b, const int* __restrict__ 
       v.qat[a] = q;
     }
     __syncthreads();
+    clk.lap(TA_PROF_MULLIKEN);
 
     // ------------------------------------------------ electrostatic energy
+    // E_ES2 = q^T gamma q / 2, E_ES3 = sum q^3 Gamma / 3, E_AES2 over atom
+    // pairs (a, i): dp_a . (amat_sd(a,i) q_i + amat_dd(a,i) dp_i / 2)
+    // + qp_a . amat_sq(a,i) q_i, plus the on-site kernels; all threads, one
+    // reduction
     double ec = 0.0;
-    for (int s = tid; s < nsh; s += NT) {
-      double acc = 0.0;
-      for (int j = 0; j < nsh; ++j) acc += v.gam[(int64_t)j * nsh + s] * v.qsh[j];
-      const double q = v.qsh[s];
-      ec += 0.5 * acc * q;
-      ec += q * q * q * P.el[v.el[v.sh_at[s]]].hubbard_derivs[v.sh_ish[s]] / 3.0;
+    for (int sh = wid; sh < nsh; sh += NW) {
+      const double* g = v.gam + (int64_t)sh * nsh;
+      const double qs = v.qsh[sh];
+      for (int j = lane; j < nsh; j += 64) ec += 0.5 * qs * g[j] * v.qsh[j];
+      if (lane == 0)
+        ec += qs * qs * qs * P.el[v.el[v.sh_at[sh]]].hubbard_derivs[v.sh_ish[sh]] / 3.0;
     }
-    for (int a = tid; a < nat; a += NT) {
-      const DevElem& e = P.el[v.el[a]];
-      double d[3] = {0.0, 0.0, 0.0}, q[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-      for (int i = 0; i < nat; ++i) {
-        const int64_t ia = (int64_t)i * nat + a;
+    for (int a = wid; a < nat; a += NW) {
+      const int64_t row = (int64_t)a * nat;
+      const double dpa[3] = {v.dpat[3 * a], v.dpat[3 * a + 1], v.dpat[3 * a + 2]};
+      double qpa[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) qpa[k] = v.qpat[6 * a + k];
+      for (int i = lane; i < nat; i += 64) {
+        const int64_t ai = row + i;
         const double qi = v.qat[i];
-        const double dpi0 = v.dpat[3 * i], dpi1 = v.dpat[3 * i + 1], dpi2 = v.dpat[3 * i + 2];
-        for (int k = 0; k < 3; ++k)
-          d[k] += v.sd[k * p2 + ia] * qi +
-                  0.5 * (v.dd[(3 * k + 0) * p2 + ia] * dpi0 + v.dd[(3 * k + 1) * p2 + ia] * dpi1 +
-                         v.dd[(3 * k + 2) * p2 + ia] * dpi2);
-        for (int k = 0; k < 6; ++k) q[k] += v.sq[k * p2 + ia] * qi;
+        const double dpi[3] = {v.dpat[3 * i], v.dpat[3 * i + 1], v.dpat[3 * i + 2]};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          double t = -v.sd[k * p2 + ai] * qi;
+#pragma unroll
+          for (int l = 0; l < 3; ++l) t += 0.5 * v.dd[(3 * l + k) * p2 + ai] * dpi[l];
+          ec += dpa[k] * t;
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) ec += qpa[k] * v.sq[k * p2 + ai] * qi;
       }
-      const double* dpa = v.dpat + 3 * a;
-      const double* qpa = v.qpat + 6 * a;
-      ec += dpa[0] * d[0] + dpa[1] * d[1] + dpa[2] * d[2];
-      for (int k = 0; k < 6; ++k) ec += qpa[k] * q[k];
-      ec += e.dkernel * (dpa[0] * dpa[0] + dpa[1] * dpa[1] + dpa[2] * dpa[2]);
-      ec += e.qkernel * (qpa[0] * qpa[0] + 2.0 * qpa[1] * qpa[1] + qpa[2] * qpa[2] +
-                         2.0 * qpa[3] * qpa[3] + 2.0 * qpa[4] * qpa[4] + qpa[5] * qpa[5]);
+      if (lane == 0) {
+        const DevElem& e = P.el[v.el[a]];
+        ec += e.dkernel * (dpa[0] * dpa[0] + dpa[1] * dpa[1] + dpa[2] * dpa[2]);
+        ec += e.qkernel * (qpa[0] * qpa[0] + 2.0 * qpa[1] * qpa[1] + qpa[2] * qpa[2] +
+                           2.0 * qpa[3] * qpa[3] + 2.0 * qpa[4] * qpa[4] + qpa[5] * qpa[5]);
+      }
     }
     ec = block_sum(ec, red);
     eband = block_sum(eband, red);
@@ -939,6 +1058,7 @@ __global__ __launch_bounds__(NT) void k_scf(BatchDev b, const int* __restrict__ 
     const double perr = sqrt(err / ndim);
     converged = (fabs(esum - elast) < econv) && (perr < pconv);
     if (!isfinite(esum)) status |= TA_STATUS_NAN;
+    clk.lap(TA_PROF_ENERGY);
   }
   if (!converged && status == 0) status |= TA_STATUS_NOT_CONVERGED;
 
@@ -951,8 +1071,12 @@ __global__ __launch_bounds__(NT) void k_scf(BatchDev b, const int* __restrict__ 
     double* co = b.coeff + moff;
     for (int idx = tid; idx < n * n; idx += NT) {
       const int i = idx / n, r = idx - (idx / n) * n;
-      co[idx] = C[i * ld + inv[r]];
+      co[idx] = C[inv[r] * ld + i];
     }
+  }
+  if (clk.on) {
+    clk.acc[TA_PROF_ITERS] = iter;
+    for (int i = 0; i < TA_PROF_SLOTS; ++i) b.prof[(int64_t)m * TA_PROF_SLOTS + i] = clk.acc[i];
   }
   if (tid == 0) {
     b.energy[m] = erep + esum;
@@ -964,7 +1088,7 @@ __global__ __launch_bounds__(NT) void k_scf(BatchDev b, const int* __restrict__ 
 }
 
 // explicit instantiations, launched by the host per size bucket
-template __global__ void k_scf<128, 1, true>(BatchDev, const int*, int, int, int);
-template __global__ void k_scf<256, 1, true>(BatchDev, const int*, int, int, int);
-template __global__ void k_scf<512, 2, true>(BatchDev, const int*, int, int, int);
-template __global__ void k_scf<512, 1, false>(BatchDev, const int*, int, int, int);
+template __global__ void k_scf<128, 1, 2, 8, true>(BatchDev, const int*, int, int, int);
+template __global__ void k_scf<256, 1, 3, 11, true>(BatchDev, const int*, int, int, int);
+template __global__ void k_scf<512, 2, 3, 11, true>(BatchDev, const int*, int, int, int);
+template __global__ void k_scf<512, 1, 0, 0, false>(BatchDev, const int*, int, int, int);

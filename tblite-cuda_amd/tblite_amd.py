@@ -77,6 +77,11 @@ _PROTOS = {
     "tblite_amd_xtb_singlepoint": (_I, [_P, _P, _P, _P, _D, _DP, _IP, _I]),
     "tblite_amd_xtb_singlepoint_batch": (_I, [_P, _I, C.POINTER(_P), _P, C.POINTER(_P), _D, _DP, _IP, _IP]),
     "tblite_amd_singlepoint_flat": (_I, [_P, _P, _I, _IP, _IP, _DP, _DP, _IP, _D, _DP, _IP, _IP]),
+    "tblite_amd_batch_prepare": (_I, [_P, _P, _I, _IP, _IP, _DP, _IP, _D]),
+    "tblite_amd_batch_set_geometry": (_I, [_P, _DP, _I]),
+    "tblite_amd_batch_geometry_device": (_P, [_P]),
+    "tblite_amd_batch_run": (_I, [_P]),
+    "tblite_amd_batch_results": (_I, [_P, _DP, _IP, _IP]),
     "tblite_amd_batch_dims": (_I, [_P, _I, _IP, _IP, _IP]),
     "tblite_amd_batch_get_cn": (_I, [_P, _I, _DP]),
     "tblite_amd_batch_get_adjlist": (_I, [_P, _I, _IP, _IP, _IP, _I]),
@@ -85,7 +90,11 @@ _PROTOS = {
     "tblite_amd_batch_get_charges": (_I, [_P, _I, _DP, _DP, _DP, _DP, _DP, _DP]),
     "tblite_amd_batch_get_energy_parts": (_I, [_P, _I, _DP, _DP]),
     "tblite_amd_last_timings": (_I, [_P, _DP, _I]),
+    "tblite_amd_last_profile": (_I, [_P, C.POINTER(C.c_longlong)]),
 }
+
+PROFILE_SLOTS = ["init", "mix", "potential", "fock", "transform", "jacobi", "occupation",
+                 "density", "mulliken", "energy", "sweeps", "iterations"]
 
 EXPORTED = sorted(_PROTOS)
 
@@ -179,6 +188,7 @@ class Engine:
         num = np.ascontiguousarray(num, dtype=np.int32)
         xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
         nmol = len(mol_nat)
+        self._nmol = nmol
         energies = np.zeros(nmol)
         iters = np.zeros(nmol, dtype=np.int32)
         status = np.zeros(nmol, dtype=np.int32)
@@ -189,6 +199,59 @@ class Engine:
             self.accuracy if accuracy is None else accuracy, _dp(energies), _ip(iters), _ip(status))
         self._check(rc, allow=(-3,))
         return energies, iters, status
+
+    def prepare(self, mol_nat, num, charge=None, uhf=None, accuracy=None):
+        """Plan a device-resident batch from its composition."""
+        mol_nat = np.ascontiguousarray(mol_nat, dtype=np.int32)
+        num = np.ascontiguousarray(num, dtype=np.int32)
+        ch = None if charge is None else np.ascontiguousarray(charge, dtype=np.float64)
+        uh = None if uhf is None else np.ascontiguousarray(uhf, dtype=np.int32)
+        self._nmol = len(mol_nat)
+        self._check(self.lib.tblite_amd_batch_prepare(
+            self.ctx, self.calc, len(mol_nat), _ip(mol_nat), _ip(num), _dp(ch), _ip(uh),
+            self.accuracy if accuracy is None else accuracy))
+
+    def set_geometry(self, xyz):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        self._check(self.lib.tblite_amd_batch_set_geometry(self.ctx, _dp(xyz), 0))
+
+    def run(self):
+        self._check(self.lib.tblite_amd_batch_run(self.ctx))
+
+    def results(self):
+        e = np.zeros(self._nmol)
+        it = np.zeros(self._nmol, dtype=np.int32)
+        st = np.zeros(self._nmol, dtype=np.int32)
+        self._check(self.lib.tblite_amd_batch_results(self.ctx, _dp(e), _ip(it), _ip(st)), allow=(-3,))
+        return e, it, st
+
+    def singlepoint(self, num, xyz, charge=0.0, uhf=0, accuracy=None):
+        """One molecule through tblite_amd_xtb_singlepoint (the per-molecule path)."""
+        num = np.ascontiguousarray(num, dtype=np.int32)
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        mol = self.lib.tblite_amd_new_structure(self.ctx, len(num), _ip(num), _dp(xyz), charge, uhf)
+        if not mol:
+            raise EngineError(self.error())
+        wfn = self.lib.tblite_amd_new_wavefunction(self.ctx)
+        e = C.c_double()
+        it = C.c_int()
+        try:
+            rc = self.lib.tblite_amd_xtb_singlepoint(
+                self.ctx, mol, self.calc, wfn, self.accuracy if accuracy is None else accuracy,
+                C.byref(e), C.byref(it), 0)
+            out = {"energy": e.value, "iterations": it.value, "rc": rc}
+            for name in ("qsh", "qat", "dpat", "qpat", "emo", "focc"):
+                cnt = C.c_int()
+                self.lib.tblite_amd_get_wavefunction_array(wfn, name.encode(), None, C.byref(cnt))
+                arr = np.zeros(cnt.value)
+                self.lib.tblite_amd_get_wavefunction_array(wfn, name.encode(), _dp(arr), C.byref(cnt))
+                out[name] = arr
+        finally:
+            h = C.c_void_p(mol)
+            self.lib.tblite_amd_delete_structure(C.byref(h))
+            h = C.c_void_p(wfn)
+            self.lib.tblite_amd_delete_wavefunction(C.byref(h))
+        return out
 
     def dims(self, imol):
         nat, nsh, nao = C.c_int(), C.c_int(), C.c_int()
@@ -246,6 +309,14 @@ class Engine:
         erep, eelec = C.c_double(), C.c_double()
         self._check(self.lib.tblite_amd_batch_get_energy_parts(self.ctx, imol, C.byref(erep), C.byref(eelec)))
         return erep.value, eelec.value
+
+    def profile(self):
+        """Per-molecule SCF phase clocks (needs TBLITE_AMD_PROFILE=1 at context creation)."""
+        out = np.zeros((self._nmol if hasattr(self, "_nmol") else 1) * len(PROFILE_SLOTS), dtype=np.int64)
+        rc = self.lib.tblite_amd_last_profile(self.ctx, out.ctypes.data_as(C.POINTER(C.c_longlong)))
+        if rc < 0:
+            raise EngineError(self.error())
+        return out.reshape(-1, len(PROFILE_SLOTS))
 
     def timings(self):
         t = np.zeros(5)

@@ -374,6 +374,21 @@ int or_multipole_cgto(const or_calc* calc, int zj, int jsh, int zi, int ish, dou
   return 0;
 }
 
+int or_get_cgto(const or_calc* calc, int z, int ish, int* ang, int* nprim, double* alpha,
+                double* coeff)
+{
+  int e = (z > 0 && z < 120) ? calc->zmap[z] : -1;
+  if (e < 0 || ish < 0 || ish >= calc->par.elem[e].nsh) return -1;
+  const cgto_t* c = &calc->cgto[e][ish];
+  *ang = c->ang;
+  *nprim = c->nprim;
+  for (int i = 0; i < c->nprim; ++i) {
+    alpha[i] = c->alpha[i];
+    coeff[i] = c->coeff[i];
+  }
+  return 0;
+}
+
 /* shift_operator, src/h0.cu:251-289 (text): dipole/quadrupole with the
  * operator moved from center i to center j. */
 static void shift_operator(const double vec[3], double s, const double* di, const double* qi,

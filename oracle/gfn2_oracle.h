@@ -64,6 +64,10 @@ void or_transform0(int lj, int li, const double* cart, double* sphr);
  * shell index within the species. */
 int or_multipole_cgto(const or_calc* calc, int zj, int jsh, int zi, int ish, double r2,
                       const double vec[3], double intcut, double* S, double* D, double* Q);
+/* Contracted basis of shell ish of element z: ang, nprim, alpha(nprim),
+ * coeff(nprim) (coefficients include the primitive norms). */
+int or_get_cgto(const or_calc* calc, int z, int ish, int* ang, int* nprim, double* alpha,
+                double* coeff);
 int or_ncoord(const or_calc* calc, int nat, const int* num, const double* xyz, double* cn);
 int or_repulsion(const or_calc* calc, int nat, const int* num, const double* xyz, double* erep);
 double or_get_cutoff(const or_calc* calc, int nat, const int* num, double accuracy);

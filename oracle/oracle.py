@@ -68,6 +68,8 @@ def lib():
         L.or_transform0.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.or_multipole_cgto.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
                                         C.POINTER(C.c_double), C.c_double] + [C.POINTER(C.c_double)] * 3
+        L.or_get_cgto.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                  C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.or_ncoord.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
                                 C.POINTER(C.c_double)]
         L.or_repulsion.argtypes = L.or_ncoord.argtypes
@@ -212,6 +214,14 @@ class Oracle:
         self.L.or_coulomb_eval(self.calc, nat, _ip(num), _dp(xyz), _dp(f(qsh)), _dp(f(dpat)), _dp(f(qpat)),
                                _dp(e), _dp(vsh), _dp(vat), _dp(vdp), _dp(vqp))
         return e, vsh, vat, vdp, vqp
+
+    def cgto(self, z, ish):
+        ang, npr = C.c_int(), C.c_int()
+        a = np.zeros(6)
+        c = np.zeros(6)
+        if self.L.or_get_cgto(self.calc, z, ish, C.byref(ang), C.byref(npr), _dp(a), _dp(c)):
+            raise ValueError("no such shell")
+        return ang.value, a[:npr.value].copy(), c[:npr.value].copy()
 
     def multipole_cgto(self, zj, jsh, zi, ish, vec, intcut=25.0):
         vec = np.ascontiguousarray(vec, dtype=np.float64)

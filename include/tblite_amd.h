@@ -237,9 +237,11 @@ int tblite_amd_last_timings(tblite_amd_context ctx, double* times, int n);
 
 /* Per-molecule SCF phase clocks of the last batch (only when the context was
  * created with TBLITE_AMD_PROFILE=1 in the environment): out receives
- * nmol x 12 values, wall-clock ticks at 100 MHz for init, mixing, potential,
- * Fock, transform, Jacobi, occupation, density, Mulliken, energy, then the
- * Jacobi sweep count and the iteration count.  Returns 12 or < 0. */
+ * nmol x 16 values, wall-clock ticks at 100 MHz for init, mixing, potential,
+ * Fock, transform, Jacobi (global-memory path: MO basis product), occupation,
+ * density, Mulliken, energy, then the Jacobi sweep count, the iteration
+ * count, and the global-memory eigensolver phases tridiagonalisation,
+ * bisection, inverse iteration and back-transformation.  Returns 16 or < 0. */
 int tblite_amd_last_profile(tblite_amd_context ctx, long long* out);
 
 #ifdef __cplusplus

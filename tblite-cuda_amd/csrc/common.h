@@ -138,11 +138,17 @@ struct BatchDev {
   int* iterations;
   int* status;
 
-  // global-memory matrices for molecules too large for LDS (ld x ld, three per molecule)
+  // global-memory matrices for molecules too large for LDS (ld x ld each):
+  // big_a Fock / reduced matrix / reflectors, big_c X = L^-1, big_t
+  // eigenvectors / density, big_ct MO coefficients (one MO per row)
   double* big_a;
   double* big_c;
   double* big_t;
-  const int64_t* mol_big_off;     // offset into big_a / big_c / big_t, -1 when LDS-resident
+  double* big_ct;
+  const int64_t* mol_big_off;     // offset into the big_* matrices, -1 when LDS-resident
+  // eigensolver scratch of those molecules (big_scratch_doubles(ld) each)
+  double* big_s;
+  const int64_t* mol_scr_off;
 
   // optional per-molecule phase clocks of the SCF kernel (TA_PROF_SLOTS each), may be null
   long long* prof;
@@ -161,7 +167,25 @@ struct BatchDev {
 #define TA_PROF_ENERGY 9
 #define TA_PROF_SWEEPS 10
 #define TA_PROF_ITERS 11
-#define TA_PROF_SLOTS 12
+// global-memory eigensolver phases (the Jacobi slot then holds the MO basis GEMM)
+#define TA_PROF_TRIDIAG 12
+#define TA_PROF_BISECT 13
+#define TA_PROF_INVIT 14
+#define TA_PROF_BACKTR 15
+#define TA_PROF_SLOTS 16
+
+// Global-memory eigensolver constants: threads per workgroup, reflectors per
+// back-transformation block, largest padded AO dimension served.
+#define TA_BIG_NT 512
+#define TA_BIG_NB 8
+#define TA_BIG_MAXLD 832
+
+// scratch per molecule of the global-memory path: inverse-iteration factors
+// (3 ld x NT) and per-wave partial sums (NT/64 x ld)
+__host__ __device__ inline int64_t big_scratch_doubles(int ld)
+{
+  return 3 * (int64_t)ld * TA_BIG_NT + (int64_t)(TA_BIG_NT / 64) * ld;
+}
 
 // LDS layout of the SCF kernel, in doubles (integer arrays in units of ints
 // from the start of the region).  ld is the padded AO dimension of the
@@ -173,6 +197,7 @@ struct ScfLds {
   int ev, focc, emo, fs;
   int rc, rs, rt;     // rotation cos, sin, tan (ld/2 each)
   int work;           // phase-shared vectors
+  int eig;            // global-memory eigensolver region (overlays work)
   int iint;           // start of the int region, in ints
   int rank, inv, occ, rp, rq;  // int arrays (ints, relative to the LDS base)
   int total;          // doubles
@@ -194,8 +219,15 @@ __host__ __device__ inline ScfLds scf_lds_layout(int ld, int natmax, int nshmax,
   L.rc = L.rs = L.rt = off;
   // work: potentials (ld + nsh + 10 nat), Jacobi records (7 ld, double2
   // aligned), Mulliken partial sums (11 ld)
+  // eigensolver region of the global-memory path: 9 ld persistent vectors
+  // (d, e, tau, e^2, v, w, v', y, row sums) + max(GEMM tiles, reflector block)
   int w1 = ld + nshmax + 10 * natmax, w2 = 11 * ld;
-  L.work = off; off += (w1 > w2 ? w1 : w2);
+  int tmp = TA_BIG_NB * ld + 3 * TA_BIG_NB * TA_BIG_NB;
+  int w3 = mats ? 0 : 9 * ld + (tmp > 2048 ? tmp : 2048);
+  L.work = off;
+  L.eig = off;
+  int wmax = w1 > w2 ? w1 : w2;
+  off += (wmax > w3 ? wmax : w3);
   L.iint = 2 * off;
   L.rank = L.iint;
   L.inv = L.rank + ld;

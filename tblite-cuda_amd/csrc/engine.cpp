@@ -188,7 +188,7 @@ struct Plan {
   int mem = 0;
   std::vector<int> mol_atom_off, mol_nat, mol_sh_off, mol_nsh, mol_ao_off, mol_nao, mol_uhf;
   std::vector<int64_t> mol_mat_off, mol_nl_off, mol_gam_off, mol_pair_off, mol_mix_off,
-      mol_mixmem_off, mol_mixa_off, mol_big_off;
+      mol_mixmem_off, mol_mixa_off, mol_big_off, mol_scr_off;
   std::vector<double> mol_charge, mol_cutoff;
   std::vector<int> at_el, at_ish, at_iao;
   std::vector<int> sh_at, sh_ish, sh_iao;
@@ -202,7 +202,7 @@ struct Plan {
   };
   std::vector<Bucket> buckets;
   int64_t tot_atoms = 0, tot_sh = 0, tot_ao = 0, tot_mat = 0, tot_nl = 0, tot_gam = 0,
-          tot_pair = 0, tot_mix = 0, tot_mixmem = 0, tot_big = 0;
+          tot_pair = 0, tot_mix = 0, tot_mixmem = 0, tot_big = 0, tot_scr = 0;
   std::vector<int> too_large;
 };
 
@@ -310,10 +310,11 @@ int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* 
   pl.mol_mixmem_off.resize(nmol);
   pl.mol_mixa_off.resize(nmol);
   pl.mol_big_off.resize(nmol);
+  pl.mol_scr_off.resize(nmol);
   pl.mol_charge.resize(nmol);
   pl.mol_cutoff.resize(nmol);
   int64_t atom = 0, shell = 0, ao = 0, mat = 0, nl = 0, gam = 0, pair = 0, mix = 0, mixmem = 0,
-          big = 0;
+          big = 0, scr = 0;
   for (int m = 0; m < nmol; ++m) {
     const int nat = mol_nat[m];
     if (nat < 1) {
@@ -365,11 +366,19 @@ int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* 
     pl.mol_mixa_off[m] = (int64_t)m * pl.mem * pl.mem;
     const int ld = (nao + 3) & ~3;
     const int ndim = nsh + 9 * nat;
+    if (ld > TA_BIG_MAXLD) {
+      set_error(ctx, "molecule %d has %d atomic orbitals; the engine serves up to %d", m, nao,
+                TA_BIG_MAXLD);
+      return TBLITE_AMD_ERR_TOO_LARGE;
+    }
     if (ld > kLdsMaxLd) {
       pl.mol_big_off[m] = big;
       big += (int64_t)ld * ld;
+      pl.mol_scr_off[m] = scr;
+      scr += big_scratch_doubles(ld);
     } else {
       pl.mol_big_off[m] = -1;
+      pl.mol_scr_off[m] = -1;
     }
     atom += nat;
     shell += nsh;
@@ -403,6 +412,7 @@ int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* 
   pl.tot_mix = mix;
   pl.tot_mixmem = mixmem;
   pl.tot_big = big;
+  pl.tot_scr = scr;
 
   // size buckets over the padded AO dimension; largest molecules first
   const int bounds[7] = {16, 32, 48, 64, 80, 96, 1 << 30};
@@ -440,7 +450,7 @@ int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* 
       bk.maxt = 2;
       bk.lds = true;
     } else {
-      bk.nt = 512;
+      bk.nt = TA_BIG_NT;
       bk.maxt = 1;
       bk.lds = false;
     }
@@ -494,6 +504,7 @@ int upload_plan(tblite_amd_context ctx, Calculator& c, const Plan& pl, DeviceBat
   host_pack(img, seg, pl.work_kind);      // 25
   host_pack(img, seg, pl.work_start);     // 26
   host_pack(img, seg, pl.mol_big_off);    // 27
+  host_pack(img, seg, pl.mol_scr_off);    // 28
   const size_t nfixed = seg.size();
   for (const auto& bk : pl.buckets) host_pack(img, seg, bk.mols);
 
@@ -511,7 +522,8 @@ int upload_plan(tblite_amd_context ctx, Calculator& c, const Plan& pl, DeviceBat
   need += dbytes((int64_t)nmol * mem * mem) + dbytes((int64_t)nmol * mem * (mem + 1)) +
           2 * dbytes((int64_t)nmol * mem);
   need += 3 * dbytes(nmol) + 2 * ibytes(nmol);
-  need += 3 * dbytes(std::max<int64_t>(pl.tot_big, 1));
+  need += 4 * dbytes(std::max<int64_t>(pl.tot_big, 1));
+  need += dbytes(std::max<int64_t>(pl.tot_scr, 1));
   need += dbytes((int64_t)nmol * TA_PROF_SLOTS);
   need += 64 * 256;
 
@@ -571,6 +583,7 @@ int upload_plan(tblite_amd_context ctx, Calculator& c, const Plan& pl, DeviceBat
   db.d_work_kind = I(25);
   db.d_work_start = I(26);
   b.mol_big_off = L(27);
+  b.mol_scr_off = L(28);
   db.d_bucket_mols.clear();
   for (size_t k = 0; k < pl.buckets.size(); ++k) db.d_bucket_mols.push_back(I(nfixed + k));
 
@@ -617,6 +630,8 @@ int upload_plan(tblite_amd_context ctx, Calculator& c, const Plan& pl, DeviceBat
   b.big_a = carve<double>(ar, std::max<int64_t>(pl.tot_big, 1));
   b.big_c = carve<double>(ar, std::max<int64_t>(pl.tot_big, 1));
   b.big_t = carve<double>(ar, std::max<int64_t>(pl.tot_big, 1));
+  b.big_ct = carve<double>(ar, std::max<int64_t>(pl.tot_big, 1));
+  b.big_s = carve<double>(ar, std::max<int64_t>(pl.tot_scr, 1));
   b.intcut = 0.0;
   b.prof = ctx->profile ? carve<long long>(ar, (int64_t)nmol * TA_PROF_SLOTS) : nullptr;
   db.bytes = ar.used;
@@ -693,7 +708,8 @@ int run_batch(tblite_amd_context ctx, Calculator& c, const double* xyz, double a
     // NT - 64 worker threads: ld <= 32: 120 / 512 over 64; ld <= 64: 496 / 2048
     // over 192; ld <= 96: 1128 / 4608 over 448
     if (!bk.lds)
-      rc = launch_scf<512, 1, 0, 0, false>(ctx, as, b, mols, cnt, bk.ldmax, bk.natmax, bk.nshmax);
+      rc = launch_scf<TA_BIG_NT, 1, 0, 0, false>(ctx, as, b, mols, cnt, bk.ldmax, bk.natmax,
+                                                 bk.nshmax);
     else if (bk.nt == 128)
       rc = launch_scf<128, 1, 2, 8, true>(ctx, as, b, mols, cnt, bk.ldmax, bk.natmax, bk.nshmax);
     else if (bk.nt == 256)

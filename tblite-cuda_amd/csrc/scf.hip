@@ -10,6 +10,9 @@
 //   Fock          F = H0 - S.(v_i+v_j)/2 - (D.vdp + Q.vqp terms)/2   -> A (LDS)
 //   transform     A <- C^T F C, C the S-orthonormal MO basis of the last step
 //   eigensolve    parallel cyclic Jacobi on A, rotations applied to C
+//                 (matrices in global memory: A <- X F X^T with X = L^-1,
+//                 Householder tridiagonalisation, bisection, inverse
+//                 iteration, back-transformation; see eig_big.h)
 //   occupation    Fermi smearing on the sorted levels
 //   density       P = C f C^T                                          -> A
 //   Mulliken      shell charges, atomic dipoles / quadrupoles, band energy
@@ -20,12 +23,14 @@
 // and the Jacobi solver needs only a few sweeps.
 //
 // A and C are ld x ld (ld = nao rounded up to a multiple of 4, padding
-// zero); they sit in LDS for ld <= 96 and in global memory above that.
+// zero); they sit in LDS for ld <= 96 and in global memory above that,
+// where the matrices are plain row-major instead of skewed.
 #include <hip/hip_runtime.h>
 #include <math.h>
 
 #include "common.h"
 #include "device_util.h"
+#include "eig_big.h"
 
 #define JACOBI_MAX_SWEEPS 40
 
@@ -489,6 +494,95 @@ struct PhaseClock {
   }
 };
 
+// ------------------------------------------- global-memory eigensolve
+// F (in A, full) -> eigenvalues ev (ascending) and MO rows CT = Z^T X of the
+// generalized problem F C = S C e, with X = L^-1 (lower triangular).  The
+// eigensolver region er (LDS) holds d, e, tau, e^2 and the work vectors of
+// the tridiagonalisation, followed by the GEMM tiles / reflector block.
+template <int NT>
+__device__ __noinline__ void big_eigensolve(double* A, const double* X, double* Y, double* CT, double* scr,
+                               int n, int ld, double* er, double* ev, double* red, double* scal,
+                               int* runs, PhaseClock& clk)
+{
+  double* d = er;
+  double* e = er + ld;
+  double* tau = er + 2 * ld;
+  double* e2 = er + 3 * ld;
+  double* vv = er + 4 * ld;
+  double* ww = er + 5 * ld;
+  double* vn = er + 6 * ld;
+  double* yy = er + 7 * ld;
+  double* yrow = er + 8 * ld;
+  double* tmp = er + 9 * ld;
+  auto k_lower = [n](int ti, int, int* k0, int* k1) {
+    *k0 = 0;
+    *k1 = min(n, ti * 64 + 64);
+  };
+  auto no_skip = [](int, int) { return false; };
+  auto upper_tile = [](int ti, int tj) { return tj > ti; };
+  // U = X F, stored transposed in Y; A = X U^T = X F X^T (lower tiles)
+  big::gemm<NT, true, true>(
+      n, n, [=](int i, int k) { return X[(int64_t)i * ld + k]; },
+      [=](int k, int j) { return A[(int64_t)k * ld + j]; }, k_lower, no_skip,
+      [=](int i, int j, double val) { Y[(int64_t)j * ld + i] = val; }, tmp);
+  big::gemm<NT, true, true>(
+      n, n, [=](int i, int k) { return X[(int64_t)i * ld + k]; },
+      [=](int k, int j) { return Y[(int64_t)k * ld + j]; }, k_lower, upper_tile,
+      [=](int i, int j, double val) { A[(int64_t)i * ld + j] = val; }, tmp);
+  clk.lap(TA_PROF_TRANSFORM);
+  double* part = scr + 3 * (int64_t)ld * TA_BIG_NT;
+  big::tridiagonalize<NT, 16>(A, n, ld, d, e, tau, vv, ww, vn, yy, yrow, part, red, scal);
+  clk.lap(TA_PROF_TRIDIAG);
+  big::bisect_all<NT>(d, e, e2, n, ev, red, scal);
+  const double tnorm = scal[0];
+  __syncthreads();
+  clk.lap(TA_PROF_BISECT);
+  big::inverse_iteration<NT>(d, e, ev, n, tnorm, Y, ld, scr);
+  big::orthogonalize_clusters<NT>(ev, n, tnorm, Y, ld, runs, scal);
+  clk.lap(TA_PROF_INVIT);
+  big::back_transform<NT, TA_BIG_NB>(A, tau, n, ld, Y, tmp, tmp + TA_BIG_NB * ld,
+                                     tmp + TA_BIG_NB * ld + TA_BIG_NB * TA_BIG_NB);
+  clk.lap(TA_PROF_BACKTR);
+  // CT = Z^T X; X[k][a] = 0 for a > k
+  big::gemm<NT, false, true>(
+      n, n, [=](int i, int k) { return Y[(int64_t)k * ld + i]; },
+      [=](int k, int j) { return X[(int64_t)k * ld + j]; },
+      [n](int, int tj, int* k0, int* k1) {
+        *k0 = tj * 64;
+        *k1 = n;
+      },
+      no_skip, [=](int i, int j, double val) { CT[(int64_t)i * ld + j] = val; }, tmp);
+  clk.lap(TA_PROF_JACOBI);
+}
+
+// P = sum_t f_t CT[occ_t]^T CT[occ_t] (get_density_matrix,
+// src/wavefunction/type.cu:17-35 text), lower tiles mirrored; P plain n x n.
+template <int NT>
+__device__ __noinline__ void big_density(const double* CT, const double* f, const int* occ, int nocc,
+                            double* P, int n, int ld, double* er)
+{
+  double* tmp = er + 9 * ld;
+  big::gemm<NT, false, true>(
+      n, n,
+      [=](int i, int t) {
+        const int k = occ[t];
+        return CT[(int64_t)k * ld + i] * f[k];
+      },
+      [=](int t, int j) { return CT[(int64_t)occ[t] * ld + j]; },
+      [nocc](int, int, int* k0, int* k1) {
+        *k0 = 0;
+        *k1 = nocc;
+      },
+      [](int ti, int tj) { return tj > ti; },
+      [=](int i, int j, double val) {
+        if (i >= j) {
+          P[(int64_t)i * ld + j] = val;
+          P[(int64_t)j * ld + i] = val;
+        }
+      },
+      tmp);
+}
+
 // ---------------------------------------------------------------- kernel
 template <int NT, int MAXT, int MAXB, int MAXC, bool MATS_LDS>
 __global__ __launch_bounds__(NT) void k_scf(BatchDev b, const int* __restrict__ mols, int ldmax,
@@ -539,18 +633,27 @@ __global__ __launch_bounds__(NT) void k_scf(BatchDev b, const int* __restrict__ 
   v.qpat = b.qpat + 6 * (int64_t)v.a0;
   const DevParams& P = *b.par;
 
+  // LDS path: A (skewed) and C = CT in LDS.  Global path: A, X = L^-1 (in C),
+  // Y (eigenvectors, then the density) and CT, plain row-major.
   double* A;
   double* C;
-  double* T;
+  double* Y = nullptr;
+  double* CT;
+  double* scr = nullptr;
   if constexpr (MATS_LDS) {
     A = smem + L.a;
     C = smem + L.c;
-    T = A;
+    CT = C;
   } else {
     A = b.big_a + b.mol_big_off[m];
     C = b.big_c + b.mol_big_off[m];
-    T = b.big_t + b.mol_big_off[m];
+    Y = b.big_t + b.mol_big_off[m];
+    CT = b.big_ct + b.mol_big_off[m];
+    scr = b.big_s + b.mol_scr_off[m];
   }
+  // element (i, j) of the working matrix A / the density
+  auto aix = [ld](int i, int j) { return MATS_LDS ? sk(i, j, ld) : i * ld + j; };
+  double* Pm = MATS_LDS ? A : Y;
   double* red = smem + L.red;
   double* scal = smem + L.scal;
   double* ev = smem + L.ev;
@@ -862,54 +965,59 @@ __global__ __launch_bounds__(NT) void k_scf(BatchDev b, const int* __restrict__ 
         f -= 0.5 * (v.Q[e] * vq[0] + v.Q[n2 + e] * vq[1] + v.Q[2 * n2 + e] * vq[2] +
                     v.Q[3 * n2 + e] * vq[3] + v.Q[4 * n2 + e] * vq[4] + v.Q[5 * n2 + e] * vq[5]);
       }
-      A[sk(i, j, ld)] = f;
+      A[aix(i, j)] = f;
     }
     __syncthreads();
     for (int idx = tid; idx < ld * ld; idx += NT) {
       const int i = idx / ld, j = idx - (idx / ld) * ld;
       if (i < j) {
-        const double t = A[sk(i, j, ld)] + A[sk(j, i, ld)];
-        A[sk(i, j, ld)] = t;
-        A[sk(j, i, ld)] = t;
+        const double t = A[aix(i, j)] + A[aix(j, i)];
+        A[aix(i, j)] = t;
+        A[aix(j, i)] = t;
       } else if (i == j) {
-        A[sk(i, i, ld)] *= 2.0;
+        A[aix(i, i)] *= 2.0;
       }
     }
     __syncthreads();
 
     clk.lap(TA_PROF_FOCK);
     // ------------------------------------------------ eigensolve
-    // A <- C^T F C = CT F CT^T: U = CT F stored transposed, then CT U^T
     if constexpr (MATS_LDS) {
+      // A <- C^T F C = CT F CT^T: U = CT F stored transposed, then CT U^T
       mm_nn<NT, MAXT, true, true>(C, A, A, ld);
       mm_nn<NT, MAXT, true, false>(C, A, A, ld);
-    } else {
-      mm_nn<NT, 1, false, true>(C, A, T, ld);
-      mm_nn<NT, 1, false, false>(C, T, A, ld);
-    }
-    clk.lap(TA_PROF_TRANSFORM);
-    int sweeps = 0;
-    const bool jok = jacobi<NT, MAXB, MAXC>(A, C, ld, jr, jplan, red, &sweeps);
-    clk.lap(TA_PROF_JACOBI);
-    clk.acc[TA_PROF_SWEEPS] += sweeps;
-    if (!jok) {
-      status |= TA_STATUS_EIGEN;
-      break;
-    }
-    for (int k = tid; k < n; k += NT) ev[k] = A[sk(k, k, ld)];
-    __syncthreads();
-    for (int k = tid; k < n; k += NT) {
-      const double e = ev[k];
-      int r = 0;
-      for (int j = 0; j < n; ++j) {
-        const double o = ev[j];
-        r += (o < e) || (o == e && j < k);
+      clk.lap(TA_PROF_TRANSFORM);
+      int sweeps = 0;
+      const bool jok = jacobi<NT, MAXB, MAXC>(A, C, ld, jr, jplan, red, &sweeps);
+      clk.lap(TA_PROF_JACOBI);
+      clk.acc[TA_PROF_SWEEPS] += sweeps;
+      if (!jok) {
+        status |= TA_STATUS_EIGEN;
+        break;
       }
-      rank[k] = r;
-      inv[r] = k;
-      emo_s[r] = e;
+      for (int k = tid; k < n; k += NT) ev[k] = A[sk(k, k, ld)];
+      __syncthreads();
+      for (int k = tid; k < n; k += NT) {
+        const double e = ev[k];
+        int r = 0;
+        for (int j = 0; j < n; ++j) {
+          const double o = ev[j];
+          r += (o < e) || (o == e && j < k);
+        }
+        rank[k] = r;
+        inv[r] = k;
+        emo_s[r] = e;
+      }
+      __syncthreads();
+    } else {
+      big_eigensolve<NT>(A, C, Y, CT, scr, n, ld, smem + L.eig, ev, red, scal, rank, clk);
+      for (int k = tid; k < n; k += NT) {
+        rank[k] = k;
+        inv[k] = k;
+        emo_s[k] = ev[k];
+      }
+      __syncthreads();
     }
-    __syncthreads();
 
     // ------------------------------------------------ occupations
     double ts = 0.0;
@@ -936,7 +1044,8 @@ __global__ __launch_bounds__(NT) void k_scf(BatchDev b, const int* __restrict__ 
     clk.lap(TA_PROF_OCC);
 
     // ------------------------------------------------ density -> A
-    density<NT>(C, focc_u, occl, last_nocc, A, ld);
+    if constexpr (MATS_LDS) density<NT>(C, focc_u, occl, last_nocc, A, ld);
+    else big_density<NT>(CT, focc_u, occl, last_nocc, Y, n, ld, smem + L.eig);
     clk.lap(TA_PROF_DENSITY);
 
     // ------------------------------------------------ Mulliken + band energy
@@ -947,7 +1056,7 @@ __global__ __launch_bounds__(NT) void k_scf(BatchDev b, const int* __restrict__ 
       for (int c = 0; c < 11; ++c) acc[c] = 0.0;
       const int64_t row = (int64_t)i * n;
       for (int j = lane; j < n; j += 64) {
-        const double p = A[sk(i, j, ld)];
+        const double p = Pm[aix(i, j)];
         const int64_t e = row + j;
         acc[0] += p * v.S[e];
         acc[1] += p * v.D[e];
@@ -1071,7 +1180,7 @@ __global__ __launch_bounds__(NT) void k_scf(BatchDev b, const int* __restrict__ 
     double* co = b.coeff + moff;
     for (int idx = tid; idx < n * n; idx += NT) {
       const int i = idx / n, r = idx - (idx / n) * n;
-      co[idx] = C[inv[r] * ld + i];
+      co[idx] = CT[inv[r] * ld + i];
     }
   }
   if (clk.on) {

@@ -94,7 +94,8 @@ _PROTOS = {
 }
 
 PROFILE_SLOTS = ["init", "mix", "potential", "fock", "transform", "jacobi", "occupation",
-                 "density", "mulliken", "energy", "sweeps", "iterations"]
+                 "density", "mulliken", "energy", "sweeps", "iterations",
+                 "tridiagonal", "bisection", "inverse_iteration", "back_transform"]
 
 EXPORTED = sorted(_PROTOS)
 

@@ -6,7 +6,9 @@ neighbour list, integrals/H0, SCF to convergence) over the whole batch, with
 the coordinates already resident in HBM; the batch is the workload's seeded
 molecules (default: 4096 (H2O)n clusters, n ~ U{2..16}).  Prints one JSON
 line.  Beside the GPU number it reports:
-  roofline      FP64 work of the SCF (the dominant kernel) / its measured time
+  roofline      FP64 work of the dominant SCF kernel (the size bucket with the
+                longest launch) / that launch's duration from HIP events
+                recorded on its stream
   cpu_baseline  the CPU oracle (FP64 restatement, OpenMP over molecules) on a
                 bounded sample of the same molecules, timed on this host
   per_molecule  the plain one-molecule-per-call path on a sample
@@ -35,13 +37,39 @@ WORKLOADS = {
 
 
 def scf_flops(nao, nocc, iters):
-    """Algorithmic FP64 flops of the SCF: per iteration (16/3) n^3 + 2 n^2 o + 40 n^2
-    (reduction, tridiagonal-equivalent eigensolve, back-transform, density, Fock,
-    Mulliken), plus n^3/3 once for the factorisation of S."""
+    """Algorithmic FP64 flops of the SCF per molecule: per iteration
+    (16/3) n^3 + 2 n^2 o + 40 n^2 (reduction, tridiagonalisation,
+    back-transforms, density, Fock, Mulliken), plus n^3/3 once for the
+    factorisation of S; o = MOs with occupation above 1e-12."""
     n = nao.astype(np.float64)
     o = nocc.astype(np.float64)
     per_iter = (16.0 / 3.0) * n**3 + 2.0 * n**2 * o + 40.0 * n**2
-    return float(np.sum(iters * per_iter + n**3 / 3.0))
+    return iters * per_iter + n**3 / 3.0
+
+
+BUCKET_BOUNDS = [16, 32, 48, 64, 80, 96, 1 << 30]  # padded AO dimension, as planned by the engine
+
+
+def scf_kernel_name(ldmax, nthreads):
+    if ldmax > 96:
+        return "k_scf<512, 1, 0, 0, false> (global-memory matrices, Householder eigensolver)"
+    maxt = 2 if nthreads == 512 else 1
+    maxb, maxc = (2, 8) if nthreads == 128 else (3, 11)
+    return f"k_scf<{nthreads}, {maxt}, {maxb}, {maxc}, true> (LDS matrices, Jacobi)"
+
+
+def copy_probe(mib, reps=3):
+    """Plain device copy of a known size (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)."""
+    import torch
+
+    n = mib * 1024 * 1024 // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 8, "reps": reps}))
 
 
 def cpu_baseline(mols, seconds, threads):
@@ -78,7 +106,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--per-molecule-sample", type=int, default=64)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a device copy of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_probe_mib > 0:
+        copy_probe(args.copy_probe_mib)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); molecules are independent, "
               "so a multi-GPU run is a split of the batch with no exchange", file=sys.stderr)
@@ -119,13 +152,24 @@ def main():
           f"integrals {phase[:, 2].mean():.3f} scf {phase[:, 3].mean():.3f}",
           file=sys.stderr, flush=True)
 
-    # algorithmic FP64 work of the SCF kernels
+    # algorithmic FP64 work of the SCF, per molecule; roofline of the dominant kernel
     nao = np.array([eng.dims(m)[2] for m in range(nmol)])
-    nel = np.array([sum({1: 1, 6: 4, 7: 5, 8: 6, 16: 6}[int(z)] for z in m[0]) for m in mols])
-    nocc = (nel + 1) // 2
-    flops = scf_flops(nao, nocc, it.astype(np.float64))
+    nocc = np.array([int(np.count_nonzero(eng.charges(m)[5] > 1e-12)) for m in range(nmol)])
+    fl = scf_flops(nao, nocc, it.astype(np.float64))
+    ld = (nao + 3) & ~3
+    buckets = eng.bucket_timings()
+    per_bucket = []
+    for bk in buckets:
+        hi = bk["ldmax"]
+        lo = max([b for b in BUCKET_BOUNDS if b < hi] + [0]) if hi <= 96 else 96
+        sel = (ld > lo) & (ld <= hi)
+        f = float(fl[sel].sum())
+        per_bucket.append({"kernel": scf_kernel_name(hi, bk["nthreads"]), "ldmax": hi,
+                           "molecules": bk["nmol"], "ms": round(bk["ms"], 3), "flops": f,
+                           "tflops": round(f / (bk["ms"] / 1e3) / 1e12, 4) if bk["ms"] > 0 else 0.0})
+    dom = max(per_bucket, key=lambda b: b["ms"])
+    achieved = dom["flops"] / (dom["ms"] / 1e3)
     scf_s = phase[:, 3].mean() / 1e3
-    achieved = flops / scf_s
     roofline = {
         "bound": "mfma",
         "achieved": round(achieved / 1e12, 4),
@@ -133,8 +177,11 @@ def main():
         "unit": "TFLOP/s",
         "frac": round(achieved / PEAK_FP64, 5),
         "traffic": None,
-        "kernel": "k_scf (all size buckets, HIP events around the SCF phase)",
-        "flops_per_step": flops,
+        "kernel": dom["kernel"],
+        "kernel_ms": dom["ms"],
+        "flops_per_launch": dom["flops"],
+        "all_scf_tflops": round(float(fl.sum()) / scf_s / 1e12, 4),
+        "buckets": per_bucket,
     }
 
     # plain one-molecule-per-call path on a sample
@@ -165,7 +212,7 @@ def main():
         cpu["cpu_model"] = _cpu_model()
 
     line = {
-        "metric": "singlepoints/s",
+        "metric": "singlepoints/s + p99 latency, batched vs per-molecule, |dE|<=1e-6 Eh",
         "value": round(value, 2),
         "unit": "singlepoints/s",
         "n_gpus": 1,

@@ -201,6 +201,32 @@ int tblite_amd_batch_run(tblite_amd_context ctx);
 int tblite_amd_batch_results(tblite_amd_context ctx, double* energies, int* iterations,
                              int* status);
 
+/* ------------------------------------------------------------- serving
+ * Continuous batching of independent singlepoint requests on one device.
+ * The server owns `slots` internal contexts (streams + a device workspace of
+ * workspace_bytes / slots reserved at creation).  submit() queues a request
+ * (host arrays are copied) and returns its ticket; poll() never blocks: it
+ * collects finished batches, refills every free slot from the queue head
+ * while the requests' estimated device footprint fits the slot, launches
+ * those batches (batches of different slots run concurrently), and copies
+ * out up to `capacity` finished results (ticket, energy in Eh, SCF
+ * iterations, status bits, latency from submit to collection in ms).  Calls
+ * on one server must come from one host thread.  Requests keep their
+ * wavefunction and mixer state on the device until their batch ends.  No
+ * counterpart in the reference (one molecule per GPU thread,
+ * src/xtb.cu:479-497). */
+typedef struct tblite_amd_server_s* tblite_amd_server;
+tblite_amd_server tblite_amd_new_server(tblite_amd_context ctx, tblite_amd_calculator calc,
+                                        int slots, double workspace_bytes);
+void tblite_amd_delete_server(tblite_amd_server* srv);
+int tblite_amd_server_submit(tblite_amd_server srv, int nat, const int* num, const double* xyz,
+                             double charge, int uhf, long long* ticket);
+/* Returns the number of results written (>= 0) or < 0 on error. */
+int tblite_amd_server_poll(tblite_amd_server srv, int capacity, long long* tickets,
+                           double* energies, int* iterations, int* status, double* latency_ms);
+int tblite_amd_server_stats(tblite_amd_server srv, long long* submitted, long long* completed,
+                            int* queued, int* busy_slots, long long* batches);
+
 /* ------------------------------------------------- stage-level access
  * Intermediate results of the last flat/batched call, for stage-by-stage
  * parity checks.  All copy to host memory; imol indexes the last batch.
@@ -235,13 +261,26 @@ int tblite_amd_batch_get_energy_parts(tblite_amd_context ctx, int imol, double* 
  * [2] integrals/H0, [3] SCF, [4] total.  n = number of entries wanted. */
 int tblite_amd_last_timings(tblite_amd_context ctx, double* times, int n);
 
+/* SCF size buckets of the last batched call, in launch order (largest AO
+ * dimension first): ms receives the time of each bucket's SCF kernel (HIP
+ * events recorded around the launch on the bucket's stream), ldmax its padded
+ * AO dimension, nmol its molecule count, nthreads its workgroup size.  Any
+ * array may be NULL.  Returns the number of buckets (entries beyond capacity
+ * are not written) or < 0.  Engine instrumentation; no counterpart in the
+ * reference. */
+int tblite_amd_last_bucket_timings(tblite_amd_context ctx, double* ms, int* ldmax, int* nmol,
+                                   int* nthreads, int capacity);
+
 /* Per-molecule SCF phase clocks of the last batch (only when the context was
  * created with TBLITE_AMD_PROFILE=1 in the environment): out receives
- * nmol x 16 values, wall-clock ticks at 100 MHz for init, mixing, potential,
+ * nmol x 22 values, wall-clock ticks at 100 MHz for init, mixing, potential,
  * Fock, transform, Jacobi (global-memory path: MO basis product), occupation,
  * density, Mulliken, energy, then the Jacobi sweep count, the iteration
- * count, and the global-memory eigensolver phases tridiagonalisation,
- * bisection, inverse iteration and back-transformation.  Returns 16 or < 0. */
+ * count, the global-memory eigensolver phases tridiagonalisation (remainder),
+ * bisection, inverse iteration and back-transformation, the steps of the
+ * blocked tridiagonalisation (column update, reflector, symmetric product,
+ * panel corrections, panel update) and the cluster re-orthogonalisation.
+ * Returns 22 or < 0. */
 int tblite_amd_last_profile(tblite_amd_context ctx, long long* out);
 
 #ifdef __cplusplus

@@ -172,19 +172,30 @@ struct BatchDev {
 #define TA_PROF_BISECT 13
 #define TA_PROF_INVIT 14
 #define TA_PROF_BACKTR 15
-#define TA_PROF_SLOTS 16
+// steps of the blocked tridiagonalisation (column update, reflector, symmetric
+// product, panel corrections, panel update) and the cluster re-orthogonalisation
+#define TA_PROF_TRI_COLUMN 16
+#define TA_PROF_TRI_REFLECTOR 17
+#define TA_PROF_TRI_SYMV 18
+#define TA_PROF_TRI_CORRECT 19
+#define TA_PROF_TRI_PANEL 20
+#define TA_PROF_CLUSTERS 21
+#define TA_PROF_SLOTS 22
 
 // Global-memory eigensolver constants: threads per workgroup, reflectors per
-// back-transformation block, largest padded AO dimension served.
+// back-transformation block, columns per tridiagonalisation panel, largest
+// padded AO dimension served.
 #define TA_BIG_NT 512
-#define TA_BIG_NB 8
-#define TA_BIG_MAXLD 832
+#define TA_BIG_NB 32
+#define TA_BIG_NP 16
+#define TA_BIG_MAXLD 1024
 
 // scratch per molecule of the global-memory path: inverse-iteration factors
-// (3 ld x NT) and per-wave partial sums (NT/64 x ld)
+// (3 ld x NT), back-transformation products (NB x ld), tridiagonalisation
+// panel vectors (2 NP x ld)
 __host__ __device__ inline int64_t big_scratch_doubles(int ld)
 {
-  return 3 * (int64_t)ld * TA_BIG_NT + (int64_t)(TA_BIG_NT / 64) * ld;
+  return 3 * (int64_t)ld * TA_BIG_NT + (int64_t)(TA_BIG_NB + 2 * TA_BIG_NP) * ld;
 }
 
 // LDS layout of the SCF kernel, in doubles (integer arrays in units of ints
@@ -220,10 +231,12 @@ __host__ __device__ inline ScfLds scf_lds_layout(int ld, int natmax, int nshmax,
   // work: potentials (ld + nsh + 10 nat), Jacobi records (7 ld, double2
   // aligned), Mulliken partial sums (11 ld)
   // eigensolver region of the global-memory path: 9 ld persistent vectors
-  // (d, e, tau, e^2, v, w, v', y, row sums) + max(GEMM tiles, reflector block)
+  // (d, e, tau, e^2, v, w, v', y, row sums), then either the GEMM tiles and
+  // T^-1, T of a reflector block or the per-wave column sums of the
+  // tridiagonalisation
   int w1 = ld + nshmax + 10 * natmax, w2 = 11 * ld;
-  int tmp = TA_BIG_NB * ld + 3 * TA_BIG_NB * TA_BIG_NB;
-  int w3 = mats ? 0 : 9 * ld + (tmp > 2048 ? tmp : 2048);
+  int t3 = 2048 + 2 * TA_BIG_NB * TA_BIG_NB, t4 = (TA_BIG_NT / 64) * ld;
+  int w3 = mats ? 0 : 9 * ld + (t3 > t4 ? t3 : t4);
   L.work = off;
   L.eig = off;
   int wmax = w1 > w2 ? w1 : w2;

@@ -3,7 +3,26 @@
 
 #include <hip/hip_runtime.h>
 
+#include "common.h"
+
 #define TA_WAVE 64
+
+// Address-space-qualified element types for device functions that are not
+// inlined: a callee that only sees generic pointers issues flat memory
+// instructions, which wait on both the LDS and the vector memory counters.
+typedef __attribute__((address_space(3))) double lds_dbl;
+typedef __attribute__((address_space(3))) int lds_int;
+typedef __attribute__((address_space(1))) double gbl_dbl;
+#define TA_LDS(p) ((lds_dbl*)(p))
+#define TA_LDSI(p) ((lds_int*)(p))
+#define TA_GBL(p) ((gbl_dbl*)(p))
+
+// Wave index within the workgroup as a wave-uniform (scalar) value, so that
+// loops and addresses derived from it stay in scalar registers.
+__device__ __forceinline__ int wave_id()
+{
+  return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+}
 
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -21,7 +40,8 @@ __device__ __forceinline__ double wave_max(double v)
 
 // Workgroup sum; red must hold blockDim.x / 64 doubles.  Every thread gets
 // the result.  Contains two barriers.
-__device__ __forceinline__ double block_sum(double v, double* red)
+template <class P>
+__device__ __forceinline__ double block_sum(double v, P red)
 {
   const int lane = threadIdx.x & (TA_WAVE - 1);
   const int wid = threadIdx.x / TA_WAVE;
@@ -35,7 +55,8 @@ __device__ __forceinline__ double block_sum(double v, double* red)
   return r;
 }
 
-__device__ __forceinline__ double block_max(double v, double* red)
+template <class P>
+__device__ __forceinline__ double block_max(double v, P red)
 {
   const int lane = threadIdx.x & (TA_WAVE - 1);
   const int wid = threadIdx.x / TA_WAVE;
@@ -73,3 +94,25 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* red, int* total)
   *total = tot;
   return base + incl - v;
 }
+
+// Per-phase wall-clock accumulator of one workgroup (thread 0 only); used by
+// the SCF kernel when profiling is enabled.
+struct PhaseClock {
+  long long acc[TA_PROF_SLOTS];
+  long long last;
+  bool on;
+  __device__ void start(bool enable)
+  {
+    on = enable && threadIdx.x == 0;
+    for (int i = 0; i < TA_PROF_SLOTS; ++i) acc[i] = 0;
+    if (on) last = wall_clock64();
+  }
+  __device__ void lap(int slot)
+  {
+    if (on) {
+      const long long t = wall_clock64();
+      acc[slot] += t - last;
+      last = t;
+    }
+  }
+};

@@ -38,73 +38,81 @@ constexpr double ORTOL = 1.0e-5;  // relative level gap below which vectors are 
 // a tile (triangular operands), skip(ti, tj) drops tiles, st(i, j, value)
 // stores.  AK: a is read k-fastest (row-major a), else i-fastest.  BJ: b is
 // read j-fastest (row-major b), else k-fastest.  tile: 2048 doubles of LDS.
-// Each thread holds a 4 x 2 micro tile.
+//
+// The product runs on the FP64 matrix cores (v_mfma_f64_16x16x4f64: lane l
+// supplies a[l%16][l/16] and b[l/16][l%16] and holds d[4i + l/16][l%16] of
+// the 16 x 16 result).  Each of the 8 waves owns a 16 x 32 strip of the tile;
+// 16-deep k slices are staged through LDS, the next slice's global loads are
+// issued before the current slice is multiplied.
+typedef double ta_double4 __attribute__((ext_vector_type(4)));
+
 template <int NT, bool AK, bool BJ, class LA, class LB, class KR, class SK, class ST>
-__device__ __noinline__ void gemm(int M, int N, LA la, LB lb, KR kr, SK skip, ST st, double* tile)
+__device__ __noinline__ void gemm(int M, int N, LA la, LB lb, KR kr, SK skip, ST st, lds_dbl* tile)
 {
   static_assert(NT == 512, "gemm tiling assumes 512 threads");
-  double* as = tile;         // [16][64]: as[k * 64 + i]
-  double* bs = tile + 1024;  // [16][64]: bs[k * 64 + j]
+  lds_dbl* as = tile;         // [16][64]: as[k * 64 + i]
+  lds_dbl* bs = tile + 1024;  // [16][64]: bs[k * 64 + j]
   const int tid = threadIdx.x;
-  const int tx = tid & 31, ty = tid >> 5;
+  const int lane = tid & 63, w = wave_id();
+  const int bi = w >> 1, bj = (w & 1) * 2;
+  const int lr = lane & 15, lk = lane >> 4;
   const int mt = (M + 63) >> 6, nt = (N + 63) >> 6;
+  // staging coordinates of this thread's two elements of each operand slice
+  int ai[2], ak[2], bjj[2], bk[2];
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int idx = tid + r * NT;
+    ai[r] = AK ? idx >> 4 : idx & 63;
+    ak[r] = AK ? idx & 15 : idx >> 6;
+    bjj[r] = BJ ? idx & 63 : idx >> 4;
+    bk[r] = BJ ? idx >> 6 : idx & 15;
+  }
   for (int ti = 0; ti < mt; ++ti)
     for (int tj = 0; tj < nt; ++tj) {
       if (skip(ti, tj)) continue;
       int k0, k1;
       kr(ti, tj, &k0, &k1);
       const int i0 = ti * 64, j0 = tj * 64;
-      double acc[4][2];
+      ta_double4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+      double ra[2], rb[2];
+      auto fetch = [&](int kb) {
 #pragma unroll
-      for (int u = 0; u < 4; ++u) acc[u][0] = acc[u][1] = 0.0;
+        for (int r = 0; r < 2; ++r) {
+          const int gi = i0 + ai[r], gk = kb + ak[r];
+          ra[r] = (gi < M && gk < k1) ? la(gi, gk) : 0.0;
+          const int gj = j0 + bjj[r], gk2 = kb + bk[r];
+          rb[r] = (gj < N && gk2 < k1) ? lb(gk2, gj) : 0.0;
+        }
+      };
+      if (k0 < k1) fetch(k0);
       for (int kb = k0; kb < k1; kb += 16) {
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
-          const int idx = tid + r * NT;
-          int i, k;
-          if constexpr (AK) {
-            i = idx >> 4;
-            k = idx & 15;
-          } else {
-            i = idx & 63;
-            k = idx >> 6;
-          }
-          const int gi = i0 + i, gk = kb + k;
-          as[k * 64 + i] = (gi < M && gk < k1) ? la(gi, gk) : 0.0;
-          int j;
-          if constexpr (BJ) {
-            j = idx & 63;
-            k = idx >> 6;
-          } else {
-            j = idx >> 4;
-            k = idx & 15;
-          }
-          const int gj = j0 + j, gk2 = kb + k;
-          bs[k * 64 + j] = (gj < N && gk2 < k1) ? lb(gk2, gj) : 0.0;
+          as[ak[r] * 64 + ai[r]] = ra[r];
+          bs[bk[r] * 64 + bjj[r]] = rb[r];
         }
         __syncthreads();
+        if (kb + 16 < k1) fetch(kb + 16);
 #pragma unroll
-        for (int k = 0; k < 16; ++k) {
-          double a[4], bv[2];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) a[u] = as[k * 64 + 4 * ty + u];
-          bv[0] = bs[k * 64 + 2 * tx];
-          bv[1] = bs[k * 64 + 2 * tx + 1];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            acc[u][0] += a[u] * bv[0];
-            acc[u][1] += a[u] * bv[1];
-          }
+        for (int kk = 0; kk < 4; ++kk) {
+          const int kr_ = (4 * kk + lk) * 64;
+          const double a = as[kr_ + 16 * bi + lr];
+          const double b0 = bs[kr_ + 16 * bj + lr];
+          const double b1 = bs[kr_ + 16 * bj + 16 + lr];
+          acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, acc0, 0, 0, 0);
+          acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, acc1, 0, 0, 0);
         }
         __syncthreads();
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-          const int gi = i0 + 4 * ty + u, gj = j0 + 2 * tx + c;
-          if (gi < M && gj < N) st(gi, gj, acc[u][c]);
+      for (int i = 0; i < 4; ++i) {
+        const int gi = i0 + 16 * bi + 4 * i + lk;
+        const int gj = j0 + 16 * bj + lr;
+        if (gi < M) {
+          if (gj < N) st(gi, gj, acc0[i]);
+          if (gj + 16 < N) st(gi, gj + 16, acc1[i]);
         }
+      }
     }
   __syncthreads();
 }
@@ -113,7 +121,7 @@ __device__ __noinline__ void gemm(int M, int N, LA la, LB lb, KR kr, SK skip, ST
 // x[lo..hi) (LDS) -> v with v[lo] = 1, H = I - tau v v^T, H x = beta e_lo
 // (dlarfg).  Wave 0 computes the scalars; returns beta, tau via scal[0..1].
 template <int NT>
-__device__ void householder(double* x, int lo, int hi, double* scal)
+__device__ void householder(lds_dbl* x, int lo, int hi, lds_dbl* scal)
 {
   if (threadIdx.x < 64) {
     double s = 0.0;
@@ -148,58 +156,82 @@ __device__ void householder(double* x, int lo, int hi, double* scal)
 // Rows r in [lo, n), columns c in [lo, r]:
 //   a' = a - v_r w_c - w_r v_c   (when UPDATE)
 //   y_r += sum_c a' u_c (c <= r),  y_c += a' u_r (c < r)
-// Row sums go to yrow (LDS, one writer per row), column sums to the
-// per-wave partials part[w * ld + c] (global), combined afterwards.
-template <int NT, int MAXT, bool UPDATE>
-__device__ void trail_pass(double* A, int n, int ld, int lo, const double* v, const double* w,
-                           const double* u, double* yrow, double* part)
+// A wave takes RQ rows at a time (rows r0 + q NW) and walks their column
+// blocks of 64 together, so RQ loads per lane are in flight; row sums go to
+// yrow (LDS, one writer per row), column sums to the wave's own LDS row of
+// ycol (NW x ld), combined afterwards.
+template <int NT, bool UPDATE, int RQ = 4>
+__device__ __noinline__ void trail_pass(gbl_dbl* A, int n, int ld, int lo, const lds_dbl* v, const lds_dbl* w,
+                           const lds_dbl* u, lds_dbl* yrow, lds_dbl* ycol)
 {
   constexpr int NW = NT / 64;
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  double ycol[MAXT];
+  const int lane = threadIdx.x & 63, wid = wave_id();
+  lds_dbl* yc = ycol + wid * ld;
+  for (int c = lo + lane; c < n; c += 64) yc[c] = 0.0;
+  for (int r0 = lo + wid; r0 < n; r0 += RQ * NW) {
+    int r[RQ];
+    bool h[RQ];
+    gbl_dbl* row[RQ];
+    double vr[RQ], wr[RQ], ur[RQ], rs[RQ];
+    int rmax = r0;
 #pragma unroll
-  for (int t = 0; t < MAXT; ++t) ycol[t] = 0.0;
-  for (int r = lo + wid; r < n; r += NW) {
-    double* row = A + (int64_t)r * ld;
-    const double vr = UPDATE ? v[r] : 0.0, wr = UPDATE ? w[r] : 0.0, ur = u[r];
-    const int len = r - lo + 1;
-    double rs = 0.0;
-#pragma unroll
-    for (int t = 0; t < MAXT; ++t) {
-      // wave-uniform guard first: the blocks stay separate basic blocks, so
-      // the loads of later column blocks are not hoisted (register pressure)
-      if (64 * t >= len) break;
-      const int c = lo + lane + 64 * t;
-      if (c <= r) {
-        double a = row[c];
-        if constexpr (UPDATE) {
-          a -= vr * w[c] + wr * v[c];
-          row[c] = a;
-        }
-        rs += a * u[c];
-        if (c < r) ycol[t] += a * ur;
-      }
+    for (int q = 0; q < RQ; ++q) {
+      r[q] = r0 + q * NW;
+      h[q] = r[q] < n;
+      if (h[q]) rmax = r[q];
+      row[q] = A + (int64_t)(h[q] ? r[q] : r0) * ld;
+      vr[q] = (UPDATE && h[q]) ? v[r[q]] : 0.0;
+      wr[q] = (UPDATE && h[q]) ? w[r[q]] : 0.0;
+      ur[q] = h[q] ? u[r[q]] : 0.0;
+      rs[q] = 0.0;
     }
-    rs = wave_sum(rs);
-    if (lane == 0) yrow[r] = rs;
-  }
+#pragma unroll 2
+    for (int cb = lo; cb <= rmax; cb += 64) {
+      const int c = cb + lane;
+      double x[RQ];
+      bool in[RQ];
 #pragma unroll
-  for (int t = 0; t < MAXT; ++t) {
-    const int c = lo + lane + 64 * t;
-    if (c < n) part[(int64_t)wid * ld + c] = ycol[t];
+      for (int q = 0; q < RQ; ++q) {
+        in[q] = h[q] && c <= r[q];
+        x[q] = in[q] ? row[q][c] : 0.0;
+      }
+      const bool kc = c < n;
+      const double uc = kc ? u[c] : 0.0;
+      if constexpr (UPDATE) {
+        const double wc = kc ? w[c] : 0.0, vc = kc ? v[c] : 0.0;
+#pragma unroll
+        for (int q = 0; q < RQ; ++q) {
+          x[q] -= vr[q] * wc + wr[q] * vc;
+          if (in[q]) row[q][c] = x[q];
+          if (!in[q]) x[q] = 0.0;
+        }
+      }
+      double cs = 0.0;
+#pragma unroll
+      for (int q = 0; q < RQ; ++q) {
+        rs[q] += x[q] * uc;
+        if (c < r[q]) cs += x[q] * ur[q];
+      }
+      if (kc) yc[c] += cs;
+    }
+#pragma unroll
+    for (int q = 0; q < RQ; ++q) {
+      const double t = wave_sum(rs[q]);
+      if (lane == 0 && h[q]) yrow[r[q]] = t;
+    }
   }
   __syncthreads();
 }
 
 template <int NT>
-__device__ void combine_partials(int lo, int n, int ld, const double* yrow, const double* part,
-                                 double* y)
+__device__ void combine_partials(int lo, int n, int ld, const lds_dbl* yrow, const lds_dbl* ycol,
+                                 lds_dbl* y)
 {
   constexpr int NW = NT / 64;
   for (int c = lo + threadIdx.x; c < n; c += NT) {
     double s = yrow[c];
 #pragma unroll
-    for (int w = 0; w < NW; ++w) s += part[(int64_t)w * ld + c];
+    for (int w = 0; w < NW; ++w) s += ycol[w * ld + c];
     y[c] = s;
   }
   __syncthreads();
@@ -209,11 +241,12 @@ __device__ void combine_partials(int lo, int n, int ld, const double* yrow, cons
 // A (global, lower triangle) -> d[0..n), e[0..n-1) (T[k+1][k] = e[k]);
 // reflector k (k = 0..n-3) stored as v_k[r] = A[r][k], r >= k+1 (v_k[k+1] = 1),
 // tau[k].  Work vectors v, w, vn, y, yrow: ld doubles of LDS each; part:
-// NW * ld doubles of global scratch.  Requires n >= 3.
-template <int NT, int MAXT>
-__device__ __noinline__ void tridiagonalize(double* A, int n, int ld, double* d, double* e, double* tau,
-                               double* v, double* w, double* vn, double* y, double* yrow,
-                               double* part, double* red, double* scal)
+// NW * ld doubles of LDS.  Requires n >= 3.
+template <int NT>
+__device__ __noinline__ void tridiagonalize(gbl_dbl* A, int n, int ld, lds_dbl* d, lds_dbl* e,
+                                           lds_dbl* tau, lds_dbl* v, lds_dbl* w, lds_dbl* vn,
+                                           lds_dbl* y, lds_dbl* yrow, lds_dbl* part, lds_dbl* red,
+                                           lds_dbl* scal)
 {
   const int tid = threadIdx.x;
   // column 0
@@ -227,7 +260,7 @@ __device__ __noinline__ void tridiagonalize(double* A, int n, int ld, double* d,
   }
   for (int r = 1 + tid; r < n; r += NT) A[(int64_t)r * ld] = v[r];
   // y = B_0 v_0, B_0 = A[1:, 1:]
-  trail_pass<NT, MAXT, false>(A, n, ld, 1, v, v, v, yrow, part);
+  trail_pass<NT, false>(A, n, ld, 1, v, v, v, yrow, part);
   combine_partials<NT>(1, n, ld, yrow, part, y);
 
   for (int k = 0; k <= n - 3; ++k) {
@@ -262,17 +295,159 @@ __device__ __noinline__ void tridiagonalize(double* A, int n, int ld, double* d,
     }
     for (int r = c1 + 1 + tid; r < n; r += NT) A[(int64_t)r * ld + c1] = vn[r];
     // rows/cols >= k+2: rank-2 update fused with y = B_{k+1} v_{k+1}
-    trail_pass<NT, MAXT, true>(A, n, ld, c1 + 1, v, w, vn, yrow, part);
+    trail_pass<NT, true>(A, n, ld, c1 + 1, v, w, vn, yrow, part);
     combine_partials<NT>(c1 + 1, n, ld, yrow, part, y);
     for (int r = c1 + 1 + tid; r < n; r += NT) v[r] = vn[r];
     __syncthreads();
   }
 }
 
+// --------------------------------------------- blocked tridiagonalisation
+// Same result layout as tridiagonalize(), in panels of NP columns (dlatrd
+// scheme): within a panel each column j is first brought up to date with
+// the panel's earlier reflectors,
+//   a_j -= V W_j^T + W V_j^T,
+// its reflector is formed, and y = A_trail v is a read-only pass over the
+// trailing triangle as it was at the start of the panel, corrected by
+//   y -= V (W^T v) + W (V^T v),
+//   w = tau y - (tau^2/2)(y.v) v.
+// After the panel the trailing triangle takes the rank-2 NP update
+// A -= V W^T + W V^T on the matrix cores.  Per column this reads the trailing
+// triangle once (the unblocked scheme reads and writes it).  vt, wt: NP x ld
+// doubles of global scratch (reflectors and w vectors, one per row); dots:
+// 2 NP doubles of LDS; part: NW x ld doubles of LDS (also the GEMM tile
+// region, free between passes).  Requires n >= 3.
+template <int NT, int NP>
+__device__ __noinline__ void tridiagonalize_blocked(gbl_dbl* A, int n, int ld, lds_dbl* d,
+                                                   lds_dbl* e, lds_dbl* tau, lds_dbl* v,
+                                                   lds_dbl* x, lds_dbl* y, lds_dbl* yrow,
+                                                   lds_dbl* dots, lds_dbl* part, gbl_dbl* vt,
+                                                   gbl_dbl* wt, lds_dbl* red, lds_dbl* scal,
+                                                   lds_dbl* tprof)
+{
+  constexpr int NW = NT / 64;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wid = wave_id();
+  const int nr = n - 2;  // reflectors 0 .. n-3
+  // optional step clocks (thread 0, wall-clock ticks added to tprof[0..4])
+  const bool clk_on = tprof != nullptr && tid == 0;
+  long long clk_last = clk_on ? wall_clock64() : 0;
+  auto lap = [&](int slot) {
+    if (clk_on) {
+      const long long t = wall_clock64();
+      tprof[slot] += (double)(t - clk_last);
+      clk_last = t;
+    }
+  };
+  for (int k0 = 0; k0 < nr; k0 += NP) {
+    const int np = (k0 + NP <= nr) ? NP : nr - k0;
+    for (int i = 0; i < np; ++i) {
+      const int j = k0 + i;
+      // (a) column j, rows j..n-1, brought up to date with the panel's reflectors 0..i-1
+      for (int r = j + tid; r < n; r += NT) {
+        double a = A[(int64_t)r * ld + j];
+        for (int q = 0; q < i; ++q)
+          a -= vt[(int64_t)q * ld + r] * wt[(int64_t)q * ld + j] +
+               wt[(int64_t)q * ld + r] * vt[(int64_t)q * ld + j];
+        x[r] = a;
+      }
+      __syncthreads();
+      lap(0);
+      if (tid == 0) d[j] = x[j];
+      // (b) reflector from x[j+1..n)
+      householder<NT>(x, j + 1, n, scal);
+      const double tj = scal[1];
+      if (tid == 0) {
+        e[j] = scal[0];
+        tau[j] = tj;
+      }
+      for (int r = j + 1 + tid; r < n; r += NT) {
+        A[(int64_t)r * ld + j] = x[r];
+        vt[(int64_t)i * ld + r] = x[r];
+        v[r] = x[r];
+      }
+      __syncthreads();
+      lap(1);
+      // (c) panel dot products dots[q] = w_q . v, dots[NP + q] = v_q . v (waves
+      // 0 .. 2i-1 first), then y = A_trail v over rows/cols j+1.. (read-only)
+      for (int pq = wid; pq < 2 * i; pq += NW) {
+        const int q = pq < i ? pq : pq - i;
+        const gbl_dbl* col = (pq < i ? wt : vt) + (int64_t)q * ld;
+        double sacc = 0.0;
+        for (int r = j + 1 + lane; r < n; r += 64) sacc += col[r] * v[r];
+        sacc = wave_sum(sacc);
+        if (lane == 0) dots[(pq < i ? 0 : NP) + q] = sacc;
+      }
+      trail_pass<NT, false, 4>(A, n, ld, j + 1, v, v, v, yrow, part);
+      lap(2);
+      // (d) y = tau (A_trail v - V (W^T v) - W (V^T v)), combined with the
+      // per-wave partial sums in the same pass
+      double sv = 0.0;
+      for (int r = j + 1 + tid; r < n; r += NT) {
+        double yr = yrow[r];
+#pragma unroll
+        for (int w = 0; w < NW; ++w) yr += part[w * ld + r];
+        for (int q = 0; q < i; ++q)
+          yr -= vt[(int64_t)q * ld + r] * dots[q] + wt[(int64_t)q * ld + r] * dots[NP + q];
+        yr *= tj;
+        y[r] = yr;
+        sv += yr * v[r];
+      }
+      sv = block_sum(sv, red);
+      // (e) w = tau y - (tau/2)(tau y . v) v
+      const double alpha = -0.5 * tj * sv;
+      for (int r = j + 1 + tid; r < n; r += NT) wt[(int64_t)i * ld + r] = y[r] + alpha * v[r];
+      // rows <= j of the panel vectors are zero (keeps the rank-2 update exact)
+      for (int r = tid; r <= j; r += NT) {
+        wt[(int64_t)i * ld + r] = 0.0;
+        vt[(int64_t)i * ld + r] = 0.0;
+      }
+      __syncthreads();
+      lap(3);
+    }
+    const int r0 = k0 + np;
+    if (r0 >= nr) {
+      // last two rows/columns: bring the 2 x 2 block up to date, no reflector
+      if (tid < 3) {
+        const int rr = (tid == 0) ? n - 2 : n - 1, cc = (tid == 2) ? n - 1 : n - 2;
+        double a = A[(int64_t)rr * ld + cc];
+        for (int q = 0; q < np; ++q)
+          a -= vt[(int64_t)q * ld + rr] * wt[(int64_t)q * ld + cc] +
+               wt[(int64_t)q * ld + rr] * vt[(int64_t)q * ld + cc];
+        if (tid == 0) d[n - 2] = a;
+        if (tid == 1) e[n - 2] = a;
+        if (tid == 2) d[n - 1] = a;
+      }
+      __syncthreads();
+      break;
+    }
+    // trailing update A[r0:, r0:] -= V W^T + W V^T (lower tiles; element-wise lower)
+    const int m = n - r0;
+    gemm<NT, false, true>(
+        m, m,
+        [=](int ii, int k) {
+          return k < NP ? vt[(int64_t)k * ld + r0 + ii] : wt[(int64_t)(k - NP) * ld + r0 + ii];
+        },
+        [=](int k, int jj) {
+          return k < NP ? wt[(int64_t)k * ld + r0 + jj] : vt[(int64_t)(k - NP) * ld + r0 + jj];
+        },
+        [](int, int, int* ka, int* kb) {
+          *ka = 0;
+          *kb = 2 * NP;
+        },
+        [](int ti, int tj) { return tj > ti; },
+        [=](int ii, int jj, double val) {
+          if (ii >= jj) A[(int64_t)(r0 + ii) * ld + r0 + jj] -= val;
+        },
+        part);
+    lap(4);
+  }
+}
+
 // ---------------------------------------------------------- eigenvalues
 // Number of eigenvalues below each of three points (Sturm sequences run in
 // lockstep for instruction-level parallelism).
-__device__ __forceinline__ void sturm3(const double* d, const double* e2, int n, double x1,
+__device__ __forceinline__ void sturm3(const lds_dbl* d, const lds_dbl* e2, int n, double x1,
                                        double x2, double x3, double pivmin, int& c1, int& c2,
                                        int& c3)
 {
@@ -301,8 +476,8 @@ __device__ __forceinline__ void sturm3(const double* d, const double* e2, int n,
 // All eigenvalues of T (d, e) in ascending order into lam; e2 = e^2 scratch
 // (LDS).  Returns |T| (Gershgorin bound) through scal[0].
 template <int NT>
-__device__ __noinline__ void bisect_all(const double* d, const double* e, double* e2, int n, double* lam,
-                           double* red, double* scal)
+__device__ __noinline__ void bisect_all(const lds_dbl* d, const lds_dbl* e, lds_dbl* e2, int n,
+                                       lds_dbl* lam, lds_dbl* red, lds_dbl* scal)
 {
   const int tid = threadIdx.x;
   double gl = INFINITY, gu = -INFINITY, emax = 0.0;
@@ -365,8 +540,9 @@ __device__ __forceinline__ double start_vector(int i, int k)
 // eigenvector of T for lam[i].  scr: 3 * ld * NT doubles of global scratch
 // (pivots^-1, first and second superdiagonals of U, slot-fastest).
 template <int NT>
-__device__ __noinline__ void inverse_iteration(const double* d, const double* e, const double* lam, int n,
-                                  double tnorm, double* Y, int ld, double* scr)
+__device__ __noinline__ void inverse_iteration(const lds_dbl* d, const lds_dbl* e,
+                                              const lds_dbl* lam, int n, double tnorm, gbl_dbl* Y,
+                                              int ld, gbl_dbl* scr)
 {
   const double pert = EPS * tnorm + 1e-300;
   constexpr int NIT = 3;
@@ -374,9 +550,9 @@ __device__ __noinline__ void inverse_iteration(const double* d, const double* e,
     const int i = i0 + threadIdx.x;
     if (i < n) {
       const int s = threadIdx.x;
-      double* u0 = scr;                         // [k][NT]
-      double* u1 = scr + (int64_t)ld * NT;
-      double* u2 = scr + 2 * (int64_t)ld * NT;
+      gbl_dbl* u0 = scr;                         // [k][NT]
+      gbl_dbl* u1 = scr + (int64_t)ld * NT;
+      gbl_dbl* u2 = scr + 2 * (int64_t)ld * NT;
       const double l = lam[i];
       double scale = 1.0;
       for (int it = 0; it < NIT; ++it) {
@@ -438,11 +614,12 @@ __device__ __noinline__ void inverse_iteration(const double* d, const double* e,
 // ORTOL |T|): one wave per run, modified Gram-Schmidt twice, then normalise.
 // runs: int scratch of ld entries (LDS); nruns via scal.
 template <int NT>
-__device__ __noinline__ void orthogonalize_clusters(const double* lam, int n, double tnorm, double* Y, int ld,
-                                       int* runs, double* scal)
+__device__ __noinline__ void orthogonalize_clusters(const lds_dbl* lam, int n, double tnorm,
+                                                   gbl_dbl* Y, int ld, lds_int* runs,
+                                                   lds_dbl* scal)
 {
   constexpr int NW = NT / 64;
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wid = wave_id();
   if (threadIdx.x == 0) {
     int nr = 0;
     const double gap = ORTOL * tnorm;
@@ -486,14 +663,16 @@ __device__ __noinline__ void orthogonalize_clusters(const double* lam, int n, do
 
 // -------------------------------------------------------- back-transform
 // Y <- Q Y, Q = H_0 H_1 ... H_{n-3}, reflectors from tridiagonalize().  Blocks
-// of NB reflectors, last block first; vb: NB * ld doubles of LDS, tm: NB*NB.
+// of NB reflectors, last block first.  A block is I - V T V^T with T^-1 =
+// diag(1/tau) + strict_upper(V^T V) (the reflectors satisfy tau = 2 / v.v), so
+//   W = V^T Y,  solve T^-1 W' = W,  Y -= V W'
+// three matrix-core products and a small triangular inversion per block.
+// W: NB x ld doubles of global scratch; sinv, tm: NB x NB doubles of LDS.
 template <int NT, int NB>
-__device__ __noinline__ void back_transform(const double* A, const double* tau, int n, int ld, double* Y,
-                               double* vb, double* tm, double* gm)
+__device__ __noinline__ void back_transform(const gbl_dbl* A, const lds_dbl* tau, int n, int ld,
+                                            gbl_dbl* Y, gbl_dbl* W, lds_dbl* sinv, lds_dbl* tm,
+                                            lds_dbl* tile)
 {
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wid = tid >> 6;
-  constexpr int NW = NT / 64;
   const int nr = n - 2;
   const int nblk = (nr + NB - 1) / NB;
   for (int bi = nblk - 1; bi >= 0; --bi) {
@@ -501,65 +680,67 @@ __device__ __noinline__ void back_transform(const double* A, const double* tau, 
     const int nb = (k0 + NB <= nr) ? NB : nr - k0;
     const int r0 = k0 + 1;
     const int m = n - r0;
-    // V block: vb[(r - r0) * NB + j] = v_{k0+j}[r]
-    for (int idx = tid; idx < m * NB; idx += NT) {
-      const int rr = idx / NB, j = idx - (idx / NB) * NB;
-      const int r = r0 + rr;
-      vb[idx] = (j < nb && r >= k0 + j + 1) ? A[(int64_t)r * ld + k0 + j] : 0.0;
-    }
-    __syncthreads();
-    // Gram entries G[l][j] = v_l . v_j (l < j), one wave per pair
-    for (int pr = wid; pr < NB * NB; pr += NW) {
-      const int l = pr / NB, j = pr - (pr / NB) * NB;
-      if (l < j && j < nb) {
-        double s = 0.0;
-        for (int rr = lane; rr < m; rr += 64) s += vb[rr * NB + l] * vb[rr * NB + j];
-        s = wave_sum(s);
-        if (lane == 0) gm[l * NB + j] = s;
-      }
-    }
-    __syncthreads();
-    // compact WY factor (dlarft, forward, columnwise)
-    if (tid == 0) {
-      for (int j = 0; j < NB; ++j)
-        for (int l = 0; l < NB; ++l) tm[l * NB + j] = 0.0;
-      for (int j = 0; j < nb; ++j) {
-        const double tj = tau[k0 + j];
-        tm[j * NB + j] = tj;
-        for (int l = 0; l < j; ++l) {
+    // V(rr, j) = v_{k0+j}[r0 + rr]: nonzero for rr >= j; a reflector with
+    // tau = 0 is the identity and gets a zero column
+    auto vel = [=](int rr, int j) -> double {
+      return (j < nb && rr >= j && tau[k0 + j] != 0.0) ? A[(int64_t)(r0 + rr) * ld + k0 + j]
+                                                       : 0.0;
+    };
+    auto all_k = [m](int, int, int* ka, int* kb) {
+      *ka = 0;
+      *kb = m;
+    };
+    auto no_skip = [](int, int) { return false; };
+    // T^-1 = diag(1/tau) + strict_upper(V^T V)
+    gemm<NT, false, true>(
+        NB, NB, [=](int i, int k) { return vel(k, i); }, [=](int k, int j) { return vel(k, j); },
+        all_k, no_skip,
+        [=](int i, int j, double val) {
           double s = 0.0;
-          for (int q = l; q < j; ++q) s += tm[l * NB + q] * gm[q * NB + j];
-          tm[l * NB + j] = -tj * s;
+          if (i < j) s = val;
+          if (i == j) s = (i < nb && tau[k0 + i] != 0.0) ? 1.0 / tau[k0 + i] : 1.0;
+          sinv[i * NB + j] = s;
+        },
+        tile);
+    // W = V^T Y[r0:, :]
+    gemm<NT, false, true>(
+        NB, n, [=](int i, int k) { return vel(k, i); },
+        [=](int k, int j) { return Y[(int64_t)(r0 + k) * ld + j]; }, all_k, no_skip,
+        [=](int i, int j, double val) { W[(int64_t)i * ld + j] = val; }, tile);
+    // T = (T^-1)^-1, upper triangular: one thread per column, back substitution
+    if (threadIdx.x < NB) {
+      const int j = threadIdx.x;
+      for (int i = NB - 1; i >= 0; --i) {
+        double t = 0.0;
+        if (i == j) {
+          t = 1.0 / sinv[i * NB + i];
+        } else if (i < j) {
+          double acc = 0.0;
+          for (int k = i + 1; k <= j; ++k) acc += sinv[i * NB + k] * tm[k * NB + j];
+          t = -acc / sinv[i * NB + i];
         }
+        tm[i * NB + j] = t;
       }
     }
     __syncthreads();
-    for (int c = tid; c < n; c += NT) {
-      double wv[NB];
-#pragma unroll
-      for (int j = 0; j < NB; ++j) wv[j] = 0.0;
-      for (int rr = 0; rr < m; ++rr) {
-        const double y = Y[(int64_t)(r0 + rr) * ld + c];
-#pragma unroll
-        for (int j = 0; j < NB; ++j) wv[j] += vb[rr * NB + j] * y;
-      }
-      double w2[NB];
-#pragma unroll
-      for (int l = 0; l < NB; ++l) {
-        double s = 0.0;
-#pragma unroll
-        for (int j = 0; j < NB; ++j)
-          if (j >= l) s += tm[l * NB + j] * wv[j];
-        w2[l] = s;
-      }
-      for (int rr = 0; rr < m; ++rr) {
-        double s = 0.0;
-#pragma unroll
-        for (int j = 0; j < NB; ++j) s += vb[rr * NB + j] * w2[j];
-        Y[(int64_t)(r0 + rr) * ld + c] -= s;
-      }
-    }
-    __syncthreads();
+    // W' = T W (in place: a tile reads its own columns of W before storing them)
+    gemm<NT, true, true>(
+        NB, n, [=](int i, int k) { return tm[i * NB + k]; },
+        [=](int k, int j) { return W[(int64_t)k * ld + j]; },
+        [](int, int, int* ka, int* kb) {
+          *ka = 0;
+          *kb = NB;
+        },
+        no_skip, [=](int i, int j, double val) { W[(int64_t)i * ld + j] = val; }, tile);
+    // Y[r0:, :] -= V W'
+    gemm<NT, true, true>(
+        m, n, [=](int i, int k) { return vel(i, k); },
+        [=](int k, int j) { return W[(int64_t)k * ld + j]; },
+        [](int, int, int* ka, int* kb) {
+          *ka = 0;
+          *kb = NB;
+        },
+        no_skip, [=](int i, int j, double val) { Y[(int64_t)(r0 + i) * ld + j] -= val; }, tile);
   }
 }
 

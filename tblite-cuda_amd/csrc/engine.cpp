@@ -4,11 +4,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <new>
 #include <string>
 #include <vector>
@@ -32,6 +34,7 @@ namespace {
 constexpr int kIntegralChunk = 128;
 constexpr int kLdsMaxLd = 96;
 constexpr int kNumStreams = 4;
+constexpr int kMaxBuckets = 8;
 
 // ------------------------------------------------------------ calculator
 struct Calculator {
@@ -224,6 +227,8 @@ struct tblite_amd_context_s {
   hipStream_t aux[kNumStreams] = {};
   hipEvent_t ev[8] = {};
   hipEvent_t aux_done[kNumStreams] = {};
+  hipEvent_t bev[2 * kMaxBuckets] = {};  // around each SCF bucket launch, on its stream
+  double bucket_ms[kMaxBuckets] = {};
   std::string error;
   Arena arena;
   Plan plan;
@@ -660,10 +665,15 @@ int launch_scf(tblite_amd_context ctx, hipStream_t st, const BatchDev& b, const 
   return TBLITE_AMD_OK;
 }
 
+int finish_batch(tblite_amd_context ctx);
+
 // Run the whole pipeline for the planned batch.  xyz (host) is uploaded
 // first when given; with xyz == nullptr the coordinates already resident in
-// the batch buffer are used.
-int run_batch(tblite_amd_context ctx, Calculator& c, const double* xyz, double acc)
+// the batch buffer are used.  With wait == false the work is only enqueued:
+// ev[4] on the context stream marks its end and finish_batch() collects the
+// phase timings.
+int run_batch(tblite_amd_context ctx, Calculator& c, const double* xyz, double acc,
+              bool wait = true)
 {
   Plan& pl = ctx->plan;
   DeviceBatch& db = ctx->batch;
@@ -704,6 +714,7 @@ int run_batch(tblite_amd_context ctx, Calculator& c, const double* xyz, double a
     const int* mols = db.d_bucket_mols[k];
     const int cnt = (int)bk.mols.size();
     int rc;
+    HIP_TRY(ctx, hipEventRecord(ctx->bev[2 * k], as));
     // per-worker-thread Jacobi items (MAXB blocks of A, MAXC rows of CT) over the
     // NT - 64 worker threads: ld <= 32: 120 / 512 over 64; ld <= 64: 496 / 2048
     // over 192; ld <= 96: 1128 / 4608 over 448
@@ -717,13 +728,21 @@ int run_batch(tblite_amd_context ctx, Calculator& c, const double* xyz, double a
     else
       rc = launch_scf<512, 2, 3, 11, true>(ctx, as, b, mols, cnt, bk.ldmax, bk.natmax, bk.nshmax);
     if (rc) return rc;
+    HIP_TRY(ctx, hipEventRecord(ctx->bev[2 * k + 1], as));
   }
   for (int s = 0; s < kNumStreams; ++s) {
     HIP_TRY(ctx, hipEventRecord(ctx->aux_done[s], ctx->aux[s]));
     HIP_TRY(ctx, hipStreamWaitEvent(st, ctx->aux_done[s], 0));
   }
   HIP_TRY(ctx, hipEventRecord(ctx->ev[4], st));
-  HIP_TRY(ctx, hipStreamSynchronize(st));
+  if (!wait) return TBLITE_AMD_OK;
+  return finish_batch(ctx);
+}
+
+int finish_batch(tblite_amd_context ctx)
+{
+  Plan& pl = ctx->plan;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   float t01, t12, t23, t34, t04;
   HIP_TRY(ctx, hipEventElapsedTime(&t01, ctx->ev[0], ctx->ev[1]));
   HIP_TRY(ctx, hipEventElapsedTime(&t12, ctx->ev[1], ctx->ev[2]));
@@ -735,6 +754,11 @@ int run_batch(tblite_amd_context ctx, Calculator& c, const double* xyz, double a
   ctx->timings[2] = t23;
   ctx->timings[3] = t34;
   ctx->timings[4] = t04;
+  for (size_t k = 0; k < pl.buckets.size(); ++k) {
+    float tb = 0.0f;
+    HIP_TRY(ctx, hipEventElapsedTime(&tb, ctx->bev[2 * k], ctx->bev[2 * k + 1]));
+    ctx->bucket_ms[k] = tb;
+  }
   return TBLITE_AMD_OK;
 }
 
@@ -825,6 +849,7 @@ tblite_amd_context tblite_amd_new_context(int device)
     ok = hipStreamCreateWithFlags(&ctx->aux[s], hipStreamNonBlocking) == hipSuccess &&
          hipEventCreateWithFlags(&ctx->aux_done[s], hipEventDisableTiming) == hipSuccess;
   for (int e = 0; e < 8 && ok; ++e) ok = hipEventCreate(&ctx->ev[e]) == hipSuccess;
+  for (int e = 0; e < 2 * kMaxBuckets && ok; ++e) ok = hipEventCreate(&ctx->bev[e]) == hipSuccess;
   int lds = 0;
   if (ok) ok = hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess;
   ctx->lds_limit = (size_t)lds;
@@ -852,6 +877,8 @@ void tblite_amd_delete_context(tblite_amd_context* pctx)
   }
   for (int e = 0; e < 8; ++e)
     if (ctx->ev[e]) (void)hipEventDestroy(ctx->ev[e]);
+  for (int e = 0; e < 2 * kMaxBuckets; ++e)
+    if (ctx->bev[e]) (void)hipEventDestroy(ctx->bev[e]);
   if (ctx->arena.base) (void)hipFree(ctx->arena.base);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -1293,10 +1320,315 @@ int tblite_amd_last_profile(tblite_amd_context ctx, long long* out)
   return TA_PROF_SLOTS;
 }
 
+int tblite_amd_last_bucket_timings(tblite_amd_context ctx, double* ms, int* ldmax, int* nmol,
+                                   int* nthreads, int capacity)
+{
+  if (!ctx || capacity < 0) return TBLITE_AMD_ERR_ARGUMENT;
+  if (!ctx->have_batch) {
+    set_error(ctx, "no batch has been run on this context");
+    return TBLITE_AMD_ERR_ARGUMENT;
+  }
+  const int nb = (int)ctx->plan.buckets.size();
+  for (int k = 0; k < nb && k < capacity; ++k) {
+    const Plan::Bucket& bk = ctx->plan.buckets[k];
+    if (ms) ms[k] = ctx->bucket_ms[k];
+    if (ldmax) ldmax[k] = bk.ldmax;
+    if (nmol) nmol[k] = (int)bk.mols.size();
+    if (nthreads) nthreads[k] = bk.nt;
+  }
+  return nb;
+}
+
 int tblite_amd_last_timings(tblite_amd_context ctx, double* times, int n)
 {
   if (!ctx || !times) return TBLITE_AMD_ERR_ARGUMENT;
   for (int i = 0; i < n && i < 5; ++i) times[i] = ctx->timings[i];
+  return TBLITE_AMD_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ serving
+// Continuous batching over independent singlepoint requests.  The server
+// owns `slots` engine contexts on the same device, each with its own streams
+// and a device workspace reserved once (budget / slots bytes).  A slot takes
+// requests from the head of the FIFO queue while their estimated footprint
+// fits its workspace, runs them as one batch, and is refilled as soon as its
+// batch has finished; batches of different slots run concurrently.  Requests
+// that arrive while others are on the device wait only for the next free
+// slot.  Wavefunction and Broyden state stay on the device for the whole SCF
+// of a batch.  No counterpart in the reference (one molecule per GPU thread,
+// src/xtb.cu:479-497).
+namespace {
+
+struct ServeRequest {
+  long long ticket;
+  int nat;
+  std::vector<int> num;
+  std::vector<double> xyz;
+  double charge;
+  int uhf;
+  double t_submit;
+  size_t bytes;
+};
+
+struct ServeResult {
+  long long ticket;
+  double energy;
+  int iterations;
+  int status;
+  double latency_ms;
+};
+
+struct ServeSlot {
+  tblite_amd_context ctx = nullptr;
+  std::vector<ServeRequest> reqs;
+  std::vector<double> xyz;  // host coordinates of the running batch (kept until it ends)
+  bool busy = false;
+};
+
+double host_ms()
+{
+  using clk = std::chrono::steady_clock;
+  return std::chrono::duration<double, std::milli>(clk::now().time_since_epoch()).count();
+}
+
+// Upper estimate of the device workspace a molecule adds to a batch (all
+// per-molecule arrays of upload_plan, alignment slack included).
+size_t request_bytes(const Calculator& c, int nat, const int* num)
+{
+  int nsh = 0, nao = 0;
+  for (int i = 0; i < nat; ++i) {
+    const int z = num[i];
+    const int e = (z > 0 && z < 120) ? c.zmap[z] : -1;
+    if (e < 0) return 0;
+    const DevElem& el = c.dev.el[e];
+    nsh += el.nsh;
+    for (int s = 0; s < el.nsh; ++s) nao += 2 * el.ang[s] + 1;
+  }
+  const int64_t mem = std::max(1, c.max_iter);
+  const int64_t ndim = nsh + 9 * (int64_t)nat;
+  const int64_t ld = (nao + 3) & ~3;
+  int64_t d = 11 * (int64_t)nao * nao + (int64_t)nsh * nsh + 18 * (int64_t)nat * nat;
+  d += (int64_t)nat * (nat + 1) / 2 + 24 * (int64_t)nat + 6 * nsh + 4 * nao;
+  d += 4 * ndim + 2 * ndim * mem + mem * mem + mem * (mem + 1) + 2 * mem + 8 + TA_PROF_SLOTS;
+  if (ld > kLdsMaxLd) d += 4 * ld * ld + big_scratch_doubles((int)ld);
+  return (size_t)d * 8 + 64 * 256;
+}
+
+}  // namespace
+
+struct tblite_amd_server_s {
+  tblite_amd_context owner = nullptr;
+  tblite_amd_calculator calc = nullptr;
+  std::vector<ServeSlot> slots;
+  std::deque<ServeRequest> queue;
+  std::deque<ServeResult> done;
+  size_t slot_bytes = 0;
+  int max_batch = 4096;
+  long long next_ticket = 1, submitted = 0, completed = 0, batches = 0;
+};
+
+extern "C" {
+
+tblite_amd_server tblite_amd_new_server(tblite_amd_context ctx, tblite_amd_calculator calc,
+                                        int slots, double workspace_bytes)
+{
+  if (!ctx || !calc || slots < 1 || slots > 16 || !(workspace_bytes > 0.0)) {
+    if (ctx) set_error(ctx, "new_server: need 1..16 slots and a positive workspace");
+    return nullptr;
+  }
+  if (hipSetDevice(ctx->device) != hipSuccess) return nullptr;
+  if (ensure_params_on_device(ctx, calc->c)) return nullptr;
+  tblite_amd_server srv = new (std::nothrow) tblite_amd_server_s();
+  if (!srv) return nullptr;
+  srv->owner = ctx;
+  srv->calc = calc;
+  srv->slot_bytes = (size_t)(workspace_bytes / slots);
+  srv->slots.resize(slots);
+  for (auto& sl : srv->slots) {
+    sl.ctx = tblite_amd_new_context(ctx->device);
+    bool ok = sl.ctx != nullptr;
+    if (ok) {
+      // reserve the slot's workspace once: growing it later would free device
+      // memory, which synchronises the whole device
+      Arena& ar = sl.ctx->arena;
+      ok = hipMalloc(&ar.base, srv->slot_bytes) == hipSuccess;
+      if (ok) ar.cap = srv->slot_bytes;
+    }
+    if (!ok) {
+      set_error(ctx, "new_server: cannot create slot context / reserve %zu bytes", srv->slot_bytes);
+      tblite_amd_delete_server(&srv);
+      return nullptr;
+    }
+  }
+  return srv;
+}
+
+void tblite_amd_delete_server(tblite_amd_server* psrv)
+{
+  if (!psrv || !*psrv) return;
+  for (auto& sl : (*psrv)->slots)
+    if (sl.ctx) tblite_amd_delete_context(&sl.ctx);
+  delete *psrv;
+  *psrv = nullptr;
+}
+
+int tblite_amd_server_submit(tblite_amd_server srv, int nat, const int* num, const double* xyz,
+                             double charge, int uhf, long long* ticket)
+{
+  if (!srv) return TBLITE_AMD_ERR_ARGUMENT;
+  tblite_amd_context ctx = srv->owner;
+  if (nat < 1 || !num || !xyz) {
+    set_error(ctx, "server_submit: invalid molecule");
+    return TBLITE_AMD_ERR_ARGUMENT;
+  }
+  const size_t bytes = request_bytes(srv->calc->c, nat, num);
+  if (bytes == 0) {
+    set_error(ctx, "server_submit: element without parameters");
+    return TBLITE_AMD_ERR_ARGUMENT;
+  }
+  if (bytes + 64 * 1024 > srv->slot_bytes) {
+    set_error(ctx, "server_submit: molecule needs %zu bytes, a slot holds %zu", bytes,
+              srv->slot_bytes);
+    return TBLITE_AMD_ERR_TOO_LARGE;
+  }
+  ServeRequest r;
+  r.ticket = srv->next_ticket++;
+  r.nat = nat;
+  r.num.assign(num, num + nat);
+  r.xyz.assign(xyz, xyz + 3 * (size_t)nat);
+  r.charge = charge;
+  r.uhf = uhf;
+  r.t_submit = host_ms();
+  r.bytes = bytes;
+  srv->queue.push_back(std::move(r));
+  ++srv->submitted;
+  if (ticket) *ticket = srv->queue.back().ticket;
+  return TBLITE_AMD_OK;
+}
+
+namespace {
+
+// Collect a finished slot's results; returns OK or an error code.
+int serve_collect(tblite_amd_server srv, ServeSlot& sl)
+{
+  tblite_amd_context sc = sl.ctx;
+  int rc = finish_batch(sc);
+  if (rc) return rc;
+  const int nmol = (int)sl.reqs.size();
+  std::vector<double> e(nmol);
+  std::vector<int> it(nmol), st(nmol);
+  const BatchDev& b = sc->batch.b;
+  HIP_TRY(srv->owner, hipMemcpy(e.data(), b.energy, 8 * nmol, hipMemcpyDeviceToHost));
+  HIP_TRY(srv->owner, hipMemcpy(it.data(), b.iterations, 4 * nmol, hipMemcpyDeviceToHost));
+  HIP_TRY(srv->owner, hipMemcpy(st.data(), b.status, 4 * nmol, hipMemcpyDeviceToHost));
+  const double now = host_ms();
+  for (int m = 0; m < nmol; ++m) {
+    ServeResult res;
+    res.ticket = sl.reqs[m].ticket;
+    res.energy = e[m];
+    res.iterations = it[m];
+    res.status = st[m];
+    res.latency_ms = now - sl.reqs[m].t_submit;
+    srv->done.push_back(res);
+  }
+  srv->completed += nmol;
+  sc->have_batch = true;
+  sl.reqs.clear();
+  sl.busy = false;
+  return TBLITE_AMD_OK;
+}
+
+// Fill a free slot from the queue head and launch its batch.
+int serve_launch(tblite_amd_server srv, ServeSlot& sl)
+{
+  size_t used = 64 * 1024 * 8;
+  while (!srv->queue.empty() && (int)sl.reqs.size() < srv->max_batch &&
+         used + srv->queue.front().bytes <= srv->slot_bytes) {
+    used += srv->queue.front().bytes;
+    sl.reqs.push_back(std::move(srv->queue.front()));
+    srv->queue.pop_front();
+  }
+  if (sl.reqs.empty()) return TBLITE_AMD_OK;
+  const int nmol = (int)sl.reqs.size();
+  std::vector<int> mol_nat(nmol), num, uhf(nmol);
+  std::vector<double>& xyz = sl.xyz;
+  std::vector<double> charge(nmol);
+  xyz.clear();
+  for (int m = 0; m < nmol; ++m) {
+    const ServeRequest& r = sl.reqs[m];
+    mol_nat[m] = r.nat;
+    num.insert(num.end(), r.num.begin(), r.num.end());
+    xyz.insert(xyz.end(), r.xyz.begin(), r.xyz.end());
+    charge[m] = r.charge;
+    uhf[m] = r.uhf;
+  }
+  tblite_amd_context sc = sl.ctx;
+  Calculator& c = srv->calc->c;
+  sc->have_batch = false;
+  sc->prepared = nullptr;
+  int rc = make_plan(sc, c, nmol, mol_nat.data(), num.data(), charge.data(), uhf.data(),
+                     c.accuracy, sc->plan);
+  if (!rc) rc = upload_plan(sc, c, sc->plan, sc->batch);
+  if (!rc) rc = run_batch(sc, c, xyz.data(), c.accuracy, false);
+  if (rc) {
+    set_error(srv->owner, "server: batch launch failed: %s", sc->error.c_str());
+    return rc;
+  }
+  sl.busy = true;
+  ++srv->batches;
+  return TBLITE_AMD_OK;
+}
+
+}  // namespace
+
+int tblite_amd_server_poll(tblite_amd_server srv, int capacity, long long* tickets,
+                           double* energies, int* iterations, int* status, double* latency_ms)
+{
+  if (!srv || capacity < 0) return TBLITE_AMD_ERR_ARGUMENT;
+  HIP_TRY(srv->owner, hipSetDevice(srv->owner->device));
+  for (auto& sl : srv->slots) {
+    if (sl.busy) {
+      const hipError_t q = hipEventQuery(sl.ctx->ev[4]);
+      if (q == hipErrorNotReady) continue;
+      if (q != hipSuccess) {
+        set_error(srv->owner, "server: batch failed: %s", hipGetErrorString(q));
+        return TBLITE_AMD_ERR_HIP;
+      }
+      int rc = serve_collect(srv, sl);
+      if (rc) return rc;
+    }
+    if (!sl.busy && !srv->queue.empty()) {
+      int rc = serve_launch(srv, sl);
+      if (rc) return rc;
+    }
+  }
+  int k = 0;
+  while (k < capacity && !srv->done.empty()) {
+    const ServeResult& r = srv->done.front();
+    if (tickets) tickets[k] = r.ticket;
+    if (energies) energies[k] = r.energy;
+    if (iterations) iterations[k] = r.iterations;
+    if (status) status[k] = r.status;
+    if (latency_ms) latency_ms[k] = r.latency_ms;
+    srv->done.pop_front();
+    ++k;
+  }
+  return k;
+}
+
+int tblite_amd_server_stats(tblite_amd_server srv, long long* submitted, long long* completed,
+                            int* queued, int* busy_slots, long long* batches)
+{
+  if (!srv) return TBLITE_AMD_ERR_ARGUMENT;
+  int busy = 0;
+  for (const auto& sl : srv->slots) busy += sl.busy ? 1 : 0;
+  if (submitted) *submitted = srv->submitted;
+  if (completed) *completed = srv->completed;
+  if (queued) *queued = (int)srv->queue.size();
+  if (busy_slots) *busy_slots = busy;
+  if (batches) *batches = srv->batches;
   return TBLITE_AMD_OK;
 }
 

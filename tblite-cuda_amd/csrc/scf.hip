@@ -474,25 +474,6 @@ __device__ double fermi_wave(double nel, double kt, int n, const double* emo, do
 
 // Phase clock: thread 0 accumulates wall-clock ticks per phase when the
 // batch carries a profile buffer.
-struct PhaseClock {
-  long long acc[TA_PROF_SLOTS];
-  long long last;
-  bool on;
-  __device__ void start(bool enable)
-  {
-    on = enable && threadIdx.x == 0;
-    for (int i = 0; i < TA_PROF_SLOTS; ++i) acc[i] = 0;
-    if (on) last = wall_clock64();
-  }
-  __device__ void lap(int slot)
-  {
-    if (on) {
-      const long long t = wall_clock64();
-      acc[slot] += t - last;
-      last = t;
-    }
-  }
-};
 
 // ------------------------------------------- global-memory eigensolve
 // F (in A, full) -> eigenvalues ev (ascending) and MO rows CT = Z^T X of the
@@ -500,20 +481,31 @@ struct PhaseClock {
 // eigensolver region er (LDS) holds d, e, tau, e^2 and the work vectors of
 // the tridiagonalisation, followed by the GEMM tiles / reflector block.
 template <int NT>
-__device__ __noinline__ void big_eigensolve(double* A, const double* X, double* Y, double* CT, double* scr,
-                               int n, int ld, double* er, double* ev, double* red, double* scal,
-                               int* runs, PhaseClock& clk)
+__device__ __noinline__ void big_eigensolve(double* A_, const double* X_, double* Y_, double* CT_,
+                                            double* scr_, int n, int ld, double* er_, double* ev_,
+                                            double* red_, double* scal_, int* runs_,
+                                            PhaseClock& clk)
 {
-  double* d = er;
-  double* e = er + ld;
-  double* tau = er + 2 * ld;
-  double* e2 = er + 3 * ld;
-  double* vv = er + 4 * ld;
-  double* ww = er + 5 * ld;
-  double* vn = er + 6 * ld;
-  double* yy = er + 7 * ld;
-  double* yrow = er + 8 * ld;
-  double* tmp = er + 9 * ld;
+  gbl_dbl* A = TA_GBL(A_);
+  const gbl_dbl* X = TA_GBL(X_);
+  gbl_dbl* Y = TA_GBL(Y_);
+  gbl_dbl* CT = TA_GBL(CT_);
+  gbl_dbl* scr = TA_GBL(scr_);
+  lds_dbl* er = TA_LDS(er_);
+  lds_dbl* ev = TA_LDS(ev_);
+  lds_dbl* red = TA_LDS(red_);
+  lds_dbl* scal = TA_LDS(scal_);
+  lds_int* runs = TA_LDSI(runs_);
+  lds_dbl* d = er;
+  lds_dbl* e = er + ld;
+  lds_dbl* tau = er + 2 * ld;
+  lds_dbl* e2 = er + 3 * ld;
+  lds_dbl* vv = er + 4 * ld;
+  lds_dbl* ww = er + 5 * ld;
+  lds_dbl* vn = er + 6 * ld;
+  lds_dbl* yy = er + 7 * ld;
+  lds_dbl* yrow = er + 8 * ld;
+  lds_dbl* tmp = er + 9 * ld;
   auto k_lower = [n](int ti, int, int* k0, int* k1) {
     *k0 = 0;
     *k1 = min(n, ti * 64 + 64);
@@ -530,18 +522,29 @@ __device__ __noinline__ void big_eigensolve(double* A, const double* X, double* 
       [=](int k, int j) { return Y[(int64_t)k * ld + j]; }, k_lower, upper_tile,
       [=](int i, int j, double val) { A[(int64_t)i * ld + j] = val; }, tmp);
   clk.lap(TA_PROF_TRANSFORM);
-  double* part = scr + 3 * (int64_t)ld * TA_BIG_NT;
-  big::tridiagonalize<NT, 16>(A, n, ld, d, e, tau, vv, ww, vn, yy, yrow, part, red, scal);
+  // per-wave column partials of the tridiagonalisation share the GEMM tile region
+  gbl_dbl* wbt = scr + 3 * (int64_t)ld * TA_BIG_NT;
+  gbl_dbl* pvt = wbt + (int64_t)TA_BIG_NB * ld;
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 5; ++q) red[16 + q] = 0.0;
+  big::tridiagonalize_blocked<NT, TA_BIG_NP>(A, n, ld, d, e, tau, vv, vn, yy, yrow, ww, tmp, pvt,
+                                             pvt + (int64_t)TA_BIG_NP * ld, red, scal,
+                                             clk.on ? red + 16 : nullptr);
+  if (clk.on) {
+    for (int q = 0; q < 5; ++q) clk.acc[TA_PROF_TRI_COLUMN + q] += (long long)red[16 + q];
+    clk.last = wall_clock64();
+  }
   clk.lap(TA_PROF_TRIDIAG);
   big::bisect_all<NT>(d, e, e2, n, ev, red, scal);
   const double tnorm = scal[0];
   __syncthreads();
   clk.lap(TA_PROF_BISECT);
   big::inverse_iteration<NT>(d, e, ev, n, tnorm, Y, ld, scr);
-  big::orthogonalize_clusters<NT>(ev, n, tnorm, Y, ld, runs, scal);
   clk.lap(TA_PROF_INVIT);
-  big::back_transform<NT, TA_BIG_NB>(A, tau, n, ld, Y, tmp, tmp + TA_BIG_NB * ld,
-                                     tmp + TA_BIG_NB * ld + TA_BIG_NB * TA_BIG_NB);
+  big::orthogonalize_clusters<NT>(ev, n, tnorm, Y, ld, runs, scal);
+  clk.lap(TA_PROF_CLUSTERS);
+  big::back_transform<NT, TA_BIG_NB>(A, tau, n, ld, Y, wbt, tmp + 2048,
+                                     tmp + 2048 + TA_BIG_NB * TA_BIG_NB, tmp);
   clk.lap(TA_PROF_BACKTR);
   // CT = Z^T X; X[k][a] = 0 for a > k
   big::gemm<NT, false, true>(
@@ -558,10 +561,14 @@ __device__ __noinline__ void big_eigensolve(double* A, const double* X, double* 
 // P = sum_t f_t CT[occ_t]^T CT[occ_t] (get_density_matrix,
 // src/wavefunction/type.cu:17-35 text), lower tiles mirrored; P plain n x n.
 template <int NT>
-__device__ __noinline__ void big_density(const double* CT, const double* f, const int* occ, int nocc,
-                            double* P, int n, int ld, double* er)
+__device__ __noinline__ void big_density(const double* CT_, const double* f_, const int* occ_,
+                                         int nocc, double* P_, int n, int ld, double* er_)
 {
-  double* tmp = er + 9 * ld;
+  const gbl_dbl* CT = TA_GBL(CT_);
+  gbl_dbl* P = TA_GBL(P_);
+  const lds_dbl* f = TA_LDS(f_);
+  const lds_int* occ = TA_LDSI(occ_);
+  lds_dbl* tmp = TA_LDS(er_) + 9 * ld;
   big::gemm<NT, false, true>(
       n, n,
       [=](int i, int t) {
@@ -584,8 +591,10 @@ __device__ __noinline__ void big_density(const double* CT, const double* f, cons
 }
 
 // ---------------------------------------------------------------- kernel
+// The global-memory variant runs two workgroups per CU (128 VGPRs): its phases
+// are bound by memory latency, not by arithmetic.
 template <int NT, int MAXT, int MAXB, int MAXC, bool MATS_LDS>
-__global__ __launch_bounds__(NT) void k_scf(BatchDev b, const int* __restrict__ mols, int ldmax,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? 1 : 4))) void k_scf(BatchDev b, const int* __restrict__ mols, int ldmax,
                                             int natmax, int nshmax)
 {
   extern __shared__ double smem[];
@@ -952,20 +961,40 @@ __global__ __launch_bounds__(NT) void k_scf(BatchDev b, const int* __restrict__ 
 
     // ------------------------------------------------ Fock matrix
     // A1[i][j] = (H_ij - S_ij (v_i + v_j)/2)/2 - (D(.,j,i).vdp(at i) + Q(.,j,i).vqp(at i))/2
-    for (int idx = tid; idx < ld * ld; idx += NT) {
-      const int i = idx / ld, j = idx - (idx / ld) * ld;
-      double f = 0.0;
-      if (i < n && j < n) {
-        const int64_t e = (int64_t)i * n + j;
-        const int ai = v.ao_at[i];
-        const double* vd = vdp + 3 * ai;
-        const double* vq = vqp + 6 * ai;
-        f = 0.5 * (v.H[e] - v.S[e] * 0.5 * (vao[i] + vao[j]));
-        f -= 0.5 * (v.D[e] * vd[0] + v.D[n2 + e] * vd[1] + v.D[2 * n2 + e] * vd[2]);
-        f -= 0.5 * (v.Q[e] * vq[0] + v.Q[n2 + e] * vq[1] + v.Q[2 * n2 + e] * vq[2] +
-                    v.Q[3 * n2 + e] * vq[3] + v.Q[4 * n2 + e] * vq[4] + v.Q[5 * n2 + e] * vq[5]);
+    // four elements per thread and step: their eleven integral loads each are
+    // issued before any store
+    {
+      constexpr int FB = 4;
+      const int tot = ld * ld;
+      for (int base = tid; base < tot; base += FB * NT) {
+        double fv[FB];
+#pragma unroll
+        for (int u = 0; u < FB; ++u) {
+          const int idx = base + u * NT;
+          const int i = idx / ld, j = idx - (idx / ld) * ld;
+          double f = 0.0;
+          if (idx < tot && i < n && j < n) {
+            const int64_t e = (int64_t)i * n + j;
+            const int ai = v.ao_at[i];
+            const double* vd = vdp + 3 * ai;
+            const double* vq = vqp + 6 * ai;
+            f = 0.5 * (v.H[e] - v.S[e] * 0.5 * (vao[i] + vao[j]));
+            f -= 0.5 * (v.D[e] * vd[0] + v.D[n2 + e] * vd[1] + v.D[2 * n2 + e] * vd[2]);
+            f -= 0.5 * (v.Q[e] * vq[0] + v.Q[n2 + e] * vq[1] + v.Q[2 * n2 + e] * vq[2] +
+                        v.Q[3 * n2 + e] * vq[3] + v.Q[4 * n2 + e] * vq[4] +
+                        v.Q[5 * n2 + e] * vq[5]);
+          }
+          fv[u] = f;
+        }
+#pragma unroll
+        for (int u = 0; u < FB; ++u) {
+          const int idx = base + u * NT;
+          if (idx < tot) {
+            const int i = idx / ld, j = idx - (idx / ld) * ld;
+            A[aix(i, j)] = fv[u];
+          }
+        }
       }
-      A[aix(i, j)] = f;
     }
     __syncthreads();
     for (int idx = tid; idx < ld * ld; idx += NT) {
@@ -1055,6 +1084,7 @@ __global__ __launch_bounds__(NT) void k_scf(BatchDev b, const int* __restrict__ 
 #pragma unroll
       for (int c = 0; c < 11; ++c) acc[c] = 0.0;
       const int64_t row = (int64_t)i * n;
+#pragma unroll 2
       for (int j = lane; j < n; j += 64) {
         const double p = Pm[aix(i, j)];
         const int64_t e = row + j;

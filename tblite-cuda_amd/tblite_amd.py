@@ -90,12 +90,20 @@ _PROTOS = {
     "tblite_amd_batch_get_charges": (_I, [_P, _I, _DP, _DP, _DP, _DP, _DP, _DP]),
     "tblite_amd_batch_get_energy_parts": (_I, [_P, _I, _DP, _DP]),
     "tblite_amd_last_timings": (_I, [_P, _DP, _I]),
+    "tblite_amd_last_bucket_timings": (_I, [_P, _DP, _IP, _IP, _IP, _I]),
     "tblite_amd_last_profile": (_I, [_P, C.POINTER(C.c_longlong)]),
+    "tblite_amd_new_server": (_P, [_P, _P, _I, _D]),
+    "tblite_amd_delete_server": (None, [C.POINTER(_P)]),
+    "tblite_amd_server_submit": (_I, [_P, _I, _IP, _DP, _D, _I, C.POINTER(C.c_longlong)]),
+    "tblite_amd_server_poll": (_I, [_P, _I, C.POINTER(C.c_longlong), _DP, _IP, _IP, _DP]),
+    "tblite_amd_server_stats": (_I, [_P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), _IP, _IP,
+                                     C.POINTER(C.c_longlong)]),
 }
 
 PROFILE_SLOTS = ["init", "mix", "potential", "fock", "transform", "jacobi", "occupation",
                  "density", "mulliken", "energy", "sweeps", "iterations",
-                 "tridiagonal", "bisection", "inverse_iteration", "back_transform"]
+                 "tridiagonal", "bisection", "inverse_iteration", "back_transform",
+                 "tri_column", "tri_reflector", "tri_symv", "tri_correct", "tri_panel", "clusters"]
 
 EXPORTED = sorted(_PROTOS)
 
@@ -323,3 +331,70 @@ class Engine:
         t = np.zeros(5)
         self._check(self.lib.tblite_amd_last_timings(self.ctx, _dp(t), 5))
         return t
+
+    def bucket_timings(self):
+        """SCF size buckets of the last run: list of dicts (ms, ldmax, nmol, nthreads)."""
+        cap = 16
+        ms = np.zeros(cap)
+        ld, nm, nt = (np.zeros(cap, dtype=np.int32) for _ in range(3))
+        nb = self.lib.tblite_amd_last_bucket_timings(self.ctx, _dp(ms), _ip(ld), _ip(nm), _ip(nt), cap)
+        if nb < 0:
+            raise EngineError(self.error())
+        return [{"ms": float(ms[k]), "ldmax": int(ld[k]), "nmol": int(nm[k]), "nthreads": int(nt[k])}
+                for k in range(min(nb, cap))]
+
+
+class Server:
+    """Continuous-batching request loop on the device of an Engine: submit()
+    queues a molecule and returns its ticket, poll() collects finished results
+    (ticket, energy, iterations, status, latency_ms) and keeps every slot busy."""
+
+    def __init__(self, engine, slots=4, workspace_bytes=16e9):
+        self.eng = engine
+        self.lib = engine.lib
+        self.srv = self.lib.tblite_amd_new_server(engine.ctx, engine.calc, slots, float(workspace_bytes))
+        if not self.srv:
+            raise EngineError(engine.error())
+        self._cap = 4096
+        self._t = np.zeros(self._cap, dtype=np.int64)
+        self._e = np.zeros(self._cap)
+        self._i = np.zeros(self._cap, dtype=np.int32)
+        self._s = np.zeros(self._cap, dtype=np.int32)
+        self._l = np.zeros(self._cap)
+
+    def submit(self, num, xyz, charge=0.0, uhf=0):
+        num = np.ascontiguousarray(num, dtype=np.int32)
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1)
+        t = C.c_longlong(0)
+        self.eng._check(self.lib.tblite_amd_server_submit(self.srv, len(num), _ip(num), _dp(xyz),
+                                                          float(charge), int(uhf), C.byref(t)))
+        return t.value
+
+    def poll(self):
+        k = self.lib.tblite_amd_server_poll(
+            self.srv, self._cap, self._t.ctypes.data_as(C.POINTER(C.c_longlong)), _dp(self._e),
+            _ip(self._i), _ip(self._s), _dp(self._l))
+        if k < 0:
+            raise EngineError(self.eng.error())
+        return [(int(self._t[j]), float(self._e[j]), int(self._i[j]), int(self._s[j]), float(self._l[j]))
+                for j in range(k)]
+
+    def stats(self):
+        sub, comp, bat = C.c_longlong(0), C.c_longlong(0), C.c_longlong(0)
+        q, b = C.c_int(0), C.c_int(0)
+        self.eng._check(self.lib.tblite_amd_server_stats(self.srv, C.byref(sub), C.byref(comp),
+                                                         C.byref(q), C.byref(b), C.byref(bat)))
+        return {"submitted": sub.value, "completed": comp.value, "queued": q.value,
+                "busy_slots": b.value, "batches": bat.value}
+
+    def close(self):
+        if getattr(self, "srv", None):
+            h = C.c_void_p(self.srv)
+            self.lib.tblite_amd_delete_server(C.byref(h))
+            self.srv = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

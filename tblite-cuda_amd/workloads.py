@@ -94,6 +94,7 @@ def water_cluster(n, radius, rng):
 
 
 VALENCE = {6: 4, 7: 3, 8: 2, 16: 2}
+VALENCE_ELECTRONS = {1: 1, 6: 4, 7: 5, 8: 6, 16: 6}
 
 
 def _unit(rng):
@@ -101,15 +102,25 @@ def _unit(rng):
     return v / np.linalg.norm(v)
 
 
-def organic_molecule(nheavy, rng, max_restart=1000):
+def organic_molecule(nheavy, rng, max_restart=1000, h_tries=200, allow_open=False):
     """Tree-grown molecule: attach C/N/O/S (0.70/0.15/0.10/0.05) at 1.50 A to a
     random heavy atom with free valence, reject pairs closer than 1.2 A, then
-    saturate the free valences with H at 1.09 A (same rejection)."""
-    elems = np.array([6, 7, 8, 16])
-    prob = np.array([0.70, 0.15, 0.10, 0.05])
+    saturate the free valences with H at 1.09 A (same rejection).  Positions
+    live in a preallocated array; the random draws are the same as a
+    list-based implementation (same calls, same order)."""
+    elems = [6, 7, 8, 16]
+    cdf = np.cumsum([0.70, 0.15, 0.10, 0.05])
+    cdf /= cdf[-1]
+
+    def pick():
+        # same draw as rng.choice(elems, p=prob): one uniform, right-sided search
+        return elems[int(np.searchsorted(cdf, rng.random(), side="right"))]
+
+    cap = 5 * nheavy + 8
     for _ in range(max_restart):
-        num = [int(rng.choice(elems, p=prob))]
-        pos = [np.zeros(3)]
+        num = [pick()]
+        pos = np.zeros((cap, 3))
+        npos = 1
         free = [VALENCE[num[0]]]
         failed = False
         while len(num) < nheavy:
@@ -117,42 +128,54 @@ def organic_molecule(nheavy, rng, max_restart=1000):
             if not cand:
                 failed = True
                 break
-            z = int(rng.choice(elems, p=prob))
+            z = pick()
             placed = False
             for _try in range(100):
                 host = cand[rng.integers(len(cand))]
                 p = pos[host] + 1.50 * _unit(rng)
-                d = np.linalg.norm(np.array(pos) - p, axis=1)
-                d[host] = 10.0
-                if np.all(d >= 1.2):
+                dv = pos[:npos] - p
+                d2 = np.einsum("ij,ij->i", dv, dv)
+                d2[host] = 100.0
+                if d2.min() >= 1.44:
                     placed = True
                     break
             if not placed:
                 failed = True
                 break
             num.append(z)
-            pos.append(p)
+            pos[npos] = p
+            npos += 1
             free.append(VALENCE[z] - 1)
             free[host] -= 1
         if failed:
             continue
         for i in range(nheavy):
             while free[i] > 0 and not failed:
-                for _try in range(200):
+                for _try in range(h_tries):
                     p = pos[i] + 1.09 * _unit(rng)
-                    d = np.linalg.norm(np.array(pos) - p, axis=1)
-                    d[i] = 10.0
-                    if np.all(d >= 1.2):
+                    dv = pos[:npos] - p
+                    d2 = np.einsum("ij,ij->i", dv, dv)
+                    d2[i] = 100.0
+                    if d2.min() >= 1.44:
                         break
                 else:
+                    if allow_open:
+                        # no room for this hydrogen: the valence stays open
+                        free[i] -= 1
+                        continue
                     failed = True
                     break
                 num.append(1)
-                pos.append(p)
+                pos[npos] = p
+                npos += 1
                 free[i] -= 1
         if failed:
             continue
-        return np.array(num, dtype=np.int32), np.array(pos) * AATOAU
+        if allow_open and sum(VALENCE_ELECTRONS[z] for z in num) % 2 == 1:
+            # closed shell: drop the last hydrogen to make the electron count even
+            num.pop()
+            npos -= 1
+        return np.array(num, dtype=np.int32), pos[:npos].copy() * AATOAU
     raise RuntimeError("organic growth failed")
 
 
@@ -193,9 +216,12 @@ def config4(count=20000):
             n = max(2, int(round(natoms / 3)))
             out.append(water_cluster(n, liquid_radius(n), rng))
         else:
-            # grow until heavy + H reaches natoms (heavy ~ natoms / 2.55)
+            # grow until heavy + H reaches natoms (heavy ~ natoms / 2.55).  Large
+            # trees are crowded: a hydrogen gets 2000 placement attempts, a
+            # valence with no room is left open, and an odd electron count is
+            # made even by dropping the last hydrogen (closed shell)
             nh = max(2, int(round((natoms - 2) / 2.55)))
-            out.append(organic_molecule(nh, rng))
+            out.append(organic_molecule(nh, rng, h_tries=2000, allow_open=True))
     return out
 
 

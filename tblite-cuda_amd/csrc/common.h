@@ -190,6 +190,10 @@ struct BatchDev {
 #define TA_BIG_NP 16
 #define TA_BIG_MAXLD 1024
 
+// rows of per-wave column sums kept in LDS by the tridiagonalisation: one per
+// wave, or one per two waves above ld 768 (LDS budget)
+__host__ __device__ inline int big_npart(int ld) { return ld <= 768 ? TA_BIG_NT / 64 : TA_BIG_NT / 128; }
+
 // scratch per molecule of the global-memory path: inverse-iteration factors
 // (3 ld x NT), back-transformation products (NB x ld), tridiagonalisation
 // panel vectors (2 NP x ld)
@@ -235,7 +239,7 @@ __host__ __device__ inline ScfLds scf_lds_layout(int ld, int natmax, int nshmax,
   // T^-1, T of a reflector block or the per-wave column sums of the
   // tridiagonalisation
   int w1 = ld + nshmax + 10 * natmax, w2 = 11 * ld;
-  int t3 = 2048 + 2 * TA_BIG_NB * TA_BIG_NB, t4 = (TA_BIG_NT / 64) * ld;
+  int t3 = 2048 + 2 * TA_BIG_NB * TA_BIG_NB, t4 = big_npart(ld) * ld;
   int w3 = mats ? 0 : 9 * ld + (t3 > t4 ? t3 : t4);
   L.work = off;
   L.eig = off;

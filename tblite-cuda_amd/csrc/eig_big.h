@@ -158,16 +158,19 @@ __device__ void householder(lds_dbl* x, int lo, int hi, lds_dbl* scal)
 //   y_r += sum_c a' u_c (c <= r),  y_c += a' u_r (c < r)
 // A wave takes RQ rows at a time (rows r0 + q NW) and walks their column
 // blocks of 64 together, so RQ loads per lane are in flight; row sums go to
-// yrow (LDS, one writer per row), column sums to the wave's own LDS row of
-// ycol (NW x ld), combined afterwards.
+// yrow (LDS, one writer per row), column sums to row (wave mod npart) of ycol
+// (npart x ld), combined afterwards.  With npart == NW every wave owns its
+// row (deterministic sums); with fewer rows (large ld, less LDS) waves that
+// share a row add atomically.  ycol must be zero on entry.
 template <int NT, bool UPDATE, int RQ = 4>
-__device__ __noinline__ void trail_pass(gbl_dbl* A, int n, int ld, int lo, const lds_dbl* v, const lds_dbl* w,
-                           const lds_dbl* u, lds_dbl* yrow, lds_dbl* ycol)
+__device__ __noinline__ void trail_pass(gbl_dbl* A, int n, int ld, int lo, const lds_dbl* v,
+                                        const lds_dbl* w, const lds_dbl* u, lds_dbl* yrow,
+                                        lds_dbl* ycol, int npart)
 {
   constexpr int NW = NT / 64;
   const int lane = threadIdx.x & 63, wid = wave_id();
-  lds_dbl* yc = ycol + wid * ld;
-  for (int c = lo + lane; c < n; c += 64) yc[c] = 0.0;
+  lds_dbl* yc = ycol + (wid % npart) * ld;
+  const bool shared_row = npart < NW;
   for (int r0 = lo + wid; r0 < n; r0 += RQ * NW) {
     int r[RQ];
     bool h[RQ];
@@ -212,7 +215,12 @@ __device__ __noinline__ void trail_pass(gbl_dbl* A, int n, int ld, int lo, const
         rs[q] += x[q] * uc;
         if (c < r[q]) cs += x[q] * ur[q];
       }
-      if (kc) yc[c] += cs;
+      if (kc) {
+        if (shared_row)
+          __hip_atomic_fetch_add(yc + c, cs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        else
+          yc[c] += cs;
+      }
     }
 #pragma unroll
     for (int q = 0; q < RQ; ++q) {
@@ -221,85 +229,6 @@ __device__ __noinline__ void trail_pass(gbl_dbl* A, int n, int ld, int lo, const
     }
   }
   __syncthreads();
-}
-
-template <int NT>
-__device__ void combine_partials(int lo, int n, int ld, const lds_dbl* yrow, const lds_dbl* ycol,
-                                 lds_dbl* y)
-{
-  constexpr int NW = NT / 64;
-  for (int c = lo + threadIdx.x; c < n; c += NT) {
-    double s = yrow[c];
-#pragma unroll
-    for (int w = 0; w < NW; ++w) s += ycol[w * ld + c];
-    y[c] = s;
-  }
-  __syncthreads();
-}
-
-// ------------------------------------------------------ tridiagonalisation
-// A (global, lower triangle) -> d[0..n), e[0..n-1) (T[k+1][k] = e[k]);
-// reflector k (k = 0..n-3) stored as v_k[r] = A[r][k], r >= k+1 (v_k[k+1] = 1),
-// tau[k].  Work vectors v, w, vn, y, yrow: ld doubles of LDS each; part:
-// NW * ld doubles of LDS.  Requires n >= 3.
-template <int NT>
-__device__ __noinline__ void tridiagonalize(gbl_dbl* A, int n, int ld, lds_dbl* d, lds_dbl* e,
-                                           lds_dbl* tau, lds_dbl* v, lds_dbl* w, lds_dbl* vn,
-                                           lds_dbl* y, lds_dbl* yrow, lds_dbl* part, lds_dbl* red,
-                                           lds_dbl* scal)
-{
-  const int tid = threadIdx.x;
-  // column 0
-  if (tid == 0) d[0] = A[0];
-  for (int r = 1 + tid; r < n; r += NT) v[r] = A[(int64_t)r * ld];
-  __syncthreads();
-  householder<NT>(v, 1, n, scal);
-  if (tid == 0) {
-    e[0] = scal[0];
-    tau[0] = scal[1];
-  }
-  for (int r = 1 + tid; r < n; r += NT) A[(int64_t)r * ld] = v[r];
-  // y = B_0 v_0, B_0 = A[1:, 1:]
-  trail_pass<NT, false>(A, n, ld, 1, v, v, v, yrow, part);
-  combine_partials<NT>(1, n, ld, yrow, part, y);
-
-  for (int k = 0; k <= n - 3; ++k) {
-    const int lo = k + 1;
-    const double tk = tau[k];
-    // w = tau y - (tau/2)(tau y . v) v
-    double s = 0.0;
-    for (int r = lo + tid; r < n; r += NT) s += y[r] * v[r];
-    s = block_sum(s, red);
-    const double K = 0.5 * tk * tk * s;
-    for (int r = lo + tid; r < n; r += NT) w[r] = tk * y[r] - K * v[r];
-    __syncthreads();
-    // updated column k+1 of the trailing matrix
-    const int c1 = k + 1;
-    if (tid == 0) d[c1] = A[(int64_t)c1 * ld + c1] - 2.0 * v[c1] * w[c1];
-    for (int r = c1 + 1 + tid; r < n; r += NT)
-      vn[r] = A[(int64_t)r * ld + c1] - v[r] * w[c1] - w[r] * v[c1];
-    __syncthreads();
-    if (k == n - 3) {
-      // last 2 x 2 block: one off-diagonal element, last diagonal element
-      if (tid == 0) {
-        e[n - 2] = vn[n - 1];
-        d[n - 1] = A[(int64_t)(n - 1) * ld + (n - 1)] - 2.0 * v[n - 1] * w[n - 1];
-      }
-      __syncthreads();
-      break;
-    }
-    householder<NT>(vn, c1 + 1, n, scal);
-    if (tid == 0) {
-      e[c1] = scal[0];
-      tau[c1] = scal[1];
-    }
-    for (int r = c1 + 1 + tid; r < n; r += NT) A[(int64_t)r * ld + c1] = vn[r];
-    // rows/cols >= k+2: rank-2 update fused with y = B_{k+1} v_{k+1}
-    trail_pass<NT, true>(A, n, ld, c1 + 1, v, w, vn, yrow, part);
-    combine_partials<NT>(c1 + 1, n, ld, yrow, part, y);
-    for (int r = c1 + 1 + tid; r < n; r += NT) v[r] = vn[r];
-    __syncthreads();
-  }
 }
 
 // --------------------------------------------- blocked tridiagonalisation
@@ -313,22 +242,26 @@ __device__ __noinline__ void tridiagonalize(gbl_dbl* A, int n, int ld, lds_dbl* 
 //   w = tau y - (tau^2/2)(y.v) v.
 // After the panel the trailing triangle takes the rank-2 NP update
 // A -= V W^T + W V^T on the matrix cores.  Per column this reads the trailing
-// triangle once (the unblocked scheme reads and writes it).  vt, wt: NP x ld
-// doubles of global scratch (reflectors and w vectors, one per row); dots:
-// 2 NP doubles of LDS; part: NW x ld doubles of LDS (also the GEMM tile
-// region, free between passes).  Requires n >= 3.
+// triangle once (an unblocked two-sided update reads and writes it).  vt, wt:
+// NP x ld doubles of global scratch (reflectors and w vectors, one per row);
+// dots: 2 NP doubles of LDS; part: npart x ld doubles of LDS (also the GEMM
+// tile region, free between passes).  Result: d[0..n), e[0..n-1)
+// (T[k+1][k] = e[k]); reflector k (k = 0..n-3) stored as v_k[r] = A[r][k],
+// r >= k+1 (v_k[k+1] = 1), tau[k].  Requires n >= 3.
 template <int NT, int NP>
 __device__ __noinline__ void tridiagonalize_blocked(gbl_dbl* A, int n, int ld, lds_dbl* d,
                                                    lds_dbl* e, lds_dbl* tau, lds_dbl* v,
                                                    lds_dbl* x, lds_dbl* y, lds_dbl* yrow,
                                                    lds_dbl* dots, lds_dbl* part, gbl_dbl* vt,
-                                                   gbl_dbl* wt, lds_dbl* red, lds_dbl* scal,
-                                                   lds_dbl* tprof)
+                                                   gbl_dbl* wt, int npart, lds_dbl* red,
+                                                   lds_dbl* scal, lds_dbl* tprof)
 {
   constexpr int NW = NT / 64;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wid = wave_id();
   const int nr = n - 2;  // reflectors 0 .. n-3
+  for (int r = tid; r < ld; r += NT)
+    for (int w = 0; w < npart; ++w) part[w * ld + r] = 0.0;
   // optional step clocks (thread 0, wall-clock ticks added to tprof[0..4])
   const bool clk_on = tprof != nullptr && tid == 0;
   long long clk_last = clk_on ? wall_clock64() : 0;
@@ -378,15 +311,17 @@ __device__ __noinline__ void tridiagonalize_blocked(gbl_dbl* A, int n, int ld, l
         sacc = wave_sum(sacc);
         if (lane == 0) dots[(pq < i ? 0 : NP) + q] = sacc;
       }
-      trail_pass<NT, false, 4>(A, n, ld, j + 1, v, v, v, yrow, part);
+      trail_pass<NT, false, 4>(A, n, ld, j + 1, v, v, v, yrow, part, npart);
       lap(2);
       // (d) y = tau (A_trail v - V (W^T v) - W (V^T v)), combined with the
       // per-wave partial sums in the same pass
       double sv = 0.0;
       for (int r = j + 1 + tid; r < n; r += NT) {
         double yr = yrow[r];
-#pragma unroll
-        for (int w = 0; w < NW; ++w) yr += part[w * ld + r];
+        for (int w = 0; w < npart; ++w) {
+          yr += part[w * ld + r];
+          part[w * ld + r] = 0.0;  // ready for the next column
+        }
         for (int q = 0; q < i; ++q)
           yr -= vt[(int64_t)q * ld + r] * dots[q] + wt[(int64_t)q * ld + r] * dots[NP + q];
         yr *= tj;
@@ -440,6 +375,11 @@ __device__ __noinline__ void tridiagonalize_blocked(gbl_dbl* A, int n, int ld, l
           if (ii >= jj) A[(int64_t)(r0 + ii) * ld + r0 + jj] -= val;
         },
         part);
+    // the product used the partial-sum rows as its tile: clear the columns
+    // the next panel accumulates into
+    for (int r = r0 + tid; r < n; r += NT)
+      for (int w = 0; w < npart; ++w) part[w * ld + r] = 0.0;
+    __syncthreads();
     lap(4);
   }
 }
@@ -473,11 +413,17 @@ __device__ __forceinline__ void sturm3(const lds_dbl* d, const lds_dbl* e2, int 
   c3 = a3;
 }
 
-// All eigenvalues of T (d, e) in ascending order into lam; e2 = e^2 scratch
-// (LDS).  Returns |T| (Gershgorin bound) through scal[0].
+// All eigenvalues of T (d, e) in ascending order into lam, to an absolute
+// accuracy of 1e-12 |T| (far below what the SCF observables resolve); e2 =
+// e^2 scratch (LDS).  Warm start: with *warm > 0 each level is first
+// bracketed in prev[i] -/+ *warm (the previous SCF step's levels), checked
+// by Sturm counts, and falls back to the Gershgorin interval when the check
+// fails.  On return *warm holds the bracket for the next call (twice the
+// largest level shift seen, plus a margin) and scal[0] = |T| (Gershgorin).
 template <int NT>
 __device__ __noinline__ void bisect_all(const lds_dbl* d, const lds_dbl* e, lds_dbl* e2, int n,
-                                       lds_dbl* lam, lds_dbl* red, lds_dbl* scal)
+                                       lds_dbl* lam, const lds_dbl* prev, lds_dbl* warm,
+                                       lds_dbl* red, lds_dbl* scal)
 {
   const int tid = threadIdx.x;
   double gl = INFINITY, gu = -INFINITY, emax = 0.0;
@@ -493,13 +439,24 @@ __device__ __noinline__ void bisect_all(const lds_dbl* d, const lds_dbl* e, lds_
   gl = -block_max(-gl, red);
   gu = block_max(gu, red);
   emax = block_max(emax, red);
+  const double wd = warm[0];
   const double tnorm = fmax(fabs(gl), fabs(gu));
   const double pivmin = 2.2250738585072014e-308 * fmax(1.0, emax);
-  const double tol = 4.0 * EPS * tnorm + pivmin;
+  const double tol = 1.0e-12 * tnorm + 4.0 * EPS * tnorm + pivmin;
   gl -= 2.0 * tol;
   gu += 2.0 * tol;
+  double shift = 0.0;
   for (int i = tid; i < n; i += NT) {
     double lo = gl, hi = gu;
+    if (wd > 0.0) {
+      const double a = prev[i] - wd, b = prev[i] + wd;
+      int ca, cb, cc;
+      sturm3(d, e2, n, a, b, b, pivmin, ca, cb, cc);
+      if (ca <= i && cb > i) {
+        lo = a;
+        hi = b;
+      }
+    }
     for (int it = 0; it < 64 && hi - lo > tol; ++it) {
       const double h = hi - lo;
       const double x1 = lo + 0.25 * h, x2 = lo + 0.5 * h, x3 = lo + 0.75 * h;
@@ -517,9 +474,15 @@ __device__ __noinline__ void bisect_all(const lds_dbl* d, const lds_dbl* e, lds_
         lo = x3;
       }
     }
-    lam[i] = 0.5 * (lo + hi);
+    const double l = 0.5 * (lo + hi);
+    if (wd > 0.0) shift = fmax(shift, fabs(l - prev[i]));
+    lam[i] = l;
   }
-  if (tid == 0) scal[0] = tnorm;
+  shift = block_max(shift, red);
+  if (tid == 0) {
+    scal[0] = tnorm;
+    warm[0] = wd > 0.0 ? 2.0 * shift + 1.0e-10 * tnorm : 1.0e-2 * tnorm;
+  }
   __syncthreads();
 }
 

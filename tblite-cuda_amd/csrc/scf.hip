@@ -484,7 +484,7 @@ template <int NT>
 __device__ __noinline__ void big_eigensolve(double* A_, const double* X_, double* Y_, double* CT_,
                                             double* scr_, int n, int ld, double* er_, double* ev_,
                                             double* red_, double* scal_, int* runs_,
-                                            PhaseClock& clk)
+                                            const double* prev_, double* warm_, PhaseClock& clk)
 {
   gbl_dbl* A = TA_GBL(A_);
   const gbl_dbl* X = TA_GBL(X_);
@@ -528,14 +528,14 @@ __device__ __noinline__ void big_eigensolve(double* A_, const double* X_, double
   if (threadIdx.x == 0)
     for (int q = 0; q < 5; ++q) red[16 + q] = 0.0;
   big::tridiagonalize_blocked<NT, TA_BIG_NP>(A, n, ld, d, e, tau, vv, vn, yy, yrow, ww, tmp, pvt,
-                                             pvt + (int64_t)TA_BIG_NP * ld, red, scal,
-                                             clk.on ? red + 16 : nullptr);
+                                             pvt + (int64_t)TA_BIG_NP * ld, big_npart(ld), red,
+                                             scal, clk.on ? red + 16 : nullptr);
   if (clk.on) {
     for (int q = 0; q < 5; ++q) clk.acc[TA_PROF_TRI_COLUMN + q] += (long long)red[16 + q];
     clk.last = wall_clock64();
   }
   clk.lap(TA_PROF_TRIDIAG);
-  big::bisect_all<NT>(d, e, e2, n, ev, red, scal);
+  big::bisect_all<NT>(d, e, e2, n, ev, TA_LDS(prev_), TA_LDS(warm_), red, scal);
   const double tnorm = scal[0];
   __syncthreads();
   clk.lap(TA_PROF_BISECT);
@@ -736,6 +736,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? 1
   if (!orthonormalizer<NT>(A, C, n, ld, scal)) status |= TA_STATUS_CHOLESKY;
   clk.lap(TA_PROF_INIT);
 
+  if (tid == 0) red[24] = 0.0;  // bisection warm-start bracket (global-memory path)
+  __syncthreads();
   const double econv = 1.0e-6 * b.accuracy, pconv = 2.0e-5 * b.accuracy;
   const double kt = b.kt;
   double esum = 0.0, elast = 0.0;
@@ -1039,7 +1041,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? 1
       }
       __syncthreads();
     } else {
-      big_eigensolve<NT>(A, C, Y, CT, scr, n, ld, smem + L.eig, ev, red, scal, rank, clk);
+      // levels of the previous step (emo_s) warm-start the bisection
+      big_eigensolve<NT>(A, C, Y, CT, scr, n, ld, smem + L.eig, ev, red, scal, rank, emo_s,
+                         red + 24, clk);
       for (int k = tid; k < n; k += NT) {
         rank[k] = k;
         inv[k] = k;

@@ -46,8 +46,12 @@ constexpr double ORTOL = 1.0e-5;  // relative level gap below which vectors are 
 // issued before the current slice is multiplied.
 typedef double ta_double4 __attribute__((ext_vector_type(4)));
 
-template <int NT, bool AK, bool BJ, class LA, class LB, class KR, class SK, class ST>
-__device__ __noinline__ void gemm(int M, int N, LA la, LB lb, KR kr, SK skip, ST st, lds_dbl* tile)
+// One 64 x 64 output tile (rows i0.., columns j0..) over k in [k0, k1): the
+// body shared by the single-workgroup product below and the grid-wide one of
+// the multi-workgroup path.  All NT = 512 threads of the workgroup take part.
+template <int NT, bool AK, bool BJ, class LA, class LB, class ST>
+__device__ __noinline__ void gemm_tile(int M, int N, int i0, int j0, int k0, int k1, LA la,
+                                          LB lb, ST st, lds_dbl* tile)
 {
   static_assert(NT == 512, "gemm tiling assumes 512 threads");
   lds_dbl* as = tile;         // [16][64]: as[k * 64 + i]
@@ -56,7 +60,6 @@ __device__ __noinline__ void gemm(int M, int N, LA la, LB lb, KR kr, SK skip, ST
   const int lane = tid & 63, w = wave_id();
   const int bi = w >> 1, bj = (w & 1) * 2;
   const int lr = lane & 15, lk = lane >> 4;
-  const int mt = (M + 63) >> 6, nt = (N + 63) >> 6;
   // staging coordinates of this thread's two elements of each operand slice
   int ai[2], ak[2], bjj[2], bk[2];
 #pragma unroll
@@ -67,52 +70,58 @@ __device__ __noinline__ void gemm(int M, int N, LA la, LB lb, KR kr, SK skip, ST
     bjj[r] = BJ ? idx & 63 : idx >> 4;
     bk[r] = BJ ? idx >> 6 : idx & 15;
   }
+  ta_double4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+  double ra[2], rb[2];
+  auto fetch = [&](int kb) {
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const int gi = i0 + ai[r], gk = kb + ak[r];
+      ra[r] = (gi < M && gk < k1) ? la(gi, gk) : 0.0;
+      const int gj = j0 + bjj[r], gk2 = kb + bk[r];
+      rb[r] = (gj < N && gk2 < k1) ? lb(gk2, gj) : 0.0;
+    }
+  };
+  if (k0 < k1) fetch(k0);
+  for (int kb = k0; kb < k1; kb += 16) {
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      as[ak[r] * 64 + ai[r]] = ra[r];
+      bs[bk[r] * 64 + bjj[r]] = rb[r];
+    }
+    __syncthreads();
+    if (kb + 16 < k1) fetch(kb + 16);
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      const int kr_ = (4 * kk + lk) * 64;
+      const double a = as[kr_ + 16 * bi + lr];
+      const double b0 = bs[kr_ + 16 * bj + lr];
+      const double b1 = bs[kr_ + 16 * bj + 16 + lr];
+      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, acc1, 0, 0, 0);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int gi = i0 + 16 * bi + 4 * i + lk;
+    const int gj = j0 + 16 * bj + lr;
+    if (gi < M) {
+      if (gj < N) st(gi, gj, acc0[i]);
+      if (gj + 16 < N) st(gi, gj + 16, acc1[i]);
+    }
+  }
+}
+
+template <int NT, bool AK, bool BJ, class LA, class LB, class KR, class SK, class ST>
+__device__ __noinline__ void gemm(int M, int N, LA la, LB lb, KR kr, SK skip, ST st, lds_dbl* tile)
+{
+  const int mt = (M + 63) >> 6, nt = (N + 63) >> 6;
   for (int ti = 0; ti < mt; ++ti)
     for (int tj = 0; tj < nt; ++tj) {
       if (skip(ti, tj)) continue;
       int k0, k1;
       kr(ti, tj, &k0, &k1);
-      const int i0 = ti * 64, j0 = tj * 64;
-      ta_double4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
-      double ra[2], rb[2];
-      auto fetch = [&](int kb) {
-#pragma unroll
-        for (int r = 0; r < 2; ++r) {
-          const int gi = i0 + ai[r], gk = kb + ak[r];
-          ra[r] = (gi < M && gk < k1) ? la(gi, gk) : 0.0;
-          const int gj = j0 + bjj[r], gk2 = kb + bk[r];
-          rb[r] = (gj < N && gk2 < k1) ? lb(gk2, gj) : 0.0;
-        }
-      };
-      if (k0 < k1) fetch(k0);
-      for (int kb = k0; kb < k1; kb += 16) {
-#pragma unroll
-        for (int r = 0; r < 2; ++r) {
-          as[ak[r] * 64 + ai[r]] = ra[r];
-          bs[bk[r] * 64 + bjj[r]] = rb[r];
-        }
-        __syncthreads();
-        if (kb + 16 < k1) fetch(kb + 16);
-#pragma unroll
-        for (int kk = 0; kk < 4; ++kk) {
-          const int kr_ = (4 * kk + lk) * 64;
-          const double a = as[kr_ + 16 * bi + lr];
-          const double b0 = bs[kr_ + 16 * bj + lr];
-          const double b1 = bs[kr_ + 16 * bj + 16 + lr];
-          acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, acc0, 0, 0, 0);
-          acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, acc1, 0, 0, 0);
-        }
-        __syncthreads();
-      }
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int gi = i0 + 16 * bi + 4 * i + lk;
-        const int gj = j0 + 16 * bj + lr;
-        if (gi < M) {
-          if (gj < N) st(gi, gj, acc0[i]);
-          if (gj + 16 < N) st(gi, gj + 16, acc1[i]);
-        }
-      }
+      gemm_tile<NT, AK, BJ>(M, N, ti * 64, tj * 64, k0, k1, la, lb, st, tile);
     }
   __syncthreads();
 }
@@ -387,7 +396,8 @@ __device__ __noinline__ void tridiagonalize_blocked(gbl_dbl* A, int n, int ld, l
 // ---------------------------------------------------------- eigenvalues
 // Number of eigenvalues below each of three points (Sturm sequences run in
 // lockstep for instruction-level parallelism).
-__device__ __forceinline__ void sturm3(const lds_dbl* d, const lds_dbl* e2, int n, double x1,
+template <class PD, class PE>
+__device__ __forceinline__ void sturm3(PD d, PE e2, int n, double x1,
                                        double x2, double x3, double pivmin, int& c1, int& c2,
                                        int& c3)
 {

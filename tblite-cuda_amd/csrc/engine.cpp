@@ -18,6 +18,7 @@
 #include "../params/gfn2_default.h"
 #include "../params/sto_ng_table.h"
 #include "common.h"
+#include "scf_grid.h"
 #include "tblite_amd.h"
 
 // kernels (setup.hip / scf.hip)
@@ -35,6 +36,7 @@ constexpr int kIntegralChunk = 128;
 constexpr int kLdsMaxLd = 96;
 constexpr int kNumStreams = 4;
 constexpr int kMaxBuckets = 8;
+constexpr int kGridMaxLd = 12288;  // largest padded AO dimension of the multi-workgroup path
 
 // ------------------------------------------------------------ calculator
 struct Calculator {
@@ -197,6 +199,7 @@ struct Plan {
   std::vector<int> sh_at, sh_ish, sh_iao;
   std::vector<int> ao_at, ao_sh;
   std::vector<int> work_mol, work_kind, work_start;
+  std::vector<int> grid_mols;  // molecules run by the multi-workgroup SCF path
   struct Bucket {
     int nt, maxt;
     bool lds;
@@ -239,6 +242,10 @@ struct tblite_amd_context_s {
   tblite_amd_calculator prepared = nullptr;
   double prepared_acc = 1.0;
   bool profile = false;
+  // padded AO dimension above which a molecule runs on the multi-workgroup
+  // path (TBLITE_AMD_GRID_MIN_LD may lower it, e.g. to test that path on
+  // molecules the oracle handles quickly)
+  int grid_min_ld = TA_BIG_MAXLD;
 };
 
 struct tblite_amd_structure_s {
@@ -371,12 +378,18 @@ int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* 
     pl.mol_mixa_off[m] = (int64_t)m * pl.mem * pl.mem;
     const int ld = (nao + 3) & ~3;
     const int ndim = nsh + 9 * nat;
-    if (ld > TA_BIG_MAXLD) {
+    if (ld > kGridMaxLd) {
       set_error(ctx, "molecule %d has %d atomic orbitals; the engine serves up to %d", m, nao,
-                TA_BIG_MAXLD);
+                kGridMaxLd);
       return TBLITE_AMD_ERR_TOO_LARGE;
     }
-    if (ld > kLdsMaxLd) {
+    if (ld > ctx->grid_min_ld) {
+      pl.grid_mols.push_back(m);
+      pl.mol_big_off[m] = big;
+      big += (int64_t)ld * ld;
+      pl.mol_scr_off[m] = scr;
+      scr += grid_scratch_doubles(nao, ld, nat, nsh);
+    } else if (ld > kLdsMaxLd) {
       pl.mol_big_off[m] = big;
       big += (int64_t)ld * ld;
       pl.mol_scr_off[m] = scr;
@@ -434,7 +447,7 @@ int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* 
     bk.nshmax = 0;
     for (int m : order) {
       const int ld = (pl.mol_nao[m] + 3) & ~3;
-      if (ld > lo && ld <= hi) {
+      if (ld > lo && ld <= hi && ld <= ctx->grid_min_ld) {
         bk.mols.push_back(m);
         bk.ldmax = std::max(bk.ldmax, ld);
         bk.natmax = std::max(bk.natmax, pl.mol_nat[m]);
@@ -667,6 +680,114 @@ int launch_scf(tblite_amd_context ctx, hipStream_t st, const BatchDev& b, const 
 
 int finish_batch(tblite_amd_context ctx);
 
+// Device view of grid-path molecule m of the planned batch.
+int run_grid_molecule(tblite_amd_context ctx, Calculator& c, int m)
+{
+  const Plan& pl = ctx->plan;
+  const BatchDev& b = ctx->batch.b;
+  GridMol g;
+  std::memset(&g, 0, sizeof(g));
+  const int n = pl.mol_nao[m], nat = pl.mol_nat[m], nsh = pl.mol_nsh[m];
+  const int ld = (n + 3) & ~3;
+  const int a0 = pl.mol_atom_off[m], s0 = pl.mol_sh_off[m], o0 = pl.mol_ao_off[m];
+  const int64_t moff = pl.mol_mat_off[m], poff = pl.mol_pair_off[m];
+  const int mem = b.mix_memory;
+  g.n = n;
+  g.ld = ld;
+  g.nat = nat;
+  g.nsh = nsh;
+  g.ndim = nsh + 9 * nat;
+  g.mem = mem;
+  g.nbmax = (n + 63) / 64;
+  g.kt = b.kt;
+  g.damping = b.damping;
+  g.econv = 1.0e-6 * b.accuracy;
+  g.pconv = 2.0e-5 * b.accuracy;
+  g.charge = pl.mol_charge[m];
+  g.uhf = pl.mol_uhf[m];
+  g.par = b.par;
+  g.el = b.at_el + a0;
+  g.sh_at = b.sh_at + s0;
+  g.sh_ish = b.sh_ish + s0;
+  g.sh_iao = b.sh_iao + s0;
+  g.ao_at = b.ao_at + o0;
+  g.ao_sh = b.ao_sh + o0;
+  g.at_ish = b.at_ish + a0;
+  g.at_iao = b.at_iao + a0;
+  g.S = b.S + moff;
+  g.H = b.H + moff;
+  g.D = b.D + 3 * moff;
+  g.Q = b.Q + 6 * moff;
+  g.gam = b.gam + pl.mol_gam_off[m];
+  g.sd = b.aes_sd + 3 * poff;
+  g.dd = b.aes_dd + 9 * poff;
+  g.sq = b.aes_sq + 6 * poff;
+  g.n0sh = b.n0sh + s0;
+  g.erep_at = b.erep + a0;
+  g.qsh = b.qsh + s0;
+  g.qat = b.qat + a0;
+  g.dpat = b.dpat + 3 * (int64_t)a0;
+  g.qpat = b.qpat + 6 * (int64_t)a0;
+  g.qin = b.mix_qin + pl.mol_mix_off[m];
+  g.dq = b.mix_dq + pl.mol_mix_off[m];
+  g.dqlast = b.mix_dqlast + pl.mol_mix_off[m];
+  g.qlast = b.mix_qlast + pl.mol_mix_off[m];
+  g.dfm = b.mix_df + pl.mol_mixmem_off[m];
+  g.um = b.mix_u + pl.mol_mixmem_off[m];
+  g.amix = b.mix_a + (int64_t)m * mem * mem;
+  g.beta = b.mix_beta + (int64_t)m * mem * (mem + 1);
+  g.omega = b.mix_omega + (int64_t)m * mem;
+  g.cvec = b.mix_c + (int64_t)m * mem;
+  const int64_t boff = pl.mol_big_off[m];
+  g.A = b.big_a + boff;
+  g.X = b.big_c + boff;
+  g.Y = b.big_t + boff;
+  g.CT = b.big_ct + boff;
+  double* p = b.big_s + pl.mol_scr_off[m];
+  auto take = [&p](int64_t cnt) {
+    double* q = p;
+    p += cnt;
+    return q;
+  };
+  const int64_t nbm = g.nbmax;
+  g.scr = take(3 * (int64_t)n * n);
+  g.vt = take((int64_t)TA_BIG_NP * ld);
+  g.wt = take((int64_t)TA_BIG_NP * ld);
+  g.wbt = take((int64_t)TA_BIG_NB * ld);
+  g.part = take(nbm * nbm * 64);
+  g.part11 = take(11 * (int64_t)n);
+  g.epart = take(nsh + nat);
+  g.d = take(ld);
+  g.e = take(ld);
+  g.tau = take(ld);
+  g.e2 = take(ld);
+  g.lam = take(ld);
+  g.prevlam = take(ld);
+  g.v = take(ld);
+  g.y = take(ld);
+  g.focc = take(ld);
+  g.fs = take(ld);
+  g.vsh = take(nsh);
+  g.vat = take(nat);
+  g.vdp = take(3 * (int64_t)nat);
+  g.vqp = take(6 * (int64_t)nat);
+  g.dots = take(2 * TA_BIG_NP);
+  g.tm = take(TA_BIG_NB * TA_BIG_NB);
+  g.sc = take(GS_COUNT);
+  g.ic = reinterpret_cast<int*>(p);
+  g.runs = g.ic + GI_COUNT;
+  GridHostStats hs;
+  int rc = grid_scf(g, ctx->stream, c.max_iter, b.energy + m, b.eelec + m, b.erep_tot + m,
+                    b.iterations + m, b.status + m, b.emo + o0, b.focc + o0,
+                    b.coeff ? b.coeff + moff : nullptr, &hs);
+  if (rc) {
+    set_error(ctx, "multi-workgroup SCF of molecule %d failed: %s", m,
+              hipGetErrorString(hipGetLastError()));
+    return TBLITE_AMD_ERR_HIP;
+  }
+  return TBLITE_AMD_OK;
+}
+
 // Run the whole pipeline for the planned batch.  xyz (host) is uploaded
 // first when given; with xyz == nullptr the coordinates already resident in
 // the batch buffer are used.  With wait == false the work is only enqueued:
@@ -729,6 +850,12 @@ int run_batch(tblite_amd_context ctx, Calculator& c, const double* xyz, double a
       rc = launch_scf<512, 2, 3, 11, true>(ctx, as, b, mols, cnt, bk.ldmax, bk.natmax, bk.nshmax);
     if (rc) return rc;
     HIP_TRY(ctx, hipEventRecord(ctx->bev[2 * k + 1], as));
+  }
+  // molecules above the single-workgroup size: one at a time across the GPU,
+  // on the main stream while the buckets run on the auxiliary streams
+  for (int m : pl.grid_mols) {
+    int rc = run_grid_molecule(ctx, c, m);
+    if (rc) return rc;
   }
   for (int s = 0; s < kNumStreams; ++s) {
     HIP_TRY(ctx, hipEventRecord(ctx->aux_done[s], ctx->aux[s]));
@@ -855,6 +982,8 @@ tblite_amd_context tblite_amd_new_context(int device)
   ctx->lds_limit = (size_t)lds;
   const char* pe = std::getenv("TBLITE_AMD_PROFILE");
   ctx->profile = pe && pe[0] == '1';
+  const char* pg = std::getenv("TBLITE_AMD_GRID_MIN_LD");
+  if (pg) ctx->grid_min_ld = std::max(16, std::min(TA_BIG_MAXLD, std::atoi(pg)));
   if (!ok) {
     tblite_amd_delete_context(&ctx);
     return nullptr;

@@ -31,6 +31,7 @@
 #include "common.h"
 #include "device_util.h"
 #include "eig_big.h"
+#include "fermi.h"
 
 #define JACOBI_MAX_SWEEPS 40
 
@@ -426,54 +427,6 @@ __device__ bool jacobi(double* A, double* CT, int ld, const JRot& R, const JPlan
   return ok;
 }
 
-// ----------------------------------------------------------------- Fermi
-// get_fermi_filling (src/wavefunction/fermi.cu:100-124) on sorted levels,
-// run by one wave; occ receives this spin channel's occupations, returns
-// the electronic entropy term kT * sum(f ln f + (1-f) ln(1-f)).
-__device__ double fermi_wave(double nel, double kt, int n, const double* emo, double* occ)
-{
-  const int lane = threadIdx.x & 63;
-  const double thr = 1.4901161193847656e-08; // sqrt(epsilon(1.0_wp))
-  int homo = (int)floor(nel);
-  const double frac = fmod(nel, 1.0);
-  for (int i = lane; i < n; i += 64) occ[i] = (i < homo) ? 1.0 : ((i == homo) ? frac : 0.0);
-  int homo_idx = homo;
-  if (frac > 0.5) homo_idx += 1;
-  if (homo_idx > 0) {
-    const double occt = (double)homo_idx;
-    const int lo = homo_idx - 1 < 0 ? 0 : homo_idx - 1;
-    const int hi = homo_idx < n - 1 ? homo_idx : n - 1;
-    double ef = 0.5 * (emo[lo] + emo[hi]);
-    for (int cyc = 0; cyc < 200; ++cyc) {
-      double tn = 0.0, td = 0.0;
-      for (int i = lane; i < n; i += 64) {
-        const double x = (emo[i] - ef) / kt;
-        double f = 0.0, dfn = 0.0;
-        if (x < 50.0) {
-          const double ex = exp(x);
-          f = 1.0 / (ex + 1.0);
-          dfn = ex / (kt * (ex + 1.0) * (ex + 1.0));
-        }
-        occ[i] = f;
-        tn += f;
-        td += dfn;
-      }
-      tn = wave_sum(tn);
-      td = wave_sum(td);
-      ef += (occt - tn) / td;
-      if (fabs(occt - tn) <= thr) break;
-    }
-  }
-  double s = 0.0;
-  for (int i = lane; i < n; i += 64) {
-    const double f = occ[i];
-    if (f > thr && 1.0 - f > thr) s += f * log(f) + (1.0 - f) * log(1.0 - f);
-  }
-  return wave_sum(s) * kt;
-}
-
-// Phase clock: thread 0 accumulates wall-clock ticks per phase when the
-// batch carries a profile buffer.
 
 // ------------------------------------------- global-memory eigensolve
 // F (in A, full) -> eigenvalues ev (ascending) and MO rows CT = Z^T X of the

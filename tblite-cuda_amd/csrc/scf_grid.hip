@@ -1,0 +1,1165 @@
+// Multi-workgroup SCF for one molecule above the single-workgroup size
+// (padded AO dimension > TA_BIG_MAXLD, e.g. a (H2O)700 cluster with 4200
+// AOs).  The iteration of next_scf (src/scf/iterators.cu:317-424) runs as a
+// host loop over kernels, each phase spread over the whole GPU:
+//
+//   mixing        modified Broyden (one workgroup; O(ndim x memory))
+//   potential     one workgroup per shell (ES2 + ES3) and per atom (AES2)
+//   Fock          grid-stride over the n^2 elements, then symmetrised
+//   reduction     A = X F X on the matrix cores, X = S^-1/2 (Loewdin; its
+//                 S-orthonormal columns serve exactly as the Cholesky basis)
+//   tridiagonal   blocked Householder: per column one workgroup forms the
+//                 reflector, the symmetric product is tiled over 64 x 64
+//                 blocks of the trailing triangle (one workgroup each), one
+//                 workgroup folds the partial sums and the panel corrections;
+//                 after NP columns a rank-2 NP matrix-core update
+//   eigenvalues   bisection, one thread per level, warm-started
+//   eigenvectors  inverse iteration, one thread per level, clusters
+//                 re-orthogonalised by one wave each
+//   back-transf.  compact WY blocks, three matrix-core products per block
+//   occupation    Fermi smearing (one wave), density on the matrix cores
+//   Mulliken      one wave per AO row, then one workgroup for shells / atoms
+//   energy        one wave per shell / atom, then one workgroup
+// The host reads one status word per iteration.
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "common.h"
+#include "device_util.h"
+#include "eig_big.h"
+#include "fermi.h"
+#include "scf_grid.h"
+
+namespace {
+
+constexpr int GT = 512;  // threads of the one-workgroup kernels and GEMM tiles
+
+// ---------------------------------------------------------------- GEMM
+template <bool AK, bool BJ, class LA, class LB, class KR, class SK, class ST>
+__global__ __launch_bounds__(512) void k_g_gemm(int M, int N, LA la, LB lb, KR kr, SK skip, ST st)
+{
+  __shared__ double tile_[2048];
+  lds_dbl* tile = TA_LDS(tile_);
+  const int mt = (M + 63) >> 6, nt = (N + 63) >> 6;
+  for (int t = blockIdx.x; t < mt * nt; t += gridDim.x) {
+    const int ti = t / nt, tj = t - (t / nt) * nt;
+    if (skip(ti, tj)) continue;
+    int k0, k1;
+    kr(ti, tj, &k0, &k1);
+    big::gemm_tile<512, AK, BJ>(M, N, ti * 64, tj * 64, k0, k1, la, lb, st, tile);
+  }
+}
+
+template <bool AK, bool BJ, class LA, class LB, class KR, class SK, class ST>
+void g_gemm(hipStream_t s, int M, int N, LA la, LB lb, KR kr, SK skip, ST st)
+{
+  const int tiles = ((M + 63) >> 6) * ((N + 63) >> 6);
+  const int grid = std::max(1, std::min(tiles, 2048));
+  hipLaunchKernelGGL((k_g_gemm<AK, BJ, LA, LB, KR, SK, ST>), dim3(grid), dim3(512), 0, s, M, N,
+                     la, lb, kr, skip, st);
+}
+
+struct AllK {
+  int k;
+  __device__ void operator()(int, int, int* a, int* b) const
+  {
+    *a = 0;
+    *b = k;
+  }
+};
+struct NoSkip {
+  __device__ bool operator()(int, int) const { return false; }
+};
+struct UpperSkip {
+  __device__ bool operator()(int ti, int tj) const { return tj > ti; }
+};
+
+// ---------------------------------------------------------------- init
+// zero charges and multipoles; repulsion total, occupation numbers
+// (src/xtb.cu:290-313 text) in the device scalars
+__global__ __launch_bounds__(256) void k_g_init(GridMol g)
+{
+  __shared__ double red_[8];
+  lds_dbl* red = TA_LDS(red_);
+  double nref = 0.0, erep = 0.0;
+  for (int s = threadIdx.x; s < g.nsh; s += blockDim.x) nref += g.n0sh[s];
+  for (int a = threadIdx.x; a < g.nat; a += blockDim.x) erep += g.erep_at[a];
+  nref = block_sum(nref, red);
+  erep = block_sum(erep, red);
+  if (threadIdx.x == 0) {
+    const double nel = nref - g.charge;
+    const int nint_nel = (int)lround(nel);
+    const double nuhf =
+        ((g.uhf % 2) == (nint_nel % 2)) ? (double)g.uhf : (double)(nint_nel % 2);
+    const double diff = fmin(nuhf, nel);
+    g.sc[GS_NALP] = (nel - diff) / 2.0 + diff;
+    g.sc[GS_NBET] = (nel - diff) / 2.0;
+    g.sc[GS_EREP] = erep;
+  }
+  for (int i = threadIdx.x; i < g.nsh; i += blockDim.x) g.qsh[i] = 0.0;
+  for (int i = threadIdx.x; i < g.nat; i += blockDim.x) g.qat[i] = 0.0;
+  for (int i = threadIdx.x; i < 3 * g.nat; i += blockDim.x) g.dpat[i] = 0.0;
+  for (int i = threadIdx.x; i < 6 * g.nat; i += blockDim.x) g.qpat[i] = 0.0;
+  for (int i = threadIdx.x; i < g.ndim; i += blockDim.x) g.qin[i] = 0.0;
+  if (threadIdx.x == 0) {
+    g.sc[GS_WARM] = 0.0;
+    g.sc[GS_ESUM] = 0.0;
+    g.ic[GI_STATUS] = 0;
+    g.ic[GI_CONVERGED] = 0;
+  }
+}
+
+// S (n-strided) -> A (ld-strided, full)
+__global__ void k_g_copy_s(GridMol g)
+{
+  const int64_t tot = (int64_t)g.n * g.n;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < tot;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(idx / g.n), j = (int)(idx - (idx / g.n) * g.n);
+    g.A[(int64_t)i * g.ld + j] = g.S[idx];
+  }
+}
+
+// ---------------------------------------------------------------- potential
+// blocks [0, nsh): ES2 + ES3 shell potential; [nsh, nsh + nat): AES2 atom potentials
+__global__ __launch_bounds__(256) void k_g_potential(GridMol g)
+{
+  __shared__ double red_[8];
+  lds_dbl* red = TA_LDS(red_);
+  const DevParams& P = *g.par;
+  const int nat = g.nat, nsh = g.nsh;
+  const int64_t p2 = (int64_t)nat * nat;
+  if ((int)blockIdx.x < nsh) {
+    const int sh = blockIdx.x;
+    const double* gr = g.gam + (int64_t)sh * nsh;
+    double acc = 0.0;
+    for (int j = threadIdx.x; j < nsh; j += blockDim.x) acc += gr[j] * g.qsh[j];
+    acc = block_sum(acc, red);
+    if (threadIdx.x == 0) {
+      const double q = g.qsh[sh];
+      g.vsh[sh] = acc + q * q * P.el[g.el[g.sh_at[sh]]].hubbard_derivs[g.sh_ish[sh]];
+    }
+    return;
+  }
+  const int a = blockIdx.x - nsh;
+  double va = 0.0, d[3] = {0.0, 0.0, 0.0}, q[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const int64_t row = (int64_t)a * nat;
+  for (int i = threadIdx.x; i < nat; i += blockDim.x) {
+    const int64_t ai = row + i;
+    const double qi = g.qat[i];
+    const double dpi[3] = {g.dpat[3 * i], g.dpat[3 * i + 1], g.dpat[3 * i + 2]};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double sdk = g.sd[k * p2 + ai];
+      va += sdk * dpi[k];
+      d[k] -= sdk * qi;
+#pragma unroll
+      for (int l = 0; l < 3; ++l) d[k] += g.dd[(3 * l + k) * p2 + ai] * dpi[l];
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const double sqk = g.sq[k * p2 + ai];
+      va += sqk * g.qpat[6 * i + k];
+      q[k] += sqk * qi;
+    }
+  }
+  va = block_sum(va, red);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) d[k] = block_sum(d[k], red);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) q[k] = block_sum(q[k], red);
+  if (threadIdx.x == 0) {
+    const DevElem& e = P.el[g.el[a]];
+    g.vat[a] = va;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) g.vdp[3 * a + k] = d[k] + 2.0 * e.dkernel * g.dpat[3 * a + k];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const double sc = (k == 1 || k == 3 || k == 4) ? 2.0 : 1.0;
+      g.vqp[6 * a + k] = q[k] + 2.0 * e.qkernel * g.qpat[6 * a + k] * sc;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- Fock
+__global__ void k_g_fock(GridMol g)
+{
+  const int n = g.n;
+  const int64_t n2 = (int64_t)n * n;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < n2;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(idx / n), j = (int)(idx - (idx / n) * n);
+    const int ai = g.ao_at[i], aj = g.ao_at[j];
+    const double vi = g.vsh[g.ao_sh[i]] + g.vat[ai], vj = g.vsh[g.ao_sh[j]] + g.vat[aj];
+    const double* vd = g.vdp + 3 * ai;
+    const double* vq = g.vqp + 6 * ai;
+    double f = 0.5 * (g.H[idx] - g.S[idx] * 0.5 * (vi + vj));
+    f -= 0.5 * (g.D[idx] * vd[0] + g.D[n2 + idx] * vd[1] + g.D[2 * n2 + idx] * vd[2]);
+    f -= 0.5 * (g.Q[idx] * vq[0] + g.Q[n2 + idx] * vq[1] + g.Q[2 * n2 + idx] * vq[2] +
+                g.Q[3 * n2 + idx] * vq[3] + g.Q[4 * n2 + idx] * vq[4] + g.Q[5 * n2 + idx] * vq[5]);
+    g.Y[(int64_t)i * g.ld + j] = f;
+  }
+}
+
+// F = F1 + F1^T from Y into A
+__global__ void k_g_symmetrize(GridMol g)
+{
+  const int n = g.n;
+  const int64_t n2 = (int64_t)n * n;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < n2;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(idx / n), j = (int)(idx - (idx / n) * n);
+    g.A[(int64_t)i * g.ld + j] = g.Y[(int64_t)i * g.ld + j] + g.Y[(int64_t)j * g.ld + i];
+  }
+}
+
+// ------------------------------------------------------ tridiagonalisation
+// Column j (panel position i): update with the panel's reflectors, form the
+// reflector, store it (A column, panel row, vector v), panel dot products.
+__global__ __launch_bounds__(512) void k_g_tri_col(GridMol g, int j, int i)
+{
+  extern __shared__ double smem[];
+  lds_dbl* x = TA_LDS(smem);
+  lds_dbl* scal = x + g.ld;
+  lds_dbl* red = scal + 16;
+  const int n = g.n, ld = g.ld, tid = threadIdx.x;
+  const int lane = tid & 63, wid = wave_id();
+  const gbl_dbl* vt = TA_GBL(g.vt);
+  const gbl_dbl* wt = TA_GBL(g.wt);
+  for (int r = j + tid; r < n; r += GT) {
+    double a = g.A[(int64_t)r * ld + j];
+    for (int q = 0; q < i; ++q)
+      a -= vt[(int64_t)q * ld + r] * wt[(int64_t)q * ld + j] +
+           wt[(int64_t)q * ld + r] * vt[(int64_t)q * ld + j];
+    x[r] = a;
+  }
+  __syncthreads();
+  if (tid == 0) g.d[j] = x[j];
+  big::householder<GT>(x, j + 1, n, scal);
+  if (tid == 0) {
+    g.e[j] = scal[0];
+    g.tau[j] = scal[1];
+  }
+  for (int r = j + 1 + tid; r < n; r += GT) {
+    const double xr = x[r];
+    g.A[(int64_t)r * ld + j] = xr;
+    g.vt[(int64_t)i * ld + r] = xr;
+    g.v[r] = xr;
+  }
+  __syncthreads();
+  // dots[q] = w_q . v, dots[NP + q] = v_q . v over rows > j
+  for (int pq = wid; pq < 2 * i; pq += GT / 64) {
+    const int q = pq < i ? pq : pq - i;
+    const gbl_dbl* col = (pq < i ? wt : vt) + (int64_t)q * ld;
+    double s = 0.0;
+    for (int r = j + 1 + lane; r < n; r += 64) s += col[r] * x[r];
+    s = wave_sum(s);
+    if (lane == 0) g.dots[(pq < i ? 0 : TA_BIG_NP) + q] = s;
+  }
+  (void)red;
+}
+
+// Symmetric product over 64 x 64 tiles (bi >= bj) of the trailing triangle
+// [lo, n): tile (bi, bj) writes its row sums to part[bi][bj] and, off the
+// diagonal, its column sums to part[bj][bi] (64 doubles each).
+__global__ __launch_bounds__(256) void k_g_tri_symv(GridMol g, int lo)
+{
+  __shared__ double t_[64 * 65];
+  __shared__ double vr_[64], vc_[64];
+  lds_dbl* T = TA_LDS(t_);
+  lds_dbl* vr = TA_LDS(vr_);
+  lds_dbl* vc = TA_LDS(vc_);
+  const int n = g.n, ld = g.ld, m = n - lo;
+  const int nb = (m + 63) >> 6;
+  const int t = blockIdx.x;
+  if (t >= nb * (nb + 1) / 2) return;
+  int bi = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while (bi * (bi + 1) / 2 > t) --bi;
+  while ((bi + 1) * (bi + 2) / 2 <= t) ++bi;
+  const int bj = t - bi * (bi + 1) / 2;
+  const int r0 = lo + 64 * bi, c0 = lo + 64 * bj;
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < 4096; idx += 256) {
+    const int r = idx >> 6, c = idx & 63;
+    const int gr = r0 + r, gc = c0 + c;
+    double a = 0.0;
+    if (gr < n && gc < n) {
+      if (gc <= gr) a = g.A[(int64_t)gr * ld + gc];
+      else a = g.A[(int64_t)gc * ld + gr];  // diagonal tile: mirror of the lower triangle
+    }
+    T[r * 65 + c] = a;
+  }
+  if (tid < 64) {
+    vr[tid] = (r0 + tid < n) ? g.v[r0 + tid] : 0.0;
+    vc[tid] = (c0 + tid < n) ? g.v[c0 + tid] : 0.0;
+  }
+  __syncthreads();
+  const int rr = tid >> 2, qq = tid & 3;
+  double s = 0.0;
+#pragma unroll
+  for (int c = 0; c < 16; ++c) s += T[rr * 65 + 16 * qq + c] * vc[16 * qq + c];
+  s += __shfl_xor(s, 1, 4);
+  s += __shfl_xor(s, 2, 4);
+  const int64_t nbm = g.nbmax;
+  if (qq == 0) g.part[((int64_t)bi * nbm + bj) * 64 + rr] = s;
+  if (bi != bj) {
+    double cs = 0.0;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) cs += T[(16 * qq + r) * 65 + rr] * vr[16 * qq + r];
+    cs += __shfl_xor(cs, 1, 4);
+    cs += __shfl_xor(cs, 2, 4);
+    if (qq == 0) g.part[((int64_t)bj * nbm + bi) * 64 + rr] = cs;
+  }
+}
+
+// y = tau (sum of partials - V (W^T v) - W (V^T v)); w = y - (tau/2)(y.v) v
+__global__ __launch_bounds__(512) void k_g_tri_fin(GridMol g, int j, int i)
+{
+  __shared__ double red_[16];
+  lds_dbl* red = TA_LDS(red_);
+  const int n = g.n, ld = g.ld, lo = j + 1, m = n - lo;
+  const int nb = (m + 63) >> 6;
+  const int64_t nbm = g.nbmax;
+  const double tj = g.tau[j];
+  double sv = 0.0;
+  for (int r = lo + threadIdx.x; r < n; r += GT) {
+    const int R = (r - lo) >> 6, l = (r - lo) & 63;
+    double y = 0.0;
+    for (int q = 0; q < nb; ++q) y += g.part[((int64_t)R * nbm + q) * 64 + l];
+    for (int q = 0; q < i; ++q)
+      y -= g.vt[(int64_t)q * ld + r] * g.dots[q] + g.wt[(int64_t)q * ld + r] * g.dots[TA_BIG_NP + q];
+    y *= tj;
+    g.y[r] = y;
+    sv += y * g.v[r];
+  }
+  sv = block_sum(sv, red);
+  const double alpha = -0.5 * tj * sv;
+  for (int r = lo + threadIdx.x; r < n; r += GT) g.wt[(int64_t)i * ld + r] = g.y[r] + alpha * g.v[r];
+  for (int r = threadIdx.x; r <= j; r += GT) {
+    g.wt[(int64_t)i * ld + r] = 0.0;
+    g.vt[(int64_t)i * ld + r] = 0.0;
+  }
+}
+
+// last 2 x 2 block after the final panel (np reflectors in the panel)
+__global__ void k_g_tri_last(GridMol g, int np)
+{
+  const int n = g.n, ld = g.ld, tid = threadIdx.x;
+  if (tid < 3) {
+    const int rr = (tid == 0) ? n - 2 : n - 1, cc = (tid == 2) ? n - 1 : n - 2;
+    double a = g.A[(int64_t)rr * ld + cc];
+    for (int q = 0; q < np; ++q)
+      a -= g.vt[(int64_t)q * ld + rr] * g.wt[(int64_t)q * ld + cc] +
+           g.wt[(int64_t)q * ld + rr] * g.vt[(int64_t)q * ld + cc];
+    if (tid == 0) g.d[n - 2] = a;
+    if (tid == 1) g.e[n - 2] = a;
+    if (tid == 2) g.d[n - 1] = a;
+  }
+}
+
+// ------------------------------------------------------------ eigenvalues
+__global__ __launch_bounds__(512) void k_g_gersh(GridMol g)
+{
+  __shared__ double red_[16];
+  lds_dbl* red = TA_LDS(red_);
+  const int n = g.n;
+  double gl = INFINITY, gu = -INFINITY, emax = 0.0;
+  for (int k = threadIdx.x; k < n; k += GT) {
+    const double el = k > 0 ? fabs(g.e[k - 1]) : 0.0, er = k < n - 1 ? fabs(g.e[k]) : 0.0;
+    gl = fmin(gl, g.d[k] - el - er);
+    gu = fmax(gu, g.d[k] + el + er);
+    if (k < n - 1) {
+      g.e2[k] = g.e[k] * g.e[k];
+      emax = fmax(emax, g.e2[k]);
+    }
+  }
+  gl = -block_max(-gl, red);
+  gu = block_max(gu, red);
+  emax = block_max(emax, red);
+  if (threadIdx.x == 0) {
+    const double tnorm = fmax(fabs(gl), fabs(gu));
+    const double pivmin = 2.2250738585072014e-308 * fmax(1.0, emax);
+    const double tol = 1.0e-12 * tnorm + 4.0 * big::EPS * tnorm + pivmin;
+    g.sc[GS_GL] = gl - 2.0 * tol;
+    g.sc[GS_GU] = gu + 2.0 * tol;
+    g.sc[GS_TNORM] = tnorm;
+    g.sc[GS_PIVMIN] = pivmin;
+    g.sc[GS_TOL] = tol;
+    g.ic[GI_SHIFT] = 0;
+  }
+}
+
+// one thread per level: bisection (warm start from the previous levels)
+__global__ __launch_bounds__(256) void k_g_bisect(GridMol g, int warm_ok)
+{
+  const int n = g.n;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double gl = g.sc[GS_GL], gu = g.sc[GS_GU], pivmin = g.sc[GS_PIVMIN], tol = g.sc[GS_TOL];
+  const double wd = warm_ok ? g.sc[GS_WARM] : 0.0;
+  const gbl_dbl* d = TA_GBL(g.d);
+  const gbl_dbl* e2 = TA_GBL(g.e2);
+  double lo = gl, hi = gu;
+  if (wd > 0.0) {
+    const double a = g.prevlam[i] - wd, b = g.prevlam[i] + wd;
+    int ca, cb, cc;
+    big::sturm3(d, e2, n, a, b, b, pivmin, ca, cb, cc);
+    if (ca <= i && cb > i) {
+      lo = a;
+      hi = b;
+    }
+  }
+  for (int it = 0; it < 64 && hi - lo > tol; ++it) {
+    const double h = hi - lo;
+    const double x1 = lo + 0.25 * h, x2 = lo + 0.5 * h, x3 = lo + 0.75 * h;
+    int c1, c2, c3;
+    big::sturm3(d, e2, n, x1, x2, x3, pivmin, c1, c2, c3);
+    if (c1 > i) {
+      hi = x1;
+    } else if (c2 > i) {
+      lo = x1;
+      hi = x2;
+    } else if (c3 > i) {
+      lo = x2;
+      hi = x3;
+    } else {
+      lo = x3;
+    }
+  }
+  const double l = 0.5 * (lo + hi);
+  g.lam[i] = l;
+  if (wd > 0.0) {
+    // non-negative doubles order like their bit patterns
+    const double sh = fabs(l - g.prevlam[i]);
+    atomicMax((unsigned long long*)(g.ic + GI_SHIFT), (unsigned long long)__double_as_longlong(sh));
+  }
+}
+
+__global__ void k_g_warm(GridMol g, int warm_ok)
+{
+  const double tnorm = g.sc[GS_TNORM];
+  if (threadIdx.x == 0) {
+    const double sh = __longlong_as_double(*(const long long*)(g.ic + GI_SHIFT));
+    g.sc[GS_WARM] = (warm_ok && g.sc[GS_WARM] > 0.0) ? 2.0 * sh + 1.0e-10 * tnorm : 1.0e-2 * tnorm;
+  }
+  for (int i = threadIdx.x; i < g.n; i += blockDim.x) g.prevlam[i] = g.lam[i];
+}
+
+// ---------------------------------------------------------- eigenvectors
+// one thread per level; factors in scr ([k][i], three planes of n x n)
+__global__ __launch_bounds__(256) void k_g_invit(GridMol g)
+{
+  const int n = g.n, ld = g.ld;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double tnorm = g.sc[GS_TNORM];
+  const double pert = big::EPS * tnorm + 1e-300;
+  const int64_t pl = (int64_t)n * n;
+  double* u0 = g.scr;
+  double* u1 = g.scr + pl;
+  double* u2 = g.scr + 2 * pl;
+  double* Y = g.Y;
+  const double l = g.lam[i];
+  double scale = 1.0;
+  for (int it = 0; it < 3; ++it) {
+    double p = g.d[0] - l, q = n > 1 ? g.e[0] : 0.0;
+    double bp = it == 0 ? big::start_vector(i, 0) : Y[i] * scale;
+    for (int k = 0; k < n - 1; ++k) {
+      const double sk = g.e[k], dk1 = g.d[k + 1] - l, ek1 = (k + 1 < n - 1) ? g.e[k + 1] : 0.0;
+      const double bk1 = it == 0 ? big::start_vector(i, k + 1) : Y[(int64_t)(k + 1) * ld + i] * scale;
+      double U0, U1, U2, rk;
+      if (fabs(p) >= fabs(sk)) {
+        if (fabs(p) < pert) p = p < 0.0 ? -pert : pert;
+        const double mlt = sk / p;
+        U0 = p;
+        U1 = q;
+        U2 = 0.0;
+        rk = bp;
+        p = dk1 - mlt * q;
+        q = ek1;
+        bp = bk1 - mlt * bp;
+      } else {
+        const double mlt = p / sk;
+        U0 = sk;
+        U1 = dk1;
+        U2 = ek1;
+        rk = bk1;
+        p = q - mlt * dk1;
+        q = -mlt * ek1;
+        bp = bp - mlt * bk1;
+      }
+      u0[(int64_t)k * n + i] = 1.0 / U0;
+      u1[(int64_t)k * n + i] = U1;
+      u2[(int64_t)k * n + i] = U2;
+      Y[(int64_t)k * ld + i] = rk;
+    }
+    if (fabs(p) < pert) p = p < 0.0 ? -pert : pert;
+    double x2 = 0.0, x1 = bp / p, nrm = x1 * x1;
+    Y[(int64_t)(n - 1) * ld + i] = x1;
+    for (int k = n - 2; k >= 0; --k) {
+      const double x = (Y[(int64_t)k * ld + i] - u1[(int64_t)k * n + i] * x1 -
+                        u2[(int64_t)k * n + i] * x2) *
+                       u0[(int64_t)k * n + i];
+      Y[(int64_t)k * ld + i] = x;
+      nrm += x * x;
+      x2 = x1;
+      x1 = x;
+    }
+    scale = 1.0 / sqrt(nrm);
+  }
+  for (int k = 0; k < n; ++k) Y[(int64_t)k * ld + i] *= scale;
+}
+
+// runs of close levels (gap <= ORTOL |T|) with more than one member
+__global__ void k_g_runs(GridMol g)
+{
+  if (threadIdx.x != 0) return;
+  const int n = g.n;
+  const double gap = big::ORTOL * g.sc[GS_TNORM];
+  int nr = 0, i = 0;
+  while (i < n) {
+    int j = i + 1;
+    while (j < n && g.lam[j] - g.lam[j - 1] <= gap) ++j;
+    if (j - i > 1) {
+      g.runs[2 * nr] = i;
+      g.runs[2 * nr + 1] = j;
+      ++nr;
+    }
+    i = j;
+  }
+  g.ic[GI_NRUNS] = nr;
+}
+
+// one wave per run: modified Gram-Schmidt twice, then normalise
+__global__ __launch_bounds__(64) void k_g_mgs(GridMol g)
+{
+  const int c = blockIdx.x;
+  if (c >= g.ic[GI_NRUNS]) return;
+  const int n = g.n, ld = g.ld, lane = threadIdx.x;
+  const int a = g.runs[2 * c], b = g.runs[2 * c + 1];
+  double* Y = g.Y;
+  for (int j = a; j < b; ++j) {
+    for (int pass = 0; pass < 2; ++pass)
+      for (int l = a; l < j; ++l) {
+        double s = 0.0;
+        for (int k = lane; k < n; k += 64) s += Y[(int64_t)k * ld + l] * Y[(int64_t)k * ld + j];
+        s = wave_sum(s);
+        for (int k = lane; k < n; k += 64) Y[(int64_t)k * ld + j] -= s * Y[(int64_t)k * ld + l];
+      }
+    double s = 0.0;
+    for (int k = lane; k < n; k += 64) {
+      const double y = Y[(int64_t)k * ld + j];
+      s += y * y;
+    }
+    s = wave_sum(s);
+    const double inv = 1.0 / sqrt(s);
+    for (int k = lane; k < n; k += 64) Y[(int64_t)k * ld + j] *= inv;
+  }
+}
+
+// -------------------------------------------------------- back-transform
+// T of reflector block k0 .. k0 + nb - 1 (T^-1 = diag(1/tau) + strict_upper(V^T V))
+__global__ __launch_bounds__(512) void k_g_bt_t(GridMol g, int k0, int nb)
+{
+  __shared__ double tile_[2048];
+  __shared__ double sinv_[TA_BIG_NB * TA_BIG_NB];
+  lds_dbl* tile = TA_LDS(tile_);
+  lds_dbl* sinv = TA_LDS(sinv_);
+  const int n = g.n, ld = g.ld, r0 = k0 + 1, m = n - r0;
+  const gbl_dbl* A = TA_GBL(g.A);
+  const gbl_dbl* tau = TA_GBL(g.tau);
+  auto vel = [=](int rr, int j) -> double {
+    return (j < nb && rr >= j && tau[k0 + j] != 0.0) ? A[(int64_t)(r0 + rr) * ld + k0 + j] : 0.0;
+  };
+  big::gemm<GT, false, true>(
+      TA_BIG_NB, TA_BIG_NB, [=](int i, int k) { return vel(k, i); },
+      [=](int k, int j) { return vel(k, j); }, AllK{m}, NoSkip{},
+      [=](int i, int j, double val) {
+        double s = 0.0;
+        if (i < j) s = val;
+        if (i == j) s = (i < nb && tau[k0 + i] != 0.0) ? 1.0 / tau[k0 + i] : 1.0;
+        sinv[i * TA_BIG_NB + j] = s;
+      },
+      tile);
+  if (threadIdx.x < TA_BIG_NB) {
+    const int j = threadIdx.x;
+    for (int i = TA_BIG_NB - 1; i >= 0; --i) {
+      double t = 0.0;
+      if (i == j) {
+        t = 1.0 / sinv[i * TA_BIG_NB + i];
+      } else if (i < j) {
+        double acc = 0.0;
+        for (int k = i + 1; k <= j; ++k) acc += sinv[i * TA_BIG_NB + k] * g.tm[k * TA_BIG_NB + j];
+        t = -acc / sinv[i * TA_BIG_NB + i];
+      }
+      g.tm[i * TA_BIG_NB + j] = t;
+    }
+  }
+}
+
+// ---------------------------------------------------------- occupation
+__global__ __launch_bounds__(64) void k_g_fermi(GridMol g)
+{
+  const int n = g.n;
+  double ts = fermi_wave(g.sc[GS_NALP], g.kt, n, g.lam, g.fs);
+  for (int r = threadIdx.x; r < n; r += 64) g.focc[r] = g.fs[r];
+  ts += fermi_wave(g.sc[GS_NBET], g.kt, n, g.lam, g.fs);
+  for (int r = threadIdx.x; r < n; r += 64) g.focc[r] += g.fs[r];
+  if (threadIdx.x == 0) {
+    int last = 0;
+    for (int r = 0; r < n; ++r)
+      if (g.focc[r] != 0.0) last = r + 1;
+    g.ic[GI_NOCC] = last;
+    g.sc[GS_TS] = ts;
+  }
+}
+
+// ---------------------------------------------------------- Mulliken
+// one wave per AO row: part[c][i], c = 0 PS, 1..3 PD, 4..9 PQ, 10 PH
+__global__ __launch_bounds__(64) void k_g_mull_rows(GridMol g)
+{
+  const int i = blockIdx.x, n = g.n, lane = threadIdx.x;
+  const int64_t n2 = (int64_t)n * n, row = (int64_t)i * n;
+  double acc[11];
+#pragma unroll
+  for (int c = 0; c < 11; ++c) acc[c] = 0.0;
+  for (int j = lane; j < n; j += 64) {
+    const double p = g.Y[(int64_t)i * g.ld + j];
+    const int64_t e = row + j;
+    acc[0] += p * g.S[e];
+    acc[1] += p * g.D[e];
+    acc[2] += p * g.D[n2 + e];
+    acc[3] += p * g.D[2 * n2 + e];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) acc[4 + c] += p * g.Q[c * n2 + e];
+    acc[10] += p * g.H[e];
+  }
+#pragma unroll
+  for (int c = 0; c < 11; ++c) acc[c] = wave_sum(acc[c]);
+  if (lane == 0)
+#pragma unroll
+    for (int c = 0; c < 11; ++c) g.part11[(int64_t)c * n + i] = acc[c];
+}
+
+__global__ __launch_bounds__(512) void k_g_mull_fin(GridMol g)
+{
+  __shared__ double red_[16];
+  lds_dbl* red = TA_LDS(red_);
+  const DevParams& P = *g.par;
+  const int n = g.n, nsh = g.nsh, nat = g.nat, tid = threadIdx.x;
+  const double* part = g.part11;
+  double err = 0.0;
+  for (int s = tid; s < nsh; s += GT) {
+    const int i0 = g.sh_iao[s];
+    const int l = P.el[g.el[g.sh_at[s]]].ang[g.sh_ish[s]];
+    double q = g.n0sh[s];
+    for (int i = i0; i < i0 + 2 * l + 1; ++i) q -= part[i];
+    g.qsh[s] = q;
+    const double d = q - g.qin[s];
+    g.dq[s] = d;
+    err += d * d;
+  }
+  for (int a = tid; a < nat; a += GT) {
+    const DevElem& e = P.el[g.el[a]];
+    const int i0 = g.at_iao[a];
+    int nao_a = 0;
+    for (int s = 0; s < e.nsh; ++s) nao_a += 2 * e.ang[s] + 1;
+    double dp[3] = {0.0, 0.0, 0.0}, qp[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int i = i0; i < i0 + nao_a; ++i) {
+      for (int k = 0; k < 3; ++k) dp[k] -= part[(int64_t)(1 + k) * n + i];
+      for (int k = 0; k < 6; ++k) qp[k] -= part[(int64_t)(4 + k) * n + i];
+    }
+    for (int k = 0; k < 3; ++k) {
+      g.dpat[3 * a + k] = dp[k];
+      const double d = dp[k] - g.qin[nsh + 3 * a + k];
+      g.dq[nsh + 3 * a + k] = d;
+      err += d * d;
+    }
+    for (int k = 0; k < 6; ++k) {
+      g.qpat[6 * a + k] = qp[k];
+      const double d = qp[k] - g.qin[nsh + 3 * nat + 6 * a + k];
+      g.dq[nsh + 3 * nat + 6 * a + k] = d;
+      err += d * d;
+    }
+  }
+  double eband = 0.0;
+  for (int i = tid; i < n; i += GT) eband += part[(int64_t)10 * n + i];
+  err = block_sum(err, red);
+  eband = block_sum(eband, red);
+  __syncthreads();
+  for (int a = tid; a < nat; a += GT) {
+    const int s0a = g.at_ish[a];
+    const int ns = P.el[g.el[a]].nsh;
+    double q = 0.0;
+    for (int s = 0; s < ns; ++s) q += g.qsh[s0a + s];
+    g.qat[a] = q;
+  }
+  if (tid == 0) {
+    g.sc[GS_ERR] = err;
+    g.sc[GS_EBAND] = eband;
+  }
+}
+
+// ---------------------------------------------------------- energy
+// blocks [0, nsh): ES2 + ES3 of one shell; [nsh, nsh + nat): AES2 of one atom
+__global__ __launch_bounds__(64) void k_g_energy(GridMol g)
+{
+  const DevParams& P = *g.par;
+  const int nat = g.nat, nsh = g.nsh, lane = threadIdx.x;
+  const int64_t p2 = (int64_t)nat * nat;
+  double ec = 0.0;
+  if ((int)blockIdx.x < nsh) {
+    const int sh = blockIdx.x;
+    const double* gr = g.gam + (int64_t)sh * nsh;
+    const double qs = g.qsh[sh];
+    for (int j = lane; j < nsh; j += 64) ec += 0.5 * qs * gr[j] * g.qsh[j];
+    ec = wave_sum(ec);
+    if (lane == 0)
+      g.epart[sh] = ec + qs * qs * qs * P.el[g.el[g.sh_at[sh]]].hubbard_derivs[g.sh_ish[sh]] / 3.0;
+    return;
+  }
+  const int a = blockIdx.x - nsh;
+  const int64_t row = (int64_t)a * nat;
+  const double dpa[3] = {g.dpat[3 * a], g.dpat[3 * a + 1], g.dpat[3 * a + 2]};
+  double qpa[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) qpa[k] = g.qpat[6 * a + k];
+  for (int i = lane; i < nat; i += 64) {
+    const int64_t ai = row + i;
+    const double qi = g.qat[i];
+    const double dpi[3] = {g.dpat[3 * i], g.dpat[3 * i + 1], g.dpat[3 * i + 2]};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      double t = -g.sd[k * p2 + ai] * qi;
+#pragma unroll
+      for (int l = 0; l < 3; ++l) t += 0.5 * g.dd[(3 * l + k) * p2 + ai] * dpi[l];
+      ec += dpa[k] * t;
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) ec += qpa[k] * g.sq[k * p2 + ai] * qi;
+  }
+  ec = wave_sum(ec);
+  if (lane == 0) {
+    const DevElem& e = P.el[g.el[a]];
+    ec += e.dkernel * (dpa[0] * dpa[0] + dpa[1] * dpa[1] + dpa[2] * dpa[2]);
+    ec += e.qkernel * (qpa[0] * qpa[0] + 2.0 * qpa[1] * qpa[1] + qpa[2] * qpa[2] +
+                       2.0 * qpa[3] * qpa[3] + 2.0 * qpa[4] * qpa[4] + qpa[5] * qpa[5]);
+    g.epart[nsh + a] = ec;
+  }
+}
+
+__global__ __launch_bounds__(512) void k_g_energy_fin(GridMol g, int iter)
+{
+  __shared__ double red_[16];
+  lds_dbl* red = TA_LDS(red_);
+  double ec = 0.0;
+  for (int k = threadIdx.x; k < g.nsh + g.nat; k += GT) ec += g.epart[k];
+  ec = block_sum(ec, red);
+  if (threadIdx.x == 0) {
+    const double elast = g.sc[GS_ESUM];
+    const double esum = g.sc[GS_EBAND] + g.sc[GS_TS] + ec;
+    const double perr = sqrt(g.sc[GS_ERR] / g.ndim);
+    const int conv = (fabs(esum - elast) < g.econv) && (perr < g.pconv);
+    g.sc[GS_ESUM] = esum;
+    g.ic[GI_CONVERGED] = conv;
+    if (!isfinite(esum)) g.ic[GI_STATUS] |= TA_STATUS_NAN;
+    (void)iter;
+  }
+}
+
+// ---------------------------------------------------------- mixing
+// modified Broyden step (same scheme as the single-workgroup kernel), then
+// unpack into the wavefunction
+__global__ __launch_bounds__(512) void k_g_broyden(GridMol g, int mixiter)
+{
+  __shared__ double red_[16];
+  __shared__ double scal_[4];
+  lds_dbl* red = TA_LDS(red_);
+  const int tid = threadIdx.x, lane = tid & 63, wid = wave_id();
+  constexpr int NW = GT / 64;
+  const int ndim = g.ndim, mem = g.mem, nsh = g.nsh, nat = g.nat;
+  const double damp = g.damping;
+  double* qin = g.qin;
+  double* dq = g.dq;
+  double* dqlast = g.dqlast;
+  double* qlast = g.qlast;
+  if (mixiter == 1) {
+    for (int i = tid; i < ndim; i += GT) {
+      dqlast[i] = dq[i];
+      qlast[i] = qin[i];
+      qin[i] += damp * dq[i];
+    }
+    __syncthreads();
+  } else {
+    const int it1 = (mixiter - 2) % mem;
+    const int ldim = (mixiter - 1) < mem ? (mixiter - 1) : mem;
+    double* dfc = g.dfm + (int64_t)it1 * ndim;
+    double* uc = g.um + (int64_t)it1 * ndim;
+    double a1 = 0.0, a2 = 0.0;
+    for (int i = tid; i < ndim; i += GT) {
+      const double t = dq[i] - dqlast[i];
+      dfc[i] = t;
+      a1 += dq[i] * dq[i];
+      a2 += t * t;
+    }
+    a1 = block_sum(a1, red);
+    a2 = block_sum(a2, red);
+    double om = sqrt(a1);
+    om = (om > 0.01 / 100000.0) ? 0.01 / om : 100000.0;
+    if (om < 1.0) om = 1.0;
+    const double inv_ = 1.0 / fmax(sqrt(a2), 2.220446049250313e-16);
+    if (tid == 0) g.omega[it1] = om;
+    for (int i = tid; i < ndim; i += GT) {
+      const double d = dfc[i] * inv_;
+      dfc[i] = d;
+      uc[i] = damp * d + inv_ * (qin[i] - qlast[i]);
+    }
+    __syncthreads();
+    for (int j = wid; j < ldim; j += NW) {
+      const double* dfj = g.dfm + (int64_t)j * ndim;
+      double s1 = 0.0, s2 = 0.0;
+      for (int i = lane; i < ndim; i += 64) {
+        s1 += dfj[i] * dfc[i];
+        s2 += dfj[i] * dq[i];
+      }
+      s1 = wave_sum(s1);
+      s2 = wave_sum(s2);
+      if (lane == 0) {
+        g.amix[j + (int64_t)mem * it1] = s1;
+        g.amix[it1 + (int64_t)mem * j] = s1;
+        g.cvec[j] = (j == it1 ? om : g.omega[j]) * s2;
+      }
+    }
+    __syncthreads();
+    const int lda = ldim + 1;
+    double* beta = g.beta;
+    for (int idx = tid; idx < ldim * ldim; idx += GT) {
+      const int i = idx / ldim, j = idx - (idx / ldim) * ldim;
+      double val = g.omega[i] * g.omega[j] * g.amix[i + (int64_t)mem * j];
+      if (i == j) val += 0.01 * 0.01;
+      beta[i * lda + j] = val;
+    }
+    for (int i = tid; i < ldim; i += GT) beta[i * lda + ldim] = g.cvec[i];
+    __syncthreads();
+    bool fail = false;
+    for (int col = 0; col < ldim; ++col) {
+      if (wid == 0) {
+        double best = -1.0;
+        int bi = col;
+        for (int r = col + lane; r < ldim; r += 64) {
+          const double a = fabs(beta[r * lda + col]);
+          if (a > best) {
+            best = a;
+            bi = r;
+          }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+          const double ob = __shfl_xor(best, off, 64);
+          const int oi = __shfl_xor(bi, off, 64);
+          if (ob > best || (ob == best && oi < bi)) {
+            best = ob;
+            bi = oi;
+          }
+        }
+        if (lane == 0) {
+          scal_[1] = (double)bi;
+          scal_[2] = best;
+        }
+      }
+      __syncthreads();
+      const int piv = (int)scal_[1];
+      if (!(scal_[2] > 0.0)) {
+        fail = true;
+        break;
+      }
+      if (piv != col)
+        for (int k = tid; k <= ldim; k += GT) {
+          const double t = beta[col * lda + k];
+          beta[col * lda + k] = beta[piv * lda + k];
+          beta[piv * lda + k] = t;
+        }
+      __syncthreads();
+      const double pv = beta[col * lda + col];
+      const int rows = ldim - col - 1, cols = ldim - col;
+      for (int idx = tid; idx < rows * cols; idx += GT) {
+        const int r = col + 1 + idx / cols, k = col + 1 + idx % cols;
+        beta[r * lda + k] -= beta[r * lda + col] / pv * beta[col * lda + k];
+      }
+      __syncthreads();
+      for (int r = col + 1 + tid; r < ldim; r += GT) beta[r * lda + col] = 0.0;
+      __syncthreads();
+    }
+    if (fail) {
+      if (tid == 0) g.ic[GI_STATUS] |= TA_STATUS_MIXER;
+      return;
+    }
+    if (wid == 0) {
+      for (int r = ldim - 1; r >= 0; --r) {
+        double s = 0.0;
+        for (int k = r + 1 + lane; k < ldim; k += 64) s += beta[r * lda + k] * g.cvec[k];
+        s = wave_sum(s);
+        if (lane == 0) g.cvec[r] = (beta[r * lda + ldim] - s) / beta[r * lda + r];
+      }
+    }
+    __syncthreads();
+    for (int i = tid; i < ndim; i += GT) {
+      double q = qin[i] + damp * dq[i];
+      for (int j = 0; j < ldim; ++j) q -= g.omega[j] * g.cvec[j] * g.um[(int64_t)j * ndim + i];
+      dqlast[i] = dq[i];
+      qlast[i] = qin[i];
+      qin[i] = q;
+    }
+    __syncthreads();
+  }
+  for (int i = tid; i < nsh; i += GT) g.qsh[i] = qin[i];
+  for (int i = tid; i < 3 * nat; i += GT) g.dpat[i] = qin[nsh + i];
+  for (int i = tid; i < 6 * nat; i += GT) g.qpat[i] = qin[nsh + 3 * nat + i];
+  __syncthreads();
+  const DevParams& P = *g.par;
+  for (int a = tid; a < nat; a += GT) {
+    const int s0a = g.at_ish[a];
+    const int ns = P.el[g.el[a]].nsh;
+    double q = 0.0;
+    for (int s = 0; s < ns; ++s) q += g.qsh[s0a + s];
+    g.qat[a] = q;
+  }
+}
+
+// q_in = current charges (set_mixer) before the first mixing step
+__global__ void k_g_set_mixer(GridMol g)
+{
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < g.ndim; i += gridDim.x * blockDim.x)
+    g.qin[i] = 0.0;
+}
+
+// X = U s^-1/2 U^T needs s > 0: flag a singular overlap
+__global__ void k_g_check_overlap(GridMol g)
+{
+  if (threadIdx.x == 0 && !(g.lam[0] > 1.0e-10)) g.ic[GI_STATUS] |= TA_STATUS_CHOLESKY;
+}
+
+// energy, status and iteration count of the molecule
+__global__ void k_g_final(GridMol g, int iter, int status, double* energy, double* eelec,
+                          double* erep, int* iterations, int* st)
+{
+  if (threadIdx.x == 0) {
+    const double esum = g.sc[GS_ESUM], er = g.sc[GS_EREP];
+    *energy = er + esum;
+    *eelec = esum;
+    *erep = er;
+    *iterations = iter;
+    *st = status;
+  }
+}
+
+// results: sorted levels and occupations, optional coefficients
+__global__ void k_g_results(GridMol g, double* emo, double* focc, double* coeff)
+{
+  const int n = g.n;
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+    emo[r] = g.lam[r];
+    focc[r] = g.focc[r];
+  }
+  if (coeff) {
+    const int64_t tot = (int64_t)n * n;
+    for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < tot;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+      const int i = (int)(idx / n), r = (int)(idx - (idx / n) * n);
+      coeff[idx] = g.CT[(int64_t)r * g.ld + i];
+    }
+  }
+}
+
+int grid_blocks(int64_t work, int per = 256) { return (int)std::min<int64_t>((work + per - 1) / per, 8192); }
+
+// Eigenvalues (g.lam, ascending) and eigenvectors (g.Y columns) of the
+// symmetric matrix in g.A (lower triangle used; destroyed).
+hipError_t grid_eigensolve(GridMol& g, hipStream_t s, bool warm)
+{
+  const int n = g.n, ld = g.ld;
+  const int nr = n - 2;
+  constexpr int NP = TA_BIG_NP;
+  const size_t col_lds = sizeof(double) * (ld + 32 + 16);
+  for (int k0 = 0; k0 < nr; k0 += NP) {
+    const int np = (k0 + NP <= nr) ? NP : nr - k0;
+    for (int i = 0; i < np; ++i) {
+      const int j = k0 + i;
+      hipLaunchKernelGGL(k_g_tri_col, dim3(1), dim3(GT), col_lds, s, g, j, i);
+      const int m = n - (j + 1), nb = (m + 63) >> 6;
+      hipLaunchKernelGGL(k_g_tri_symv, dim3(nb * (nb + 1) / 2), dim3(256), 0, s, g, j + 1);
+      hipLaunchKernelGGL(k_g_tri_fin, dim3(1), dim3(GT), 0, s, g, j, i);
+    }
+    const int r0 = k0 + np;
+    if (r0 >= nr) {
+      hipLaunchKernelGGL(k_g_tri_last, dim3(1), dim3(64), 0, s, g, np);
+      break;
+    }
+    const int m = n - r0;
+    double* A = g.A;
+    const double* vt = g.vt;
+    const double* wt = g.wt;
+    g_gemm<false, true>(
+        s, m, m,
+        [=] __device__(int ii, int k) {
+          return k < NP ? vt[(int64_t)k * ld + r0 + ii] : wt[(int64_t)(k - NP) * ld + r0 + ii];
+        },
+        [=] __device__(int k, int jj) {
+          return k < NP ? wt[(int64_t)k * ld + r0 + jj] : vt[(int64_t)(k - NP) * ld + r0 + jj];
+        },
+        AllK{2 * NP}, UpperSkip{},
+        [=] __device__(int ii, int jj, double val) {
+          if (ii >= jj) A[(int64_t)(r0 + ii) * ld + r0 + jj] -= val;
+        });
+  }
+  hipLaunchKernelGGL(k_g_gersh, dim3(1), dim3(GT), 0, s, g);
+  hipLaunchKernelGGL(k_g_bisect, dim3(grid_blocks(n)), dim3(256), 0, s, g, warm ? 1 : 0);
+  hipLaunchKernelGGL(k_g_warm, dim3(1), dim3(256), 0, s, g, warm ? 1 : 0);
+  hipLaunchKernelGGL(k_g_invit, dim3(grid_blocks(n)), dim3(256), 0, s, g);
+  hipLaunchKernelGGL(k_g_runs, dim3(1), dim3(64), 0, s, g);
+  hipLaunchKernelGGL(k_g_mgs, dim3((n + 1) / 2), dim3(64), 0, s, g);
+  // back-transformation, last block of reflectors first
+  constexpr int NB = TA_BIG_NB;
+  const int nblk = (nr + NB - 1) / NB;
+  for (int bi = nblk - 1; bi >= 0; --bi) {
+    const int k0 = bi * NB;
+    const int nb = (k0 + NB <= nr) ? NB : nr - k0;
+    const int r0 = k0 + 1;
+    const int m = n - r0;
+    hipLaunchKernelGGL(k_g_bt_t, dim3(1), dim3(GT), 0, s, g, k0, nb);
+    const double* A = g.A;
+    const double* tau = g.tau;
+    double* Y = g.Y;
+    double* W = g.wbt;
+    const double* tm = g.tm;
+    auto vel = [=] __device__(int rr, int j) -> double {
+      return (j < nb && rr >= j && tau[k0 + j] != 0.0) ? A[(int64_t)(r0 + rr) * ld + k0 + j] : 0.0;
+    };
+    g_gemm<false, true>(
+        s, NB, n, [=] __device__(int i, int k) { return vel(k, i); },
+        [=] __device__(int k, int j) { return Y[(int64_t)(r0 + k) * ld + j]; }, AllK{m}, NoSkip{},
+        [=] __device__(int i, int j, double val) { W[(int64_t)i * ld + j] = val; });
+    g_gemm<true, true>(
+        s, NB, n, [=] __device__(int i, int k) { return tm[i * NB + k]; },
+        [=] __device__(int k, int j) { return W[(int64_t)k * ld + j]; }, AllK{NB}, NoSkip{},
+        [=] __device__(int i, int j, double val) { W[(int64_t)i * ld + j] = val; });
+    g_gemm<true, true>(
+        s, m, n, [=] __device__(int i, int k) { return vel(i, k); },
+        [=] __device__(int k, int j) { return W[(int64_t)k * ld + j]; }, AllK{NB}, NoSkip{},
+        [=] __device__(int i, int j, double val) { Y[(int64_t)(r0 + i) * ld + j] -= val; });
+  }
+  return hipGetLastError();
+}
+
+}  // namespace
+
+// Full SCF of one molecule on the grid path; writes energy/status/iterations
+// of molecule m into the batch arrays.
+int grid_scf(GridMol g, hipStream_t s, int max_iter, double* d_energy, double* d_eelec,
+             double* d_erep, int* d_iter, int* d_status, double* emo, double* focc, double* coeff,
+             GridHostStats* hs)
+{
+  const int n = g.n, ld = g.ld;
+  hipError_t err;
+  hipLaunchKernelGGL(k_g_init, dim3(1), dim3(256), 0, s, g);
+  // X = S^-1/2 from the eigenpairs of S
+  hipLaunchKernelGGL(k_g_copy_s, dim3(grid_blocks((int64_t)n * n)), dim3(256), 0, s, g);
+  if ((err = grid_eigensolve(g, s, false)) != hipSuccess) return -2;
+  hipLaunchKernelGGL(k_g_check_overlap, dim3(1), dim3(64), 0, s, g);
+  {
+    const double* Y = g.Y;
+    const double* lam = g.lam;
+    double* X = g.X;
+    g_gemm<true, false>(
+        s, n, n,
+        [=] __device__(int i, int k) { return Y[(int64_t)i * ld + k] * rsqrt(sqrt(fmax(lam[k], 1e-300))); },
+        [=] __device__(int k, int j) { return Y[(int64_t)j * ld + k] * rsqrt(sqrt(fmax(lam[k], 1e-300))); },
+        AllK{n}, NoSkip{}, [=] __device__(int i, int j, double val) { X[(int64_t)i * ld + j] = val; });
+  }
+  int iter = 0, mixiter = 0, converged = 0, status = 0;
+  int hbuf[4];
+  if (hipMemcpyAsync(hbuf, g.ic, sizeof(int) * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return -2;
+  status = hbuf[GI_STATUS];
+  while (!converged && !status && iter < max_iter) {
+    if (iter > 0) {
+      ++mixiter;
+      hipLaunchKernelGGL(k_g_broyden, dim3(1), dim3(GT), 0, s, g, mixiter);
+    }
+    ++iter;
+    hipLaunchKernelGGL(k_g_potential, dim3(g.nsh + g.nat), dim3(256), 0, s, g);
+    if (iter == 1) hipLaunchKernelGGL(k_g_set_mixer, dim3(grid_blocks(g.ndim)), dim3(256), 0, s, g);
+    hipLaunchKernelGGL(k_g_fock, dim3(grid_blocks((int64_t)n * n)), dim3(256), 0, s, g);
+    hipLaunchKernelGGL(k_g_symmetrize, dim3(grid_blocks((int64_t)n * n)), dim3(256), 0, s, g);
+    // A = X F X: U = X F into Y, A = U X (lower tiles)
+    {
+      const double* X = g.X;
+      const double* F = g.A;
+      double* U = g.Y;
+      g_gemm<true, true>(
+          s, n, n, [=] __device__(int i, int k) { return X[(int64_t)i * ld + k]; },
+          [=] __device__(int k, int j) { return F[(int64_t)k * ld + j]; }, AllK{n}, NoSkip{},
+          [=] __device__(int i, int j, double val) { U[(int64_t)i * ld + j] = val; });
+      double* A = g.A;
+      const double* Uc = g.Y;
+      g_gemm<true, true>(
+          s, n, n, [=] __device__(int i, int k) { return Uc[(int64_t)i * ld + k]; },
+          [=] __device__(int k, int j) { return X[(int64_t)k * ld + j]; }, AllK{n}, UpperSkip{},
+          [=] __device__(int i, int j, double val) { A[(int64_t)i * ld + j] = val; });
+    }
+    if ((err = grid_eigensolve(g, s, iter > 1)) != hipSuccess) return -2;
+    // MO rows CT = Z^T X
+    {
+      const double* Z = g.Y;
+      const double* X = g.X;
+      double* CT = g.CT;
+      g_gemm<false, true>(
+          s, n, n, [=] __device__(int i, int k) { return Z[(int64_t)k * ld + i]; },
+          [=] __device__(int k, int j) { return X[(int64_t)k * ld + j]; }, AllK{n}, NoSkip{},
+          [=] __device__(int i, int j, double val) { CT[(int64_t)i * ld + j] = val; });
+    }
+    hipLaunchKernelGGL(k_g_fermi, dim3(1), dim3(64), 0, s, g);
+    if (hipMemcpyAsync(hbuf, g.ic, sizeof(int) * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return -2;
+    const int nocc = hbuf[GI_NOCC];
+    // density P = sum_k f_k c_k c_k^T into Y (lower tiles mirrored)
+    {
+      const double* CT = g.CT;
+      const double* f = g.focc;
+      double* P = g.Y;
+      g_gemm<false, true>(
+          s, n, n, [=] __device__(int i, int k) { return CT[(int64_t)k * ld + i] * f[k]; },
+          [=] __device__(int k, int j) { return CT[(int64_t)k * ld + j]; }, AllK{nocc}, UpperSkip{},
+          [=] __device__(int i, int j, double val) {
+            if (i >= j) {
+              P[(int64_t)i * ld + j] = val;
+              P[(int64_t)j * ld + i] = val;
+            }
+          });
+    }
+    hipLaunchKernelGGL(k_g_mull_rows, dim3(n), dim3(64), 0, s, g);
+    hipLaunchKernelGGL(k_g_mull_fin, dim3(1), dim3(GT), 0, s, g);
+    hipLaunchKernelGGL(k_g_energy, dim3(g.nsh + g.nat), dim3(64), 0, s, g);
+    hipLaunchKernelGGL(k_g_energy_fin, dim3(1), dim3(GT), 0, s, g, iter);
+    if ((err = hipGetLastError()) != hipSuccess) return -2;
+    if (hipMemcpyAsync(hbuf, g.ic, sizeof(int) * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return -2;
+    converged = hbuf[GI_CONVERGED];
+    status = hbuf[GI_STATUS];
+  }
+  if (!converged && status == 0) status |= TA_STATUS_NOT_CONVERGED;
+  hipLaunchKernelGGL(k_g_final, dim3(1), dim3(64), 0, s, g, iter, status, d_energy, d_eelec,
+                     d_erep, d_iter, d_status);
+  hipLaunchKernelGGL(k_g_results, dim3(grid_blocks(n)), dim3(256), 0, s, g, emo, focc, coeff);
+  if (hipStreamSynchronize(s) != hipSuccess) return -2;
+  if (hs) {
+    hs->iterations = iter;
+    hs->status = status;
+  }
+  return 0;
+}

@@ -246,6 +246,7 @@ struct tblite_amd_context_s {
   // path (TBLITE_AMD_GRID_MIN_LD may lower it, e.g. to test that path on
   // molecules the oracle handles quickly)
   int grid_min_ld = TA_BIG_MAXLD;
+  int verbose = 0;  // TBLITE_AMD_VERBOSE=1: progress lines of long runs on stderr
 };
 
 struct tblite_amd_structure_s {
@@ -779,7 +780,7 @@ int run_grid_molecule(tblite_amd_context ctx, Calculator& c, int m)
   GridHostStats hs;
   int rc = grid_scf(g, ctx->stream, c.max_iter, b.energy + m, b.eelec + m, b.erep_tot + m,
                     b.iterations + m, b.status + m, b.emo + o0, b.focc + o0,
-                    b.coeff ? b.coeff + moff : nullptr, &hs);
+                    b.coeff ? b.coeff + moff : nullptr, &hs, ctx->verbose);
   if (rc) {
     set_error(ctx, "multi-workgroup SCF of molecule %d failed: %s", m,
               hipGetErrorString(hipGetLastError()));
@@ -982,6 +983,8 @@ tblite_amd_context tblite_amd_new_context(int device)
   ctx->lds_limit = (size_t)lds;
   const char* pe = std::getenv("TBLITE_AMD_PROFILE");
   ctx->profile = pe && pe[0] == '1';
+  const char* pv = std::getenv("TBLITE_AMD_VERBOSE");
+  ctx->verbose = pv && pv[0] == '1';
   const char* pg = std::getenv("TBLITE_AMD_GRID_MIN_LD");
   if (pg) ctx->grid_min_ld = std::max(16, std::min(TA_BIG_MAXLD, std::atoi(pg)));
   if (!ok) {

@@ -25,6 +25,8 @@
 #include <math.h>
 
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
 #include <vector>
 
 #include "common.h"
@@ -1059,7 +1061,7 @@ hipError_t grid_eigensolve(GridMol& g, hipStream_t s, bool warm)
 // of molecule m into the batch arrays.
 int grid_scf(GridMol g, hipStream_t s, int max_iter, double* d_energy, double* d_eelec,
              double* d_erep, int* d_iter, int* d_status, double* emo, double* focc, double* coeff,
-             GridHostStats* hs)
+             GridHostStats* hs, int verbose)
 {
   const int n = g.n, ld = g.ld;
   hipError_t err;
@@ -1080,6 +1082,7 @@ int grid_scf(GridMol g, hipStream_t s, int max_iter, double* d_energy, double* d
   }
   int iter = 0, mixiter = 0, converged = 0, status = 0;
   int hbuf[4];
+  const auto t_start = std::chrono::steady_clock::now();
   if (hipMemcpyAsync(hbuf, g.ic, sizeof(int) * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess)
     return -2;
@@ -1151,6 +1154,12 @@ int grid_scf(GridMol g, hipStream_t s, int max_iter, double* d_energy, double* d
       return -2;
     converged = hbuf[GI_CONVERGED];
     status = hbuf[GI_STATUS];
+    if (verbose) {
+      const double ms = std::chrono::duration<double, std::milli>(
+                            std::chrono::steady_clock::now() - t_start).count();
+      std::fprintf(stderr, "[tblite_amd] grid SCF n=%d iteration %d: converged %d status %d (%.0f ms)\n",
+                   n, iter, converged, status, ms);
+    }
   }
   if (!converged && status == 0) status |= TA_STATUS_NOT_CONVERGED;
   hipLaunchKernelGGL(k_g_final, dim3(1), dim3(64), 0, s, g, iter, status, d_energy, d_eelec,

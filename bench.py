@@ -58,7 +58,7 @@ def scf_kernel_name(ldmax, nthreads):
     if ldmax > 96:
         return "k_scf<512, 1, 0, 0, false> (global-memory matrices, Householder eigensolver)"
     maxt = 2 if nthreads == 512 else 1
-    maxb, maxc = (2, 8) if nthreads == 128 else (3, 11)
+    maxb, maxc = (2, 4) if nthreads == 128 else (3, 6)
     return f"k_scf<{nthreads}, {maxt}, {maxb}, {maxc}, true> (LDS matrices, Jacobi)"
 
 

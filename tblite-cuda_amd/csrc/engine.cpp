@@ -837,18 +837,18 @@ int run_batch(tblite_amd_context ctx, Calculator& c, const double* xyz, double a
     const int cnt = (int)bk.mols.size();
     int rc;
     HIP_TRY(ctx, hipEventRecord(ctx->bev[2 * k], as));
-    // per-worker-thread Jacobi items (MAXB blocks of A, MAXC rows of CT) over the
-    // NT - 64 worker threads: ld <= 32: 120 / 512 over 64; ld <= 64: 496 / 2048
-    // over 192; ld <= 96: 1128 / 4608 over 448
+    // per-worker-thread Jacobi items (MAXB blocks of A, MAXC two-AO pieces of
+    // CT rows) over the NT - 64 worker threads: ld <= 32: 120 / 256 over 64;
+    // ld <= 64: 496 / 1024 over 192; ld <= 96: 1128 / 2304 over 448
     if (!bk.lds)
       rc = launch_scf<TA_BIG_NT, 1, 0, 0, false>(ctx, as, b, mols, cnt, bk.ldmax, bk.natmax,
                                                  bk.nshmax);
     else if (bk.nt == 128)
-      rc = launch_scf<128, 1, 2, 8, true>(ctx, as, b, mols, cnt, bk.ldmax, bk.natmax, bk.nshmax);
+      rc = launch_scf<128, 1, 2, 4, true>(ctx, as, b, mols, cnt, bk.ldmax, bk.natmax, bk.nshmax);
     else if (bk.nt == 256)
-      rc = launch_scf<256, 1, 3, 11, true>(ctx, as, b, mols, cnt, bk.ldmax, bk.natmax, bk.nshmax);
+      rc = launch_scf<256, 1, 3, 6, true>(ctx, as, b, mols, cnt, bk.ldmax, bk.natmax, bk.nshmax);
     else
-      rc = launch_scf<512, 2, 3, 11, true>(ctx, as, b, mols, cnt, bk.ldmax, bk.natmax, bk.nshmax);
+      rc = launch_scf<512, 2, 3, 6, true>(ctx, as, b, mols, cnt, bk.ldmax, bk.natmax, bk.nshmax);
     if (rc) return rc;
     HIP_TRY(ctx, hipEventRecord(ctx->bev[2 * k + 1], as));
   }

@@ -270,10 +270,12 @@ __device__ void jacobi_plan(JPlan<MAXB, MAXC>& pl, int ld)
       const int l = k + 1 + rem;
       if (!jacobi_owner_block(k, l, h)) pl.bkl[j] = (k << 16) | l;
     }
+    // CT items cover two adjacent AOs (one 16-byte LDS access per row)
+    const int hl = ld >> 1;
 #pragma unroll
     for (int j = 0; j < MAXC; ++j) {
       const int ci = w + j * nw;
-      if (ci < h * ld) pl.crk[j] = ((ci / ld) << 16) | (ci % ld);
+      if (ci < h * hl) pl.crk[j] = ((ci / hl) << 16) | (2 * (ci % hl));
     }
   }
 }
@@ -390,6 +392,16 @@ __device__ bool jacobi(double* A, double* CT, int ld, const JRot& R, const JPlan
           *rp = cs.x * u - cs.y * v;
           *rq = cs.y * u + cs.x * v;
         };
+        // two adjacent AOs of both rows (ao even, rows 16-byte aligned)
+        auto crow2 = [&](int k, int ao) {
+          const int2 pq = R.pq[co + k];
+          const double2 cs = R.cs[co + k];
+          double2* rp = reinterpret_cast<double2*>(CT + pq.x * ld + ao);
+          double2* rq = reinterpret_cast<double2*>(CT + pq.y * ld + ao);
+          const double2 u = *rp, v = *rq;
+          *rp = make_double2(cs.x * u.x - cs.y * v.x, cs.x * u.y - cs.y * v.y);
+          *rq = make_double2(cs.y * u.x + cs.x * v.x, cs.y * u.y + cs.x * v.y);
+        };
         double z00, z01, z10, z11;
         int2 pk, pl_;
         if constexpr (MAXB > 0) {
@@ -398,7 +410,7 @@ __device__ bool jacobi(double* A, double* CT, int ld, const JRot& R, const JPlan
             if (pl.bkl[j] >= 0) block(pl.bkl[j] >> 16, pl.bkl[j] & 0xffff, z00, z01, z10, z11, pk, pl_);
 #pragma unroll
           for (int j = 0; j < MAXC; ++j)
-            if (pl.crk[j] >= 0) crow(pl.crk[j] >> 16, pl.crk[j] & 0xffff);
+            if (pl.crk[j] >= 0) crow2(pl.crk[j] >> 16, pl.crk[j] & 0xffff);
         } else {
           const int nw = NT - 64, w = (int)threadIdx.x - 64;
           const int nblk = h * (h - 1) / 2;
@@ -1184,7 +1196,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? 1
 }
 
 // explicit instantiations, launched by the host per size bucket
-template __global__ void k_scf<128, 1, 2, 8, true>(BatchDev, const int*, int, int, int);
-template __global__ void k_scf<256, 1, 3, 11, true>(BatchDev, const int*, int, int, int);
-template __global__ void k_scf<512, 2, 3, 11, true>(BatchDev, const int*, int, int, int);
+template __global__ void k_scf<128, 1, 2, 4, true>(BatchDev, const int*, int, int, int);
+template __global__ void k_scf<256, 1, 3, 6, true>(BatchDev, const int*, int, int, int);
+template __global__ void k_scf<512, 2, 3, 6, true>(BatchDev, const int*, int, int, int);
 template __global__ void k_scf<512, 1, 0, 0, false>(BatchDev, const int*, int, int, int);

@@ -14,8 +14,9 @@
 //                 workgroup folds the partial sums and the panel corrections;
 //                 after NP columns a rank-2 NP matrix-core update
 //   eigenvalues   bisection, one thread per level, warm-started
-//   eigenvectors  inverse iteration, one thread per level, clusters
-//                 re-orthogonalised by one wave each
+//   eigenvectors  inverse iteration, one thread per level; runs of close
+//                 levels orthonormalised in panels (block Gram-Schmidt on
+//                 the matrix cores, Cholesky QR per panel)
 //   back-transf.  compact WY blocks, three matrix-core products per block
 //   occupation    Fermi smearing (one wave), density on the matrix cores
 //   Mulliken      one wave per AO row, then one workgroup for shells / atoms
@@ -535,30 +536,226 @@ __global__ void k_g_runs(GridMol g)
   g.ic[GI_NRUNS] = nr;
 }
 
-// one wave per run: modified Gram-Schmidt twice, then normalise
-__global__ __launch_bounds__(64) void k_g_mgs(GridMol g)
+// ------------------------------------------------ cluster orthogonalisation
+// The vectors of a run of close levels span the run's invariant subspace but
+// are only approximately orthogonal after inverse iteration.  A run gets an
+// orthonormal basis panel by panel (OW columns): the run's earlier columns
+// are projected out of the panel twice (block classical Gram-Schmidt; the
+// products run on the matrix cores, split over row chunks), then the panel
+// is orthonormalised by Cholesky QR, twice, in one workgroup.  A panel whose
+// Gram matrix is numerically singular (a pivot below OPIV of its diagonal)
+// is orthonormalised column by column instead (classical Gram-Schmidt, two
+// passes).
+constexpr int OW = 64;     // panel width
+constexpr int OT = 256;    // threads of the panel kernel
+constexpr int OR = 16;     // rows staged per step in the panel kernel
+constexpr int OCH = 256;   // rows per chunk of the split projection products
+constexpr double OPIV = 1.0e-14;
+
+// LDS of the panel kernel: Gram / Cholesky factor, its inverse, the Gram
+// diagonal, a reduction area and a flag word.
+constexpr int orth_panel_lds_doubles() { return 2 * OW * OW + OW + 4 * OW + 2; }
+
+// Column-by-column classical Gram-Schmidt, two passes, of the w columns at
+// c0 (the fallback for a numerically dependent panel).
+__device__ void orth_panel_cgs(gbl_dbl* Y, int n, int ld, int c0, int w, lds_dbl* part,
+                               lds_dbl* red)
 {
-  const int c = blockIdx.x;
-  if (c >= g.ic[GI_NRUNS]) return;
-  const int n = g.n, ld = g.ld, lane = threadIdx.x;
-  const int a = g.runs[2 * c], b = g.runs[2 * c + 1];
-  double* Y = g.Y;
-  for (int j = a; j < b; ++j) {
-    for (int pass = 0; pass < 2; ++pass)
-      for (int l = a; l < j; ++l) {
-        double s = 0.0;
-        for (int k = lane; k < n; k += 64) s += Y[(int64_t)k * ld + l] * Y[(int64_t)k * ld + j];
-        s = wave_sum(s);
-        for (int k = lane; k < n; k += 64) Y[(int64_t)k * ld + j] -= s * Y[(int64_t)k * ld + l];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  constexpr int NWV = OT / 64;
+  for (int j = 0; j < w; ++j) {
+    for (int pass = 0; pass < 2 && j > 0; ++pass) {
+      double s = 0.0;
+      if (lane < j)
+        for (int k = wv; k < n; k += NWV)
+          s += Y[(int64_t)k * ld + c0 + lane] * Y[(int64_t)k * ld + c0 + j];
+      part[wv * OW + lane] = s;
+      __syncthreads();
+      if (t < OW) {
+        double c = 0.0;
+        for (int q = 0; q < NWV; ++q) c += part[q * OW + t];
+        red[t] = c;
       }
+      __syncthreads();
+      for (int k = t; k < n; k += OT) {
+        double y = Y[(int64_t)k * ld + c0 + j];
+        for (int l = 0; l < j; ++l) y -= red[l] * Y[(int64_t)k * ld + c0 + l];
+        Y[(int64_t)k * ld + c0 + j] = y;
+      }
+      __syncthreads();
+    }
     double s = 0.0;
-    for (int k = lane; k < n; k += 64) {
-      const double y = Y[(int64_t)k * ld + j];
+    for (int k = t; k < n; k += OT) {
+      const double y = Y[(int64_t)k * ld + c0 + j];
       s += y * y;
     }
-    s = wave_sum(s);
-    const double inv = 1.0 / sqrt(s);
-    for (int k = lane; k < n; k += 64) Y[(int64_t)k * ld + j] *= inv;
+    s = block_sum(s, red + OW);
+    const double inv = s > 0.0 ? 1.0 / sqrt(s) : 0.0;
+    for (int k = t; k < n; k += OT) Y[(int64_t)k * ld + c0 + j] *= inv;
+    __syncthreads();
+  }
+}
+
+// Orthonormalises the w <= OW columns at c0 of Y (n rows).  col0 < 0: the
+// first panel of run blockIdx.x (runs of any length), else the panel
+// (col0, w_in).  Dynamic LDS: orth_panel_lds_doubles().
+__global__ __launch_bounds__(OT) void k_g_orth_panel(GridMol g, int col0, int w_in)
+{
+  extern __shared__ double smem[];
+  lds_dbl* G = TA_LDS(smem);   // OW x OW: Gram matrix, then R (upper), staging rows
+  lds_dbl* X = G + OW * OW;    // OW x OW: R^-1 (upper)
+  lds_dbl* dg = X + OW * OW;   // Gram diagonal
+  lds_dbl* red = dg + OW;      // 4 OW
+  lds_dbl* flag = red + 4 * OW;
+  int c0 = col0, w = w_in;
+  if (col0 < 0) {
+    if ((int)blockIdx.x >= g.ic[GI_NRUNS]) return;
+    c0 = g.runs[2 * blockIdx.x];
+    w = min(OW, g.runs[2 * blockIdx.x + 1] - c0);
+  }
+  const int n = g.n, ld = g.ld, t = threadIdx.x;
+  gbl_dbl* Y = TA_GBL(g.Y);
+  // Gram blocks: thread (bi, bj) owns G[4bi..4bi+3][4bj..4bj+3]
+  const int bi = t >> 4, bj = t & 15;
+  const bool gown = 4 * bi < w && 4 * bj < w;
+  // product blocks: thread owns rows 2rq, 2rq+1 of a staged chunk, columns 4cb..4cb+3
+  const int rq = t >> 4, cb = t & 15;
+  for (int pass = 0; pass < 2; ++pass) {
+    double acc[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc[q] = 0.0;
+    for (int k0 = 0; k0 < n; k0 += OR) {
+      const int rows = min(OR, n - k0);
+      for (int idx = t; idx < OR * OW; idx += OT) {
+        const int r = idx >> 6, l = idx & 63;
+        G[idx] = (r < rows && l < w) ? Y[(int64_t)(k0 + r) * ld + c0 + l] : 0.0;
+      }
+      __syncthreads();
+      if (gown) {
+        for (int r = 0; r < OR; ++r) {
+          double a[4], b[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            a[u] = G[r * OW + 4 * bi + u];
+            b[u] = G[r * OW + 4 * bj + u];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[4 * u + v] += a[u] * b[v];
+        }
+      }
+      __syncthreads();
+    }
+    if (gown) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) G[(4 * bi + u) * OW + 4 * bj + v] = acc[4 * u + v];
+    }
+    if (t == 0) flag[0] = 0.0;
+    __syncthreads();
+    if (t < w) dg[t] = G[t * OW + t];
+    __syncthreads();
+    // Cholesky G = R^T R, R in the upper triangle of G
+    for (int p = 0; p < w; ++p) {
+      if (t == 0) {
+        const double dpp = G[p * OW + p];
+        if (!(dpp > OPIV * dg[p])) flag[0] = 1.0;
+        G[p * OW + p] = sqrt(fmax(dpp, 1e-300));
+      }
+      __syncthreads();
+      if (flag[0] != 0.0) break;
+      const double rpp = G[p * OW + p];
+      for (int j = p + 1 + t; j < w; j += OT) G[p * OW + j] /= rpp;
+      __syncthreads();
+      const int m = w - p - 1;
+      for (int e = t; e < m * m; e += OT) {
+        const int i = p + 1 + e / m, j = p + 1 + e % m;
+        if (j >= i) G[i * OW + j] -= G[p * OW + i] * G[p * OW + j];
+      }
+      __syncthreads();
+    }
+    if (flag[0] != 0.0) {
+      orth_panel_cgs(Y, n, ld, c0, w, X, red);
+      return;
+    }
+    // X = R^-1, one column per thread
+    if (t < w) {
+      const int j = t;
+      for (int i = j; i >= 0; --i) {
+        double s = (i == j) ? 1.0 : 0.0;
+        for (int l = i + 1; l <= j; ++l) s -= G[i * OW + l] * X[l * OW + j];
+        X[i * OW + j] = s / G[i * OW + i];
+      }
+      for (int i = j + 1; i < OW; ++i) X[i * OW + j] = 0.0;
+    }
+    for (int idx = t; idx < OW * (OW - w); idx += OT) {
+      const int i = idx / (OW - w), j = w + idx % (OW - w);
+      X[i * OW + j] = 0.0;
+    }
+    __syncthreads();
+    // panel <- panel R^-1, OR rows at a time
+    for (int k0 = 0; k0 < n; k0 += OR) {
+      const int rows = min(OR, n - k0);
+      for (int idx = t; idx < OR * OW; idx += OT) {
+        const int r = idx >> 6, l = idx & 63;
+        G[idx] = (r < rows && l < w) ? Y[(int64_t)(k0 + r) * ld + c0 + l] : 0.0;
+      }
+      __syncthreads();
+      if (2 * rq < rows && 4 * cb < w) {
+        double o0[4] = {0.0, 0.0, 0.0, 0.0}, o1[4] = {0.0, 0.0, 0.0, 0.0};
+        const int lmax = min(w, 4 * cb + 4);
+        for (int l = 0; l < lmax; ++l) {
+          const double s0 = G[(2 * rq) * OW + l], s1 = G[(2 * rq + 1) * OW + l];
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const double x = X[l * OW + 4 * cb + v];
+            o0[v] += s0 * x;
+            o1[v] += s1 * x;
+          }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int j = 4 * cb + v;
+          if (j < w) {
+            Y[(int64_t)(k0 + 2 * rq) * ld + c0 + j] = o0[v];
+            if (2 * rq + 1 < rows) Y[(int64_t)(k0 + 2 * rq + 1) * ld + c0 + j] = o1[v];
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Split product C_c = Q^T P over row chunk c = blockIdx.y (OCH rows), Q the
+// q columns at qa, P the w columns at p0: part[(c * q + i) * OW + j].
+__global__ __launch_bounds__(512) void k_g_orth_part(GridMol g, int qa, int q, int p0, int w,
+                                                     double* part)
+{
+  __shared__ double tile_[2048];
+  const int ld = g.ld;
+  const double* Y = g.Y;
+  const int c = blockIdx.y;
+  const int k0 = c * OCH, k1 = min(g.n, k0 + OCH);
+  big::gemm_tile<512, false, true>(
+      q, w, blockIdx.x * 64, 0, k0, k1,
+      [=] __device__(int i, int k) { return Y[(int64_t)k * ld + qa + i]; },
+      [=] __device__(int k, int j) { return Y[(int64_t)k * ld + p0 + j]; },
+      [=] __device__(int i, int j, double v) { part[((int64_t)c * q + i) * OW + j] = v; },
+      TA_LDS(tile_));
+}
+
+// C[i * OW + j] = sum over the chunks of the split product
+__global__ void k_g_orth_sum(int q, int w, int nch, const double* part, double* C)
+{
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < q * OW; idx += gridDim.x * blockDim.x) {
+    const int i = idx / OW, j = idx % OW;
+    double s = 0.0;
+    if (j < w)
+      for (int c = 0; c < nch; ++c) s += part[((int64_t)c * q + i) * OW + j];
+    C[idx] = s;
   }
 }
 
@@ -1021,7 +1218,40 @@ hipError_t grid_eigensolve(GridMol& g, hipStream_t s, bool warm)
   hipLaunchKernelGGL(k_g_warm, dim3(1), dim3(256), 0, s, g, warm ? 1 : 0);
   hipLaunchKernelGGL(k_g_invit, dim3(grid_blocks(n)), dim3(256), 0, s, g);
   hipLaunchKernelGGL(k_g_runs, dim3(1), dim3(64), 0, s, g);
-  hipLaunchKernelGGL(k_g_mgs, dim3((n + 1) / 2), dim3(64), 0, s, g);
+  {
+    // runs of close levels (the host needs their lengths for the panel loop)
+    std::vector<int> hr(GI_COUNT + n + 2);
+    hipError_t e = hipMemcpyAsync(hr.data(), g.ic, sizeof(int) * (GI_COUNT + n),
+                                  hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    const int nruns = hr[GI_NRUNS];
+    const int* runs = hr.data() + GI_COUNT;
+    const size_t plds = sizeof(double) * orth_panel_lds_doubles();
+    if (nruns > 0) hipLaunchKernelGGL(k_g_orth_panel, dim3(nruns), dim3(OT), plds, s, g, -1, 0);
+    double* part = g.scr;
+    double* C = g.scr + 2 * (int64_t)n * n;
+    const int nch = (n + OCH - 1) / OCH;
+    for (int c = 0; c < nruns; ++c) {
+      const int a = runs[2 * c], b = runs[2 * c + 1];
+      for (int p0 = a + OW; p0 < b; p0 += OW) {
+        const int w = std::min(OW, b - p0), q = p0 - a;
+        for (int pass = 0; pass < 2; ++pass) {
+          hipLaunchKernelGGL(k_g_orth_part, dim3((q + 63) / 64, nch), dim3(512), 0, s, g, a, q, p0,
+                             w, part);
+          hipLaunchKernelGGL(k_g_orth_sum, dim3(grid_blocks((int64_t)q * OW)), dim3(256), 0, s, q,
+                             w, nch, (const double*)part, C);
+          double* Y = g.Y;
+          const double* Cc = C;
+          g_gemm<true, true>(
+              s, n, w, [=] __device__(int i, int k) { return Y[(int64_t)i * ld + a + k]; },
+              [=] __device__(int k, int j) { return Cc[k * OW + j]; }, AllK{q}, NoSkip{},
+              [=] __device__(int i, int j, double val) { Y[(int64_t)i * ld + p0 + j] -= val; });
+        }
+        hipLaunchKernelGGL(k_g_orth_panel, dim3(1), dim3(OT), plds, s, g, p0, w);
+      }
+    }
+  }
   // back-transformation, last block of reflectors first
   constexpr int NB = TA_BIG_NB;
   const int nblk = (nr + NB - 1) / NB;
@@ -1065,6 +1295,9 @@ int grid_scf(GridMol g, hipStream_t s, int max_iter, double* d_energy, double* d
 {
   const int n = g.n, ld = g.ld;
   hipError_t err;
+  if (hipFuncSetAttribute((const void*)k_g_orth_panel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)(sizeof(double) * orth_panel_lds_doubles())) != hipSuccess)
+    return -2;
   hipLaunchKernelGGL(k_g_init, dim3(1), dim3(256), 0, s, g);
   // X = S^-1/2 from the eigenpairs of S
   hipLaunchKernelGGL(k_g_copy_s, dim3(grid_blocks((int64_t)n * n)), dim3(256), 0, s, g);

@@ -191,7 +191,9 @@ struct BatchDev {
 #define TA_BIG_MAXLD 1024
 
 // rows of per-wave column sums kept in LDS by the tridiagonalisation: one per
-// wave, or one per two waves above ld 768 (LDS budget)
+// wave, or one per two waves above ld 768 (LDS budget).  The LDS layout of a
+// size bucket reserves big_npart(ldmax) * ldmax doubles, so every molecule of
+// the bucket uses the count of the bucket's largest ld.
 __host__ __device__ inline int big_npart(int ld) { return ld <= 768 ? TA_BIG_NT / 64 : TA_BIG_NT / 128; }
 
 // scratch per molecule of the global-memory path: inverse-iteration factors
@@ -254,6 +256,11 @@ __host__ __device__ inline ScfLds scf_lds_layout(int ld, int natmax, int nshmax,
   L.total = off;
   return L;
 }
+
+// Levels whose occupation (electrons, both spins) is at or below this do not
+// enter the density; on the global-memory paths their eigenvectors are not
+// formed.  Their total weight is below n * 1e-15 electrons.
+#define TA_OCC_MIN 1.0e-15
 
 #define TA_STATUS_NOT_CONVERGED 1
 #define TA_STATUS_CHOLESKY 2

@@ -12,8 +12,9 @@
 //   eigenvalues    bisection on Sturm counts, four-way multisection (three
 //                  independent recurrences per step), one thread per level
 //   eigenvectors   inverse iteration with a partial-pivoting tridiagonal LU,
-//                  one thread per level; levels closer than ORTOL |T| are
-//                  re-orthogonalised (modified Gram-Schmidt, two passes)
+//                  one thread per level, for the levels that carry occupation
+//                  only; levels closer than ORTOL |T| are re-orthogonalised
+//                  (modified Gram-Schmidt, two passes)
 //   back-transform Z = Q Y in blocks of reflectors (compact WY form,
 //                  I - V T V^T), one thread per column of Y
 //   MO basis       C^T = Z^T X
@@ -514,14 +515,14 @@ __device__ __forceinline__ double start_vector(int i, int k)
 // (pivots^-1, first and second superdiagonals of U, slot-fastest).
 template <int NT>
 __device__ __noinline__ void inverse_iteration(const lds_dbl* d, const lds_dbl* e,
-                                              const lds_dbl* lam, int n, double tnorm, gbl_dbl* Y,
-                                              int ld, gbl_dbl* scr)
+                                              const lds_dbl* lam, int n, int nvec, double tnorm,
+                                              gbl_dbl* Y, int ld, gbl_dbl* scr)
 {
   const double pert = EPS * tnorm + 1e-300;
   constexpr int NIT = 3;
-  for (int i0 = 0; i0 < n; i0 += NT) {
+  for (int i0 = 0; i0 < nvec; i0 += NT) {
     const int i = i0 + threadIdx.x;
-    if (i < n) {
+    if (i < nvec) {
       const int s = threadIdx.x;
       gbl_dbl* u0 = scr;                         // [k][NT]
       gbl_dbl* u1 = scr + (int64_t)ld * NT;
@@ -583,13 +584,14 @@ __device__ __noinline__ void inverse_iteration(const lds_dbl* d, const lds_dbl* 
   __syncthreads();
 }
 
-// Re-orthogonalise eigenvectors of close levels (runs of lam with gaps below
-// ORTOL |T|): one wave per run, modified Gram-Schmidt twice, then normalise.
+// Re-orthogonalise eigenvectors of close levels among the first nvec (runs of
+// lam with gaps below ORTOL |T|): one wave per run, modified Gram-Schmidt
+// twice, then normalise.
 // runs: int scratch of ld entries (LDS); nruns via scal.
 template <int NT>
-__device__ __noinline__ void orthogonalize_clusters(const lds_dbl* lam, int n, double tnorm,
-                                                   gbl_dbl* Y, int ld, lds_int* runs,
-                                                   lds_dbl* scal)
+__device__ __noinline__ void orthogonalize_clusters(const lds_dbl* lam, int n, int nvec,
+                                                   double tnorm, gbl_dbl* Y, int ld,
+                                                   lds_int* runs, lds_dbl* scal)
 {
   constexpr int NW = NT / 64;
   const int lane = threadIdx.x & 63, wid = wave_id();
@@ -597,9 +599,9 @@ __device__ __noinline__ void orthogonalize_clusters(const lds_dbl* lam, int n, d
     int nr = 0;
     const double gap = ORTOL * tnorm;
     int i = 0;
-    while (i < n) {
+    while (i < nvec) {
       int j = i + 1;
-      while (j < n && lam[j] - lam[j - 1] <= gap) ++j;
+      while (j < nvec && lam[j] - lam[j - 1] <= gap) ++j;
       if (j - i > 1) {
         runs[2 * nr] = i;
         runs[2 * nr + 1] = j;
@@ -635,7 +637,8 @@ __device__ __noinline__ void orthogonalize_clusters(const lds_dbl* lam, int n, d
 }
 
 // -------------------------------------------------------- back-transform
-// Y <- Q Y, Q = H_0 H_1 ... H_{n-3}, reflectors from tridiagonalize().  Blocks
+// Y <- Q Y on the first ncol columns of Y, Q = H_0 H_1 ... H_{n-3},
+// reflectors from tridiagonalize().  Blocks
 // of NB reflectors, last block first.  A block is I - V T V^T with T^-1 =
 // diag(1/tau) + strict_upper(V^T V) (the reflectors satisfy tau = 2 / v.v), so
 //   W = V^T Y,  solve T^-1 W' = W,  Y -= V W'
@@ -643,8 +646,8 @@ __device__ __noinline__ void orthogonalize_clusters(const lds_dbl* lam, int n, d
 // W: NB x ld doubles of global scratch; sinv, tm: NB x NB doubles of LDS.
 template <int NT, int NB>
 __device__ __noinline__ void back_transform(const gbl_dbl* A, const lds_dbl* tau, int n, int ld,
-                                            gbl_dbl* Y, gbl_dbl* W, lds_dbl* sinv, lds_dbl* tm,
-                                            lds_dbl* tile)
+                                            int ncol, gbl_dbl* Y, gbl_dbl* W, lds_dbl* sinv,
+                                            lds_dbl* tm, lds_dbl* tile)
 {
   const int nr = n - 2;
   const int nblk = (nr + NB - 1) / NB;
@@ -675,9 +678,9 @@ __device__ __noinline__ void back_transform(const gbl_dbl* A, const lds_dbl* tau
           sinv[i * NB + j] = s;
         },
         tile);
-    // W = V^T Y[r0:, :]
+    // W = V^T Y[r0:, :ncol]
     gemm<NT, false, true>(
-        NB, n, [=](int i, int k) { return vel(k, i); },
+        NB, ncol, [=](int i, int k) { return vel(k, i); },
         [=](int k, int j) { return Y[(int64_t)(r0 + k) * ld + j]; }, all_k, no_skip,
         [=](int i, int j, double val) { W[(int64_t)i * ld + j] = val; }, tile);
     // T = (T^-1)^-1, upper triangular: one thread per column, back substitution
@@ -698,16 +701,16 @@ __device__ __noinline__ void back_transform(const gbl_dbl* A, const lds_dbl* tau
     __syncthreads();
     // W' = T W (in place: a tile reads its own columns of W before storing them)
     gemm<NT, true, true>(
-        NB, n, [=](int i, int k) { return tm[i * NB + k]; },
+        NB, ncol, [=](int i, int k) { return tm[i * NB + k]; },
         [=](int k, int j) { return W[(int64_t)k * ld + j]; },
         [](int, int, int* ka, int* kb) {
           *ka = 0;
           *kb = NB;
         },
         no_skip, [=](int i, int j, double val) { W[(int64_t)i * ld + j] = val; }, tile);
-    // Y[r0:, :] -= V W'
+    // Y[r0:, :ncol] -= V W'
     gemm<NT, true, true>(
-        m, n, [=](int i, int k) { return vel(i, k); },
+        m, ncol, [=](int i, int k) { return vel(i, k); },
         [=](int k, int j) { return W[(int64_t)k * ld + j]; },
         [](int, int, int* ka, int* kb) {
           *ka = 0;

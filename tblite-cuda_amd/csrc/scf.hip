@@ -441,26 +441,29 @@ __device__ bool jacobi(double* A, double* CT, int ld, const JRot& R, const JPlan
 
 
 // ------------------------------------------- global-memory eigensolve
-// F (in A, full) -> eigenvalues ev (ascending) and MO rows CT = Z^T X of the
-// generalized problem F C = S C e, with X = L^-1 (lower triangular).  The
+// F (in A, full) -> eigenvalues ev (ascending) of the generalized problem
+// F C = S C e, with X = L^-1 (lower triangular): A = X F X^T, reduced to
+// tridiagonal form in place (reflectors in the lower triangle of A).  The
 // eigensolver region er (LDS) holds d, e, tau, e^2 and the work vectors of
-// the tridiagonalisation, followed by the GEMM tiles / reflector block.
+// the tridiagonalisation, followed by the GEMM tiles / reflector block; d, e
+// and tau stay there for big_eigenvectors.  |T| ends in scal[0].  npart:
+// per-wave column-sum partials of the tridiagonalisation, npart * ld doubles
+// after the work vectors; the LDS layout reserves big_npart(ldmax) * ldmax,
+// so npart follows the bucket's largest molecule, not this one.
 template <int NT>
-__device__ __noinline__ void big_eigensolve(double* A_, const double* X_, double* Y_, double* CT_,
-                                            double* scr_, int n, int ld, double* er_, double* ev_,
-                                            double* red_, double* scal_, int* runs_,
-                                            const double* prev_, double* warm_, PhaseClock& clk)
+__device__ __noinline__ void big_eigenvalues(double* A_, const double* X_, double* Y_,
+                                             double* scr_, int n, int ld, int npart, double* er_,
+                                             double* ev_, double* red_, double* scal_,
+                                             const double* prev_, double* warm_, PhaseClock& clk)
 {
   gbl_dbl* A = TA_GBL(A_);
   const gbl_dbl* X = TA_GBL(X_);
   gbl_dbl* Y = TA_GBL(Y_);
-  gbl_dbl* CT = TA_GBL(CT_);
   gbl_dbl* scr = TA_GBL(scr_);
   lds_dbl* er = TA_LDS(er_);
   lds_dbl* ev = TA_LDS(ev_);
   lds_dbl* red = TA_LDS(red_);
   lds_dbl* scal = TA_LDS(scal_);
-  lds_int* runs = TA_LDSI(runs_);
   lds_dbl* d = er;
   lds_dbl* e = er + ld;
   lds_dbl* tau = er + 2 * ld;
@@ -493,7 +496,7 @@ __device__ __noinline__ void big_eigensolve(double* A_, const double* X_, double
   if (threadIdx.x == 0)
     for (int q = 0; q < 5; ++q) red[16 + q] = 0.0;
   big::tridiagonalize_blocked<NT, TA_BIG_NP>(A, n, ld, d, e, tau, vv, vn, yy, yrow, ww, tmp, pvt,
-                                             pvt + (int64_t)TA_BIG_NP * ld, big_npart(ld), red,
+                                             pvt + (int64_t)TA_BIG_NP * ld, npart, red,
                                              scal, clk.on ? red + 16 : nullptr);
   if (clk.on) {
     for (int q = 0; q < 5; ++q) clk.acc[TA_PROF_TRI_COLUMN + q] += (long long)red[16 + q];
@@ -501,25 +504,53 @@ __device__ __noinline__ void big_eigensolve(double* A_, const double* X_, double
   }
   clk.lap(TA_PROF_TRIDIAG);
   big::bisect_all<NT>(d, e, e2, n, ev, TA_LDS(prev_), TA_LDS(warm_), red, scal);
-  const double tnorm = scal[0];
   __syncthreads();
   clk.lap(TA_PROF_BISECT);
-  big::inverse_iteration<NT>(d, e, ev, n, tnorm, Y, ld, scr);
+}
+
+// Eigenvectors of the first nvec levels (those that carry occupation) from
+// the tridiagonal form left by big_eigenvalues: inverse iteration, clusters,
+// back-transform, then the MO rows CT[0:nvec] = Z^T X.  runs_: ld ints of
+// LDS scratch.
+template <int NT>
+__device__ __noinline__ void big_eigenvectors(const double* A_, const double* X_, double* Y_,
+                                              double* CT_, double* scr_, int n, int ld, int nvec,
+                                              double* er_, const double* ev_, double* scal_,
+                                              int* runs_, PhaseClock& clk)
+{
+  const gbl_dbl* A = TA_GBL(A_);
+  const gbl_dbl* X = TA_GBL(X_);
+  gbl_dbl* Y = TA_GBL(Y_);
+  gbl_dbl* CT = TA_GBL(CT_);
+  gbl_dbl* scr = TA_GBL(scr_);
+  lds_dbl* er = TA_LDS(er_);
+  const lds_dbl* ev = TA_LDS(ev_);
+  lds_dbl* scal = TA_LDS(scal_);
+  lds_int* runs = TA_LDSI(runs_);
+  const lds_dbl* d = er;
+  const lds_dbl* e = er + ld;
+  const lds_dbl* tau = er + 2 * ld;
+  lds_dbl* tmp = er + 9 * ld;
+  const double tnorm = scal[0];
+  __syncthreads();
+  gbl_dbl* wbt = scr + 3 * (int64_t)ld * TA_BIG_NT;
+  big::inverse_iteration<NT>(d, e, ev, n, nvec, tnorm, Y, ld, scr);
   clk.lap(TA_PROF_INVIT);
-  big::orthogonalize_clusters<NT>(ev, n, tnorm, Y, ld, runs, scal);
+  big::orthogonalize_clusters<NT>(ev, n, nvec, tnorm, Y, ld, runs, scal);
   clk.lap(TA_PROF_CLUSTERS);
-  big::back_transform<NT, TA_BIG_NB>(A, tau, n, ld, Y, wbt, tmp + 2048,
+  big::back_transform<NT, TA_BIG_NB>(A, tau, n, ld, nvec, Y, wbt, tmp + 2048,
                                      tmp + 2048 + TA_BIG_NB * TA_BIG_NB, tmp);
   clk.lap(TA_PROF_BACKTR);
   // CT = Z^T X; X[k][a] = 0 for a > k
   big::gemm<NT, false, true>(
-      n, n, [=](int i, int k) { return Y[(int64_t)k * ld + i]; },
+      nvec, n, [=](int i, int k) { return Y[(int64_t)k * ld + i]; },
       [=](int k, int j) { return X[(int64_t)k * ld + j]; },
       [n](int, int tj, int* k0, int* k1) {
         *k0 = tj * 64;
         *k1 = n;
       },
-      no_skip, [=](int i, int j, double val) { CT[(int64_t)i * ld + j] = val; }, tmp);
+      [](int, int) { return false; },
+      [=](int i, int j, double val) { CT[(int64_t)i * ld + j] = val; }, tmp);
   clk.lap(TA_PROF_JACOBI);
 }
 
@@ -1006,9 +1037,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? 1
       }
       __syncthreads();
     } else {
-      // levels of the previous step (emo_s) warm-start the bisection
-      big_eigensolve<NT>(A, C, Y, CT, scr, n, ld, smem + L.eig, ev, red, scal, rank, emo_s,
-                         red + 24, clk);
+      // levels of the previous step (emo_s) warm-start the bisection; the
+      // eigenvectors follow once the occupations are known
+      big_eigenvalues<NT>(A, C, Y, scr, n, ld, big_npart(ldmax), smem + L.eig, ev, red, scal,
+                          emo_s, red + 24, clk);
       for (int k = tid; k < n; k += NT) {
         rank[k] = k;
         inv[k] = k;
@@ -1029,17 +1061,25 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? 1
     }
     __syncthreads();
     ts = scal[3];
+    // levels with an occupation above TA_OCC_MIN enter the density; the
+    // occupations fall with the level, so on the global-memory path (levels
+    // in ascending order) these are a leading range
     if (tid == 0) {
       int c = 0;
       for (int r = 0; r < n; ++r) {
         const int k = inv[r];
-        if (focc_u[k] != 0.0) occl[c++] = k;
+        if (focc_u[k] > TA_OCC_MIN) occl[c++] = k;
       }
       scal[4] = (double)c;
     }
     __syncthreads();
     last_nocc = (int)scal[4];
     clk.lap(TA_PROF_OCC);
+    if constexpr (!MATS_LDS) {
+      int nvec = 0;
+      if (last_nocc > 0) nvec = occl[last_nocc - 1] + 1;
+      big_eigenvectors<NT>(A, C, Y, CT, scr, n, ld, nvec, smem + L.eig, ev, scal, rank, clk);
+    }
 
     // ------------------------------------------------ density -> A
     if constexpr (MATS_LDS) density<NT>(C, focc_u, occl, last_nocc, A, ld);
@@ -1179,7 +1219,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? 1
     double* co = b.coeff + moff;
     for (int idx = tid; idx < n * n; idx += NT) {
       const int i = idx / n, r = idx - (idx / n) * n;
-      co[idx] = CT[inv[r] * ld + i];
+      // the global-memory path forms the occupied levels' coefficients only
+      co[idx] = (MATS_LDS || focc_u[inv[r]] > TA_OCC_MIN) ? CT[inv[r] * ld + i] : 0.0;
     }
   }
   if (clk.on) {

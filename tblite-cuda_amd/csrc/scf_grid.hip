@@ -453,11 +453,11 @@ __global__ void k_g_warm(GridMol g, int warm_ok)
 
 // ---------------------------------------------------------- eigenvectors
 // one thread per level; factors in scr ([k][i], three planes of n x n)
-__global__ __launch_bounds__(256) void k_g_invit(GridMol g)
+__global__ __launch_bounds__(256) void k_g_invit(GridMol g, int nvec)
 {
   const int n = g.n, ld = g.ld;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= nvec) return;
   const double tnorm = g.sc[GS_TNORM];
   const double pert = big::EPS * tnorm + 1e-300;
   const int64_t pl = (int64_t)n * n;
@@ -516,16 +516,16 @@ __global__ __launch_bounds__(256) void k_g_invit(GridMol g)
   for (int k = 0; k < n; ++k) Y[(int64_t)k * ld + i] *= scale;
 }
 
-// runs of close levels (gap <= ORTOL |T|) with more than one member
-__global__ void k_g_runs(GridMol g)
+// runs of close levels (gap <= ORTOL |T|) with more than one member among
+// the first nvec
+__global__ void k_g_runs(GridMol g, int nvec)
 {
   if (threadIdx.x != 0) return;
-  const int n = g.n;
   const double gap = big::ORTOL * g.sc[GS_TNORM];
   int nr = 0, i = 0;
-  while (i < n) {
+  while (i < nvec) {
     int j = i + 1;
-    while (j < n && g.lam[j] - g.lam[j - 1] <= gap) ++j;
+    while (j < nvec && g.lam[j] - g.lam[j - 1] <= gap) ++j;
     if (j - i > 1) {
       g.runs[2 * nr] = i;
       g.runs[2 * nr + 1] = j;
@@ -810,7 +810,7 @@ __global__ __launch_bounds__(64) void k_g_fermi(GridMol g)
   if (threadIdx.x == 0) {
     int last = 0;
     for (int r = 0; r < n; ++r)
-      if (g.focc[r] != 0.0) last = r + 1;
+      if (g.focc[r] > TA_OCC_MIN) last = r + 1;
     g.ic[GI_NOCC] = last;
     g.sc[GS_TS] = ts;
   }
@@ -1163,20 +1163,23 @@ __global__ void k_g_results(GridMol g, double* emo, double* focc, double* coeff)
     focc[r] = g.focc[r];
   }
   if (coeff) {
+    // only the occupied levels' coefficients are formed; the rest read as 0
+    const int nocc = g.ic[GI_NOCC];
     const int64_t tot = (int64_t)n * n;
     for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < tot;
          idx += (int64_t)gridDim.x * blockDim.x) {
       const int i = (int)(idx / n), r = (int)(idx - (idx / n) * n);
-      coeff[idx] = g.CT[(int64_t)r * g.ld + i];
+      coeff[idx] = r < nocc ? g.CT[(int64_t)r * g.ld + i] : 0.0;
     }
   }
 }
 
 int grid_blocks(int64_t work, int per = 256) { return (int)std::min<int64_t>((work + per - 1) / per, 8192); }
 
-// Eigenvalues (g.lam, ascending) and eigenvectors (g.Y columns) of the
-// symmetric matrix in g.A (lower triangle used; destroyed).
-hipError_t grid_eigensolve(GridMol& g, hipStream_t s, bool warm)
+// Eigenvalues (g.lam, ascending) of the symmetric matrix in g.A (lower
+// triangle used): reduced to tridiagonal form in place (d, e, tau and the
+// reflectors in A), then bisection.
+hipError_t grid_eigenvalues(GridMol& g, hipStream_t s, bool warm)
 {
   const int n = g.n, ld = g.ld;
   const int nr = n - 2;
@@ -1216,8 +1219,18 @@ hipError_t grid_eigensolve(GridMol& g, hipStream_t s, bool warm)
   hipLaunchKernelGGL(k_g_gersh, dim3(1), dim3(GT), 0, s, g);
   hipLaunchKernelGGL(k_g_bisect, dim3(grid_blocks(n)), dim3(256), 0, s, g, warm ? 1 : 0);
   hipLaunchKernelGGL(k_g_warm, dim3(1), dim3(256), 0, s, g, warm ? 1 : 0);
-  hipLaunchKernelGGL(k_g_invit, dim3(grid_blocks(n)), dim3(256), 0, s, g);
-  hipLaunchKernelGGL(k_g_runs, dim3(1), dim3(64), 0, s, g);
+  return hipGetLastError();
+}
+
+// Eigenvectors of the first nvec levels (g.Y columns 0..nvec-1) from the
+// tridiagonal form left by grid_eigenvalues.
+hipError_t grid_eigenvectors(GridMol& g, hipStream_t s, int nvec)
+{
+  const int n = g.n, ld = g.ld;
+  const int nr = n - 2;
+  if (nvec <= 0) return hipGetLastError();
+  hipLaunchKernelGGL(k_g_invit, dim3(grid_blocks(nvec)), dim3(256), 0, s, g, nvec);
+  hipLaunchKernelGGL(k_g_runs, dim3(1), dim3(64), 0, s, g, nvec);
   {
     // runs of close levels (the host needs their lengths for the panel loop)
     std::vector<int> hr(GI_COUNT + n + 2);
@@ -1270,15 +1283,15 @@ hipError_t grid_eigensolve(GridMol& g, hipStream_t s, bool warm)
       return (j < nb && rr >= j && tau[k0 + j] != 0.0) ? A[(int64_t)(r0 + rr) * ld + k0 + j] : 0.0;
     };
     g_gemm<false, true>(
-        s, NB, n, [=] __device__(int i, int k) { return vel(k, i); },
+        s, NB, nvec, [=] __device__(int i, int k) { return vel(k, i); },
         [=] __device__(int k, int j) { return Y[(int64_t)(r0 + k) * ld + j]; }, AllK{m}, NoSkip{},
         [=] __device__(int i, int j, double val) { W[(int64_t)i * ld + j] = val; });
     g_gemm<true, true>(
-        s, NB, n, [=] __device__(int i, int k) { return tm[i * NB + k]; },
+        s, NB, nvec, [=] __device__(int i, int k) { return tm[i * NB + k]; },
         [=] __device__(int k, int j) { return W[(int64_t)k * ld + j]; }, AllK{NB}, NoSkip{},
         [=] __device__(int i, int j, double val) { W[(int64_t)i * ld + j] = val; });
     g_gemm<true, true>(
-        s, m, n, [=] __device__(int i, int k) { return vel(i, k); },
+        s, m, nvec, [=] __device__(int i, int k) { return vel(i, k); },
         [=] __device__(int k, int j) { return W[(int64_t)k * ld + j]; }, AllK{NB}, NoSkip{},
         [=] __device__(int i, int j, double val) { Y[(int64_t)(r0 + i) * ld + j] -= val; });
   }
@@ -1301,7 +1314,8 @@ int grid_scf(GridMol g, hipStream_t s, int max_iter, double* d_energy, double* d
   hipLaunchKernelGGL(k_g_init, dim3(1), dim3(256), 0, s, g);
   // X = S^-1/2 from the eigenpairs of S
   hipLaunchKernelGGL(k_g_copy_s, dim3(grid_blocks((int64_t)n * n)), dim3(256), 0, s, g);
-  if ((err = grid_eigensolve(g, s, false)) != hipSuccess) return -2;
+  if ((err = grid_eigenvalues(g, s, false)) != hipSuccess) return -2;
+  if ((err = grid_eigenvectors(g, s, n)) != hipSuccess) return -2;
   hipLaunchKernelGGL(k_g_check_overlap, dim3(1), dim3(64), 0, s, g);
   {
     const double* Y = g.Y;
@@ -1346,22 +1360,24 @@ int grid_scf(GridMol g, hipStream_t s, int max_iter, double* d_energy, double* d
           [=] __device__(int k, int j) { return X[(int64_t)k * ld + j]; }, AllK{n}, UpperSkip{},
           [=] __device__(int i, int j, double val) { A[(int64_t)i * ld + j] = val; });
     }
-    if ((err = grid_eigensolve(g, s, iter > 1)) != hipSuccess) return -2;
-    // MO rows CT = Z^T X
-    {
-      const double* Z = g.Y;
-      const double* X = g.X;
-      double* CT = g.CT;
-      g_gemm<false, true>(
-          s, n, n, [=] __device__(int i, int k) { return Z[(int64_t)k * ld + i]; },
-          [=] __device__(int k, int j) { return X[(int64_t)k * ld + j]; }, AllK{n}, NoSkip{},
-          [=] __device__(int i, int j, double val) { CT[(int64_t)i * ld + j] = val; });
-    }
+    if ((err = grid_eigenvalues(g, s, iter > 1)) != hipSuccess) return -2;
+    // occupations from the levels; only occupied levels need eigenvectors
     hipLaunchKernelGGL(k_g_fermi, dim3(1), dim3(64), 0, s, g);
     if (hipMemcpyAsync(hbuf, g.ic, sizeof(int) * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
       return -2;
     const int nocc = hbuf[GI_NOCC];
+    if ((err = grid_eigenvectors(g, s, nocc)) != hipSuccess) return -2;
+    // MO rows CT = Z^T X of the occupied levels
+    {
+      const double* Z = g.Y;
+      const double* X = g.X;
+      double* CT = g.CT;
+      g_gemm<false, true>(
+          s, nocc, n, [=] __device__(int i, int k) { return Z[(int64_t)k * ld + i]; },
+          [=] __device__(int k, int j) { return X[(int64_t)k * ld + j]; }, AllK{n}, NoSkip{},
+          [=] __device__(int i, int j, double val) { CT[(int64_t)i * ld + j] = val; });
+    }
     // density P = sum_k f_k c_k c_k^T into Y (lower tiles mirrored)
     {
       const double* CT = g.CT;

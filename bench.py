@@ -102,6 +102,25 @@ def replay(srv, mols, arrivals):
     return lat, time.perf_counter() - t0 - arrivals[0]
 
 
+AO_PER_ELEMENT = {1: 1, 6: 4, 7: 4, 8: 4, 16: 9}
+
+
+def latency_row(lat, mols, offered, span, arrivals):
+    """p50 / p99 over all requests and per size class (LDS-resident sizes,
+    padded AO dimension <= 96, and larger), and the drain time after the last
+    arrival (close to the slowest latency when the server kept up)."""
+    nao = np.array([sum(AO_PER_ELEMENT[int(z)] for z in m[0]) for m in mols])
+    small = ((nao + 3) // 4 * 4) <= 96
+    pct = lambda a, q: round(float(np.percentile(a, q)), 2) if len(a) else None
+    return {"offered_per_s": round(float(offered), 2), "requests": len(mols),
+            "throughput_per_s": round(len(mols) / span, 2),
+            "arrival_window_s": round(float(arrivals[-1] - arrivals[0]), 2),
+            "drain_s": round(float(span - (arrivals[-1] - arrivals[0])), 2),
+            "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+            "small_p50_ms": pct(lat[small], 50), "small_p99_ms": pct(lat[small], 99),
+            "large_p50_ms": pct(lat[~small], 50), "large_p99_ms": pct(lat[~small], 99)}
+
+
 def serve_bench(args):
     """Config 4: closed-loop saturation throughput, then open-loop Poisson
     arrivals at fractions of it with p50 / p99 latency, beside the plain
@@ -129,10 +148,7 @@ def serve_bench(args):
         arr = np.cumsum(rng.exponential(1.0 / rate, m))
         arr -= arr[0]
         lat, span = replay(srv, mols[:m], arr)
-        row = {"offered_per_s": round(rate, 2), "requests": m,
-               "throughput_per_s": round(m / span, 2),
-               "p50_ms": round(float(np.percentile(lat, 50)), 2),
-               "p99_ms": round(float(np.percentile(lat, 99)), 2)}
+        row = latency_row(lat, mols[:m], rate, span, arr)
         sweep.append(row)
         print(f"[bench] rate {rate:.1f}/s: {row}", file=sys.stderr, flush=True)
     stats = srv.stats()

@@ -44,6 +44,17 @@ struct DevParams {
 };
 
 // Pointers into one batch's device buffers (passed by value to kernels).
+// Completion record of one molecule, written by the SCF kernel into pinned
+// host memory when the molecule ends (flag last, with a system-scope release)
+// so a host loop can report results before the whole batch has finished.
+struct TaDoneRecord {
+  double energy;
+  int iterations;
+  int status;
+  int flag;
+  int pad;
+};
+
 struct BatchDev {
   int nmol;
   int max_iter;
@@ -152,6 +163,8 @@ struct BatchDev {
 
   // optional per-molecule phase clocks of the SCF kernel (TA_PROF_SLOTS each), may be null
   long long* prof;
+  // optional completion records (one per molecule, pinned host memory), may be null
+  TaDoneRecord* done;
 };
 
 // SCF phase clock slots (wall clock ticks at 100 MHz; last two are counts)

@@ -247,6 +247,9 @@ struct tblite_amd_context_s {
   // molecules the oracle handles quickly)
   int grid_min_ld = TA_BIG_MAXLD;
   int verbose = 0;  // TBLITE_AMD_VERBOSE=1: progress lines of long runs on stderr
+  // every kernel on the main stream (the serving slots: one stream per slot,
+  // so concurrent slots do not queue behind each other's kernels)
+  bool single_stream = false;
 };
 
 struct tblite_amd_structure_s {
@@ -828,11 +831,18 @@ int run_batch(tblite_amd_context ctx, Calculator& c, const double* xyz, double a
                        db.d_work_kind, db.d_work_start);
   HIP_TRY(ctx, hipGetLastError());
   HIP_TRY(ctx, hipEventRecord(ctx->ev[3], st));
-  // SCF buckets on the auxiliary streams, largest bucket first
-  for (int s = 0; s < kNumStreams; ++s) HIP_TRY(ctx, hipStreamWaitEvent(ctx->aux[s], ctx->ev[3], 0));
-  for (size_t k = 0; k < pl.buckets.size(); ++k) {
+  // SCF buckets on the auxiliary streams, largest bucket first; on a
+  // single-stream context one after the other, smallest first (they finish
+  // in milliseconds and report early)
+  const bool one = ctx->single_stream;
+  if (!one)
+    for (int s = 0; s < kNumStreams; ++s)
+      HIP_TRY(ctx, hipStreamWaitEvent(ctx->aux[s], ctx->ev[3], 0));
+  const int nbk = (int)pl.buckets.size();
+  for (int q = 0; q < nbk; ++q) {
+    const int k = one ? nbk - 1 - q : q;
     const Plan::Bucket& bk = pl.buckets[k];
-    hipStream_t as = ctx->aux[k % kNumStreams];
+    hipStream_t as = one ? st : ctx->aux[k % kNumStreams];
     const int* mols = db.d_bucket_mols[k];
     const int cnt = (int)bk.mols.size();
     int rc;
@@ -858,10 +868,11 @@ int run_batch(tblite_amd_context ctx, Calculator& c, const double* xyz, double a
     int rc = run_grid_molecule(ctx, c, m);
     if (rc) return rc;
   }
-  for (int s = 0; s < kNumStreams; ++s) {
-    HIP_TRY(ctx, hipEventRecord(ctx->aux_done[s], ctx->aux[s]));
-    HIP_TRY(ctx, hipStreamWaitEvent(st, ctx->aux_done[s], 0));
-  }
+  if (!one)
+    for (int s = 0; s < kNumStreams; ++s) {
+      HIP_TRY(ctx, hipEventRecord(ctx->aux_done[s], ctx->aux[s]));
+      HIP_TRY(ctx, hipStreamWaitEvent(st, ctx->aux_done[s], 0));
+    }
   HIP_TRY(ctx, hipEventRecord(ctx->ev[4], st));
   if (!wait) return TBLITE_AMD_OK;
   return finish_batch(ctx);
@@ -953,6 +964,8 @@ int check_batch(tblite_amd_context ctx, int imol)
 }  // namespace
 
 // ================================================================= C ABI
+static tblite_amd_context make_context(int device, bool single_stream);
+
 extern "C" {
 
 int tblite_amd_param_default(tblite_amd_param* param)
@@ -962,7 +975,13 @@ int tblite_amd_param_default(tblite_amd_param* param)
   return TBLITE_AMD_OK;
 }
 
-tblite_amd_context tblite_amd_new_context(int device)
+tblite_amd_context tblite_amd_new_context(int device) { return make_context(device, false); }
+
+}  // extern "C"
+
+// A context on one device; single_stream: no auxiliary streams, every kernel
+// on the main stream.
+static tblite_amd_context make_context(int device, bool single_stream)
 {
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count < 1) return nullptr;
@@ -972,8 +991,9 @@ tblite_amd_context tblite_amd_new_context(int device)
   tblite_amd_context ctx = new (std::nothrow) tblite_amd_context_s();
   if (!ctx) return nullptr;
   ctx->device = device;
+  ctx->single_stream = single_stream;
   bool ok = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) == hipSuccess;
-  for (int s = 0; s < kNumStreams && ok; ++s)
+  for (int s = 0; s < kNumStreams && ok && !single_stream; ++s)
     ok = hipStreamCreateWithFlags(&ctx->aux[s], hipStreamNonBlocking) == hipSuccess &&
          hipEventCreateWithFlags(&ctx->aux_done[s], hipEventDisableTiming) == hipSuccess;
   for (int e = 0; e < 8 && ok; ++e) ok = hipEventCreate(&ctx->ev[e]) == hipSuccess;
@@ -993,6 +1013,8 @@ tblite_amd_context tblite_amd_new_context(int device)
   }
   return ctx;
 }
+
+extern "C" {
 
 void tblite_amd_delete_context(tblite_amd_context* pctx)
 {
@@ -1502,6 +1524,7 @@ struct ServeRequest {
   int uhf;
   double t_submit;
   size_t bytes;
+  bool big;  // above the LDS-resident size
 };
 
 struct ServeResult {
@@ -1516,7 +1539,11 @@ struct ServeSlot {
   tblite_amd_context ctx = nullptr;
   std::vector<ServeRequest> reqs;
   std::vector<double> xyz;  // host coordinates of the running batch (kept until it ends)
+  std::vector<char> reported;       // per request of the running batch
+  TaDoneRecord* rec = nullptr;      // completion records, pinned host memory (max_batch)
+  TaDoneRecord* rec_dev = nullptr;  // the same memory as the device sees it
   bool busy = false;
+  bool small_class = false;  // takes only molecules of the LDS-resident sizes
 };
 
 double host_ms()
@@ -1527,7 +1554,7 @@ double host_ms()
 
 // Upper estimate of the device workspace a molecule adds to a batch (all
 // per-molecule arrays of upload_plan, alignment slack included).
-size_t request_bytes(const Calculator& c, int nat, const int* num)
+size_t request_bytes(const Calculator& c, int nat, const int* num, int* nao_out)
 {
   int nsh = 0, nao = 0;
   for (int i = 0; i < nat; ++i) {
@@ -1545,6 +1572,7 @@ size_t request_bytes(const Calculator& c, int nat, const int* num)
   d += (int64_t)nat * (nat + 1) / 2 + 24 * (int64_t)nat + 6 * nsh + 4 * nao;
   d += 4 * ndim + 2 * ndim * mem + mem * mem + mem * (mem + 1) + 2 * mem + 8 + TA_PROF_SLOTS;
   if (ld > kLdsMaxLd) d += 4 * ld * ld + big_scratch_doubles((int)ld);
+  if (nao_out) *nao_out = nao;
   return (size_t)d * 8 + 64 * 256;
 }
 
@@ -1554,7 +1582,8 @@ struct tblite_amd_server_s {
   tblite_amd_context owner = nullptr;
   tblite_amd_calculator calc = nullptr;
   std::vector<ServeSlot> slots;
-  std::deque<ServeRequest> queue;
+  // waiting requests by size class: LDS-resident sizes, larger ones
+  std::deque<ServeRequest> queue_small, queue_big;
   std::deque<ServeResult> done;
   size_t slot_bytes = 0;
   int max_batch = 4096;
@@ -1578,8 +1607,12 @@ tblite_amd_server tblite_amd_new_server(tblite_amd_context ctx, tblite_amd_calcu
   srv->calc = calc;
   srv->slot_bytes = (size_t)(workspace_bytes / slots);
   srv->slots.resize(slots);
-  for (auto& sl : srv->slots) {
-    sl.ctx = tblite_amd_new_context(ctx->device);
+  for (size_t k = 0; k < srv->slots.size(); ++k) {
+    ServeSlot& sl = srv->slots[k];
+    // with two or more slots the first one serves small molecules only, so
+    // they never wait behind a batch of large ones
+    sl.small_class = slots >= 2 && k == 0;
+    sl.ctx = make_context(ctx->device, true);
     bool ok = sl.ctx != nullptr;
     if (ok) {
       // reserve the slot's workspace once: growing it later would free device
@@ -1588,6 +1621,10 @@ tblite_amd_server tblite_amd_new_server(tblite_amd_context ctx, tblite_amd_calcu
       ok = hipMalloc(&ar.base, srv->slot_bytes) == hipSuccess;
       if (ok) ar.cap = srv->slot_bytes;
     }
+    if (ok)
+      ok = hipHostMalloc((void**)&sl.rec, sizeof(TaDoneRecord) * srv->max_batch,
+                         hipHostMallocCoherent) == hipSuccess &&
+           hipHostGetDevicePointer((void**)&sl.rec_dev, sl.rec, 0) == hipSuccess;
     if (!ok) {
       set_error(ctx, "new_server: cannot create slot context / reserve %zu bytes", srv->slot_bytes);
       tblite_amd_delete_server(&srv);
@@ -1600,8 +1637,10 @@ tblite_amd_server tblite_amd_new_server(tblite_amd_context ctx, tblite_amd_calcu
 void tblite_amd_delete_server(tblite_amd_server* psrv)
 {
   if (!psrv || !*psrv) return;
-  for (auto& sl : (*psrv)->slots)
+  for (auto& sl : (*psrv)->slots) {
     if (sl.ctx) tblite_amd_delete_context(&sl.ctx);
+    if (sl.rec) (void)hipHostFree(sl.rec);
+  }
   delete *psrv;
   *psrv = nullptr;
 }
@@ -1615,7 +1654,8 @@ int tblite_amd_server_submit(tblite_amd_server srv, int nat, const int* num, con
     set_error(ctx, "server_submit: invalid molecule");
     return TBLITE_AMD_ERR_ARGUMENT;
   }
-  const size_t bytes = request_bytes(srv->calc->c, nat, num);
+  int nao = 0;
+  const size_t bytes = request_bytes(srv->calc->c, nat, num, &nao);
   if (bytes == 0) {
     set_error(ctx, "server_submit: element without parameters");
     return TBLITE_AMD_ERR_ARGUMENT;
@@ -1634,29 +1674,63 @@ int tblite_amd_server_submit(tblite_amd_server srv, int nat, const int* num, con
   r.uhf = uhf;
   r.t_submit = host_ms();
   r.bytes = bytes;
-  srv->queue.push_back(std::move(r));
+  r.big = ((nao + 3) & ~3) > kLdsMaxLd;
+  if (ticket) *ticket = r.ticket;
+  (r.big ? srv->queue_big : srv->queue_small).push_back(std::move(r));
   ++srv->submitted;
-  if (ticket) *ticket = srv->queue.back().ticket;
   return TBLITE_AMD_OK;
 }
 
 namespace {
 
-// Collect a finished slot's results; returns OK or an error code.
+// Report the molecules of a running batch whose completion records have
+// appeared since the last call.
+void serve_scan(tblite_amd_server srv, ServeSlot& sl)
+{
+  const int nmol = (int)sl.reqs.size();
+  double now = -1.0;
+  for (int m = 0; m < nmol; ++m) {
+    if (sl.reported[m]) continue;
+    const TaDoneRecord* r = sl.rec + m;
+    if (__atomic_load_n(&r->flag, __ATOMIC_ACQUIRE) == 0) continue;
+    if (now < 0.0) now = host_ms();
+    ServeResult res;
+    res.ticket = sl.reqs[m].ticket;
+    res.energy = r->energy;
+    res.iterations = r->iterations;
+    res.status = r->status;
+    res.latency_ms = now - sl.reqs[m].t_submit;
+    srv->done.push_back(res);
+    sl.reported[m] = 1;
+    ++srv->completed;
+  }
+}
+
+// Collect a finished slot: the molecules without a completion record (the
+// multi-workgroup path) come from the batch arrays; returns OK or an error.
 int serve_collect(tblite_amd_server srv, ServeSlot& sl)
 {
   tblite_amd_context sc = sl.ctx;
   int rc = finish_batch(sc);
   if (rc) return rc;
+  serve_scan(srv, sl);
   const int nmol = (int)sl.reqs.size();
+  int left = 0;
+  for (int m = 0; m < nmol; ++m) left += sl.reported[m] ? 0 : 1;
   std::vector<double> e(nmol);
   std::vector<int> it(nmol), st(nmol);
   const BatchDev& b = sc->batch.b;
-  HIP_TRY(srv->owner, hipMemcpy(e.data(), b.energy, 8 * nmol, hipMemcpyDeviceToHost));
-  HIP_TRY(srv->owner, hipMemcpy(it.data(), b.iterations, 4 * nmol, hipMemcpyDeviceToHost));
-  HIP_TRY(srv->owner, hipMemcpy(st.data(), b.status, 4 * nmol, hipMemcpyDeviceToHost));
+  if (left > 0) {
+    // on the slot's own stream: a copy on another stream could wait behind
+    // another slot's kernels
+    HIP_TRY(srv->owner, hipMemcpyAsync(e.data(), b.energy, 8 * nmol, hipMemcpyDeviceToHost, sc->stream));
+    HIP_TRY(srv->owner, hipMemcpyAsync(it.data(), b.iterations, 4 * nmol, hipMemcpyDeviceToHost, sc->stream));
+    HIP_TRY(srv->owner, hipMemcpyAsync(st.data(), b.status, 4 * nmol, hipMemcpyDeviceToHost, sc->stream));
+    HIP_TRY(srv->owner, hipStreamSynchronize(sc->stream));
+  }
   const double now = host_ms();
   for (int m = 0; m < nmol; ++m) {
+    if (sl.reported[m]) continue;
     ServeResult res;
     res.ticket = sl.reqs[m].ticket;
     res.energy = e[m];
@@ -1664,8 +1738,8 @@ int serve_collect(tblite_amd_server srv, ServeSlot& sl)
     res.status = st[m];
     res.latency_ms = now - sl.reqs[m].t_submit;
     srv->done.push_back(res);
+    ++srv->completed;
   }
-  srv->completed += nmol;
   sc->have_batch = true;
   sl.reqs.clear();
   sl.busy = false;
@@ -1675,12 +1749,17 @@ int serve_collect(tblite_amd_server srv, ServeSlot& sl)
 // Fill a free slot from the queue head and launch its batch.
 int serve_launch(tblite_amd_server srv, ServeSlot& sl)
 {
+  // a small-class slot takes small molecules; the others take large ones,
+  // then fill up with small ones (those run first and report early)
   size_t used = 64 * 1024 * 8;
-  while (!srv->queue.empty() && (int)sl.reqs.size() < srv->max_batch &&
-         used + srv->queue.front().bytes <= srv->slot_bytes) {
-    used += srv->queue.front().bytes;
-    sl.reqs.push_back(std::move(srv->queue.front()));
-    srv->queue.pop_front();
+  for (int pass = sl.small_class ? 1 : 0; pass < 2; ++pass) {
+    std::deque<ServeRequest>& q = pass == 0 ? srv->queue_big : srv->queue_small;
+    while (!q.empty() && (int)sl.reqs.size() < srv->max_batch &&
+           used + q.front().bytes <= srv->slot_bytes) {
+      used += q.front().bytes;
+      sl.reqs.push_back(std::move(q.front()));
+      q.pop_front();
+    }
   }
   if (sl.reqs.empty()) return TBLITE_AMD_OK;
   const int nmol = (int)sl.reqs.size();
@@ -1703,7 +1782,12 @@ int serve_launch(tblite_amd_server srv, ServeSlot& sl)
   int rc = make_plan(sc, c, nmol, mol_nat.data(), num.data(), charge.data(), uhf.data(),
                      c.accuracy, sc->plan);
   if (!rc) rc = upload_plan(sc, c, sc->plan, sc->batch);
-  if (!rc) rc = run_batch(sc, c, xyz.data(), c.accuracy, false);
+  if (!rc) {
+    std::memset(sl.rec, 0, sizeof(TaDoneRecord) * nmol);
+    sl.reported.assign(nmol, 0);
+    sc->batch.b.done = sl.rec_dev;
+    rc = run_batch(sc, c, xyz.data(), c.accuracy, false);
+  }
   if (rc) {
     set_error(srv->owner, "server: batch launch failed: %s", sc->error.c_str());
     return rc;
@@ -1723,7 +1807,10 @@ int tblite_amd_server_poll(tblite_amd_server srv, int capacity, long long* ticke
   for (auto& sl : srv->slots) {
     if (sl.busy) {
       const hipError_t q = hipEventQuery(sl.ctx->ev[4]);
-      if (q == hipErrorNotReady) continue;
+      if (q == hipErrorNotReady) {
+        serve_scan(srv, sl);
+        continue;
+      }
       if (q != hipSuccess) {
         set_error(srv->owner, "server: batch failed: %s", hipGetErrorString(q));
         return TBLITE_AMD_ERR_HIP;
@@ -1731,7 +1818,7 @@ int tblite_amd_server_poll(tblite_amd_server srv, int capacity, long long* ticke
       int rc = serve_collect(srv, sl);
       if (rc) return rc;
     }
-    if (!sl.busy && !srv->queue.empty()) {
+    if (!sl.busy && !(srv->queue_small.empty() && srv->queue_big.empty())) {
       int rc = serve_launch(srv, sl);
       if (rc) return rc;
     }
@@ -1758,7 +1845,7 @@ int tblite_amd_server_stats(tblite_amd_server srv, long long* submitted, long lo
   for (const auto& sl : srv->slots) busy += sl.busy ? 1 : 0;
   if (submitted) *submitted = srv->submitted;
   if (completed) *completed = srv->completed;
-  if (queued) *queued = (int)srv->queue.size();
+  if (queued) *queued = (int)(srv->queue_small.size() + srv->queue_big.size());
   if (busy_slots) *busy_slots = busy;
   if (batches) *batches = srv->batches;
   return TBLITE_AMD_OK;

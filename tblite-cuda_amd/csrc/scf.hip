@@ -1233,6 +1233,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? 1
     b.erep_tot[m] = erep;
     b.iterations[m] = iter;
     b.status[m] = status;
+    if (b.done != nullptr) {
+      TaDoneRecord* r = b.done + m;
+      r->energy = erep + esum;
+      r->iterations = iter;
+      r->status = status;
+      __hip_atomic_store(&r->flag, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
   }
 }
 

@@ -86,3 +86,29 @@ def test_submit_rejects_unknown_element(engine):
     with pytest.raises(tblite_amd.EngineError):
         srv.submit(np.array([92, 1]), np.array([[0.0, 0, 0], [0, 0, 2.0]]))
     srv.close()
+
+
+def test_results_reported_as_molecules_finish(engine):
+    """One batch holding small molecules and one large one: each small result is
+    reported when its molecule finishes, not when the batch ends."""
+    rng = np.random.default_rng(45)
+    big = workloads.water_cluster(100, workloads.liquid_radius(100), rng)
+    small = [workloads.water_cluster(nw, workloads.liquid_radius(nw), rng) for nw in (2, 3, 4, 5, 6, 8)]
+    srv = tblite_amd.Server(engine, slots=1, workspace_bytes=4e9)
+    tickets = {srv.submit(*big): "big"}
+    for k, m in enumerate(small):
+        tickets[srv.submit(*m)] = k
+    lat = {}
+    t0 = time.time()
+    while len(lat) < len(tickets):
+        for t, e, it, st, ms in srv.poll():
+            assert st == 0
+            lat[tickets[t]] = ms
+        assert time.time() - t0 < 120, "server did not finish"
+        time.sleep(0.0005)
+    stats = srv.stats()
+    srv.close()
+    print(f"[gpu] early reporting: big {lat['big']:.1f} ms, small max "
+          f"{max(lat[k] for k in range(len(small))):.1f} ms", flush=True)
+    assert stats["batches"] == 1
+    assert max(lat[k] for k in range(len(small))) < 0.5 * lat["big"]

@@ -246,6 +246,9 @@ struct tblite_amd_context_s {
   // path (TBLITE_AMD_GRID_MIN_LD may lower it, e.g. to test that path on
   // molecules the oracle handles quickly)
   int grid_min_ld = TA_BIG_MAXLD;
+  // padded AO dimension above which a molecule's matrices leave LDS for the
+  // global-memory single-workgroup path (TBLITE_AMD_LDS_MAX_LD may lower it)
+  int lds_max_ld = kLdsMaxLd;
   int verbose = 0;  // TBLITE_AMD_VERBOSE=1: progress lines of long runs on stderr
   // every kernel on the main stream (the serving slots: one stream per slot,
   // so concurrent slots do not queue behind each other's kernels)
@@ -393,7 +396,7 @@ int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* 
       big += (int64_t)ld * ld;
       pl.mol_scr_off[m] = scr;
       scr += grid_scratch_doubles(nao, ld, nat, nsh);
-    } else if (ld > kLdsMaxLd) {
+    } else if (ld > ctx->lds_max_ld) {
       pl.mol_big_off[m] = big;
       big += (int64_t)ld * ld;
       pl.mol_scr_off[m] = scr;
@@ -459,7 +462,11 @@ int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* 
       }
     }
     if (bk.mols.empty()) continue;
-    if (hi <= 32) {
+    if (hi > ctx->lds_max_ld) {
+      bk.nt = TA_BIG_NT;
+      bk.maxt = 1;
+      bk.lds = false;
+    } else if (hi <= 32) {
       bk.nt = 128;
       bk.maxt = 1;
       bk.lds = true;
@@ -1007,6 +1014,8 @@ static tblite_amd_context make_context(int device, bool single_stream)
   ctx->verbose = pv && pv[0] == '1';
   const char* pg = std::getenv("TBLITE_AMD_GRID_MIN_LD");
   if (pg) ctx->grid_min_ld = std::max(16, std::min(TA_BIG_MAXLD, std::atoi(pg)));
+  const char* pl = std::getenv("TBLITE_AMD_LDS_MAX_LD");
+  if (pl) ctx->lds_max_ld = std::max(16, std::min(kLdsMaxLd, std::atoi(pl) / 16 * 16));
   if (!ok) {
     tblite_amd_delete_context(&ctx);
     return nullptr;

@@ -45,7 +45,7 @@ __device__ __forceinline__ int sk(int i, int j, int ld)
   c = c >= ld ? c - ld : c;
   return i * ld + c;
 }
-#define JACOBI_TOL 1.0e-8
+#define JACOBI_TOL 1.0e-6
 
 struct MolView {
   int m, n, ld, nat, nsh, a0, s0, o0, ndim, mem;

@@ -6,9 +6,9 @@ neighbour list, integrals/H0, SCF to convergence) over the whole batch, with
 the coordinates already resident in HBM; the batch is the workload's seeded
 molecules (default: 4096 (H2O)n clusters, n ~ U{2..16}).  Prints one JSON
 line.  Beside the GPU number it reports:
-  roofline      FP64 work of the dominant SCF kernel (the size bucket with the
-                longest launch) / that launch's duration from HIP events
-                recorded on its stream
+  roofline      FP64 work of the dominant SCF kernel (the kernel instantiation
+                carrying the most work) / its launches' mean duration from HIP
+                events recorded on their streams
   cpu_baseline  the CPU oracle (FP64 restatement, OpenMP over molecules) on a
                 bounded sample of the same molecules, timed on this host
   per_molecule  the plain one-molecule-per-call path on a sample
@@ -287,9 +287,11 @@ def main():
         per_bucket.append({"kernel": scf_kernel_name(hi, bk["nthreads"]), "ldmax": hi,
                            "molecules": bk["nmol"], "ms": round(bk["ms"], 3), "flops": f,
                            "tflops": round(f / (bk["ms"] / 1e3) / 1e12, 4) if bk["ms"] > 0 else 0.0})
-    # the dominant kernel is the instantiation with the largest total launch time;
-    # buckets sharing an instantiation are launches of the same kernel, so its
-    # average launch duration is the mean over them (what rocprofv3 reports)
+    # the dominant kernel is the instantiation that carries the most FP64 work
+    # (buckets run concurrently on separate streams, so a small bucket's wall
+    # time can stretch while it waits for CUs held by a large one); buckets
+    # sharing an instantiation are launches of the same kernel, so its average
+    # launch duration is the mean over them (what rocprofv3 reports)
     kern = {}
     for b in per_bucket:
         k = kern.setdefault(b["kernel"], {"ms": [], "flops": []})
@@ -299,7 +301,7 @@ def main():
     if big.any():
         kern["multi-workgroup SCF (kernel sequence, main stream)"] = {
             "ms": [float(phase[:, 3].mean())], "flops": [float(fl[big].sum())]}
-    dname, dk = max(kern.items(), key=lambda kv: sum(kv[1]["ms"]))
+    dname, dk = max(kern.items(), key=lambda kv: sum(kv[1]["flops"]))
     avg_ms = float(np.mean(dk["ms"]))
     avg_flops = float(np.mean(dk["flops"]))
     achieved = avg_flops / (avg_ms / 1e3)

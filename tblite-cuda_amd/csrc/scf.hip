@@ -73,16 +73,19 @@ struct MolView {
 
 // ------------------------------------------------------------- GEMM tiles
 // out = X * Y (TRANS: out = (X * Y)^T) on ld x ld matrices with 4x4 register
-// tiles; X plain row-major, Y and out skewed.  Threads of a wave share a row
-// tile, so X is read as a broadcast.  When HOLD is set the output aliases an
-// input: a thread's tiles stay in registers until every thread has read.
+// tiles; X plain row-major, Y and out skewed.  A thread's tile is rows
+// r0..r0+3 and the columns c, c + nb, c + 2 nb, c + 3 nb (nb = ld / 4), so
+// neighbouring threads read neighbouring elements of Y (no LDS bank
+// conflicts) and share the rows of X (a broadcast).  When HOLD is set the
+// output aliases an input: a thread's tiles stay in registers until every
+// thread has read.
 template <int NT, int MAXT, bool HOLD, bool TRANS>
 __device__ void mm_nn(const double* X, const double* Y, double* out, int ld)
 {
   const int nb = ld >> 2;
   const int ntile = nb * nb;
   auto tile_mm = [&](int tile, double* acc) {
-    const int r0 = (tile / nb) * 4, c0 = (tile % nb) * 4;
+    const int r0 = (tile / nb) * 4, c0 = tile % nb;
 #pragma unroll
     for (int e = 0; e < 16; ++e) acc[e] = 0.0;
     for (int k = 0; k < ld; ++k) {
@@ -90,7 +93,7 @@ __device__ void mm_nn(const double* X, const double* Y, double* out, int ld)
 #pragma unroll
       for (int u = 0; u < 4; ++u) x[u] = X[(r0 + u) * ld + k];
 #pragma unroll
-      for (int v = 0; v < 4; ++v) y[v] = Y[sk(k, c0 + v, ld)];
+      for (int v = 0; v < 4; ++v) y[v] = Y[sk(k, c0 + v * nb, ld)];
 #pragma unroll
       for (int u = 0; u < 4; ++u)
 #pragma unroll
@@ -98,13 +101,13 @@ __device__ void mm_nn(const double* X, const double* Y, double* out, int ld)
     }
   };
   auto tile_store = [&](int tile, const double* acc) {
-    const int r0 = (tile / nb) * 4, c0 = (tile % nb) * 4;
+    const int r0 = (tile / nb) * 4, c0 = tile % nb;
 #pragma unroll
     for (int u = 0; u < 4; ++u)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        if constexpr (TRANS) out[sk(c0 + v, r0 + u, ld)] = acc[u * 4 + v];
-        else out[sk(r0 + u, c0 + v, ld)] = acc[u * 4 + v];
+        if constexpr (TRANS) out[sk(c0 + v * nb, r0 + u, ld)] = acc[u * 4 + v];
+        else out[sk(r0 + u, c0 + v * nb, ld)] = acc[u * 4 + v];
       }
   };
   if constexpr (HOLD) {
@@ -136,9 +139,10 @@ template <int NT>
 __device__ void density(const double* CT, const double* f, const int* occ, int nocc, double* P,
                         int ld)
 {
+  // tiles as in mm_nn: rows r0..r0+3, columns c, c + nb, c + 2 nb, c + 3 nb
   const int nb = ld >> 2;
   for (int tile = threadIdx.x; tile < nb * nb; tile += NT) {
-    const int r0 = (tile / nb) * 4, c0 = (tile % nb) * 4;
+    const int r0 = (tile / nb) * 4, c0 = tile % nb;
     double acc[16];
 #pragma unroll
     for (int e = 0; e < 16; ++e) acc[e] = 0.0;
@@ -150,7 +154,7 @@ __device__ void density(const double* CT, const double* f, const int* occ, int n
 #pragma unroll
       for (int u = 0; u < 4; ++u) x[u] = row[r0 + u] * fk;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) y[v] = row[c0 + v];
+      for (int v = 0; v < 4; ++v) y[v] = row[c0 + v * nb];
 #pragma unroll
       for (int u = 0; u < 4; ++u)
 #pragma unroll
@@ -159,7 +163,7 @@ __device__ void density(const double* CT, const double* f, const int* occ, int n
 #pragma unroll
     for (int u = 0; u < 4; ++u)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) P[sk(r0 + u, c0 + v, ld)] = acc[u * 4 + v];
+      for (int v = 0; v < 4; ++v) P[sk(r0 + u, c0 + v * nb, ld)] = acc[u * 4 + v];
   }
   __syncthreads();
 }

@@ -594,7 +594,7 @@ __device__ __noinline__ void big_density(const double* CT_, const double* f_, co
 // The global-memory variant runs two workgroups per CU (128 VGPRs): its phases
 // are bound by memory latency, not by arithmetic.
 template <int NT, int MAXT, int MAXB, int MAXC, bool MATS_LDS>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? 1 : 4))) void k_scf(BatchDev b, const int* __restrict__ mols, int ldmax,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (NT >= 512 ? 1 : 2) : 4))) void k_scf(BatchDev b, const int* __restrict__ mols, int ldmax,
                                             int natmax, int nshmax)
 {
   extern __shared__ double smem[];

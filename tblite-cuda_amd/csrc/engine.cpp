@@ -766,6 +766,7 @@ int run_grid_molecule(tblite_amd_context ctx, Calculator& c, int m)
   g.wt = take((int64_t)TA_BIG_NP * ld);
   g.wbt = take((int64_t)TA_BIG_NB * ld);
   g.part = take(nbm * nbm * 64);
+  g.svp = take(nbm);
   g.part11 = take(11 * (int64_t)n);
   g.epart = take(nsh + nat);
   g.d = take(ld);

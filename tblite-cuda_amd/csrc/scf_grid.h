@@ -30,6 +30,7 @@ struct GridMol {
   // scratch
   double *scr;                    // inverse iteration factors, 3 n^2
   double *vt, *wt, *wbt, *part;   // panel vectors, W of the back-transform, symv partials
+  double *svp;                    // per 64-row block partial sums of y.v (nbmax)
   double *part11, *epart;
   double *d, *e, *tau, *e2, *lam, *prevlam, *v, *y, *focc, *fs, *vsh, *vat, *vdp, *vqp, *dots, *tm;
   double *sc;
@@ -42,7 +43,7 @@ inline int64_t grid_scratch_doubles(int n, int ld, int nat, int nsh)
   const int64_t nbm = (n + 63) / 64;
   int64_t s = 3 * (int64_t)n * n;                          // scr
   s += (int64_t)(2 * TA_BIG_NP + TA_BIG_NB) * ld;          // vt, wt, wbt
-  s += nbm * nbm * 64;                                     // symv partials
+  s += nbm * nbm * 64 + nbm;                               // symv partials, y.v partials
   s += 11 * (int64_t)n + (nsh + nat);                      // Mulliken partials, energy parts
   s += 10 * (int64_t)ld + nsh + 10 * (int64_t)nat;         // vectors
   s += 2 * TA_BIG_NP + TA_BIG_NB * TA_BIG_NB + GS_COUNT;   // dots, tm, scalars

@@ -233,7 +233,7 @@ __global__ __launch_bounds__(512) void k_g_tri_col(GridMol g, int j, int i)
   const gbl_dbl* vt = TA_GBL(g.vt);
   const gbl_dbl* wt = TA_GBL(g.wt);
   for (int r = j + tid; r < n; r += GT) {
-    double a = g.A[(int64_t)r * ld + j];
+    double a = g.A[(int64_t)j * ld + r];  // the trailing square is symmetric: row j
     for (int q = 0; q < i; ++q)
       a -= vt[(int64_t)q * ld + r] * wt[(int64_t)q * ld + j] +
            wt[(int64_t)q * ld + r] * vt[(int64_t)q * ld + j];
@@ -318,26 +318,41 @@ __global__ __launch_bounds__(256) void k_g_tri_symv(GridMol g, int lo)
   }
 }
 
-// y = tau (sum of partials - V (W^T v) - W (V^T v)); w = y - (tau/2)(y.v) v
-__global__ __launch_bounds__(512) void k_g_tri_fin(GridMol g, int j, int i)
+// y = tau (sum of partials - V (W^T v) - W (V^T v)) over one 64-row block
+// per wave (blockIdx.x), and the block's share of y.v in svp
+__global__ __launch_bounds__(64) void k_g_tri_rows(GridMol g, int j, int i)
 {
-  __shared__ double red_[16];
-  lds_dbl* red = TA_LDS(red_);
   const int n = g.n, ld = g.ld, lo = j + 1, m = n - lo;
   const int nb = (m + 63) >> 6;
   const int64_t nbm = g.nbmax;
+  const int R = blockIdx.x, l = threadIdx.x;
+  const int r = lo + 64 * R + l;
   const double tj = g.tau[j];
   double sv = 0.0;
-  for (int r = lo + threadIdx.x; r < n; r += GT) {
-    const int R = (r - lo) >> 6, l = (r - lo) & 63;
+  if (r < n) {
     double y = 0.0;
     for (int q = 0; q < nb; ++q) y += g.part[((int64_t)R * nbm + q) * 64 + l];
     for (int q = 0; q < i; ++q)
       y -= g.vt[(int64_t)q * ld + r] * g.dots[q] + g.wt[(int64_t)q * ld + r] * g.dots[TA_BIG_NP + q];
     y *= tj;
     g.y[r] = y;
-    sv += y * g.v[r];
+    sv = y * g.v[r];
   }
+  sv = wave_sum(sv);
+  if (l == 0) g.svp[R] = sv;
+}
+
+// w = y - (tau/2)(y.v) v, y.v summed over the row blocks in order
+__global__ __launch_bounds__(512) void k_g_tri_fin(GridMol g, int j, int i)
+{
+  __shared__ double red_[16];
+  lds_dbl* red = TA_LDS(red_);
+  const int n = g.n, ld = g.ld, lo = j + 1, m = n - lo;
+  const int nb = (m + 63) >> 6;
+  const double tj = g.tau[j];
+  double sv = 0.0;
+  if (threadIdx.x < 64)
+    for (int R = threadIdx.x; R < nb; R += 64) sv += g.svp[R];
   sv = block_sum(sv, red);
   const double alpha = -0.5 * tj * sv;
   for (int r = lo + threadIdx.x; r < n; r += GT) g.wt[(int64_t)i * ld + r] = g.y[r] + alpha * g.v[r];
@@ -761,28 +776,43 @@ __global__ void k_g_orth_sum(int q, int w, int nch, const double* part, double* 
 
 // -------------------------------------------------------- back-transform
 // T of reflector block k0 .. k0 + nb - 1 (T^-1 = diag(1/tau) + strict_upper(V^T V))
-__global__ __launch_bounds__(512) void k_g_bt_t(GridMol g, int k0, int nb)
+// V^T V of reflector block k0 .. k0 + nb - 1 over row chunk blockIdx.x
+// (OCH rows): part[c * NB * NB + i * NB + j]
+__global__ __launch_bounds__(512) void k_g_bt_part(GridMol g, int k0, int nb, double* part)
 {
   __shared__ double tile_[2048];
-  __shared__ double sinv_[TA_BIG_NB * TA_BIG_NB];
-  lds_dbl* tile = TA_LDS(tile_);
-  lds_dbl* sinv = TA_LDS(sinv_);
   const int n = g.n, ld = g.ld, r0 = k0 + 1, m = n - r0;
-  const gbl_dbl* A = TA_GBL(g.A);
-  const gbl_dbl* tau = TA_GBL(g.tau);
-  auto vel = [=](int rr, int j) -> double {
+  const double* A = g.A;
+  const double* tau = g.tau;
+  const int c = blockIdx.x;
+  const int ka = c * OCH, kb = min(m, ka + OCH);
+  auto vel = [=] __device__(int rr, int j) -> double {
     return (j < nb && rr >= j && tau[k0 + j] != 0.0) ? A[(int64_t)(r0 + rr) * ld + k0 + j] : 0.0;
   };
-  big::gemm<GT, false, true>(
-      TA_BIG_NB, TA_BIG_NB, [=](int i, int k) { return vel(k, i); },
-      [=](int k, int j) { return vel(k, j); }, AllK{m}, NoSkip{},
-      [=](int i, int j, double val) {
-        double s = 0.0;
-        if (i < j) s = val;
-        if (i == j) s = (i < nb && tau[k0 + i] != 0.0) ? 1.0 / tau[k0 + i] : 1.0;
-        sinv[i * TA_BIG_NB + j] = s;
-      },
-      tile);
+  big::gemm_tile<512, false, true>(
+      TA_BIG_NB, TA_BIG_NB, 0, 0, ka, kb, [=] __device__(int i, int k) { return vel(k, i); },
+      [=] __device__(int k, int j) { return vel(k, j); },
+      [=] __device__(int i, int j, double v) { part[(int64_t)c * TA_BIG_NB * TA_BIG_NB + i * TA_BIG_NB + j] = v; },
+      TA_LDS(tile_));
+}
+
+// T of reflector block k0 .. k0 + nb - 1 from the chunk sums of V^T V
+// (T^-1 = diag(1/tau) + strict_upper(V^T V))
+__global__ __launch_bounds__(512) void k_g_bt_t(GridMol g, int k0, int nb, int nch,
+                                                const double* part)
+{
+  __shared__ double sinv_[TA_BIG_NB * TA_BIG_NB];
+  lds_dbl* sinv = TA_LDS(sinv_);
+  const double* tau = g.tau;
+  for (int idx = threadIdx.x; idx < TA_BIG_NB * TA_BIG_NB; idx += GT) {
+    const int i = idx / TA_BIG_NB, j = idx % TA_BIG_NB;
+    double s = 0.0;
+    if (i < j)
+      for (int c = 0; c < nch; ++c) s += part[(int64_t)c * TA_BIG_NB * TA_BIG_NB + idx];
+    if (i == j) s = (i < nb && tau[k0 + i] != 0.0) ? 1.0 / tau[k0 + i] : 1.0;
+    sinv[idx] = s;
+  }
+  __syncthreads();
   if (threadIdx.x < TA_BIG_NB) {
     const int j = threadIdx.x;
     for (int i = TA_BIG_NB - 1; i >= 0; --i) {
@@ -1192,6 +1222,7 @@ hipError_t grid_eigenvalues(GridMol& g, hipStream_t s, bool warm)
       hipLaunchKernelGGL(k_g_tri_col, dim3(1), dim3(GT), col_lds, s, g, j, i);
       const int m = n - (j + 1), nb = (m + 63) >> 6;
       hipLaunchKernelGGL(k_g_tri_symv, dim3(nb * (nb + 1) / 2), dim3(256), 0, s, g, j + 1);
+      hipLaunchKernelGGL(k_g_tri_rows, dim3(nb), dim3(64), 0, s, g, j, i);
       hipLaunchKernelGGL(k_g_tri_fin, dim3(1), dim3(GT), 0, s, g, j, i);
     }
     const int r0 = k0 + np;
@@ -1213,7 +1244,14 @@ hipError_t grid_eigenvalues(GridMol& g, hipStream_t s, bool warm)
         },
         AllK{2 * NP}, UpperSkip{},
         [=] __device__(int ii, int jj, double val) {
-          if (ii >= jj) A[(int64_t)(r0 + ii) * ld + r0 + jj] -= val;
+          // lower tiles, mirrored: the trailing square stays symmetric, so a
+          // column of it can be read as a (contiguous) row
+          if (ii > jj) {
+            A[(int64_t)(r0 + ii) * ld + r0 + jj] -= val;
+            A[(int64_t)(r0 + jj) * ld + r0 + ii] -= val;
+          } else if (ii == jj) {
+            A[(int64_t)(r0 + ii) * ld + r0 + jj] -= val;
+          }
         });
   }
   hipLaunchKernelGGL(k_g_gersh, dim3(1), dim3(GT), 0, s, g);
@@ -1273,7 +1311,12 @@ hipError_t grid_eigenvectors(GridMol& g, hipStream_t s, int nvec)
     const int nb = (k0 + NB <= nr) ? NB : nr - k0;
     const int r0 = k0 + 1;
     const int m = n - r0;
-    hipLaunchKernelGGL(k_g_bt_t, dim3(1), dim3(GT), 0, s, g, k0, nb);
+    {
+      const int nch_bt = (m + OCH - 1) / OCH;
+      hipLaunchKernelGGL(k_g_bt_part, dim3(nch_bt), dim3(512), 0, s, g, k0, nb, g.scr);
+      hipLaunchKernelGGL(k_g_bt_t, dim3(1), dim3(GT), 0, s, g, k0, nb, nch_bt,
+                         (const double*)g.scr);
+    }
     const double* A = g.A;
     const double* tau = g.tau;
     double* Y = g.Y;
@@ -1358,7 +1401,13 @@ int grid_scf(GridMol g, hipStream_t s, int max_iter, double* d_energy, double* d
       g_gemm<true, true>(
           s, n, n, [=] __device__(int i, int k) { return Uc[(int64_t)i * ld + k]; },
           [=] __device__(int k, int j) { return X[(int64_t)k * ld + j]; }, AllK{n}, UpperSkip{},
-          [=] __device__(int i, int j, double val) { A[(int64_t)i * ld + j] = val; });
+          [=] __device__(int i, int j, double val) {
+            // lower tiles, mirrored (the tridiagonalisation reads rows)
+            if (i >= j) {
+              A[(int64_t)i * ld + j] = val;
+              A[(int64_t)j * ld + i] = val;
+            }
+          });
     }
     if ((err = grid_eigenvalues(g, s, iter > 1)) != hipSuccess) return -2;
     // occupations from the levels; only occupied levels need eigenvectors

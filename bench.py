@@ -29,6 +29,16 @@ import workloads  # noqa: E402
 
 PEAK_FP64 = 78.6e12  # MI355X dense FP64 (matrix and vector), FLOP/s
 
+# HBM bytes per launch of the dominant SCF kernel (fetched + written), from
+# two rocprofv3 counter runs on the same workload (FETCH_SIZE, WRITE_SIZE;
+# FETCH_SIZE scaled x2 and WRITE_SIZE x1 by a 1 GiB device-copy calibration;
+# mean over the kernel's launches in one batch).  Summaries with the full
+# commands: profiles/k_scf_config1.md, profiles/k_scf_config3.md.
+MEASURED_TRAFFIC = {
+    ("config1", "k_scf<512, 2, 3, 6, true>"): 11.72e9 + 0.43e9,
+    ("config3", "k_scf<512, 1, 0, 0, false>"): 3202.0e9 + 674.3e9,
+}
+
 WORKLOADS = {
     "config1": ("4096 seeded (H2O)n clusters, n ~ U{2..16}, liquid density", workloads.config1),
     "config0": ("one seeded (H2O)8 cluster", workloads.config0),
@@ -312,7 +322,8 @@ def main():
         "peak": PEAK_FP64 / 1e12,
         "unit": "TFLOP/s",
         "frac": round(achieved / PEAK_FP64, 5),
-        "traffic": None,
+        "traffic": next((v for (w, k), v in MEASURED_TRAFFIC.items()
+                         if w == args.workload and dname.startswith(k)), None),
         "kernel": dname,
         "launches_per_step": len(dk["ms"]),
         "avg_launch_ms": round(avg_ms, 3),

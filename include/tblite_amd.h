@@ -203,18 +203,21 @@ int tblite_amd_batch_results(tblite_amd_context ctx, double* energies, int* iter
 
 /* ------------------------------------------------------------- serving
  * Continuous batching of independent singlepoint requests on one device.
- * The server owns `slots` internal contexts (streams + a device workspace of
- * workspace_bytes / slots reserved at creation).  submit() queues a request
- * (host arrays are copied) and returns its ticket; poll() never blocks: it
- * collects finished batches, refills every free slot from the queue head
- * while the requests' estimated device footprint fits the slot, launches
- * those batches (batches of different slots run concurrently), and copies
- * out up to `capacity` finished results (ticket, energy in Eh, SCF
- * iterations, status bits, latency from submit to collection in ms).  Calls
- * on one server must come from one host thread.  Requests keep their
- * wavefunction and mixer state on the device until their batch ends.  No
- * counterpart in the reference (one molecule per GPU thread,
- * src/xtb.cu:479-497). */
+ * The server owns `slots` internal contexts (one stream + a device workspace
+ * of workspace_bytes / slots reserved at creation each).  With two or more
+ * slots the first serves only molecules of the LDS-resident sizes (padded AO
+ * dimension <= 96), the others take larger molecules first and fill up with
+ * small ones.  submit() queues a request (host arrays are copied) and
+ * returns its ticket; poll() never blocks: it reports every molecule whose
+ * SCF has ended (per molecule, not per batch), collects finished batches,
+ * refills every free slot from its queue while the requests' estimated
+ * device footprint fits the slot, launches those batches (batches of
+ * different slots run concurrently), and copies out up to `capacity`
+ * finished results (ticket, energy in Eh, SCF iterations, status bits,
+ * latency from submit to collection in ms).  Calls on one server must come
+ * from one host thread.  Requests keep their wavefunction and mixer state on
+ * the device until their batch ends.  No counterpart in the reference (one
+ * molecule per GPU thread, src/xtb.cu:479-497). */
 typedef struct tblite_amd_server_s* tblite_amd_server;
 tblite_amd_server tblite_amd_new_server(tblite_amd_context ctx, tblite_amd_calculator calc,
                                         int slots, double workspace_bytes);

@@ -249,6 +249,9 @@ struct tblite_amd_context_s {
   // padded AO dimension above which a molecule's matrices leave LDS for the
   // global-memory single-workgroup path (TBLITE_AMD_LDS_MAX_LD may lower it)
   int lds_max_ld = kLdsMaxLd;
+  // replay the multi-workgroup path's eigenvalue phase from a HIP graph
+  // (TBLITE_AMD_GRID_GRAPH=0 launches the kernels one by one instead)
+  bool grid_graph = true;
   int verbose = 0;  // TBLITE_AMD_VERBOSE=1: progress lines of long runs on stderr
   // every kernel on the main stream (the serving slots: one stream per slot,
   // so concurrent slots do not queue behind each other's kernels)
@@ -791,7 +794,8 @@ int run_grid_molecule(tblite_amd_context ctx, Calculator& c, int m)
   GridHostStats hs;
   int rc = grid_scf(g, ctx->stream, c.max_iter, b.energy + m, b.eelec + m, b.erep_tot + m,
                     b.iterations + m, b.status + m, b.emo + o0, b.focc + o0,
-                    b.coeff ? b.coeff + moff : nullptr, &hs, ctx->verbose);
+                    b.coeff ? b.coeff + moff : nullptr, &hs, ctx->verbose,
+                    ctx->grid_graph ? 1 : 0);
   if (rc) {
     set_error(ctx, "multi-workgroup SCF of molecule %d failed: %s", m,
               hipGetErrorString(hipGetLastError()));
@@ -1015,6 +1019,8 @@ static tblite_amd_context make_context(int device, bool single_stream)
   ctx->verbose = pv && pv[0] == '1';
   const char* pg = std::getenv("TBLITE_AMD_GRID_MIN_LD");
   if (pg) ctx->grid_min_ld = std::max(16, std::min(TA_BIG_MAXLD, std::atoi(pg)));
+  const char* pgg = std::getenv("TBLITE_AMD_GRID_GRAPH");
+  ctx->grid_graph = !(pgg && pgg[0] == '0');
   const char* pl = std::getenv("TBLITE_AMD_LDS_MAX_LD");
   if (pl) ctx->lds_max_ld = std::max(16, std::min(kLdsMaxLd, std::atoi(pl) / 16 * 16));
   if (!ok) {

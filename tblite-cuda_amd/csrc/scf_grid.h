@@ -59,7 +59,8 @@ struct GridHostStats {
 // molecule's energy, electronic energy, repulsion, iterations and status
 // into the given device slots and its levels / occupations (and optionally
 // coefficients).  verbose != 0 prints one line per SCF iteration to stderr.
-// Returns 0 or < 0 on a HIP failure.
+// use_graph != 0 replays the eigenvalue phase of iterations 2.. from a HIP
+// graph captured in iteration 2.  Returns 0 or < 0 on a HIP failure.
 int grid_scf(GridMol g, hipStream_t s, int max_iter, double* d_energy, double* d_eelec,
              double* d_erep, int* d_iter, int* d_status, double* emo, double* focc, double* coeff,
-             GridHostStats* hs, int verbose);
+             GridHostStats* hs, int verbose, int use_graph);

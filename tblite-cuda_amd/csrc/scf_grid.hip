@@ -1345,12 +1345,43 @@ hipError_t grid_eigenvectors(GridMol& g, hipStream_t s, int nvec)
 
 // Full SCF of one molecule on the grid path; writes energy/status/iterations
 // of molecule m into the batch arrays.
+namespace {
+// The eigenvalue phase of an SCF iteration (tridiagonalisation, about four
+// kernels per column, and bisection) is the same kernel sequence with the
+// same arguments in every iteration after the first: captured once into a
+// HIP graph and replayed.
+struct EigGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~EigGraph()
+  {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+  hipError_t run(GridMol& g, hipStream_t s)
+  {
+    if (!exec) {
+      hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+      if (e != hipSuccess) return e;
+      const hipError_t ek = grid_eigenvalues(g, s, true);
+      e = hipStreamEndCapture(s, &graph);
+      if (ek != hipSuccess) return ek;
+      if (e != hipSuccess) return e;
+      e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      if (e != hipSuccess) return e;
+    }
+    return hipGraphLaunch(exec, s);
+  }
+};
+}  // namespace
+
 int grid_scf(GridMol g, hipStream_t s, int max_iter, double* d_energy, double* d_eelec,
              double* d_erep, int* d_iter, int* d_status, double* emo, double* focc, double* coeff,
-             GridHostStats* hs, int verbose)
+             GridHostStats* hs, int verbose, int use_graph)
 {
   const int n = g.n, ld = g.ld;
   hipError_t err;
+  EigGraph eg;
   if (hipFuncSetAttribute((const void*)k_g_orth_panel, hipFuncAttributeMaxDynamicSharedMemorySize,
                           (int)(sizeof(double) * orth_panel_lds_doubles())) != hipSuccess)
     return -2;
@@ -1409,7 +1440,11 @@ int grid_scf(GridMol g, hipStream_t s, int max_iter, double* d_energy, double* d
             }
           });
     }
-    if ((err = grid_eigenvalues(g, s, iter > 1)) != hipSuccess) return -2;
+    if (use_graph && iter > 1) {
+      if ((err = eg.run(g, s)) != hipSuccess) return -2;
+    } else if ((err = grid_eigenvalues(g, s, iter > 1)) != hipSuccess) {
+      return -2;
+    }
     // occupations from the levels; only occupied levels need eigenvectors
     hipLaunchKernelGGL(k_g_fermi, dim3(1), dim3(64), 0, s, g);
     if (hipMemcpyAsync(hbuf, g.ic, sizeof(int) * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||

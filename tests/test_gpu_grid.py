@@ -59,3 +59,33 @@ def test_grid_vs_single_workgroup(grid_engine, engine, mols):
     assert np.all(st1 == 0) and np.all(st2 == 0)
     assert np.max(np.abs(e1 - e2)) < 1e-8, (e1, e2)
     assert np.max(np.abs(it1.astype(int) - it2.astype(int))) <= 1
+
+
+def test_grid_graph_replay_equals_eager(mols):
+    """The eigenvalue phase of SCF iterations 2.. replays a captured HIP graph;
+    between replays the eager eigenvector, density and mixing kernels and host
+    synchronisations run.  Every replay feeds the next iteration, so equal
+    energies and iteration counts with and without the graph check them all."""
+    out = {}
+    for flag in ("1", "0"):
+        os.environ["TBLITE_AMD_GRID_MIN_LD"] = "64"
+        os.environ["TBLITE_AMD_GRID_GRAPH"] = flag
+        try:
+            eng = tblite_amd.Engine()
+        finally:
+            del os.environ["TBLITE_AMD_GRID_MIN_LD"]
+            del os.environ["TBLITE_AMD_GRID_GRAPH"]
+        res = []
+        for num, xyz in mols[1:]:
+            print(f"[gpu] grid graph={flag} nat={len(num)}", flush=True)
+            e, it, st = eng.singlepoint_flat(np.array([len(num)]), num, xyz)
+            assert st[0] == 0
+            res.append((float(e[0]), int(it[0])))
+        # the same engine twice: the second run reuses nothing from the first graph
+        e2, it2, st2 = eng.singlepoint_flat(np.array([len(mols[1][0])]), *mols[1])
+        assert abs(float(e2[0]) - res[0][0]) <= 1e-12 and int(it2[0]) == res[0][1]
+        eng.close()
+        out[flag] = res
+    for (eg, ig), (ee, ie) in zip(out["1"], out["0"]):
+        assert ig == ie
+        assert abs(eg - ee) <= 1e-10, (eg, ee)

@@ -264,6 +264,37 @@ int tblite_amd_batch_get_energy_parts(tblite_amd_context ctx, int imol, double* 
  * [2] integrals/H0, [3] SCF, [4] total.  n = number of entries wanted. */
 int tblite_amd_last_timings(tblite_amd_context ctx, double* times, int n);
 
+/* ------------------------------------------------------------- tuning
+ * Which SCF path a molecule takes, by its padded AO dimension ld:
+ *   ld <= lds_max_ld                 matrices in LDS, one workgroup per molecule
+ *   lds_max_ld < ld <= grid_min_ld   matrices in global memory, one workgroup
+ *                                    per molecule (throughput: many molecules
+ *                                    share the device)
+ *   ld > grid_min_ld                 multi-workgroup path, one molecule across
+ *                                    the device (latency)
+ * With auto_grid set, molecules of the middle range also take the
+ * multi-workgroup path when the batch is too small to fill the device and
+ * the cost model (one SCF iteration: wg_a * ld^wg_p seconds on one
+ * workgroup, grid_b * ld + grid_c seconds across the device) says the batch
+ * finishes sooner.  The coefficients come from measurement on the device:
+ * tblite-cuda_amd/autotune.py times both paths and writes them.  A server's
+ * slots never use auto_grid.  TBLITE_AMD_LDS_MAX_LD, TBLITE_AMD_GRID_MIN_LD
+ * and TBLITE_AMD_GRID_GRAPH=0 override the defaults at context creation.  No
+ * counterpart in the reference (one molecule per GPU thread,
+ * src/xtb.cu:479-497). */
+typedef struct tblite_amd_tuning {
+  int lds_max_ld;   /* 16 .. 96, multiple of 16 */
+  int grid_min_ld;  /* 16 .. 1024 */
+  int auto_grid;    /* nonzero: cost-model routing of the middle range */
+  int grid_graph;   /* nonzero: replay the multi-workgroup eigenvalue phase from a HIP graph */
+  double wg_a, wg_p, grid_b, grid_c;
+} tblite_amd_tuning;
+int tblite_amd_get_tuning(tblite_amd_context ctx, tblite_amd_tuning* out);
+int tblite_amd_set_tuning(tblite_amd_context ctx, const tblite_amd_tuning* in);
+/* SCF path of molecule imol of the last planned batch: 0 LDS, 1 one
+ * workgroup in global memory, 2 multi-workgroup; < 0 on error. */
+int tblite_amd_molecule_path(tblite_amd_context ctx, int imol);
+
 /* SCF size buckets of the last batched call, in launch order (largest AO
  * dimension first): ms receives the time of each bucket's SCF kernel (HIP
  * events recorded around the launch on the bucket's stream), ldmax its padded

@@ -210,6 +210,7 @@ struct Plan {
   int64_t tot_atoms = 0, tot_sh = 0, tot_ao = 0, tot_mat = 0, tot_nl = 0, tot_gam = 0,
           tot_pair = 0, tot_mix = 0, tot_mixmem = 0, tot_big = 0, tot_scr = 0;
   std::vector<int> too_large;
+  std::vector<char> mol_path;  // 0 LDS, 1 one workgroup in global memory, 2 multi-workgroup
 };
 
 struct DeviceBatch {
@@ -252,6 +253,13 @@ struct tblite_amd_context_s {
   // replay the multi-workgroup path's eigenvalue phase from a HIP graph
   // (TBLITE_AMD_GRID_GRAPH=0 launches the kernels one by one instead)
   bool grid_graph = true;
+  // cost-model routing between one workgroup per molecule and the
+  // multi-workgroup path (seconds per SCF iteration, measured on MI355X by
+  // autotune.py: one workgroup alone wg_a * ld^wg_p, across the device
+  // grid_b * ld + grid_c)
+  bool auto_grid = true;
+  double wg_a = 9.13e-8, wg_p = 2.10, grid_b = 4.37e-5, grid_c = -2.92e-3;
+  int wg_slots = 512;  // one-workgroup molecules resident at once (two per CU)
   int verbose = 0;  // TBLITE_AMD_VERBOSE=1: progress lines of long runs on stderr
   // every kernel on the main stream (the serving slots: one stream per slot,
   // so concurrent slots do not queue behind each other's kernels)
@@ -311,6 +319,48 @@ int ensure_params_on_device(tblite_amd_context ctx, Calculator& c)
 }
 
 // Build the host plan for a batch.  Returns OK or ERR_ARGUMENT.
+// SCF path of every molecule from its padded AO dimension (see the tuning
+// block of the header).  With auto_grid, a batch whose one-workgroup
+// molecules would fill less than a quarter of the device's workgroup slots
+// moves its largest molecules, one at a time, to the multi-workgroup path
+// while the estimated finish time (the longer of the multi-workgroup chain
+// on the main stream and the one-workgroup bucket: its slowest molecule or
+// its total work over the slots) goes down.
+void route_molecules(const tblite_amd_context ctx, const std::vector<int>& ld,
+                     std::vector<char>& path)
+{
+  const int nmol = (int)ld.size();
+  path.assign(nmol, 0);
+  std::vector<int> wg;
+  for (int m = 0; m < nmol; ++m) {
+    if (ld[m] > ctx->grid_min_ld) {
+      path[m] = 2;
+    } else if (ld[m] > ctx->lds_max_ld) {
+      path[m] = 1;
+      wg.push_back(m);
+    }
+  }
+  if (!ctx->auto_grid || wg.empty() || 4 * (int64_t)wg.size() > ctx->wg_slots) return;
+  auto t_wg = [&](int m) { return ctx->wg_a * std::pow((double)ld[m], ctx->wg_p); };
+  auto t_grid = [&](int m) { return std::max(0.0, ctx->grid_b * ld[m] + ctx->grid_c); };
+  std::stable_sort(wg.begin(), wg.end(), [&](int a, int b) { return ld[a] > ld[b]; });
+  double total = 0.0, chain = 0.0;
+  for (int m : wg) total += t_wg(m);
+  for (int m = 0; m < nmol; ++m)
+    if (path[m] == 2) chain += t_grid(m);
+  const double slots = (double)ctx->wg_slots;
+  for (size_t k = 0; k < wg.size(); ++k) {
+    const int m = wg[k];
+    const double before = std::max(chain, std::max(t_wg(m), total / slots));
+    const double rest = k + 1 < wg.size() ? t_wg(wg[k + 1]) : 0.0;
+    const double after = std::max(chain + t_grid(m), std::max(rest, (total - t_wg(m)) / slots));
+    if (!(after < before)) break;
+    path[m] = 2;
+    chain += t_grid(m);
+    total -= t_wg(m);
+  }
+}
+
 int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* mol_nat,
               const int* num, const double* charge, const int* uhf, double acc, Plan& pl)
 {
@@ -335,6 +385,24 @@ int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* 
   pl.mol_scr_off.resize(nmol);
   pl.mol_charge.resize(nmol);
   pl.mol_cutoff.resize(nmol);
+  {
+    // padded AO dimensions first: they decide the SCF path of each molecule
+    std::vector<int> lds(nmol);
+    int64_t a0 = 0;
+    for (int m = 0; m < nmol; ++m) {
+      int nao = 0;
+      for (int i = 0; i < mol_nat[m] && mol_nat[m] > 0; ++i) {
+        const int z = num[a0 + i];
+        const int e = (z > 0 && z < 120) ? c.zmap[z] : -1;
+        if (e < 0) break;  // reported by the pass below
+        const DevElem& el = c.dev.el[e];
+        for (int s = 0; s < el.nsh; ++s) nao += 2 * el.ang[s] + 1;
+      }
+      a0 += std::max(0, mol_nat[m]);
+      lds[m] = (nao + 3) & ~3;
+    }
+    route_molecules(ctx, lds, pl.mol_path);
+  }
   int64_t atom = 0, shell = 0, ao = 0, mat = 0, nl = 0, gam = 0, pair = 0, mix = 0, mixmem = 0,
           big = 0, scr = 0;
   for (int m = 0; m < nmol; ++m) {
@@ -393,13 +461,13 @@ int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* 
                 kGridMaxLd);
       return TBLITE_AMD_ERR_TOO_LARGE;
     }
-    if (ld > ctx->grid_min_ld) {
+    if (pl.mol_path[m] == 2) {
       pl.grid_mols.push_back(m);
       pl.mol_big_off[m] = big;
       big += (int64_t)ld * ld;
       pl.mol_scr_off[m] = scr;
       scr += grid_scratch_doubles(nao, ld, nat, nsh);
-    } else if (ld > ctx->lds_max_ld) {
+    } else if (pl.mol_path[m] == 1) {
       pl.mol_big_off[m] = big;
       big += (int64_t)ld * ld;
       pl.mol_scr_off[m] = scr;
@@ -457,7 +525,7 @@ int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* 
     bk.nshmax = 0;
     for (int m : order) {
       const int ld = (pl.mol_nao[m] + 3) & ~3;
-      if (ld > lo && ld <= hi && ld <= ctx->grid_min_ld) {
+      if (ld > lo && ld <= hi && pl.mol_path[m] != 2) {
         bk.mols.push_back(m);
         bk.ldmax = std::max(bk.ldmax, ld);
         bk.natmax = std::max(bk.natmax, pl.mol_nat[m]);
@@ -1010,8 +1078,10 @@ static tblite_amd_context make_context(int device, bool single_stream)
          hipEventCreateWithFlags(&ctx->aux_done[s], hipEventDisableTiming) == hipSuccess;
   for (int e = 0; e < 8 && ok; ++e) ok = hipEventCreate(&ctx->ev[e]) == hipSuccess;
   for (int e = 0; e < 2 * kMaxBuckets && ok; ++e) ok = hipEventCreate(&ctx->bev[e]) == hipSuccess;
-  int lds = 0;
+  int lds = 0, ncu = 0;
   if (ok) ok = hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess;
+  if (ok) ok = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess;
+  ctx->wg_slots = std::max(1, 2 * ncu);
   ctx->lds_limit = (size_t)lds;
   const char* pe = std::getenv("TBLITE_AMD_PROFILE");
   ctx->profile = pe && pe[0] == '1';
@@ -1509,6 +1579,48 @@ int tblite_amd_last_bucket_timings(tblite_amd_context ctx, double* ms, int* ldma
   return nb;
 }
 
+int tblite_amd_get_tuning(tblite_amd_context ctx, tblite_amd_tuning* out)
+{
+  if (!ctx || !out) return TBLITE_AMD_ERR_ARGUMENT;
+  out->lds_max_ld = ctx->lds_max_ld;
+  out->grid_min_ld = ctx->grid_min_ld;
+  out->auto_grid = ctx->auto_grid ? 1 : 0;
+  out->grid_graph = ctx->grid_graph ? 1 : 0;
+  out->wg_a = ctx->wg_a;
+  out->wg_p = ctx->wg_p;
+  out->grid_b = ctx->grid_b;
+  out->grid_c = ctx->grid_c;
+  return TBLITE_AMD_OK;
+}
+
+int tblite_amd_set_tuning(tblite_amd_context ctx, const tblite_amd_tuning* in)
+{
+  if (!ctx || !in) return TBLITE_AMD_ERR_ARGUMENT;
+  if (in->lds_max_ld < 16 || in->lds_max_ld > kLdsMaxLd || in->lds_max_ld % 16 != 0 ||
+      in->grid_min_ld < 16 || in->grid_min_ld > TA_BIG_MAXLD || !(in->wg_a > 0.0) ||
+      !(in->wg_p > 0.0) || !(in->grid_b >= 0.0) || !std::isfinite(in->grid_c)) {
+    set_error(ctx, "set_tuning: lds_max_ld must be 16..%d in steps of 16, grid_min_ld 16..%d, "
+              "wg_a, wg_p > 0, grid_b >= 0", kLdsMaxLd, TA_BIG_MAXLD);
+    return TBLITE_AMD_ERR_ARGUMENT;
+  }
+  ctx->lds_max_ld = in->lds_max_ld;
+  ctx->grid_min_ld = in->grid_min_ld;
+  ctx->auto_grid = in->auto_grid != 0;
+  ctx->grid_graph = in->grid_graph != 0;
+  ctx->wg_a = in->wg_a;
+  ctx->wg_p = in->wg_p;
+  ctx->grid_b = in->grid_b;
+  ctx->grid_c = in->grid_c;
+  ctx->prepared = nullptr;  // a prepared batch was planned with the old routing
+  return TBLITE_AMD_OK;
+}
+
+int tblite_amd_molecule_path(tblite_amd_context ctx, int imol)
+{
+  if (!ctx || imol < 0 || imol >= (int)ctx->plan.mol_path.size()) return TBLITE_AMD_ERR_ARGUMENT;
+  return ctx->plan.mol_path[imol];
+}
+
 int tblite_amd_last_timings(tblite_amd_context ctx, double* times, int n)
 {
   if (!ctx || !times) return TBLITE_AMD_ERR_ARGUMENT;
@@ -1630,6 +1742,9 @@ tblite_amd_server tblite_amd_new_server(tblite_amd_context ctx, tblite_amd_calcu
     sl.small_class = slots >= 2 && k == 0;
     sl.ctx = make_context(ctx->device, true);
     bool ok = sl.ctx != nullptr;
+    // a multi-workgroup molecule runs synchronously on the host thread, which
+    // would stall poll(): slots keep one workgroup per molecule
+    if (ok) sl.ctx->auto_grid = false;
     if (ok) {
       // reserve the slot's workspace once: growing it later would free device
       // memory, which synchronises the whole device

@@ -51,6 +51,15 @@ class Param(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    """tblite_amd_tuning: SCF path thresholds and the routing cost model."""
+    _fields_ = [
+        ("lds_max_ld", C.c_int), ("grid_min_ld", C.c_int), ("auto_grid", C.c_int),
+        ("grid_graph", C.c_int), ("wg_a", C.c_double), ("wg_p", C.c_double),
+        ("grid_b", C.c_double), ("grid_c", C.c_double),
+    ]
+
+
 _P = C.c_void_p
 _I = C.c_int
 _D = C.c_double
@@ -92,6 +101,9 @@ _PROTOS = {
     "tblite_amd_last_timings": (_I, [_P, _DP, _I]),
     "tblite_amd_last_bucket_timings": (_I, [_P, _DP, _IP, _IP, _IP, _I]),
     "tblite_amd_last_profile": (_I, [_P, C.POINTER(C.c_longlong)]),
+    "tblite_amd_get_tuning": (_I, [_P, C.POINTER(Tuning)]),
+    "tblite_amd_set_tuning": (_I, [_P, C.POINTER(Tuning)]),
+    "tblite_amd_molecule_path": (_I, [_P, _I]),
     "tblite_amd_new_server": (_P, [_P, _P, _I, _D]),
     "tblite_amd_delete_server": (None, [C.POINTER(_P)]),
     "tblite_amd_server_submit": (_I, [_P, _I, _IP, _DP, _D, _I, C.POINTER(C.c_longlong)]),
@@ -151,7 +163,7 @@ class Engine:
     """One device context with one GFN2 calculator."""
 
     def __init__(self, device=0, param=None, accuracy=1.0, max_iter=250, damping=0.4,
-                 temperature=300.0):
+                 temperature=300.0, tuning=None):
         self.lib = load()
         self.ctx = self.lib.tblite_amd_new_context(device)
         if not self.ctx:
@@ -165,6 +177,14 @@ class Engine:
         self._check(self.lib.tblite_amd_set_calculator_mixer_damping(self.calc, damping))
         self._check(self.lib.tblite_amd_set_calculator_temperature(self.calc, temperature))
         self.accuracy = accuracy
+        if tuning is not None:
+            # a dict of tblite_amd_tuning fields or the path of a JSON file
+            # written by autotune.py
+            if isinstance(tuning, str):
+                import json
+                with open(tuning) as f:
+                    tuning = json.load(f)
+            self.set_tuning(**tuning)
 
     def error(self):
         buf = C.create_string_buffer(1024)
@@ -331,6 +351,30 @@ class Engine:
         t = np.zeros(5)
         self._check(self.lib.tblite_amd_last_timings(self.ctx, _dp(t), 5))
         return t
+
+    def tuning(self):
+        """Current tuning as a dict (see tblite_amd_tuning)."""
+        t = Tuning()
+        self._check(self.lib.tblite_amd_get_tuning(self.ctx, C.byref(t)))
+        return {name: getattr(t, name) for name, _ in Tuning._fields_}
+
+    def set_tuning(self, **kw):
+        """Change some tuning fields (keyword names as in tblite_amd_tuning)."""
+        t = Tuning()
+        self._check(self.lib.tblite_amd_get_tuning(self.ctx, C.byref(t)))
+        for k, v in kw.items():
+            if k not in dict(Tuning._fields_):
+                raise KeyError(k)
+            setattr(t, k, v)
+        self._check(self.lib.tblite_amd_set_tuning(self.ctx, C.byref(t)))
+
+    def molecule_path(self, imol):
+        """SCF path of molecule imol of the last planned batch: 0 LDS, 1 one
+        workgroup in global memory, 2 multi-workgroup."""
+        r = self.lib.tblite_amd_molecule_path(self.ctx, imol)
+        if r < 0:
+            raise EngineError("no such molecule in the last batch")
+        return r
 
     def bucket_timings(self):
         """SCF size buckets of the last run: list of dicts (ms, ldmax, nmol, nthreads)."""

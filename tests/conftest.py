@@ -28,6 +28,9 @@ def orc(oracle_mod):
 @pytest.fixture(scope="session")
 def engine():
     import tblite_amd
-    eng = tblite_amd.Engine()
+    # fixed routing by size (no cost-model routing of small batches to the
+    # multi-workgroup path), so a molecule takes the same path alone and in a
+    # batch; tests/test_gpu_tuning.py covers the routing
+    eng = tblite_amd.Engine(tuning={"auto_grid": 0})
     yield eng
     eng.close()

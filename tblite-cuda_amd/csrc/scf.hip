@@ -369,22 +369,24 @@ __device__ bool jacobi(double* A, double* CT, int ld, const JRot& R, const JPlan
             A[sk(pq.y, pq.y, ld)] = dg.y + ta.x * ta.y;
             aup(A, ld, pq.x, pq.y) = 0.0;
           }
+          // The block whose new off-diagonal element is slot kn's pivot next
+          // round: slot 0 takes element 00 of block (0, 1), a middle slot
+          // element 10 of block (kn-1, kn+1), the last slot element 11 of
+          // block (h-2, h-1).  One code path for every lane (no divergence:
+          // the owner wave is the critical path of a round).
+          const int e = kn == 0 ? 0 : (kn <= h - 2 ? 1 : 2);
+          const int bk = e == 0 ? 0 : (e == 1 ? kn - 1 : h - 2);
+          const int bl = e == 0 ? 1 : (e == 1 ? kn + 1 : h - 1);
           double z00, z01, z10, z11;
           int2 pk, pl_;
-          if (kn == 0) {
-            block(0, 1, z00, z01, z10, z11, pk, pl_);
-            make_rotation(R, nxt, h, 0, pk.x, pl_.x, diag_after(0, 0), diag_after(1, 0), z00);
-            if (h == 2)
-              make_rotation(R, nxt, h, 1, pl_.y, pk.y, diag_after(1, 1), diag_after(0, 1), z11);
-          } else if (kn <= h - 2) {
-            block(kn - 1, kn + 1, z00, z01, z10, z11, pk, pl_);
-            make_rotation(R, nxt, h, kn, pl_.x, pk.y, diag_after(kn + 1, 0), diag_after(kn - 1, 1),
-                          z10);
-          } else if (h > 2) {
-            block(h - 2, h - 1, z00, z01, z10, z11, pk, pl_);
-            make_rotation(R, nxt, h, h - 1, pl_.y, pk.y, diag_after(h - 1, 1), diag_after(h - 2, 1),
-                          z11);
-          }
+          block(bk, bl, z00, z01, z10, z11, pk, pl_);
+          const int p = e == 0 ? pk.x : (e == 1 ? pl_.x : pl_.y);
+          const int q = e == 0 ? pl_.x : pk.y;
+          const double apq = e == 0 ? z00 : (e == 1 ? z10 : z11);
+          // diagonal elements after this round: slot/index holding p and q
+          const int sp = e == 0 ? 0 : (e == 1 ? kn + 1 : h - 1), ip = e == 2 ? 1 : 0;
+          const int sq = e == 0 ? 1 : (e == 1 ? kn - 1 : h - 2), iq = e == 0 ? 0 : 1;
+          make_rotation(R, nxt, h, kn, p, q, diag_after(sp, ip), diag_after(sq, iq), apq);
         }
       } else {
         auto crow = [&](int k, int ao) {

@@ -31,6 +31,15 @@ __device__ __forceinline__ double wave_sum(double v)
   return v;
 }
 
+// Sum over the 16-lane segment of a wave the lane belongs to (lanes 0-15,
+// 16-31, 32-47, 48-63); every lane of the segment gets the result.
+__device__ __forceinline__ double seg16_sum(double v)
+{
+#pragma unroll
+  for (int off = 8; off > 0; off >>= 1) v += __shfl_xor(v, off, TA_WAVE);
+  return v;
+}
+
 __device__ __forceinline__ double wave_max(double v)
 {
 #pragma unroll

@@ -902,24 +902,30 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
     double* vat = vsh + nshmax;
     double* vdp = vat + natmax;
     double* vqp = vdp + 3 * natmax;
-    // ES2 + ES3 shell potential: one wave per shell, lanes over partner shells
-    for (int sh = wid; sh < nsh; sh += NW) {
-      const double* g = v.gam + (int64_t)sh * nsh;
+    // ES2 + ES3 shell potential: four shells per wave (16-lane segments),
+    // lanes over partner shells; four rows share one memory round trip
+    const int seg = lane >> 4, sl = lane & 15;
+    for (int sh0 = 4 * wid; sh0 < nsh; sh0 += 4 * NW) {
+      const int sh = sh0 + seg;
       double acc = 0.0;
-      for (int j = lane; j < nsh; j += 64) acc += g[j] * v.qsh[j];
-      acc = wave_sum(acc);
-      if (lane == 0) {
+      if (sh < nsh) {
+        const double* g = v.gam + (int64_t)sh * nsh;
+        for (int j = sl; j < nsh; j += 16) acc += g[j] * v.qsh[j];
+      }
+      acc = seg16_sum(acc);
+      if (sl == 0 && sh < nsh) {
         const double q = v.qsh[sh];
         vsh[sh] = acc + q * q * P.el[v.el[v.sh_at[sh]]].hubbard_derivs[v.sh_ish[sh]];
       }
     }
-    // AES2 atomic potentials: one wave per atom a, lanes over partners i, reading
-    // row a of every interaction plane (amat_sd is antisymmetric in the atom
-    // pair, amat_dd and amat_sq symmetric)
-    for (int a = wid; a < nat; a += NW) {
+    // AES2 atomic potentials: four atoms a per wave (16-lane segments), lanes
+    // over partners i, reading row a of every interaction plane (amat_sd is
+    // antisymmetric in the atom pair, amat_dd and amat_sq symmetric)
+    for (int a0 = 4 * wid; a0 < nat; a0 += 4 * NW) {
+      const int a = a0 + seg;
       double va = 0.0, d[3] = {0.0, 0.0, 0.0}, q[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
       const int64_t row = (int64_t)a * nat;
-      for (int i = lane; i < nat; i += 64) {
+      for (int i = sl; i < nat && a < nat; i += 16) {
         const int64_t ai = row + i;
         const double qi = v.qat[i];
         const double dpi[3] = {v.dpat[3 * i], v.dpat[3 * i + 1], v.dpat[3 * i + 2]};
@@ -938,12 +944,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
           q[k] += sqk * qi;
         }
       }
-      va = wave_sum(va);
+      va = seg16_sum(va);
 #pragma unroll
-      for (int k = 0; k < 3; ++k) d[k] = wave_sum(d[k]);
+      for (int k = 0; k < 3; ++k) d[k] = seg16_sum(d[k]);
 #pragma unroll
-      for (int k = 0; k < 6; ++k) q[k] = wave_sum(q[k]);
-      if (lane == 0) {
+      for (int k = 0; k < 6; ++k) q[k] = seg16_sum(q[k]);
+      if (sl == 0 && a < nat) {
         const DevElem& e = P.el[v.el[a]];
         vat[a] = va;
 #pragma unroll
@@ -1094,13 +1100,16 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
 
     // ------------------------------------------------ Mulliken + band energy
     double* part = work; // part[c * ld + i], c = 0 (PS), 1..3 (PD), 4..9 (PQ), 10 (PH)
-    for (int i = wid; i < n; i += NW) {
+    // four rows per wave (16-lane segments, lanes over j): four rows share
+    // each memory round trip
+    for (int i0 = 4 * wid; i0 < n; i0 += 4 * NW) {
+      const int i = i0 + seg;
       double acc[11];
 #pragma unroll
       for (int c = 0; c < 11; ++c) acc[c] = 0.0;
       const int64_t row = (int64_t)i * n;
 #pragma unroll 2
-      for (int j = lane; j < n; j += 64) {
+      for (int j = sl; j < n && i < n; j += 16) {
         const double p = Pm[aix(i, j)];
         const int64_t e = row + j;
         acc[0] += p * v.S[e];
@@ -1112,8 +1121,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
         acc[10] += p * v.H[e];
       }
 #pragma unroll
-      for (int c = 0; c < 11; ++c) acc[c] = wave_sum(acc[c]);
-      if (lane == 0)
+      for (int c = 0; c < 11; ++c) acc[c] = seg16_sum(acc[c]);
+      if (sl == 0 && i < n)
 #pragma unroll
         for (int c = 0; c < 11; ++c) part[c * ld + i] = acc[c];
     }
@@ -1168,42 +1177,38 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
     // ------------------------------------------------ electrostatic energy
     // E_ES2 = q^T gamma q / 2, E_ES3 = sum q^3 Gamma / 3, E_AES2 over atom
     // pairs (a, i): dp_a . (amat_sd(a,i) q_i + amat_dd(a,i) dp_i / 2)
-    // + qp_a . amat_sq(a,i) q_i, plus the on-site kernels; all threads, one
-    // reduction
+    // + qp_a . amat_sq(a,i) q_i, plus the on-site kernels; all threads over
+    // all shell and atom pairs (no per-row reduction needed), one reduction
     double ec = 0.0;
-    for (int sh = wid; sh < nsh; sh += NW) {
-      const double* g = v.gam + (int64_t)sh * nsh;
-      const double qs = v.qsh[sh];
-      for (int j = lane; j < nsh; j += 64) ec += 0.5 * qs * g[j] * v.qsh[j];
-      if (lane == 0)
-        ec += qs * qs * qs * P.el[v.el[v.sh_at[sh]]].hubbard_derivs[v.sh_ish[sh]] / 3.0;
+    for (int idx = tid; idx < nsh * nsh; idx += NT) {
+      const int sh = idx / nsh, j = idx - sh * nsh;
+      ec += 0.5 * v.qsh[sh] * v.gam[idx] * v.qsh[j];
     }
-    for (int a = wid; a < nat; a += NW) {
-      const int64_t row = (int64_t)a * nat;
-      const double dpa[3] = {v.dpat[3 * a], v.dpat[3 * a + 1], v.dpat[3 * a + 2]};
-      double qpa[6];
+    for (int sh = tid; sh < nsh; sh += NT) {
+      const double qs = v.qsh[sh];
+      ec += qs * qs * qs * P.el[v.el[v.sh_at[sh]]].hubbard_derivs[v.sh_ish[sh]] / 3.0;
+    }
+    for (int ai = tid; ai < nat * nat; ai += NT) {
+      const int a = ai / nat, i = ai - a * nat;
+      const double qi = v.qat[i];
+      const double dpi[3] = {v.dpat[3 * i], v.dpat[3 * i + 1], v.dpat[3 * i + 2]};
 #pragma unroll
-      for (int k = 0; k < 6; ++k) qpa[k] = v.qpat[6 * a + k];
-      for (int i = lane; i < nat; i += 64) {
-        const int64_t ai = row + i;
-        const double qi = v.qat[i];
-        const double dpi[3] = {v.dpat[3 * i], v.dpat[3 * i + 1], v.dpat[3 * i + 2]};
+      for (int k = 0; k < 3; ++k) {
+        double t = -v.sd[k * p2 + ai] * qi;
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          double t = -v.sd[k * p2 + ai] * qi;
-#pragma unroll
-          for (int l = 0; l < 3; ++l) t += 0.5 * v.dd[(3 * l + k) * p2 + ai] * dpi[l];
-          ec += dpa[k] * t;
-        }
-#pragma unroll
-        for (int k = 0; k < 6; ++k) ec += qpa[k] * v.sq[k * p2 + ai] * qi;
+        for (int l = 0; l < 3; ++l) t += 0.5 * v.dd[(3 * l + k) * p2 + ai] * dpi[l];
+        ec += v.dpat[3 * a + k] * t;
       }
-      if (lane == 0) {
-        const DevElem& e = P.el[v.el[a]];
-        ec += e.dkernel * (dpa[0] * dpa[0] + dpa[1] * dpa[1] + dpa[2] * dpa[2]);
-        ec += e.qkernel * (qpa[0] * qpa[0] + 2.0 * qpa[1] * qpa[1] + qpa[2] * qpa[2] +
-                           2.0 * qpa[3] * qpa[3] + 2.0 * qpa[4] * qpa[4] + qpa[5] * qpa[5]);
-      }
+#pragma unroll
+      for (int k = 0; k < 6; ++k) ec += v.qpat[6 * a + k] * v.sq[k * p2 + ai] * qi;
+    }
+    for (int a = tid; a < nat; a += NT) {
+      const DevElem& e = P.el[v.el[a]];
+      const double* dpa = v.dpat + 3 * a;
+      const double* qpa = v.qpat + 6 * a;
+      ec += e.dkernel * (dpa[0] * dpa[0] + dpa[1] * dpa[1] + dpa[2] * dpa[2]);
+      ec += e.qkernel * (qpa[0] * qpa[0] + 2.0 * qpa[1] * qpa[1] + qpa[2] * qpa[2] +
+                         2.0 * qpa[3] * qpa[3] + 2.0 * qpa[4] * qpa[4] + qpa[5] * qpa[5]);
     }
     ec = block_sum(ec, red);
     eband = block_sum(eband, red);

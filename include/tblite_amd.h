@@ -264,6 +264,28 @@ int tblite_amd_batch_get_energy_parts(tblite_amd_context ctx, int imol, double* 
  * [2] integrals/H0, [3] SCF, [4] total.  n = number of entries wanted. */
 int tblite_amd_last_timings(tblite_amd_context ctx, double* times, int n);
 
+/* ----------------------------------------------------------------- I/O
+ * Host-side helpers around the singlepoint (no device needed).  The
+ * reference has no I/O (its driver builds molecules in code, src/main.cu);
+ * the formats are the ones tblite reads.
+ *
+ * read_structure_file: an xyz file (Angstrom; element symbols or numbers) or
+ * a Turbomole coord file ($coord in bohr, optional $eht charge= unpaired=).
+ * format "xyz", "coord", or NULL to choose by name (*.xyz, else coord).
+ * Writes *nat always; fills num[nat] and xyz[3 nat] (bohr) when nat <=
+ * capacity (call with capacity 0 to size the buffers).  TBLITE_AMD_ERR_ARGUMENT
+ * for an unreadable file or an unparsable line.
+ *
+ * write_json: one singlepoint result as a JSON object (energy, SCF
+ * iterations, numbers, positions, charges, atomic dipoles and quadrupoles,
+ * orbital energies and occupations, molecular dipole); null arrays are
+ * left out. */
+int tblite_amd_read_structure_file(const char* path, const char* format, int capacity, int* nat,
+                                   int* num, double* xyz, double* charge, int* uhf);
+int tblite_amd_write_json(const char* path, int nat, const int* num, const double* xyz,
+                          double energy, int iterations, const double* qat, const double* dpat,
+                          const double* qpat, int nao, const double* emo, const double* focc);
+
 /* ------------------------------------------------------------- tuning
  * Which SCF path a molecule takes, by its padded AO dimension ld:
  *   ld <= lds_max_ld                 matrices in LDS, one workgroup per molecule

@@ -104,6 +104,8 @@ _PROTOS = {
     "tblite_amd_get_tuning": (_I, [_P, C.POINTER(Tuning)]),
     "tblite_amd_set_tuning": (_I, [_P, C.POINTER(Tuning)]),
     "tblite_amd_molecule_path": (_I, [_P, _I]),
+    "tblite_amd_read_structure_file": (_I, [C.c_char_p, C.c_char_p, _I, _IP, _IP, _DP, _DP, _IP]),
+    "tblite_amd_write_json": (_I, [C.c_char_p, _I, _IP, _DP, _D, _I, _DP, _DP, _DP, _I, _DP, _DP]),
     "tblite_amd_new_server": (_P, [_P, _P, _I, _D]),
     "tblite_amd_delete_server": (None, [C.POINTER(_P)]),
     "tblite_amd_server_submit": (_I, [_P, _I, _IP, _DP, _D, _I, C.POINTER(C.c_longlong)]),
@@ -153,6 +155,38 @@ def default_param():
     p = Param()
     lib.tblite_amd_param_default(C.byref(p))
     return p
+
+
+def read_structure(path, fmt=None):
+    """(num, xyz_bohr (nat, 3), charge, uhf) from an xyz or Turbomole coord file."""
+    lib = load()
+    nat, uhf = C.c_int(), C.c_int()
+    charge = C.c_double()
+    p = str(path).encode()
+    f = fmt.encode() if fmt else None
+    if lib.tblite_amd_read_structure_file(p, f, 0, C.byref(nat), None, None, C.byref(charge),
+                                          C.byref(uhf)) != 0:
+        raise ValueError(f"cannot read structure from {path}")
+    num = np.zeros(nat.value, dtype=np.int32)
+    xyz = np.zeros((nat.value, 3))
+    if lib.tblite_amd_read_structure_file(p, f, nat.value, C.byref(nat), _ip(num), _dp(xyz),
+                                          C.byref(charge), C.byref(uhf)) != 0:
+        raise ValueError(f"cannot read structure from {path}")
+    return num, xyz, charge.value, uhf.value
+
+
+def write_json(path, num, xyz, energy, iterations=0, qat=None, dpat=None, qpat=None, emo=None,
+               focc=None):
+    """Write one singlepoint result (as from Engine.singlepoint) as JSON."""
+    f = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+    num = np.ascontiguousarray(num, dtype=np.int32)
+    xyz, qat, dpat, qpat, emo, focc = (f(a) for a in (xyz, qat, dpat, qpat, emo, focc))
+    nao = 0 if emo is None else len(emo)
+    rc = load().tblite_amd_write_json(str(path).encode(), len(num), _ip(num), _dp(xyz),
+                                      float(energy), int(iterations), _dp(qat), _dp(dpat),
+                                      _dp(qpat), nao, _dp(emo), _dp(focc))
+    if rc != 0:
+        raise ValueError(f"cannot write {path}")
 
 
 class EngineError(RuntimeError):

@@ -161,6 +161,9 @@ struct BatchDev {
   double* big_s;
   const int64_t* mol_scr_off;
 
+  // Jacobi stopping rule: the solve ends after the sweep whose largest rotated
+  // off-diagonal element is below this
+  double jacobi_tol;
   // optional per-molecule phase clocks of the SCF kernel (TA_PROF_SLOTS each), may be null
   long long* prof;
   // optional completion records (one per molecule, pinned host memory), may be null

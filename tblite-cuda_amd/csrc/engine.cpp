@@ -253,6 +253,13 @@ struct tblite_amd_context_s {
   // replay the multi-workgroup path's eigenvalue phase from a HIP graph
   // (TBLITE_AMD_GRID_GRAPH=0 launches the kernels one by one instead)
   bool grid_graph = true;
+  // Jacobi stopping threshold on the largest rotated element of a sweep
+  // (TBLITE_AMD_JACOBI_TOL may change it).  After such a sweep the
+  // off-diagonal elements are of order tol^2 / gap: 1e-8 / gap here, which
+  // moves energies at second order only (4096 water clusters: |dE| vs the
+  // CPU oracle 1.6e-10 Eh, identical iteration counts; 1e-6 costs one more
+  // sweep per SCF iteration, 9 % of the batch time)
+  double jacobi_tol = 1.0e-4;
   // cost-model routing between one workgroup per molecule and the
   // multi-workgroup path (seconds per SCF iteration, measured on MI355X by
   // autotune.py: one workgroup alone wg_a * ld^wg_p, across the device
@@ -733,6 +740,7 @@ int upload_plan(tblite_amd_context ctx, Calculator& c, const Plan& pl, DeviceBat
   b.big_ct = carve<double>(ar, std::max<int64_t>(pl.tot_big, 1));
   b.big_s = carve<double>(ar, std::max<int64_t>(pl.tot_scr, 1));
   b.intcut = 0.0;
+  b.jacobi_tol = ctx->jacobi_tol;
   b.prof = ctx->profile ? carve<long long>(ar, (int64_t)nmol * TA_PROF_SLOTS) : nullptr;
   db.bytes = ar.used;
   if (ar.used > ar.cap) {
@@ -1091,6 +1099,8 @@ static tblite_amd_context make_context(int device, bool single_stream)
   if (pg) ctx->grid_min_ld = std::max(16, std::min(TA_BIG_MAXLD, std::atoi(pg)));
   const char* pgg = std::getenv("TBLITE_AMD_GRID_GRAPH");
   ctx->grid_graph = !(pgg && pgg[0] == '0');
+  const char* pj = std::getenv("TBLITE_AMD_JACOBI_TOL");
+  if (pj) ctx->jacobi_tol = std::max(1e-12, std::min(1e-2, std::atof(pj)));
   const char* pl = std::getenv("TBLITE_AMD_LDS_MAX_LD");
   if (pl) ctx->lds_max_ld = std::max(16, std::min(kLdsMaxLd, std::atoi(pl) / 16 * 16));
   if (!ok) {

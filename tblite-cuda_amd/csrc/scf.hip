@@ -45,7 +45,6 @@ __device__ __forceinline__ int sk(int i, int j, int ld)
   c = c >= ld ? c - ld : c;
   return i * ld + c;
 }
-#define JACOBI_TOL 1.0e-6
 
 struct MolView {
   int m, n, ld, nat, nsh, a0, s0, o0, ndim, mem;
@@ -230,7 +229,7 @@ __device__ bool orthonormalizer(double* A, double* C, int n, int ld, double* sca
 // molecules).
 //
 // Convergence: a sweep whose largest rotated off-diagonal element is below
-// JACOBI_TOL leaves off-diagonals of order JACOBI_TOL^2 / gap (quadratic
+// a tolerance tol leaves off-diagonals of order tol^2 / gap (quadratic
 // convergence), so the solve stops after that sweep.
 struct JRot {
   int2* pq;      // [2][h] indices (p, q) held by the slot
@@ -317,7 +316,7 @@ __device__ __forceinline__ double& aup(double* A, int ld, int i, int j)
 
 template <int NT, int MAXB, int MAXC>
 __device__ bool jacobi(double* A, double* CT, int ld, const JRot& R, const JPlan<MAXB, MAXC>& pl,
-                       double* red, int* sweeps_out)
+                       double* red, double tol, int* sweeps_out)
 {
   const int h = ld >> 1;
   const int lane = threadIdx.x & 63;
@@ -435,7 +434,7 @@ __device__ bool jacobi(double* A, double* CT, int ld, const JRot& R, const JPlan
       __syncthreads();
     }
     offmax = block_max(offmax, red);
-    if (offmax < JACOBI_TOL) {
+    if (offmax < tol) {
       ok = true;
       ++sweep;
       break;
@@ -1027,7 +1026,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
       mm_nn<NT, MAXT, true, false>(C, A, A, ld);
       clk.lap(TA_PROF_TRANSFORM);
       int sweeps = 0;
-      const bool jok = jacobi<NT, MAXB, MAXC>(A, C, ld, jr, jplan, red, &sweeps);
+      const bool jok = jacobi<NT, MAXB, MAXC>(A, C, ld, jr, jplan, red, b.jacobi_tol, &sweeps);
       clk.lap(TA_PROF_JACOBI);
       clk.acc[TA_PROF_SWEEPS] += sweeps;
       if (!jok) {

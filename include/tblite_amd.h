@@ -85,6 +85,43 @@ typedef struct {
   double atomic_rad;                      /* atomic radius for shpoly / bohr */
 } tblite_amd_element;
 
+/* ------------------------------------------------- D4 dispersion data
+ * Self-consistent D4 dispersion of GFN2 (tblite's d4_dispersion: the hooks
+ * dispersion%get_potential / get_energy of next_scf, src/scf/iterators.cu:353-354
+ * and :415-417, and dispersion_get_engrad of xtb_singlepoint,
+ * src/xtb.cu:274-277, all left as comments in the reference).
+ *
+ * Reference polarisabilities alpha(i w) are tabulated on the fixed 23-point
+ * imaginary-frequency grid of the D4 model (w = 1e-6, 0.05, 0.1, 0.2, ..., 1.0,
+ * 1.2, 1.4, 1.6, 1.8, 2.0, 2.5, 3.0, 4.0, 5.0, 7.5, 10.0 Eh), integrated with
+ * trapezoidal weights (Casimir-Polder).  Element records follow the order of
+ * tblite_amd_param.elem. */
+#define TBLITE_AMD_D4_MAXREF 7
+#define TBLITE_AMD_D4_NFREQ 23
+
+typedef struct {
+  int nref;                                                 /* reference systems (1..7) */
+  double cn[TBLITE_AMD_D4_MAXREF];                          /* reference coordination numbers */
+  double q[TBLITE_AMD_D4_MAXREF];                           /* reference atomic charges */
+  double alpha[TBLITE_AMD_D4_MAXREF][TBLITE_AMD_D4_NFREQ];  /* alpha(i w) / bohr^3 */
+  double zeff;                                              /* effective nuclear charge */
+  double eta;                                               /* chemical hardness / Eh */
+  double r4r2;                                              /* sqrt(0.5 <r^4>/<r^2> sqrt(Z)) */
+} tblite_amd_d4_element;
+
+typedef struct {
+  int enabled;                  /* 0: no dispersion term */
+  double s6, s8, a1, a2;        /* rational (Becke-Johnson) damping of the two-body term */
+  double s9, alp;               /* three-body (Axilrod-Teller-Muto) scaling and damping exponent */
+  double ga, gc;                /* charge scaling: exp(ga (1 - exp(gc eta (1 - (z+q_ref)/(z+q))))) */
+  double wf;                    /* Gaussian weighting of the reference CNs */
+  double cn_k4, cn_k5, cn_k6;   /* electronegativity factor of the D4 CN */
+  double cn_kn;                 /* steepness of its error-function count */
+  double cn_cutoff;             /* real-space cutoffs / bohr: CN, two-body, three-body */
+  double disp2_cutoff, disp3_cutoff;
+  tblite_amd_d4_element elem[TBLITE_AMD_MAX_ELEM];
+} tblite_amd_d4_param;
+
 /* Global GFN2 parameters plus the element table. */
 typedef struct {
   double kshell[3][3];       /* H0 shell-pair scaling k_ll' */
@@ -106,6 +143,7 @@ typedef struct {
   double cn_cutoff;          /* CN real-space cutoff / bohr */
   int nelem;
   tblite_amd_element elem[TBLITE_AMD_MAX_ELEM];
+  tblite_amd_d4_param d4;    /* dispersion (element records in the order of elem) */
 } tblite_amd_param;
 
 /* Fill *param with the built-in GFN2 data for H, C, N, O and S. */
@@ -255,9 +293,16 @@ int tblite_amd_batch_get_selfenergy(tblite_amd_context ctx, int imol, double* se
  * dpat(3,nat), qpat(6,nat), emo(nao), focc(nao); any pointer may be NULL. */
 int tblite_amd_batch_get_charges(tblite_amd_context ctx, int imol, double* qsh, double* qat,
                                  double* dpat, double* qpat, double* emo, double* focc);
-/* Energy decomposition: erep, eelec (band + entropy + ES2 + AES2 + ES3). */
+/* Energy decomposition: erep, eelec (band + entropy + ES2 + AES2 + ES3 + D4
+ * two-body); energy = erep + edisp3 + eelec. */
 int tblite_amd_batch_get_energy_parts(tblite_amd_context ctx, int imol, double* erep,
                                       double* eelec);
+/* Dispersion: D4 two-body energy at the final charges (part of eelec; the
+ * dispersion%get_energy hook of next_scf, src/scf/iterators.cu:415-417) and
+ * the three-body energy at zero charges (dispersion_get_engrad,
+ * src/xtb.cu:274-277); both zero when the parameter set disables D4. */
+int tblite_amd_batch_get_dispersion(tblite_amd_context ctx, int imol, double* edisp2,
+                                    double* edisp3);
 
 /* Milliseconds spent in each engine phase during the last batched call:
  * times[0] setup (CN, repulsion, Coulomb matrices), [1] neighbour list,

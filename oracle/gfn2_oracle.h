@@ -27,7 +27,9 @@ int or_dims(const or_calc* calc, int nat, const int* num, int* nsh, int* nao);
 typedef struct {
   double energy;  /* total energy */
   double erep;    /* repulsion */
-  double eelec;   /* electronic: band + entropy + ES2 + AES2 + ES3 */
+  double eelec;   /* electronic: band + entropy + ES2 + AES2 + ES3 + D4 two-body */
+  double edisp2;  /* D4 two-body part of eelec (self-consistent charges) */
+  double edisp3;  /* D4 three-body energy (zero charges); energy = erep + edisp3 + eelec */
   int iterations;
   int converged;
   int status;     /* TBLITE_AMD_STATUS_* bits */
@@ -53,6 +55,15 @@ int or_singlepoint_many(const or_calc* calc, int nmol, const int* mol_nat, const
                         int* status);
 
 /* ---- stage functions, exported for stage-level checks ---- */
+/* D4: coordination numbers; reference weights of one atom (returns nref);
+ * reference C6 (Casimir-Polder) of two element references (-1 if invalid);
+ * per-atom two-body energy at charges qat, three-body energy (zero charges)
+ * and the two-body potential dE2/dq. */
+int or_d4_ncoord(const or_calc* calc, int nat, const int* num, const double* xyz, double* cn);
+int or_d4_weights(const or_calc* calc, int z, double cn, double q, double* gw, double* dgw);
+double or_d4_c6ref(const or_calc* calc, int zi, int ri, int zj, int rj);
+int or_d4_eval(const or_calc* calc, int nat, const int* num, const double* xyz, const double* qat,
+               double* e2, double* e3, double* vat);
 void or_multipole_3d(const double rpj[3], const double rpi[3], double aj, double ai,
                      const int lj[3], const int li[3], const double* s1d, double* s3d,
                      double d3d[3], double q3d[6]);

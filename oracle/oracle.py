@@ -29,7 +29,7 @@ def build(force=False):
 class Result(C.Structure):
     _fields_ = [
         ("energy", C.c_double), ("erep", C.c_double), ("eelec", C.c_double),
-        ("iterations", C.c_int), ("converged", C.c_int), ("status", C.c_int),
+        ("edisp2", C.c_double), ("edisp3", C.c_double), ("iterations", C.c_int), ("converged", C.c_int), ("status", C.c_int),
         ("qsh", C.POINTER(C.c_double)), ("qat", C.POINTER(C.c_double)),
         ("dpat", C.POINTER(C.c_double)), ("qpat", C.POINTER(C.c_double)),
         ("emo", C.POINTER(C.c_double)), ("focc", C.POINTER(C.c_double)),
@@ -90,6 +90,12 @@ def lib():
         L.or_entropy.argtypes = [C.c_int, C.POINTER(C.c_double), C.c_double]
         L.or_coulomb_eval.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)] + \
             [C.POINTER(C.c_double)] * 8
+        L.or_d4_ncoord.argtypes = L.or_ncoord.argtypes
+        L.or_d4_weights.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double,
+                                    C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.or_d4_c6ref.restype = C.c_double
+        L.or_d4_c6ref.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.or_d4_eval.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)] + [C.POINTER(C.c_double)] * 5
         _lib = L
     return _lib
 
@@ -130,7 +136,8 @@ class Oracle:
             setattr(r, k, _dp(out[k]))
         rc = self.L.or_singlepoint(self.calc, nat, _ip(num), _dp(xyz), charge, uhf, accuracy,
                                    temperature * KB_EH, max_iter, damping, C.byref(r))
-        out.update(energy=r.energy, erep=r.erep, eelec=r.eelec, iterations=r.iterations,
+        out.update(energy=r.energy, erep=r.erep, eelec=r.eelec, edisp2=r.edisp2,
+                   edisp3=r.edisp3, iterations=r.iterations,
                    converged=bool(r.converged), status=r.status, rc=rc)
         return out
 
@@ -214,6 +221,34 @@ class Oracle:
         self.L.or_coulomb_eval(self.calc, nat, _ip(num), _dp(xyz), _dp(f(qsh)), _dp(f(dpat)), _dp(f(qpat)),
                                _dp(e), _dp(vsh), _dp(vat), _dp(vdp), _dp(vqp))
         return e, vsh, vat, vdp, vqp
+
+    def d4_cn(self, num, xyz):
+        num, xyz = self._mol(num, xyz)
+        out = np.zeros(len(num))
+        if self.L.or_d4_ncoord(self.calc, len(num), _ip(num), _dp(xyz), _dp(out)):
+            raise ValueError("unknown element")
+        return out
+
+    def d4_weights(self, z, cn, q):
+        gw, dgw = np.zeros(7), np.zeros(7)
+        nref = self.L.or_d4_weights(self.calc, int(z), float(cn), float(q), _dp(gw), _dp(dgw))
+        if nref < 0:
+            raise ValueError("no D4 data")
+        return gw[:nref].copy(), dgw[:nref].copy()
+
+    def d4_c6ref(self, zi, ri, zj, rj):
+        return self.L.or_d4_c6ref(self.calc, int(zi), int(ri), int(zj), int(rj))
+
+    def d4_eval(self, num, xyz, qat):
+        """Per-atom D4 two-body energy at charges qat, three-body energy (zero
+        charges) and the two-body potential dE2/dq."""
+        num, xyz = self._mol(num, xyz)
+        nat = len(num)
+        e2, e3, vat = np.zeros(nat), np.zeros(nat), np.zeros(nat)
+        if self.L.or_d4_eval(self.calc, nat, _ip(num), _dp(xyz),
+                             _dp(np.ascontiguousarray(qat, dtype=np.float64)), _dp(e2), _dp(e3), _dp(vat)):
+            raise ValueError("D4 disabled or unknown element")
+        return e2, e3, vat
 
     def cgto(self, z, ish):
         ang, npr = C.c_int(), C.c_int()

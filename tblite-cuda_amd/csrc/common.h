@@ -32,6 +32,25 @@ struct DevElem {
   double rcov, atomic_rad, zeff, arep, dkernel, qkernel, mprad, mpvcn, en;
 };
 
+#define TA_D4_MAXREF 7
+
+// D4 dispersion data in device form (zeff, gi = gc * eta, reference C6 of
+// every element/reference pair from the Casimir-Polder integral)
+struct DevD4 {
+  int enabled;
+  // every element's references share one reference charge: the charge scale
+  // factors out, g_r(q) = w_r(CN) zeta(q), and the SCF works with
+  // s = zeta(q) / zeta(0) per atom and d_ij C6_ij(q = 0) per pair
+  int qref_uniform;
+  double s6, s8, a1, a2, s9, alp, ga, wf;
+  double k4, k5, k6, kn, cn_cutoff, disp2_cutoff, disp3_cutoff;
+  int nref[TA_MAX_ELEM];
+  double cnref[TA_MAX_ELEM][TA_D4_MAXREF];
+  double qref[TA_MAX_ELEM][TA_D4_MAXREF];
+  double zeff[TA_MAX_ELEM], gi[TA_MAX_ELEM], r4r2[TA_MAX_ELEM];
+  double c6[TA_MAX_ELEM][TA_MAX_ELEM][TA_D4_MAXREF][TA_D4_MAXREF];
+};
+
 struct DevParams {
   int nelem;
   DevElem el[TA_MAX_ELEM];
@@ -41,6 +60,7 @@ struct DevParams {
   double gexp;
   double dmp3, dmp5, mp_shift, mp_kexp, mp_rmax;
   double cn_ka, cn_kb, cn_rshift, cn_cutoff;
+  DevD4 d4;
 };
 
 // Pointers into one batch's device buffers (passed by value to kernels).
@@ -161,6 +181,23 @@ struct BatchDev {
   double* big_s;
   const int64_t* mol_scr_off;
 
+  // D4 dispersion: per atom the D4 CN, the reference weights g_r(q) and
+  // dg_r/dq of the current charges (TA_D4_MAXREF each, SCF scratch; with
+  // DevD4::qref_uniform only the first entry, the charge scale s and ds/dq), the
+  // three-body energy of a chunk of triples (e3[a]: triples whose largest atom
+  // index lies in the chunk starting at atom a, zero for other atoms); per
+  // atom pair (nat x nat per molecule, mol_pair_off) the two-body damping
+  // factors d_ij (d_ij C6_ij(0) with DevD4::qref_uniform) and the zero-charge
+  // C6 of the three-body term; per molecule
+  // the final two-body and the three-body energy
+  double* d4_cn;
+  double* d4_gw;
+  double* d4_dgw;
+  double* d4_e3;
+  double* d4_disp;
+  double* d4_c60;
+  double* edisp2;
+  double* edisp3;
   // Jacobi stopping rule: the solve ends after the sweep whose largest rotated
   // off-diagonal element is below this
   double jacobi_tol;

@@ -25,6 +25,7 @@
 __global__ void k_fill(double* p, int64_t n, double v);
 __global__ void k_setup(BatchDev b);
 __global__ void k_adjlist(BatchDev b);
+__global__ void k_d4_atm(BatchDev b, const int* work_mol, const int* work_i0, const int* work_i1);
 __global__ void k_integrals(BatchDev b, const int* work_mol, const int* work_kind,
                             const int* work_start);
 template <int NT, int MAXT, int MAXB, int MAXC, bool MATS_LDS>
@@ -33,6 +34,7 @@ __global__ void k_scf(BatchDev b, const int* mols, int ldmax, int natmax, int ns
 namespace {
 
 constexpr int kIntegralChunk = 128;
+constexpr int64_t kAtmChunk = 1 << 18;  // triples per three-body dispersion work item
 constexpr int kLdsMaxLd = 96;
 constexpr int kNumStreams = 4;
 constexpr int kMaxBuckets = 8;
@@ -155,6 +157,56 @@ bool build_calculator(Calculator& c, std::string& err)
   c.dev.cn_kb = p.cn_kb;
   c.dev.cn_rshift = p.cn_rshift;
   c.dev.cn_cutoff = p.cn_cutoff;
+
+  // D4: device copy of the damping / weighting constants and the reference
+  // C6 of every element and reference pair, C6 = 3/pi int alpha_A alpha_B dw
+  // (trapezoid rule on the model's imaginary-frequency grid)
+  const tblite_amd_d4_param& d4 = p.d4;
+  DevD4& dd = c.dev.d4;
+  dd.enabled = d4.enabled ? 1 : 0;
+  if (dd.enabled) {
+    dd.s6 = d4.s6; dd.s8 = d4.s8; dd.a1 = d4.a1; dd.a2 = d4.a2;
+    dd.s9 = d4.s9; dd.alp = d4.alp; dd.ga = d4.ga; dd.wf = d4.wf;
+    dd.k4 = d4.cn_k4; dd.k5 = d4.cn_k5; dd.k6 = d4.cn_k6; dd.kn = d4.cn_kn;
+    dd.cn_cutoff = d4.cn_cutoff;
+    dd.disp2_cutoff = d4.disp2_cutoff;
+    dd.disp3_cutoff = d4.disp3_cutoff;
+    double w[TBLITE_AMD_D4_NFREQ];
+    for (int k = 0; k < TBLITE_AMD_D4_NFREQ; ++k) {
+      const double* f = tblite_amd_d4_freq;
+      const double left = k > 0 ? f[k] - f[k - 1] : 0.0;
+      const double right = k + 1 < TBLITE_AMD_D4_NFREQ ? f[k + 1] - f[k] : 0.0;
+      w[k] = 0.5 * (left + right);
+    }
+    for (int e = 0; e < p.nelem; ++e) {
+      const tblite_amd_d4_element& de = d4.elem[e];
+      if (de.nref < 1 || de.nref > TA_D4_MAXREF || !(de.zeff > 0.0)) {
+        err = "D4 data missing or invalid for an element (set d4.enabled = 0 to run without dispersion)";
+        return false;
+      }
+      dd.nref[e] = de.nref;
+      for (int r = 0; r < de.nref; ++r) {
+        dd.cnref[e][r] = de.cn[r];
+        dd.qref[e][r] = de.q[r];
+      }
+      dd.zeff[e] = de.zeff;
+      dd.gi[e] = d4.gc * de.eta;
+      dd.r4r2[e] = de.r4r2;
+    }
+    dd.qref_uniform = 1;
+    for (int e = 0; e < p.nelem; ++e)
+      for (int r = 1; r < d4.elem[e].nref; ++r)
+        if (d4.elem[e].q[r] != d4.elem[e].q[0]) dd.qref_uniform = 0;
+    for (int ei = 0; ei < p.nelem; ++ei)
+      for (int ej = 0; ej < p.nelem; ++ej)
+        for (int r = 0; r < d4.elem[ei].nref; ++r)
+          for (int s = 0; s < d4.elem[ej].nref; ++s) {
+            double acc = 0.0;
+            for (int k = 0; k < TBLITE_AMD_D4_NFREQ; ++k)
+              acc += w[k] * d4.elem[ei].alpha[r][k] * d4.elem[ej].alpha[s][k];
+            dd.c6[ei][ej][r][s] = 3.0 / M_PI * acc;
+          }
+  }
   return true;
 }
 
@@ -199,6 +251,8 @@ struct Plan {
   std::vector<int> sh_at, sh_ish, sh_iao;
   std::vector<int> ao_at, ao_sh;
   std::vector<int> work_mol, work_kind, work_start;
+  // three-body dispersion work: chunks [atm_i0, atm_i1) of anchor atoms
+  std::vector<int> atm_mol, atm_i0, atm_i1;
   std::vector<int> grid_mols;  // molecules run by the multi-workgroup SCF path
   struct Bucket {
     int nt, maxt;
@@ -218,6 +272,9 @@ struct DeviceBatch {
   int* d_work_mol = nullptr;
   int* d_work_kind = nullptr;
   int* d_work_start = nullptr;
+  int* d_atm_mol = nullptr;
+  int* d_atm_i0 = nullptr;
+  int* d_atm_i1 = nullptr;
   std::vector<int*> d_bucket_mols;
   size_t bytes = 0;
 };
@@ -504,6 +561,22 @@ int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* 
       pl.work_kind.push_back(1);
       pl.work_start.push_back(s);
     }
+    // three-body work: anchor atom i owns the triples k < j < i; chunks of
+    // consecutive anchors hold about kAtmChunk triples each
+    if (c.dev.d4.enabled) {
+      int i0 = 0;
+      int64_t acc = 0;
+      for (int i = 0; i < nat; ++i) {
+        acc += (int64_t)i * (i - 1) / 2;
+        if (acc >= kAtmChunk || i == nat - 1) {
+          pl.atm_mol.push_back(m);
+          pl.atm_i0.push_back(i0);
+          pl.atm_i1.push_back(i + 1);
+          i0 = i + 1;
+          acc = 0;
+        }
+      }
+    }
   }
   pl.tot_atoms = atom;
   pl.tot_sh = shell;
@@ -612,6 +685,9 @@ int upload_plan(tblite_amd_context ctx, Calculator& c, const Plan& pl, DeviceBat
   host_pack(img, seg, pl.work_start);     // 26
   host_pack(img, seg, pl.mol_big_off);    // 27
   host_pack(img, seg, pl.mol_scr_off);    // 28
+  host_pack(img, seg, pl.atm_mol);        // 29
+  host_pack(img, seg, pl.atm_i0);         // 30
+  host_pack(img, seg, pl.atm_i1);         // 31
   const size_t nfixed = seg.size();
   for (const auto& bk : pl.buckets) host_pack(img, seg, bk.mols);
 
@@ -632,6 +708,7 @@ int upload_plan(tblite_amd_context ctx, Calculator& c, const Plan& pl, DeviceBat
   need += 4 * dbytes(std::max<int64_t>(pl.tot_big, 1));
   need += dbytes(std::max<int64_t>(pl.tot_scr, 1));
   need += dbytes((int64_t)nmol * TA_PROF_SLOTS);
+  need += 2 * dbytes(A) + 2 * dbytes(TA_D4_MAXREF * A) + 2 * dbytes(pl.tot_pair) + 2 * dbytes(nmol); // D4
   need += 64 * 256;
 
   Arena& ar = ctx->arena;
@@ -691,6 +768,9 @@ int upload_plan(tblite_amd_context ctx, Calculator& c, const Plan& pl, DeviceBat
   db.d_work_start = I(26);
   b.mol_big_off = L(27);
   b.mol_scr_off = L(28);
+  db.d_atm_mol = I(29);
+  db.d_atm_i0 = I(30);
+  db.d_atm_i1 = I(31);
   db.d_bucket_mols.clear();
   for (size_t k = 0; k < pl.buckets.size(); ++k) db.d_bucket_mols.push_back(I(nfixed + k));
 
@@ -739,6 +819,14 @@ int upload_plan(tblite_amd_context ctx, Calculator& c, const Plan& pl, DeviceBat
   b.big_t = carve<double>(ar, std::max<int64_t>(pl.tot_big, 1));
   b.big_ct = carve<double>(ar, std::max<int64_t>(pl.tot_big, 1));
   b.big_s = carve<double>(ar, std::max<int64_t>(pl.tot_scr, 1));
+  b.d4_cn = carve<double>(ar, A);
+  b.d4_e3 = carve<double>(ar, A);
+  b.d4_gw = carve<double>(ar, TA_D4_MAXREF * A);
+  b.d4_dgw = carve<double>(ar, TA_D4_MAXREF * A);
+  b.d4_disp = carve<double>(ar, pl.tot_pair);
+  b.d4_c60 = carve<double>(ar, pl.tot_pair);
+  b.edisp2 = carve<double>(ar, nmol);
+  b.edisp3 = carve<double>(ar, nmol);
   b.intcut = 0.0;
   b.jacobi_tol = ctx->jacobi_tol;
   b.prof = ctx->profile ? carve<long long>(ar, (int64_t)nmol * TA_PROF_SLOTS) : nullptr;
@@ -814,6 +902,14 @@ int run_grid_molecule(tblite_amd_context ctx, Calculator& c, int m)
   g.sq = b.aes_sq + 6 * poff;
   g.n0sh = b.n0sh + s0;
   g.erep_at = b.erep + a0;
+  g.d4_on = c.dev.d4.enabled;
+  g.d4_cn = b.d4_cn + a0;
+  g.d4_disp = b.d4_disp + poff;
+  g.d4_e3 = b.d4_e3 + a0;
+  g.d4_gw = b.d4_gw + TA_D4_MAXREF * (int64_t)a0;
+  g.d4_dgw = b.d4_dgw + TA_D4_MAXREF * (int64_t)a0;
+  g.edisp2 = b.edisp2 + m;
+  g.edisp3 = b.edisp3 + m;
   g.qsh = b.qsh + s0;
   g.qat = b.qat + a0;
   g.dpat = b.dpat + 3 * (int64_t)a0;
@@ -862,6 +958,7 @@ int run_grid_molecule(tblite_amd_context ctx, Calculator& c, int m)
   g.vat = take(nat);
   g.vdp = take(3 * (int64_t)nat);
   g.vqp = take(6 * (int64_t)nat);
+  g.ed4 = take(nat);
   g.dots = take(2 * TA_BIG_NP);
   g.tm = take(TA_BIG_NB * TA_BIG_NB);
   g.sc = take(GS_COUNT);
@@ -909,6 +1006,14 @@ int run_batch(tblite_amd_context ctx, Calculator& c, const double* xyz, double a
   }
   hipLaunchKernelGGL(k_setup, dim3(pl.nmol), dim3(256), 0, st, b);
   HIP_TRY(ctx, hipGetLastError());
+  // three-body dispersion (writes every atom's d4_e3 entry; needs the D4 CN
+  // and zero-charge C6 of k_setup)
+  const int natm = (int)pl.atm_mol.size();
+  if (natm > 0) {
+    hipLaunchKernelGGL(k_d4_atm, dim3(natm), dim3(256), 0, st, b, db.d_atm_mol, db.d_atm_i0,
+                       db.d_atm_i1);
+    HIP_TRY(ctx, hipGetLastError());
+  }
   HIP_TRY(ctx, hipEventRecord(ctx->ev[1], st));
   hipLaunchKernelGGL(k_adjlist, dim3(pl.nmol), dim3(256), 0, st, b);
   HIP_TRY(ctx, hipGetLastError());
@@ -1558,6 +1663,21 @@ int tblite_amd_batch_get_energy_parts(tblite_amd_context ctx, int imol, double* 
   return TBLITE_AMD_OK;
 }
 
+int tblite_amd_batch_get_dispersion(tblite_amd_context ctx, int imol, double* edisp2,
+                                    double* edisp3)
+{
+  int rc = check_batch(ctx, imol);
+  if (rc) return rc;
+  const BatchDev& b = ctx->batch.b;
+  // the SCF kernels write both slots of every molecule (zero without D4)
+  double e[2] = {0.0, 0.0};
+  HIP_TRY(ctx, hipMemcpy(&e[0], b.edisp2 + imol, 8, hipMemcpyDeviceToHost));
+  HIP_TRY(ctx, hipMemcpy(&e[1], b.edisp3 + imol, 8, hipMemcpyDeviceToHost));
+  if (edisp2) *edisp2 = e[0];
+  if (edisp3) *edisp3 = e[1];
+  return TBLITE_AMD_OK;
+}
+
 int tblite_amd_last_profile(tblite_amd_context ctx, long long* out)
 {
   if (!ctx || !out) return TBLITE_AMD_ERR_ARGUMENT;
@@ -1706,8 +1826,8 @@ size_t request_bytes(const Calculator& c, int nat, const int* num, int* nao_out)
   const int64_t mem = std::max(1, c.max_iter);
   const int64_t ndim = nsh + 9 * (int64_t)nat;
   const int64_t ld = (nao + 3) & ~3;
-  int64_t d = 11 * (int64_t)nao * nao + (int64_t)nsh * nsh + 18 * (int64_t)nat * nat;
-  d += (int64_t)nat * (nat + 1) / 2 + 24 * (int64_t)nat + 6 * nsh + 4 * nao;
+  int64_t d = 11 * (int64_t)nao * nao + (int64_t)nsh * nsh + 20 * (int64_t)nat * nat;
+  d += (int64_t)nat * (nat + 1) / 2 + (24 + 2 + 2 * TA_D4_MAXREF) * (int64_t)nat + 6 * nsh + 4 * nao + 2;
   d += 4 * ndim + 2 * ndim * mem + mem * mem + mem * (mem + 1) + 2 * mem + 8 + TA_PROF_SLOTS;
   if (ld > kLdsMaxLd) d += 4 * ld * ld + big_scratch_doubles((int)ld);
   if (nao_out) *nao_out = nao;

@@ -29,6 +29,7 @@
 #include <math.h>
 
 #include "common.h"
+#include "d4.h"
 #include "device_util.h"
 #include "eig_big.h"
 #include "fermi.h"
@@ -591,6 +592,60 @@ __device__ __noinline__ void big_density(const double* CT_, const double* f_, co
       tmp);
 }
 
+// ------------------------------------------------------------- D4 phases
+// Kept out of line so that their reference-weight accumulators do not add to
+// the register pressure of the SCF kernel's other phases.
+//
+// Potential dE2/dq_a added to vat[a]: four atoms per wave (16-lane segments),
+// lanes over partners i; the lane with sl == 0 of atom a's segment writes.
+template <int NW>
+__device__ __noinline__ void d4_potential(const DevD4& d4, const int* __restrict__ el, int nat,
+                                          const double* __restrict__ disp,
+                                          const double* __restrict__ gw,
+                                          const double* __restrict__ dgw, lds_dbl* vat)
+{
+  const int lane = threadIdx.x & 63, wid = wave_id();
+  const int seg = lane >> 4, sl = lane & 15;
+  for (int a0 = 4 * wid; a0 < nat; a0 += 4 * NW) {
+    const int a = a0 + seg;
+    double vv[TA_D4_MAXREF] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const int ea = a < nat ? el[a] : 0;
+    for (int i = sl; i < nat && a < nat; i += 16) {
+      const double dai = disp[(int64_t)a * nat + i];
+      if (dai != 0.0) d4_scf_accumulate(d4, ea, el[i], dai, gw + TA_D4_MAXREF * i, vv);
+    }
+    double va = 0.0;
+#pragma unroll
+    for (int r = 0; r < TA_D4_MAXREF; ++r) {
+      vv[r] = seg16_sum(vv[r]);
+      if (a < nat && r < d4_nw(d4, ea)) va += vv[r] * dgw[TA_D4_MAXREF * a + r];
+    }
+    if (sl == 0 && a < nat) vat[a] += va;
+  }
+}
+
+// This thread's share of E2 = 1/2 sum_{a != i} sum_rs g_ar d_ai C6ref g_is,
+// threads over all atom pairs.
+template <int NT>
+__device__ __noinline__ double d4_energy_part(const DevD4& d4, const int* __restrict__ el, int nat,
+                                              const double* __restrict__ disp,
+                                              const double* __restrict__ gw)
+{
+  double ed = 0.0;
+  for (int ai = threadIdx.x; ai < nat * nat; ai += NT) {
+    const double dai = disp[ai];
+    if (dai == 0.0) continue;
+    const int a = ai / nat, i = ai - a * nat;
+    const int ea = el[a];
+    double vv[TA_D4_MAXREF] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    d4_scf_accumulate(d4, ea, el[i], 0.5 * dai, gw + TA_D4_MAXREF * i, vv);
+#pragma unroll
+    for (int r = 0; r < TA_D4_MAXREF; ++r)
+      if (r < d4_nw(d4, ea)) ed += vv[r] * gw[TA_D4_MAXREF * a + r];
+  }
+  return ed;
+}
+
 // ---------------------------------------------------------------- kernel
 // The global-memory variant runs two workgroups per CU (128 VGPRs): its phases
 // are bound by memory latency, not by arithmetic.
@@ -716,10 +771,20 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
   const double diff = fmin(nuhf, nocc);
   const double nalp = (nocc - diff) / 2.0 + diff, nbet = (nocc - diff) / 2.0;
 
-  // repulsion total
-  double erep = 0.0;
-  for (int i = tid; i < nat; i += NT) erep += b.erep[v.a0 + i];
+  // repulsion total; three-body dispersion total (zero charges, k_d4_atm)
+  const DevD4& d4 = P.d4;
+  double erep = 0.0, e3 = 0.0;
+  for (int i = tid; i < nat; i += NT) {
+    erep += b.erep[v.a0 + i];
+    if (d4.enabled) e3 += b.d4_e3[v.a0 + i];
+  }
   erep = block_sum(erep, red);
+  e3 = block_sum(e3, red);
+  double* d4gw = b.d4_gw + TA_D4_MAXREF * (int64_t)v.a0;
+  double* d4dgw = b.d4_dgw + TA_D4_MAXREF * (int64_t)v.a0;
+  const double* d4cn = b.d4_cn + v.a0;
+  const double* d4disp = b.d4_disp + b.mol_pair_off[m];
+  double ed2 = 0.0;
 
   // --- initial guess: zero charges and multipoles (superposition of neutral atoms)
   for (int i = tid; i < nsh; i += NT) v.qsh[i] = 0.0;
@@ -896,6 +961,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
     clk.lap(TA_PROF_MIX);
 
     // ------------------------------------------------ potential
+    // D4 reference weights at the charges of this iteration
+    if (d4.enabled) {
+      for (int a = tid; a < nat; a += NT)
+        d4_atom_weights(d4, v.el[a], d4cn[a], v.qat[a], d4gw + TA_D4_MAXREF * a, d4dgw + TA_D4_MAXREF * a);
+      __syncthreads();
+    }
     double* vao = work;
     double* vsh = work + ld;
     double* vat = vsh + nshmax;
@@ -960,6 +1031,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
         }
       }
     }
+    // D4 potential dE2/dq_a, same atom-to-lane mapping as above (the lane
+    // that wrote vat[a] adds to it)
+    if (d4.enabled) d4_potential<NW>(d4, v.el, nat, d4disp, d4gw, d4dgw, TA_LDS(vat));
     __syncthreads();
     for (int i = tid; i < n; i += NT) vao[i] = vsh[v.ao_sh[i]] + vat[v.ao_at[i]];
     // set_mixer: q_in = current charges
@@ -1169,6 +1243,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
       double q = 0.0;
       for (int s = 0; s < ns; ++s) q += v.qsh[s0a + s];
       v.qat[a] = q;
+      // D4 weights at the output charges (dispersion energy of this iteration)
+      if (d4.enabled)
+        d4_atom_weights(d4, v.el[a], d4cn[a], q, d4gw + TA_D4_MAXREF * a, d4dgw + TA_D4_MAXREF * a);
     }
     __syncthreads();
     clk.lap(TA_PROF_MULLIKEN);
@@ -1209,10 +1286,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
       ec += e.qkernel * (qpa[0] * qpa[0] + 2.0 * qpa[1] * qpa[1] + qpa[2] * qpa[2] +
                          2.0 * qpa[3] * qpa[3] + 2.0 * qpa[4] * qpa[4] + qpa[5] * qpa[5]);
     }
+    // D4 two-body energy E2 = 1/2 sum_{a != i} sum_rs g_ar d_ai C6ref g_is
+    const double ed = d4.enabled ? d4_energy_part<NT>(d4, v.el, nat, d4disp, d4gw) : 0.0;
     ec = block_sum(ec, red);
     eband = block_sum(eband, red);
     err = block_sum(err, red);
-    esum = eband + ts + ec;
+    ed2 = d4.enabled ? block_sum(ed, red) : 0.0;
+    esum = eband + ts + ec + ed2;
     const double perr = sqrt(err / ndim);
     converged = (fabs(esum - elast) < econv) && (perr < pconv);
     if (!isfinite(esum)) status |= TA_STATUS_NAN;
@@ -1238,14 +1318,16 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
     for (int i = 0; i < TA_PROF_SLOTS; ++i) b.prof[(int64_t)m * TA_PROF_SLOTS + i] = clk.acc[i];
   }
   if (tid == 0) {
-    b.energy[m] = erep + esum;
+    b.energy[m] = erep + e3 + esum;
+    b.edisp2[m] = ed2;
+    b.edisp3[m] = e3;
     b.eelec[m] = esum;
     b.erep_tot[m] = erep;
     b.iterations[m] = iter;
     b.status[m] = status;
     if (b.done != nullptr) {
       TaDoneRecord* r = b.done + m;
-      r->energy = erep + esum;
+      r->energy = erep + e3 + esum;
       r->iterations = iter;
       r->status = status;
       __hip_atomic_store(&r->flag, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);

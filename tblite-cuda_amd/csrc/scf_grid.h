@@ -10,7 +10,7 @@
 // device scalars (GridMol::sc) and integer words (GridMol::ic)
 enum {
   GS_WARM = 0, GS_ESUM, GS_GL, GS_GU, GS_TNORM, GS_PIVMIN, GS_TOL, GS_TS, GS_ERR, GS_EBAND,
-  GS_EREP, GS_NALP, GS_NBET, GS_COUNT = 16
+  GS_EREP, GS_NALP, GS_NBET, GS_E3, GS_ED2, GS_COUNT = 16
 };
 enum { GI_STATUS = 0, GI_CONVERGED, GI_NOCC, GI_NRUNS, GI_SHIFT = 4, GI_COUNT = 8 };
 
@@ -35,6 +35,13 @@ struct GridMol {
   double *d, *e, *tau, *e2, *lam, *prevlam, *v, *y, *focc, *fs, *vsh, *vat, *vdp, *vqp, *dots, *tm;
   double *sc;
   int *ic, *runs;
+  // D4 dispersion: per atom CN, weights g_r(q) and dg_r/dq (TA_D4_MAXREF each),
+  // three-body chunk energies, per-atom two-body energies of the iteration;
+  // per atom pair the two-body damping factors; per molecule result slots
+  int d4_on;  // host copy of the parameter set's D4 switch
+  const double *d4_cn, *d4_disp, *d4_e3;
+  double *d4_gw, *d4_dgw, *ed4;
+  double *edisp2, *edisp3;
 };
 
 // doubles of scratch a molecule of the grid path needs beyond its four ld x ld matrices
@@ -45,7 +52,7 @@ inline int64_t grid_scratch_doubles(int n, int ld, int nat, int nsh)
   s += (int64_t)(2 * TA_BIG_NP + TA_BIG_NB) * ld;          // vt, wt, wbt
   s += nbm * nbm * 64 + nbm;                               // symv partials, y.v partials
   s += 11 * (int64_t)n + (nsh + nat);                      // Mulliken partials, energy parts
-  s += 10 * (int64_t)ld + nsh + 10 * (int64_t)nat;         // vectors
+  s += 10 * (int64_t)ld + nsh + 11 * (int64_t)nat;         // vectors
   s += 2 * TA_BIG_NP + TA_BIG_NB * TA_BIG_NB + GS_COUNT;   // dots, tm, scalars
   s += (GI_COUNT + ld + 4 + 1) / 2 + 16;                   // int words and runs (as doubles)
   return s + 64;

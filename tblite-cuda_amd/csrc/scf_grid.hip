@@ -31,6 +31,7 @@
 #include <vector>
 
 #include "common.h"
+#include "d4.h"
 #include "device_util.h"
 #include "eig_big.h"
 #include "fermi.h"
@@ -87,11 +88,16 @@ __global__ __launch_bounds__(256) void k_g_init(GridMol g)
 {
   __shared__ double red_[8];
   lds_dbl* red = TA_LDS(red_);
-  double nref = 0.0, erep = 0.0;
+  double nref = 0.0, erep = 0.0, e3 = 0.0;
+  const bool d4on = g.par->d4.enabled;
   for (int s = threadIdx.x; s < g.nsh; s += blockDim.x) nref += g.n0sh[s];
-  for (int a = threadIdx.x; a < g.nat; a += blockDim.x) erep += g.erep_at[a];
+  for (int a = threadIdx.x; a < g.nat; a += blockDim.x) {
+    erep += g.erep_at[a];
+    if (d4on) e3 += g.d4_e3[a];
+  }
   nref = block_sum(nref, red);
   erep = block_sum(erep, red);
+  e3 = block_sum(e3, red);
   if (threadIdx.x == 0) {
     const double nel = nref - g.charge;
     const int nint_nel = (int)lround(nel);
@@ -101,6 +107,8 @@ __global__ __launch_bounds__(256) void k_g_init(GridMol g)
     g.sc[GS_NALP] = (nel - diff) / 2.0 + diff;
     g.sc[GS_NBET] = (nel - diff) / 2.0;
     g.sc[GS_EREP] = erep;
+    g.sc[GS_E3] = e3;
+    g.sc[GS_ED2] = 0.0;
   }
   for (int i = threadIdx.x; i < g.nsh; i += blockDim.x) g.qsh[i] = 0.0;
   for (int i = threadIdx.x; i < g.nat; i += blockDim.x) g.qat[i] = 0.0;
@@ -185,6 +193,36 @@ __global__ __launch_bounds__(256) void k_g_potential(GridMol g)
       g.vqp[6 * a + k] = q[k] + 2.0 * e.qkernel * g.qpat[6 * a + k] * sc;
     }
   }
+}
+
+// ---------------------------------------------------------------- D4
+// reference weights at the current atomic charges, one thread per atom
+__global__ void k_g_d4_weights(GridMol g)
+{
+  const DevD4& d4 = g.par->d4;
+  for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < g.nat; a += gridDim.x * blockDim.x)
+    d4_atom_weights(d4, g.el[a], g.d4_cn[a], g.qat[a], g.d4_gw + TA_D4_MAXREF * a,
+               g.d4_dgw + TA_D4_MAXREF * a);
+}
+
+// D4 potential dE2/dq_a added to vat (after k_g_potential), one wave per atom
+__global__ __launch_bounds__(64) void k_g_d4_potential(GridMol g)
+{
+  const DevD4& d4 = g.par->d4;
+  const int a = blockIdx.x, nat = g.nat, lane = threadIdx.x;
+  const int ea = g.el[a];
+  double vv[TA_D4_MAXREF] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int i = lane; i < nat; i += 64) {
+    const double dai = g.d4_disp[(int64_t)a * nat + i];
+    if (dai != 0.0) d4_scf_accumulate(d4, ea, g.el[i], dai, g.d4_gw + TA_D4_MAXREF * i, vv);
+  }
+  double va = 0.0;
+#pragma unroll
+  for (int r = 0; r < TA_D4_MAXREF; ++r) {
+    vv[r] = wave_sum(vv[r]);
+    if (r < d4_nw(d4, ea)) va += vv[r] * g.d4_dgw[TA_D4_MAXREF * a + r];
+  }
+  if (lane == 0) g.vat[a] += va;
 }
 
 // ---------------------------------------------------------------- Fock
@@ -971,8 +1009,25 @@ __global__ __launch_bounds__(64) void k_g_energy(GridMol g)
     for (int k = 0; k < 6; ++k) ec += qpa[k] * g.sq[k * p2 + ai] * qi;
   }
   ec = wave_sum(ec);
+  // D4 two-body energy of atom a: 1/2 sum_i sum_rs g_ar d_ai C6ref g_is
+  double ed = 0.0;
+  if (P.d4.enabled) {
+    const int ea = g.el[a];
+    double vv[TA_D4_MAXREF] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int i = lane; i < nat; i += 64) {
+      const double dai = g.d4_disp[row + i];
+      if (dai != 0.0) d4_scf_accumulate(P.d4, ea, g.el[i], 0.5 * dai, g.d4_gw + TA_D4_MAXREF * i, vv);
+    }
+#pragma unroll
+    for (int r = 0; r < TA_D4_MAXREF; ++r) {
+      vv[r] = wave_sum(vv[r]);
+      if (r < d4_nw(P.d4, ea)) ed += vv[r] * g.d4_gw[TA_D4_MAXREF * a + r];
+    }
+  }
   if (lane == 0) {
     const DevElem& e = P.el[g.el[a]];
+    g.ed4[a] = ed;
+    ec += ed;
     ec += e.dkernel * (dpa[0] * dpa[0] + dpa[1] * dpa[1] + dpa[2] * dpa[2]);
     ec += e.qkernel * (qpa[0] * qpa[0] + 2.0 * qpa[1] * qpa[1] + qpa[2] * qpa[2] +
                        2.0 * qpa[3] * qpa[3] + 2.0 * qpa[4] * qpa[4] + qpa[5] * qpa[5]);
@@ -984,10 +1039,13 @@ __global__ __launch_bounds__(512) void k_g_energy_fin(GridMol g, int iter)
 {
   __shared__ double red_[16];
   lds_dbl* red = TA_LDS(red_);
-  double ec = 0.0;
+  double ec = 0.0, ed = 0.0;
   for (int k = threadIdx.x; k < g.nsh + g.nat; k += GT) ec += g.epart[k];
+  for (int a = threadIdx.x; a < g.nat; a += GT) ed += g.ed4[a];
   ec = block_sum(ec, red);
+  ed = block_sum(ed, red);
   if (threadIdx.x == 0) {
+    g.sc[GS_ED2] = ed;
     const double elast = g.sc[GS_ESUM];
     const double esum = g.sc[GS_EBAND] + g.sc[GS_TS] + ec;
     const double perr = sqrt(g.sc[GS_ERR] / g.ndim);
@@ -1176,7 +1234,9 @@ __global__ void k_g_final(GridMol g, int iter, int status, double* energy, doubl
 {
   if (threadIdx.x == 0) {
     const double esum = g.sc[GS_ESUM], er = g.sc[GS_EREP];
-    *energy = er + esum;
+    *energy = er + g.sc[GS_E3] + esum;
+    *g.edisp2 = g.sc[GS_ED2];
+    *g.edisp3 = g.sc[GS_E3];
     *eelec = esum;
     *erep = er;
     *iterations = iter;
@@ -1414,7 +1474,10 @@ int grid_scf(GridMol g, hipStream_t s, int max_iter, double* d_energy, double* d
       hipLaunchKernelGGL(k_g_broyden, dim3(1), dim3(GT), 0, s, g, mixiter);
     }
     ++iter;
+    const bool d4on = g.d4_on != 0;
+    if (d4on) hipLaunchKernelGGL(k_g_d4_weights, dim3(grid_blocks(g.nat)), dim3(256), 0, s, g);
     hipLaunchKernelGGL(k_g_potential, dim3(g.nsh + g.nat), dim3(256), 0, s, g);
+    if (d4on) hipLaunchKernelGGL(k_g_d4_potential, dim3(g.nat), dim3(64), 0, s, g);
     if (iter == 1) hipLaunchKernelGGL(k_g_set_mixer, dim3(grid_blocks(g.ndim)), dim3(256), 0, s, g);
     hipLaunchKernelGGL(k_g_fock, dim3(grid_blocks((int64_t)n * n)), dim3(256), 0, s, g);
     hipLaunchKernelGGL(k_g_symmetrize, dim3(grid_blocks((int64_t)n * n)), dim3(256), 0, s, g);
@@ -1479,6 +1542,7 @@ int grid_scf(GridMol g, hipStream_t s, int max_iter, double* d_energy, double* d
     }
     hipLaunchKernelGGL(k_g_mull_rows, dim3(n), dim3(64), 0, s, g);
     hipLaunchKernelGGL(k_g_mull_fin, dim3(1), dim3(GT), 0, s, g);
+    if (d4on) hipLaunchKernelGGL(k_g_d4_weights, dim3(grid_blocks(g.nat)), dim3(256), 0, s, g);
     hipLaunchKernelGGL(k_g_energy, dim3(g.nsh + g.nat), dim3(64), 0, s, g);
     hipLaunchKernelGGL(k_g_energy_fin, dim3(1), dim3(GT), 0, s, g, iter);
     if ((err = hipGetLastError()) != hipSuccess) return -2;

@@ -10,6 +10,7 @@
 #include <math.h>
 
 #include "common.h"
+#include "d4.h"
 #include "device_util.h"
 
 #define SETUP_THREADS 256
@@ -45,10 +46,12 @@ __global__ __launch_bounds__(SETUP_THREADS) void k_setup(BatchDev b)
 
   const double cn_cut2 = P.cn_cutoff * P.cn_cutoff;
   const double rep_cut2 = P.rep_cutoff * P.rep_cutoff;
+  const DevD4& d4 = P.d4;
+  const double d4_cut2 = d4.cn_cutoff * d4.cn_cutoff;
   for (int i = tid; i < nat; i += blockDim.x) {
     const DevElem& ei = P.el[el[i]];
     const double xi = xyz[3 * i], yi = xyz[3 * i + 1], zi = xyz[3 * i + 2];
-    double cn = 0.0, er = 0.0;
+    double cn = 0.0, er = 0.0, cn4 = 0.0;
     for (int j = 0; j < nat; ++j) {
       if (j == i) continue;
       const DevElem& ej = P.el[el[j]];
@@ -59,6 +62,12 @@ __global__ __launch_bounds__(SETUP_THREADS) void k_setup(BatchDev b)
       if (r2 <= cn_cut2) {
         double rc = ei.rcov + ej.rcov;
         cn += exp_count(P.cn_ka, r1, rc) * exp_count(P.cn_kb, r1, rc + P.cn_rshift);
+      }
+      if (d4.enabled && r2 <= d4_cut2) {
+        // D4 CN: electronegativity-weighted error-function count
+        const double rc = ei.rcov + ej.rcov;
+        const double den = fabs(ei.en - ej.en) + d4.k5;
+        cn4 += d4.k4 * exp(-den * den / d4.k6) * 0.5 * (1.0 + erf(-d4.kn * (r1 - rc) / rc));
       }
       if (r2 <= rep_cut2) {
         double alpha = sqrt(ei.arep * ej.arep);
@@ -71,6 +80,12 @@ __global__ __launch_bounds__(SETUP_THREADS) void k_setup(BatchDev b)
     }
     b.cn[a0 + i] = cn;
     b.erep[a0 + i] = er;
+    if (d4.enabled) {
+      // zero-charge reference weights (C6 of the three-body term)
+      b.d4_cn[a0 + i] = cn4;
+      d4_weights(d4, el[i], cn4, 0.0, b.d4_gw + TA_D4_MAXREF * (int64_t)(a0 + i),
+                 b.d4_dgw + TA_D4_MAXREF * (int64_t)(a0 + i));
+    }
     double arg = cn - ei.mpvcn - P.mp_shift;
     double t1 = exp(-P.mp_kexp * arg);
     b.mrad[a0 + i] = ei.mprad + (P.mp_rmax - ei.mprad) / (1.0 + t1);
@@ -139,6 +154,98 @@ __global__ __launch_bounds__(SETUP_THREADS) void k_setup(BatchDev b)
     sq[4 * n2 + idx] = 2.0 * v[1] * v[2] * gf;
     sq[5 * n2 + idx] = v[2] * v[2] * gf;
   }
+
+  // D4 two-body damping factors d_ij and zero-charge C6_ij (three-body term)
+  if (d4.enabled) {
+    double* __restrict__ disp = b.d4_disp + b.mol_pair_off[m];
+    double* __restrict__ c60 = b.d4_c60 + b.mol_pair_off[m];
+    const double* __restrict__ gw0 = b.d4_gw + TA_D4_MAXREF * (int64_t)a0;
+    const double cut2 = d4.disp2_cutoff * d4.disp2_cutoff;
+    for (int64_t idx = tid; idx < n2; idx += blockDim.x) {
+      const int i = (int)(idx / nat), j = (int)(idx - (int64_t)i * nat);
+      double dv = 0.0, cv = 0.0;
+      if (i != j) {
+        const int ei = el[i], ej = el[j];
+        const double dx = xyz[3 * i] - xyz[3 * j], dy = xyz[3 * i + 1] - xyz[3 * j + 1];
+        const double dz = xyz[3 * i + 2] - xyz[3 * j + 2];
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 <= cut2) {
+          const double rr = 3.0 * d4.r4r2[ei] * d4.r4r2[ej];
+          const double r0 = d4.a1 * sqrt(rr) + d4.a2;
+          const double r02 = r0 * r0, r06 = r02 * r02 * r02;
+          const double r6 = r2 * r2 * r2;
+          dv = -(d4.s6 / (r6 + r06) + d4.s8 * rr / (r6 * r2 + r06 * r02));
+        }
+        double acc[TA_D4_MAXREF] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        d4_accumulate(d4, ei, ej, 1.0, gw0 + TA_D4_MAXREF * j, acc);
+        for (int r = 0; r < d4.nref[ei]; ++r) cv += gw0[TA_D4_MAXREF * i + r] * acc[r];
+      }
+      disp[idx] = d4.qref_uniform ? dv * cv : dv;
+      c60[idx] = cv;
+    }
+  }
+}
+
+// ------------------------------------------------------------- k_d4_atm
+// Three-body (Axilrod-Teller-Muto) dispersion with zero-charge C6 and
+// rational damping of the product of the pair radii:
+//   E3 = sum_{k<j<i} s9 sqrt(C6_ij C6_ik C6_jk) (3 cos a cos b cos c + 1)
+//        / (r_ij r_ik r_jk)^3 / (1 + 6 (R0_ij R0_ik R0_jk / r_ij r_ik r_jk)^(alp/3))
+// Work item w: anchors i in [i0, i1) of one molecule.  Its atom pairs
+// (i, j < i) are numbered p = T(i) - T(i0) + j with T(i) = i (i - 1) / 2; the
+// threads stride over p and loop over k < j.  The item's energy goes to
+// d4_e3 of its first anchor, zero to the others (a deterministic
+// per-molecule sum).
+__device__ __forceinline__ int64_t tri_count(int64_t i) { return i * (i - 1) / 2; }
+
+__global__ __launch_bounds__(SETUP_THREADS) void k_d4_atm(BatchDev b, const int* __restrict__ work_mol,
+                                                          const int* __restrict__ work_i0,
+                                                          const int* __restrict__ work_i1)
+{
+  __shared__ double red[SETUP_THREADS / 64];
+  const int w = blockIdx.x, tid = threadIdx.x;
+  const int m = work_mol[w], i0 = work_i0[w], i1 = work_i1[w];
+  const int a0 = b.mol_atom_off[m], nat = b.mol_nat[m];
+  const double* __restrict__ xyz = b.xyz + 3 * (int64_t)a0;
+  const int* __restrict__ el = b.at_el + a0;
+  const double* __restrict__ c60 = b.d4_c60 + b.mol_pair_off[m];
+  const DevD4& d = b.par->d4;
+  const double cut2 = d.disp3_cutoff * d.disp3_cutoff;
+  const double p3 = d.alp / 3.0;
+  const int64_t t0 = tri_count(i0), npair = tri_count(i1) - t0;
+  double e = 0.0;
+  for (int64_t p = tid; p < npair; p += SETUP_THREADS) {
+    // anchor i: T(i) <= t0 + p < T(i + 1)
+    const int64_t q = t0 + p;
+    int i = (int)((1.0 + sqrt(1.0 + 8.0 * (double)q)) * 0.5);
+    while (tri_count(i) > q) --i;
+    while (tri_count(i + 1) <= q) ++i;
+    const int j = (int)(q - tri_count(i));
+    const double xi = xyz[3 * i], yi = xyz[3 * i + 1], zi = xyz[3 * i + 2];
+    const double xj = xyz[3 * j], yj = xyz[3 * j + 1], zj = xyz[3 * j + 2];
+    const double r2ij = (xj - xi) * (xj - xi) + (yj - yi) * (yj - yi) + (zj - zi) * (zj - zi);
+    if (r2ij > cut2) continue;
+    const double ri = d.r4r2[el[i]], rj = d.r4r2[el[j]];
+    const double r0ij = d.a1 * sqrt(3.0 * ri * rj) + d.a2;
+    const double c6ij = c60[(int64_t)i * nat + j];
+    for (int k = 0; k < j; ++k) {
+      const double xk = xyz[3 * k], yk = xyz[3 * k + 1], zk = xyz[3 * k + 2];
+      const double r2ik = (xk - xi) * (xk - xi) + (yk - yi) * (yk - yi) + (zk - zi) * (zk - zi);
+      const double r2jk = (xk - xj) * (xk - xj) + (yk - yj) * (yk - yj) + (zk - zj) * (zk - zj);
+      if (r2ik > cut2 || r2jk > cut2) continue;
+      const double rk = d.r4r2[el[k]];
+      const double r0ik = d.a1 * sqrt(3.0 * ri * rk) + d.a2;
+      const double r0jk = d.a1 * sqrt(3.0 * rj * rk) + d.a2;
+      const double c9 = d.s9 * sqrt(fabs(c6ij * c60[(int64_t)i * nat + k] * c60[(int64_t)j * nat + k]));
+      const double r2ijk = r2ij * r2ik * r2jk, rijk = sqrt(r2ijk);
+      const double fdmp = 1.0 / (1.0 + 6.0 * pow(r0ij * r0ik * r0jk / rijk, p3));
+      const double ang = 0.375 * (r2ij + r2jk - r2ik) * (r2ij - r2jk + r2ik) *
+                             (-r2ij + r2jk + r2ik) / r2ijk + 1.0;
+      e += c9 * ang * fdmp / (rijk * r2ijk);
+    }
+  }
+  e = block_sum(e, red);
+  for (int i = i0 + tid; i < i1; i += SETUP_THREADS) b.d4_e3[a0 + i] = (i == i0) ? e : 0.0;
 }
 
 // ------------------------------------------------------------- k_adjlist

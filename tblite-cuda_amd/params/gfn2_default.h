@@ -21,6 +21,7 @@
 #include <string.h>
 
 #include "tblite_amd.h"
+#include "d4_model.h"
 
 #define TBLITE_AMD_AUTOAA 0.52917721090380
 #define TBLITE_AMD_AUTOEV 27.21138602
@@ -130,6 +131,8 @@ static inline void tblite_amd_fill_gfn2_default(tblite_amd_param* p)
   e->dkernel = -0.181e-2; e->qkernel = 0.655e-2;                                 /* [est] */
   e->mprad = 3.4; e->mpvcn = 2.0;                                                /* [est] */
   e->rcov = 1.02 * 4.0 / 3.0 * aa; e->atomic_rad = 1.04 * aa;
+
+  tblite_amd_fill_d4_model(p);
 }
 
 #endif

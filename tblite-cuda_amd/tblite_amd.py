@@ -39,6 +39,29 @@ class Element(C.Structure):
     ]
 
 
+D4_MAXREF = 7
+D4_NFREQ = 23
+
+
+class D4Element(C.Structure):
+    _fields_ = [
+        ("nref", C.c_int), ("cn", C.c_double * D4_MAXREF), ("q", C.c_double * D4_MAXREF),
+        ("alpha", (C.c_double * D4_NFREQ) * D4_MAXREF), ("zeff", C.c_double), ("eta", C.c_double),
+        ("r4r2", C.c_double),
+    ]
+
+
+class D4Param(C.Structure):
+    _fields_ = [
+        ("enabled", C.c_int), ("s6", C.c_double), ("s8", C.c_double), ("a1", C.c_double),
+        ("a2", C.c_double), ("s9", C.c_double), ("alp", C.c_double), ("ga", C.c_double),
+        ("gc", C.c_double), ("wf", C.c_double), ("cn_k4", C.c_double), ("cn_k5", C.c_double),
+        ("cn_k6", C.c_double), ("cn_kn", C.c_double), ("cn_cutoff", C.c_double),
+        ("disp2_cutoff", C.c_double), ("disp3_cutoff", C.c_double),
+        ("elem", D4Element * MAX_ELEM),
+    ]
+
+
 class Param(C.Structure):
     _fields_ = [
         ("kshell", (C.c_double * 3) * 3), ("enscale", C.c_double), ("wexp", C.c_double),
@@ -48,6 +71,7 @@ class Param(C.Structure):
         ("mp_shift", C.c_double), ("mp_kexp", C.c_double), ("mp_rmax", C.c_double),
         ("cn_ka", C.c_double), ("cn_kb", C.c_double), ("cn_rshift", C.c_double),
         ("cn_cutoff", C.c_double), ("nelem", C.c_int), ("elem", Element * MAX_ELEM),
+        ("d4", D4Param),
     ]
 
 
@@ -98,6 +122,7 @@ _PROTOS = {
     "tblite_amd_batch_get_selfenergy": (_I, [_P, _I, _DP]),
     "tblite_amd_batch_get_charges": (_I, [_P, _I, _DP, _DP, _DP, _DP, _DP, _DP]),
     "tblite_amd_batch_get_energy_parts": (_I, [_P, _I, _DP, _DP]),
+    "tblite_amd_batch_get_dispersion": (_I, [_P, _I, _DP, _DP]),
     "tblite_amd_last_timings": (_I, [_P, _DP, _I]),
     "tblite_amd_last_bucket_timings": (_I, [_P, _DP, _IP, _IP, _IP, _I]),
     "tblite_amd_last_profile": (_I, [_P, C.POINTER(C.c_longlong)]),
@@ -372,6 +397,12 @@ class Engine:
         erep, eelec = C.c_double(), C.c_double()
         self._check(self.lib.tblite_amd_batch_get_energy_parts(self.ctx, imol, C.byref(erep), C.byref(eelec)))
         return erep.value, eelec.value
+
+    def dispersion(self, imol):
+        """D4 two-body (at the final charges, part of eelec) and three-body energy."""
+        e2, e3 = C.c_double(), C.c_double()
+        self._check(self.lib.tblite_amd_batch_get_dispersion(self.ctx, imol, C.byref(e2), C.byref(e3)))
+        return e2.value, e3.value
 
     def profile(self):
         """Per-molecule SCF phase clocks (needs TBLITE_AMD_PROFILE=1 at context creation)."""

@@ -1,6 +1,7 @@
 """Write the golden fixtures for the seeded (H2O)8 workload and a few small
-molecules: bohr coordinates (the exact input bits) and the oracle's energy,
-charges and iteration counts.  The fixture is data; the tests compare both
+molecules: bohr coordinates (the exact input bits) and the oracle's energy
+(with its D4 two- and three-body parts, and the energy with dispersion
+switched off), charges and iteration counts.  The fixture is data; the tests compare both
 the oracle (regression) and the HIP engine against it.
 
     python tests/golden/make_golden.py
@@ -17,11 +18,15 @@ sys.path.insert(0, os.path.join(ROOT, "tblite-cuda_amd"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 import oracle  # noqa: E402
+import tblite_amd  # noqa: E402
 import workloads  # noqa: E402
 
 
 def main():
     orc = oracle.Oracle()
+    nodisp = tblite_amd.default_param()
+    nodisp.d4.enabled = 0
+    orc0 = oracle.Oracle(nodisp)
     rng = np.random.default_rng(2024)
     cases = {
         "water8_seed0": workloads.config0()[0],
@@ -38,6 +43,9 @@ def main():
             "xyz_bohr": [[float(v) for v in row] for row in xyz],
             "energy": r["energy"],
             "erep": r["erep"],
+            "edisp2": r["edisp2"],
+            "edisp3": r["edisp3"],
+            "energy_nodisp": orc0.singlepoint(num, xyz)["energy"],
             "iterations": int(r["iterations"]),
             "qsh": [float(v) for v in r["qsh"]],
             "qat": [float(v) for v in r["qat"]],

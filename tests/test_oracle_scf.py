@@ -219,7 +219,7 @@ def test_singlepoint_invariants(orc):
     assert abs(r["qsh"].sum()) < 1e-10
     assert abs(r["qat"].sum()) < 1e-10
     assert abs(r["focc"].sum() - 64.0) < 1e-8
-    assert abs(r["energy"] - (r["erep"] + r["eelec"])) < 1e-12
+    assert abs(r["energy"] - (r["erep"] + r["edisp3"] + r["eelec"])) < 1e-12
     assert np.all(np.diff(r["emo"]) >= 0)
     # rotation / translation / permutation invariance
     r2 = orc.singlepoint(num, _rot(xyz, 1) + 2.0)

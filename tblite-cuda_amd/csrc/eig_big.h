@@ -241,6 +241,11 @@ __device__ __noinline__ void trail_pass(gbl_dbl* A, int n, int ld, int lo, const
   __syncthreads();
 }
 
+// rows per wave in flight in the read-only symmetric product
+#ifndef TA_SYMV_RQ
+#define TA_SYMV_RQ 4
+#endif
+
 // --------------------------------------------- blocked tridiagonalisation
 // Same result layout as tridiagonalize(), in panels of NP columns (dlatrd
 // scheme): within a panel each column j is first brought up to date with
@@ -321,7 +326,7 @@ __device__ __noinline__ void tridiagonalize_blocked(gbl_dbl* A, int n, int ld, l
         sacc = wave_sum(sacc);
         if (lane == 0) dots[(pq < i ? 0 : NP) + q] = sacc;
       }
-      trail_pass<NT, false, 4>(A, n, ld, j + 1, v, v, v, yrow, part, npart);
+      trail_pass<NT, false, TA_SYMV_RQ>(A, n, ld, j + 1, v, v, v, yrow, part, npart);
       lap(2);
       // (d) y = tau (A_trail v - V (W^T v) - W (V^T v)), combined with the
       // per-wave partial sums in the same pass

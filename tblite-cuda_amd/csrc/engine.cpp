@@ -317,6 +317,9 @@ struct tblite_amd_context_s {
   // CPU oracle 1.6e-10 Eh, identical iteration counts; 1e-6 costs one more
   // sweep per SCF iteration, 9 % of the batch time)
   double jacobi_tol = 1.0e-4;
+  // minimum LDS request of the global-memory SCF kernel (bytes); above half a
+  // CU's LDS it runs one molecule per CU (TBLITE_AMD_BIG_LDS_MIN_KB)
+  size_t big_lds_min = 0;
   // cost-model routing between one workgroup per molecule and the
   // multi-workgroup path (seconds per SCF iteration, measured on MI355X by
   // autotune.py: one workgroup alone wg_a * ld^wg_p, across the device
@@ -843,7 +846,9 @@ int launch_scf(tblite_amd_context ctx, hipStream_t st, const BatchDev& b, const 
                int count, int ldmax, int natmax, int nshmax)
 {
   const ScfLds L = scf_lds_layout(ldmax, natmax, nshmax, LDS);
-  const size_t bytes = (size_t)L.total * sizeof(double);
+  size_t bytes = (size_t)L.total * sizeof(double);
+  // a larger request than the layout needs limits how many workgroups share a CU
+  if (!LDS) bytes = std::max(bytes, ctx->big_lds_min);
   if (bytes > ctx->lds_limit) {
     set_error(ctx, "SCF bucket needs %zu bytes of LDS (limit %zu)", bytes, ctx->lds_limit);
     return TBLITE_AMD_ERR_TOO_LARGE;
@@ -1204,6 +1209,8 @@ static tblite_amd_context make_context(int device, bool single_stream)
   if (pg) ctx->grid_min_ld = std::max(16, std::min(TA_BIG_MAXLD, std::atoi(pg)));
   const char* pgg = std::getenv("TBLITE_AMD_GRID_GRAPH");
   ctx->grid_graph = !(pgg && pgg[0] == '0');
+  const char* pb = std::getenv("TBLITE_AMD_BIG_LDS_MIN_KB");
+  if (pb) ctx->big_lds_min = (size_t)std::max(0, std::min(160, std::atoi(pb))) * 1024;
   const char* pj = std::getenv("TBLITE_AMD_JACOBI_TOL");
   if (pj) ctx->jacobi_tol = std::max(1e-12, std::min(1e-2, std::atof(pj)));
   const char* pl = std::getenv("TBLITE_AMD_LDS_MAX_LD");

@@ -169,39 +169,62 @@ __device__ void density(const double* CT, const double* f, const int* occ, int n
 }
 
 // ------------------------------------------------------------ Cholesky
-// S = L L^T in place (lower), then C = L^-T.  Returns false if S is not
-// positive definite.
-template <int NT>
-__device__ bool orthonormalizer(double* A, double* C, int n, int ld, double* scal)
+// S = L L^T in place (lower), then C = L^-T (stored as its transpose L^-1,
+// one start MO per row).  Returns false if S is not positive definite.  col:
+// n doubles of LDS scratch.  Right-looking: the scaled pivot column is kept
+// in LDS, and the trailing lower triangle is updated one wave per row with
+// lanes over its columns (coalesced along the row).  L^-1 is formed by
+// forward substitution: with the matrices in LDS one thread per column of
+// L^-1 (no barriers); in global memory row by row (row i of L in LDS, all
+// columns j <= i in parallel, reads of earlier rows coalesced).
+template <int NT, bool MATS_LDS>
+__device__ bool orthonormalizer(double* A, double* C, int n, int ld, double* scal, double* col)
 {
+  constexpr int NW = NT / 64;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   for (int j = 0; j < n; ++j) {
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
       const double d = A[sk(j, j, ld)];
       scal[0] = (d > 0.0) ? sqrt(d) : -1.0;
     }
     __syncthreads();
     const double d = scal[0];
     if (!(d > 0.0)) return false;
-    for (int i = j + 1 + threadIdx.x; i < n; i += NT) A[sk(i, j, ld)] /= d;
-    if (threadIdx.x == 0) A[sk(j, j, ld)] = d;
+    for (int i = j + 1 + tid; i < n; i += NT) {
+      const double v = A[sk(i, j, ld)] / d;
+      A[sk(i, j, ld)] = v;
+      col[i] = v;
+    }
+    if (tid == 0) A[sk(j, j, ld)] = d;
     __syncthreads();
     // trailing update of the lower triangle
-    const int m = n - j - 1;
-    for (int idx = threadIdx.x; idx < m * m; idx += NT) {
-      const int r = j + 1 + idx / m, c = j + 1 + idx % m;
-      if (c <= r) A[sk(r, c, ld)] -= A[sk(r, j, ld)] * A[sk(c, j, ld)];
+    for (int r = j + 1 + wid; r < n; r += NW) {
+      const double lr = col[r];
+      for (int c = j + 1 + lane; c <= r; c += 64) A[sk(r, c, ld)] -= lr * col[c];
     }
     __syncthreads();
   }
-  // CT = L^-1 (row j = MO j of the start basis C = L^-T); one thread per
-  // column j of L^-1, forward substitution
-  for (int idx = threadIdx.x; idx < ld * ld; idx += NT) C[idx] = 0.0;
+  for (int idx = tid; idx < ld * ld; idx += NT) C[idx] = 0.0;
   __syncthreads();
-  for (int j = threadIdx.x; j < n; j += NT) {
-    for (int i = j; i < n; ++i) {
-      double s = (i == j) ? 1.0 : 0.0;
-      for (int k = j; k < i; ++k) s -= A[sk(i, k, ld)] * C[k * ld + j];
-      C[i * ld + j] = s / A[sk(i, i, ld)];
+  if constexpr (MATS_LDS) {
+    for (int j = tid; j < n; j += NT) {
+      for (int i = j; i < n; ++i) {
+        double s = (i == j) ? 1.0 : 0.0;
+        for (int k = j; k < i; ++k) s -= A[sk(i, k, ld)] * C[k * ld + j];
+        C[i * ld + j] = s / A[sk(i, i, ld)];
+      }
+    }
+  } else {
+    for (int i = 0; i < n; ++i) {
+      for (int k = tid; k <= i; k += NT) col[k] = A[sk(i, k, ld)];
+      __syncthreads();
+      const double dii = col[i];
+      for (int j = tid; j <= i; j += NT) {
+        double s = (i == j) ? 1.0 : 0.0;
+        for (int k = j; k < i; ++k) s -= col[k] * C[k * ld + j];
+        C[i * ld + j] = s / dii;
+      }
+      __syncthreads();
     }
   }
   __syncthreads();
@@ -799,7 +822,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
     A[sk(i, j, ld)] = (i < n && j < n) ? v.S[(int64_t)i * n + j] : 0.0;
   }
   __syncthreads();
-  if (!orthonormalizer<NT>(A, C, n, ld, scal)) status |= TA_STATUS_CHOLESKY;
+  if (!orthonormalizer<NT, MATS_LDS>(A, C, n, ld, scal, work)) status |= TA_STATUS_CHOLESKY;
   clk.lap(TA_PROF_INIT);
 
   if (tid == 0) red[24] = 0.0;  // bisection warm-start bracket (global-memory path)

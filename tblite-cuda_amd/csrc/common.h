@@ -201,6 +201,10 @@ struct BatchDev {
   // Jacobi stopping rule: the solve ends after the sweep whose largest rotated
   // off-diagonal element is below this
   double jacobi_tol;
+  // rotations whose off-diagonal element is at most this (absolute) are
+  // identities: the element is set to zero and the workers skip the blocks
+  // and MO rows they would have touched (0: every nonzero element rotates)
+  double jacobi_skip;
   // optional per-molecule phase clocks of the SCF kernel (TA_PROF_SLOTS each), may be null
   long long* prof;
   // optional completion records (one per molecule, pinned host memory), may be null

@@ -26,6 +26,7 @@ __global__ void k_fill(double* p, int64_t n, double v);
 __global__ void k_setup(BatchDev b);
 __global__ void k_adjlist(BatchDev b);
 __global__ void k_d4_atm(BatchDev b, const int* work_mol, const int* work_i0, const int* work_i1);
+__global__ void k_setup_pairs(BatchDev b, const int* work_mol, const int* work_a0, const int* work_a1);
 __global__ void k_integrals(BatchDev b, const int* work_mol, const int* work_kind,
                             const int* work_start);
 template <int NT, int MAXT, int MAXB, int MAXC, bool MATS_LDS>
@@ -35,6 +36,7 @@ namespace {
 
 constexpr int kIntegralChunk = 128;
 constexpr int64_t kAtmChunk = 1 << 18;  // triples per three-body dispersion work item
+constexpr int64_t kPairChunk = 1 << 15; // atom pairs per pair-plane work item
 constexpr int kLdsMaxLd = 96;
 constexpr int kNumStreams = 4;
 constexpr int kMaxBuckets = 8;
@@ -253,6 +255,8 @@ struct Plan {
   std::vector<int> work_mol, work_kind, work_start;
   // three-body dispersion work: chunks [atm_i0, atm_i1) of anchor atoms
   std::vector<int> atm_mol, atm_i0, atm_i1;
+  // pair-plane work: atom rows [pair_a0, pair_a1) of one molecule
+  std::vector<int> pair_mol, pair_a0, pair_a1;
   std::vector<int> grid_mols;  // molecules run by the multi-workgroup SCF path
   struct Bucket {
     int nt, maxt;
@@ -272,6 +276,9 @@ struct DeviceBatch {
   int* d_work_mol = nullptr;
   int* d_work_kind = nullptr;
   int* d_work_start = nullptr;
+  int* d_pair_mol = nullptr;
+  int* d_pair_a0 = nullptr;
+  int* d_pair_a1 = nullptr;
   int* d_atm_mol = nullptr;
   int* d_atm_i0 = nullptr;
   int* d_atm_i1 = nullptr;
@@ -317,6 +324,13 @@ struct tblite_amd_context_s {
   // CPU oracle 1.6e-10 Eh, identical iteration counts; 1e-6 costs one more
   // sweep per SCF iteration, 9 % of the batch time)
   double jacobi_tol = 1.0e-4;
+  // Jacobi rotations with |a_pq| at or below this are skipped
+  // (TBLITE_AMD_JACOBI_SKIP): the size of the off-diagonal elements the
+  // stopping rule leaves anyway (jacobi_tol^2); the worker waves then skip
+  // the blocks and MO rows of identity rotations, which in the last sweeps
+  // are most of them (4096 water clusters: Jacobi time at 65-96 AOs -9 %,
+  // energies unchanged to 5e-14 Eh, same iteration counts)
+  double jacobi_skip = 1.0e-8;
   // minimum LDS request of the global-memory SCF kernel (bytes); above half a
   // CU's LDS it runs one molecule per CU (TBLITE_AMD_BIG_LDS_MIN_KB)
   size_t big_lds_min = 0;
@@ -564,6 +578,15 @@ int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* 
       pl.work_kind.push_back(1);
       pl.work_start.push_back(s);
     }
+    // pair planes: rows of about kPairChunk pairs per work item
+    {
+      const int rows = (int)std::max<int64_t>(1, kPairChunk / nat);
+      for (int a = 0; a < nat; a += rows) {
+        pl.pair_mol.push_back(m);
+        pl.pair_a0.push_back(a);
+        pl.pair_a1.push_back(std::min(nat, a + rows));
+      }
+    }
     // three-body work: anchor atom i owns the triples k < j < i; chunks of
     // consecutive anchors hold about kAtmChunk triples each
     if (c.dev.d4.enabled) {
@@ -691,6 +714,9 @@ int upload_plan(tblite_amd_context ctx, Calculator& c, const Plan& pl, DeviceBat
   host_pack(img, seg, pl.atm_mol);        // 29
   host_pack(img, seg, pl.atm_i0);         // 30
   host_pack(img, seg, pl.atm_i1);         // 31
+  host_pack(img, seg, pl.pair_mol);       // 32
+  host_pack(img, seg, pl.pair_a0);        // 33
+  host_pack(img, seg, pl.pair_a1);        // 34
   const size_t nfixed = seg.size();
   for (const auto& bk : pl.buckets) host_pack(img, seg, bk.mols);
 
@@ -774,6 +800,9 @@ int upload_plan(tblite_amd_context ctx, Calculator& c, const Plan& pl, DeviceBat
   db.d_atm_mol = I(29);
   db.d_atm_i0 = I(30);
   db.d_atm_i1 = I(31);
+  db.d_pair_mol = I(32);
+  db.d_pair_a0 = I(33);
+  db.d_pair_a1 = I(34);
   db.d_bucket_mols.clear();
   for (size_t k = 0; k < pl.buckets.size(); ++k) db.d_bucket_mols.push_back(I(nfixed + k));
 
@@ -832,6 +861,7 @@ int upload_plan(tblite_amd_context ctx, Calculator& c, const Plan& pl, DeviceBat
   b.edisp3 = carve<double>(ar, nmol);
   b.intcut = 0.0;
   b.jacobi_tol = ctx->jacobi_tol;
+  b.jacobi_skip = ctx->jacobi_skip;
   b.prof = ctx->profile ? carve<long long>(ar, (int64_t)nmol * TA_PROF_SLOTS) : nullptr;
   db.bytes = ar.used;
   if (ar.used > ar.cap) {
@@ -1010,6 +1040,9 @@ int run_batch(tblite_amd_context ctx, Calculator& c, const double* xyz, double a
     if (n > 0) hipLaunchKernelGGL(k_fill, dim3(grid), dim3(256), 0, st, b.Q, 6 * pl.tot_mat, 0.0);
   }
   hipLaunchKernelGGL(k_setup, dim3(pl.nmol), dim3(256), 0, st, b);
+  HIP_TRY(ctx, hipGetLastError());
+  hipLaunchKernelGGL(k_setup_pairs, dim3((int)pl.pair_mol.size()), dim3(256), 0, st, b,
+                     db.d_pair_mol, db.d_pair_a0, db.d_pair_a1);
   HIP_TRY(ctx, hipGetLastError());
   // three-body dispersion (writes every atom's d4_e3 entry; needs the D4 CN
   // and zero-charge C6 of k_setup)
@@ -1211,6 +1244,8 @@ static tblite_amd_context make_context(int device, bool single_stream)
   ctx->grid_graph = !(pgg && pgg[0] == '0');
   const char* pb = std::getenv("TBLITE_AMD_BIG_LDS_MIN_KB");
   if (pb) ctx->big_lds_min = (size_t)std::max(0, std::min(160, std::atoi(pb))) * 1024;
+  const char* ps = std::getenv("TBLITE_AMD_JACOBI_SKIP");
+  if (ps) ctx->jacobi_skip = std::max(0.0, std::min(1e-6, std::atof(ps)));
   const char* pj = std::getenv("TBLITE_AMD_JACOBI_TOL");
   if (pj) ctx->jacobi_tol = std::max(1e-12, std::min(1e-2, std::atof(pj)));
   const char* pl = std::getenv("TBLITE_AMD_LDS_MAX_LD");

@@ -313,10 +313,11 @@ __device__ __forceinline__ int rr_player(int pos, int r, int ld)
 }
 
 __device__ __forceinline__ void make_rotation(const JRot& R, int buf, int h, int slot, int p,
-                                              int q, double app, double aqq, double apq)
+                                              int q, double app, double aqq, double apq,
+                                              double skip)
 {
   double c = 1.0, sn = 0.0, t = 0.0;
-  if (apq != 0.0) {
+  if (fabs(apq) > skip) {
     // t = 2 apq sgn(d) / (|d| + sqrt(d^2 + 4 apq^2)), the smaller rotation angle
     const double d = aqq - app;
     const double a2 = 2.0 * apq;
@@ -340,7 +341,7 @@ __device__ __forceinline__ double& aup(double* A, int ld, int i, int j)
 
 template <int NT, int MAXB, int MAXC>
 __device__ bool jacobi(double* A, double* CT, int ld, const JRot& R, const JPlan<MAXB, MAXC>& pl,
-                       double* red, double tol, int* sweeps_out)
+                       double* red, double tol, double skip, int* sweeps_out)
 {
   const int h = ld >> 1;
   const int lane = threadIdx.x & 63;
@@ -349,7 +350,7 @@ __device__ bool jacobi(double* A, double* CT, int ld, const JRot& R, const JPlan
   if (owner_wave)
     for (int k = lane; k < h; k += 64) {
       const int p = rr_player(k, 0, ld), q = rr_player(ld - 1 - k, 0, ld);
-      make_rotation(R, 0, h, k, p, q, A[sk(p, p, ld)], A[sk(q, q, ld)], aup(A, ld, p, q));
+      make_rotation(R, 0, h, k, p, q, A[sk(p, p, ld)], A[sk(q, q, ld)], aup(A, ld, p, q), skip);
     }
   __syncthreads();
 
@@ -409,7 +410,7 @@ __device__ bool jacobi(double* A, double* CT, int ld, const JRot& R, const JPlan
           // diagonal elements after this round: slot/index holding p and q
           const int sp = e == 0 ? 0 : (e == 1 ? kn + 1 : h - 1), ip = e == 2 ? 1 : 0;
           const int sq = e == 0 ? 1 : (e == 1 ? kn - 1 : h - 2), iq = e == 0 ? 0 : 1;
-          make_rotation(R, nxt, h, kn, p, q, diag_after(sp, ip), diag_after(sq, iq), apq);
+          make_rotation(R, nxt, h, kn, p, q, diag_after(sp, ip), diag_after(sq, iq), apq, skip);
         }
       } else {
         auto crow = [&](int k, int ao) {
@@ -423,8 +424,9 @@ __device__ bool jacobi(double* A, double* CT, int ld, const JRot& R, const JPlan
         };
         // two adjacent AOs of both rows (ao even, rows 16-byte aligned)
         auto crow2 = [&](int k, int ao) {
-          const int2 pq = R.pq[co + k];
           const double2 cs = R.cs[co + k];
+          if (cs.y == 0.0) return;  // identity rotation
+          const int2 pq = R.pq[co + k];
           double2* rp = reinterpret_cast<double2*>(CT + pq.x * ld + ao);
           double2* rq = reinterpret_cast<double2*>(CT + pq.y * ld + ao);
           const double2 u = *rp, v = *rq;
@@ -436,7 +438,12 @@ __device__ bool jacobi(double* A, double* CT, int ld, const JRot& R, const JPlan
         if constexpr (MAXB > 0) {
 #pragma unroll
           for (int j = 0; j < MAXB; ++j)
-            if (pl.bkl[j] >= 0) block(pl.bkl[j] >> 16, pl.bkl[j] & 0xffff, z00, z01, z10, z11, pk, pl_);
+            if (pl.bkl[j] >= 0) {
+              const int k = pl.bkl[j] >> 16, l = pl.bkl[j] & 0xffff;
+              // both rotations identities: the block is unchanged
+              if (R.cs[co + k].y == 0.0 && R.cs[co + l].y == 0.0) continue;
+              block(k, l, z00, z01, z10, z11, pk, pl_);
+            }
 #pragma unroll
           for (int j = 0; j < MAXC; ++j)
             if (pl.crk[j] >= 0) crow2(pl.crk[j] >> 16, pl.crk[j] & 0xffff);
@@ -1123,7 +1130,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
       mm_nn<NT, MAXT, true, false>(C, A, A, ld);
       clk.lap(TA_PROF_TRANSFORM);
       int sweeps = 0;
-      const bool jok = jacobi<NT, MAXB, MAXC>(A, C, ld, jr, jplan, red, b.jacobi_tol, &sweeps);
+      const bool jok =
+          jacobi<NT, MAXB, MAXC>(A, C, ld, jr, jplan, red, b.jacobi_tol, b.jacobi_skip, &sweeps);
       clk.lap(TA_PROF_JACOBI);
       clk.acc[TA_PROF_SWEEPS] += sweeps;
       if (!jok) {

@@ -24,11 +24,13 @@ __global__ void k_fill(double* __restrict__ p, int64_t n, double v)
 }
 
 // --------------------------------------------------------------- k_setup
-// ncoord_dexp (src/coulomb.cu:173-245 text), get_repulsion_derivs energy
-// (src/xtb.cu:62-91 text), get_mrad (src/coulomb.cu:387-397 text),
-// get_selfenergy (src/h0.cu:137-170 text), get_occupation (src/h0.cu:61-91),
-// get_amat_0d (src/coulomb.cu:15-47 text), get_multipole_matrix_0d
-// (src/coulomb.cu:439-467 text).
+// Per atom and shell: ncoord_dexp (src/coulomb.cu:173-245 text),
+// get_repulsion_derivs energy (src/xtb.cu:62-91 text), get_mrad
+// (src/coulomb.cu:387-397 text), the D4 CN and zero-charge weights,
+// get_selfenergy (src/h0.cu:137-170 text), get_occupation (src/h0.cu:61-91).
+// The pair planes (get_amat_0d, src/coulomb.cu:15-47 text;
+// get_multipole_matrix_0d, src/coulomb.cu:439-467 text; D4) follow in
+// k_setup_pairs.
 __device__ __forceinline__ double exp_count(double k, double r, double r0)
 {
   return 1.0 / (1.0 + exp(-k * (r0 / r - 1.0)));
@@ -99,12 +101,35 @@ __global__ __launch_bounds__(SETUP_THREADS) void k_setup(BatchDev b)
     b.selfenergy[s0 + s] = e.selfenergy[ish] - e.kcn[ish] * b.cn[a0 + at];
     b.n0sh[s0 + s] = e.refocc[ish];
   }
+}
 
-  // ES2 interaction matrix, nsh x nsh, arithmetic mean of shell hardness
+// ---------------------------------------------------------- k_setup_pairs
+// Atom-pair and shell-pair interaction planes of rows [a_begin, a_end) of one
+// molecule (work item w; k_setup has written the CN-dependent radii and the
+// zero-charge D4 weights): the ES2 matrix rows of the shells of those atoms,
+// the AES2 planes, the D4 damping factors and zero-charge C6.  Large
+// molecules spread over many workgroups.
+__global__ __launch_bounds__(SETUP_THREADS) void k_setup_pairs(BatchDev b, const int* __restrict__ work_mol,
+                                                               const int* __restrict__ work_a0,
+                                                               const int* __restrict__ work_a1)
+{
+  const int w = blockIdx.x, tid = threadIdx.x;
+  const int m = work_mol[w], ab = work_a0[w], ae = work_a1[w];
+  const int a0 = b.mol_atom_off[m], nat = b.mol_nat[m];
+  const int s0 = b.mol_sh_off[m], nsh = b.mol_nsh[m];
+  const double* __restrict__ xyz = b.xyz + 3 * (int64_t)a0;
+  const int* __restrict__ el = b.at_el + a0;
+  const DevParams& P = *b.par;
+  const DevD4& d4 = P.d4;
+
+  // ES2 interaction matrix rows of the shells of atoms [ab, ae), arithmetic
+  // mean of shell hardness
   double* __restrict__ gam = b.gam + b.mol_gam_off[m];
   const double g = P.gexp;
-  for (int idx = tid; idx < nsh * nsh; idx += blockDim.x) {
-    int i = idx / nsh, j = idx - i * nsh;
+  const int sb = b.at_ish[a0 + ab];
+  const int se = ae < nat ? b.at_ish[a0 + ae] : nsh;
+  for (int idx = tid; idx < (se - sb) * nsh; idx += blockDim.x) {
+    int i = sb + idx / nsh, j = idx - (idx / nsh) * nsh;
     int ai = b.sh_at[s0 + i], aj = b.sh_at[s0 + j];
     const DevElem& ei = P.el[el[ai]];
     const DevElem& ej = P.el[el[aj]];
@@ -118,7 +143,7 @@ __global__ __launch_bounds__(SETUP_THREADS) void k_setup(BatchDev b)
       double r1 = sqrt(dx * dx + dy * dy + dz * dz);
       v = 1.0 / pow(pow(r1, g) + pow(eta, -g), 1.0 / g);
     }
-    gam[idx] = v;
+    gam[(int64_t)i * nsh + j] = v;
   }
 
   // AES2 matrices, planar: sd[k][i][j] = amat_sd(k, j, i), etc.
@@ -127,7 +152,8 @@ __global__ __launch_bounds__(SETUP_THREADS) void k_setup(BatchDev b)
   double* __restrict__ dd = b.aes_dd + 9 * b.mol_pair_off[m];
   double* __restrict__ sq = b.aes_sq + 6 * b.mol_pair_off[m];
   const double* __restrict__ mrad = b.mrad + a0;
-  for (int64_t idx = tid; idx < n2; idx += blockDim.x) {
+  const int64_t p0 = (int64_t)ab * nat, p1 = (int64_t)ae * nat;
+  for (int64_t idx = p0 + tid; idx < p1; idx += blockDim.x) {
     int i = (int)(idx / nat), j = (int)(idx - (int64_t)i * nat);
     double v[3] = {0.0, 0.0, 0.0};
     double g3 = 0.0, g5 = 0.0, f3 = 0.0, f5 = 0.0;
@@ -161,7 +187,7 @@ __global__ __launch_bounds__(SETUP_THREADS) void k_setup(BatchDev b)
     double* __restrict__ c60 = b.d4_c60 + b.mol_pair_off[m];
     const double* __restrict__ gw0 = b.d4_gw + TA_D4_MAXREF * (int64_t)a0;
     const double cut2 = d4.disp2_cutoff * d4.disp2_cutoff;
-    for (int64_t idx = tid; idx < n2; idx += blockDim.x) {
+    for (int64_t idx = p0 + tid; idx < p1; idx += blockDim.x) {
       const int i = (int)(idx / nat), j = (int)(idx - (int64_t)i * nat);
       double dv = 0.0, cv = 0.0;
       if (i != j) {

@@ -205,6 +205,9 @@ struct BatchDev {
   // identities: the element is set to zero and the workers skip the blocks
   // and MO rows they would have touched (0: every nonzero element rotates)
   double jacobi_skip;
+  // > 0: per-iteration stopping threshold max(jacobi_tol, min(1e-2,
+  // jacobi_adapt * previous charge residual))
+  double jacobi_adapt;
   // optional per-molecule phase clocks of the SCF kernel (TA_PROF_SLOTS each), may be null
   long long* prof;
   // optional completion records (one per molecule, pinned host memory), may be null

@@ -331,6 +331,15 @@ struct tblite_amd_context_s {
   // are most of them (4096 water clusters: Jacobi time at 65-96 AOs -9 %,
   // energies unchanged to 5e-14 Eh, same iteration counts)
   double jacobi_skip = 1.0e-8;
+  // > 0: Jacobi threshold follows the SCF charge residual,
+  // max(jacobi_tol, min(1e-2, jacobi_adapt * residual of the last iteration))
+  // (TBLITE_AMD_JACOBI_ADAPT; 0 keeps jacobi_tol throughout).  The vectors
+  // are carried to the next iteration, which refines them, and at
+  // self-consistency (residual < 2e-5) the threshold is jacobi_tol.  4096
+  // water clusters: 3.19 -> 2.09 sweeps per iteration at 65-96 AOs, SCF
+  // iterations unchanged (one molecule one fewer), |dE| vs the CPU oracle
+  // 6.9e-10 Eh; a factor 1.0 adds an iteration to some molecules.
+  double jacobi_adapt = 0.1;
   // minimum LDS request of the global-memory SCF kernel (bytes); above half a
   // CU's LDS it runs one molecule per CU (TBLITE_AMD_BIG_LDS_MIN_KB)
   size_t big_lds_min = 0;
@@ -862,6 +871,7 @@ int upload_plan(tblite_amd_context ctx, Calculator& c, const Plan& pl, DeviceBat
   b.intcut = 0.0;
   b.jacobi_tol = ctx->jacobi_tol;
   b.jacobi_skip = ctx->jacobi_skip;
+  b.jacobi_adapt = ctx->jacobi_adapt;
   b.prof = ctx->profile ? carve<long long>(ar, (int64_t)nmol * TA_PROF_SLOTS) : nullptr;
   db.bytes = ar.used;
   if (ar.used > ar.cap) {
@@ -1244,6 +1254,8 @@ static tblite_amd_context make_context(int device, bool single_stream)
   ctx->grid_graph = !(pgg && pgg[0] == '0');
   const char* pb = std::getenv("TBLITE_AMD_BIG_LDS_MIN_KB");
   if (pb) ctx->big_lds_min = (size_t)std::max(0, std::min(160, std::atoi(pb))) * 1024;
+  const char* pa = std::getenv("TBLITE_AMD_JACOBI_ADAPT");
+  if (pa) ctx->jacobi_adapt = std::max(0.0, std::min(10.0, std::atof(pa)));
   const char* ps = std::getenv("TBLITE_AMD_JACOBI_SKIP");
   if (ps) ctx->jacobi_skip = std::max(0.0, std::min(1e-6, std::atof(ps)));
   const char* pj = std::getenv("TBLITE_AMD_JACOBI_TOL");

@@ -840,6 +840,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
   int iter = 0, mixiter = 0;
   bool converged = false;
   int last_nocc = 0;
+  double perr_last = 1.0;  // charge residual of the previous iteration
 
   while (!status && !converged && iter < b.max_iter) {
     elast = esum;
@@ -1130,8 +1131,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
       mm_nn<NT, MAXT, true, false>(C, A, A, ld);
       clk.lap(TA_PROF_TRANSFORM);
       int sweeps = 0;
-      const bool jok =
-          jacobi<NT, MAXB, MAXC>(A, C, ld, jr, jplan, red, b.jacobi_tol, b.jacobi_skip, &sweeps);
+      // looser stopping rule while the charges are still far from
+      // self-consistency (jacobi_adapt > 0): the next iterations start from
+      // these vectors and refine them
+      const double jtol =
+          b.jacobi_adapt > 0.0 ? fmax(b.jacobi_tol, fmin(1.0e-2, b.jacobi_adapt * perr_last))
+                               : b.jacobi_tol;
+      const bool jok = jacobi<NT, MAXB, MAXC>(A, C, ld, jr, jplan, red, jtol, b.jacobi_skip, &sweeps);
       clk.lap(TA_PROF_JACOBI);
       clk.acc[TA_PROF_SWEEPS] += sweeps;
       if (!jok) {
@@ -1325,6 +1331,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
     ed2 = d4.enabled ? block_sum(ed, red) : 0.0;
     esum = eband + ts + ec + ed2;
     const double perr = sqrt(err / ndim);
+    perr_last = perr;
     converged = (fabs(esum - elast) < econv) && (perr < pconv);
     if (!isfinite(esum)) status |= TA_STATUS_NAN;
     clk.lap(TA_PROF_ENERGY);

@@ -133,6 +133,108 @@ __device__ void mm_nn(const double* X, const double* Y, double* out, int ld)
   __syncthreads();
 }
 
+// The same products on the FP64 matrix cores (v_mfma_f64_16x16x4f64: lane l
+// supplies a[l%16][l/16] and b[l/16][l%16] and holds d[4i + l/16][l%16] of
+// the 16 x 16 result), operands read straight from LDS.  Each wave owns
+// 16 x 16 output tiles (tile w, w + NW, ...); at most MFMA_TILES(NT) per wave,
+// enough for the largest ld of each instantiation (128 threads: ld <= 32,
+// 256: ld <= 64, 512: ld <= 96).  Rows and columns beyond ld read index 0
+// and are masked to zero.
+#define MFMA_TILES(NT) ((NT) >= 512 ? 5 : ((NT) >= 256 ? 4 : 2))
+typedef double scf_double4 __attribute__((ext_vector_type(4)));
+
+template <int NT>
+__device__ __forceinline__ bool mfma_fits(int ld)
+{
+  const int nb = (ld + 15) >> 4;
+  return nb * nb <= MFMA_TILES(NT) * (NT / 64);
+}
+
+// out = X * Y (TRANS: out = (X * Y)^T); X plain row-major, Y and out skewed.
+// With HOLD the output aliases an input: every wave's tiles stay in
+// registers until all waves have read.
+template <int NT, bool HOLD, bool TRANS>
+__device__ void mm_mfma(const double* X, const double* Y, double* out, int ld)
+{
+  constexpr int NW = NT / 64;
+  constexpr int MT = MFMA_TILES(NT);
+  const int lane = threadIdx.x & 63, w = wave_id();
+  const int lr = lane & 15, lk = lane >> 4;
+  const int nb = (ld + 15) >> 4, ntile = nb * nb;
+  scf_double4 acc[MT];
+#pragma unroll
+  for (int t = 0; t < MT; ++t) {
+    acc[t] = scf_double4{0.0, 0.0, 0.0, 0.0};
+    const int tile = w + t * NW;
+    if (tile < ntile) {
+      const int i0 = (tile / nb) * 16, j0 = (tile % nb) * 16;
+      const bool oka = i0 + lr < ld, okb = j0 + lr < ld;
+      const int ra = oka ? i0 + lr : 0, cb = okb ? j0 + lr : 0;
+      for (int k = 0; k < ld; k += 4) {
+        const double a = X[ra * ld + k + lk];
+        const double bv = Y[sk(k + lk, cb, ld)];
+        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(oka ? a : 0.0, okb ? bv : 0.0, acc[t], 0, 0, 0);
+      }
+    }
+  }
+  if constexpr (HOLD) __syncthreads();
+#pragma unroll
+  for (int t = 0; t < MT; ++t) {
+    const int tile = w + t * NW;
+    if (tile < ntile) {
+      const int i0 = (tile / nb) * 16, j0 = (tile % nb) * 16;
+      const int c = j0 + lr;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = i0 + 4 * i + lk;
+        if (r < ld && c < ld) {
+          if constexpr (TRANS) out[sk(c, r, ld)] = acc[t][i];
+          else out[sk(r, c, ld)] = acc[t][i];
+        }
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// P = sum_t f_t CT[occ_t, :]^T CT[occ_t, :] on the matrix cores; the operands
+// are rows of CT (consecutive lanes read consecutive elements)
+template <int NT>
+__device__ void density_mfma(const double* CT, const double* f, const int* occ, int nocc,
+                             double* P, int ld)
+{
+  constexpr int NW = NT / 64;
+  constexpr int MT = MFMA_TILES(NT);
+  const int lane = threadIdx.x & 63, w = wave_id();
+  const int lr = lane & 15, lk = lane >> 4;
+  const int nb = (ld + 15) >> 4, ntile = nb * nb;
+#pragma unroll
+  for (int t = 0; t < MT; ++t) {
+    const int tile = w + t * NW;
+    if (tile >= ntile) continue;
+    const int i0 = (tile / nb) * 16, j0 = (tile % nb) * 16;
+    const bool oki = i0 + lr < ld, okj = j0 + lr < ld;
+    const int ci = oki ? i0 + lr : 0, cj = okj ? j0 + lr : 0;
+    scf_double4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < nocc; k += 4) {
+      const bool okt = k + lk < nocc;
+      const int row = occ[okt ? k + lk : 0];
+      const double* cr = CT + row * ld;
+      const double a = cr[ci] * f[row];
+      const double bv = cr[cj];
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64((okt && oki) ? a : 0.0, (okt && okj) ? bv : 0.0,
+                                                 acc, 0, 0, 0);
+    }
+    const int c = j0 + lr;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int r = i0 + 4 * i + lk;
+      if (r < ld && c < ld) P[sk(r, c, ld)] = acc[i];
+    }
+  }
+  __syncthreads();
+}
+
 // P = sum_t f_t CT[occ_t, :]^T CT[occ_t, :] over the occupied MO rows
 // (get_density_matrix, src/wavefunction/type.cu:17-35 text)
 template <int NT>
@@ -1127,8 +1229,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
     // ------------------------------------------------ eigensolve
     if constexpr (MATS_LDS) {
       // A <- C^T F C = CT F CT^T: U = CT F stored transposed, then CT U^T
-      mm_nn<NT, MAXT, true, true>(C, A, A, ld);
-      mm_nn<NT, MAXT, true, false>(C, A, A, ld);
+      if (mfma_fits<NT>(ld)) {
+        mm_mfma<NT, true, true>(C, A, A, ld);
+        mm_mfma<NT, true, false>(C, A, A, ld);
+      } else {
+        mm_nn<NT, MAXT, true, true>(C, A, A, ld);
+        mm_nn<NT, MAXT, true, false>(C, A, A, ld);
+      }
       clk.lap(TA_PROF_TRANSFORM);
       int sweeps = 0;
       // looser stopping rule while the charges are still far from
@@ -1204,7 +1311,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
     }
 
     // ------------------------------------------------ density -> A
-    if constexpr (MATS_LDS) density<NT>(C, focc_u, occl, last_nocc, A, ld);
+    if constexpr (MATS_LDS) {
+      if (mfma_fits<NT>(ld)) density_mfma<NT>(C, focc_u, occl, last_nocc, A, ld);
+      else density<NT>(C, focc_u, occl, last_nocc, A, ld);
+    }
     else big_density<NT>(CT, focc_u, occl, last_nocc, Y, n, ld, smem + L.eig);
     clk.lap(TA_PROF_DENSITY);
 

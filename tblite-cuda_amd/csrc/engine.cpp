@@ -317,29 +317,28 @@ struct tblite_amd_context_s {
   // replay the multi-workgroup path's eigenvalue phase from a HIP graph
   // (TBLITE_AMD_GRID_GRAPH=0 launches the kernels one by one instead)
   bool grid_graph = true;
-  // Jacobi stopping threshold on the largest rotated element of a sweep
-  // (TBLITE_AMD_JACOBI_TOL may change it).  After such a sweep the
-  // off-diagonal elements are of order tol^2 / gap: 1e-8 / gap here, which
-  // moves energies at second order only (4096 water clusters: |dE| vs the
-  // CPU oracle 1.6e-10 Eh, identical iteration counts; 1e-6 costs one more
-  // sweep per SCF iteration, 9 % of the batch time)
-  double jacobi_tol = 1.0e-4;
-  // Jacobi rotations with |a_pq| at or below this are skipped
-  // (TBLITE_AMD_JACOBI_SKIP): the size of the off-diagonal elements the
-  // stopping rule leaves anyway (jacobi_tol^2); the worker waves then skip
-  // the blocks and MO rows of identity rotations, which in the last sweeps
-  // are most of them (4096 water clusters: Jacobi time at 65-96 AOs -9 %,
-  // energies unchanged to 5e-14 Eh, same iteration counts)
+  // Jacobi stopping threshold on the largest rotated element of a sweep at
+  // self-consistency (TBLITE_AMD_JACOBI_TOL).  After such a sweep the
+  // off-diagonal elements are of order tol^2 / gap.  1e-6 keeps slowly
+  // converging SCFs (cations of water clusters, 29-49 iterations) within
+  // 2e-8 e of the CPU oracle's charges; 1e-4 throughout moved them by up to
+  // 3e-5 e, beyond the 1e-5 parity bound.
+  double jacobi_tol = 1.0e-6;
+  // Jacobi rotations with |a_pq| at or below min(this, threshold^2) are
+  // skipped (TBLITE_AMD_JACOBI_SKIP): off-diagonals of that size are left by
+  // the stopping rule anyway; the worker waves then skip the blocks and MO
+  // rows of identity rotations, most of them in the last sweeps.
   double jacobi_skip = 1.0e-8;
-  // > 0: Jacobi threshold follows the SCF charge residual,
+  // > 0: the Jacobi threshold follows the SCF charge residual,
   // max(jacobi_tol, min(1e-2, jacobi_adapt * residual of the last iteration))
   // (TBLITE_AMD_JACOBI_ADAPT; 0 keeps jacobi_tol throughout).  The vectors
-  // are carried to the next iteration, which refines them, and at
-  // self-consistency (residual < 2e-5) the threshold is jacobi_tol.  4096
-  // water clusters: 3.19 -> 2.09 sweeps per iteration at 65-96 AOs, SCF
-  // iterations unchanged (one molecule one fewer), |dE| vs the CPU oracle
-  // 6.9e-10 Eh; a factor 1.0 adds an iteration to some molecules.
-  double jacobi_adapt = 0.1;
+  // are carried to the next iteration, which refines them.  With 0.001 the
+  // threshold is loose only in the first two iterations (1e-3, then about
+  // 1e-5); water-cluster cations that need 29-49 iterations stay within
+  // 1e-6 e of the oracle's charges.  Larger factors move such slowly
+  // converging SCFs by up to 1e-4 e (0.01: 1.4e-4, 0.1: 1.4e-4): the early
+  // perturbation is carried by the Broyden history to where the SCF stops.
+  double jacobi_adapt = 0.001;
   // minimum LDS request of the global-memory SCF kernel (bytes); above half a
   // CU's LDS it runs one molecule per CU (TBLITE_AMD_BIG_LDS_MIN_KB)
   size_t big_lds_min = 0;

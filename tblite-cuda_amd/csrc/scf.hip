@@ -1240,11 +1240,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
       int sweeps = 0;
       // looser stopping rule while the charges are still far from
       // self-consistency (jacobi_adapt > 0): the next iterations start from
-      // these vectors and refine them
+      // these vectors and refine them.  Rotations below jtol^2 (at most
+      // jacobi_skip) are skipped: off-diagonals of that size remain anyway.
       const double jtol =
           b.jacobi_adapt > 0.0 ? fmax(b.jacobi_tol, fmin(1.0e-2, b.jacobi_adapt * perr_last))
                                : b.jacobi_tol;
-      const bool jok = jacobi<NT, MAXB, MAXC>(A, C, ld, jr, jplan, red, jtol, b.jacobi_skip, &sweeps);
+      const double jskip = fmin(b.jacobi_skip, jtol * jtol);
+      const bool jok = jacobi<NT, MAXB, MAXC>(A, C, ld, jr, jplan, red, jtol, jskip, &sweeps);
       clk.lap(TA_PROF_JACOBI);
       clk.acc[TA_PROF_SWEEPS] += sweeps;
       if (!jok) {

@@ -736,7 +736,10 @@ int upload_plan(tblite_amd_context ctx, Calculator& c, const Plan& pl, DeviceBat
   need += 2 * dbytes(Sh);                                // selfenergy, n0sh
   need += 3 * ibytes(A) + ibytes(pl.tot_nl) + ibytes(nmol); // inl, nnl, jat, nnz (+pad)
   need += 11 * dbytes(pl.tot_mat);                       // S, H, D, Q
-  need += dbytes(pl.tot_gam) + 18 * dbytes(pl.tot_pair); // gam, sd, dd, sq
+  // gam; sd, dd, sq planes only for the multi-workgroup SCF (the
+  // one-workgroup kernels evaluate the AES2 pair kernels where they use them)
+  const bool aes_planes = !pl.grid_mols.empty();
+  need += dbytes(pl.tot_gam) + (aes_planes ? 18 * dbytes(pl.tot_pair) : 0);
   need += dbytes(Sh) + dbytes(A) + dbytes(3 * A) + dbytes(6 * A) + 2 * dbytes(AO); // wfn
   need += 4 * dbytes(pl.tot_mix) + 2 * dbytes(pl.tot_mixmem);
   need += dbytes((int64_t)nmol * mem * mem) + dbytes((int64_t)nmol * mem * (mem + 1)) +
@@ -829,9 +832,9 @@ int upload_plan(tblite_amd_context ctx, Calculator& c, const Plan& pl, DeviceBat
   b.D = carve<double>(ar, 3 * pl.tot_mat);
   b.Q = carve<double>(ar, 6 * pl.tot_mat);
   b.gam = carve<double>(ar, pl.tot_gam);
-  b.aes_sd = carve<double>(ar, 3 * pl.tot_pair);
-  b.aes_dd = carve<double>(ar, 9 * pl.tot_pair);
-  b.aes_sq = carve<double>(ar, 6 * pl.tot_pair);
+  b.aes_sd = aes_planes ? carve<double>(ar, 3 * pl.tot_pair) : nullptr;
+  b.aes_dd = aes_planes ? carve<double>(ar, 9 * pl.tot_pair) : nullptr;
+  b.aes_sq = aes_planes ? carve<double>(ar, 6 * pl.tot_pair) : nullptr;
   b.qsh = carve<double>(ar, Sh);
   b.qat = carve<double>(ar, A);
   b.dpat = carve<double>(ar, 3 * A);

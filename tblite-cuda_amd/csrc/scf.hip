@@ -28,6 +28,7 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include "aes2.h"
 #include "common.h"
 #include "d4.h"
 #include "device_util.h"
@@ -62,9 +63,8 @@ struct MolView {
   const double* D;
   const double* Q;
   const double* gam;
-  const double* sd;
-  const double* dd;
-  const double* sq;
+  const double* xyz;   // atom positions and multipole radii (AES2 pair kernels)
+  const double* mrad;
   double* qsh;
   double* qat;
   double* dpat;
@@ -820,10 +820,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
   v.D = b.D + 3 * moff;
   v.Q = b.Q + 6 * moff;
   v.gam = b.gam + b.mol_gam_off[m];
-  const int64_t p2 = (int64_t)nat * nat;
-  v.sd = b.aes_sd + 3 * b.mol_pair_off[m];
-  v.dd = b.aes_dd + 9 * b.mol_pair_off[m];
-  v.sq = b.aes_sq + 6 * b.mol_pair_off[m];
+  v.xyz = b.xyz + 3 * (int64_t)v.a0;
+  v.mrad = b.mrad + v.a0;
   v.qsh = b.qsh + v.s0;
   v.qat = b.qat + v.a0;
   v.dpat = b.dpat + 3 * (int64_t)v.a0;
@@ -1122,27 +1120,27 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
       }
     }
     // AES2 atomic potentials: four atoms a per wave (16-lane segments), lanes
-    // over partners i, reading row a of every interaction plane (amat_sd is
-    // antisymmetric in the atom pair, amat_dd and amat_sq symmetric)
+    // over partners i, each lane evaluating the pair kernels of (a, i)
+    // (amat_sd is antisymmetric in the atom pair, amat_dd and amat_sq
+    // symmetric)
     for (int a0 = 4 * wid; a0 < nat; a0 += 4 * NW) {
       const int a = a0 + seg;
       double va = 0.0, d[3] = {0.0, 0.0, 0.0}, q[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-      const int64_t row = (int64_t)a * nat;
       for (int i = sl; i < nat && a < nat; i += 16) {
-        const int64_t ai = row + i;
+        const AesPair pr = aes_pair(v.xyz, v.mrad, a, i, P.dmp3, P.dmp5);
         const double qi = v.qat[i];
         const double dpi[3] = {v.dpat[3 * i], v.dpat[3 * i + 1], v.dpat[3 * i + 2]};
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-          const double sdk = v.sd[k * p2 + ai];  // amat_sd(k, i, a) = -amat_sd(k, a, i)
+          const double sdk = aes_sd(pr, k);  // amat_sd(k, i, a) = -amat_sd(k, a, i)
           va += sdk * dpi[k];
           d[k] -= sdk * qi;
 #pragma unroll
-          for (int l = 0; l < 3; ++l) d[k] += v.dd[(3 * l + k) * p2 + ai] * dpi[l];
+          for (int l = 0; l < 3; ++l) d[k] += aes_dd(pr, l, k) * dpi[l];
         }
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
-          const double sqk = v.sq[k * p2 + ai];
+          const double sqk = aes_sq(pr, k);
           va += sqk * v.qpat[6 * i + k];
           q[k] += sqk * qi;
         }
@@ -1415,17 +1413,19 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
     }
     for (int ai = tid; ai < nat * nat; ai += NT) {
       const int a = ai / nat, i = ai - a * nat;
+      if (a == i) continue;  // the pair kernels vanish on the diagonal
+      const AesPair pr = aes_pair(v.xyz, v.mrad, a, i, P.dmp3, P.dmp5);
       const double qi = v.qat[i];
       const double dpi[3] = {v.dpat[3 * i], v.dpat[3 * i + 1], v.dpat[3 * i + 2]};
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        double t = -v.sd[k * p2 + ai] * qi;
+        double t = -aes_sd(pr, k) * qi;
 #pragma unroll
-        for (int l = 0; l < 3; ++l) t += 0.5 * v.dd[(3 * l + k) * p2 + ai] * dpi[l];
+        for (int l = 0; l < 3; ++l) t += 0.5 * aes_dd(pr, l, k) * dpi[l];
         ec += v.dpat[3 * a + k] * t;
       }
 #pragma unroll
-      for (int k = 0; k < 6; ++k) ec += v.qpat[6 * a + k] * v.sq[k * p2 + ai] * qi;
+      for (int k = 0; k < 6; ++k) ec += v.qpat[6 * a + k] * aes_sq(pr, k) * qi;
     }
     for (int a = tid; a < nat; a += NT) {
       const DevElem& e = P.el[v.el[a]];

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include "aes2.h"
 #include "common.h"
 #include "d4.h"
 #include "device_util.h"
@@ -148,38 +149,21 @@ __global__ __launch_bounds__(SETUP_THREADS) void k_setup_pairs(BatchDev b, const
 
   // AES2 matrices, planar: sd[k][i][j] = amat_sd(k, j, i), etc.
   const int64_t n2 = (int64_t)nat * nat;
-  double* __restrict__ sd = b.aes_sd + 3 * b.mol_pair_off[m];
-  double* __restrict__ dd = b.aes_dd + 9 * b.mol_pair_off[m];
-  double* __restrict__ sq = b.aes_sq + 6 * b.mol_pair_off[m];
+  // (only the multi-workgroup SCF reads the planes: null when it has no molecule)
+  double* __restrict__ sd = b.aes_sd ? b.aes_sd + 3 * b.mol_pair_off[m] : nullptr;
+  double* __restrict__ dd = b.aes_dd ? b.aes_dd + 9 * b.mol_pair_off[m] : nullptr;
+  double* __restrict__ sq = b.aes_sq ? b.aes_sq + 6 * b.mol_pair_off[m] : nullptr;
   const double* __restrict__ mrad = b.mrad + a0;
   const int64_t p0 = (int64_t)ab * nat, p1 = (int64_t)ae * nat;
-  for (int64_t idx = p0 + tid; idx < p1; idx += blockDim.x) {
-    int i = (int)(idx / nat), j = (int)(idx - (int64_t)i * nat);
-    double v[3] = {0.0, 0.0, 0.0};
-    double g3 = 0.0, g5 = 0.0, f3 = 0.0, f5 = 0.0;
-    if (i != j) {
-      for (int k = 0; k < 3; ++k) v[k] = xyz[3 * i + k] - xyz[3 * j + k];
-      double r1 = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-      double g1 = 1.0 / r1;
-      g3 = g1 * g1 * g1;
-      g5 = g3 * g1 * g1;
-      double rr = 0.5 * (mrad[j] + mrad[i]) * g1;
-      f3 = 1.0 / (1.0 + 6.0 * pow(rr, P.dmp3));
-      f5 = 1.0 / (1.0 + 6.0 * pow(rr, P.dmp5));
+  if (sd)
+    for (int64_t idx = p0 + tid; idx < p1; idx += blockDim.x) {
+      const int i = (int)(idx / nat), j = (int)(idx - (int64_t)i * nat);
+      const AesPair pr = aes_pair(xyz, mrad, i, j, P.dmp3, P.dmp5);
+      for (int k = 0; k < 3; ++k) sd[k * n2 + idx] = aes_sd(pr, k);
+      for (int k = 0; k < 3; ++k)
+        for (int l = 0; l < 3; ++l) dd[(k * 3 + l) * n2 + idx] = (i != j) ? aes_dd(pr, k, l) : 0.0;
+      for (int c = 0; c < 6; ++c) sq[c * n2 + idx] = aes_sq(pr, c);
     }
-    for (int k = 0; k < 3; ++k) sd[k * n2 + idx] = v[k] * g3 * f3;
-    for (int k = 0; k < 3; ++k)
-      for (int l = 0; l < 3; ++l)
-        dd[(k * 3 + l) * n2 + idx] =
-            (i != j) ? ((k == l ? g3 * f5 : 0.0) - v[k] * v[l] * 3.0 * g5 * f5) : 0.0;
-    double gf = g5 * f5;
-    sq[0 * n2 + idx] = v[0] * v[0] * gf;
-    sq[1 * n2 + idx] = 2.0 * v[0] * v[1] * gf;
-    sq[2 * n2 + idx] = v[1] * v[1] * gf;
-    sq[3 * n2 + idx] = 2.0 * v[0] * v[2] * gf;
-    sq[4 * n2 + idx] = 2.0 * v[1] * v[2] * gf;
-    sq[5 * n2 + idx] = v[2] * v[2] * gf;
-  }
 
   // D4 two-body damping factors d_ij and zero-charge C6_ij (three-body term)
   if (d4.enabled) {

@@ -1321,17 +1321,20 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
     // ------------------------------------------------ Mulliken + band energy
     double* part = work; // part[c * ld + i], c = 0 (PS), 1..3 (PD), 4..9 (PQ), 10 (PH)
     // four rows per wave (16-lane segments, lanes over j): four rows share
-    // each memory round trip
+    // each memory round trip.  With the matrices in LDS (n <= 16 JS) every
+    // lane issues the loads of all its columns at once: indices past n are
+    // clamped to column n - 1 and weighted by zero, so no load waits on a
+    // branch and the sums keep the order of the general loop.
     for (int i0 = 4 * wid; i0 < n; i0 += 4 * NW) {
       const int i = i0 + seg;
       double acc[11];
 #pragma unroll
       for (int c = 0; c < 11; ++c) acc[c] = 0.0;
       const int64_t row = (int64_t)i * n;
-#pragma unroll 2
-      for (int j = sl; j < n && i < n; j += 16) {
-        const double p = Pm[aix(i, j)];
-        const int64_t e = row + j;
+      auto mul_step = [&](int j, bool in) {
+        const int jc = in ? j : n - 1;
+        const double p = in ? Pm[aix(i, jc)] : 0.0;
+        const int64_t e = row + jc;
         acc[0] += p * v.S[e];
         acc[1] += p * v.D[e];
         acc[2] += p * v.D[n2 + e];
@@ -1339,6 +1342,17 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
 #pragma unroll
         for (int c = 0; c < 6; ++c) acc[4 + c] += p * v.Q[c * n2 + e];
         acc[10] += p * v.H[e];
+      };
+      if constexpr (MATS_LDS) {
+        constexpr int JS = NT >= 512 ? 6 : (NT >= 256 ? 4 : 2), CH = NT >= 512 ? 3 : 2;
+        if (i < n)
+#pragma unroll 1
+          for (int u0 = 0; u0 < JS; u0 += CH)
+#pragma unroll
+            for (int u = u0; u < u0 + CH; ++u) mul_step(sl + 16 * u, sl + 16 * u < n);
+      } else {
+#pragma unroll 2
+        for (int j = sl; j < n && i < n; j += 16) mul_step(j, true);
       }
 #pragma unroll
       for (int c = 0; c < 11; ++c) acc[c] = seg16_sum(acc[c]);

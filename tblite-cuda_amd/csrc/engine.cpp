@@ -1090,9 +1090,10 @@ int run_batch(tblite_amd_context ctx, Calculator& c, const double* xyz, double a
     const int cnt = (int)bk.mols.size();
     int rc;
     HIP_TRY(ctx, hipEventRecord(ctx->bev[2 * k], as));
-    // per-worker-thread Jacobi items (MAXB blocks of A, MAXC two-AO pieces of
-    // CT rows) over the NT - 64 worker threads: ld <= 32: 120 / 256 over 64;
-    // ld <= 64: 496 / 1024 over 192; ld <= 96: 1128 / 2304 over 448
+    // per-worker-thread Jacobi items over the NT - 64 worker threads: MAXB
+    // blocks of A (ld <= 32: 120 over 64; ld <= 64: 496 over 192; ld <= 96:
+    // 1128 over 448) and up to MAXC two-AO pieces of one slot's CT rows (4, 6
+    // and 8 threads per slot)
     if (!bk.lds)
       rc = launch_scf<TA_BIG_NT, 1, 0, 0, false>(ctx, as, b, mols, cnt, bk.ldmax, bk.natmax,
                                                  bk.nshmax);

@@ -364,10 +364,19 @@ struct JRot {
   double2* dg;   // [2][h] a_pp, a_qq before the rotation
 };
 
+// CT rows: JACOBI_TPS(NT) worker threads share one slot's row pair, thread t
+// of the group taking the two-AO pieces t, t + TPS, ... (up to MAXC of them),
+// so a thread reads its slot's rotation record once per round and a group's
+// lanes read contiguous 16-byte pieces.  Coverage: TPS * MAXC >= ld / 2 and
+// TPS * ld / 2 <= NT - 64 for the largest ld of each instantiation
+// (128 threads: ld <= 32, 256: ld <= 64, 512: ld <= 96).
+#define JACOBI_TPS(NT) ((NT) >= 512 ? 8 : ((NT) >= 256 ? 6 : 4))
+
 template <int MAXB, int MAXC>
 struct JPlan {
   int bkl[MAXB > 0 ? MAXB : 1];  // worker block items (k << 16 | l), -1 for none
-  int crk[MAXC > 0 ? MAXC : 1];  // worker CT items (k << 16 | ao), -1 for none
+  int cslot;                     // worker CT slot, -1 for none
+  int cpc;                       // first two-AO piece of the slot's rows
 };
 
 __device__ __forceinline__ bool jacobi_owner_block(int k, int l, int h)
@@ -383,9 +392,10 @@ __device__ void jacobi_plan(JPlan<MAXB, MAXC>& pl, int ld)
     const int nw = NT - 64, w = (int)threadIdx.x - 64;
 #pragma unroll
     for (int j = 0; j < MAXB; ++j) pl.bkl[j] = -1;
-#pragma unroll
-    for (int j = 0; j < MAXC; ++j) pl.crk[j] = -1;
+    pl.cslot = -1;
+    pl.cpc = 0;
     if (w < 0) return;
+    // consecutive workers take consecutive blocks (neighbouring columns of A)
     const int nblk = h * (h - 1) / 2;
 #pragma unroll
     for (int j = 0; j < MAXB; ++j) {
@@ -399,12 +409,13 @@ __device__ void jacobi_plan(JPlan<MAXB, MAXC>& pl, int ld)
       const int l = k + 1 + rem;
       if (!jacobi_owner_block(k, l, h)) pl.bkl[j] = (k << 16) | l;
     }
-    // CT items cover two adjacent AOs (one 16-byte LDS access per row)
-    const int hl = ld >> 1;
-#pragma unroll
-    for (int j = 0; j < MAXC; ++j) {
-      const int ci = w + j * nw;
-      if (ci < h * hl) pl.crk[j] = ((ci / hl) << 16) | (2 * (ci % hl));
+    // CT pieces cover two adjacent AOs (one 16-byte LDS access per row)
+    constexpr int TPS = JACOBI_TPS(NT);
+    static_assert(MAXC == 0 || TPS * MAXC >= (NT >= 512 ? 48 : (NT >= 256 ? 32 : 16)),
+                  "CT pieces per slot not covered");
+    if (w / TPS < h) {
+      pl.cslot = w / TPS;
+      pl.cpc = w % TPS;
     }
   }
 }
@@ -462,11 +473,9 @@ __device__ bool jacobi(double* A, double* CT, int ld, const JRot& R, const JPlan
     double offmax = 0.0;
     for (int r = 0; r < ld - 1; ++r, ++round) {
       const int co = (round & 1) * h, nxt = (round & 1) ^ 1;
-      auto block = [&](int k, int l, double& z00, double& z01, double& z10, double& z11,
-                       int2& pk, int2& pl_) {
-        pk = R.pq[co + k];
-        pl_ = R.pq[co + l];
-        const double2 ck = R.cs[co + k], cl = R.cs[co + l];
+      // 2x2 block (rows of slot k, columns of slot l) <- G_k^T block G_l
+      auto block_rot = [&](const int2 pk, const double2 ck, const int2 pl_, const double2 cl,
+                           double& z00, double& z01, double& z10, double& z11) {
         const double x00 = aup(A, ld, pk.x, pl_.x), x01 = aup(A, ld, pk.x, pl_.y);
         const double x10 = aup(A, ld, pk.y, pl_.x), x11 = aup(A, ld, pk.y, pl_.y);
         const double y00 = ck.x * x00 - ck.y * x10, y01 = ck.x * x01 - ck.y * x11;
@@ -479,6 +488,12 @@ __device__ bool jacobi(double* A, double* CT, int ld, const JRot& R, const JPlan
         aup(A, ld, pk.x, pl_.y) = z01;
         aup(A, ld, pk.y, pl_.x) = z10;
         aup(A, ld, pk.y, pl_.y) = z11;
+      };
+      auto block = [&](int k, int l, double& z00, double& z01, double& z10, double& z11,
+                       int2& pk, int2& pl_) {
+        pk = R.pq[co + k];
+        pl_ = R.pq[co + l];
+        block_rot(pk, R.cs[co + k], pl_, R.cs[co + l], z00, z01, z10, z11);
       };
       if (owner_wave) {
         // diagonal element after this round of the first (e = 0) or second index of slot k
@@ -524,17 +539,6 @@ __device__ bool jacobi(double* A, double* CT, int ld, const JRot& R, const JPlan
           *rp = cs.x * u - cs.y * v;
           *rq = cs.y * u + cs.x * v;
         };
-        // two adjacent AOs of both rows (ao even, rows 16-byte aligned)
-        auto crow2 = [&](int k, int ao) {
-          const double2 cs = R.cs[co + k];
-          if (cs.y == 0.0) return;  // identity rotation
-          const int2 pq = R.pq[co + k];
-          double2* rp = reinterpret_cast<double2*>(CT + pq.x * ld + ao);
-          double2* rq = reinterpret_cast<double2*>(CT + pq.y * ld + ao);
-          const double2 u = *rp, v = *rq;
-          *rp = make_double2(cs.x * u.x - cs.y * v.x, cs.x * u.y - cs.y * v.y);
-          *rq = make_double2(cs.y * u.x + cs.x * v.x, cs.y * u.y + cs.x * v.y);
-        };
         double z00, z01, z10, z11;
         int2 pk, pl_;
         if constexpr (MAXB > 0) {
@@ -546,9 +550,34 @@ __device__ bool jacobi(double* A, double* CT, int ld, const JRot& R, const JPlan
               if (R.cs[co + k].y == 0.0 && R.cs[co + l].y == 0.0) continue;
               block(k, l, z00, z01, z10, z11, pk, pl_);
             }
+          // the slot's row pair: one record read, then every piece's loads
+          // before any store
+          if (pl.cslot >= 0) {
+            constexpr int TPS = JACOBI_TPS(NT);
+            const double2 cs = R.cs[co + pl.cslot];
+            if (cs.y != 0.0) {  // identity rotation: rows unchanged
+              const int2 pq = R.pq[co + pl.cslot];
+              double2* rp = reinterpret_cast<double2*>(CT + pq.x * ld);
+              double2* rq = reinterpret_cast<double2*>(CT + pq.y * ld);
+              double2 u[MAXC], v[MAXC];
 #pragma unroll
-          for (int j = 0; j < MAXC; ++j)
-            if (pl.crk[j] >= 0) crow2(pl.crk[j] >> 16, pl.crk[j] & 0xffff);
+              for (int j = 0; j < MAXC; ++j) {
+                const int c2 = pl.cpc + j * TPS;
+                if (c2 < h) {
+                  u[j] = rp[c2];
+                  v[j] = rq[c2];
+                }
+              }
+#pragma unroll
+              for (int j = 0; j < MAXC; ++j) {
+                const int c2 = pl.cpc + j * TPS;
+                if (c2 < h) {
+                  rp[c2] = make_double2(cs.x * u[j].x - cs.y * v[j].x, cs.x * u[j].y - cs.y * v[j].y);
+                  rq[c2] = make_double2(cs.y * u[j].x + cs.x * v[j].x, cs.y * u[j].y + cs.x * v[j].y);
+                }
+              }
+            }
+          }
         } else {
           const int nw = NT - 64, w = (int)threadIdx.x - 64;
           const int nblk = h * (h - 1) / 2;

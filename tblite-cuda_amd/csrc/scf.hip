@@ -284,36 +284,48 @@ __device__ bool orthonormalizer(double* A, double* C, int n, int ld, double* sca
 {
   constexpr int NW = NT / 64;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  // Right-looking L D L^T elimination without square roots, one barrier per
+  // column: column j and the pivot d_j are final when step j starts and step
+  // j writes only columns c > j, A[r][c] -= A[r][j] A[c][j] / d_j
+  // (j < c <= r).  Then L = (unit lower part) D^1/2: L[i][j] = A[i][j] /
+  // sqrt(d_j), L[j][j] = sqrt(d_j).
   for (int j = 0; j < n; ++j) {
-    if (tid == 0) {
-      const double d = A[sk(j, j, ld)];
-      scal[0] = (d > 0.0) ? sqrt(d) : -1.0;
-    }
-    __syncthreads();
-    const double d = scal[0];
-    if (!(d > 0.0)) return false;
-    for (int i = j + 1 + tid; i < n; i += NT) {
-      const double v = A[sk(i, j, ld)] / d;
-      A[sk(i, j, ld)] = v;
-      col[i] = v;
-    }
-    if (tid == 0) A[sk(j, j, ld)] = d;
-    __syncthreads();
-    // trailing update of the lower triangle
+    const double dj = A[sk(j, j, ld)];
+    if (!(dj > 0.0)) return false;  // the same value in every thread
+    const double rdj = 1.0 / dj;
+#pragma unroll 2
     for (int r = j + 1 + wid; r < n; r += NW) {
-      const double lr = col[r];
-      for (int c = j + 1 + lane; c <= r; c += 64) A[sk(r, c, ld)] -= lr * col[c];
+      const double lr = A[sk(r, j, ld)] * rdj;
+      for (int c = j + 1 + lane; c <= r; c += 64) A[sk(r, c, ld)] -= lr * A[sk(c, j, ld)];
     }
     __syncthreads();
   }
+  for (int j = tid; j < n; j += NT) col[j] = sqrt(A[sk(j, j, ld)]);
+  __syncthreads();
+  for (int idx = tid; idx < n * n; idx += NT) {
+    const int i = idx / n, j = idx - (idx / n) * n;
+    if (i > j) A[sk(i, j, ld)] /= col[j];
+    else if (i == j) A[sk(i, i, ld)] = col[i];
+  }
+  __syncthreads();
   for (int idx = tid; idx < ld * ld; idx += NT) C[idx] = 0.0;
   __syncthreads();
   if constexpr (MATS_LDS) {
-    for (int j = tid; j < n; j += NT) {
+    // C = L^-1 by columns, four adjacent lanes per column: lane t sums the
+    // terms k = j + t (mod 4) of C[i][j] = (delta_ij - sum_k L[i][k] C[k][j])
+    // / L[i][i], two butterfly steps give every lane the same total, and lane
+    // (i - j) % 4 stores C[i][j], so a lane reads only the elements it stored
+    double* rdg = col + ld;
+    for (int i = tid; i < n; i += NT) rdg[i] = 1.0 / col[i];
+    __syncthreads();
+    for (int j = tid >> 2; j < n; j += NT >> 2) {
+      const int t = tid & 3;
       for (int i = j; i < n; ++i) {
-        double s = (i == j) ? 1.0 : 0.0;
-        for (int k = j; k < i; ++k) s -= A[sk(i, k, ld)] * C[k * ld + j];
-        C[i * ld + j] = s / A[sk(i, i, ld)];
+        double s = 0.0;
+        for (int k = j + t; k < i; k += 4) s -= A[sk(i, k, ld)] * C[k * ld + j];
+        s += __shfl_xor(s, 1, 4);
+        s += __shfl_xor(s, 2, 4);
+        if (((i - j) & 3) == t) C[i * ld + j] = ((i == j ? 1.0 : 0.0) + s) * rdg[i];
       }
     }
   } else {
@@ -953,9 +965,23 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
   for (int i = tid; i < ndim; i += NT) qin[i] = 0.0;
 
   // --- orthonormal start basis C = L^-T
-  for (int idx = tid; idx < ld * ld; idx += NT) {
-    const int i = idx / ld, j = idx - (idx / ld) * ld;
-    A[sk(i, j, ld)] = (i < n && j < n) ? v.S[(int64_t)i * n + j] : 0.0;
+  {
+    // six loads per thread in flight
+    constexpr int SB = 6;
+    for (int base = tid; base < ld * ld; base += SB * NT) {
+      double sv[SB];
+#pragma unroll
+      for (int u = 0; u < SB; ++u) {
+        const int idx = base + u * NT;
+        const int i = idx / ld, j = idx - (idx / ld) * ld;
+        sv[u] = (idx < ld * ld && i < n && j < n) ? v.S[(int64_t)i * n + j] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < SB; ++u) {
+        const int idx = base + u * NT;
+        if (idx < ld * ld) A[sk(idx / ld, idx - (idx / ld) * ld, ld)] = sv[u];
+      }
+    }
   }
   __syncthreads();
   if (!orthonormalizer<NT, MATS_LDS>(A, C, n, ld, scal, work)) status |= TA_STATUS_CHOLESKY;

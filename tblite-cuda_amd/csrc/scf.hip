@@ -1230,9 +1230,46 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
 
     // ------------------------------------------------ Fock matrix
     // A1[i][j] = (H_ij - S_ij (v_i + v_j)/2)/2 - (D(.,j,i).vdp(at i) + Q(.,j,i).vqp(at i))/2
-    // four elements per thread and step: their eleven integral loads each are
-    // issued before any store
-    {
+    // LDS path: four rows per wave (16-lane segments, lanes over j), the row's
+    // potentials read once per row, the integral loads of CH column steps
+    // issued together (as in the Mulliken loop).  Global-memory path: four
+    // elements per thread and step, their eleven integral loads each issued
+    // before any store.
+    if constexpr (MATS_LDS) {
+      constexpr int JS = NT >= 512 ? 6 : (NT >= 256 ? 4 : 2), CH = NT >= 512 ? 3 : 2;
+      for (int i0 = 4 * wid; i0 < ld; i0 += 4 * NW) {
+        const int i = i0 + seg;
+        const bool rin = i < n;
+        const int ai = rin ? v.ao_at[i] : 0;
+        const double vi = rin ? vao[i] : 0.0;
+        const double* vd = vdp + 3 * ai;
+        const double* vq = vqp + 6 * ai;
+        const double vd0 = vd[0], vd1 = vd[1], vd2 = vd[2];
+        const double vq0 = vq[0], vq1 = vq[1], vq2 = vq[2], vq3 = vq[3], vq4 = vq[4], vq5 = vq[5];
+        const int64_t row = (int64_t)(rin ? i : 0) * n;
+#pragma unroll 1
+        for (int u0 = 0; u0 < JS; u0 += CH) {
+          double fv[CH];
+#pragma unroll
+          for (int u = 0; u < CH; ++u) {
+            const int j = sl + 16 * (u0 + u);
+            const bool in = rin && j < n;
+            const int jc = in ? j : 0;
+            const int64_t e = row + jc;
+            double f = 0.5 * (v.H[e] - v.S[e] * 0.5 * (vi + vao[jc]));
+            f -= 0.5 * (v.D[e] * vd0 + v.D[n2 + e] * vd1 + v.D[2 * n2 + e] * vd2);
+            f -= 0.5 * (v.Q[e] * vq0 + v.Q[n2 + e] * vq1 + v.Q[2 * n2 + e] * vq2 +
+                        v.Q[3 * n2 + e] * vq3 + v.Q[4 * n2 + e] * vq4 + v.Q[5 * n2 + e] * vq5);
+            fv[u] = in ? f : 0.0;
+          }
+#pragma unroll
+          for (int u = 0; u < CH; ++u) {
+            const int j = sl + 16 * (u0 + u);
+            if (j < ld) A[aix(i, j)] = fv[u];
+          }
+        }
+      }
+    } else {
       constexpr int FB = 4;
       const int tot = ld * ld;
       for (int base = tid; base < tot; base += FB * NT) {

@@ -208,28 +208,49 @@ __device__ void density_mfma(const double* CT, const double* f, const int* occ, 
   const int lane = threadIdx.x & 63, w = wave_id();
   const int lr = lane & 15, lk = lane >> 4;
   const int nb = (ld + 15) >> 4, ntile = nb * nb;
+  // k outer, tiles inner: the MO row and its occupation are read once per
+  // k step for all of the wave's tiles, whose accumulation chains overlap
+  scf_double4 acc[MT];
+  int ci[MT], cj[MT];
+  bool oki[MT], okj[MT];
+#pragma unroll
+  for (int t = 0; t < MT; ++t) {
+    acc[t] = scf_double4{0.0, 0.0, 0.0, 0.0};
+    const int tile = w + t * NW;
+    const int i0 = (tile / nb) * 16, j0 = (tile % nb) * 16;
+    oki[t] = i0 + lr < ld;
+    okj[t] = j0 + lr < ld;
+    ci[t] = oki[t] ? i0 + lr : 0;
+    cj[t] = okj[t] ? j0 + lr : 0;
+  }
+  for (int k = 0; k < nocc; k += 4) {
+    const bool okt = k + lk < nocc;
+    const int row = occ[okt ? k + lk : 0];
+    const double* cr = CT + row * ld;
+    const double fr = f[row];
+    double a[MT], bv[MT];
+#pragma unroll
+    for (int t = 0; t < MT; ++t)
+      if (w + t * NW < ntile) {
+        a[t] = cr[ci[t]] * fr;
+        bv[t] = cr[cj[t]];
+      }
+#pragma unroll
+    for (int t = 0; t < MT; ++t)
+      if (w + t * NW < ntile)
+        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64((okt && oki[t]) ? a[t] : 0.0,
+                                                      (okt && okj[t]) ? bv[t] : 0.0, acc[t], 0, 0, 0);
+  }
 #pragma unroll
   for (int t = 0; t < MT; ++t) {
     const int tile = w + t * NW;
     if (tile >= ntile) continue;
     const int i0 = (tile / nb) * 16, j0 = (tile % nb) * 16;
-    const bool oki = i0 + lr < ld, okj = j0 + lr < ld;
-    const int ci = oki ? i0 + lr : 0, cj = okj ? j0 + lr : 0;
-    scf_double4 acc = {0.0, 0.0, 0.0, 0.0};
-    for (int k = 0; k < nocc; k += 4) {
-      const bool okt = k + lk < nocc;
-      const int row = occ[okt ? k + lk : 0];
-      const double* cr = CT + row * ld;
-      const double a = cr[ci] * f[row];
-      const double bv = cr[cj];
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64((okt && oki) ? a : 0.0, (okt && okj) ? bv : 0.0,
-                                                 acc, 0, 0, 0);
-    }
     const int c = j0 + lr;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int r = i0 + 4 * i + lk;
-      if (r < ld && c < ld) P[sk(r, c, ld)] = acc[i];
+      if (r < ld && c < ld) P[sk(r, c, ld)] = acc[t][i];
     }
   }
   __syncthreads();
@@ -1378,22 +1399,27 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
       for (int r = lane; r < n; r += 64) focc_u[inv[r]] = fsp[r];
       ts += fermi_wave(nbet, kt, n, emo_s, fsp);
       for (int r = lane; r < n; r += 64) focc_u[inv[r]] += fsp[r];
-      if (lane == 0) scal[3] = ts;
+      // levels with an occupation above TA_OCC_MIN enter the density, listed
+      // in level order (ballot compaction; each lane reads the occupations
+      // it wrote); the occupations fall with the level, so on the
+      // global-memory path (levels in ascending order) these are a leading
+      // range
+      int c = 0;
+      for (int r0 = 0; r0 < n; r0 += 64) {
+        const int r = r0 + lane;
+        const int k = r < n ? inv[r] : 0;
+        const bool take = r < n && focc_u[k] > TA_OCC_MIN;
+        const unsigned long long m = __ballot(take);
+        if (take) occl[c + __popcll(m & ((1ull << lane) - 1ull))] = k;
+        c += __popcll(m);
+      }
+      if (lane == 0) {
+        scal[3] = ts;
+        scal[4] = (double)c;
+      }
     }
     __syncthreads();
     ts = scal[3];
-    // levels with an occupation above TA_OCC_MIN enter the density; the
-    // occupations fall with the level, so on the global-memory path (levels
-    // in ascending order) these are a leading range
-    if (tid == 0) {
-      int c = 0;
-      for (int r = 0; r < n; ++r) {
-        const int k = inv[r];
-        if (focc_u[k] > TA_OCC_MIN) occl[c++] = k;
-      }
-      scal[4] = (double)c;
-    }
-    __syncthreads();
     last_nocc = (int)scal[4];
     clk.lap(TA_PROF_OCC);
     if constexpr (!MATS_LDS) {

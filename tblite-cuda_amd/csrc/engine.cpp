@@ -314,6 +314,8 @@ struct tblite_amd_context_s {
   // padded AO dimension above which a molecule's matrices leave LDS for the
   // global-memory single-workgroup path (TBLITE_AMD_LDS_MAX_LD may lower it)
   int lds_max_ld = kLdsMaxLd;
+  // size buckets in steps of 16 AOs instead of one per kernel instantiation
+  bool fine_buckets = false;
   // replay the multi-workgroup path's eigenvalue phase from a HIP graph
   // (TBLITE_AMD_GRID_GRAPH=0 launches the kernels one by one instead)
   bool grid_graph = true;
@@ -625,12 +627,27 @@ int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* 
   pl.tot_scr = scr;
 
   // size buckets over the padded AO dimension; largest molecules first
-  const int bounds[7] = {16, 32, 48, 64, 80, 96, 1 << 30};
+  // One bucket per kernel instantiation (ld <= 32, 33..64, 65..96) plus the
+  // LDS limit of the tuning as an extra bound; a bucket's ldmax sets its LDS
+  // request, and within each instantiation the occupancy is the same for
+  // every ldmax (the 512-thread kernel runs one molecule per CU, the 256- and
+  // 128-thread ones are held by waves), so wider buckets cost nothing and
+  // let the largest-first order cover all sizes (shorter tail).
+  // TBLITE_AMD_FINE_BUCKETS=1 restores steps of 16.
+  std::vector<int> bounds;
+  if (ctx->fine_buckets)
+    bounds = {16, 32, 48, 64, 80, 96};
+  else
+    bounds = {32, 64, 96};
+  if (std::find(bounds.begin(), bounds.end(), ctx->lds_max_ld) == bounds.end())
+    bounds.push_back(ctx->lds_max_ld);
+  std::sort(bounds.begin(), bounds.end());
+  bounds.push_back(1 << 30);
   std::vector<int> order(nmol);
   for (int m = 0; m < nmol; ++m) order[m] = m;
   std::stable_sort(order.begin(), order.end(),
                    [&](int a, int b) { return pl.mol_nao[a] > pl.mol_nao[b]; });
-  for (int k = 6; k >= 0; --k) {
+  for (int k = (int)bounds.size() - 1; k >= 0; --k) {
     Plan::Bucket bk;
     const int lo = (k == 0) ? 0 : bounds[k - 1];
     const int hi = bounds[k];
@@ -1257,6 +1274,8 @@ static tblite_amd_context make_context(int device, bool single_stream)
   ctx->grid_graph = !(pgg && pgg[0] == '0');
   const char* pb = std::getenv("TBLITE_AMD_BIG_LDS_MIN_KB");
   if (pb) ctx->big_lds_min = (size_t)std::max(0, std::min(160, std::atoi(pb))) * 1024;
+  const char* pfb = std::getenv("TBLITE_AMD_FINE_BUCKETS");
+  if (pfb) ctx->fine_buckets = pfb[0] == '1';
   const char* pa = std::getenv("TBLITE_AMD_JACOBI_ADAPT");
   if (pa) ctx->jacobi_adapt = std::max(0.0, std::min(10.0, std::atof(pa)));
   const char* ps = std::getenv("TBLITE_AMD_JACOBI_SKIP");

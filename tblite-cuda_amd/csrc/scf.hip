@@ -294,14 +294,14 @@ __device__ void density(const double* CT, const double* f, const int* occ, int n
 // ------------------------------------------------------------ Cholesky
 // S = L L^T in place (lower), then C = L^-T (stored as its transpose L^-1,
 // one start MO per row).  Returns false if S is not positive definite.  col:
-// n doubles of LDS scratch.  Right-looking: the scaled pivot column is kept
-// in LDS, and the trailing lower triangle is updated one wave per row with
-// lanes over its columns (coalesced along the row).  L^-1 is formed by
-// forward substitution: with the matrices in LDS one thread per column of
-// L^-1 (no barriers); in global memory row by row (row i of L in LDS, all
+// LDS scratch, n doubles (2 ld with the matrices in LDS).  Right-looking,
+// the trailing lower triangle updated one wave per row with lanes over its
+// columns (coalesced along the row).  L^-1 is formed by forward
+// substitution: with the matrices in LDS by columns of L^-1, four lanes per
+// column (no barriers); in global memory row by row (row i of L in LDS, all
 // columns j <= i in parallel, reads of earlier rows coalesced).
 template <int NT, bool MATS_LDS>
-__device__ bool orthonormalizer(double* A, double* C, int n, int ld, double* scal, double* col)
+__device__ bool orthonormalizer(double* A, double* C, int n, int ld, double* col)
 {
   constexpr int NW = NT / 64;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -1005,7 +1005,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
     }
   }
   __syncthreads();
-  if (!orthonormalizer<NT, MATS_LDS>(A, C, n, ld, scal, work)) status |= TA_STATUS_CHOLESKY;
+  if (!orthonormalizer<NT, MATS_LDS>(A, C, n, ld, work)) status |= TA_STATUS_CHOLESKY;
   clk.lap(TA_PROF_INIT);
 
   if (tid == 0) red[24] = 0.0;  // bisection warm-start bracket (global-memory path)

@@ -27,6 +27,7 @@ __global__ void k_setup(BatchDev b);
 __global__ void k_adjlist(BatchDev b);
 __global__ void k_d4_atm(BatchDev b, const int* work_mol, const int* work_i0, const int* work_i1);
 __global__ void k_setup_pairs(BatchDev b, const int* work_mol, const int* work_a0, const int* work_a1);
+template <int LMAX>
 __global__ void k_integrals(BatchDev b, const int* work_mol, const int* work_kind,
                             const int* work_start);
 template <int NT, int MAXT, int MAXB, int MAXC, bool MATS_LDS>
@@ -252,7 +253,10 @@ struct Plan {
   std::vector<int> at_el, at_ish, at_iao;
   std::vector<int> sh_at, sh_ish, sh_iao;
   std::vector<int> ao_at, ao_sh;
+  // integral work items: those of molecules without d shells first (nwork_sp
+  // of them, the s/p-only kernel), then the rest
   std::vector<int> work_mol, work_kind, work_start;
+  int nwork_sp = 0;
   // three-body dispersion work: chunks [atm_i0, atm_i1) of anchor atoms
   std::vector<int> atm_mol, atm_i0, atm_i1;
   // pair-plane work: atom rows [pair_a0, pair_a1) of one molecule
@@ -496,8 +500,10 @@ int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* 
   }
   int64_t atom = 0, shell = 0, ao = 0, mat = 0, nl = 0, gam = 0, pair = 0, mix = 0, mixmem = 0,
           big = 0, scr = 0;
+  std::vector<int> wd_mol, wd_kind, wd_start;  // integral work of molecules with d shells
   for (int m = 0; m < nmol; ++m) {
     const int nat = mol_nat[m];
+    bool has_d = false;
     if (nat < 1) {
       set_error(ctx, "molecule %d has %d atoms", m, nat);
       return TBLITE_AMD_ERR_ARGUMENT;
@@ -521,6 +527,7 @@ int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* 
       pl.at_iao.push_back(nao);
       min_alpha = std::min(min_alpha, c.min_alpha_el[e]);
       for (int s = 0; s < el.nsh; ++s) {
+        if (el.ang[s] >= 2) has_d = true;
         pl.sh_at.push_back(i);
         pl.sh_ish.push_back(s);
         pl.sh_iao.push_back(nao);
@@ -577,16 +584,21 @@ int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* 
     mix += ndim;
     mixmem += (int64_t)ndim * pl.mem;
     // integral work: neighbour chunks over the list capacity, on-site chunks
-    const int64_t cap = (int64_t)nat * (nat + 1) / 2;
-    for (int64_t s = 0; s < cap; s += kIntegralChunk) {
-      pl.work_mol.push_back(m);
-      pl.work_kind.push_back(0);
-      pl.work_start.push_back((int)s);
-    }
-    for (int s = 0; s < nat; s += kIntegralChunk) {
-      pl.work_mol.push_back(m);
-      pl.work_kind.push_back(1);
-      pl.work_start.push_back(s);
+    {
+      std::vector<int>& wm = has_d ? wd_mol : pl.work_mol;
+      std::vector<int>& wk = has_d ? wd_kind : pl.work_kind;
+      std::vector<int>& ws = has_d ? wd_start : pl.work_start;
+      const int64_t cap = (int64_t)nat * (nat + 1) / 2;
+      for (int64_t s = 0; s < cap; s += kIntegralChunk) {
+        wm.push_back(m);
+        wk.push_back(0);
+        ws.push_back((int)s);
+      }
+      for (int s = 0; s < nat; s += kIntegralChunk) {
+        wm.push_back(m);
+        wk.push_back(1);
+        ws.push_back(s);
+      }
     }
     // pair planes: rows of about kPairChunk pairs per work item
     {
@@ -619,6 +631,10 @@ int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* 
   pl.tot_ao = ao;
   pl.tot_mat = mat;
   pl.tot_nl = nl;
+  pl.nwork_sp = (int)pl.work_mol.size();
+  pl.work_mol.insert(pl.work_mol.end(), wd_mol.begin(), wd_mol.end());
+  pl.work_kind.insert(pl.work_kind.end(), wd_kind.begin(), wd_kind.end());
+  pl.work_start.insert(pl.work_start.end(), wd_start.begin(), wd_start.end());
   pl.tot_gam = gam;
   pl.tot_pair = pair;
   pl.tot_mix = mix;
@@ -1085,10 +1101,13 @@ int run_batch(tblite_amd_context ctx, Calculator& c, const double* xyz, double a
   hipLaunchKernelGGL(k_adjlist, dim3(pl.nmol), dim3(256), 0, st, b);
   HIP_TRY(ctx, hipGetLastError());
   HIP_TRY(ctx, hipEventRecord(ctx->ev[2], st));
-  const int nwork = (int)pl.work_mol.size();
-  if (nwork > 0)
-    hipLaunchKernelGGL(k_integrals, dim3(nwork), dim3(kIntegralChunk), 0, st, b, db.d_work_mol,
+  const int nwork = (int)pl.work_mol.size(), nsp = pl.nwork_sp;
+  if (nsp > 0)
+    hipLaunchKernelGGL(k_integrals<1>, dim3(nsp), dim3(kIntegralChunk), 0, st, b, db.d_work_mol,
                        db.d_work_kind, db.d_work_start);
+  if (nwork > nsp)
+    hipLaunchKernelGGL(k_integrals<2>, dim3(nwork - nsp), dim3(kIntegralChunk), 0, st, b,
+                       db.d_work_mol + nsp, db.d_work_kind + nsp, db.d_work_start + nsp);
   HIP_TRY(ctx, hipGetLastError());
   HIP_TRY(ctx, hipEventRecord(ctx->ev[3], st));
   // SCF buckets on the auxiliary streams, largest bucket first; on a

@@ -521,20 +521,33 @@ __device__ void multipole_block(const DevElem& ej, int jsh, const DevElem& ei, i
   }
 }
 
+// LMAX: the largest angular momentum the kernel instantiation handles (1:
+// s and p shells only, so the d-shell blocks and their registers are not
+// compiled in; 2: all)
+template <int LMAX>
 __device__ void shell_block(int li, int lj, const DevElem& ej, int jsh, const DevElem& ei, int ish,
                             double r2, const double vec[3], double intcut, double* s, double* d,
                             double* q)
 {
-  switch (li * 3 + lj) {
-    case 0: multipole_block<0, 0>(ej, jsh, ei, ish, r2, vec, intcut, s, d, q); break;
-    case 1: multipole_block<0, 1>(ej, jsh, ei, ish, r2, vec, intcut, s, d, q); break;
-    case 2: multipole_block<0, 2>(ej, jsh, ei, ish, r2, vec, intcut, s, d, q); break;
-    case 3: multipole_block<1, 0>(ej, jsh, ei, ish, r2, vec, intcut, s, d, q); break;
-    case 4: multipole_block<1, 1>(ej, jsh, ei, ish, r2, vec, intcut, s, d, q); break;
-    case 5: multipole_block<1, 2>(ej, jsh, ei, ish, r2, vec, intcut, s, d, q); break;
-    case 6: multipole_block<2, 0>(ej, jsh, ei, ish, r2, vec, intcut, s, d, q); break;
-    case 7: multipole_block<2, 1>(ej, jsh, ei, ish, r2, vec, intcut, s, d, q); break;
-    default: multipole_block<2, 2>(ej, jsh, ei, ish, r2, vec, intcut, s, d, q); break;
+  if constexpr (LMAX < 2) {
+    switch (li * 2 + lj) {
+      case 0: multipole_block<0, 0>(ej, jsh, ei, ish, r2, vec, intcut, s, d, q); break;
+      case 1: multipole_block<0, 1>(ej, jsh, ei, ish, r2, vec, intcut, s, d, q); break;
+      case 2: multipole_block<1, 0>(ej, jsh, ei, ish, r2, vec, intcut, s, d, q); break;
+      default: multipole_block<1, 1>(ej, jsh, ei, ish, r2, vec, intcut, s, d, q); break;
+    }
+  } else {
+    switch (li * 3 + lj) {
+      case 0: multipole_block<0, 0>(ej, jsh, ei, ish, r2, vec, intcut, s, d, q); break;
+      case 1: multipole_block<0, 1>(ej, jsh, ei, ish, r2, vec, intcut, s, d, q); break;
+      case 2: multipole_block<0, 2>(ej, jsh, ei, ish, r2, vec, intcut, s, d, q); break;
+      case 3: multipole_block<1, 0>(ej, jsh, ei, ish, r2, vec, intcut, s, d, q); break;
+      case 4: multipole_block<1, 1>(ej, jsh, ei, ish, r2, vec, intcut, s, d, q); break;
+      case 5: multipole_block<1, 2>(ej, jsh, ei, ish, r2, vec, intcut, s, d, q); break;
+      case 6: multipole_block<2, 0>(ej, jsh, ei, ish, r2, vec, intcut, s, d, q); break;
+      case 7: multipole_block<2, 1>(ej, jsh, ei, ish, r2, vec, intcut, s, d, q); break;
+      default: multipole_block<2, 2>(ej, jsh, ei, ish, r2, vec, intcut, s, d, q); break;
+    }
   }
 }
 
@@ -544,7 +557,9 @@ __device__ void shell_block(int li, int lj, const DevElem& ej, int jsh, const De
 
 // get_hamiltonian (src/h0.cu:306-477 text) plus the on-site blocks.  One
 // thread per atom pair (neighbour-list entry) or per atom (on-site); each
-// thread writes a disjoint set of matrix elements, so no atomics.
+// thread writes a disjoint set of matrix elements, so no atomics.  LMAX = 1
+// serves molecules without d shells (the host splits the work list by it).
+template <int LMAX>
 __global__ __launch_bounds__(128) void k_integrals(BatchDev b, const int* __restrict__ work_mol,
                                                  const int* __restrict__ work_kind,
                                                  const int* __restrict__ work_start)
@@ -621,14 +636,15 @@ __global__ __launch_bounds__(128) void k_integrals(BatchDev b, const int* __rest
   const double r2 = vec[0] * vec[0] + vec[1] * vec[1] + vec[2] * vec[2];
   const double rr = sqrt(sqrt(r2) / (ej.atomic_rad + ei.atomic_rad));
 
-  double st[25], dt[75], qt[150];
+  constexpr int NB = (2 * LMAX + 1) * (2 * LMAX + 1);  // largest shell block
+  double st[NB], dt[3 * NB], qt[6 * NB];
   for (int ish = 0; ish < ei.nsh; ++ish) {
     const int li = ei.ang[ish];
     const int ii = b.sh_iao[s0 + is + ish];
     for (int jsh = 0; jsh < ej.nsh; ++jsh) {
       const int lj = ej.ang[jsh];
       const int jj = b.sh_iao[s0 + js + jsh];
-      shell_block(li, lj, ej, jsh, ei, ish, r2, vec, intcut, st, dt, qt);
+      shell_block<LMAX>(li, lj, ej, jsh, ei, ish, r2, vec, intcut, st, dt, qt);
       const int si = 2 * li + 1, sj = 2 * lj + 1;
       if (iat != jat) {
         const double shp = (1.0 + ei.shpoly[ish] * rr) * (1.0 + ej.shpoly[jsh] * rr);
@@ -687,3 +703,6 @@ __global__ __launch_bounds__(128) void k_integrals(BatchDev b, const int* __rest
     }
   }
 }
+
+template __global__ void k_integrals<1>(BatchDev, const int*, const int*, const int*);
+template __global__ void k_integrals<2>(BatchDev, const int*, const int*, const int*);

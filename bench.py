@@ -35,8 +35,8 @@ PEAK_FP64 = 78.6e12  # MI355X dense FP64 (matrix and vector), FLOP/s
 # mean over the kernel's launches in one batch).  Summaries with the full
 # commands: profiles/k_scf_config1.md, profiles/k_scf_config3.md.
 MEASURED_TRAFFIC = {
-    ("config1", "k_scf<512, 2, 3, 6, true>"): 16.60e9 + 0.93e9,
-    ("config3", "k_scf<512, 1, 0, 0, false>"): 2977.3e9 + 669.7e9,
+    ("config1", "k_scf<512, 2, 3, 6, true>"): 16.59e9 + 0.93e9,
+    ("config3", "k_scf<512, 1, 0, 0, false>"): 2976.4e9 + 669.6e9,
 }
 
 WORKLOADS = {

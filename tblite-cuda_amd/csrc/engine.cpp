@@ -1086,17 +1086,28 @@ int run_batch(tblite_amd_context ctx, Calculator& c, const double* xyz, double a
   }
   hipLaunchKernelGGL(k_setup, dim3(pl.nmol), dim3(256), 0, st, b);
   HIP_TRY(ctx, hipGetLastError());
-  hipLaunchKernelGGL(k_setup_pairs, dim3((int)pl.pair_mol.size()), dim3(256), 0, st, b,
+  // The pair planes and the three-body dispersion need only k_setup's radii,
+  // CN and weights: on a multi-stream context they run on the first
+  // auxiliary stream beside the neighbour list and the integrals (whose
+  // kernels leave CUs room), and the SCF waits for both.
+  const bool one = ctx->single_stream;
+  hipStream_t ps = one ? st : ctx->aux[0];
+  if (!one) {
+    HIP_TRY(ctx, hipEventRecord(ctx->ev[5], st));
+    HIP_TRY(ctx, hipStreamWaitEvent(ps, ctx->ev[5], 0));
+  }
+  hipLaunchKernelGGL(k_setup_pairs, dim3((int)pl.pair_mol.size()), dim3(256), 0, ps, b,
                      db.d_pair_mol, db.d_pair_a0, db.d_pair_a1);
   HIP_TRY(ctx, hipGetLastError());
   // three-body dispersion (writes every atom's d4_e3 entry; needs the D4 CN
-  // and zero-charge C6 of k_setup)
+  // of k_setup and the zero-charge C6 planes of k_setup_pairs)
   const int natm = (int)pl.atm_mol.size();
   if (natm > 0) {
-    hipLaunchKernelGGL(k_d4_atm, dim3(natm), dim3(256), 0, st, b, db.d_atm_mol, db.d_atm_i0,
+    hipLaunchKernelGGL(k_d4_atm, dim3(natm), dim3(256), 0, ps, b, db.d_atm_mol, db.d_atm_i0,
                        db.d_atm_i1);
     HIP_TRY(ctx, hipGetLastError());
   }
+  if (!one) HIP_TRY(ctx, hipEventRecord(ctx->ev[6], ps));
   HIP_TRY(ctx, hipEventRecord(ctx->ev[1], st));
   hipLaunchKernelGGL(k_adjlist, dim3(pl.nmol), dim3(256), 0, st, b);
   HIP_TRY(ctx, hipGetLastError());
@@ -1112,11 +1123,15 @@ int run_batch(tblite_amd_context ctx, Calculator& c, const double* xyz, double a
   HIP_TRY(ctx, hipEventRecord(ctx->ev[3], st));
   // SCF buckets on the auxiliary streams, largest bucket first; on a
   // single-stream context one after the other, smallest first (they finish
-  // in milliseconds and report early)
-  const bool one = ctx->single_stream;
-  if (!one)
-    for (int s = 0; s < kNumStreams; ++s)
+  // in milliseconds and report early).  Every stream waits for the
+  // integrals and for the pair planes / three-body term.
+  if (!one) {
+    for (int s = 0; s < kNumStreams; ++s) {
       HIP_TRY(ctx, hipStreamWaitEvent(ctx->aux[s], ctx->ev[3], 0));
+      HIP_TRY(ctx, hipStreamWaitEvent(ctx->aux[s], ctx->ev[6], 0));
+    }
+    HIP_TRY(ctx, hipStreamWaitEvent(st, ctx->ev[6], 0));
+  }
   const int nbk = (int)pl.buckets.size();
   for (int q = 0; q < nbk; ++q) {
     const int k = one ? nbk - 1 - q : q;

@@ -1440,7 +1440,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
     double* part = work; // part[c * ld + i], c = 0 (PS), 1..3 (PD), 4..9 (PQ), 10 (PH)
     // four rows per wave (16-lane segments, lanes over j): four rows share
     // each memory round trip.  With the matrices in LDS (n <= 16 JS) every
-    // lane issues the loads of all its columns at once: indices past n are
+    // lane issues the loads of CH column steps at once: indices past n are
     // clamped to column n - 1 and weighted by zero, so no load waits on a
     // branch and the sums keep the order of the general loop.
     for (int i0 = 4 * wid; i0 < n; i0 += 4 * NW) {

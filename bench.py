@@ -4,14 +4,17 @@
 A step is one full singlepoint pass (CN, repulsion, Coulomb matrices,
 neighbour list, integrals/H0, SCF to convergence) over the whole batch, with
 the coordinates already resident in HBM; the batch is the workload's seeded
-molecules (default: 4096 (H2O)n clusters, n ~ U{2..16}).  Prints one JSON
-line.  Beside the GPU number it reports:
+molecules (default: config 3, 2048 seeded tree-grown C/N/O/S organics with
+4..60 heavy atoms, the workload the headline metric is quoted on).  Prints one
+JSON line.  Beside the GPU number it reports:
   roofline      FP64 work of the dominant SCF kernel (the kernel instantiation
                 carrying the most work) / its launches' mean duration from HIP
                 events recorded on their streams
-  cpu_baseline  the CPU oracle (FP64 restatement, OpenMP over molecules) on a
-                bounded sample of the same molecules, timed on this host
-  per_molecule  the plain one-molecule-per-call path on a sample
+  cpu_baseline  the CPU oracle (FP64 restatement) on bounded samples of the
+                same molecules, timed on this host: OpenMP over molecules on
+                all cores (value) and a per-molecule loop on one thread
+  per_molecule  the plain one-molecule-per-call path (tblite_amd_xtb_singlepoint)
+                on an evenly spaced sample, and the batched/per-molecule ratio
 """
 import argparse
 import json
@@ -61,15 +64,30 @@ def scf_flops(nao, nocc, iters):
     return iters * per_iter + n**3 / 3.0
 
 
-BUCKET_BOUNDS = [16, 32, 48, 64, 80, 96, 1 << 30]  # padded AO dimension, as planned by the engine
-
-
-def scf_kernel_name(ldmax, nthreads):
-    if ldmax > 96:
+def scf_kernel_name(path, nthreads):
+    """Kernel instantiation of an SCF bucket from its molecules' path (0: LDS
+    matrices, 1: global-memory matrices) and its workgroup size."""
+    if path == 1:
         return "k_scf<512, 1, 0, 0, false> (global-memory matrices, Householder eigensolver)"
     maxt = 2 if nthreads == 512 else 1
     maxb, maxc = (2, 4) if nthreads == 128 else (3, 6)
     return f"k_scf<{nthreads}, {maxt}, {maxb}, {maxc}, true> (LDS matrices, Jacobi)"
+
+
+def bucket_members(buckets, ld, path):
+    """Molecule indices of each SCF bucket.  The engine's buckets cover
+    disjoint ranges of the padded AO dimension and are listed largest first,
+    so bucket k holds the one-workgroup molecules (path 0 or 1) with
+    ldmax(k+1) < ld <= ldmax(k)."""
+    out = []
+    for k, bk in enumerate(buckets):
+        lo = buckets[k + 1]["ldmax"] if k + 1 < len(buckets) else 0
+        sel = np.nonzero((ld > lo) & (ld <= bk["ldmax"]) & (path != 2))[0]
+        if len(sel) != bk["nmol"]:
+            raise RuntimeError(f"bucket {k}: {len(sel)} molecules in ({lo}, {bk['ldmax']}] "
+                               f"but the engine launched {bk['nmol']}")
+        out.append(sel)
+    return out
 
 
 def copy_probe(mib, reps=3):
@@ -166,7 +184,7 @@ def serve_bench(args):
     k = min(args.per_molecule_sample, len(mols))
     t = time.perf_counter()
     for m in range(k):
-        eng.singlepoint(mols[m][0], mols[m][1])
+        eng.singlepoint(mols[m][0], mols[m][1], wavefunction=False)
     per_mol = k / (time.perf_counter() - t)
     best = sweep[-1]
     line = {
@@ -195,29 +213,51 @@ def serve_bench(args):
     print(json.dumps(line))
 
 
+def _spread(nmol, count):
+    """count molecule indices evenly spaced over the batch."""
+    return np.unique(np.linspace(0, nmol - 1, max(1, count)).round().astype(int))
+
+
 def cpu_baseline(mols, seconds, threads):
+    """The CPU oracle timed on this host on bounded, evenly spaced samples of
+    the workload: (ii) OpenMP over molecules on `threads` cores (the reported
+    value) and (i) a per-molecule loop on one thread.  Each sample is sized
+    from a short calibration run to about `seconds` of CPU work.  Returns the
+    indices and results of the larger sample (for the parity check) and the
+    baseline record."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
     orc = oracle.Oracle()
-    # calibrate on a small sample, then run a sample sized to ~`seconds`
-    probe = mols[: min(len(mols), 4 * threads)]
-    t = time.perf_counter()
-    orc.singlepoint_many(probe, nthreads=threads)
-    dt = max(time.perf_counter() - t, 1e-3)
-    count = int(min(len(mols), max(len(probe), len(probe) * seconds / dt)))
-    sample = mols[:count]
-    t = time.perf_counter()
-    e, it, st = orc.singlepoint_many(sample, nthreads=threads)
-    dt = time.perf_counter() - t
-    return sample, e, {
-        "value": round(count / dt, 3),
+
+    def timed(idx, nthreads):
+        t = time.perf_counter()
+        e, it, st = orc.singlepoint_many([mols[m] for m in idx], nthreads=nthreads)
+        return e, it, time.perf_counter() - t
+
+    def sized(nthreads, secs):
+        probe = _spread(len(mols), 2 * nthreads)
+        _, _, dt = timed(probe, nthreads)
+        count = int(min(len(mols), max(len(probe), len(probe) * secs / max(dt, 1e-3))))
+        idx = _spread(len(mols), count)
+        e, it, dt = timed(idx, nthreads)
+        return idx, e, it, dt
+
+    idx, e, it, dt = sized(threads, seconds)
+    idx1, _, _, dt1 = sized(1, seconds / 2)
+    rec = {
+        "value": round(len(idx) / dt, 3),
         "unit": "singlepoints/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"first {count} molecules of the same workload, CPU oracle (FP64 C restatement), "
-                  f"OpenMP over molecules, {dt:.1f} s",
+        "sample": f"{len(idx)} evenly spaced molecules of the same workload, CPU oracle (FP64 C "
+                  f"restatement of tblite's singlepoint, not tblite), OpenMP over molecules, "
+                  f"{dt:.1f} s",
+        "single_thread": {"value": round(len(idx1) / dt1, 3), "unit": "singlepoints/s", "cores": 1,
+                          "sample": f"{len(idx1)} evenly spaced molecules, per-molecule loop on "
+                                    f"one thread, {dt1:.1f} s"},
     }
+    return idx, e, it, rec
 
 
 def main():
@@ -225,7 +265,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="config1", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="config3", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--per-molecule-sample", type=int, default=64)
@@ -287,15 +327,15 @@ def main():
     nocc = np.array([int(np.count_nonzero(eng.charges(m)["focc"] > 1e-12)) for m in range(nmol)])
     fl = scf_flops(nao, nocc, it.astype(np.float64))
     ld = (nao + 3) & ~3
+    path = np.array([eng.molecule_path(m) for m in range(nmol)])
     buckets = eng.bucket_timings()
+    members = bucket_members(buckets, ld, path)
     per_bucket = []
-    for bk in buckets:
-        hi = bk["ldmax"]
-        lo = max([b for b in BUCKET_BOUNDS if b < hi] + [0]) if hi <= 96 else 96
-        sel = (ld > lo) & (ld <= hi)
+    for bk, sel in zip(buckets, members):
         f = float(fl[sel].sum())
-        per_bucket.append({"kernel": scf_kernel_name(hi, bk["nthreads"]), "ldmax": hi,
-                           "molecules": bk["nmol"], "ms": round(bk["ms"], 3), "flops": f,
+        per_bucket.append({"kernel": scf_kernel_name(int(path[sel[0]]), bk["nthreads"]),
+                           "ld_range": [int(ld[sel].min()), int(ld[sel].max())],
+                           "molecules": int(len(sel)), "ms": round(bk["ms"], 3), "flops": f,
                            "tflops": round(f / (bk["ms"] / 1e3) / 1e12, 4) if bk["ms"] > 0 else 0.0})
     # the dominant kernel is the instantiation that carries the most FP64 work
     # (buckets run concurrently on separate streams, so a small bucket's wall
@@ -307,7 +347,7 @@ def main():
         k = kern.setdefault(b["kernel"], {"ms": [], "flops": []})
         k["ms"].append(b["ms"])
         k["flops"].append(b["flops"])
-    big = ld > 1024  # molecules of the multi-workgroup path: a kernel sequence on the main stream
+    big = path == 2  # multi-workgroup molecules: a kernel sequence on the main stream
     if big.any():
         kern["multi-workgroup SCF (kernel sequence, main stream)"] = {
             "ms": [float(phase[:, 3].mean())], "flops": [float(fl[big].sum())]}
@@ -332,13 +372,17 @@ def main():
         "buckets": per_bucket,
     }
 
-    # plain one-molecule-per-call path on a sample
+    # plain one-molecule-per-call path (tblite_amd_xtb_singlepoint) on an
+    # evenly spaced sample of the batch
     k = min(args.per_molecule_sample, nmol)
+    pick = _spread(nmol, k) if k > 0 else []
+    k = len(pick)
     t = time.perf_counter()
-    for m in range(k):
-        eng.singlepoint(mols[m][0], mols[m][1])
+    for m in pick:
+        r = eng.singlepoint(mols[m][0], mols[m][1], wavefunction=False)
+        if r["rc"] != 0:
+            raise RuntimeError(f"per-molecule path: molecule {m} returned {r['rc']}")
     per_mol = k / (time.perf_counter() - t) if k > 0 else float("nan")
-
     extra = {
         "prepare_s": round(t_prep, 3),
         "step_ms_p50": round(float(np.percentile(step_ms, 50)), 3),
@@ -359,9 +403,11 @@ def main():
         args.no_cpu = True
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
-        sample, e_cpu, cpu = cpu_baseline(mols, args.cpu_seconds, threads)
-        d = np.abs(e[: len(sample)] - e_cpu)
-        extra["parity_sample"] = {"molecules": len(sample), "max_abs_dE": float(d.max())}
+        idx, e_cpu, it_cpu, cpu = cpu_baseline(mols, args.cpu_seconds, threads)
+        extra["parity_sample"] = {
+            "molecules": int(len(idx)),
+            "max_abs_dE": float(np.abs(e[idx] - e_cpu).max()),
+            "max_abs_diter": int(np.abs(it[idx].astype(int) - np.asarray(it_cpu, dtype=int)).max())}
         cpu["cpu_model"] = _cpu_model()
 
     line = {

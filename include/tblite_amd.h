@@ -51,6 +51,7 @@ extern "C" {
 #define TBLITE_AMD_STATUS_NAN 8
 #define TBLITE_AMD_STATUS_MIXER 16
 #define TBLITE_AMD_STATUS_TOO_LARGE 32
+#define TBLITE_AMD_STATUS_FAILED 64     /* not run: its batch could not be launched (server) */
 
 /* ------------------------------------------------------- parameter data */
 #define TBLITE_AMD_MAX_SHELL 3
@@ -167,7 +168,10 @@ int tblite_amd_get_error(tblite_amd_context ctx, char* buffer, int buffer_size);
 tblite_amd_structure tblite_amd_new_structure(tblite_amd_context ctx, int nat, const int* num,
                                               const double* xyz, double charge, int uhf);
 void tblite_amd_delete_structure(tblite_amd_structure* mol);
-int tblite_amd_get_structure_dims(tblite_amd_structure mol, int* nat, int* nsh, int* nao);
+/* Atom, shell and orbital counts of a structure under calc's parameter set
+ * (calc may be NULL: the built-in GFN2 data). */
+int tblite_amd_get_structure_dims(tblite_amd_structure mol, tblite_amd_calculator calc, int* nat,
+                                  int* nsh, int* nao);
 
 /* Calculator: replaces xtb_calculator (src/xtb.h:105-126) together with its
  * never-defined constructor new_hamiltonian (src/h0.h:80-86).  param may be
@@ -267,6 +271,18 @@ int tblite_amd_server_poll(tblite_amd_server srv, int capacity, long long* ticke
                            double* energies, int* iterations, int* status, double* latency_ms);
 int tblite_amd_server_stats(tblite_amd_server srv, long long* submitted, long long* completed,
                             int* queued, int* busy_slots, long long* batches);
+
+/* Workspace accounting of the serving admission, on the host (no device):
+ * need_bytes receives the device workspace the engine allocates for one batch
+ * of these nmol neutral molecules, admit_bytes the footprint the server's
+ * admission counts for the same requests.  A slot admits a batch while
+ * admit_bytes fits its share of the workspace; admit_bytes >= need_bytes
+ * keeps the engine from growing (freeing and reallocating) a slot's reserved
+ * workspace.  grid_min_ld as in tblite_amd_tuning; param may be NULL.  Engine
+ * instrumentation; no counterpart in the reference. */
+int tblite_amd_workspace_estimate(const tblite_amd_param* param, int nmol, const int* mol_nat,
+                                  const int* num, int grid_min_ld, double* need_bytes,
+                                  double* admit_bytes);
 
 /* ------------------------------------------------- stage-level access
  * Intermediate results of the last flat/batched call, for stage-by-stage

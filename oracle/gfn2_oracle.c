@@ -1778,6 +1778,8 @@ int or_singlepoint(const or_calc* calc, int nat, const int* num, const double* x
     if (res->qpat) memcpy(res->qpat, qpat, sizeof(double) * 6 * nat);
     if (res->emo) memcpy(res->emo, emo, sizeof(double) * nao);
     if (res->focc) memcpy(res->focc, focc, sizeof(double) * nao);
+    if (res->density) memcpy(res->density, P, sizeof(double) * n2);
+    if (res->coeff) memcpy(res->coeff, C, sizeof(double) * n2);
   }
   broyden_free(&mix);
   free(erep); free(cn); free(se); free(S); free(D); free(Q); free(H); free(F); free(C); free(P);

@@ -40,6 +40,8 @@ typedef struct {
   double* qpat;   /* 6*nat */
   double* emo;    /* nao */
   double* focc;   /* nao */
+  double* density; /* nao*nao, column-major: P of the last SCF iteration */
+  double* coeff;   /* nao*nao, column-major: coeff(:, k) = level k (ascending) */
 } or_result;
 
 /* Full singlepoint (xtb_singlepoint, src/xtb.cu:188-434). kt in Eh. */

@@ -33,6 +33,7 @@ class Result(C.Structure):
         ("qsh", C.POINTER(C.c_double)), ("qat", C.POINTER(C.c_double)),
         ("dpat", C.POINTER(C.c_double)), ("qpat", C.POINTER(C.c_double)),
         ("emo", C.POINTER(C.c_double)), ("focc", C.POINTER(C.c_double)),
+        ("density", C.POINTER(C.c_double)), ("coeff", C.POINTER(C.c_double)),
     ]
 
 
@@ -126,11 +127,17 @@ class Oracle:
         return len(num), nsh.value, nao.value
 
     def singlepoint(self, num, xyz, charge=0.0, uhf=0, accuracy=1.0, temperature=300.0,
-                    max_iter=250, damping=0.4):
+                    max_iter=250, damping=0.4, wavefunction=False):
+        """wavefunction=True also returns density and coeff (nao x nao numpy
+        arrays indexed [row, column]; coeff[:, k] is level k)."""
         num, xyz = self._mol(num, xyz)
         nat, nsh, nao = self.dims(num)
         out = dict(qsh=np.zeros(nsh), qat=np.zeros(nat), dpat=np.zeros((nat, 3)),
                    qpat=np.zeros((nat, 6)), emo=np.zeros(nao), focc=np.zeros(nao))
+        if wavefunction:
+            # column-major in C: allocated as the transpose, read back as [row, col]
+            out["density"] = np.zeros((nao, nao))
+            out["coeff"] = np.zeros((nao, nao))
         r = Result()
         for k in out:
             setattr(r, k, _dp(out[k]))
@@ -139,6 +146,9 @@ class Oracle:
         out.update(energy=r.energy, erep=r.erep, eelec=r.eelec, edisp2=r.edisp2,
                    edisp3=r.edisp3, iterations=r.iterations,
                    converged=bool(r.converged), status=r.status, rc=rc)
+        if wavefunction:
+            out["density"] = out["density"].T.copy()
+            out["coeff"] = out["coeff"].T.copy()
         return out
 
     def singlepoint_many(self, mols, accuracy=1.0, temperature=300.0, max_iter=250,

@@ -148,7 +148,12 @@ struct BatchDev {
   double* qpat;                   // 6 per atom
   double* emo;                    // per AO (sorted)
   double* focc;                   // per AO
-  double* coeff;                  // optional n x n per molecule (may be null)
+  // optional (both null unless a wavefunction is requested): per molecule at
+  // mol_mat_off, in tblite's column-major layout, the MO coefficients
+  // coeff(nao, nao) (column r = level r, ascending) and the density matrix
+  // density(nao, nao) of the last SCF iteration
+  double* coeff;
+  double* density;
 
   // Broyden mixer state
   double* mix_qin;
@@ -259,9 +264,18 @@ __host__ __device__ inline int big_npart(int ld) { return ld <= 768 ? TA_BIG_NT 
 // scratch per molecule of the global-memory path: inverse-iteration factors
 // (3 ld x NT), back-transformation products (NB x ld), tridiagonalisation
 // panel vectors (2 NP x ld)
-__host__ __device__ inline int64_t big_scratch_doubles(int ld)
+// Offset, in a global-memory-path molecule's scratch, of the saved
+// tridiagonal form of its last eigensolve (d, e, tau: 3 ld doubles, then |T|):
+// the LDS copy is overwritten by the Mulliken phase, and the full eigenvector
+// set of a requested wavefunction is formed from it after the SCF.
+__host__ __device__ inline int64_t big_tri_save_offset(int ld)
 {
   return 3 * (int64_t)ld * TA_BIG_NT + (int64_t)(TA_BIG_NB + 2 * TA_BIG_NP) * ld;
+}
+
+__host__ __device__ inline int64_t big_scratch_doubles(int ld)
+{
+  return big_tri_save_offset(ld) + 3 * (int64_t)ld + 8;
 }
 
 // LDS layout of the SCF kernel, in doubles (integer arrays in units of ints
@@ -328,3 +342,4 @@ __host__ __device__ inline ScfLds scf_lds_layout(int ld, int natmax, int nshmax,
 #define TA_STATUS_NAN 8
 #define TA_STATUS_MIXER 16
 #define TA_STATUS_TOO_LARGE 32
+#define TA_STATUS_FAILED 64

@@ -337,14 +337,13 @@ struct tblite_amd_context_s {
   double jacobi_skip = 1.0e-8;
   // > 0: the Jacobi threshold follows the SCF charge residual,
   // max(jacobi_tol, min(1e-2, jacobi_adapt * residual of the last iteration))
-  // (TBLITE_AMD_JACOBI_ADAPT; 0 keeps jacobi_tol throughout).  The vectors
-  // are carried to the next iteration, which refines them.  With 0.001 the
-  // threshold is loose only in the first two iterations (1e-3, then about
-  // 1e-5); water-cluster cations that need 29-49 iterations stay within
-  // 1e-6 e of the oracle's charges.  Larger factors move such slowly
-  // converging SCFs by up to 1e-4 e (0.01: 1.4e-4, 0.1: 1.4e-4): the early
-  // perturbation is carried by the Broyden history to where the SCF stops.
-  double jacobi_adapt = 0.001;
+  // (TBLITE_AMD_JACOBI_ADAPT; 0, the default, keeps jacobi_tol throughout).
+  // The loose early thresholds this allows perturb the SCF path: a (H2O)5
+  // cation (uhf 1) took 34 iterations with factor 0.001 against the CPU
+  // oracle's 32 (32 with 0, energy within 1e-11 Eh), beyond the +-1 parity
+  // bound, so the faster schedule (about 7 % on 4096 small water clusters) is
+  // off by default.
+  double jacobi_adapt = 0.0;
   // minimum LDS request of the global-memory SCF kernel (bytes); above half a
   // CU's LDS it runs one molecule per CU (TBLITE_AMD_BIG_LDS_MIN_KB)
   size_t big_lds_min = 0;
@@ -716,13 +715,10 @@ void host_pack(std::vector<char>& buf, std::vector<std::pair<size_t, size_t>>& s
   segs.push_back({off, v.size() * sizeof(T)});
 }
 
-// Allocate (or reuse) the workspace for a plan and upload the plan arrays.
-int upload_plan(tblite_amd_context ctx, Calculator& c, const Plan& pl, DeviceBatch& db)
+// Packed host image of a plan's integer / offset arrays: segment k of seg
+// holds array k below (fixed ones first, then one per bucket).
+void pack_plan(const Plan& pl, std::vector<char>& img, std::vector<std::pair<size_t, size_t>>& seg)
 {
-  const int nmol = pl.nmol, mem = pl.mem;
-  // packed host image of all integer / offset arrays
-  std::vector<char> img;
-  std::vector<std::pair<size_t, size_t>> seg;
   host_pack(img, seg, pl.mol_atom_off);   // 0
   host_pack(img, seg, pl.mol_nat);        // 1
   host_pack(img, seg, pl.mol_sh_off);     // 2
@@ -758,17 +754,24 @@ int upload_plan(tblite_amd_context ctx, Calculator& c, const Plan& pl, DeviceBat
   host_pack(img, seg, pl.pair_mol);       // 32
   host_pack(img, seg, pl.pair_a0);        // 33
   host_pack(img, seg, pl.pair_a1);        // 34
-  const size_t nfixed = seg.size();
   for (const auto& bk : pl.buckets) host_pack(img, seg, bk.mols);
 
+}
+
+// Device workspace of a plan in bytes (every array upload_plan carves, each
+// rounded to 256 bytes).  img_bytes: size of the packed plan image.
+size_t workspace_need(const Plan& pl, size_t img_bytes, bool wfn_arrays)
+{
+  const int nmol = pl.nmol, mem = pl.mem;
   const int64_t A = pl.tot_atoms, Sh = pl.tot_sh, AO = pl.tot_ao;
   auto dbytes = [](int64_t n) { return ((size_t)n * 8 + 255) & ~size_t(255); };
   auto ibytes = [](int64_t n) { return ((size_t)n * 4 + 255) & ~size_t(255); };
-  size_t need = ((img.size() + 255) & ~size_t(255));
+  size_t need = ((img_bytes + 255) & ~size_t(255));
   need += dbytes(3 * A) + 3 * dbytes(A);                 // xyz, cn, mrad, erep
   need += 2 * dbytes(Sh);                                // selfenergy, n0sh
   need += 3 * ibytes(A) + ibytes(pl.tot_nl) + ibytes(nmol); // inl, nnl, jat, nnz (+pad)
   need += 11 * dbytes(pl.tot_mat);                       // S, H, D, Q
+  if (wfn_arrays) need += 2 * dbytes(pl.tot_mat);        // coeff, density
   // gam; sd, dd, sq planes only for the multi-workgroup SCF (the
   // one-workgroup kernels evaluate the AES2 pair kernels where they use them)
   const bool aes_planes = !pl.grid_mols.empty();
@@ -783,6 +786,24 @@ int upload_plan(tblite_amd_context ctx, Calculator& c, const Plan& pl, DeviceBat
   need += dbytes((int64_t)nmol * TA_PROF_SLOTS);
   need += 2 * dbytes(A) + 2 * dbytes(TA_D4_MAXREF * A) + 2 * dbytes(pl.tot_pair) + 2 * dbytes(nmol); // D4
   need += 64 * 256;
+
+  return need;
+}
+
+// Allocate (or reuse) the workspace for a plan and upload the plan arrays.
+// wfn_arrays: also the per-molecule coefficient and density matrices of a
+// requested wavefunction (2 n^2 doubles per molecule).
+int upload_plan(tblite_amd_context ctx, Calculator& c, const Plan& pl, DeviceBatch& db,
+                bool wfn_arrays = false)
+{
+  const int nmol = pl.nmol, mem = pl.mem;
+  std::vector<char> img;
+  std::vector<std::pair<size_t, size_t>> seg;
+  pack_plan(pl, img, seg);
+  const size_t nfixed = seg.size() - pl.buckets.size();  // fixed arrays before the bucket lists
+  const int64_t A = pl.tot_atoms, Sh = pl.tot_sh, AO = pl.tot_ao;
+  const size_t need = workspace_need(pl, img.size(), wfn_arrays);
+  const bool aes_planes = !pl.grid_mols.empty();
 
   Arena& ar = ctx->arena;
   if (ar.cap < need) {
@@ -874,7 +895,8 @@ int upload_plan(tblite_amd_context ctx, Calculator& c, const Plan& pl, DeviceBat
   b.qpat = carve<double>(ar, 6 * A);
   b.emo = carve<double>(ar, AO);
   b.focc = carve<double>(ar, AO);
-  b.coeff = nullptr;
+  b.coeff = wfn_arrays ? carve<double>(ar, pl.tot_mat) : nullptr;
+  b.density = wfn_arrays ? carve<double>(ar, pl.tot_mat) : nullptr;
   b.mix_qin = carve<double>(ar, pl.tot_mix);
   b.mix_dq = carve<double>(ar, pl.tot_mix);
   b.mix_dqlast = carve<double>(ar, pl.tot_mix);
@@ -1047,7 +1069,8 @@ int run_grid_molecule(tblite_amd_context ctx, Calculator& c, int m)
   GridHostStats hs;
   int rc = grid_scf(g, ctx->stream, c.max_iter, b.energy + m, b.eelec + m, b.erep_tot + m,
                     b.iterations + m, b.status + m, b.emo + o0, b.focc + o0,
-                    b.coeff ? b.coeff + moff : nullptr, &hs, ctx->verbose,
+                    b.coeff ? b.coeff + moff : nullptr, b.density ? b.density + moff : nullptr,
+                    &hs, ctx->verbose,
                     ctx->grid_graph ? 1 : 0);
   if (rc) {
     set_error(ctx, "multi-workgroup SCF of molecule %d failed: %s", m,
@@ -1201,7 +1224,8 @@ int copy_results(tblite_amd_context ctx, int nmol, double* energies, int* iterat
 
 int flat_singlepoint(tblite_amd_context ctx, Calculator& c, int nmol, const int* mol_nat,
                      const int* num, const double* xyz, const double* charge, const int* uhf,
-                     double acc, double* energies, int* iterations, int* status)
+                     double acc, double* energies, int* iterations, int* status,
+                     bool wfn_arrays = false)
 {
   if (!ctx) return TBLITE_AMD_ERR_ARGUMENT;
   if (nmol < 1 || !mol_nat || !num || !xyz || !energies) {
@@ -1215,7 +1239,7 @@ int flat_singlepoint(tblite_amd_context ctx, Calculator& c, int nmol, const int*
   ctx->prepared = nullptr;
   rc = make_plan(ctx, c, nmol, mol_nat, num, charge, uhf, acc, ctx->plan);
   if (rc) return rc;
-  rc = upload_plan(ctx, c, ctx->plan, ctx->batch);
+  rc = upload_plan(ctx, c, ctx->plan, ctx->batch, wfn_arrays);
   if (rc) return rc;
   rc = run_batch(ctx, c, xyz, acc);
   if (rc) return rc;
@@ -1388,19 +1412,27 @@ void tblite_amd_delete_structure(tblite_amd_structure* mol)
   *mol = nullptr;
 }
 
-int tblite_amd_get_structure_dims(tblite_amd_structure mol, int* nat, int* nsh, int* nao)
+int tblite_amd_get_structure_dims(tblite_amd_structure mol, tblite_amd_calculator calc, int* nat,
+                                  int* nsh, int* nao)
 {
   if (!mol) return TBLITE_AMD_ERR_ARGUMENT;
-  tblite_amd_param p;
-  tblite_amd_fill_gfn2_default(&p);
+  // the calculator's element table (shell layout per element), else the
+  // built-in GFN2 data
+  tblite_amd_param def;
+  const tblite_amd_param* p = &def;
+  if (calc) p = &calc->c.param;
+  else tblite_amd_fill_gfn2_default(&def);
   int ns = 0, na = 0;
   for (int i = 0; i < mol->nat; ++i) {
     int e = -1;
-    for (int k = 0; k < p.nelem; ++k)
-      if (p.elem[k].z == mol->num[i]) e = k;
-    if (e < 0) return TBLITE_AMD_ERR_ARGUMENT;
-    ns += p.elem[e].nsh;
-    for (int s = 0; s < p.elem[e].nsh; ++s) na += 2 * p.elem[e].ang[s] + 1;
+    for (int k = 0; k < p->nelem; ++k)
+      if (p->elem[k].z == mol->num[i]) e = k;
+    if (e < 0) {
+      set_error(mol->ctx, "atom %d: element Z=%d has no parameters", i, mol->num[i]);
+      return TBLITE_AMD_ERR_ARGUMENT;
+    }
+    ns += p->elem[e].nsh;
+    for (int s = 0; s < p->elem[e].nsh; ++s) na += 2 * p->elem[e].ang[s] + 1;
   }
   if (nat) *nat = mol->nat;
   if (nsh) *nsh = ns;
@@ -1493,7 +1525,15 @@ int tblite_amd_get_wavefunction_array(tblite_amd_wavefunction wfn, const char* n
   else if (n == "n0at") v = &wfn->n0at;
   else if (n == "coeff") v = &wfn->coeff;
   else if (n == "density") v = &wfn->density;
-  else return TBLITE_AMD_ERR_ARGUMENT;
+  else {
+    set_error(wfn->ctx, "unknown wavefunction array '%s'", name);
+    return TBLITE_AMD_ERR_ARGUMENT;
+  }
+  if (v->empty()) {
+    set_error(wfn->ctx, "wavefunction array '%s' is empty: no singlepoint has filled this "
+              "wavefunction", name);
+    return TBLITE_AMD_ERR_ARGUMENT;
+  }
   *count = (int)v->size();
   if (out && !v->empty()) std::memcpy(out, v->data(), sizeof(double) * v->size());
   return TBLITE_AMD_OK;
@@ -1525,6 +1565,18 @@ static int fill_wavefunction(tblite_amd_context ctx, int imol, tblite_amd_wavefu
   HIP_TRY(ctx, hipMemcpy(w->focc.data(), b.focc + o0, 8 * nao, hipMemcpyDeviceToHost));
   HIP_TRY(ctx, hipMemcpy(w->n0sh.data(), b.n0sh + s0, 8 * nsh, hipMemcpyDeviceToHost));
   for (int s = 0; s < nsh; ++s) w->n0at[pl.sh_at[s0 + s]] += w->n0sh[s];
+  const int64_t moff = pl.mol_mat_off[imol];
+  const size_t n2 = (size_t)nao * nao;
+  w->coeff.clear();
+  w->density.clear();
+  if (b.coeff) {
+    w->coeff.resize(n2);
+    HIP_TRY(ctx, hipMemcpy(w->coeff.data(), b.coeff + moff, 8 * n2, hipMemcpyDeviceToHost));
+  }
+  if (b.density) {
+    w->density.resize(n2);
+    HIP_TRY(ctx, hipMemcpy(w->density.data(), b.density + moff, 8 * n2, hipMemcpyDeviceToHost));
+  }
   return TBLITE_AMD_OK;
 }
 
@@ -1536,7 +1588,7 @@ int tblite_amd_xtb_singlepoint(tblite_amd_context ctx, tblite_amd_structure mol,
   const double acc = accuracy > 0.0 ? accuracy : calc->c.accuracy;
   int st = 0, it = 0;
   int rc = flat_singlepoint(ctx, calc->c, 1, &mol->nat, mol->num.data(), mol->xyz.data(),
-                            &mol->charge, &mol->uhf, acc, energy, &it, &st);
+                            &mol->charge, &mol->uhf, acc, energy, &it, &st, wfn != nullptr);
   if (iterations) *iterations = it;
   if (ctx->have_batch && wfn) {
     int rc2 = fill_wavefunction(ctx, 0, wfn);
@@ -1574,8 +1626,10 @@ int tblite_amd_xtb_singlepoint_batch(tblite_amd_context ctx, int nmol,
     uhf[m] = mols[m]->uhf;
   }
   const double acc = accuracy > 0.0 ? accuracy : calc->c.accuracy;
+  bool any_wfn = false;
+  for (int m = 0; wfns && m < nmol; ++m) any_wfn = any_wfn || wfns[m] != nullptr;
   int rc = flat_singlepoint(ctx, calc->c, nmol, mol_nat.data(), num.data(), xyz.data(),
-                            charge.data(), uhf.data(), acc, energies, iterations, status);
+                            charge.data(), uhf.data(), acc, energies, iterations, status, any_wfn);
   if (ctx->have_batch && wfns)
     for (int m = 0; m < nmol; ++m)
       if (wfns[m]) {
@@ -1892,7 +1946,10 @@ struct ServeRequest {
   int uhf;
   double t_submit;
   size_t bytes;
-  bool big;  // above the LDS-resident size
+  size_t plane_bytes;  // AES2 pair planes, allocated for the whole batch once it has a
+                       // multi-workgroup molecule
+  bool big;   // above the LDS-resident size
+  bool grid;  // multi-workgroup path
 };
 
 struct ServeResult {
@@ -1921,8 +1978,13 @@ double host_ms()
 }
 
 // Upper estimate of the device workspace a molecule adds to a batch (all
-// per-molecule arrays of upload_plan, alignment slack included).
-size_t request_bytes(const Calculator& c, int nat, const int* num, int* nao_out)
+// per-molecule arrays of upload_plan, alignment slack included).  grid_min_ld:
+// padded AO dimension above which the molecule takes the multi-workgroup path
+// (its scratch differs); *plane_bytes receives the molecule's share of the
+// AES2 pair planes that a batch holding a multi-workgroup molecule allocates
+// for all its molecules.
+size_t request_bytes(const Calculator& c, int nat, const int* num, int grid_min_ld, int* nao_out,
+                     size_t* plane_bytes, bool* grid)
 {
   int nsh = 0, nao = 0;
   for (int i = 0; i < nat; ++i) {
@@ -1936,12 +1998,32 @@ size_t request_bytes(const Calculator& c, int nat, const int* num, int* nao_out)
   const int64_t mem = std::max(1, c.max_iter);
   const int64_t ndim = nsh + 9 * (int64_t)nat;
   const int64_t ld = (nao + 3) & ~3;
+  const bool on_grid = ld > grid_min_ld;
   int64_t d = 11 * (int64_t)nao * nao + (int64_t)nsh * nsh + 20 * (int64_t)nat * nat;
   d += (int64_t)nat * (nat + 1) / 2 + (24 + 2 + 2 * TA_D4_MAXREF) * (int64_t)nat + 6 * nsh + 4 * nao + 2;
   d += 4 * ndim + 2 * ndim * mem + mem * mem + mem * (mem + 1) + 2 * mem + 8 + TA_PROF_SLOTS;
-  if (ld > kLdsMaxLd) d += 4 * ld * ld + big_scratch_doubles((int)ld);
+  if (on_grid)
+    d += 4 * ld * ld + grid_scratch_doubles(nao, (int)ld, nat, nsh);
+  else if (ld > kLdsMaxLd)
+    d += 4 * ld * ld + big_scratch_doubles((int)ld);
   if (nao_out) *nao_out = nao;
+  if (plane_bytes) *plane_bytes = (size_t)18 * 8 * nat * nat + 3 * 256;
+  if (grid) *grid = on_grid;
   return (size_t)d * 8 + 64 * 256;
+}
+
+// Workspace of a batch of requests: their own arrays, the AES2 pair planes of
+// all of them once one takes the multi-workgroup path, and a fixed reserve.
+size_t batch_bytes(const std::vector<ServeRequest>& reqs)
+{
+  size_t used = 64 * 1024 * 8, planes = 0;
+  bool grid = false;
+  for (const ServeRequest& r : reqs) {
+    used += r.bytes;
+    planes += r.plane_bytes;
+    grid = grid || r.grid;
+  }
+  return used + (grid ? planes : 0);
 }
 
 }  // namespace
@@ -2026,12 +2108,15 @@ int tblite_amd_server_submit(tblite_amd_server srv, int nat, const int* num, con
     return TBLITE_AMD_ERR_ARGUMENT;
   }
   int nao = 0;
-  const size_t bytes = request_bytes(srv->calc->c, nat, num, &nao);
+  size_t plane_bytes = 0;
+  bool grid = false;
+  const size_t bytes = request_bytes(srv->calc->c, nat, num, srv->slots[0].ctx->grid_min_ld, &nao,
+                                     &plane_bytes, &grid);
   if (bytes == 0) {
     set_error(ctx, "server_submit: element without parameters");
     return TBLITE_AMD_ERR_ARGUMENT;
   }
-  if (bytes + 64 * 1024 > srv->slot_bytes) {
+  if (bytes + 64 * 1024 * 8 + (grid ? plane_bytes : 0) > srv->slot_bytes) {
     set_error(ctx, "server_submit: molecule needs %zu bytes, a slot holds %zu", bytes,
               srv->slot_bytes);
     return TBLITE_AMD_ERR_TOO_LARGE;
@@ -2045,6 +2130,8 @@ int tblite_amd_server_submit(tblite_amd_server srv, int nat, const int* num, con
   r.uhf = uhf;
   r.t_submit = host_ms();
   r.bytes = bytes;
+  r.plane_bytes = plane_bytes;
+  r.grid = grid;
   r.big = ((nao + 3) & ~3) > kLdsMaxLd;
   if (ticket) *ticket = r.ticket;
   (r.big ? srv->queue_big : srv->queue_small).push_back(std::move(r));
@@ -2121,15 +2208,19 @@ int serve_collect(tblite_amd_server srv, ServeSlot& sl)
 int serve_launch(tblite_amd_server srv, ServeSlot& sl)
 {
   // a small-class slot takes small molecules; the others take large ones,
-  // then fill up with small ones (those run first and report early)
-  size_t used = 64 * 1024 * 8;
+  // then fill up with small ones (those run first and report early).  The
+  // footprint counts the requests already in the slot (none unless a caller
+  // changes that) and the pair planes a multi-workgroup molecule brings.
   for (int pass = sl.small_class ? 1 : 0; pass < 2; ++pass) {
     std::deque<ServeRequest>& q = pass == 0 ? srv->queue_big : srv->queue_small;
-    while (!q.empty() && (int)sl.reqs.size() < srv->max_batch &&
-           used + q.front().bytes <= srv->slot_bytes) {
-      used += q.front().bytes;
+    while (!q.empty() && (int)sl.reqs.size() < srv->max_batch) {
       sl.reqs.push_back(std::move(q.front()));
       q.pop_front();
+      if (batch_bytes(sl.reqs) > srv->slot_bytes && sl.reqs.size() > 1) {
+        q.push_front(std::move(sl.reqs.back()));
+        sl.reqs.pop_back();
+        break;
+      }
     }
   }
   if (sl.reqs.empty()) return TBLITE_AMD_OK;
@@ -2160,7 +2251,22 @@ int serve_launch(tblite_amd_server srv, ServeSlot& sl)
     rc = run_batch(sc, c, xyz.data(), c.accuracy, false);
   }
   if (rc) {
+    // the batch's requests end here, each reported with TA_STATUS_FAILED, so
+    // no ticket is lost and the next poll starts from an empty slot
     set_error(srv->owner, "server: batch launch failed: %s", sc->error.c_str());
+    (void)hipStreamSynchronize(sc->stream);
+    const double now = host_ms();
+    for (const ServeRequest& r : sl.reqs) {
+      ServeResult res;
+      res.ticket = r.ticket;
+      res.energy = 0.0;
+      res.iterations = 0;
+      res.status = TA_STATUS_FAILED;
+      res.latency_ms = now - r.t_submit;
+      srv->done.push_back(res);
+      ++srv->completed;
+    }
+    sl.reqs.clear();
     return rc;
   }
   sl.busy = true;
@@ -2206,6 +2312,38 @@ int tblite_amd_server_poll(tblite_amd_server srv, int capacity, long long* ticke
     ++k;
   }
   return k;
+}
+
+int tblite_amd_workspace_estimate(const tblite_amd_param* param, int nmol, const int* mol_nat,
+                                  const int* num, int grid_min_ld, double* need_bytes,
+                                  double* admit_bytes)
+{
+  if (nmol < 1 || !mol_nat || !num) return TBLITE_AMD_ERR_ARGUMENT;
+  Calculator c;
+  if (param) c.param = *param;
+  else tblite_amd_fill_gfn2_default(&c.param);
+  std::string err;
+  if (!build_calculator(c, err)) return TBLITE_AMD_ERR_ARGUMENT;
+  // a host-only context: the planner reads only its routing settings
+  tblite_amd_context_s cx;
+  cx.grid_min_ld = std::max(16, std::min(TA_BIG_MAXLD, grid_min_ld));
+  cx.auto_grid = false;
+  Plan pl;
+  int rc = make_plan(&cx, c, nmol, mol_nat, num, nullptr, nullptr, c.accuracy, pl);
+  if (rc) return rc;
+  std::vector<char> img;
+  std::vector<std::pair<size_t, size_t>> seg;
+  pack_plan(pl, img, seg);
+  std::vector<ServeRequest> reqs(nmol);
+  int64_t a0 = 0;
+  for (int m = 0; m < nmol; ++m) {
+    ServeRequest& r = reqs[m];
+    r.bytes = request_bytes(c, mol_nat[m], num + a0, cx.grid_min_ld, nullptr, &r.plane_bytes, &r.grid);
+    a0 += mol_nat[m];
+  }
+  if (need_bytes) *need_bytes = (double)workspace_need(pl, img.size(), false);
+  if (admit_bytes) *admit_bytes = (double)batch_bytes(reqs);
+  return TBLITE_AMD_OK;
 }
 
 int tblite_amd_server_stats(tblite_amd_server srv, long long* submitted, long long* completed,

@@ -1017,6 +1017,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
   bool converged = false;
   int last_nocc = 0;
   double perr_last = 1.0;  // charge residual of the previous iteration
+  // levels, occupations and their order (emo_s, focc_u, rank, inv) hold the
+  // values of a completed eigensolve (false if the first one failed)
+  bool have_levels = false;
 
   while (!status && !converged && iter < b.max_iter) {
     elast = esum;
@@ -1383,6 +1386,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
       // eigenvectors follow once the occupations are known
       big_eigenvalues<NT>(A, C, Y, scr, n, ld, big_npart(ldmax), smem + L.eig, ev, red, scal,
                           emo_s, red + 24, clk);
+      if (b.coeff != nullptr) {
+        // keep d, e, tau and |T| for the eigenvectors of a requested wavefunction
+        double* sv = scr + big_tri_save_offset(ld);
+        const double* er = smem + L.eig;
+        for (int k = tid; k < 3 * ld; k += NT) sv[k] = er[k];
+        if (tid == 0) sv[3 * ld] = scal[0];
+      }
       for (int k = tid; k < n; k += NT) {
         rank[k] = k;
         inv[k] = k;
@@ -1421,6 +1431,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
     __syncthreads();
     ts = scal[3];
     last_nocc = (int)scal[4];
+    have_levels = true;
     clk.lap(TA_PROF_OCC);
     if constexpr (!MATS_LDS) {
       int nvec = 0;
@@ -1584,15 +1595,40 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
 
   // ------------------------------------------------ results
   for (int r = tid; r < n; r += NT) {
-    b.emo[v.o0 + r] = iter > 0 ? emo_s[r] : 0.0;
-    b.focc[v.o0 + r] = iter > 0 ? focc_u[inv[r]] : 0.0;
+    b.emo[v.o0 + r] = have_levels ? emo_s[r] : 0.0;
+    b.focc[v.o0 + r] = have_levels ? focc_u[inv[r]] : 0.0;
   }
-  if (b.coeff != nullptr && iter > 0) {
-    double* co = b.coeff + moff;
+  // wavefunction arrays in tblite's column-major layout: density(i, j) of the
+  // last iteration (still in A / Y: the energy phase does not touch it) and
+  // coeff(i, r) = coefficient of AO i in level r (levels ascending)
+  const bool wfn_ok = have_levels && !(status & (TA_STATUS_EIGEN | TA_STATUS_NAN));
+  if (b.density != nullptr) {
+    double* dm = b.density + moff;
     for (int idx = tid; idx < n * n; idx += NT) {
-      const int i = idx / n, r = idx - (idx / n) * n;
-      // the global-memory path forms the occupied levels' coefficients only
-      co[idx] = (MATS_LDS || focc_u[inv[r]] > TA_OCC_MIN) ? CT[inv[r] * ld + i] : 0.0;
+      const int j = idx / n, i = idx - j * n;
+      dm[idx] = wfn_ok ? Pm[aix(i, j)] : 0.0;
+    }
+  }
+  if (b.coeff != nullptr) {
+    double* co = b.coeff + moff;
+    if constexpr (!MATS_LDS) {
+      // the SCF formed the occupied levels' vectors only: all n from the
+      // tridiagonal form of the last iteration (its reflectors stay in A,
+      // d, e, tau and |T| in LDS); overwrites Y, read out above
+      __syncthreads();
+      if (wfn_ok) {
+        const double* sv = scr + big_tri_save_offset(ld);
+        double* er = smem + L.eig;
+        for (int k = tid; k < 3 * ld; k += NT) er[k] = sv[k];
+        if (tid == 0) scal[0] = sv[3 * ld];
+        __syncthreads();
+        big_eigenvectors<NT>(A, C, Y, CT, scr, n, ld, n, smem + L.eig, ev, scal, rank, clk);
+      }
+      __syncthreads();
+    }
+    for (int idx = tid; idx < n * n; idx += NT) {
+      const int r = idx / n, i = idx - r * n;
+      co[idx] = wfn_ok ? CT[(int64_t)inv[r] * ld + i] : 0.0;
     }
   }
   if (clk.on) {

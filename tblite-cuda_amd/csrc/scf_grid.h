@@ -65,9 +65,9 @@ struct GridHostStats {
 // Runs the SCF of one molecule on the grid path (stream s); writes the
 // molecule's energy, electronic energy, repulsion, iterations and status
 // into the given device slots and its levels / occupations (and optionally
-// coefficients).  verbose != 0 prints one line per SCF iteration to stderr.
+// the coefficients and the density matrix, n x n column-major each).  verbose != 0 prints one line per SCF iteration to stderr.
 // use_graph != 0 replays the eigenvalue phase of iterations 2.. from a HIP
 // graph captured in iteration 2.  Returns 0 or < 0 on a HIP failure.
 int grid_scf(GridMol g, hipStream_t s, int max_iter, double* d_energy, double* d_eelec,
              double* d_erep, int* d_iter, int* d_status, double* emo, double* focc, double* coeff,
-             GridHostStats* hs, int verbose, int use_graph);
+             double* density, GridHostStats* hs, int verbose, int use_graph);

@@ -1244,23 +1244,25 @@ __global__ void k_g_final(GridMol g, int iter, int status, double* energy, doubl
   }
 }
 
-// results: sorted levels and occupations, optional coefficients
-__global__ void k_g_results(GridMol g, double* emo, double* focc, double* coeff)
+// results: sorted levels and occupations
+__global__ void k_g_results(GridMol g, double* emo, double* focc)
 {
   const int n = g.n;
   for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
     emo[r] = g.lam[r];
     focc[r] = g.focc[r];
   }
-  if (coeff) {
-    // only the occupied levels' coefficients are formed; the rest read as 0
-    const int nocc = g.ic[GI_NOCC];
-    const int64_t tot = (int64_t)n * n;
-    for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < tot;
-         idx += (int64_t)gridDim.x * blockDim.x) {
-      const int i = (int)(idx / n), r = (int)(idx - (idx / n) * n);
-      coeff[idx] = r < nocc ? g.CT[(int64_t)r * g.ld + i] : 0.0;
-    }
+}
+
+// n x n matrix out(i, j) = M[j * ld + i] (column-major, tblite's layout) from
+// a row-major ld-strided matrix; rows past nrow read as 0
+__global__ void k_g_unpad(const double* M, int n, int ld, int nrow, double* out)
+{
+  const int64_t tot = (int64_t)n * n;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < tot;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int j = (int)(idx / n), i = (int)(idx - (idx / n) * n);
+    out[idx] = j < nrow ? M[(int64_t)j * ld + i] : 0.0;
   }
 }
 
@@ -1437,7 +1439,7 @@ struct EigGraph {
 
 int grid_scf(GridMol g, hipStream_t s, int max_iter, double* d_energy, double* d_eelec,
              double* d_erep, int* d_iter, int* d_status, double* emo, double* focc, double* coeff,
-             GridHostStats* hs, int verbose, int use_graph)
+             double* density, GridHostStats* hs, int verbose, int use_graph)
 {
   const int n = g.n, ld = g.ld;
   hipError_t err;
@@ -1561,7 +1563,28 @@ int grid_scf(GridMol g, hipStream_t s, int max_iter, double* d_energy, double* d
   if (!converged && status == 0) status |= TA_STATUS_NOT_CONVERGED;
   hipLaunchKernelGGL(k_g_final, dim3(1), dim3(64), 0, s, g, iter, status, d_energy, d_eelec,
                      d_erep, d_iter, d_status);
-  hipLaunchKernelGGL(k_g_results, dim3(grid_blocks(n)), dim3(256), 0, s, g, emo, focc, coeff);
+  hipLaunchKernelGGL(k_g_results, dim3(grid_blocks(n)), dim3(256), 0, s, g, emo, focc);
+  // wavefunction arrays: the density of the last iteration (in Y), then the
+  // coefficients of all n levels from the last tridiagonal form (the SCF
+  // formed the occupied ones only), MO rows CT = Z^T X
+  const bool wfn_ok = iter > 0 && !(status & (TA_STATUS_EIGEN | TA_STATUS_NAN | TA_STATUS_CHOLESKY));
+  if (density)
+    hipLaunchKernelGGL(k_g_unpad, dim3(grid_blocks((int64_t)n * n)), dim3(256), 0, s, g.Y, n, ld,
+                       wfn_ok ? n : 0, density);
+  if (coeff) {
+    if (wfn_ok) {
+      if (grid_eigenvectors(g, s, n) != hipSuccess) return -2;
+      const double* Z = g.Y;
+      const double* X = g.X;
+      double* CT = g.CT;
+      g_gemm<false, true>(
+          s, n, n, [=] __device__(int i, int k) { return Z[(int64_t)k * ld + i]; },
+          [=] __device__(int k, int j) { return X[(int64_t)k * ld + j]; }, AllK{n}, NoSkip{},
+          [=] __device__(int i, int j, double val) { CT[(int64_t)i * ld + j] = val; });
+    }
+    hipLaunchKernelGGL(k_g_unpad, dim3(grid_blocks((int64_t)n * n)), dim3(256), 0, s, g.CT, n, ld,
+                       wfn_ok ? n : 0, coeff);
+  }
   if (hipStreamSynchronize(s) != hipSuccess) return -2;
   if (hs) {
     hs->iterations = iter;

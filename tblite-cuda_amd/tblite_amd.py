@@ -97,7 +97,7 @@ _PROTOS = {
     "tblite_amd_get_error": (_I, [_P, C.c_char_p, _I]),
     "tblite_amd_new_structure": (_P, [_P, _I, _IP, _DP, _D, _I]),
     "tblite_amd_delete_structure": (None, [C.POINTER(_P)]),
-    "tblite_amd_get_structure_dims": (_I, [_P, _IP, _IP, _IP]),
+    "tblite_amd_get_structure_dims": (_I, [_P, _P, _IP, _IP, _IP]),
     "tblite_amd_new_gfn2_calculator": (_P, [_P, C.POINTER(Param)]),
     "tblite_amd_delete_calculator": (None, [C.POINTER(_P)]),
     "tblite_amd_set_calculator_accuracy": (_I, [_P, _D]),
@@ -137,6 +137,7 @@ _PROTOS = {
     "tblite_amd_server_poll": (_I, [_P, _I, C.POINTER(C.c_longlong), _DP, _IP, _IP, _DP]),
     "tblite_amd_server_stats": (_I, [_P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), _IP, _IP,
                                      C.POINTER(C.c_longlong)]),
+    "tblite_amd_workspace_estimate": (_I, [C.POINTER(Param), _I, _IP, _IP, _I, _DP, _DP]),
 }
 
 PROFILE_SLOTS = ["init", "mix", "potential", "fock", "transform", "jacobi", "occupation",
@@ -313,14 +314,20 @@ class Engine:
         self._check(self.lib.tblite_amd_batch_results(self.ctx, _dp(e), _ip(it), _ip(st)), allow=(-3,))
         return e, it, st
 
-    def singlepoint(self, num, xyz, charge=0.0, uhf=0, accuracy=None):
-        """One molecule through tblite_amd_xtb_singlepoint (the per-molecule path)."""
+    def singlepoint(self, num, xyz, charge=0.0, uhf=0, accuracy=None, wavefunction=True):
+        """One molecule through tblite_amd_xtb_singlepoint (the per-molecule path).
+
+        With wavefunction=True a wavefunction handle is passed and every array
+        of it is returned (tblite's layout: density and coeff are nao x nao,
+        column-major, so coeff[:, r] is level r after the transpose below);
+        with False the call is energy-only (no handle, as a caller that wants
+        the energy alone would make it)."""
         num = np.ascontiguousarray(num, dtype=np.int32)
         xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
         mol = self.lib.tblite_amd_new_structure(self.ctx, len(num), _ip(num), _dp(xyz), charge, uhf)
         if not mol:
             raise EngineError(self.error())
-        wfn = self.lib.tblite_amd_new_wavefunction(self.ctx)
+        wfn = self.lib.tblite_amd_new_wavefunction(self.ctx) if wavefunction else None
         e = C.c_double()
         it = C.c_int()
         try:
@@ -328,17 +335,26 @@ class Engine:
                 self.ctx, mol, self.calc, wfn, self.accuracy if accuracy is None else accuracy,
                 C.byref(e), C.byref(it), 0)
             out = {"energy": e.value, "iterations": it.value, "rc": rc}
-            for name in ("qsh", "qat", "dpat", "qpat", "emo", "focc"):
-                cnt = C.c_int()
-                self.lib.tblite_amd_get_wavefunction_array(wfn, name.encode(), None, C.byref(cnt))
-                arr = np.zeros(cnt.value)
-                self.lib.tblite_amd_get_wavefunction_array(wfn, name.encode(), _dp(arr), C.byref(cnt))
-                out[name] = arr
+            if wavefunction and rc in (0, -3):
+                for name in ("qsh", "qat", "dpat", "qpat", "emo", "focc", "n0sh", "n0at",
+                             "density", "coeff"):
+                    cnt = C.c_int()
+                    self._check(self.lib.tblite_amd_get_wavefunction_array(
+                        wfn, name.encode(), None, C.byref(cnt)))
+                    arr = np.zeros(cnt.value)
+                    self._check(self.lib.tblite_amd_get_wavefunction_array(
+                        wfn, name.encode(), _dp(arr), C.byref(cnt)))
+                    out[name] = arr
+                nao = len(out["emo"])
+                # column-major (Fortran) nao x nao -> numpy [row, column]
+                out["density"] = out["density"].reshape(nao, nao).T.copy()
+                out["coeff"] = out["coeff"].reshape(nao, nao).T.copy()
         finally:
             h = C.c_void_p(mol)
             self.lib.tblite_amd_delete_structure(C.byref(h))
-            h = C.c_void_p(wfn)
-            self.lib.tblite_amd_delete_wavefunction(C.byref(h))
+            if wfn:
+                h = C.c_void_p(wfn)
+                self.lib.tblite_amd_delete_wavefunction(C.byref(h))
         return out
 
     def dims(self, imol):

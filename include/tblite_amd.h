@@ -225,6 +225,28 @@ int tblite_amd_singlepoint_flat(tblite_amd_context ctx, tblite_amd_calculator ca
                                 const double* charge, const int* uhf, double accuracy,
                                 double* energies, int* iterations, int* status);
 
+/* No-allocation variant of tblite_amd_singlepoint_flat, with the same
+ * arguments and results.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  It calls no hipMalloc,
+ * hipFree or other allocating routine: the parameter block and the whole
+ * batch workspace live in one static device array of
+ * TBLITE_AMD_NOALLOC_SCRATCH_BYTES that each device the library runs on
+ * holds until the process ends.  A batch whose workspace exceeds it returns
+ * TBLITE_AMD_ERR_TOO_LARGE (tblite_amd_workspace_estimate gives the size
+ * beforehand; at 250 SCF iterations the Broyden history takes about 3 MB per
+ * molecule), as does a molecule above the one-workgroup size (padded AO
+ * dimension above grid_min_ld).  Every call on a device shares that array
+ * and leaves its contents behind: calls that overlap on one device, from any
+ * context or thread, race, and the caller must serialise them.  The
+ * stage-level accessors below read the array until the next call. */
+#define TBLITE_AMD_NOALLOC_SCRATCH_BYTES (64u << 20)
+int tblite_amd_singlepoint_flat_noalloc(tblite_amd_context ctx, tblite_amd_calculator calc,
+                                        int nmol, const int* mol_nat, const int* num,
+                                        const double* xyz, const double* charge, const int* uhf,
+                                        double accuracy, double* energies, int* iterations,
+                                        int* status);
+
 /* Device-resident batches.  prepare() plans a batch from its composition
  * (atom counts, elements, charges, spins) and allocates the device workspace;
  * set_geometry() loads coordinates (host or device pointer, xyz(3,total)

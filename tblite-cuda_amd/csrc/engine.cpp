@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -292,10 +293,48 @@ struct DeviceBatch {
 
 }  // namespace
 
+// ------------------------------------------------- fixed device workspace
+// Workspace of tblite_amd_singlepoint_flat_noalloc: a static array that every
+// device the library runs on holds (its own copy) until the process ends.  In
+// host code `scratch` names a host-side stand-in, never the device memory:
+// the device address comes from hipGetSymbolAddress, resolved once per device
+// (scratch_address) and used for every kernel argument and copy.
+static __device__ __align__(256) char scratch[TBLITE_AMD_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+bool g_scratch_resolved[kMaxDevices] = {};
+char* g_scratch_ptr[kMaxDevices] = {};
+hipError_t g_scratch_err[kMaxDevices] = {};
+
+// Device address of `scratch` on `device` (which must be current); resolved
+// on the first call for that device, then cached.
+char* scratch_address(int device, hipError_t* err)
+{
+  if (device < 0 || device >= kMaxDevices) {
+    *err = hipErrorInvalidDevice;
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_resolved[device]) {
+    void* ptr = nullptr;
+    g_scratch_err[device] = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    g_scratch_ptr[device] = g_scratch_err[device] == hipSuccess ? static_cast<char*>(ptr) : nullptr;
+    g_scratch_resolved[device] = true;
+  }
+  *err = g_scratch_err[device];
+  return g_scratch_ptr[device];
+}
+}  // namespace
+
 // ------------------------------------------------------------- handles
 struct tblite_amd_context_s {
   int device = 0;
   hipStream_t stream = nullptr;
+  // device address of this device's fixed workspace (the static array of the
+  // no-allocation entry point), or null if it could not be resolved
+  char* fixed_ws = nullptr;
   hipStream_t aux[kNumStreams] = {};
   hipEvent_t ev[8] = {};
   hipEvent_t aux_done[kNumStreams] = {};
@@ -793,8 +832,11 @@ size_t workspace_need(const Plan& pl, size_t img_bytes, bool wfn_arrays)
 // Allocate (or reuse) the workspace for a plan and upload the plan arrays.
 // wfn_arrays: also the per-molecule coefficient and density matrices of a
 // requested wavefunction (2 n^2 doubles per molecule).
+// fixed: carve from this caller-provided arena instead of the context's and
+// never allocate (a plan that does not fit is an error); par: device address
+// of the parameter block when it is not the calculator's own copy.
 int upload_plan(tblite_amd_context ctx, Calculator& c, const Plan& pl, DeviceBatch& db,
-                bool wfn_arrays = false)
+                bool wfn_arrays = false, Arena* fixed = nullptr, const DevParams* par = nullptr)
 {
   const int nmol = pl.nmol, mem = pl.mem;
   std::vector<char> img;
@@ -805,7 +847,12 @@ int upload_plan(tblite_amd_context ctx, Calculator& c, const Plan& pl, DeviceBat
   const size_t need = workspace_need(pl, img.size(), wfn_arrays);
   const bool aes_planes = !pl.grid_mols.empty();
 
-  Arena& ar = ctx->arena;
+  Arena& ar = fixed ? *fixed : ctx->arena;
+  if (fixed && ar.cap < need) {
+    set_error(ctx, "batch needs %zu bytes of device workspace; the fixed workspace holds %zu",
+              need, ar.cap);
+    return TBLITE_AMD_ERR_TOO_LARGE;
+  }
   if (ar.cap < need) {
     if (ar.base) {
       HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -832,7 +879,7 @@ int upload_plan(tblite_amd_context ctx, Calculator& c, const Plan& pl, DeviceBat
   b.damping = c.damping;
   b.kt = c.etemp * TBLITE_AMD_KB_EH;
   b.accuracy = 0.0; // set by the caller
-  b.par = c.d_par;
+  b.par = par ? par : c.d_par;
   b.mol_atom_off = I(0);
   b.mol_nat = I(1);
   b.mol_sh_off = I(2);
@@ -1311,6 +1358,12 @@ static tblite_amd_context make_context(int device, bool single_stream)
   if (!ctx) return nullptr;
   ctx->device = device;
   ctx->single_stream = single_stream;
+  {
+    // the fixed workspace's device address, once per device (a failure only
+    // disables the no-allocation entry point, which reports it)
+    hipError_t se = hipSuccess;
+    ctx->fixed_ws = scratch_address(device, &se);
+  }
   bool ok = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) == hipSuccess;
   for (int s = 0; s < kNumStreams && ok && !single_stream; ++s)
     ok = hipStreamCreateWithFlags(&ctx->aux[s], hipStreamNonBlocking) == hipSuccess &&
@@ -1648,6 +1701,60 @@ int tblite_amd_singlepoint_flat(tblite_amd_context ctx, tblite_amd_calculator ca
   const double acc = accuracy > 0.0 ? accuracy : calc->c.accuracy;
   return flat_singlepoint(ctx, calc->c, nmol, mol_nat, num, xyz, charge, uhf, acc, energies,
                           iterations, status);
+}
+
+int tblite_amd_singlepoint_flat_noalloc(tblite_amd_context ctx, tblite_amd_calculator calc,
+                                        int nmol, const int* mol_nat, const int* num,
+                                        const double* xyz, const double* charge, const int* uhf,
+                                        double accuracy, double* energies, int* iterations,
+                                        int* status)
+{
+  if (!ctx || !calc) return TBLITE_AMD_ERR_ARGUMENT;
+  if (nmol < 1 || !mol_nat || !num || !xyz || !energies) {
+    set_error(ctx, "invalid arguments to the no-allocation singlepoint");
+    return TBLITE_AMD_ERR_ARGUMENT;
+  }
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  if (!ctx->fixed_ws) {
+    hipError_t e = hipSuccess;
+    (void)scratch_address(ctx->device, &e);
+    set_error(ctx, "fixed device workspace unavailable on device %d: %s", ctx->device,
+              hipGetErrorString(e));
+    return TBLITE_AMD_ERR_HIP;
+  }
+  Calculator& c = calc->c;
+  const double acc = accuracy > 0.0 ? accuracy : c.accuracy;
+  // the parameter block first, then the batch; nothing else is allocated
+  Arena ws;
+  ws.base = ctx->fixed_ws;
+  ws.cap = TBLITE_AMD_NOALLOC_SCRATCH_BYTES;
+  ws.used = 0;
+  DevParams* dpar = carve<DevParams>(ws, 1);
+  Arena rest;
+  rest.base = ws.base + ws.used;
+  rest.cap = ws.cap - ws.used;
+  ctx->have_batch = false;
+  ctx->prepared = nullptr;
+  // fixed routing by size: the multi-workgroup path (host loop with graph
+  // capture) is not taken here
+  const bool ag = ctx->auto_grid;
+  ctx->auto_grid = false;
+  int rc = make_plan(ctx, c, nmol, mol_nat, num, charge, uhf, acc, ctx->plan);
+  ctx->auto_grid = ag;
+  if (rc) return rc;
+  if (!ctx->plan.grid_mols.empty()) {
+    const int m = ctx->plan.grid_mols[0];
+    set_error(ctx, "molecule %d (%d AOs) needs the multi-workgroup path, which the "
+              "no-allocation entry point does not run", m, ctx->plan.mol_nao[m]);
+    return TBLITE_AMD_ERR_TOO_LARGE;
+  }
+  HIP_TRY(ctx, hipMemcpyAsync(dpar, &c.dev, sizeof(DevParams), hipMemcpyHostToDevice, ctx->stream));
+  rc = upload_plan(ctx, c, ctx->plan, ctx->batch, false, &rest, dpar);
+  if (rc) return rc;
+  rc = run_batch(ctx, c, xyz, acc);
+  if (rc) return rc;
+  ctx->have_batch = true;
+  return copy_results(ctx, nmol, energies, iterations, status);
 }
 
 int tblite_amd_batch_prepare(tblite_amd_context ctx, tblite_amd_calculator calc, int nmol,

@@ -110,6 +110,7 @@ _PROTOS = {
     "tblite_amd_xtb_singlepoint": (_I, [_P, _P, _P, _P, _D, _DP, _IP, _I]),
     "tblite_amd_xtb_singlepoint_batch": (_I, [_P, _I, C.POINTER(_P), _P, C.POINTER(_P), _D, _DP, _IP, _IP]),
     "tblite_amd_singlepoint_flat": (_I, [_P, _P, _I, _IP, _IP, _DP, _DP, _IP, _D, _DP, _IP, _IP]),
+    "tblite_amd_singlepoint_flat_noalloc": (_I, [_P, _P, _I, _IP, _IP, _DP, _DP, _IP, _D, _DP, _IP, _IP]),
     "tblite_amd_batch_prepare": (_I, [_P, _P, _I, _IP, _IP, _DP, _IP, _D]),
     "tblite_amd_batch_set_geometry": (_I, [_P, _DP, _I]),
     "tblite_amd_batch_geometry_device": (_P, [_P]),

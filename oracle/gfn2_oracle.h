@@ -104,6 +104,12 @@ int or_coulomb_eval(const or_calc* calc, int nat, const int* num, const double* 
                     const double* qsh, const double* dpat, const double* qpat, double* e,
                     double* vsh, double* vat, double* vdp, double* vqp);
 
+/* Mixer on an affine map F(q) = A q + b (test hook for broyden_next). */
+int or_broyden_linear(int n, const double* A, const double* b, const double* q0, double damp,
+                      double omega0, int memory, int nstep, double* qs, double* res);
+/* H0 scaling of a shell pair (element indices into the parameter table). */
+double or_hscale(const or_calc* calc, int ei, int ish, int ej, int jsh);
+
 #ifdef __cplusplus
 }
 #endif

@@ -71,6 +71,11 @@ def lib():
                                         C.POINTER(C.c_double), C.c_double] + [C.POINTER(C.c_double)] * 3
         L.or_get_cgto.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.or_broyden_linear.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                        C.POINTER(C.c_double), C.c_double, C.c_double, C.c_int,
+                                        C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.or_hscale.restype = C.c_double
+        L.or_hscale.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
         L.or_ncoord.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
                                 C.POINTER(C.c_double)]
         L.or_repulsion.argtypes = L.or_ncoord.argtypes
@@ -164,6 +169,23 @@ class Oracle:
                                    temperature * KB_EH, max_iter, damping, nthreads, _dp(e),
                                    _ip(it), _ip(st))
         return e, it, st
+
+    def broyden_linear(self, A, b, q0, damping=0.4, memory=250, nstep=10, omega0=0.01):
+        """Mixer iterates on F(q) = A q + b: visited q's (nstep+1, n) and
+        their residual norms |F(q) - q|; omega0 as in the SCF by default."""
+        A = np.asfortranarray(A, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        q0 = np.ascontiguousarray(q0, dtype=np.float64)
+        n = len(b)
+        qs = np.zeros((nstep + 1, n))
+        res = np.zeros(nstep + 1)
+        rc = self.L.or_broyden_linear(n, A.ctypes.data_as(C.POINTER(C.c_double)), _dp(b), _dp(q0),
+                                      damping, omega0, memory, nstep, _dp(qs), _dp(res))
+        return rc, qs, res
+
+    def hscale(self, ei, ish, ej, jsh):
+        """H0 scaling of shell ish of element index ei with shell jsh of ej."""
+        return self.L.or_hscale(self.calc, ei, ish, ej, jsh)
 
     def cn(self, num, xyz):
         num, xyz = self._mol(num, xyz)

@@ -213,6 +213,9 @@ struct BatchDev {
   // > 0: per-iteration stopping threshold max(jacobi_tol, min(1e-2,
   // jacobi_adapt * previous charge residual))
   double jacobi_adapt;
+  // eigensolver of the LDS-resident kernels: 1 Householder tridiagonalisation
+  // + bisection + twisted-factorisation eigenvectors, 0 warm-started Jacobi
+  int lds_eig;
   // optional per-molecule phase clocks of the SCF kernel (TA_PROF_SLOTS each), may be null
   long long* prof;
   // optional completion records (one per molecule, pinned host memory), may be null

@@ -402,6 +402,16 @@ __device__ __noinline__ void tridiagonalize_blocked(gbl_dbl* A, int n, int ld, l
 // ---------------------------------------------------------- eigenvalues
 // Number of eigenvalues below each of three points (Sturm sequences run in
 // lockstep for instruction-level parallelism).
+// 1 / x to full double precision without the IEEE division sequence: the
+// hardware reciprocal estimate refined by one Newton step (the Sturm
+// recurrence only needs the sign of its pivots, and a relative error near
+// the rounding unit perturbs T by that much).
+__device__ __forceinline__ double rcp_nr(double x)
+{
+  const double r = __builtin_amdgcn_rcp(x);
+  return fma(r, fma(-x, r, 1.0), r);
+}
+
 template <class PD, class PE>
 __device__ __forceinline__ void sturm3(PD d, PE e2, int n, double x1,
                                        double x2, double x3, double pivmin, int& c1, int& c2,
@@ -414,9 +424,9 @@ __device__ __forceinline__ void sturm3(PD d, PE e2, int n, double x1,
   int a1 = q1 < 0.0, a2 = q2 < 0.0, a3 = q3 < 0.0;
   for (int k = 1; k < n; ++k) {
     const double dk = d[k], ek = e2[k - 1];
-    q1 = (dk - x1) - ek / q1;
-    q2 = (dk - x2) - ek / q2;
-    q3 = (dk - x3) - ek / q3;
+    q1 = (dk - x1) - ek * rcp_nr(q1);
+    q2 = (dk - x2) - ek * rcp_nr(q2);
+    q3 = (dk - x3) - ek * rcp_nr(q3);
     if (fabs(q1) < pivmin) q1 = -pivmin;
     if (fabs(q2) < pivmin) q2 = -pivmin;
     if (fabs(q3) < pivmin) q3 = -pivmin;
@@ -431,16 +441,23 @@ __device__ __forceinline__ void sturm3(PD d, PE e2, int n, double x1,
 
 // All eigenvalues of T (d, e) in ascending order into lam, to an absolute
 // accuracy of 1e-12 |T| (far below what the SCF observables resolve); e2 =
-// e^2 scratch (LDS).  Warm start: with *warm > 0 each level is first
-// bracketed in prev[i] -/+ *warm (the previous SCF step's levels), checked
-// by Sturm counts, and falls back to the Gershgorin interval when the check
-// fails.  On return *warm holds the bracket for the next call (twice the
-// largest level shift seen, plus a margin) and scal[0] = |T| (Gershgorin).
+// e^2 scratch (LDS).  BTPL = 4 adjacent lanes share a level: each evaluates
+// three Sturm counts per step, so a step splits the bracket into 13 parts
+// (log13 instead of log4 steps) and a workgroup holds NT / 4 levels at once.
+// Warm start: with *warm > 0 each level is first bracketed in prev[i] -/+
+// *warm (the previous SCF step's levels), checked by Sturm counts, and falls
+// back to the Gershgorin interval when the check fails.  On return *warm
+// holds the bracket for the next call (twice the largest level shift seen,
+// plus a margin) and scal[0] = |T| (Gershgorin).
+constexpr int BTPL = 4;
+
 template <int NT>
 __device__ __noinline__ void bisect_all(const lds_dbl* d, const lds_dbl* e, lds_dbl* e2, int n,
                                        lds_dbl* lam, const lds_dbl* prev, lds_dbl* warm,
                                        lds_dbl* red, lds_dbl* scal)
 {
+  static_assert(NT % BTPL == 0, "lane groups of BTPL");
+  constexpr int NPT = 3 * BTPL;  // points per step
   const int tid = threadIdx.x;
   double gl = INFINITY, gu = -INFINITY, emax = 0.0;
   for (int k = tid; k < n; k += NT) {
@@ -462,7 +479,8 @@ __device__ __noinline__ void bisect_all(const lds_dbl* d, const lds_dbl* e, lds_
   gl -= 2.0 * tol;
   gu += 2.0 * tol;
   double shift = 0.0;
-  for (int i = tid; i < n; i += NT) {
+  const int sub = tid % BTPL;
+  for (int i = tid / BTPL; i < n; i += NT / BTPL) {
     double lo = gl, hi = gu;
     if (wd > 0.0) {
       const double a = prev[i] - wd, b = prev[i] + wd;
@@ -473,26 +491,25 @@ __device__ __noinline__ void bisect_all(const lds_dbl* d, const lds_dbl* e, lds_
         hi = b;
       }
     }
-    for (int it = 0; it < 64 && hi - lo > tol; ++it) {
-      const double h = hi - lo;
-      const double x1 = lo + 0.25 * h, x2 = lo + 0.5 * h, x3 = lo + 0.75 * h;
+    for (int it = 0; it < 40 && hi - lo > tol; ++it) {
+      const double h = (hi - lo) / (double)(NPT + 1);
+      const double x1 = lo + (3 * sub + 1) * h, x2 = lo + (3 * sub + 2) * h,
+                   x3 = lo + (3 * sub + 3) * h;
       int c1, c2, c3;
       sturm3(d, e2, n, x1, x2, x3, pivmin, c1, c2, c3);
-      if (c1 > i) {
-        hi = x1;
-      } else if (c2 > i) {
-        lo = x1;
-        hi = x2;
-      } else if (c3 > i) {
-        lo = x2;
-        hi = x3;
-      } else {
-        lo = x3;
-      }
+      // first point of this lane whose count exceeds i (NPT + 1: none)
+      int j = (c1 > i) ? 3 * sub : ((c2 > i) ? 3 * sub + 1 : ((c3 > i) ? 3 * sub + 2 : NPT));
+#pragma unroll
+      for (int off = 1; off < BTPL; off <<= 1) j = min(j, __shfl_xor(j, off, BTPL));
+      // the level lies between points j-1 and j (points -1 and NPT: lo, hi)
+      const double nlo = j == 0 ? lo : lo + j * h;
+      const double nhi = j == NPT ? hi : lo + (j + 1) * h;
+      lo = nlo;
+      hi = nhi;
     }
     const double l = 0.5 * (lo + hi);
     if (wd > 0.0) shift = fmax(shift, fabs(l - prev[i]));
-    lam[i] = l;
+    if (sub == 0) lam[i] = l;
   }
   shift = block_max(shift, red);
   if (tid == 0) {

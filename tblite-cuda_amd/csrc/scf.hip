@@ -33,6 +33,7 @@
 #include "d4.h"
 #include "device_util.h"
 #include "eig_big.h"
+#include "eig_lds.h"
 #include "fermi.h"
 
 #define JACOBI_MAX_SWEEPS 40
@@ -1351,6 +1352,27 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
         mm_nn<NT, MAXT, true, false>(C, A, A, ld);
       }
       clk.lap(TA_PROF_TRANSFORM);
+      if (b.lds_eig) {
+        // Householder in LDS: reflectors applied to the basis rows as they are
+        // formed, levels by bisection, eigenvectors of T into A, then the new
+        // MO rows CT <- Z^T CT (ascending levels)
+        const ldseig::Work W = ldseig::work_layout(work, ld);
+        ldseig::tridiagonalize<NT>(TA_LDS(A), TA_LDS(C), n, ld, W, TA_LDS(red), TA_LDS(scal));
+        clk.lap(TA_PROF_TRIDIAG);
+        big::bisect_all<NT>(W.d, W.e, W.e2, n, TA_LDS(ev), TA_LDS(emo_s),
+                            TA_LDS(red + 24), TA_LDS(red), TA_LDS(scal));
+        clk.lap(TA_PROF_BISECT);
+        ldseig::eigenvectors<NT>(W.d, W.e, TA_LDS(ev), n, ld, TA_LDS(A), scal[0], W.vb);
+        clk.lap(TA_PROF_INVIT);
+        ldseig::rotate_basis<NT, MFMA_TILES(NT)>(TA_LDS(A), TA_LDS(C), ld);
+        clk.lap(TA_PROF_BACKTR);
+        for (int k = tid; k < n; k += NT) {
+          rank[k] = k;
+          inv[k] = k;
+          emo_s[k] = ev[k];
+        }
+        __syncthreads();
+      } else {
       int sweeps = 0;
       // looser stopping rule while the charges are still far from
       // self-consistency (jacobi_adapt > 0): the next iterations start from
@@ -1381,6 +1403,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
         emo_s[r] = e;
       }
       __syncthreads();
+      }
     } else {
       // levels of the previous step (emo_s) warm-start the bisection; the
       // eigenvectors follow once the occupations are known

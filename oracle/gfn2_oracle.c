@@ -1670,6 +1670,732 @@ int or_hamiltonian(const or_calc* calc, int nat, const int* num, const double* x
  * convergence test (|dE| < 1e-6 acc and mixer error < 2e-5 acc) and the
  * mixing step are restated from upstream tblite (the reference never sets
  * `converged`, src/xtb.cu:413-433). */
+/* ------------------------------------------------------ analytic gradient
+ * dE/dR of the converged SCF energy.  The free energy is stationary in the
+ * orbitals and the Fermi occupations (Mermin), so the orbital response drops
+ * out and only an overlap term remains:
+ *   dE/dR = sum P dH0/dR - sum W dS/dR + sum_s v_s dq_s/dR|_P
+ *         + sum_a vdp_a . ddpat_a/dR|_P + sum_a vqp_a . dqpat_a/dR|_P
+ *         + explicit derivatives of repulsion, ES2, AES2 and D4 at fixed
+ *           moments, and the CN chains of the self energies, the multipole
+ *           radii and the D4 weights,
+ * with W = sum_k f_k e_k c_k c_k^T.  This is the upstream gradient that the
+ * reference sketches: multipole_grad_3d / multipole_grad_cgto
+ * (src/multipole.cu:422-519, :846-981 text), the gradient shift_operator
+ * (:290-323), the repulsion dG (src/xtb.cu:118-121 text) and the CN
+ * derivative dcndr (src/coulomb.cu:289-292 text). */
+
+/* one axis of multipole_3d: overlap, first and second moment about center i
+ * of the product of (x - P + rpi)^li and (x - P + rpj)^lj */
+static void axis_moments(double rpi, double rpj, int li, int lj, const double* s1d, double v[3])
+{
+  double vi[5] = {0}, vj[5] = {0}, vv[9] = {0};
+  shift_poly(rpi, li, vi);
+  shift_poly(rpj, lj, vj);
+  for (int a = 0; a <= li; ++a)
+    for (int b = 0; b <= lj; ++b) vv[a + b] += vi[a] * vj[b];
+  v[0] = v[1] = v[2] = 0.0;
+  for (int l = 0; l <= li + lj; ++l) {
+    v[0] += s1d[l] * vv[l];
+    v[1] += (s1d[l + 1] + rpi * s1d[l]) * vv[l];
+    v[2] += (s1d[l + 2] + 2.0 * rpi * s1d[l + 1] + rpi * rpi * s1d[l]) * vv[l];
+  }
+}
+
+/* moment exponents of the ten components s, d(x,y,z), q(xx,xy,yy,xz,yz,zz) */
+static const int mom3[10][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {2, 0, 0},
+                                {1, 1, 0}, {0, 2, 0}, {1, 0, 1}, {0, 1, 1}, {0, 0, 2}};
+
+/* multipole_grad_3d: the ten moments of one primitive pair and their
+ * derivatives by vec = R_i - R_j.  The integral depends on vec only (the
+ * operator sits on center i), and d/dX_j of a cartesian Gaussian is
+ * -l phi(l-1) + 2 a_j phi(l+1), so dI/dvec_x = l_x I(l_x-1) - 2 a_j I(l_x+1). */
+static void multipole_grad_3d(const double rpj[3], const double rpi[3], double aj, const int lj[3],
+                              const int li[3], const double* s1d, double val[10],
+                              double grad[3][10])
+{
+  double v[3][3], g[3][3];
+  for (int k = 0; k < 3; ++k) {
+    double up[3], dn[3] = {0.0, 0.0, 0.0};
+    axis_moments(rpi[k], rpj[k], li[k], lj[k], s1d, v[k]);
+    axis_moments(rpi[k], rpj[k], li[k], lj[k] + 1, s1d, up);
+    if (lj[k] > 0) axis_moments(rpi[k], rpj[k], li[k], lj[k] - 1, s1d, dn);
+    for (int m = 0; m < 3; ++m) g[k][m] = lj[k] * dn[m] - 2.0 * aj * up[m];
+  }
+  for (int c = 0; c < 10; ++c) {
+    const int* e = mom3[c];
+    val[c] = v[0][e[0]] * v[1][e[1]] * v[2][e[2]];
+    grad[0][c] = g[0][e[0]] * v[1][e[1]] * v[2][e[2]];
+    grad[1][c] = v[0][e[0]] * g[1][e[1]] * v[2][e[2]];
+    grad[2][c] = v[0][e[0]] * v[1][e[1]] * g[2][e[2]];
+  }
+}
+
+static void remove_trace(double* q)
+{
+  double tr = 0.5 * (q[0] + q[2] + q[5]);
+  q[0] = 1.5 * q[0] - tr;
+  q[1] = 1.5 * q[1];
+  q[2] = 1.5 * q[2] - tr;
+  q[3] = 1.5 * q[3];
+  q[4] = 1.5 * q[4];
+  q[5] = 1.5 * q[5] - tr;
+}
+
+/* multipole_grad_cgto: S(sj,si), D(3,sj,si), Q(6,sj,si) as multipole_cgto,
+ * and their derivatives by vec: dS[m + 3 ij], dD[m + 3 (k + 3 ij)],
+ * dQ[m + 3 (k + 6 ij)] (ij = jao + sj * iao, operator on center i). */
+static void multipole_grad_cgto(const cgto_t* cj, const cgto_t* ci, double r2, const double vec[3],
+                                double intcut, double* S, double* D, double* Q, double* dS,
+                                double* dD, double* dQ)
+{
+  const double sqrtpi3 = pow(PI, 1.5);
+  int li = ci->ang, lj = cj->ang;
+  int mi = mlao(li), mj = mlao(lj), si = msao(li), sj = msao(lj);
+  double cart[10][36], dcart[3][10][36];
+  memset(cart, 0, sizeof(cart));
+  memset(dcart, 0, sizeof(dcart));
+  for (int ip = 0; ip < ci->nprim; ++ip)
+    for (int jp = 0; jp < cj->nprim; ++jp) {
+      double eab = ci->alpha[ip] + cj->alpha[jp];
+      double oab = 1.0 / eab;
+      double est = ci->alpha[ip] * cj->alpha[jp] * r2 * oab;
+      if (est > intcut) continue;
+      double pre = exp(-est) * sqrtpi3 * pow(sqrt(oab), 3);
+      double rpi[3], rpj[3], s1d[8];
+      for (int k = 0; k < 3; ++k) {
+        rpi[k] = -vec[k] * cj->alpha[jp] * oab;
+        rpj[k] = vec[k] * ci->alpha[ip] * oab;
+      }
+      for (int l = 0; l <= li + lj + 3; ++l) s1d[l] = overlap_1d(l, eab);
+      double cc = ci->coeff[ip] * cj->coeff[jp] * pre;
+      for (int mli = 0; mli < mi; ++mli)
+        for (int mlj = 0; mlj < mj; ++mlj) {
+          double val[10], grad[3][10];
+          multipole_grad_3d(rpj, rpi, cj->alpha[jp], lxyz[mlj + lmap[lj]], lxyz[mli + lmap[li]],
+                            s1d, val, grad);
+          int ij = mlj + mj * mli;
+          for (int c = 0; c < 10; ++c) {
+            cart[c][ij] += cc * val[c];
+            for (int m = 0; m < 3; ++m) dcart[m][c][ij] += cc * grad[m][c];
+          }
+        }
+    }
+  double tmp[25];
+  for (int c = 0; c < 10; ++c) {
+    or_transform0(lj, li, cart[c], tmp);
+    for (int ij = 0; ij < sj * si; ++ij) {
+      if (c == 0) S[ij] = tmp[ij];
+      else if (c < 4) D[(c - 1) + 3 * ij] = tmp[ij];
+      else Q[(c - 4) + 6 * ij] = tmp[ij];
+    }
+    for (int m = 0; m < 3; ++m) {
+      or_transform0(lj, li, dcart[m][c], tmp);
+      for (int ij = 0; ij < sj * si; ++ij) {
+        if (c == 0) dS[m + 3 * ij] = tmp[ij];
+        else if (c < 4) dD[m + 3 * ((c - 1) + 3 * ij)] = tmp[ij];
+        else dQ[m + 3 * ((c - 4) + 6 * ij)] = tmp[ij];
+      }
+    }
+  }
+  for (int ij = 0; ij < sj * si; ++ij) {
+    remove_trace(Q + 6 * ij);
+    for (int m = 0; m < 3; ++m) {
+      double q[6];
+      for (int k = 0; k < 6; ++k) q[k] = dQ[m + 3 * (k + 6 * ij)];
+      remove_trace(q);
+      for (int k = 0; k < 6; ++k) dQ[m + 3 * (k + 6 * ij)] = q[k];
+    }
+  }
+}
+
+int or_multipole_grad_cgto(const or_calc* calc, int zj, int jsh, int zi, int ish, double r2,
+                           const double vec[3], double intcut, double* S, double* D, double* Q,
+                           double* dS, double* dD, double* dQ)
+{
+  int ej = calc->zmap[zj], ei = calc->zmap[zi];
+  if (ej < 0 || ei < 0) return -1;
+  multipole_grad_cgto(&calc->cgto[ej][jsh], &calc->cgto[ei][ish], r2, vec, intcut, S, D, Q, dS, dD,
+                      dQ);
+  return 0;
+}
+
+/* Q component index of the cartesian pair (a, b): xx xy yy xz yz zz */
+static const int qidx[3][3] = {{0, 1, 3}, {1, 2, 4}, {3, 4, 5}};
+
+/* derivative by vec_m of shift_operator's dj, qj (the gradient
+ * shift_operator, src/multipole.cu:290-323 text), given s, di and the
+ * derivatives ds = ds/dvec_m, ddi = ddi/dvec_m, dqi = dqi/dvec_m */
+static void shift_operator_deriv(const double vec[3], int m, double s, const double di[3], double ds,
+                                 const double ddi[3], const double dqi[6], double ddj[3],
+                                 double dqj[6])
+{
+  for (int k = 0; k < 3; ++k) ddj[k] = ddi[k] + (k == m ? s : 0.0) + vec[k] * ds;
+  double raw[6];
+  for (int a = 0; a < 3; ++a)
+    for (int b = a; b < 3; ++b) {
+      double v = vec[a] * ddi[b] + vec[b] * ddi[a] + vec[a] * vec[b] * ds;
+      if (a == m) v += di[b] + vec[b] * s;
+      if (b == m) v += di[a] + vec[a] * s;
+      raw[qidx[a][b]] = v;
+    }
+  double tr = 0.5 * (raw[0] + raw[2] + raw[5]);
+  for (int k = 0; k < 6; ++k) dqj[k] = dqi[k] + 1.5 * raw[k];
+  dqj[0] -= tr;
+  dqj[2] -= tr;
+  dqj[5] -= tr;
+}
+
+/* d(exp_count)/dr = -k r0 / r^2 f (1 - f) */
+static double dexp_count(double k, double r, double r0)
+{
+  double f = exp_count(k, r, r0);
+  return -k * r0 / (r * r) * f * (1.0 - f);
+}
+
+/* grad += sum_i dEdcn_i dCN_i/dR (GFN CN, ncoord) */
+static void ncoord_grad(const or_calc* c, const basis_t* b, const double* xyz, const double* dEdcn,
+                        double* grad)
+{
+  const tblite_amd_param* p = &c->par;
+  double cutoff2 = p->cn_cutoff * p->cn_cutoff;
+  for (int i = 0; i < b->nat; ++i)
+    for (int j = 0; j < i; ++j) {
+      double rij[3];
+      for (int k = 0; k < 3; ++k) rij[k] = xyz[3 * i + k] - xyz[3 * j + k];
+      double r2 = rij[0] * rij[0] + rij[1] * rij[1] + rij[2] * rij[2];
+      if (r2 > cutoff2 || r2 < 1.0e-12) continue;
+      double r1 = sqrt(r2);
+      double rc = p->elem[b->el[i]].rcov + p->elem[b->el[j]].rcov;
+      double fa = exp_count(p->cn_ka, r1, rc), fb = exp_count(p->cn_kb, r1, rc + p->cn_rshift);
+      double d = dexp_count(p->cn_ka, r1, rc) * fb + fa * dexp_count(p->cn_kb, r1, rc + p->cn_rshift);
+      double g = (dEdcn[i] + dEdcn[j]) * d / r1;
+      for (int k = 0; k < 3; ++k) {
+        grad[3 * i + k] += g * rij[k];
+        grad[3 * j + k] -= g * rij[k];
+      }
+    }
+}
+
+/* repulsion dG, src/xtb.cu:118-121 (text) */
+static void repulsion_grad(const or_calc* c, const basis_t* b, const double* xyz, double* grad)
+{
+  const tblite_amd_param* p = &c->par;
+  double cutoff2 = p->rep_cutoff * p->rep_cutoff;
+  for (int i = 0; i < b->nat; ++i) {
+    const tblite_amd_element* a = &p->elem[b->el[i]];
+    for (int j = 0; j < i; ++j) {
+      const tblite_amd_element* bb = &p->elem[b->el[j]];
+      double rij[3];
+      for (int k = 0; k < 3; ++k) rij[k] = xyz[3 * i + k] - xyz[3 * j + k];
+      double r2 = rij[0] * rij[0] + rij[1] * rij[1] + rij[2] * rij[2];
+      if (r2 > cutoff2 || r2 < 1.0e-12) continue;
+      double r1 = sqrt(r2);
+      double alpha = sqrt(a->arep * bb->arep);
+      double zeff = a->zeff * bb->zeff;
+      double kexp = (a->z <= 2 && bb->z <= 2) ? p->rep_kexp_light : p->rep_kexp;
+      double r1k = pow(r1, kexp);
+      double dE = zeff * exp(-alpha * r1k) / pow(r1, p->rep_rexp);
+      double g = -(alpha * r1k * kexp + p->rep_rexp) * dE / r2;
+      for (int k = 0; k < 3; ++k) {
+        grad[3 * i + k] += g * rij[k];
+        grad[3 * j + k] -= g * rij[k];
+      }
+    }
+  }
+}
+
+/* integral terms at fixed P, W and potentials (see the section head);
+ * vao: shell + atom potential of each AO.  dEdcn receives the CN
+ * derivative of the self energies. */
+static void integral_gradient(const or_calc* c, const basis_t* b, const double* xyz, const int* inl,
+                              const int* nnl, const int* nlat, double intcut, const double* se,
+                              const double* P, const double* W, const double* vao,
+                              const double* vdp, const double* vqp, double* grad, double* dEdcn)
+{
+  int nao = b->nao;
+  double stmp[25], dtmp[75], qtmp[150], ds[75], dd[225], dq[450];
+  for (int iat = 0; iat < b->nat; ++iat) {
+    int ei = b->el[iat], is = b->ish_at[iat];
+    for (int img = 0; img < nnl[iat]; ++img) {
+      int jat = nlat[inl[iat] + img];
+      if (jat == iat) continue;
+      int ej = b->el[jat], js = b->ish_at[jat];
+      double vec[3];
+      for (int k = 0; k < 3; ++k) vec[k] = xyz[3 * iat + k] - xyz[3 * jat + k];
+      double r2 = vec[0] * vec[0] + vec[1] * vec[1] + vec[2] * vec[2];
+      double r1 = sqrt(r2);
+      double rr = sqrt(r1 / (c->par.elem[ej].atomic_rad + c->par.elem[ei].atomic_rad));
+      double drr = rr / (2.0 * r2); /* d rr / d vec = drr * vec */
+      double gv[3] = {0.0, 0.0, 0.0};
+      for (int ish = 0; ish < b->nsh_at[iat]; ++ish) {
+        int ii = b->iao_sh[is + ish];
+        for (int jsh = 0; jsh < b->nsh_at[jat]; ++jsh) {
+          int jj = b->iao_sh[js + jsh];
+          const cgto_t* ci = &c->cgto[ei][ish];
+          const cgto_t* cj = &c->cgto[ej][jsh];
+          multipole_grad_cgto(cj, ci, r2, vec, intcut, stmp, dtmp, qtmp, ds, dd, dq);
+          double ki = c->par.elem[ei].shpoly[ish], kj = c->par.elem[ej].shpoly[jsh];
+          double shpoly = (1.0 + ki * rr) * (1.0 + kj * rr);
+          double dshpoly = (ki * (1.0 + kj * rr) + kj * (1.0 + ki * rr)) * drr;
+          double hs = hscale_of(c, ei, ish, ej, jsh);
+          double hav = 0.5 * (se[is + ish] + se[js + jsh]);
+          double hij = hav * hs * shpoly;
+          double dhij = hav * hs * dshpoly; /* times vec_m */
+          int nj = msao(cj->ang);
+          double psh = 0.0; /* sum P S over the block, for the CN chain */
+          for (int iao = 0; iao < msao(ci->ang); ++iao)
+            for (int jao = 0; jao < nj; ++jao) {
+              int ij = jao + nj * iao;
+              int a = jj + jao, bb = ii + iao;
+              double pab = P[a + (size_t)nao * bb], wab = W[a + (size_t)nao * bb];
+              double s = stmp[ij];
+              psh += pab * s;
+              const double* di = dtmp + 3 * ij;
+              const double* qi = qtmp + 6 * ij;
+              for (int m = 0; m < 3; ++m) {
+                double dsm = ds[m + 3 * ij];
+                double ddi[3], dqi[6], ddj[3], dqj[6];
+                for (int k = 0; k < 3; ++k) ddi[k] = dd[m + 3 * (k + 3 * ij)];
+                for (int k = 0; k < 6; ++k) dqi[k] = dq[m + 3 * (k + 6 * ij)];
+                shift_operator_deriv(vec, m, s, di, dsm, ddi, dqi, ddj, dqj);
+                double g = 2.0 * pab * (dsm * hij + s * dhij * vec[m]) - 2.0 * wab * dsm -
+                           pab * (vao[a] + vao[bb]) * dsm;
+                double gd = 0.0;
+                for (int k = 0; k < 3; ++k)
+                  gd += vdp[k + 3 * iat] * ddi[k] + vdp[k + 3 * jat] * ddj[k];
+                for (int k = 0; k < 6; ++k)
+                  gd += vqp[k + 6 * iat] * dqi[k] + vqp[k + 6 * jat] * dqj[k];
+                gv[m] += g - pab * gd;
+              }
+              (void)qi;
+            }
+          dEdcn[iat] -= psh * c->kcn[ei][ish] * hs * shpoly;
+          dEdcn[jat] -= psh * c->kcn[ej][jsh] * hs * shpoly;
+        }
+      }
+      for (int k = 0; k < 3; ++k) {
+        grad[3 * iat + k] += gv[k];
+        grad[3 * jat + k] -= gv[k];
+      }
+    }
+  }
+  /* on-site self energies */
+  for (int iat = 0; iat < b->nat; ++iat) {
+    int ei = b->el[iat], is = b->ish_at[iat];
+    for (int ish = 0; ish < b->nsh_at[iat]; ++ish) {
+      int ii = b->iao_sh[is + ish];
+      for (int iao = 0; iao < b->nao_sh[is + ish]; ++iao)
+        dEdcn[iat] -= P[(ii + iao) + (size_t)nao * (ii + iao)] * c->kcn[ei][ish];
+    }
+  }
+}
+
+/* ES2 at fixed shell charges: d gamma / dr = -r^(g-1) gamma^(1+g) */
+static void es2_grad(const or_calc* c, const basis_t* b, const double* xyz, const double* qsh,
+                     double* grad)
+{
+  double gexp = c->par.gexp;
+  for (int iat = 0; iat < b->nat; ++iat) {
+    int ei = b->el[iat], ii = b->ish_at[iat];
+    for (int jat = 0; jat < iat; ++jat) {
+      int ej = b->el[jat], jj = b->ish_at[jat];
+      double rij[3];
+      for (int k = 0; k < 3; ++k) rij[k] = xyz[3 * iat + k] - xyz[3 * jat + k];
+      double r1 = sqrt(rij[0] * rij[0] + rij[1] * rij[1] + rij[2] * rij[2]);
+      double r1g = pow(r1, gexp);
+      double dE = 0.0;
+      for (int ish = 0; ish < b->nsh_at[iat]; ++ish)
+        for (int jsh = 0; jsh < b->nsh_at[jat]; ++jsh) {
+          double gam = 0.5 * (c->hardness[ei][ish] + c->hardness[ej][jsh]);
+          double tmp = 1.0 / pow(r1g + pow(gam, -gexp), 1.0 / gexp);
+          dE -= qsh[ii + ish] * qsh[jj + jsh] * pow(r1, gexp - 1.0) * pow(tmp, 1.0 + gexp);
+        }
+      for (int k = 0; k < 3; ++k) {
+        grad[3 * iat + k] += dE * rij[k] / r1;
+        grad[3 * jat + k] -= dE * rij[k] / r1;
+      }
+    }
+  }
+}
+
+/* AES2 at fixed moments; dEdmrad receives the multipole-radius derivative */
+static void aes2_grad(const or_calc* c, const basis_t* b, const double* xyz, const double* mrad,
+                      const double* qat, const double* dpat, const double* qpat, double* grad,
+                      double* dEdmrad)
+{
+  int nat = b->nat;
+  double k3 = c->par.dmp3, k5 = c->par.dmp5;
+  for (int iat = 0; iat < nat; ++iat)
+    for (int jat = 0; jat < nat; ++jat) {
+      if (iat == jat) continue;
+      double u[3];
+      for (int k = 0; k < 3; ++k) u[k] = xyz[3 * iat + k] - xyz[3 * jat + k];
+      double r1 = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+      double g1 = 1.0 / r1, g3 = g1 * g1 * g1, g5 = g3 * g1 * g1;
+      double rr = 0.5 * (mrad[jat] + mrad[iat]) * g1;
+      double f3 = 1.0 / (1.0 + 6.0 * pow(rr, k3)), f5 = 1.0 / (1.0 + 6.0 * pow(rr, k5));
+      double df3 = -6.0 * k3 * pow(rr, k3 - 1.0) * f3 * f3; /* d f / d rr */
+      double df5 = -6.0 * k5 * pow(rr, k5 - 1.0) * f5 * f5;
+      /* d(g_n f)/dr = -n g_n f / r - g_n f'(rr) rr / r */
+      double d3f3 = -3.0 * g3 * f3 * g1 - g3 * df3 * rr * g1;
+      double d3f5 = -3.0 * g3 * f5 * g1 - g3 * df5 * rr * g1;
+      double d5f5 = -5.0 * g5 * f5 * g1 - g5 * df5 * rr * g1;
+      const double* dj = dpat + 3 * jat;
+      const double* di = dpat + 3 * iat;
+      const double* qj = qpat + 6 * jat;
+      double qi = qat[iat];
+      double dju = dj[0] * u[0] + dj[1] * u[1] + dj[2] * u[2];
+      double diu = di[0] * u[0] + di[1] * u[1] + di[2] * u[2];
+      double djdi = dj[0] * di[0] + dj[1] * di[1] + dj[2] * di[2];
+      double t[6] = {u[0] * u[0], 2 * u[0] * u[1], u[1] * u[1], 2 * u[0] * u[2], 2 * u[1] * u[2],
+                     u[2] * u[2]};
+      double qt = 0.0;
+      for (int k = 0; k < 6; ++k) qt += qj[k] * t[k];
+      double dqt[3] = {qj[0] * 2 * u[0] + qj[1] * 2 * u[1] + qj[3] * 2 * u[2],
+                       qj[1] * 2 * u[0] + qj[2] * 2 * u[1] + qj[4] * 2 * u[2],
+                       qj[3] * 2 * u[0] + qj[4] * 2 * u[1] + qj[5] * 2 * u[2]};
+      double radial = qi * dju * d3f3 + 0.5 * (djdi * d3f5 - 3.0 * d5f5 * dju * diu) +
+                      qi * d5f5 * qt;
+      double gv[3];
+      for (int k = 0; k < 3; ++k)
+        gv[k] = radial * u[k] * g1 + qi * dj[k] * g3 * f3 -
+                1.5 * g5 * f5 * (dj[k] * diu + di[k] * dju) + qi * g5 * f5 * dqt[k];
+      for (int k = 0; k < 3; ++k) {
+        grad[3 * iat + k] += gv[k];
+        grad[3 * jat + k] -= gv[k];
+      }
+      double dm = (qi * dju * g3 * df3 + 0.5 * (djdi * g3 - 3.0 * g5 * dju * diu) * df5 +
+                   qi * g5 * qt * df5) * 0.5 * g1;
+      dEdmrad[iat] += dm;
+      dEdmrad[jat] += dm;
+    }
+}
+
+/* d mrad / d CN of mp_radii */
+static void mp_radii_deriv(const or_calc* c, const basis_t* b, const double* cn, double* dmrad)
+{
+  const tblite_amd_param* p = &c->par;
+  for (int i = 0; i < b->nat; ++i) {
+    const tblite_amd_element* e = &p->elem[b->el[i]];
+    double t1 = exp(-p->mp_kexp * (cn[i] - e->mpvcn - p->mp_shift));
+    dmrad[i] = (p->mp_rmax - e->mprad) * p->mp_kexp * t1 / ((1.0 + t1) * (1.0 + t1));
+  }
+}
+
+/* D4 weights' CN derivative dg_r/dCN at charge q (the Gaussian weights of
+ * d4_weights; zero where the weights fall back to the largest-CN reference) */
+static void d4_weights_dcn(const or_calc* c, int e, double cn, double q, double* dgdcn)
+{
+  const tblite_amd_d4_param* d = &c->par.d4;
+  const tblite_amd_d4_element* de = &d->elem[e];
+  double gw[TBLITE_AMD_D4_MAXREF], dgw[TBLITE_AMD_D4_MAXREF];
+  d4_weights(c, e, cn, q, gw, dgw);
+  double norm = 0.0, dnorm = 0.0;
+  for (int r = 0; r < de->nref; ++r) {
+    double x = exp(-d->wf * (cn - de->cn[r]) * (cn - de->cn[r]));
+    norm += x;
+    dnorm += -2.0 * d->wf * (cn - de->cn[r]) * x;
+  }
+  for (int r = 0; r < de->nref; ++r) {
+    double x = exp(-d->wf * (cn - de->cn[r]) * (cn - de->cn[r]));
+    double w = x / norm;
+    double dw = (-2.0 * d->wf * (cn - de->cn[r]) * x * norm - x * dnorm) / (norm * norm);
+    if (!isfinite(w) || !isfinite(dw)) {
+      dgdcn[r] = 0.0;
+      continue;
+    }
+    /* g = w z(q): dg/dcn = dw z, z = g / w */
+    double qmod = q + de->zeff, z;
+    if (qmod < 0.0) z = exp(d->ga);
+    else z = exp(d->ga * (1.0 - exp(d->gc * de->eta * (1.0 - (de->q[r] + de->zeff) / qmod))));
+    dgdcn[r] = dw * z;
+  }
+}
+
+/* grad += sum_i dEdcn_i dCN4_i/dR (D4 CN, d4_ncoord) */
+static void d4_ncoord_grad(const or_calc* c, const basis_t* b, const double* xyz,
+                           const double* dEdcn, double* grad)
+{
+  const tblite_amd_d4_param* d = &c->par.d4;
+  for (int i = 0; i < b->nat; ++i)
+    for (int j = 0; j < i; ++j) {
+      double rij[3];
+      for (int k = 0; k < 3; ++k) rij[k] = xyz[3 * i + k] - xyz[3 * j + k];
+      double r2 = rij[0] * rij[0] + rij[1] * rij[1] + rij[2] * rij[2];
+      if (r2 > d->cn_cutoff * d->cn_cutoff || r2 < 1.0e-12) continue;
+      const tblite_amd_element* ei = &c->par.elem[b->el[i]];
+      const tblite_amd_element* ej = &c->par.elem[b->el[j]];
+      double r = sqrt(r2), rc = ei->rcov + ej->rcov;
+      double den = d->cn_k4 * exp(-pow(fabs(ei->en - ej->en) + d->cn_k5, 2) / d->cn_k6);
+      double t = d->cn_kn * (r - rc) / rc;
+      double dcnt = -den * d->cn_kn / (rc * sqrt(PI)) * exp(-t * t);
+      double g = (dEdcn[i] + dEdcn[j]) * dcnt / r;
+      for (int k = 0; k < 3; ++k) {
+        grad[3 * i + k] += g * rij[k];
+        grad[3 * j + k] -= g * rij[k];
+      }
+    }
+}
+
+/* D4 two-body at fixed charges: explicit pair term and dE/dCN4 */
+static void d4_twobody_grad(const or_calc* c, const basis_t* b, const double* xyz, const double* cn,
+                            const double* dmat, const double* qat, double* grad, double* dEdcn)
+{
+  const tblite_amd_d4_param* d = &c->par.d4;
+  int nat = b->nat, K = TBLITE_AMD_D4_MAXREF;
+  double* gw = (double*)calloc((size_t)nat * K, sizeof(double));
+  double* dgw = (double*)calloc((size_t)nat * K, sizeof(double));
+  double* gc = (double*)calloc((size_t)nat * K, sizeof(double));
+  for (int i = 0; i < nat; ++i) {
+    d4_weights(c, b->el[i], cn[i], qat[i], gw + K * i, dgw + K * i);
+    d4_weights_dcn(c, b->el[i], cn[i], qat[i], gc + K * i);
+  }
+  for (int i = 0; i < nat; ++i) {
+    int ei = b->el[i], ni = d->elem[ei].nref;
+    double vv[TBLITE_AMD_D4_MAXREF] = {0};
+    for (int j = 0; j < nat; ++j) {
+      int ej = b->el[j], nj = d->elem[ej].nref;
+      double dij = dmat[i * nat + j];
+      if (dij == 0.0) continue;
+      double c6 = 0.0;
+      for (int r = 0; r < ni; ++r)
+        for (int s = 0; s < nj; ++s) {
+          double cr = c->d4_c6[ei][ej][r][s] * gw[K * j + s];
+          vv[r] += dij * cr;
+          c6 += gw[K * i + r] * cr;
+        }
+      if (j < i) {
+        double rij[3];
+        for (int k = 0; k < 3; ++k) rij[k] = xyz[3 * i + k] - xyz[3 * j + k];
+        double r2 = rij[0] * rij[0] + rij[1] * rij[1] + rij[2] * rij[2];
+        double rr = 3.0 * d->elem[ei].r4r2 * d->elem[ej].r4r2;
+        double r0 = d->a1 * sqrt(rr) + d->a2;
+        double r06 = pow(r0, 6), r08 = r06 * r0 * r0;
+        double r6 = r2 * r2 * r2, r8 = r6 * r2;
+        /* d dmat / dr divided by r */
+        double dd = d->s6 * 6.0 * r2 * r2 / ((r6 + r06) * (r6 + r06)) +
+                    d->s8 * rr * 8.0 * r6 / ((r8 + r08) * (r8 + r08));
+        for (int k = 0; k < 3; ++k) {
+          grad[3 * i + k] += c6 * dd * rij[k];
+          grad[3 * j + k] -= c6 * dd * rij[k];
+        }
+      }
+    }
+    for (int r = 0; r < ni; ++r) dEdcn[i] += gc[K * i + r] * vv[r];
+  }
+  free(gw);
+  free(dgw);
+  free(gc);
+}
+
+/* D4 three-body (zero charges): geometric derivative and dE/dCN4 */
+static void d4_threebody_grad(const or_calc* c, const basis_t* b, const double* cn,
+                              const double* xyz, double* grad, double* dEdcn)
+{
+  const tblite_amd_d4_param* d = &c->par.d4;
+  int nat = b->nat, K = TBLITE_AMD_D4_MAXREF;
+  double* gw = (double*)calloc((size_t)nat * K, sizeof(double));
+  double* dgw = (double*)calloc((size_t)nat * K, sizeof(double));
+  double* gc = (double*)calloc((size_t)nat * K, sizeof(double));
+  double* c6 = (double*)calloc((size_t)nat * nat, sizeof(double));
+  double* dEdc6 = (double*)calloc((size_t)nat * nat, sizeof(double));
+  for (int i = 0; i < nat; ++i) {
+    d4_weights(c, b->el[i], cn[i], 0.0, gw + K * i, dgw + K * i);
+    d4_weights_dcn(c, b->el[i], cn[i], 0.0, gc + K * i);
+  }
+  for (int i = 0; i < nat; ++i)
+    for (int j = 0; j < nat; ++j) {
+      int ei = b->el[i], ej = b->el[j];
+      double acc = 0.0;
+      for (int r = 0; r < d->elem[ei].nref; ++r)
+        for (int s = 0; s < d->elem[ej].nref; ++s)
+          acc += gw[K * i + r] * c->d4_c6[ei][ej][r][s] * gw[K * j + s];
+      c6[i * nat + j] = acc;
+    }
+  double cut2 = d->disp3_cutoff * d->disp3_cutoff, a3 = d->alp / 3.0;
+  for (int i = 0; i < nat; ++i)
+    for (int j = 0; j < i; ++j)
+      for (int k = 0; k < j; ++k) {
+        double rij[3], rik[3], rjk[3];
+        for (int x = 0; x < 3; ++x) {
+          rij[x] = xyz[3 * j + x] - xyz[3 * i + x];
+          rik[x] = xyz[3 * k + x] - xyz[3 * i + x];
+          rjk[x] = xyz[3 * k + x] - xyz[3 * j + x];
+        }
+        double x2 = rij[0] * rij[0] + rij[1] * rij[1] + rij[2] * rij[2];
+        double y2 = rik[0] * rik[0] + rik[1] * rik[1] + rik[2] * rik[2];
+        double z2 = rjk[0] * rjk[0] + rjk[1] * rjk[1] + rjk[2] * rjk[2];
+        if (x2 > cut2 || y2 > cut2 || z2 > cut2) continue;
+        double ri = d->elem[b->el[i]].r4r2, rj = d->elem[b->el[j]].r4r2, rk = d->elem[b->el[k]].r4r2;
+        double r0p = (d->a1 * sqrt(3.0 * ri * rj) + d->a2) * (d->a1 * sqrt(3.0 * ri * rk) + d->a2) *
+                     (d->a1 * sqrt(3.0 * rj * rk) + d->a2);
+        double cij = c6[i * nat + j], cik = c6[i * nat + k], cjk = c6[j * nat + k];
+        double c9 = d->s9 * sqrt(fabs(cij * cik * cjk));
+        double pp = x2 * y2 * z2, rp = sqrt(pp);
+        double t = 6.0 * pow(r0p / rp, a3);
+        double fdmp = 1.0 / (1.0 + t);
+        double A = x2 + z2 - y2, B = x2 - z2 + y2, Cc = -x2 + z2 + y2;
+        double abc = A * B * Cc;
+        double ang = 0.375 * abc / pp + 1.0;
+        double dn = 1.0 / (rp * pp);
+        double et = c9 * ang * fdmp * dn;
+        /* partial derivatives by the squared distances x2 (ij), y2 (ik), z2 (jk) */
+        double dabc[3] = {B * Cc + A * Cc - A * B, -B * Cc + A * Cc + A * B, B * Cc - A * Cc + A * B};
+        double sq[3] = {x2, y2, z2}, de[3];
+        for (int m = 0; m < 3; ++m) {
+          double dang = 0.375 * (dabc[m] / pp - abc / (pp * sq[m]));
+          double dfd = fdmp * fdmp * t * (d->alp / 6.0) / sq[m];
+          double ddn = -1.5 * dn / sq[m];
+          de[m] = c9 * (dang * fdmp * dn + ang * dfd * dn + ang * fdmp * ddn);
+        }
+        for (int x = 0; x < 3; ++x) {
+          grad[3 * i + x] -= 2.0 * (de[0] * rij[x] + de[1] * rik[x]);
+          grad[3 * j + x] += 2.0 * (de[0] * rij[x] - de[2] * rjk[x]);
+          grad[3 * k + x] += 2.0 * (de[1] * rik[x] + de[2] * rjk[x]);
+        }
+        if (cij != 0.0) { dEdc6[i * nat + j] += 0.5 * et / cij; dEdc6[j * nat + i] += 0.5 * et / cij; }
+        if (cik != 0.0) { dEdc6[i * nat + k] += 0.5 * et / cik; dEdc6[k * nat + i] += 0.5 * et / cik; }
+        if (cjk != 0.0) { dEdc6[j * nat + k] += 0.5 * et / cjk; dEdc6[k * nat + j] += 0.5 * et / cjk; }
+      }
+  for (int i = 0; i < nat; ++i) {
+    int ei = b->el[i];
+    for (int j = 0; j < nat; ++j) {
+      double w = dEdc6[i * nat + j];
+      if (w == 0.0 || j == i) continue;
+      int ej = b->el[j];
+      double acc = 0.0;
+      for (int r = 0; r < d->elem[ei].nref; ++r)
+        for (int s = 0; s < d->elem[ej].nref; ++s)
+          acc += gc[K * i + r] * c->d4_c6[ei][ej][r][s] * gw[K * j + s];
+      dEdcn[i] += w * acc;
+    }
+  }
+  free(gw);
+  free(dgw);
+  free(gc);
+  free(c6);
+  free(dEdc6);
+}
+
+/* Classical terms at fixed moments, for stage checks: which = 0 repulsion,
+ * 1 electrostatics (ES2 + AES2 + ES3, with the multipole radii following
+ * the CN), 2 D4 two-body at the atomic charges of qsh, 3 D4 three-body.
+ * e = the term's energy, grad(3, nat) = its derivative. */
+int or_classical_gradient(const or_calc* calc, int nat, const int* num, const double* xyz, int which,
+                          const double* qsh, const double* dpat, const double* qpat, double* e,
+                          double* grad)
+{
+  basis_t b;
+  if (make_basis(calc, nat, num, &b)) return -1;
+  if ((which == 2 || which == 3) && !calc->par.d4.enabled) { free_basis(&b); return -1; }
+  double* ea = (double*)calloc(nat, sizeof(double));
+  double* cn = (double*)calloc(nat, sizeof(double));
+  double* dEdcn = (double*)calloc(nat, sizeof(double));
+  double* qat = (double*)calloc(nat, sizeof(double));
+  for (int s = 0; s < b.nsh; ++s) qat[b.sh2at[s]] += qsh[s];
+  for (int i = 0; i < 3 * nat; ++i) grad[i] = 0.0;
+  if (which == 0) {
+    repulsion(calc, &b, xyz, ea);
+    repulsion_grad(calc, &b, xyz, grad);
+  } else if (which == 1) {
+    ccache_t cc;
+    cc.amat = (double*)malloc(sizeof(double) * b.nsh * b.nsh);
+    cc.sd = (double*)malloc(sizeof(double) * 3 * nat * nat);
+    cc.dd = (double*)malloc(sizeof(double) * 9 * nat * nat);
+    cc.sq = (double*)malloc(sizeof(double) * 6 * nat * nat);
+    cc.mrad = (double*)malloc(sizeof(double) * nat);
+    double* dmr = (double*)calloc(nat, sizeof(double));
+    double* dEdm = (double*)calloc(nat, sizeof(double));
+    ncoord(calc, &b, xyz, cn);
+    mp_radii(calc, &b, cn, cc.mrad);
+    es2_matrix(calc, &b, xyz, cc.amat);
+    aes2_matrices(calc, &b, xyz, cc.mrad, cc.sd, cc.dd, cc.sq);
+    coulomb_energy(calc, &b, &cc, qsh, qat, dpat, qpat, ea);
+    es2_grad(calc, &b, xyz, qsh, grad);
+    aes2_grad(calc, &b, xyz, cc.mrad, qat, dpat, qpat, grad, dEdm);
+    mp_radii_deriv(calc, &b, cn, dmr);
+    for (int i = 0; i < nat; ++i) dEdcn[i] = dEdm[i] * dmr[i];
+    ncoord_grad(calc, &b, xyz, dEdcn, grad);
+    free(cc.amat); free(cc.sd); free(cc.dd); free(cc.sq); free(cc.mrad);
+    free(dmr); free(dEdm);
+  } else if (which == 2 || which == 3) {
+    double* dmat = (double*)calloc((size_t)nat * nat, sizeof(double));
+    d4_ncoord(calc, &b, xyz, cn);
+    d4_dispmat(calc, &b, xyz, dmat);
+    if (which == 2) {
+      d4_twobody(calc, &b, cn, dmat, qat, ea, NULL);
+      d4_twobody_grad(calc, &b, xyz, cn, dmat, qat, grad, dEdcn);
+    } else {
+      d4_threebody(calc, &b, cn, xyz, ea);
+      d4_threebody_grad(calc, &b, cn, xyz, grad, dEdcn);
+    }
+    d4_ncoord_grad(calc, &b, xyz, dEdcn, grad);
+    free(dmat);
+  }
+  *e = 0.0;
+  for (int i = 0; i < nat; ++i) *e += ea[i];
+  free(ea); free(cn); free(dEdcn); free(qat);
+  free_basis(&b);
+  return 0;
+}
+
+/* Integral terms at fixed P (density), W (energy-weighted density) and
+ * potentials vsh(nsh), vdp(3,nat), vqp(6,nat), for stage checks:
+ * e = sum P H0 - sum W S + sum_s vsh_s q_s + sum_a vdp_a dpat_a + vqp_a qpat_a
+ * with q_s, dpat, qpat the Mulliken moments of P (reference populations
+ * left out), grad(3, nat) = its derivative including the CN chain of the
+ * self energies.  P, W column-major nao x nao. */
+int or_integral_gradient(const or_calc* calc, int nat, const int* num, const double* xyz,
+                         double accuracy, const double* P, const double* W, const double* vsh,
+                         const double* vdp, const double* vqp, double* e, double* grad)
+{
+  basis_t b;
+  if (make_basis(calc, nat, num, &b)) return -1;
+  int nao = b.nao;
+  size_t n2 = (size_t)nao * nao;
+  double* cn = (double*)calloc(nat, sizeof(double));
+  double* se = (double*)calloc(b.nsh, sizeof(double));
+  double* dEdcn = (double*)calloc(nat, sizeof(double));
+  double* vao = (double*)calloc(nao, sizeof(double));
+  double *S = (double*)malloc(sizeof(double) * n2), *D = (double*)malloc(sizeof(double) * 3 * n2);
+  double *Q = (double*)malloc(sizeof(double) * 6 * n2), *H = (double*)malloc(sizeof(double) * n2);
+  ncoord(calc, &b, xyz, cn);
+  for (int i = 0; i < nat; ++i) {
+    int el = b.el[i], ii = b.ish_at[i];
+    for (int s = 0; s < b.nsh_at[i]; ++s) se[ii + s] = calc->selfenergy[el][s] - calc->kcn[el][s] * cn[i];
+  }
+  for (int s = 0; s < b.nsh; ++s)
+    for (int k = 0; k < b.nao_sh[s]; ++k) vao[b.iao_sh[s] + k] = vsh[s];
+  double cutoff = get_cutoff(b.min_alpha, accuracy);
+  int cap = nat * (nat + 1) / 2 + 1;
+  int* inl = (int*)malloc(sizeof(int) * nat);
+  int* nnl = (int*)malloc(sizeof(int) * nat);
+  int* nlat = (int*)malloc(sizeof(int) * cap);
+  or_adjlist(nat, xyz, cutoff, inl, nnl, nlat, cap);
+  hamiltonian(calc, &b, xyz, inl, nnl, nlat, integral_cutoff(accuracy), se, S, D, Q, H);
+  double acc = 0.0;
+  for (int iao = 0; iao < nao; ++iao) {
+    int iat = b.ao2at[iao];
+    for (int jao = 0; jao < nao; ++jao) {
+      size_t ji = jao + (size_t)nao * iao;
+      double p = P[ji];
+      acc += p * H[ji] - W[ji] * S[ji] - vao[iao] * p * S[ji];
+      for (int k = 0; k < 3; ++k) acc -= vdp[k + 3 * iat] * p * D[k + 3 * ji];
+      for (int k = 0; k < 6; ++k) acc -= vqp[k + 6 * iat] * p * Q[k + 6 * ji];
+    }
+  }
+  *e = acc;
+  for (int i = 0; i < 3 * nat; ++i) grad[i] = 0.0;
+  integral_gradient(calc, &b, xyz, inl, nnl, nlat, integral_cutoff(accuracy), se, P, W, vao, vdp,
+                    vqp, grad, dEdcn);
+  ncoord_grad(calc, &b, xyz, dEdcn, grad);
+  free(inl); free(nnl); free(nlat);
+  free(cn); free(se); free(dEdcn); free(vao); free(S); free(D); free(Q); free(H);
+  free_basis(&b);
+  return 0;
+}
+
 int or_singlepoint(const or_calc* calc, int nat, const int* num, const double* xyz, double charge,
                    int uhf, double accuracy, double kt, int max_iter, double damping,
                    or_result* res)
@@ -1755,7 +2481,6 @@ int or_singlepoint(const or_calc* calc, int nat, const int* num, const double* x
   int* nlat = (int*)malloc(sizeof(int) * cap);
   or_adjlist(nat, xyz, cutoff, inl, nnl, nlat, cap);
   hamiltonian(calc, &b, xyz, inl, nnl, nlat, integral_cutoff(accuracy), se, S, D, Q, H);
-  free(inl); free(nnl); free(nlat);
 
   /* SCF */
   int ndim = nsh + 9 * nat;
@@ -1893,6 +2618,63 @@ int or_singlepoint(const or_calc* calc, int nat, const int* num, const double* x
     if (!isfinite(esum)) { status |= TBLITE_AMD_STATUS_NAN; rc = TBLITE_AMD_ERR_NAN; break; }
   }
   if (!converged && rc == 0) { status |= TBLITE_AMD_STATUS_NOT_CONVERGED; rc = TBLITE_AMD_ERR_NOT_CONVERGED; }
+  if (res && (res->gradient || res->sigma) && (rc == 0 || rc == TBLITE_AMD_ERR_NOT_CONVERGED)) {
+    /* analytic gradient of the final energy: potentials of the final
+     * moments, W from the last orbitals (CR holds C row-major) */
+    double* grad = (double*)calloc(3 * nat, sizeof(double));
+    double* W = (double*)malloc(sizeof(double) * n2);
+    double* fe = (double*)malloc(sizeof(double) * nao);
+    double* dEdcn = (double*)calloc(nat, sizeof(double));
+    double* dEdcn4 = (double*)calloc(nat, sizeof(double));
+    double* dEdm = (double*)calloc(nat, sizeof(double));
+    double* dmr = (double*)calloc(nat, sizeof(double));
+    for (int k = 0; k < nao; ++k) fe[k] = focc[k] * emo[k];
+#pragma omp parallel for schedule(dynamic, 8) if (nao > 256)
+    for (int j = 0; j < nao; ++j) {
+      const double* cj = CR + (size_t)nao * j;
+      for (int i = 0; i < nao; ++i) {
+        const double* ci = CR + (size_t)nao * i;
+        double s = 0.0;
+        for (int k = 0; k < nao; ++k) s += ci[k] * fe[k] * cj[k];
+        W[i + (size_t)nao * j] = s;
+      }
+    }
+    memset(vsh, 0, sizeof(double) * nsh);
+    memset(vat, 0, sizeof(double) * nat);
+    memset(vdp, 0, sizeof(double) * 3 * nat);
+    memset(vqp, 0, sizeof(double) * 6 * nat);
+    coulomb_potential(calc, &b, &cc, qsh, qat, dpat, qpat, vsh, vat, vdp, vqp);
+    if (d4on) d4_twobody(calc, &b, cn4, dmat, qat, NULL, vat);
+    for (int i = 0; i < nat; ++i)
+      for (int s = 0; s < b.nsh_at[i]; ++s) vsh[b.ish_at[i] + s] += vat[i];
+    for (int s = 0; s < nsh; ++s)
+      for (int k = 0; k < b.nao_sh[s]; ++k) vao[b.iao_sh[s] + k] = vsh[s];
+    repulsion_grad(calc, &b, xyz, grad);
+    integral_gradient(calc, &b, xyz, inl, nnl, nlat, integral_cutoff(accuracy), se, P, W, vao, vdp,
+                      vqp, grad, dEdcn);
+    es2_grad(calc, &b, xyz, qsh, grad);
+    aes2_grad(calc, &b, xyz, cc.mrad, qat, dpat, qpat, grad, dEdm);
+    mp_radii_deriv(calc, &b, cn, dmr);
+    for (int i = 0; i < nat; ++i) dEdcn[i] += dEdm[i] * dmr[i];
+    ncoord_grad(calc, &b, xyz, dEdcn, grad);
+    if (d4on) {
+      d4_twobody_grad(calc, &b, xyz, cn4, dmat, qat, grad, dEdcn4);
+      d4_threebody_grad(calc, &b, cn4, xyz, grad, dEdcn4);
+      d4_ncoord_grad(calc, &b, xyz, dEdcn4, grad);
+    }
+    if (res->gradient) memcpy(res->gradient, grad, sizeof(double) * 3 * nat);
+    if (res->sigma) {
+      /* strain derivative of a molecule: sum_a g_a (x) R_a */
+      for (int x = 0; x < 3; ++x)
+        for (int y = 0; y < 3; ++y) {
+          double acc = 0.0;
+          for (int i = 0; i < nat; ++i) acc += grad[3 * i + x] * xyz[3 * i + y];
+          res->sigma[x + 3 * y] = acc;
+        }
+    }
+    free(grad); free(W); free(fe); free(dEdcn); free(dEdcn4); free(dEdm); free(dmr);
+  }
+  free(inl); free(nnl); free(nlat);
   double er = 0.0, ed2 = 0.0, ed3 = 0.0;
   for (int i = 0; i < nat; ++i) {
     er += erep[i];

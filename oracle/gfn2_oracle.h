@@ -42,6 +42,8 @@ typedef struct {
   double* focc;   /* nao */
   double* density; /* nao*nao, column-major: P of the last SCF iteration */
   double* coeff;   /* nao*nao, column-major: coeff(:, k) = level k (ascending) */
+  double* gradient; /* 3*nat: dE/dR in Eh/bohr of the final energy */
+  double* sigma;    /* 9: strain derivative sum_a g_a (x) R_a, column-major */
 } or_result;
 
 /* Full singlepoint (xtb_singlepoint, src/xtb.cu:188-434). kt in Eh. */
@@ -103,6 +105,24 @@ double or_entropy(int n, const double* focc, double kt);
 int or_coulomb_eval(const or_calc* calc, int nat, const int* num, const double* xyz,
                     const double* qsh, const double* dpat, const double* qpat, double* e,
                     double* vsh, double* vat, double* vdp, double* vqp);
+
+/* ---- gradient stages ----
+ * Contracted multipole block and its derivatives by vec = R_i - R_j:
+ * dS(3, sj, si), dD(3, 3, sj, si), dQ(3, 6, sj, si) (derivative index first). */
+int or_multipole_grad_cgto(const or_calc* calc, int zj, int jsh, int zi, int ish, double r2,
+                           const double vec[3], double intcut, double* S, double* D, double* Q,
+                           double* dS, double* dD, double* dQ);
+/* Classical term energy and gradient at fixed moments: which = 0 repulsion,
+ * 1 electrostatics, 2 D4 two-body, 3 D4 three-body. */
+int or_classical_gradient(const or_calc* calc, int nat, const int* num, const double* xyz, int which,
+                          const double* qsh, const double* dpat, const double* qpat, double* e,
+                          double* grad);
+/* Integral terms at fixed P, W and potentials: e = sum P H0 - sum W S
+ * + sum vsh q + sum vdp dpat + sum vqp qpat (Mulliken moments of P without
+ * reference populations) and its gradient. */
+int or_integral_gradient(const or_calc* calc, int nat, const int* num, const double* xyz,
+                         double accuracy, const double* P, const double* W, const double* vsh,
+                         const double* vdp, const double* vqp, double* e, double* grad);
 
 /* Mixer on an affine map F(q) = A q + b (test hook for broyden_next). */
 int or_broyden_linear(int n, const double* A, const double* b, const double* q0, double damp,

@@ -34,6 +34,7 @@ class Result(C.Structure):
         ("dpat", C.POINTER(C.c_double)), ("qpat", C.POINTER(C.c_double)),
         ("emo", C.POINTER(C.c_double)), ("focc", C.POINTER(C.c_double)),
         ("density", C.POINTER(C.c_double)), ("coeff", C.POINTER(C.c_double)),
+        ("gradient", C.POINTER(C.c_double)), ("sigma", C.POINTER(C.c_double)),
     ]
 
 
@@ -102,6 +103,12 @@ def lib():
         L.or_d4_c6ref.restype = C.c_double
         L.or_d4_c6ref.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
         L.or_d4_eval.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)] + [C.POINTER(C.c_double)] * 5
+        L.or_multipole_grad_cgto.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                             C.POINTER(C.c_double), C.c_double] + [C.POINTER(C.c_double)] * 6
+        L.or_classical_gradient.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                            C.c_int] + [C.POINTER(C.c_double)] * 5
+        L.or_integral_gradient.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                           C.c_double] + [C.POINTER(C.c_double)] * 7
         _lib = L
     return _lib
 
@@ -132,9 +139,10 @@ class Oracle:
         return len(num), nsh.value, nao.value
 
     def singlepoint(self, num, xyz, charge=0.0, uhf=0, accuracy=1.0, temperature=300.0,
-                    max_iter=250, damping=0.4, wavefunction=False):
+                    max_iter=250, damping=0.4, wavefunction=False, gradient=False):
         """wavefunction=True also returns density and coeff (nao x nao numpy
-        arrays indexed [row, column]; coeff[:, k] is level k)."""
+        arrays indexed [row, column]; coeff[:, k] is level k); gradient=True
+        returns gradient (nat, 3) = dE/dR in Eh/bohr and sigma (3, 3)."""
         num, xyz = self._mol(num, xyz)
         nat, nsh, nao = self.dims(num)
         out = dict(qsh=np.zeros(nsh), qat=np.zeros(nat), dpat=np.zeros((nat, 3)),
@@ -143,6 +151,9 @@ class Oracle:
             # column-major in C: allocated as the transpose, read back as [row, col]
             out["density"] = np.zeros((nao, nao))
             out["coeff"] = np.zeros((nao, nao))
+        if gradient:
+            out["gradient"] = np.zeros((nat, 3))
+            out["sigma"] = np.zeros((3, 3))
         r = Result()
         for k in out:
             setattr(r, k, _dp(out[k]))
@@ -154,6 +165,8 @@ class Oracle:
         if wavefunction:
             out["density"] = out["density"].T.copy()
             out["coeff"] = out["coeff"].T.copy()
+        if gradient:
+            out["sigma"] = out["sigma"].T.copy()
         return out
 
     def singlepoint_many(self, mols, accuracy=1.0, temperature=300.0, max_iter=250,
@@ -298,6 +311,42 @@ class Oracle:
         self.L.or_multipole_cgto(self.calc, zj, jsh, zi, ish, float(vec @ vec), _dp(vec), intcut,
                                  _dp(S), _dp(D), _dp(Q))
         return S, D, Q
+
+
+    def multipole_grad_cgto(self, zj, jsh, zi, ish, vec, intcut=25.0):
+        """S(25), D(75), Q(150) as multipole_cgto and their derivatives by
+        vec: dS (3, 25), dD (3, 75), dQ (3, 150) (component-major inside)."""
+        vec = np.ascontiguousarray(vec, dtype=np.float64)
+        S, D, Q = np.zeros(25), np.zeros(75), np.zeros(150)
+        dS, dD, dQ = np.zeros(75), np.zeros(225), np.zeros(450)
+        self.L.or_multipole_grad_cgto(self.calc, zj, jsh, zi, ish, float(vec @ vec), _dp(vec), intcut,
+                                      _dp(S), _dp(D), _dp(Q), _dp(dS), _dp(dD), _dp(dQ))
+        return S, D, Q, dS.reshape(-1, 3).T.copy(), dD.reshape(-1, 3).T.copy(), dQ.reshape(-1, 3).T.copy()
+
+    def classical_gradient(self, num, xyz, which, qsh, dpat, qpat):
+        """Energy and gradient (nat, 3) of one classical term at fixed
+        moments: which = 'repulsion', 'coulomb', 'd4_twobody', 'd4_threebody'."""
+        w = {"repulsion": 0, "coulomb": 1, "d4_twobody": 2, "d4_threebody": 3}[which]
+        num, xyz = self._mol(num, xyz)
+        f = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        e = np.zeros(1)
+        g = np.zeros((len(num), 3))
+        if self.L.or_classical_gradient(self.calc, len(num), _ip(num), _dp(xyz), w, _dp(f(qsh)),
+                                        _dp(f(dpat)), _dp(f(qpat)), _dp(e), _dp(g)):
+            raise ValueError("term unavailable")
+        return e[0], g
+
+    def integral_gradient(self, num, xyz, P, W, vsh, vdp, vqp, accuracy=1.0):
+        """Energy and gradient of the integral terms at fixed P, W (nao x nao,
+        symmetric) and potentials vsh (nsh), vdp (nat, 3), vqp (nat, 6)."""
+        num, xyz = self._mol(num, xyz)
+        f = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        e = np.zeros(1)
+        g = np.zeros((len(num), 3))
+        self.L.or_integral_gradient(self.calc, len(num), _ip(num), _dp(xyz), accuracy,
+                                    _dp(f(np.asarray(P).T)), _dp(f(np.asarray(W).T)), _dp(f(vsh)),
+                                    _dp(f(vdp)), _dp(f(vqp)), _dp(e), _dp(g))
+        return e[0], g
 
 
 def sygvd(F, S):

@@ -104,16 +104,18 @@ def copy_probe(mib, reps=3):
     print(json.dumps({"copy_probe_bytes": n * 8, "reps": reps}))
 
 
-def replay(srv, mols, arrivals):
+def replay(srv, mols, arrivals, tag=""):
     """Submit mols[k] at host time arrivals[k] (s from start), poll until all are
     done; returns per-request latency (ms, arrival to collection) and the span
-    from the first arrival to the last completion (s)."""
+    from the first arrival to the last completion (s).  Prints a progress line
+    about every 20 s."""
     n = len(mols)
     tickets = {}
     lat = np.zeros(n)
     done = 0
     k = 0
     t0 = time.perf_counter()
+    t_report = t0
     while done < n:
         now = time.perf_counter() - t0
         while k < n and arrivals[k] <= now:
@@ -125,6 +127,10 @@ def replay(srv, mols, arrivals):
             m = tickets.pop(t)
             lat[m] = (tnow - arrivals[m]) * 1e3
             done += 1
+        if tnow + t0 - t_report > 20.0:
+            t_report = tnow + t0
+            print(f"[bench] {tag} {tnow:.0f} s: {k}/{n} submitted, {done} done", file=sys.stderr,
+                  flush=True)
         if not res and k < n and arrivals[k] > tnow + 0.0005:
             time.sleep(min(arrivals[k] - tnow, 0.0005))
     return lat, time.perf_counter() - t0 - arrivals[0]
@@ -151,42 +157,52 @@ def latency_row(lat, mols, offered, span, arrivals):
 
 def serve_bench(args):
     """Config 4: closed-loop saturation throughput, then open-loop Poisson
-    arrivals at fractions of it with p50 / p99 latency, beside the plain
-    one-molecule-per-call loop."""
+    arrivals at fractions of it (or at the rates given) with p50 / p99
+    latency, beside the plain one-molecule-per-call loop."""
     t0 = time.perf_counter()
-    mols = workloads.config4(args.requests)
+    workers = min(16, os.cpu_count() or 1)
+    mols = workloads.config4(args.requests, workers=workers)
     t_gen = time.perf_counter() - t0
     nat = np.array([len(m[0]) for m in mols])
     print(f"[bench] config4: {len(mols)} requests, atoms {nat.min()}..{nat.max()} "
-          f"(generated in {t_gen:.1f} s)", file=sys.stderr, flush=True)
+          f"(generated in {t_gen:.1f} s on {workers} processes)", file=sys.stderr, flush=True)
     eng = tblite_amd.Engine(device=0)
     srv = tblite_amd.Server(eng, slots=args.slots, workspace_bytes=args.workspace_gb * 1e9)
     # warm-up on a few requests (kernel loading, workspace first touch)
-    replay(srv, mols[:16], np.zeros(16))
-    lat, span = replay(srv, mols, np.zeros(len(mols)))
-    sat = len(mols) / span
-    print(f"[bench] saturation: {sat:.1f} singlepoints/s (all {len(mols)} queued at t=0), "
-          f"p50 {np.percentile(lat, 50):.0f} ms p99 {np.percentile(lat, 99):.0f} ms",
-          file=sys.stderr, flush=True)
+    replay(srv, mols[:16], np.zeros(16), "warm-up")
+    sat = None
+    if args.saturation is not None:
+        sat = args.saturation
+    else:
+        lat, span = replay(srv, mols, np.zeros(len(mols)), "closed loop")
+        sat = len(mols) / span
+        print(f"[bench] saturation: {sat:.1f} singlepoints/s (all {len(mols)} queued at t=0), "
+              f"p50 {np.percentile(lat, 50):.0f} ms p99 {np.percentile(lat, 99):.0f} ms",
+              file=sys.stderr, flush=True)
     rng = np.random.default_rng(40)
     sweep = []
-    for frac in (0.25, 0.5, 0.8):
+    fracs = [float(f) for f in args.fractions.split(",") if f] if args.fractions else []
+    for frac in fracs:
         rate = frac * sat
         m = min(len(mols), max(100, int(rate * args.seconds_per_rate)))
         arr = np.cumsum(rng.exponential(1.0 / rate, m))
         arr -= arr[0]
-        lat, span = replay(srv, mols[:m], arr)
+        lat, span = replay(srv, mols[:m], arr, f"rate {rate:.1f}/s")
         row = latency_row(lat, mols[:m], rate, span, arr)
+        row["fraction_of_saturation"] = frac
         sweep.append(row)
         print(f"[bench] rate {rate:.1f}/s: {row}", file=sys.stderr, flush=True)
     stats = srv.stats()
     srv.close()
     k = min(args.per_molecule_sample, len(mols))
-    t = time.perf_counter()
-    for m in range(k):
-        eng.singlepoint(mols[m][0], mols[m][1], wavefunction=False)
-    per_mol = k / (time.perf_counter() - t)
-    best = sweep[-1]
+    per_mol = None
+    if k > 0:
+        pick = _spread(len(mols), k)
+        t = time.perf_counter()
+        for m in pick:
+            eng.singlepoint(mols[m][0], mols[m][1], wavefunction=False)
+        per_mol = len(pick) / (time.perf_counter() - t)
+    best = sweep[-1] if sweep else {"p50_ms": None, "p99_ms": None, "offered_per_s": None}
     line = {
         "metric": "singlepoints/s + p99 latency, batched vs per-molecule, |dE|<=1e-6 Eh",
         "value": round(sat, 2),
@@ -206,9 +222,10 @@ def serve_bench(args):
                        "at_offered_per_s": best["offered_per_s"]},
         "roofline": None,
         "cpu_baseline": None,
-        "extra": {"sweep": sweep, "server": stats,
-                  "per_molecule_path": {"value": round(per_mol, 2), "unit": "singlepoints/s",
-                                        "sample": k, "speedup_served": round(sat / per_mol, 2)}},
+        "extra": {"sweep": sweep, "server": stats, "saturation_given": args.saturation is not None,
+                  "per_molecule_path": {"value": round(per_mol, 2) if per_mol else None,
+                                        "unit": "singlepoints/s", "sample": k,
+                                        "speedup_served": round(sat / per_mol, 2) if per_mol else None}},
     }
     print(json.dumps(line))
 
@@ -272,7 +289,13 @@ def main():
     ap.add_argument("--requests", type=int, default=1000, help="config4: number of requests")
     ap.add_argument("--slots", type=int, default=4, help="config4: concurrent batches")
     ap.add_argument("--workspace-gb", type=float, default=48.0, help="config4: device workspace")
-    ap.add_argument("--seconds-per-rate", type=float, default=15.0)
+    ap.add_argument("--seconds-per-rate", type=float, default=15.0,
+                    help="config4: open-loop requests per rate = rate x this (at most --requests)")
+    ap.add_argument("--fractions", default="0.25,0.5,0.8",
+                    help="config4: open-loop rates as fractions of the saturation rate")
+    ap.add_argument("--saturation", type=float, default=None,
+                    help="config4: saturation rate (singlepoints/s) from an earlier run; skips "
+                         "the closed-loop measurement")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device copy of this many MiB (counter calibration) and exit")
     args = ap.parse_args()

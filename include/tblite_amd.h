@@ -196,8 +196,9 @@ int tblite_amd_get_wavefunction_array(tblite_amd_wavefunction wfn, const char* n
                                       int* count);
 
 /* Singlepoint: replaces xtb_singlepoint (src/xtb.cu:188-434).  Energy in Eh;
- * gradient/sigma are not computed (the reference never fills them either).
- * accuracy <= 0 uses the calculator's accuracy.  iterations may be NULL. */
+ * tblite_amd_xtb_singlepoint_gradient (below) is the form that also fills
+ * gradient and sigma.  accuracy <= 0 uses the calculator's accuracy.
+ * iterations may be NULL. */
 int tblite_amd_xtb_singlepoint(tblite_amd_context ctx, tblite_amd_structure mol,
                                tblite_amd_calculator calc, tblite_amd_wavefunction wfn,
                                double accuracy, double* energy, int* iterations, int verbosity);
@@ -224,6 +225,36 @@ int tblite_amd_singlepoint_flat(tblite_amd_context ctx, tblite_amd_calculator ca
                                 const int* mol_nat, const int* num, const double* xyz,
                                 const double* charge, const int* uhf, double accuracy,
                                 double* energies, int* iterations, int* status);
+
+/* Analytic gradients.  The reference's xtb_singlepoint takes gradient(3,nat)
+ * and sigma(3,3) arguments (src/xtb.cu:188-197) and sketches the FP64
+ * gradient integrals (multipole_grad_cgto, src/multipole.cu:846-981), the
+ * repulsion derivative (src/xtb.cu:118-121) and the CN derivative
+ * (src/coulomb.cu:289-292) without filling them.  These entry points return
+ * dE/dR of the converged SCF energy (Eh/bohr; the free energy is stationary
+ * in the orbitals and occupations, so the derivative needs no orbital
+ * response) and the strain derivative sigma = sum_a g_a (x) R_a (3 x 3,
+ * column-major, Eh).
+ *
+ * singlepoint_flat_gradient: tblite_amd_singlepoint_flat plus gradients
+ * (3, total atoms, in the atom order of xyz; required) and sigmas (9, nmol;
+ * may be NULL).  A molecule with a non-zero status still gets the gradient
+ * of its last SCF iteration.
+ * xtb_singlepoint_gradient: tblite_amd_xtb_singlepoint plus gradient(3,nat)
+ * (required) and sigma(3,3) (may be NULL).
+ * batch_get_gradient: gradient and sigma of molecule imol of the last
+ * gradient call on the context (either may be NULL). */
+int tblite_amd_singlepoint_flat_gradient(tblite_amd_context ctx, tblite_amd_calculator calc,
+                                         int nmol, const int* mol_nat, const int* num,
+                                         const double* xyz, const double* charge, const int* uhf,
+                                         double accuracy, double* energies, double* gradients,
+                                         double* sigmas, int* iterations, int* status);
+int tblite_amd_xtb_singlepoint_gradient(tblite_amd_context ctx, tblite_amd_structure mol,
+                                        tblite_amd_calculator calc, tblite_amd_wavefunction wfn,
+                                        double accuracy, double* energy, double* gradient,
+                                        double* sigma, int* iterations, int verbosity);
+int tblite_amd_batch_get_gradient(tblite_amd_context ctx, int imol, double* gradient,
+                                  double* sigma);
 
 /* No-allocation variant of tblite_amd_singlepoint_flat, with the same
  * arguments and results.  No counterpart in the reference: it is for callers
@@ -280,7 +311,12 @@ int tblite_amd_batch_results(tblite_amd_context ctx, double* energies, int* iter
  * finished results (ticket, energy in Eh, SCF iterations, status bits,
  * latency from submit to collection in ms).  Calls on one server must come
  * from one host thread.  Requests keep their wavefunction and mixer state on
- * the device until their batch ends.  No counterpart in the reference (one
+ * the device until their batch ends.  Under light load (two or more slots,
+ * at most two large requests waiting, none on the multi-workgroup path
+ * already) a large
+ * request for which the tuning's cost model puts the multi-workgroup path at
+ * least twice ahead of one workgroup runs alone on that path, its SCF host
+ * loop on a worker thread of the server (poll() still never blocks).  No counterpart in the reference (one
  * molecule per GPU thread, src/xtb.cu:479-497). */
 typedef struct tblite_amd_server_s* tblite_amd_server;
 tblite_amd_server tblite_amd_new_server(tblite_amd_context ctx, tblite_amd_calculator calc,
@@ -291,8 +327,10 @@ int tblite_amd_server_submit(tblite_amd_server srv, int nat, const int* num, con
 /* Returns the number of results written (>= 0) or < 0 on error. */
 int tblite_amd_server_poll(tblite_amd_server srv, int capacity, long long* tickets,
                            double* energies, int* iterations, int* status, double* latency_ms);
+/* grid_requests: requests run alone on the multi-workgroup path so far. */
 int tblite_amd_server_stats(tblite_amd_server srv, long long* submitted, long long* completed,
-                            int* queued, int* busy_slots, long long* batches);
+                            int* queued, int* busy_slots, long long* batches,
+                            long long* grid_requests);
 
 /* Workspace accounting of the serving admission, on the host (no device):
  * need_bytes receives the device workspace the engine allocates for one batch

@@ -1813,8 +1813,10 @@ int or_multipole_grad_cgto(const or_calc* calc, int zj, int jsh, int zi, int ish
                            const double vec[3], double intcut, double* S, double* D, double* Q,
                            double* dS, double* dD, double* dQ)
 {
-  int ej = calc->zmap[zj], ei = calc->zmap[zi];
-  if (ej < 0 || ei < 0) return -1;
+  int ej = (zj > 0 && zj < 120) ? calc->zmap[zj] : -1, ei = (zi > 0 && zi < 120) ? calc->zmap[zi] : -1;
+  if (ej < 0 || ei < 0 || jsh < 0 || ish < 0 || jsh >= calc->par.elem[ej].nsh ||
+      ish >= calc->par.elem[ei].nsh)
+    return -1;
   multipole_grad_cgto(&calc->cgto[ej][jsh], &calc->cgto[ei][ish], r2, vec, intcut, S, D, Q, dS, dD,
                       dQ);
   return 0;

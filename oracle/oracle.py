@@ -319,8 +319,9 @@ class Oracle:
         vec = np.ascontiguousarray(vec, dtype=np.float64)
         S, D, Q = np.zeros(25), np.zeros(75), np.zeros(150)
         dS, dD, dQ = np.zeros(75), np.zeros(225), np.zeros(450)
-        self.L.or_multipole_grad_cgto(self.calc, zj, jsh, zi, ish, float(vec @ vec), _dp(vec), intcut,
-                                      _dp(S), _dp(D), _dp(Q), _dp(dS), _dp(dD), _dp(dQ))
+        if self.L.or_multipole_grad_cgto(self.calc, zj, jsh, zi, ish, float(vec @ vec), _dp(vec), intcut,
+                                         _dp(S), _dp(D), _dp(Q), _dp(dS), _dp(dD), _dp(dQ)):
+            raise ValueError("no such shell")
         return S, D, Q, dS.reshape(-1, 3).T.copy(), dD.reshape(-1, 3).T.copy(), dQ.reshape(-1, 3).T.copy()
 
     def classical_gradient(self, num, xyz, which, qsh, dpat, qpat):

@@ -216,11 +216,32 @@ struct BatchDev {
   // eigensolver of the LDS-resident kernels: 1 Householder tridiagonalisation
   // + bisection + twisted-factorisation eigenvectors, 0 warm-started Jacobi
   int lds_eig;
+  // analytic gradient (all null unless a gradient is requested): per atom
+  // dE/dR (3 each), dE/dCN of the GFN and of the D4 coordination number, the
+  // atomic, dipole and quadrupole potentials of the final moments (1, 3, 6
+  // each) and the D4 weights (TA_GD4 blocks of TA_D4_MAXREF: g_r(q),
+  // dg_r/dq, dg_r/dCN at the final charge, g_r(0), dg_r(0)/dCN); per shell
+  // the shell potential including the atomic one; per molecule (mol_mat_off,
+  // column-major) the energy-weighted density W = sum_k f_k e_k c_k c_k^T;
+  // per atom pair (mol_pair_off, lower triangle i > j at i*nat + j) dE3/dC6
+  double* grad;
+  double* g_dcn;
+  double* g_dcn4;
+  double* g_vat;
+  double* g_vdp;
+  double* g_vqp;
+  double* g_d4;
+  double* g_vsh;
+  double* g_w;
+  double* g_dc6;
   // optional per-molecule phase clocks of the SCF kernel (TA_PROF_SLOTS each), may be null
   long long* prof;
   // optional completion records (one per molecule, pinned host memory), may be null
   TaDoneRecord* done;
 };
+
+// D4 weight blocks per atom of the gradient (BatchDev::g_d4)
+#define TA_GD4 5
 
 // SCF phase clock slots (wall clock ticks at 100 MHz; last two are counts)
 #define TA_PROF_INIT 0

@@ -72,6 +72,55 @@ __device__ __forceinline__ void d4_weights(const DevD4& d, int e, double cn, dou
   }
 }
 
+// CN derivative of the reference weights of d4_weights, dg_r/dCN = dw_r/dCN
+// zeta_r(q) (zero where the weights fall back to the largest-CN reference)
+__device__ __forceinline__ void d4_weights_dcn(const DevD4& d, int e, double cn, double q,
+                                               double* dgdcn)
+{
+  const int nr = d.nref[e];
+  double norm = 0.0, dnorm = 0.0;
+  for (int r = 0; r < nr; ++r) {
+    const double t = cn - d.cnref[e][r];
+    const double x = exp(-d.wf * t * t);
+    norm += x;
+    dnorm += -2.0 * d.wf * t * x;
+  }
+  const double gi = d.gi[e], zi = d.zeff[e];
+  const double qmod = q + zi;
+  for (int r = 0; r < nr; ++r) {
+    const double t = cn - d.cnref[e][r];
+    const double x = exp(-d.wf * t * t);
+    const double w = x / norm;
+    const double dw = (-2.0 * d.wf * t * x * norm - x * dnorm) / (norm * norm);
+    if (!isfinite(w) || !isfinite(dw)) {
+      dgdcn[r] = 0.0;
+      continue;
+    }
+    double z;
+    if (qmod < 0.0) {
+      z = exp(d.ga);
+    } else {
+      const double qref = d.qref[e][r] + zi;
+      z = exp(d.ga * (1.0 - exp(gi * (1.0 - qref / qmod))));
+    }
+    dgdcn[r] = dw * z;
+  }
+}
+
+// two-body damping factor d(r) = -(s6 / (r^6 + R0^6) + s8 rr / (r^8 + R0^8))
+// of elements ei, ej at squared distance r2, and (dd/dr) / r
+__device__ __forceinline__ void d4_damp(const DevD4& d, int ei, int ej, double r2, double& dmp,
+                                        double& ddr)
+{
+  const double rr = 3.0 * d.r4r2[ei] * d.r4r2[ej];
+  const double r0 = d.a1 * sqrt(rr) + d.a2;
+  const double r02 = r0 * r0, r06 = r02 * r02 * r02, r08 = r06 * r02;
+  const double r6 = r2 * r2 * r2, r8 = r6 * r2;
+  const double t6 = 1.0 / (r6 + r06), t8 = 1.0 / (r8 + r08);
+  dmp = -(d.s6 * t6 + d.s8 * rr * t8);
+  ddr = d.s6 * 6.0 * r2 * r2 * t6 * t6 + d.s8 * rr * 8.0 * r6 * t8 * t8;
+}
+
 // per-atom SCF weights: g_r and dg_r/dq, or (qref_uniform) s and ds/dq
 __device__ __forceinline__ void d4_atom_weights(const DevD4& d, int e, double cn, double q,
                                                 double* gw, double* dgw)

@@ -12,6 +12,9 @@
 #include <cstring>
 #include <deque>
 #include <mutex>
+#include <atomic>
+#include <thread>
+#include <memory>
 #include <new>
 #include <string>
 #include <vector>
@@ -33,6 +36,16 @@ __global__ void k_integrals(BatchDev b, const int* work_mol, const int* work_kin
                             const int* work_start);
 template <int NT, int MAXT, int MAXB, int MAXC, bool MATS_LDS>
 __global__ void k_scf(BatchDev b, const int* mols, int ldmax, int natmax, int nshmax);
+// analytic gradient (gradient.hip)
+__global__ void k_grad_pot(BatchDev b);
+__global__ void k_grad_wdens(BatchDev b, const int* tiles);
+template <int LMAX>
+__global__ void k_grad_h0(BatchDev b, const int* work_mol, const int* work_kind,
+                          const int* work_start);
+__global__ void k_grad_pairs(BatchDev b, const int* work_mol, const int* work_a0, const int* work_a1);
+__global__ void k_grad_atm(BatchDev b, const int* work_mol, const int* work_i0, const int* work_i1);
+__global__ void k_grad_c6(BatchDev b, const int* work_mol, const int* work_a0, const int* work_a1);
+__global__ void k_grad_cn(BatchDev b, const int* work_mol, const int* work_a0, const int* work_a1);
 
 namespace {
 
@@ -288,8 +301,28 @@ struct DeviceBatch {
   int* d_atm_i0 = nullptr;
   int* d_atm_i1 = nullptr;
   std::vector<int*> d_bucket_mols;
+  // energy-weighted density tiles of the gradient (molecule, row tile,
+  // column tile); the host copy stays alive while the upload is in flight
+  std::vector<int> h_wtiles;
+  int* d_wtiles = nullptr;
+  bool grad = false;
   size_t bytes = 0;
 };
+
+// 64 x 64 tiles of the lower triangle of each molecule's W
+void gradient_tiles(const Plan& pl, std::vector<int>& t)
+{
+  t.clear();
+  for (int m = 0; m < pl.nmol; ++m) {
+    const int nt = (pl.mol_nao[m] + 63) / 64;
+    for (int i = 0; i < nt; ++i)
+      for (int j = 0; j <= i; ++j) {
+        t.push_back(m);
+        t.push_back(i);
+        t.push_back(j);
+      }
+  }
+}
 
 }  // namespace
 
@@ -806,7 +839,7 @@ void pack_plan(const Plan& pl, std::vector<char>& img, std::vector<std::pair<siz
 
 // Device workspace of a plan in bytes (every array upload_plan carves, each
 // rounded to 256 bytes).  img_bytes: size of the packed plan image.
-size_t workspace_need(const Plan& pl, size_t img_bytes, bool wfn_arrays)
+size_t workspace_need(const Plan& pl, size_t img_bytes, bool wfn_arrays, bool grad = false)
 {
   const int nmol = pl.nmol, mem = pl.mem;
   const int64_t A = pl.tot_atoms, Sh = pl.tot_sh, AO = pl.tot_ao;
@@ -831,6 +864,17 @@ size_t workspace_need(const Plan& pl, size_t img_bytes, bool wfn_arrays)
   need += dbytes(std::max<int64_t>(pl.tot_scr, 1));
   need += dbytes((int64_t)nmol * TA_PROF_SLOTS);
   need += 2 * dbytes(A) + 2 * dbytes(TA_D4_MAXREF * A) + 2 * dbytes(pl.tot_pair) + 2 * dbytes(nmol); // D4
+  if (grad) {
+    int64_t ntiles = 0;
+    for (int m = 0; m < nmol; ++m) {
+      const int64_t nt = (pl.mol_nao[m] + 63) / 64;
+      ntiles += nt * (nt + 1) / 2;
+    }
+    if (!wfn_arrays) need += 2 * dbytes(pl.tot_mat);     // coeff, density
+    need += dbytes(3 * A) + 3 * dbytes(A) + dbytes(3 * A) + dbytes(6 * A);  // grad, dcn, dcn4, vat, vdp, vqp
+    need += dbytes((int64_t)TA_GD4 * TA_D4_MAXREF * A) + dbytes(Sh);          // D4 weights, vsh
+    need += dbytes(pl.tot_mat) + dbytes(pl.tot_pair) + ibytes(3 * ntiles);  // W, dE3/dC6, tiles
+  }
   need += 64 * 256;
 
   return need;
@@ -843,15 +887,18 @@ size_t workspace_need(const Plan& pl, size_t img_bytes, bool wfn_arrays)
 // never allocate (a plan that does not fit is an error); par: device address
 // of the parameter block when it is not the calculator's own copy.
 int upload_plan(tblite_amd_context ctx, Calculator& c, const Plan& pl, DeviceBatch& db,
-                bool wfn_arrays = false, Arena* fixed = nullptr, const DevParams* par = nullptr)
+                bool wfn_arrays = false, Arena* fixed = nullptr, const DevParams* par = nullptr,
+                bool grad = false)
 {
+  // the gradient reads the density and MO coefficients of the SCF
+  wfn_arrays = wfn_arrays || grad;
   const int nmol = pl.nmol, mem = pl.mem;
   std::vector<char> img;
   std::vector<std::pair<size_t, size_t>> seg;
   pack_plan(pl, img, seg);
   const size_t nfixed = seg.size() - pl.buckets.size();  // fixed arrays before the bucket lists
   const int64_t A = pl.tot_atoms, Sh = pl.tot_sh, AO = pl.tot_ao;
-  const size_t need = workspace_need(pl, img.size(), wfn_arrays);
+  const size_t need = workspace_need(pl, img.size(), wfn_arrays, grad);
   const bool aes_planes = !pl.grid_mols.empty();
 
   Arena& ar = fixed ? *fixed : ctx->arena;
@@ -985,6 +1032,26 @@ int upload_plan(tblite_amd_context ctx, Calculator& c, const Plan& pl, DeviceBat
   b.jacobi_adapt = ctx->jacobi_adapt;
   b.lds_eig = ctx->lds_eig;
   b.prof = ctx->profile ? carve<long long>(ar, (int64_t)nmol * TA_PROF_SLOTS) : nullptr;
+  db.grad = grad;
+  db.d_wtiles = nullptr;
+  db.h_wtiles.clear();
+  if (grad) {
+    b.grad = carve<double>(ar, 3 * A);
+    b.g_dcn = carve<double>(ar, A);
+    b.g_dcn4 = carve<double>(ar, A);
+    b.g_vat = carve<double>(ar, A);
+    b.g_vdp = carve<double>(ar, 3 * A);
+    b.g_vqp = carve<double>(ar, 6 * A);
+    b.g_d4 = carve<double>(ar, (int64_t)TA_GD4 * TA_D4_MAXREF * A);
+    b.g_vsh = carve<double>(ar, Sh);
+    b.g_w = carve<double>(ar, pl.tot_mat);
+    b.g_dc6 = carve<double>(ar, pl.tot_pair);
+    gradient_tiles(pl, db.h_wtiles);
+    db.d_wtiles = carve<int>(ar, (int64_t)db.h_wtiles.size());
+    if (ar.used <= ar.cap && !db.h_wtiles.empty())
+      HIP_TRY(ctx, hipMemcpyAsync(db.d_wtiles, db.h_wtiles.data(), sizeof(int) * db.h_wtiles.size(),
+                                  hipMemcpyHostToDevice, ctx->stream));
+  }
   db.bytes = ar.used;
   if (ar.used > ar.cap) {
     set_error(ctx, "internal: workspace estimate too small (%zu > %zu)", ar.used, ar.cap);
@@ -1277,10 +1344,83 @@ int finish_batch(tblite_amd_context ctx)
 int copy_results(tblite_amd_context ctx, int nmol, double* energies, int* iterations,
                  int* status);
 
+// Analytic gradient of the batch's SCF result (after run_batch): the kernels
+// of gradient.hip in stream order on the context stream.
+int run_gradient(tblite_amd_context ctx)
+{
+  Plan& pl = ctx->plan;
+  DeviceBatch& db = ctx->batch;
+  const BatchDev& b = db.b;
+  if (!db.grad) {
+    set_error(ctx, "internal: the batch holds no gradient buffers");
+    return TBLITE_AMD_ERR_ARGUMENT;
+  }
+  hipStream_t st = ctx->stream;
+  hipLaunchKernelGGL(k_grad_pot, dim3(pl.nmol), dim3(256), 0, st, b);
+  const int ntiles = (int)(db.h_wtiles.size() / 3);
+  if (ntiles > 0)
+    hipLaunchKernelGGL(k_grad_wdens, dim3(ntiles), dim3(256), 0, st, b, db.d_wtiles);
+  const int nwork = (int)pl.work_mol.size(), nsp = pl.nwork_sp;
+  if (nsp > 0)
+    hipLaunchKernelGGL(k_grad_h0<1>, dim3(nsp), dim3(kIntegralChunk), 0, st, b, db.d_work_mol,
+                       db.d_work_kind, db.d_work_start);
+  if (nwork > nsp)
+    hipLaunchKernelGGL(k_grad_h0<2>, dim3(nwork - nsp), dim3(kIntegralChunk), 0, st, b,
+                       db.d_work_mol + nsp, db.d_work_kind + nsp, db.d_work_start + nsp);
+  const int npw = (int)pl.pair_mol.size();
+  hipLaunchKernelGGL(k_grad_pairs, dim3(npw), dim3(256), 0, st, b, db.d_pair_mol, db.d_pair_a0,
+                     db.d_pair_a1);
+  const int natm = (int)pl.atm_mol.size();
+  if (natm > 0) {
+    hipLaunchKernelGGL(k_grad_atm, dim3(natm), dim3(256), 0, st, b, db.d_atm_mol, db.d_atm_i0,
+                       db.d_atm_i1);
+    hipLaunchKernelGGL(k_grad_c6, dim3(npw), dim3(256), 0, st, b, db.d_pair_mol, db.d_pair_a0,
+                       db.d_pair_a1);
+  }
+  hipLaunchKernelGGL(k_grad_cn, dim3(npw), dim3(256), 0, st, b, db.d_pair_mol, db.d_pair_a0,
+                     db.d_pair_a1);
+  HIP_TRY(ctx, hipGetLastError());
+  HIP_TRY(ctx, hipStreamSynchronize(st));
+  return TBLITE_AMD_OK;
+}
+
+// Gradients (3, total atoms) of the last gradient batch and, when sigmas is
+// given, each molecule's strain derivative sum_a g_a (x) R_a (column-major
+// 3 x 3 per molecule).  imol >= 0: that molecule only.
+int copy_gradients(tblite_amd_context ctx, int imol, double* gradients, double* sigmas)
+{
+  const Plan& pl = ctx->plan;
+  const BatchDev& b = ctx->batch.b;
+  if (!ctx->batch.grad) {
+    set_error(ctx, "the last batch was run without gradients");
+    return TBLITE_AMD_ERR_ARGUMENT;
+  }
+  const int m0 = imol < 0 ? 0 : imol, m1 = imol < 0 ? pl.nmol : imol + 1;
+  const int64_t a_lo = pl.mol_atom_off[m0];
+  const int64_t a_hi = pl.mol_atom_off[m1 - 1] + pl.mol_nat[m1 - 1];
+  const int64_t na = a_hi - a_lo;
+  std::vector<double> g(3 * na), x(3 * na);
+  HIP_TRY(ctx, hipMemcpy(g.data(), b.grad + 3 * a_lo, sizeof(double) * 3 * na, hipMemcpyDeviceToHost));
+  HIP_TRY(ctx, hipMemcpy(x.data(), b.xyz + 3 * a_lo, sizeof(double) * 3 * na, hipMemcpyDeviceToHost));
+  if (gradients) std::memcpy(gradients, g.data(), sizeof(double) * 3 * na);
+  if (sigmas)
+    for (int m = m0; m < m1; ++m) {
+      const int64_t o = pl.mol_atom_off[m] - a_lo;
+      double* sg = sigmas + 9 * (int64_t)(m - m0);
+      for (int p = 0; p < 3; ++p)
+        for (int q = 0; q < 3; ++q) {
+          double acc = 0.0;
+          for (int i = 0; i < pl.mol_nat[m]; ++i) acc += g[3 * (o + i) + p] * x[3 * (o + i) + q];
+          sg[p + 3 * q] = acc;
+        }
+    }
+  return TBLITE_AMD_OK;
+}
+
 int flat_singlepoint(tblite_amd_context ctx, Calculator& c, int nmol, const int* mol_nat,
                      const int* num, const double* xyz, const double* charge, const int* uhf,
                      double acc, double* energies, int* iterations, int* status,
-                     bool wfn_arrays = false)
+                     bool wfn_arrays = false, bool grad = false)
 {
   if (!ctx) return TBLITE_AMD_ERR_ARGUMENT;
   if (nmol < 1 || !mol_nat || !num || !xyz || !energies) {
@@ -1294,10 +1434,14 @@ int flat_singlepoint(tblite_amd_context ctx, Calculator& c, int nmol, const int*
   ctx->prepared = nullptr;
   rc = make_plan(ctx, c, nmol, mol_nat, num, charge, uhf, acc, ctx->plan);
   if (rc) return rc;
-  rc = upload_plan(ctx, c, ctx->plan, ctx->batch, wfn_arrays);
+  rc = upload_plan(ctx, c, ctx->plan, ctx->batch, wfn_arrays, nullptr, nullptr, grad);
   if (rc) return rc;
   rc = run_batch(ctx, c, xyz, acc);
   if (rc) return rc;
+  if (grad) {
+    rc = run_gradient(ctx);
+    if (rc) return rc;
+  }
   ctx->have_batch = true;
   return copy_results(ctx, nmol, energies, iterations, status);
 }
@@ -1713,6 +1857,66 @@ int tblite_amd_singlepoint_flat(tblite_amd_context ctx, tblite_amd_calculator ca
                           iterations, status);
 }
 
+int tblite_amd_singlepoint_flat_gradient(tblite_amd_context ctx, tblite_amd_calculator calc,
+                                         int nmol, const int* mol_nat, const int* num,
+                                         const double* xyz, const double* charge, const int* uhf,
+                                         double accuracy, double* energies, double* gradients,
+                                         double* sigmas, int* iterations, int* status)
+{
+  if (!ctx || !calc) return TBLITE_AMD_ERR_ARGUMENT;
+  if (!gradients) {
+    set_error(ctx, "singlepoint_flat_gradient: gradients must not be null");
+    return TBLITE_AMD_ERR_ARGUMENT;
+  }
+  const double acc = accuracy > 0.0 ? accuracy : calc->c.accuracy;
+  int rc = flat_singlepoint(ctx, calc->c, nmol, mol_nat, num, xyz, charge, uhf, acc, energies,
+                            iterations, status, false, true);
+  if (rc && rc != TBLITE_AMD_ERR_NOT_CONVERGED) return rc;
+  const int rc2 = copy_gradients(ctx, -1, gradients, sigmas);
+  return rc2 ? rc2 : rc;
+}
+
+int tblite_amd_xtb_singlepoint_gradient(tblite_amd_context ctx, tblite_amd_structure mol,
+                                        tblite_amd_calculator calc, tblite_amd_wavefunction wfn,
+                                        double accuracy, double* energy, double* gradient,
+                                        double* sigma, int* iterations, int verbosity)
+{
+  if (!ctx || !mol || !calc || !energy || !gradient) return TBLITE_AMD_ERR_ARGUMENT;
+  const double acc = accuracy > 0.0 ? accuracy : calc->c.accuracy;
+  int st = 0, it = 0;
+  int rc = flat_singlepoint(ctx, calc->c, 1, &mol->nat, mol->num.data(), mol->xyz.data(),
+                            &mol->charge, &mol->uhf, acc, energy, &it, &st, wfn != nullptr, true);
+  if (iterations) *iterations = it;
+  if (rc && rc != TBLITE_AMD_ERR_NOT_CONVERGED) return rc;
+  int rc2 = copy_gradients(ctx, 0, gradient, sigma);
+  if (rc2) return rc2;
+  if (wfn) {
+    rc2 = fill_wavefunction(ctx, 0, wfn);
+    if (rc2) return rc2;
+  }
+  if (verbosity > 0) {
+    double gmax = 0.0;
+    for (int i = 0; i < 3 * mol->nat; ++i) gmax = std::max(gmax, std::fabs(gradient[i]));
+    std::printf("tblite_amd: E = %.12f Eh after %d iterations (status %d), max |dE/dR| = %.3e Eh/bohr\n",
+                *energy, it, st, gmax);
+  }
+  if (rc == TBLITE_AMD_ERR_NOT_CONVERGED) {
+    if (st & TA_STATUS_CHOLESKY) return TBLITE_AMD_ERR_CHOLESKY;
+    if (st & TA_STATUS_EIGEN) return TBLITE_AMD_ERR_EIGEN;
+    if (st & TA_STATUS_NAN) return TBLITE_AMD_ERR_NAN;
+    if (st & TA_STATUS_MIXER) return TBLITE_AMD_ERR_MIXER;
+  }
+  return rc;
+}
+
+int tblite_amd_batch_get_gradient(tblite_amd_context ctx, int imol, double* gradient,
+                                  double* sigma)
+{
+  int rc = check_batch(ctx, imol);
+  if (rc) return rc;
+  return copy_gradients(ctx, imol, gradient, sigma);
+}
+
 int tblite_amd_singlepoint_flat_noalloc(tblite_amd_context ctx, tblite_amd_calculator calc,
                                         int nmol, const int* mol_nat, const int* num,
                                         const double* xyz, const double* charge, const int* uhf,
@@ -2067,6 +2271,11 @@ struct ServeRequest {
                        // multi-workgroup molecule
   bool big;   // above the LDS-resident size
   bool grid;  // multi-workgroup path
+  // light-load routing: the multi-workgroup path is at least kGridServeGain
+  // times faster per SCF iteration than one workgroup (cost model), and the
+  // request's footprint alone on that path
+  bool grid_ok = false;
+  size_t grid_bytes = 0;
 };
 
 struct ServeResult {
@@ -2079,6 +2288,12 @@ struct ServeResult {
 
 struct ServeSlot {
   tblite_amd_context ctx = nullptr;
+  // a batch of one request on the multi-workgroup path runs its host loop on
+  // a worker thread (state 1 running, 2 finished), so poll() never blocks
+  std::thread worker;
+  std::atomic<int> async_state{0};
+  int async_rc = 0;
+  bool async = false;
   std::vector<ServeRequest> reqs;
   std::vector<double> xyz;  // host coordinates of the running batch (kept until it ends)
   std::vector<char> reported;       // per request of the running batch
@@ -2129,6 +2344,12 @@ size_t request_bytes(const Calculator& c, int nat, const int* num, int grid_min_
   return (size_t)d * 8 + 64 * 256;
 }
 
+// A large request runs alone on the multi-workgroup path when the server is
+// lightly loaded (at most kGridServeQueue large requests waiting) and the cost
+// model puts that path at least this many times ahead of one workgroup.
+constexpr double kGridServeGain = 2.0;
+constexpr size_t kGridServeQueue = 2;
+
 // Workspace of a batch of requests: their own arrays, the AES2 pair planes of
 // all of them once one takes the multi-workgroup path, and a fixed reserve.
 size_t batch_bytes(const std::vector<ServeRequest>& reqs)
@@ -2148,13 +2369,14 @@ size_t batch_bytes(const std::vector<ServeRequest>& reqs)
 struct tblite_amd_server_s {
   tblite_amd_context owner = nullptr;
   tblite_amd_calculator calc = nullptr;
-  std::vector<ServeSlot> slots;
+  std::vector<std::unique_ptr<ServeSlot>> slots;
   // waiting requests by size class: LDS-resident sizes, larger ones
   std::deque<ServeRequest> queue_small, queue_big;
   std::deque<ServeResult> done;
   size_t slot_bytes = 0;
   int max_batch = 4096;
-  long long next_ticket = 1, submitted = 0, completed = 0, batches = 0;
+  long long next_ticket = 1, submitted = 0, completed = 0, batches = 0, grid_requests = 0;
+  bool grid_busy = false;  // one multi-workgroup request at a time
 };
 
 extern "C" {
@@ -2173,9 +2395,9 @@ tblite_amd_server tblite_amd_new_server(tblite_amd_context ctx, tblite_amd_calcu
   srv->owner = ctx;
   srv->calc = calc;
   srv->slot_bytes = (size_t)(workspace_bytes / slots);
-  srv->slots.resize(slots);
+  for (int k = 0; k < slots; ++k) srv->slots.emplace_back(new ServeSlot());
   for (size_t k = 0; k < srv->slots.size(); ++k) {
-    ServeSlot& sl = srv->slots[k];
+    ServeSlot& sl = *srv->slots[k];
     // with two or more slots the first one serves small molecules only, so
     // they never wait behind a batch of large ones
     sl.small_class = slots >= 2 && k == 0;
@@ -2195,6 +2417,7 @@ tblite_amd_server tblite_amd_new_server(tblite_amd_context ctx, tblite_amd_calcu
       ok = hipHostMalloc((void**)&sl.rec, sizeof(TaDoneRecord) * srv->max_batch,
                          hipHostMallocCoherent) == hipSuccess &&
            hipHostGetDevicePointer((void**)&sl.rec_dev, sl.rec, 0) == hipSuccess;
+    if (ok) std::memset(sl.rec, 0, sizeof(TaDoneRecord) * srv->max_batch);
     if (!ok) {
       set_error(ctx, "new_server: cannot create slot context / reserve %zu bytes", srv->slot_bytes);
       tblite_amd_delete_server(&srv);
@@ -2207,7 +2430,9 @@ tblite_amd_server tblite_amd_new_server(tblite_amd_context ctx, tblite_amd_calcu
 void tblite_amd_delete_server(tblite_amd_server* psrv)
 {
   if (!psrv || !*psrv) return;
-  for (auto& sl : (*psrv)->slots) {
+  for (auto& up : (*psrv)->slots) {
+    ServeSlot& sl = *up;
+    if (sl.worker.joinable()) sl.worker.join();
     if (sl.ctx) tblite_amd_delete_context(&sl.ctx);
     if (sl.rec) (void)hipHostFree(sl.rec);
   }
@@ -2227,7 +2452,7 @@ int tblite_amd_server_submit(tblite_amd_server srv, int nat, const int* num, con
   int nao = 0;
   size_t plane_bytes = 0;
   bool grid = false;
-  const size_t bytes = request_bytes(srv->calc->c, nat, num, srv->slots[0].ctx->grid_min_ld, &nao,
+  const size_t bytes = request_bytes(srv->calc->c, nat, num, srv->slots[0]->ctx->grid_min_ld, &nao,
                                      &plane_bytes, &grid);
   if (bytes == 0) {
     set_error(ctx, "server_submit: element without parameters");
@@ -2250,6 +2475,22 @@ int tblite_amd_server_submit(tblite_amd_server srv, int nat, const int* num, con
   r.plane_bytes = plane_bytes;
   r.grid = grid;
   r.big = ((nao + 3) & ~3) > kLdsMaxLd;
+  {
+    // light-load routing to the multi-workgroup path: the cost model of the
+    // owner context (one SCF iteration: wg_a ld^wg_p on one workgroup,
+    // grid_b ld + grid_c across the device)
+    const tblite_amd_context o = srv->owner;
+    const double ld = (double)((nao + 3) & ~3);
+    const double t_wg = o->wg_a * std::pow(ld, o->wg_p);
+    const double t_grid = std::max(1e-9, o->grid_b * ld + o->grid_c);
+    if (r.big && !grid && t_wg >= kGridServeGain * t_grid) {
+      size_t pb = 0;
+      bool g2 = false;
+      const size_t gb = request_bytes(srv->calc->c, nat, num, 16, nullptr, &pb, &g2);
+      r.grid_bytes = gb + pb + 64 * 1024 * 8;
+      r.grid_ok = r.grid_bytes <= srv->slot_bytes;
+    }
+  }
   if (ticket) *ticket = r.ticket;
   (r.big ? srv->queue_big : srv->queue_small).push_back(std::move(r));
   ++srv->submitted;
@@ -2321,9 +2562,81 @@ int serve_collect(tblite_amd_server srv, ServeSlot& sl)
   return TBLITE_AMD_OK;
 }
 
+// Report every request of a slot's batch as failed and empty the slot.
+void serve_fail(tblite_amd_server srv, ServeSlot& sl)
+{
+  const double now = host_ms();
+  for (const ServeRequest& r : sl.reqs) {
+    ServeResult res;
+    res.ticket = r.ticket;
+    res.energy = 0.0;
+    res.iterations = 0;
+    res.status = TA_STATUS_FAILED;
+    res.latency_ms = now - r.t_submit;
+    srv->done.push_back(res);
+    ++srv->completed;
+  }
+  sl.reqs.clear();
+}
+
+// Launch the slot's single request on the multi-workgroup path: plan and
+// upload on this thread, then the SCF host loop on a worker thread.
+int serve_launch_grid(tblite_amd_server srv, ServeSlot& sl)
+{
+  ServeRequest& r = sl.reqs[0];
+  tblite_amd_context sc = sl.ctx;
+  Calculator& c = srv->calc->c;
+  sl.xyz = r.xyz;
+  const int mol_nat = r.nat;
+  sc->have_batch = false;
+  sc->prepared = nullptr;
+  const int gmin = sc->grid_min_ld;
+  sc->grid_min_ld = 16;  // this batch: the multi-workgroup path
+  int rc = make_plan(sc, c, 1, &mol_nat, r.num.data(), &r.charge, &r.uhf, c.accuracy, sc->plan);
+  sc->grid_min_ld = gmin;
+  if (!rc) rc = upload_plan(sc, c, sc->plan, sc->batch);
+  if (rc) {
+    set_error(srv->owner, "server: batch launch failed: %s", sc->error.c_str());
+    (void)hipStreamSynchronize(sc->stream);
+    serve_fail(srv, sl);
+    return rc;
+  }
+  // no completion record on this path: the result comes from the batch
+  // arrays when the worker has finished.  The slot's record memory still
+  // holds the last batch's records, which serve_scan would report.
+  sc->batch.b.done = nullptr;
+  std::memset(sl.rec, 0, sizeof(TaDoneRecord) * srv->max_batch);
+  sl.reported.assign(1, 0);
+  sl.async = true;
+  sl.async_rc = 0;
+  sl.async_state.store(1, std::memory_order_release);
+  sl.busy = true;
+  srv->grid_busy = true;
+  ++srv->batches;
+  ++srv->grid_requests;
+  const int dev = sc->device;
+  ServeSlot* ps = &sl;
+  sl.worker = std::thread([ps, sc, &c, dev]() {
+    int e = hipSetDevice(dev) == hipSuccess ? TBLITE_AMD_OK : TBLITE_AMD_ERR_HIP;
+    if (!e) e = run_batch(sc, c, ps->xyz.data(), c.accuracy, false);
+    ps->async_rc = e;
+    ps->async_state.store(2, std::memory_order_release);
+  });
+  return TBLITE_AMD_OK;
+}
+
 // Fill a free slot from the queue head and launch its batch.
 int serve_launch(tblite_amd_server srv, ServeSlot& sl)
 {
+  // light load: a large request whose multi-workgroup run is much faster
+  // takes a slot alone, its host loop on a worker thread (with two or more
+  // slots, so that another slot keeps serving the small requests)
+  if (!sl.small_class && srv->slots.size() >= 2 && !srv->grid_busy && !srv->queue_big.empty() &&
+      srv->queue_big.size() <= kGridServeQueue && srv->queue_big.front().grid_ok && sl.reqs.empty()) {
+    sl.reqs.push_back(std::move(srv->queue_big.front()));
+    srv->queue_big.pop_front();
+    return serve_launch_grid(srv, sl);
+  }
   // a small-class slot takes small molecules; the others take large ones,
   // then fill up with small ones (those run first and report early).  The
   // footprint counts the requests already in the slot (none unless a caller
@@ -2372,18 +2685,7 @@ int serve_launch(tblite_amd_server srv, ServeSlot& sl)
     // no ticket is lost and the next poll starts from an empty slot
     set_error(srv->owner, "server: batch launch failed: %s", sc->error.c_str());
     (void)hipStreamSynchronize(sc->stream);
-    const double now = host_ms();
-    for (const ServeRequest& r : sl.reqs) {
-      ServeResult res;
-      res.ticket = r.ticket;
-      res.energy = 0.0;
-      res.iterations = 0;
-      res.status = TA_STATUS_FAILED;
-      res.latency_ms = now - r.t_submit;
-      srv->done.push_back(res);
-      ++srv->completed;
-    }
-    sl.reqs.clear();
+    serve_fail(srv, sl);
     return rc;
   }
   sl.busy = true;
@@ -2398,7 +2700,23 @@ int tblite_amd_server_poll(tblite_amd_server srv, int capacity, long long* ticke
 {
   if (!srv || capacity < 0) return TBLITE_AMD_ERR_ARGUMENT;
   HIP_TRY(srv->owner, hipSetDevice(srv->owner->device));
-  for (auto& sl : srv->slots) {
+  for (auto& up : srv->slots) {
+    ServeSlot& sl = *up;
+    if (sl.busy && sl.async) {
+      // the worker owns the slot's context until it has finished
+      if (sl.async_state.load(std::memory_order_acquire) != 2) continue;
+      sl.worker.join();
+      sl.async = false;
+      sl.async_state.store(0, std::memory_order_relaxed);
+      srv->grid_busy = false;
+      if (sl.async_rc) {
+        set_error(srv->owner, "server: multi-workgroup request failed: %s", sl.ctx->error.c_str());
+        (void)hipStreamSynchronize(sl.ctx->stream);
+        serve_fail(srv, sl);
+        sl.busy = false;
+        return sl.async_rc;
+      }
+    }
     if (sl.busy) {
       const hipError_t q = hipEventQuery(sl.ctx->ev[4]);
       if (q == hipErrorNotReady) {
@@ -2464,16 +2782,18 @@ int tblite_amd_workspace_estimate(const tblite_amd_param* param, int nmol, const
 }
 
 int tblite_amd_server_stats(tblite_amd_server srv, long long* submitted, long long* completed,
-                            int* queued, int* busy_slots, long long* batches)
+                            int* queued, int* busy_slots, long long* batches,
+                            long long* grid_requests)
 {
   if (!srv) return TBLITE_AMD_ERR_ARGUMENT;
   int busy = 0;
-  for (const auto& sl : srv->slots) busy += sl.busy ? 1 : 0;
+  for (const auto& up : srv->slots) busy += up->busy ? 1 : 0;
   if (submitted) *submitted = srv->submitted;
   if (completed) *completed = srv->completed;
   if (queued) *queued = (int)(srv->queue_small.size() + srv->queue_big.size());
   if (busy_slots) *busy_slots = busy;
   if (batches) *batches = srv->batches;
+  if (grid_requests) *grid_requests = srv->grid_requests;
   return TBLITE_AMD_OK;
 }
 

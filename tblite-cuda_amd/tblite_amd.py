@@ -111,6 +111,10 @@ _PROTOS = {
     "tblite_amd_xtb_singlepoint_batch": (_I, [_P, _I, C.POINTER(_P), _P, C.POINTER(_P), _D, _DP, _IP, _IP]),
     "tblite_amd_singlepoint_flat": (_I, [_P, _P, _I, _IP, _IP, _DP, _DP, _IP, _D, _DP, _IP, _IP]),
     "tblite_amd_singlepoint_flat_noalloc": (_I, [_P, _P, _I, _IP, _IP, _DP, _DP, _IP, _D, _DP, _IP, _IP]),
+    "tblite_amd_singlepoint_flat_gradient": (_I, [_P, _P, _I, _IP, _IP, _DP, _DP, _IP, _D, _DP, _DP, _DP,
+                                                  _IP, _IP]),
+    "tblite_amd_xtb_singlepoint_gradient": (_I, [_P, _P, _P, _P, _D, _DP, _DP, _DP, _IP, _I]),
+    "tblite_amd_batch_get_gradient": (_I, [_P, _I, _DP, _DP]),
     "tblite_amd_batch_prepare": (_I, [_P, _P, _I, _IP, _IP, _DP, _IP, _D]),
     "tblite_amd_batch_set_geometry": (_I, [_P, _DP, _I]),
     "tblite_amd_batch_geometry_device": (_P, [_P]),
@@ -137,7 +141,7 @@ _PROTOS = {
     "tblite_amd_server_submit": (_I, [_P, _I, _IP, _DP, _D, _I, C.POINTER(C.c_longlong)]),
     "tblite_amd_server_poll": (_I, [_P, _I, C.POINTER(C.c_longlong), _DP, _IP, _IP, _DP]),
     "tblite_amd_server_stats": (_I, [_P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), _IP, _IP,
-                                     C.POINTER(C.c_longlong)]),
+                                     C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
     "tblite_amd_workspace_estimate": (_I, [C.POINTER(Param), _I, _IP, _IP, _I, _DP, _DP]),
 }
 
@@ -290,6 +294,39 @@ class Engine:
         self._check(rc, allow=(-3,))
         return energies, iters, status
 
+    def singlepoint_flat_gradient(self, mol_nat, num, xyz, charge=None, uhf=None, accuracy=None):
+        """Batched singlepoint with analytic gradients: energies, gradients
+        (total atoms, 3) dE/dR in Eh/bohr, sigmas (nmol, 3, 3) strain
+        derivatives, iterations, status."""
+        mol_nat = np.ascontiguousarray(mol_nat, dtype=np.int32)
+        num = np.ascontiguousarray(num, dtype=np.int32)
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        nmol = len(mol_nat)
+        self._nmol = nmol
+        energies = np.zeros(nmol)
+        grads = np.zeros((len(num), 3))
+        sigmas = np.zeros((nmol, 3, 3))
+        iters = np.zeros(nmol, dtype=np.int32)
+        status = np.zeros(nmol, dtype=np.int32)
+        ch = None if charge is None else np.ascontiguousarray(charge, dtype=np.float64)
+        uh = None if uhf is None else np.ascontiguousarray(uhf, dtype=np.int32)
+        rc = self.lib.tblite_amd_singlepoint_flat_gradient(
+            self.ctx, self.calc, nmol, _ip(mol_nat), _ip(num), _dp(xyz), _dp(ch), _ip(uh),
+            self.accuracy if accuracy is None else accuracy, _dp(energies), _dp(grads),
+            _dp(sigmas), _ip(iters), _ip(status))
+        self._check(rc, allow=(-3,))
+        # column-major 3 x 3 per molecule -> numpy [row, column]
+        return energies, grads, sigmas.transpose(0, 2, 1).copy(), iters, status
+
+    def gradient(self, imol):
+        """Gradient (nat, 3) and sigma (3, 3) of molecule imol of the last
+        gradient batch."""
+        nat, _, _ = self.dims(imol)
+        g = np.zeros((nat, 3))
+        sg = np.zeros((3, 3))
+        self._check(self.lib.tblite_amd_batch_get_gradient(self.ctx, imol, _dp(g), _dp(sg)))
+        return g, sg.T.copy()
+
     def prepare(self, mol_nat, num, charge=None, uhf=None, accuracy=None):
         """Plan a device-resident batch from its composition."""
         mol_nat = np.ascontiguousarray(mol_nat, dtype=np.int32)
@@ -315,14 +352,17 @@ class Engine:
         self._check(self.lib.tblite_amd_batch_results(self.ctx, _dp(e), _ip(it), _ip(st)), allow=(-3,))
         return e, it, st
 
-    def singlepoint(self, num, xyz, charge=0.0, uhf=0, accuracy=None, wavefunction=True):
+    def singlepoint(self, num, xyz, charge=0.0, uhf=0, accuracy=None, wavefunction=True,
+                    gradient=False):
         """One molecule through tblite_amd_xtb_singlepoint (the per-molecule path).
 
         With wavefunction=True a wavefunction handle is passed and every array
         of it is returned (tblite's layout: density and coeff are nao x nao,
         column-major, so coeff[:, r] is level r after the transpose below);
         with False the call is energy-only (no handle, as a caller that wants
-        the energy alone would make it)."""
+        the energy alone would make it).  gradient=True calls
+        tblite_amd_xtb_singlepoint_gradient instead and also returns gradient
+        (nat, 3) in Eh/bohr and sigma (3, 3)."""
         num = np.ascontiguousarray(num, dtype=np.int32)
         xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
         mol = self.lib.tblite_amd_new_structure(self.ctx, len(num), _ip(num), _dp(xyz), charge, uhf)
@@ -332,10 +372,19 @@ class Engine:
         e = C.c_double()
         it = C.c_int()
         try:
-            rc = self.lib.tblite_amd_xtb_singlepoint(
-                self.ctx, mol, self.calc, wfn, self.accuracy if accuracy is None else accuracy,
-                C.byref(e), C.byref(it), 0)
+            acc = self.accuracy if accuracy is None else accuracy
+            if gradient:
+                g = np.zeros((len(num), 3))
+                sg = np.zeros((3, 3))
+                rc = self.lib.tblite_amd_xtb_singlepoint_gradient(
+                    self.ctx, mol, self.calc, wfn, acc, C.byref(e), _dp(g), _dp(sg), C.byref(it), 0)
+            else:
+                rc = self.lib.tblite_amd_xtb_singlepoint(self.ctx, mol, self.calc, wfn, acc,
+                                                         C.byref(e), C.byref(it), 0)
             out = {"energy": e.value, "iterations": it.value, "rc": rc}
+            if gradient:
+                out["gradient"] = g
+                out["sigma"] = sg.T.copy()
             if wavefunction and rc in (0, -3):
                 for name in ("qsh", "qat", "dpat", "qpat", "emo", "focc", "n0sh", "n0at",
                              "density", "coeff"):
@@ -506,12 +555,13 @@ class Server:
                 for j in range(k)]
 
     def stats(self):
-        sub, comp, bat = C.c_longlong(0), C.c_longlong(0), C.c_longlong(0)
+        sub, comp, bat, grid = C.c_longlong(0), C.c_longlong(0), C.c_longlong(0), C.c_longlong(0)
         q, b = C.c_int(0), C.c_int(0)
         self.eng._check(self.lib.tblite_amd_server_stats(self.srv, C.byref(sub), C.byref(comp),
-                                                         C.byref(q), C.byref(b), C.byref(bat)))
+                                                         C.byref(q), C.byref(b), C.byref(bat),
+                                                         C.byref(grid)))
         return {"submitted": sub.value, "completed": comp.value, "queued": q.value,
-                "busy_slots": b.value, "batches": bat.value}
+                "busy_slots": b.value, "batches": bat.value, "grid_requests": grid.value}
 
     def close(self):
         if getattr(self, "srv", None):

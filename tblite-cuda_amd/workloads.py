@@ -12,7 +12,8 @@ Workloads (sizes as used by the tests and the benchmark):
   config2()                      one (H2O)700 at liquid density, seed 2
   config3(count=2048)            tree-grown C/N/O/S organics, 4..60 heavy atoms, seed 3
   config4(count=20000)           request stream, 50 % water / 50 % organic,
-                                 atoms log-uniform in 6..400, seed 4
+                                 atoms log-uniform in 6..400, request k from
+                                 child k of SeedSequence(4)
 """
 import numpy as np
 
@@ -207,22 +208,31 @@ def config3(count=2048):
     return out
 
 
-def config4(count=20000):
-    rng = np.random.default_rng(4)
-    out = []
-    for _ in range(count):
-        natoms = int(np.round(np.exp(rng.uniform(np.log(6), np.log(400)))))
-        if rng.random() < 0.5:
-            n = max(2, int(round(natoms / 3)))
-            out.append(water_cluster(n, liquid_radius(n), rng))
-        else:
-            # grow until heavy + H reaches natoms (heavy ~ natoms / 2.55).  Large
-            # trees are crowded: a hydrogen gets 2000 placement attempts, a
-            # valence with no room is left open, and an odd electron count is
-            # made even by dropping the last hydrogen (closed shell)
-            nh = max(2, int(round((natoms - 2) / 2.55)))
-            out.append(organic_molecule(nh, rng, h_tries=2000, allow_open=True))
-    return out
+def _config4_request(seed_seq):
+    """One request of the serving stream from its own seed sequence."""
+    rng = np.random.default_rng(seed_seq)
+    natoms = int(np.round(np.exp(rng.uniform(np.log(6), np.log(400)))))
+    if rng.random() < 0.5:
+        n = max(2, int(round(natoms / 3)))
+        return water_cluster(n, liquid_radius(n), rng)
+    # grow until heavy + H reaches natoms (heavy ~ natoms / 2.55).  Large
+    # trees are crowded: a hydrogen gets 2000 placement attempts, a valence
+    # with no room is left open, and an odd electron count is made even by
+    # dropping the last hydrogen (closed shell)
+    nh = max(2, int(round((natoms - 2) / 2.55)))
+    return organic_molecule(nh, rng, h_tries=2000, allow_open=True)
+
+
+def config4(count=20000, workers=1):
+    """Request k is drawn from the k-th child of SeedSequence(4), so the
+    stream is the same whatever the number of worker processes."""
+    children = np.random.SeedSequence(4).spawn(count)
+    if workers <= 1:
+        return [_config4_request(c) for c in children]
+    import multiprocessing
+
+    with multiprocessing.get_context("fork").Pool(workers) as pool:
+        return pool.map(_config4_request, children, chunksize=16)
 
 
 def flatten(mols):

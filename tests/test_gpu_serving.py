@@ -112,3 +112,48 @@ def test_results_reported_as_molecules_finish(engine):
           f"{max(lat[k] for k in range(len(small))):.1f} ms", flush=True)
     assert stats["batches"] == 1
     assert max(lat[k] for k in range(len(small))) < 0.5 * lat["big"]
+
+
+def test_large_request_under_light_load_takes_multi_workgroup_path(engine):
+    """A lone 600-AO request on a two-slot server runs on the multi-workgroup
+    path (the cost model puts it about 2.7x ahead of one workgroup) on a
+    worker thread while the other slot serves small requests; its energy
+    equals the one-workgroup path's within 1e-8 Eh and the small results
+    arrive first."""
+    rng = np.random.default_rng(46)
+    big = workloads.water_cluster(100, workloads.liquid_radius(100), rng)
+    small = [workloads.water_cluster(nw, workloads.liquid_radius(nw), rng) for nw in (2, 4, 7)]
+    e_ref, it_ref, st_ref = engine.singlepoint_flat(np.array([len(big[0])]), big[0], big[1])
+    assert st_ref[0] == 0 and engine.molecule_path(0) == 1
+    srv = tblite_amd.Server(engine, slots=2, workspace_bytes=8e9)
+    # a first large-class request through the same slot leaves its completion
+    # record behind (the multi-workgroup launch must not report it)
+    warm = workloads.water_cluster(20, workloads.liquid_radius(20), rng)
+    tw = srv.submit(*warm)
+    t0 = time.time()
+    got = []
+    while not got:
+        got = [r for r in srv.poll() if r[0] == tw]
+        assert time.time() - t0 < 60, "server did not finish"
+        time.sleep(0.0005)
+    assert srv.stats()["grid_requests"] == 0
+    tickets = {srv.submit(*big): "big"}
+    for k, m in enumerate(small):
+        tickets[srv.submit(*m)] = k
+    res = {}
+    t0 = time.time()
+    print("[gpu] serving: lone large request on the multi-workgroup path", flush=True)
+    while len(res) < len(tickets):
+        for t, e, it, st, ms in srv.poll():
+            res[tickets[t]] = (e, it, st, ms)
+        assert time.time() - t0 < 120, "server did not finish"
+        time.sleep(0.0005)
+    stats = srv.stats()
+    srv.close()
+    print(f"[gpu] stats {stats}, big {res['big'][3]:.0f} ms", flush=True)
+    assert stats["grid_requests"] == 1
+    e, it, st, ms = res["big"]
+    assert st == 0
+    assert abs(e - e_ref[0]) < 1e-8, (e, e_ref[0])
+    assert abs(it - int(it_ref[0])) <= 1
+    assert max(res[k][3] for k in range(len(small))) < ms

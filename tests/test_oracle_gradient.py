@@ -39,7 +39,7 @@ def orc():
 
 
 PAIRS = [(1, 0, 1, 0), (8, 1, 1, 0), (6, 1, 8, 1), (16, 2, 6, 1), (16, 2, 16, 2),
-         (7, 0, 16, 2), (1, 1, 16, 2), (6, 0, 7, 1)]
+         (7, 0, 16, 2), (1, 0, 16, 2), (6, 0, 7, 1), (16, 1, 16, 0)]
 
 
 @pytest.mark.parametrize("pair", PAIRS)

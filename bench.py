@@ -296,6 +296,8 @@ def main():
     ap.add_argument("--saturation", type=float, default=None,
                     help="config4: saturation rate (singlepoints/s) from an earlier run; skips "
                          "the closed-loop measurement")
+    ap.add_argument("--gradient-steps", type=int, default=2,
+                    help="timed steps of singlepoint + analytic gradient (0: skip)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device copy of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -418,6 +420,27 @@ def main():
                               "unit": "singlepoints/s", "sample": k,
                               "speedup_batched": round(value / per_mol, 2) if k > 0 else None},
     }
+
+    # singlepoint + analytic gradient of the same batch (device-resident)
+    if args.gradient_steps > 0:
+        eng.prepare(mol_nat, num, gradient=True)
+        eng.set_geometry(xyz)
+        eng.run_gradient()
+        t = time.perf_counter()
+        for _ in range(args.gradient_steps):
+            eng.run_gradient()
+        g_ms = (time.perf_counter() - t) / args.gradient_steps * 1e3
+        g, sg = eng.gradients()
+        e_g, it_g, st_g = eng.results()
+        extra["gradient"] = {
+            "value": round(nmol / (g_ms / 1e3), 2), "unit": "singlepoints with gradients/s",
+            "ms_per_step": round(g_ms, 3), "gradient_ms": round(g_ms - ms_per_step, 3),
+            "steps": args.gradient_steps,
+            "max_abs_dE_vs_energy_only": float(np.abs(e_g - e).max()),
+            "max_abs_sum_g": float(max(np.abs(g[o:o + n].sum(0)).max()
+                                       for o, n in zip(np.cumsum(mol_nat) - mol_nat, mol_nat)))}
+        print(f"[bench] singlepoint + gradient: {g_ms:.2f} ms/step "
+              f"({nmol / (g_ms / 1e3):.1f}/s)", file=sys.stderr, flush=True)
 
     cpu = None
     if args.workload == "config2":

@@ -295,6 +295,19 @@ const double* tblite_amd_batch_geometry_device(tblite_amd_context ctx);
 int tblite_amd_batch_run(tblite_amd_context ctx);
 int tblite_amd_batch_results(tblite_amd_context ctx, double* energies, int* iterations,
                              int* status);
+/* The same with analytic gradients: prepare_gradient also reserves the
+ * gradient buffers (and the density and MO coefficients they need),
+ * run_gradient runs the singlepoint and then the gradient, gradients()
+ * copies gradients(3, total atoms) and sigmas(9, nmol) (either may be NULL)
+ * out, and gradient_device() gives the device address of the resident
+ * gradient array (3 per atom, atom order of the coordinates; valid until the
+ * next prepare), so a geometry step can run on the device between runs. */
+int tblite_amd_batch_prepare_gradient(tblite_amd_context ctx, tblite_amd_calculator calc,
+                                      int nmol, const int* mol_nat, const int* num,
+                                      const double* charge, const int* uhf, double accuracy);
+int tblite_amd_batch_run_gradient(tblite_amd_context ctx);
+int tblite_amd_batch_gradients(tblite_amd_context ctx, double* gradients, double* sigmas);
+const double* tblite_amd_batch_gradient_device(tblite_amd_context ctx);
 
 /* ------------------------------------------------------------- serving
  * Continuous batching of independent singlepoint requests on one device.
@@ -311,13 +324,17 @@ int tblite_amd_batch_results(tblite_amd_context ctx, double* energies, int* iter
  * finished results (ticket, energy in Eh, SCF iterations, status bits,
  * latency from submit to collection in ms).  Calls on one server must come
  * from one host thread.  Requests keep their wavefunction and mixer state on
- * the device until their batch ends.  Under light load (two or more slots,
- * at most two large requests waiting, none on the multi-workgroup path
- * already) a large
- * request for which the tuning's cost model puts the multi-workgroup path at
- * least twice ahead of one workgroup runs alone on that path, its SCF host
- * loop on a worker thread of the server (poll() still never blocks).  No counterpart in the reference (one
- * molecule per GPU thread, src/xtb.cu:479-497). */
+ * the device until their batch ends.  With two or more slots the server
+ * also has a multi-workgroup lane (with four or more slots the last slot
+ * becomes the lane, so the server never runs more streams than the
+ * device's four hardware queues per process; with two or three it is an
+ * extra context with half a slot's share of the workspace): a large request for which the tuning's cost model puts the multi-workgroup
+ * path at least twice ahead of one workgroup waits in the lane's queue, and
+ * the lane runs the largest waiting one alone on that path, its SCF host
+ * loop on a worker thread of the server (poll() still never blocks); beyond
+ * two waiting, the regular slots take the oldest of them on one workgroup
+ * each.  No counterpart in the reference (one molecule per GPU thread,
+ * src/xtb.cu:479-497). */
 typedef struct tblite_amd_server_s* tblite_amd_server;
 tblite_amd_server tblite_amd_new_server(tblite_amd_context ctx, tblite_amd_calculator calc,
                                         int slots, double workspace_bytes);

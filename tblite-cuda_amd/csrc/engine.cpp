@@ -1971,9 +1971,11 @@ int tblite_amd_singlepoint_flat_noalloc(tblite_amd_context ctx, tblite_amd_calcu
   return copy_results(ctx, nmol, energies, iterations, status);
 }
 
-int tblite_amd_batch_prepare(tblite_amd_context ctx, tblite_amd_calculator calc, int nmol,
-                             const int* mol_nat, const int* num, const double* charge,
-                             const int* uhf, double accuracy)
+}  // extern "C"
+
+namespace {
+int prepare_batch(tblite_amd_context ctx, tblite_amd_calculator calc, int nmol, const int* mol_nat,
+                  const int* num, const double* charge, const int* uhf, double accuracy, bool grad)
 {
   if (!ctx || !calc) return TBLITE_AMD_ERR_ARGUMENT;
   if (nmol < 1 || !mol_nat || !num) {
@@ -1988,12 +1990,61 @@ int tblite_amd_batch_prepare(tblite_amd_context ctx, tblite_amd_calculator calc,
   const double acc = accuracy > 0.0 ? accuracy : calc->c.accuracy;
   rc = make_plan(ctx, calc->c, nmol, mol_nat, num, charge, uhf, acc, ctx->plan);
   if (rc) return rc;
-  rc = upload_plan(ctx, calc->c, ctx->plan, ctx->batch);
+  rc = upload_plan(ctx, calc->c, ctx->plan, ctx->batch, false, nullptr, nullptr, grad);
   if (rc) return rc;
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   ctx->prepared = calc;
   ctx->prepared_acc = acc;
   return TBLITE_AMD_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int tblite_amd_batch_prepare(tblite_amd_context ctx, tblite_amd_calculator calc, int nmol,
+                             const int* mol_nat, const int* num, const double* charge,
+                             const int* uhf, double accuracy)
+{
+  return prepare_batch(ctx, calc, nmol, mol_nat, num, charge, uhf, accuracy, false);
+}
+
+int tblite_amd_batch_prepare_gradient(tblite_amd_context ctx, tblite_amd_calculator calc,
+                                      int nmol, const int* mol_nat, const int* num,
+                                      const double* charge, const int* uhf, double accuracy)
+{
+  return prepare_batch(ctx, calc, nmol, mol_nat, num, charge, uhf, accuracy, true);
+}
+
+int tblite_amd_batch_run_gradient(tblite_amd_context ctx)
+{
+  if (!ctx) return TBLITE_AMD_ERR_ARGUMENT;
+  if (!ctx->prepared || !ctx->batch.grad) {
+    set_error(ctx, "no batch prepared with tblite_amd_batch_prepare_gradient on this context");
+    return TBLITE_AMD_ERR_ARGUMENT;
+  }
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  int rc = run_batch(ctx, ctx->prepared->c, nullptr, ctx->prepared_acc);
+  if (rc) return rc;
+  rc = run_gradient(ctx);
+  if (rc) return rc;
+  ctx->have_batch = true;
+  return TBLITE_AMD_OK;
+}
+
+int tblite_amd_batch_gradients(tblite_amd_context ctx, double* gradients, double* sigmas)
+{
+  if (!ctx) return TBLITE_AMD_ERR_ARGUMENT;
+  if (!ctx->have_batch) {
+    set_error(ctx, "no batch has been run on this context");
+    return TBLITE_AMD_ERR_ARGUMENT;
+  }
+  return copy_gradients(ctx, -1, gradients, sigmas);
+}
+
+const double* tblite_amd_batch_gradient_device(tblite_amd_context ctx)
+{
+  if (!ctx || !ctx->prepared || !ctx->batch.grad) return nullptr;
+  return ctx->batch.b.grad;
 }
 
 int tblite_amd_batch_set_geometry(tblite_amd_context ctx, const double* xyz, int on_device)
@@ -2344,9 +2395,11 @@ size_t request_bytes(const Calculator& c, int nat, const int* num, int grid_min_
   return (size_t)d * 8 + 64 * 256;
 }
 
-// A large request runs alone on the multi-workgroup path when the server is
-// lightly loaded (at most kGridServeQueue large requests waiting) and the cost
-// model puts that path at least this many times ahead of one workgroup.
+// Requests for which the cost model puts the multi-workgroup path at least
+// this many times ahead of one workgroup wait in their own queue, served one
+// at a time by the server's multi-workgroup lane; beyond kGridServeQueue of
+// them waiting, the regular slots take the oldest (one workgroup each), so
+// under heavy load they batch like other large requests.
 constexpr double kGridServeGain = 2.0;
 constexpr size_t kGridServeQueue = 2;
 
@@ -2370,14 +2423,52 @@ struct tblite_amd_server_s {
   tblite_amd_context owner = nullptr;
   tblite_amd_calculator calc = nullptr;
   std::vector<std::unique_ptr<ServeSlot>> slots;
-  // waiting requests by size class: LDS-resident sizes, larger ones
-  std::deque<ServeRequest> queue_small, queue_big;
+  // multi-workgroup lane (two or more slots): one request at a time on the
+  // multi-workgroup path, its SCF host loop on a worker thread
+  std::unique_ptr<ServeSlot> lane;
+  // waiting requests by size class: LDS-resident sizes, larger ones, and
+  // those the lane serves faster
+  std::deque<ServeRequest> queue_small, queue_big, queue_grid;
   std::deque<ServeResult> done;
-  size_t slot_bytes = 0;
+  size_t slot_bytes = 0, lane_bytes = 0;
   int max_batch = 4096;
   long long next_ticket = 1, submitted = 0, completed = 0, batches = 0, grid_requests = 0;
-  bool grid_busy = false;  // one multi-workgroup request at a time
 };
+
+namespace {
+// context, workspace reservation and completion records of one slot
+bool make_slot(tblite_amd_server srv, ServeSlot& sl, size_t bytes)
+{
+  tblite_amd_context ctx = srv->owner;
+  sl.ctx = make_context(ctx->device, true);
+  bool ok = sl.ctx != nullptr;
+  // a multi-workgroup molecule runs synchronously on the host thread, which
+  // would stall poll(): slots keep one workgroup per molecule (the lane
+  // routes its requests itself)
+  if (ok) sl.ctx->auto_grid = false;
+  if (ok) {
+    // reserve the workspace once: growing it later would free device
+    // memory, which synchronises the whole device
+    Arena& ar = sl.ctx->arena;
+    ok = hipMalloc(&ar.base, bytes) == hipSuccess;
+    if (ok) ar.cap = bytes;
+  }
+  if (ok)
+    ok = hipHostMalloc((void**)&sl.rec, sizeof(TaDoneRecord) * srv->max_batch,
+                       hipHostMallocCoherent) == hipSuccess &&
+         hipHostGetDevicePointer((void**)&sl.rec_dev, sl.rec, 0) == hipSuccess;
+  if (ok) std::memset(sl.rec, 0, sizeof(TaDoneRecord) * srv->max_batch);
+  return ok;
+}
+
+void free_slot(ServeSlot& sl)
+{
+  if (sl.worker.joinable()) sl.worker.join();
+  if (sl.ctx) tblite_amd_delete_context(&sl.ctx);
+  if (sl.rec) (void)hipHostFree(sl.rec);
+  sl.rec = nullptr;
+}
+}  // namespace
 
 extern "C" {
 
@@ -2394,32 +2485,33 @@ tblite_amd_server tblite_amd_new_server(tblite_amd_context ctx, tblite_amd_calcu
   if (!srv) return nullptr;
   srv->owner = ctx;
   srv->calc = calc;
-  srv->slot_bytes = (size_t)(workspace_bytes / slots);
-  for (int k = 0; k < slots; ++k) srv->slots.emplace_back(new ServeSlot());
+  // The multi-workgroup lane (two or more slots) issues thousands of short
+  // kernels per SCF iteration; a stream that shares a hardware queue with it
+  // waits behind them, and the process has four hardware queues.  With four
+  // or more slots the last slot is the lane (one stream per slot, a slot's
+  // share of the workspace); with two or three it is an extra context with
+  // half a slot's share.
+  const int nreg = slots >= 4 ? slots - 1 : slots;
+  const double shares = slots >= 4 ? (double)slots : slots >= 2 ? slots + 0.5 : (double)slots;
+  srv->slot_bytes = (size_t)(workspace_bytes / shares);
+  srv->lane_bytes = slots >= 4 ? srv->slot_bytes : slots >= 2 ? srv->slot_bytes / 2 : 0;
+  for (int k = 0; k < nreg; ++k) srv->slots.emplace_back(new ServeSlot());
   for (size_t k = 0; k < srv->slots.size(); ++k) {
     ServeSlot& sl = *srv->slots[k];
     // with two or more slots the first one serves small molecules only, so
     // they never wait behind a batch of large ones
-    sl.small_class = slots >= 2 && k == 0;
-    sl.ctx = make_context(ctx->device, true);
-    bool ok = sl.ctx != nullptr;
-    // a multi-workgroup molecule runs synchronously on the host thread, which
-    // would stall poll(): slots keep one workgroup per molecule
-    if (ok) sl.ctx->auto_grid = false;
-    if (ok) {
-      // reserve the slot's workspace once: growing it later would free device
-      // memory, which synchronises the whole device
-      Arena& ar = sl.ctx->arena;
-      ok = hipMalloc(&ar.base, srv->slot_bytes) == hipSuccess;
-      if (ok) ar.cap = srv->slot_bytes;
-    }
-    if (ok)
-      ok = hipHostMalloc((void**)&sl.rec, sizeof(TaDoneRecord) * srv->max_batch,
-                         hipHostMallocCoherent) == hipSuccess &&
-           hipHostGetDevicePointer((void**)&sl.rec_dev, sl.rec, 0) == hipSuccess;
-    if (ok) std::memset(sl.rec, 0, sizeof(TaDoneRecord) * srv->max_batch);
-    if (!ok) {
+    sl.small_class = nreg >= 2 && k == 0;
+    if (!make_slot(srv, sl, srv->slot_bytes)) {
       set_error(ctx, "new_server: cannot create slot context / reserve %zu bytes", srv->slot_bytes);
+      tblite_amd_delete_server(&srv);
+      return nullptr;
+    }
+  }
+  if (slots >= 2) {
+    srv->lane.reset(new ServeSlot());
+    if (!make_slot(srv, *srv->lane, srv->lane_bytes)) {
+      set_error(ctx, "new_server: cannot create the multi-workgroup lane / reserve %zu bytes",
+                srv->lane_bytes);
       tblite_amd_delete_server(&srv);
       return nullptr;
     }
@@ -2430,12 +2522,8 @@ tblite_amd_server tblite_amd_new_server(tblite_amd_context ctx, tblite_amd_calcu
 void tblite_amd_delete_server(tblite_amd_server* psrv)
 {
   if (!psrv || !*psrv) return;
-  for (auto& up : (*psrv)->slots) {
-    ServeSlot& sl = *up;
-    if (sl.worker.joinable()) sl.worker.join();
-    if (sl.ctx) tblite_amd_delete_context(&sl.ctx);
-    if (sl.rec) (void)hipHostFree(sl.rec);
-  }
+  for (auto& up : (*psrv)->slots) free_slot(*up);
+  if ((*psrv)->lane) free_slot(*(*psrv)->lane);
   delete *psrv;
   *psrv = nullptr;
 }
@@ -2483,16 +2571,16 @@ int tblite_amd_server_submit(tblite_amd_server srv, int nat, const int* num, con
     const double ld = (double)((nao + 3) & ~3);
     const double t_wg = o->wg_a * std::pow(ld, o->wg_p);
     const double t_grid = std::max(1e-9, o->grid_b * ld + o->grid_c);
-    if (r.big && !grid && t_wg >= kGridServeGain * t_grid) {
+    if (srv->lane && r.big && !grid && t_wg >= kGridServeGain * t_grid) {
       size_t pb = 0;
       bool g2 = false;
       const size_t gb = request_bytes(srv->calc->c, nat, num, 16, nullptr, &pb, &g2);
       r.grid_bytes = gb + pb + 64 * 1024 * 8;
-      r.grid_ok = r.grid_bytes <= srv->slot_bytes;
+      r.grid_ok = r.grid_bytes <= srv->lane_bytes;
     }
   }
   if (ticket) *ticket = r.ticket;
-  (r.big ? srv->queue_big : srv->queue_small).push_back(std::move(r));
+  (r.grid_ok ? srv->queue_grid : r.big ? srv->queue_big : srv->queue_small).push_back(std::move(r));
   ++srv->submitted;
   return TBLITE_AMD_OK;
 }
@@ -2611,7 +2699,6 @@ int serve_launch_grid(tblite_amd_server srv, ServeSlot& sl)
   sl.async_rc = 0;
   sl.async_state.store(1, std::memory_order_release);
   sl.busy = true;
-  srv->grid_busy = true;
   ++srv->batches;
   ++srv->grid_requests;
   const int dev = sc->device;
@@ -2626,24 +2713,30 @@ int serve_launch_grid(tblite_amd_server srv, ServeSlot& sl)
 }
 
 // Fill a free slot from the queue head and launch its batch.
+// The lane takes the largest waiting request of its queue.
+int serve_launch_lane(tblite_amd_server srv, ServeSlot& sl)
+{
+  std::deque<ServeRequest>& q = srv->queue_grid;
+  size_t best = 0;
+  for (size_t k = 1; k < q.size(); ++k)
+    if (q[k].bytes > q[best].bytes) best = k;
+  sl.reqs.push_back(std::move(q[best]));
+  q.erase(q.begin() + (std::ptrdiff_t)best);
+  return serve_launch_grid(srv, sl);
+}
+
 int serve_launch(tblite_amd_server srv, ServeSlot& sl)
 {
-  // light load: a large request whose multi-workgroup run is much faster
-  // takes a slot alone, its host loop on a worker thread (with two or more
-  // slots, so that another slot keeps serving the small requests)
-  if (!sl.small_class && srv->slots.size() >= 2 && !srv->grid_busy && !srv->queue_big.empty() &&
-      srv->queue_big.size() <= kGridServeQueue && srv->queue_big.front().grid_ok && sl.reqs.empty()) {
-    sl.reqs.push_back(std::move(srv->queue_big.front()));
-    srv->queue_big.pop_front();
-    return serve_launch_grid(srv, sl);
-  }
-  // a small-class slot takes small molecules; the others take large ones,
-  // then fill up with small ones (those run first and report early).  The
-  // footprint counts the requests already in the slot (none unless a caller
-  // changes that) and the pair planes a multi-workgroup molecule brings.
-  for (int pass = sl.small_class ? 1 : 0; pass < 2; ++pass) {
-    std::deque<ServeRequest>& q = pass == 0 ? srv->queue_big : srv->queue_small;
-    while (!q.empty() && (int)sl.reqs.size() < srv->max_batch) {
+  // a small-class slot takes small molecules; the others take the lane's
+  // overflow (beyond kGridServeQueue waiting, oldest first), then large
+  // ones, then fill up with small ones (those run first and report early).
+  // The footprint counts the requests already in the slot (none unless a
+  // caller changes that) and the pair planes a multi-workgroup molecule
+  // brings.
+  for (int pass = sl.small_class ? 2 : 0; pass < 3; ++pass) {
+    std::deque<ServeRequest>& q = pass == 0 ? srv->queue_grid : pass == 1 ? srv->queue_big : srv->queue_small;
+    const size_t keep = pass == 0 ? kGridServeQueue : 0;
+    while (q.size() > keep && (int)sl.reqs.size() < srv->max_batch) {
       sl.reqs.push_back(std::move(q.front()));
       q.pop_front();
       if (batch_bytes(sl.reqs) > srv->slot_bytes && sl.reqs.size() > 1) {
@@ -2700,15 +2793,17 @@ int tblite_amd_server_poll(tblite_amd_server srv, int capacity, long long* ticke
 {
   if (!srv || capacity < 0) return TBLITE_AMD_ERR_ARGUMENT;
   HIP_TRY(srv->owner, hipSetDevice(srv->owner->device));
-  for (auto& up : srv->slots) {
-    ServeSlot& sl = *up;
+  const size_t nslot = srv->slots.size();
+  for (size_t k = 0; k <= nslot; ++k) {
+    if (k == nslot && !srv->lane) break;
+    ServeSlot& sl = k < nslot ? *srv->slots[k] : *srv->lane;
+    const bool is_lane = k == nslot;
     if (sl.busy && sl.async) {
       // the worker owns the slot's context until it has finished
       if (sl.async_state.load(std::memory_order_acquire) != 2) continue;
       sl.worker.join();
       sl.async = false;
       sl.async_state.store(0, std::memory_order_relaxed);
-      srv->grid_busy = false;
       if (sl.async_rc) {
         set_error(srv->owner, "server: multi-workgroup request failed: %s", sl.ctx->error.c_str());
         (void)hipStreamSynchronize(sl.ctx->stream);
@@ -2730,7 +2825,13 @@ int tblite_amd_server_poll(tblite_amd_server srv, int capacity, long long* ticke
       int rc = serve_collect(srv, sl);
       if (rc) return rc;
     }
-    if (!sl.busy && !(srv->queue_small.empty() && srv->queue_big.empty())) {
+    if (is_lane) {
+      if (!sl.busy && !srv->queue_grid.empty()) {
+        int rc = serve_launch_lane(srv, sl);
+        if (rc) return rc;
+      }
+    } else if (!sl.busy && !(srv->queue_small.empty() && srv->queue_big.empty() &&
+                             srv->queue_grid.size() <= kGridServeQueue)) {
       int rc = serve_launch(srv, sl);
       if (rc) return rc;
     }
@@ -2788,9 +2889,10 @@ int tblite_amd_server_stats(tblite_amd_server srv, long long* submitted, long lo
   if (!srv) return TBLITE_AMD_ERR_ARGUMENT;
   int busy = 0;
   for (const auto& up : srv->slots) busy += up->busy ? 1 : 0;
+  if (srv->lane) busy += srv->lane->busy ? 1 : 0;
   if (submitted) *submitted = srv->submitted;
   if (completed) *completed = srv->completed;
-  if (queued) *queued = (int)(srv->queue_small.size() + srv->queue_big.size());
+  if (queued) *queued = (int)(srv->queue_small.size() + srv->queue_big.size() + srv->queue_grid.size());
   if (busy_slots) *busy_slots = busy;
   if (batches) *batches = srv->batches;
   if (grid_requests) *grid_requests = srv->grid_requests;

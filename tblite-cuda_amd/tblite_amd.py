@@ -120,6 +120,10 @@ _PROTOS = {
     "tblite_amd_batch_geometry_device": (_P, [_P]),
     "tblite_amd_batch_run": (_I, [_P]),
     "tblite_amd_batch_results": (_I, [_P, _DP, _IP, _IP]),
+    "tblite_amd_batch_prepare_gradient": (_I, [_P, _P, _I, _IP, _IP, _DP, _IP, _D]),
+    "tblite_amd_batch_run_gradient": (_I, [_P]),
+    "tblite_amd_batch_gradients": (_I, [_P, _DP, _DP]),
+    "tblite_amd_batch_gradient_device": (_P, [_P]),
     "tblite_amd_batch_dims": (_I, [_P, _I, _IP, _IP, _IP]),
     "tblite_amd_batch_get_cn": (_I, [_P, _I, _DP]),
     "tblite_amd_batch_get_adjlist": (_I, [_P, _I, _IP, _IP, _IP, _I]),
@@ -327,16 +331,28 @@ class Engine:
         self._check(self.lib.tblite_amd_batch_get_gradient(self.ctx, imol, _dp(g), _dp(sg)))
         return g, sg.T.copy()
 
-    def prepare(self, mol_nat, num, charge=None, uhf=None, accuracy=None):
-        """Plan a device-resident batch from its composition."""
+    def prepare(self, mol_nat, num, charge=None, uhf=None, accuracy=None, gradient=False):
+        """Plan a device-resident batch from its composition (gradient=True:
+        also the gradient buffers, for run_gradient())."""
         mol_nat = np.ascontiguousarray(mol_nat, dtype=np.int32)
         num = np.ascontiguousarray(num, dtype=np.int32)
         ch = None if charge is None else np.ascontiguousarray(charge, dtype=np.float64)
         uh = None if uhf is None else np.ascontiguousarray(uhf, dtype=np.int32)
         self._nmol = len(mol_nat)
-        self._check(self.lib.tblite_amd_batch_prepare(
-            self.ctx, self.calc, len(mol_nat), _ip(mol_nat), _ip(num), _dp(ch), _ip(uh),
-            self.accuracy if accuracy is None else accuracy))
+        self._natoms = len(num)
+        fn = self.lib.tblite_amd_batch_prepare_gradient if gradient else self.lib.tblite_amd_batch_prepare
+        self._check(fn(self.ctx, self.calc, len(mol_nat), _ip(mol_nat), _ip(num), _dp(ch), _ip(uh),
+                       self.accuracy if accuracy is None else accuracy))
+
+    def run_gradient(self):
+        self._check(self.lib.tblite_amd_batch_run_gradient(self.ctx))
+
+    def gradients(self):
+        """Gradients (total atoms, 3) and sigmas (nmol, 3, 3) of the last run."""
+        g = np.zeros((self._natoms, 3))
+        sg = np.zeros((self._nmol, 3, 3))
+        self._check(self.lib.tblite_amd_batch_gradients(self.ctx, _dp(g), _dp(sg)))
+        return g, sg.transpose(0, 2, 1).copy()
 
     def set_geometry(self, xyz):
         xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)

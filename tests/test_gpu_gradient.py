@@ -110,3 +110,32 @@ def test_per_molecule_gradient_entry(engine, orc, cases):
     assert np.max(np.abs(r["sigma"] - ref["sigma"])) < 1e-5
     g, sg = engine.gradient(0)
     assert np.array_equal(g, r["gradient"])
+
+
+def test_device_resident_gradient_batch(orc, cases):
+    """tblite_amd_batch_prepare_gradient / run_gradient / gradients on two
+    geometries of one composition (a displaced copy second): each run's
+    gradients equal the oracle's at its own geometry."""
+    mols, charge, uhf = cases
+    sub = [mols[0], mols[3], mols[6]]
+    mol_nat, num, xyz = workloads.flatten(sub)
+    eng = tblite_amd.Engine(tuning={"auto_grid": 0})
+    try:
+        eng.prepare(mol_nat, num, gradient=True)
+        rng = np.random.default_rng(5)
+        for step, x in enumerate((xyz, xyz + 0.05 * rng.standard_normal(xyz.shape))):
+            print(f"[gpu] resident gradient batch, geometry {step}", flush=True)
+            eng.set_geometry(x)
+            eng.run_gradient()
+            g, sg = eng.gradients()
+            e, it, st = eng.results()
+            assert np.all(st == 0)
+            off = 0
+            for m, (n_, _) in enumerate(sub):
+                xm = x[off:off + len(n_)]
+                r = orc.singlepoint(n_, xm, gradient=True)
+                assert abs(e[m] - r["energy"]) < 1e-6
+                assert np.max(np.abs(g[off:off + len(n_)] - r["gradient"])) < GTOL
+                off += len(n_)
+    finally:
+        eng.close()

@@ -181,7 +181,11 @@ def serve_bench(args):
               file=sys.stderr, flush=True)
     rng = np.random.default_rng(40)
     sweep = []
-    fracs = [float(f) for f in args.fractions.split(",") if f] if args.fractions else []
+    if args.rates:
+        rates = [float(r) for r in args.rates.split(",") if r]
+        fracs = [r / sat for r in rates]
+    else:
+        fracs = [float(f) for f in args.fractions.split(",") if f] if args.fractions else []
     for frac in fracs:
         rate = frac * sat
         m = min(len(mols), max(100, int(rate * args.seconds_per_rate)))
@@ -217,7 +221,8 @@ def serve_bench(args):
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"config4: {WORKLOADS['config4'][0]}", "requests": len(mols),
-                   "slots": args.slots, "workspace_GB": args.workspace_gb},
+                   "slots": args.slots, "workspace_GB": args.workspace_gb,
+                   "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4)},
         "latency_ms": {"p50": best["p50_ms"], "p99": best["p99_ms"],
                        "at_offered_per_s": best["offered_per_s"]},
         "roofline": None,
@@ -293,6 +298,12 @@ def main():
                     help="config4: open-loop requests per rate = rate x this (at most --requests)")
     ap.add_argument("--fractions", default="0.25,0.5,0.8",
                     help="config4: open-loop rates as fractions of the saturation rate")
+    ap.add_argument("--rates", default="",
+                    help="config4: open-loop rates in singlepoints/s (instead of --fractions)")
+    ap.add_argument("--hw-queues", type=int, default=0,
+                    help="config4: hardware queues of the process (GPU_MAX_HW_QUEUES, read when "
+                         "the HIP runtime starts; 0 leaves the environment alone); the server "
+                         "runs one stream per slot")
     ap.add_argument("--saturation", type=float, default=None,
                     help="config4: saturation rate (singlepoints/s) from an earlier run; skips "
                          "the closed-loop measurement")
@@ -310,6 +321,8 @@ def main():
         sys.exit(2)
 
     if args.workload == "config4":
+        if args.hw_queues > 0:
+            os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, args.hw_queues))
         serve_bench(args)
         return
     desc, gen = WORKLOADS[args.workload]

@@ -436,8 +436,9 @@ int tblite_amd_write_json(const char* path, int nat, const int* num, const doubl
  * multi-workgroup path when the batch is too small to fill the device and
  * the cost model (one SCF iteration: wg_a * ld^wg_p seconds on one
  * workgroup, grid_b * ld + grid_c seconds across the device) says the batch
- * finishes sooner.  The coefficients come from measurement on the device:
- * tblite-cuda_amd/autotune.py times both paths and writes them.  A server's
+ * finishes sooner.  The coefficients and the work tiles below come from
+ * measurement on the device: tblite-cuda_amd/autotune.py times both paths,
+ * fits the cost model and picks each tile size by timing batches.  A server's
  * slots never use auto_grid.  TBLITE_AMD_LDS_MAX_LD, TBLITE_AMD_GRID_MIN_LD
  * and TBLITE_AMD_GRID_GRAPH=0 override the defaults at context creation.  No
  * counterpart in the reference (one molecule per GPU thread,
@@ -448,6 +449,14 @@ typedef struct tblite_amd_tuning {
   int auto_grid;    /* nonzero: cost-model routing of the middle range */
   int grid_graph;   /* nonzero: replay the multi-workgroup eigenvalue phase from a HIP graph */
   double wg_a, wg_p, grid_b, grid_c;
+  /* work tiles: neighbour-list entries per integral work item (its
+   * workgroup size: 32, 64 or 128), log2 of the atom pairs per pair-plane
+   * work item (8..22) and of the triples per three-body work item (10..26);
+   * nonzero fine_buckets: SCF size buckets in steps of 16 AOs instead of one
+   * per kernel instantiation; big_lds_min_kb: minimum LDS request of the
+   * global-memory SCF kernel (KB; above half a CU's LDS it runs one molecule
+   * per CU instead of two) */
+  int integral_chunk, pair_chunk_log2, atm_chunk_log2, fine_buckets, big_lds_min_kb;
 } tblite_amd_tuning;
 int tblite_amd_get_tuning(tblite_amd_context ctx, tblite_amd_tuning* out);
 int tblite_amd_set_tuning(tblite_amd_context ctx, const tblite_amd_tuning* in);

@@ -49,9 +49,13 @@ __global__ void k_grad_cn(BatchDev b, const int* work_mol, const int* work_a0, c
 
 namespace {
 
+// work-tile defaults (tblite_amd_tuning: integral_chunk, pair_chunk_log2,
+// atm_chunk_log2): neighbour-list entries per integral work item (its
+// workgroup size, at most the kernels' launch bound of 128), atom pairs per
+// pair-plane work item, triples per three-body work item
 constexpr int kIntegralChunk = 128;
-constexpr int64_t kAtmChunk = 1 << 18;  // triples per three-body dispersion work item
-constexpr int64_t kPairChunk = 1 << 15; // atom pairs per pair-plane work item
+constexpr int kAtmChunkLog2 = 18;
+constexpr int kPairChunkLog2 = 15;
 constexpr int kLdsMaxLd = 96;
 constexpr int kNumStreams = 4;
 constexpr int kMaxBuckets = 8;
@@ -287,6 +291,7 @@ struct Plan {
           tot_pair = 0, tot_mix = 0, tot_mixmem = 0, tot_big = 0, tot_scr = 0;
   std::vector<int> too_large;
   std::vector<char> mol_path;  // 0 LDS, 1 one workgroup in global memory, 2 multi-workgroup
+  int int_chunk = kIntegralChunk;  // integral work-item size the work list was cut with
 };
 
 struct DeviceBatch {
@@ -392,6 +397,10 @@ struct tblite_amd_context_s {
   int lds_max_ld = kLdsMaxLd;
   // size buckets in steps of 16 AOs instead of one per kernel instantiation
   bool fine_buckets = false;
+  // work tiles of the setup kernels (see kIntegralChunk)
+  int int_chunk = kIntegralChunk;
+  int pair_chunk_log2 = kPairChunkLog2;
+  int atm_chunk_log2 = kAtmChunkLog2;
   // replay the multi-workgroup path's eigenvalue phase from a HIP graph
   // (TBLITE_AMD_GRID_GRAPH=0 launches the kernels one by one instead)
   bool grid_graph = true;
@@ -431,7 +440,7 @@ struct tblite_amd_context_s {
   // autotune.py: one workgroup alone wg_a * ld^wg_p, across the device
   // grid_b * ld + grid_c)
   bool auto_grid = true;
-  double wg_a = 9.13e-8, wg_p = 2.10, grid_b = 4.37e-5, grid_c = -2.92e-3;
+  double wg_a = 9.30e-8, wg_p = 2.085, grid_b = 4.16e-5, grid_c = -2.53e-3;
   int wg_slots = 512;  // one-workgroup molecules resident at once (two per CU)
   int verbose = 0;  // TBLITE_AMD_VERBOSE=1: progress lines of long runs on stderr
   // every kernel on the main stream (the serving slots: one stream per slot,
@@ -540,6 +549,7 @@ int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* 
   pl = Plan();
   pl.nmol = nmol;
   pl.mem = std::max(1, c.max_iter);
+  pl.int_chunk = ctx->int_chunk;
   pl.mol_atom_off.resize(nmol);
   pl.mol_nat.resize(nmol);
   pl.mol_sh_off.resize(nmol);
@@ -667,12 +677,12 @@ int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* 
       std::vector<int>& wk = has_d ? wd_kind : pl.work_kind;
       std::vector<int>& ws = has_d ? wd_start : pl.work_start;
       const int64_t cap = (int64_t)nat * (nat + 1) / 2;
-      for (int64_t s = 0; s < cap; s += kIntegralChunk) {
+      for (int64_t s = 0; s < cap; s += ctx->int_chunk) {
         wm.push_back(m);
         wk.push_back(0);
         ws.push_back((int)s);
       }
-      for (int s = 0; s < nat; s += kIntegralChunk) {
+      for (int s = 0; s < nat; s += ctx->int_chunk) {
         wm.push_back(m);
         wk.push_back(1);
         ws.push_back(s);
@@ -680,7 +690,7 @@ int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* 
     }
     // pair planes: rows of about kPairChunk pairs per work item
     {
-      const int rows = (int)std::max<int64_t>(1, kPairChunk / nat);
+      const int rows = (int)std::max<int64_t>(1, ((int64_t)1 << ctx->pair_chunk_log2) / nat);
       for (int a = 0; a < nat; a += rows) {
         pl.pair_mol.push_back(m);
         pl.pair_a0.push_back(a);
@@ -694,7 +704,7 @@ int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* 
       int64_t acc = 0;
       for (int i = 0; i < nat; ++i) {
         acc += (int64_t)i * (i - 1) / 2;
-        if (acc >= kAtmChunk || i == nat - 1) {
+        if (acc >= ((int64_t)1 << ctx->atm_chunk_log2) || i == nat - 1) {
           pl.atm_mol.push_back(m);
           pl.atm_i0.push_back(i0);
           pl.atm_i1.push_back(i + 1);
@@ -1259,10 +1269,10 @@ int run_batch(tblite_amd_context ctx, Calculator& c, const double* xyz, double a
   HIP_TRY(ctx, hipEventRecord(ctx->ev[2], st));
   const int nwork = (int)pl.work_mol.size(), nsp = pl.nwork_sp;
   if (nsp > 0)
-    hipLaunchKernelGGL(k_integrals<1>, dim3(nsp), dim3(kIntegralChunk), 0, st, b, db.d_work_mol,
+    hipLaunchKernelGGL(k_integrals<1>, dim3(nsp), dim3(pl.int_chunk), 0, st, b, db.d_work_mol,
                        db.d_work_kind, db.d_work_start);
   if (nwork > nsp)
-    hipLaunchKernelGGL(k_integrals<2>, dim3(nwork - nsp), dim3(kIntegralChunk), 0, st, b,
+    hipLaunchKernelGGL(k_integrals<2>, dim3(nwork - nsp), dim3(pl.int_chunk), 0, st, b,
                        db.d_work_mol + nsp, db.d_work_kind + nsp, db.d_work_start + nsp);
   HIP_TRY(ctx, hipGetLastError());
   HIP_TRY(ctx, hipEventRecord(ctx->ev[3], st));
@@ -1362,10 +1372,10 @@ int run_gradient(tblite_amd_context ctx)
     hipLaunchKernelGGL(k_grad_wdens, dim3(ntiles), dim3(256), 0, st, b, db.d_wtiles);
   const int nwork = (int)pl.work_mol.size(), nsp = pl.nwork_sp;
   if (nsp > 0)
-    hipLaunchKernelGGL(k_grad_h0<1>, dim3(nsp), dim3(kIntegralChunk), 0, st, b, db.d_work_mol,
+    hipLaunchKernelGGL(k_grad_h0<1>, dim3(nsp), dim3(pl.int_chunk), 0, st, b, db.d_work_mol,
                        db.d_work_kind, db.d_work_start);
   if (nwork > nsp)
-    hipLaunchKernelGGL(k_grad_h0<2>, dim3(nwork - nsp), dim3(kIntegralChunk), 0, st, b,
+    hipLaunchKernelGGL(k_grad_h0<2>, dim3(nwork - nsp), dim3(pl.int_chunk), 0, st, b,
                        db.d_work_mol + nsp, db.d_work_kind + nsp, db.d_work_start + nsp);
   const int npw = (int)pl.pair_mol.size();
   hipLaunchKernelGGL(k_grad_pairs, dim3(npw), dim3(256), 0, st, b, db.d_pair_mol, db.d_pair_a0,
@@ -2256,6 +2266,11 @@ int tblite_amd_get_tuning(tblite_amd_context ctx, tblite_amd_tuning* out)
   out->wg_p = ctx->wg_p;
   out->grid_b = ctx->grid_b;
   out->grid_c = ctx->grid_c;
+  out->integral_chunk = ctx->int_chunk;
+  out->pair_chunk_log2 = ctx->pair_chunk_log2;
+  out->atm_chunk_log2 = ctx->atm_chunk_log2;
+  out->fine_buckets = ctx->fine_buckets ? 1 : 0;
+  out->big_lds_min_kb = (int)(ctx->big_lds_min / 1024);
   return TBLITE_AMD_OK;
 }
 
@@ -2269,6 +2284,13 @@ int tblite_amd_set_tuning(tblite_amd_context ctx, const tblite_amd_tuning* in)
               "wg_a, wg_p > 0, grid_b >= 0", kLdsMaxLd, TA_BIG_MAXLD);
     return TBLITE_AMD_ERR_ARGUMENT;
   }
+  if ((in->integral_chunk != 32 && in->integral_chunk != 64 && in->integral_chunk != 128) ||
+      in->pair_chunk_log2 < 8 || in->pair_chunk_log2 > 22 || in->atm_chunk_log2 < 10 ||
+      in->atm_chunk_log2 > 26 || in->big_lds_min_kb < 0 || in->big_lds_min_kb > 160) {
+    set_error(ctx, "set_tuning: integral_chunk must be 32, 64 or 128, pair_chunk_log2 8..22, "
+              "atm_chunk_log2 10..26, big_lds_min_kb 0..160");
+    return TBLITE_AMD_ERR_ARGUMENT;
+  }
   ctx->lds_max_ld = in->lds_max_ld;
   ctx->grid_min_ld = in->grid_min_ld;
   ctx->auto_grid = in->auto_grid != 0;
@@ -2277,6 +2299,11 @@ int tblite_amd_set_tuning(tblite_amd_context ctx, const tblite_amd_tuning* in)
   ctx->wg_p = in->wg_p;
   ctx->grid_b = in->grid_b;
   ctx->grid_c = in->grid_c;
+  ctx->int_chunk = in->integral_chunk;
+  ctx->pair_chunk_log2 = in->pair_chunk_log2;
+  ctx->atm_chunk_log2 = in->atm_chunk_log2;
+  ctx->fine_buckets = in->fine_buckets != 0;
+  ctx->big_lds_min = (size_t)in->big_lds_min_kb * 1024;
   ctx->prepared = nullptr;  // a prepared batch was planned with the old routing
   return TBLITE_AMD_OK;
 }

@@ -81,6 +81,8 @@ class Tuning(C.Structure):
         ("lds_max_ld", C.c_int), ("grid_min_ld", C.c_int), ("auto_grid", C.c_int),
         ("grid_graph", C.c_int), ("wg_a", C.c_double), ("wg_p", C.c_double),
         ("grid_b", C.c_double), ("grid_c", C.c_double),
+        ("integral_chunk", C.c_int), ("pair_chunk_log2", C.c_int), ("atm_chunk_log2", C.c_int),
+        ("fine_buckets", C.c_int), ("big_lds_min_kb", C.c_int),
     ]
 
 

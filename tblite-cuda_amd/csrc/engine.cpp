@@ -1176,6 +1176,7 @@ int run_grid_molecule(tblite_amd_context ctx, Calculator& c, int m)
   g.wbt = take((int64_t)TA_BIG_NB * ld);
   g.part = take(nbm * nbm * 64);
   g.svp = take(nbm);
+  g.dpart = take(nbm * 2 * TA_BIG_NP);
   g.part11 = take(11 * (int64_t)n);
   g.epart = take(nsh + nat);
   g.d = take(ld);

@@ -157,23 +157,32 @@ __host__ __device__ void shell_pair_grad(const double* aj, const double* cj, int
       }
       // v1[axis][pi][pj][moment], pj up to LJ + 1
       double v1[3][LI + 1][LJ + 2][3];
+#pragma unroll
       for (int ax = 0; ax < 3; ++ax) {
         const double rpi = -vec[ax] * eb * oab;
         const double rpj = vec[ax] * ea * oab;
+#pragma unroll
         for (int pi = 0; pi <= LI; ++pi)
+#pragma unroll
           for (int pj = 0; pj <= LJ + 1; ++pj) {
             double vv[LL + 1];
+#pragma unroll
             for (int l = 0; l <= LL; ++l) vv[l] = 0.0;
+#pragma unroll
             for (int a = 0; a <= pi; ++a) {
               double pa = bin[pi][a];
+#pragma unroll
               for (int e = 0; e < pi - a; ++e) pa *= rpi;
+#pragma unroll
               for (int c = 0; c <= pj; ++c) {
                 double pc = bin[pj][c];
+#pragma unroll
                 for (int e = 0; e < pj - c; ++e) pc *= rpj;
                 vv[a + c] += pa * pc;
               }
             }
             double e0 = 0.0, e1 = 0.0, e2 = 0.0;
+#pragma unroll
             for (int l = 0; l <= pi + pj; ++l) {
               e0 += s1d[l] * vv[l];
               e1 += (s1d[l + 1] + rpi * s1d[l]) * vv[l];
@@ -184,9 +193,11 @@ __host__ __device__ void shell_pair_grad(const double* aj, const double* cj, int
             v1[ax][pi][pj][2] = e2;
           }
       }
+#pragma unroll
       for (int mi = 0; mi < MI; ++mi) {
         int ip3[3];
         gcart_pow(LI, mi, ip3[0], ip3[1], ip3[2]);
+#pragma unroll
         for (int mj = 0; mj < MJ; ++mj) {
           int jp3[3];
           gcart_pow(LJ, mj, jp3[0], jp3[1], jp3[2]);
@@ -195,8 +206,10 @@ __host__ __device__ void shell_pair_grad(const double* aj, const double* cj, int
           if (wa == 0.0 && wp == 0.0) continue;
           // value and derivative factors per axis and moment
           double A[3][3], dA[3][3];
+#pragma unroll
           for (int ax = 0; ax < 3; ++ax) {
             const int pi = ip3[ax], pj = jp3[ax];
+#pragma unroll
             for (int mo = 0; mo < 3; ++mo) {
               A[ax][mo] = v1[ax][pi][pj][mo];
               const double dn = pj > 0 ? v1[ax][pi][pj - 1][mo] : 0.0;
@@ -207,6 +220,7 @@ __host__ __device__ void shell_pair_grad(const double* aj, const double* cj, int
           const double Di[3] = {A[0][1] * A[1][0] * A[2][0], A[0][0] * A[1][1] * A[2][0],
                                 A[0][0] * A[1][0] * A[2][1]};
           psh += wp * S;
+#pragma unroll
           for (int m = 0; m < 3; ++m) {
             // factors with the derivative on axis m
             const double* X = m == 0 ? dA[0] : A[0];

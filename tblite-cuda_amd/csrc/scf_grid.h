@@ -31,6 +31,7 @@ struct GridMol {
   double *scr;                    // inverse iteration factors, 3 n^2
   double *vt, *wt, *wbt, *part;   // panel vectors, W of the back-transform, symv partials
   double *svp;                    // per 64-row block partial sums of y.v (nbmax)
+  double *dpart;                  // per 64-row block partial panel dot products (nbmax x 2 NP)
   double *part11, *epart;
   double *d, *e, *tau, *e2, *lam, *prevlam, *v, *y, *focc, *fs, *vsh, *vat, *vdp, *vqp, *dots, *tm;
   double *sc;
@@ -51,6 +52,7 @@ inline int64_t grid_scratch_doubles(int n, int ld, int nat, int nsh)
   int64_t s = 3 * (int64_t)n * n;                          // scr
   s += (int64_t)(2 * TA_BIG_NP + TA_BIG_NB) * ld;          // vt, wt, wbt
   s += nbm * nbm * 64 + nbm;                               // symv partials, y.v partials
+  s += nbm * 2 * TA_BIG_NP;                                // panel dot partials
   s += 11 * (int64_t)n + (nsh + nat);                      // Mulliken partials, energy parts
   s += 10 * (int64_t)ld + nsh + 11 * (int64_t)nat;         // vectors
   s += 2 * TA_BIG_NP + TA_BIG_NB * TA_BIG_NB + GS_COUNT;   // dots, tm, scalars

@@ -258,55 +258,102 @@ __global__ void k_g_symmetrize(GridMol g)
 }
 
 // ------------------------------------------------------ tridiagonalisation
-// Column j (panel position i): update with the panel's reflectors, form the
-// reflector, store it (A column, panel row, vector v), panel dot products.
-__global__ __launch_bounds__(512) void k_g_tri_col(GridMol g, int j, int i)
+// Per column j (panel position i) four kernels: the column brought up to
+// date across the device (k_g_tri_colupd), its reflector (k_g_tri_refl, one
+// workgroup), the symmetric product with the panel dot products
+// (k_g_tri_symv) and y (k_g_tri_rows); w = y - (tau/2)(y.v) v of a column is
+// formed by the next column's k_g_tri_colupd, or by k_g_tri_fin at the end
+// of a panel.
+#define GC_T 256
+
+// Rows [j, n) over all workgroups.  With i > 0 first w of column j - 1
+// (rows j..), then x_r = a_rj - sum_{q<i}
+// (v_q[r] w_q[j] + w_q[r] v_q[j]) into g.e2 (free until the eigenvalues),
+// d[j] = x_j, and each workgroup's partial sum of x_r^2 over r >= j + 2 into
+// part[blockIdx.x].
+__global__ __launch_bounds__(GC_T) void k_g_tri_colupd(GridMol g, int j, int i)
 {
-  extern __shared__ double smem[];
-  lds_dbl* x = TA_LDS(smem);
-  lds_dbl* scal = x + g.ld;
-  lds_dbl* red = scal + 16;
+  __shared__ double red_[GC_T / 64];
+  lds_dbl* red = TA_LDS(red_);
   const int n = g.n, ld = g.ld, tid = threadIdx.x;
-  const int lane = tid & 63, wid = wave_id();
+  const int r = j + blockIdx.x * GC_T + tid;
   const gbl_dbl* vt = TA_GBL(g.vt);
-  const gbl_dbl* wt = TA_GBL(g.wt);
-  for (int r = j + tid; r < n; r += GT) {
+  double* wt = g.wt;
+  double alpha = 0.0, wj = 0.0;
+  if (i > 0) {
+    // y.v of column j - 1 over its row blocks (rows j..), in block order
+    const int nbp = (n - j + 63) >> 6;
+    double sv = 0.0;
+    if (tid < 64)
+      for (int R = tid; R < nbp; R += 64) sv += g.svp[R];
+    sv = block_sum(sv, red);
+    alpha = -0.5 * g.tau[j - 1] * sv;
+    wj = g.y[j] + alpha * g.v[j];
+  }
+  double x2 = 0.0;
+  if (r < n) {
     double a = g.A[(int64_t)j * ld + r];  // the trailing square is symmetric: row j
-    for (int q = 0; q < i; ++q)
-      a -= vt[(int64_t)q * ld + r] * wt[(int64_t)q * ld + j] +
-           wt[(int64_t)q * ld + r] * vt[(int64_t)q * ld + j];
-    x[r] = a;
+    for (int q = 0; q < i; ++q) {
+      const double wr = (q == i - 1) ? g.y[r] + alpha * g.v[r] : wt[(int64_t)q * ld + r];
+      const double wjq = (q == i - 1) ? wj : wt[(int64_t)q * ld + j];
+      a -= vt[(int64_t)q * ld + r] * wjq + wr * vt[(int64_t)q * ld + j];
+    }
+    if (i > 0) wt[(int64_t)(i - 1) * ld + r] = g.y[r] + alpha * g.v[r];
+    g.e2[r] = a;
+    if (r == j) g.d[j] = a;
+    if (r >= j + 2) x2 = a * a;
   }
-  __syncthreads();
-  if (tid == 0) g.d[j] = x[j];
-  big::householder<GT>(x, j + 1, n, scal);
+  x2 = block_sum(x2, red);
+  if (tid == 0) g.part[blockIdx.x] = x2;
+}
+
+// Reflector of column j from x = g.e2[j+1..n) (dlarfg, as big::householder):
+// stored as A column j, panel row i of vt and g.v; e[j], tau[j].  Rows <= j
+// of the panel vectors are never read on this path (the column update, the
+// products and the panel GEMM read rows > j of vector q >= the panel's first
+// column), so they are not cleared.
+__global__ __launch_bounds__(512) void k_g_tri_refl(GridMol g, int j, int i)
+{
+  __shared__ double red_[16];
+  __shared__ double scal_[2];
+  lds_dbl* red = TA_LDS(red_);
+  lds_dbl* scal = TA_LDS(scal_);
+  const int n = g.n, ld = g.ld, tid = threadIdx.x;
+  const int nwg = (n - j + GC_T - 1) / GC_T;
+  double s = 0.0;
+  if (tid < 64)
+    for (int b = tid; b < nwg; b += 64) s += g.part[b];
+  s = block_sum(s, red);
   if (tid == 0) {
-    g.e[j] = scal[0];
-    g.tau[j] = scal[1];
-  }
-  for (int r = j + 1 + tid; r < n; r += GT) {
-    const double xr = x[r];
-    g.A[(int64_t)r * ld + j] = xr;
-    g.vt[(int64_t)i * ld + r] = xr;
-    g.v[r] = xr;
+    const double alpha = g.e2[j + 1];
+    double beta, tau, scale;
+    if (s == 0.0) {
+      beta = alpha;
+      tau = 0.0;
+      scale = 0.0;
+    } else {
+      beta = -copysign(sqrt(alpha * alpha + s), alpha);
+      tau = (beta - alpha) / beta;
+      scale = 1.0 / (alpha - beta);
+    }
+    g.e[j] = beta;
+    g.tau[j] = tau;
+    scal[0] = scale;
   }
   __syncthreads();
-  // dots[q] = w_q . v, dots[NP + q] = v_q . v over rows > j
-  for (int pq = wid; pq < 2 * i; pq += GT / 64) {
-    const int q = pq < i ? pq : pq - i;
-    const gbl_dbl* col = (pq < i ? wt : vt) + (int64_t)q * ld;
-    double s = 0.0;
-    for (int r = j + 1 + lane; r < n; r += 64) s += col[r] * x[r];
-    s = wave_sum(s);
-    if (lane == 0) g.dots[(pq < i ? 0 : TA_BIG_NP) + q] = s;
+  const double scale = scal[0];
+  for (int r = j + 1 + tid; r < n; r += GT) {
+    const double v = (r == j + 1) ? 1.0 : g.e2[r] * scale;
+    g.A[(int64_t)r * ld + j] = v;
+    g.vt[(int64_t)i * ld + r] = v;
+    g.v[r] = v;
   }
-  (void)red;
 }
 
 // Symmetric product over 64 x 64 tiles (bi >= bj) of the trailing triangle
 // [lo, n): tile (bi, bj) writes its row sums to part[bi][bj] and, off the
 // diagonal, its column sums to part[bj][bi] (64 doubles each).
-__global__ __launch_bounds__(256) void k_g_tri_symv(GridMol g, int lo)
+__global__ __launch_bounds__(256) void k_g_tri_symv(GridMol g, int lo, int i)
 {
   __shared__ double t_[64 * 65];
   __shared__ double vr_[64], vc_[64];
@@ -354,30 +401,72 @@ __global__ __launch_bounds__(256) void k_g_tri_symv(GridMol g, int lo)
     cs += __shfl_xor(cs, 2, 4);
     if (qq == 0) g.part[((int64_t)bj * nbm + bi) * 64 + rr] = cs;
   }
+  // diagonal tiles: the block's share of the panel dot products
+  // dpart[bi][q] = w_q . v, dpart[bi][NP + q] = v_q . v over its rows
+  if (bi == bj && i > 0) {
+    const int lane = tid & 63, w = tid >> 6;
+    const int gr = r0 + lane;
+    const double vl = vr[lane];
+    for (int pq = w; pq < 2 * i; pq += 4) {
+      const int q = pq < i ? pq : pq - i;
+      const double* col = (pq < i ? g.wt : g.vt) + (int64_t)q * ld;
+      double t = gr < n ? col[gr] * vl : 0.0;
+      t = wave_sum(t);
+      if (lane == 0) g.dpart[(int64_t)bi * 2 * TA_BIG_NP + (pq < i ? 0 : TA_BIG_NP) + q] = t;
+    }
+  }
 }
 
 // y = tau (sum of partials - V (W^T v) - W (V^T v)) over one 64-row block
-// per wave (blockIdx.x), and the block's share of y.v in svp
-__global__ __launch_bounds__(64) void k_g_tri_rows(GridMol g, int j, int i)
+// (blockIdx.x; four waves split the partial columns), the panel dot
+// products summed from the diagonal tiles' shares, and the block's share of
+// y.v in svp
+__global__ __launch_bounds__(256) void k_g_tri_rows(GridMol g, int j, int i)
 {
+  __shared__ double acc_[4 * 64];
+  __shared__ double dacc_[8 * 2 * TA_BIG_NP];
+  __shared__ double dots_[2 * TA_BIG_NP];
+  lds_dbl* acc = TA_LDS(acc_);
+  lds_dbl* dacc = TA_LDS(dacc_);
+  lds_dbl* dots = TA_LDS(dots_);
   const int n = g.n, ld = g.ld, lo = j + 1, m = n - lo;
   const int nb = (m + 63) >> 6;
   const int64_t nbm = g.nbmax;
-  const int R = blockIdx.x, l = threadIdx.x;
-  const int r = lo + 64 * R + l;
-  const double tj = g.tau[j];
-  double sv = 0.0;
-  if (r < n) {
-    double y = 0.0;
-    for (int q = 0; q < nb; ++q) y += g.part[((int64_t)R * nbm + q) * 64 + l];
-    for (int q = 0; q < i; ++q)
-      y -= g.vt[(int64_t)q * ld + r] * g.dots[q] + g.wt[(int64_t)q * ld + r] * g.dots[TA_BIG_NP + q];
-    y *= tj;
-    g.y[r] = y;
-    sv = y * g.v[r];
+  const int R = blockIdx.x, tid = threadIdx.x, l = tid & 63, w = tid >> 6;
+  // panel dots: 32 slots x 8 groups of row blocks, then the 8 partials in order
+  static_assert(2 * TA_BIG_NP == 32, "slot layout assumes 16 panel columns");
+  {
+    const int slot = tid & 31, grp = tid >> 5;
+    double s = 0.0;
+    if ((slot & 15) < i)
+      for (int b = grp; b < nb; b += 8) s += g.dpart[(int64_t)b * 2 * TA_BIG_NP + slot];
+    dacc[grp * 32 + slot] = s;
   }
-  sv = wave_sum(sv);
-  if (l == 0) g.svp[R] = sv;
+  double s = 0.0;
+  for (int q = w; q < nb; q += 4) s += g.part[((int64_t)R * nbm + q) * 64 + l];
+  acc[w * 64 + l] = s;
+  __syncthreads();
+  if (tid < 32) {
+    double t = 0.0;
+    for (int gi = 0; gi < 8; ++gi) t += dacc[gi * 32 + tid];
+    dots[tid] = t;
+  }
+  __syncthreads();
+  if (w == 0) {
+    const int r = lo + 64 * R + l;
+    const double tj = g.tau[j];
+    double sv = 0.0;
+    if (r < n) {
+      double y = ((acc[l] + acc[64 + l]) + acc[128 + l]) + acc[192 + l];
+      for (int q = 0; q < i; ++q)
+        y -= g.vt[(int64_t)q * ld + r] * dots[q] + g.wt[(int64_t)q * ld + r] * dots[TA_BIG_NP + q];
+      y *= tj;
+      g.y[r] = y;
+      sv = y * g.v[r];
+    }
+    sv = wave_sum(sv);
+    if (l == 0) g.svp[R] = sv;
+  }
 }
 
 // w = y - (tau/2)(y.v) v, y.v summed over the row blocks in order
@@ -1276,17 +1365,18 @@ hipError_t grid_eigenvalues(GridMol& g, hipStream_t s, bool warm)
   const int n = g.n, ld = g.ld;
   const int nr = n - 2;
   constexpr int NP = TA_BIG_NP;
-  const size_t col_lds = sizeof(double) * (ld + 32 + 16);
   for (int k0 = 0; k0 < nr; k0 += NP) {
     const int np = (k0 + NP <= nr) ? NP : nr - k0;
     for (int i = 0; i < np; ++i) {
       const int j = k0 + i;
-      hipLaunchKernelGGL(k_g_tri_col, dim3(1), dim3(GT), col_lds, s, g, j, i);
+      hipLaunchKernelGGL(k_g_tri_colupd, dim3((n - j + GC_T - 1) / GC_T), dim3(GC_T), 0, s, g, j, i);
+      hipLaunchKernelGGL(k_g_tri_refl, dim3(1), dim3(GT), 0, s, g, j, i);
       const int m = n - (j + 1), nb = (m + 63) >> 6;
-      hipLaunchKernelGGL(k_g_tri_symv, dim3(nb * (nb + 1) / 2), dim3(256), 0, s, g, j + 1);
-      hipLaunchKernelGGL(k_g_tri_rows, dim3(nb), dim3(64), 0, s, g, j, i);
-      hipLaunchKernelGGL(k_g_tri_fin, dim3(1), dim3(GT), 0, s, g, j, i);
+      hipLaunchKernelGGL(k_g_tri_symv, dim3(nb * (nb + 1) / 2), dim3(256), 0, s, g, j + 1, i);
+      hipLaunchKernelGGL(k_g_tri_rows, dim3(nb), dim3(256), 0, s, g, j, i);
     }
+    // w of the panel's last column (the next column's update forms the others)
+    hipLaunchKernelGGL(k_g_tri_fin, dim3(1), dim3(GT), 0, s, g, k0 + np - 1, np - 1);
     const int r0 = k0 + np;
     if (r0 >= nr) {
       hipLaunchKernelGGL(k_g_tri_last, dim3(1), dim3(64), 0, s, g, np);

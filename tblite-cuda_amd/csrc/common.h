@@ -223,7 +223,8 @@ struct BatchDev {
   // dg_r/dq, dg_r/dCN at the final charge, g_r(0), dg_r(0)/dCN); per shell
   // the shell potential including the atomic one; per molecule (mol_mat_off,
   // column-major) the energy-weighted density W = sum_k f_k e_k c_k c_k^T;
-  // per atom pair (mol_pair_off, lower triangle i > j at i*nat + j) dE3/dC6
+  // per atom pair (mol_pair_off, i*nat + k) dC6_ik(0)/dCN_i of the
+  // three-body term
   double* grad;
   double* g_dcn;
   double* g_dcn4;
@@ -233,7 +234,7 @@ struct BatchDev {
   double* g_d4;
   double* g_vsh;
   double* g_w;
-  double* g_dc6;
+  double* g_c6d;
   // optional per-molecule phase clocks of the SCF kernel (TA_PROF_SLOTS each), may be null
   long long* prof;
   // optional completion records (one per molecule, pinned host memory), may be null

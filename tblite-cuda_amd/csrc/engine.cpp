@@ -43,8 +43,9 @@ template <int LMAX>
 __global__ void k_grad_h0(BatchDev b, const int* work_mol, const int* work_kind,
                           const int* work_start);
 __global__ void k_grad_pairs(BatchDev b, const int* work_mol, const int* work_a0, const int* work_a1);
-__global__ void k_grad_atm(BatchDev b, const int* work_mol, const int* work_i0, const int* work_i1);
-__global__ void k_grad_c6(BatchDev b, const int* work_mol, const int* work_a0, const int* work_a1);
+__global__ void k_grad_c6d(BatchDev b, const int* work_mol, const int* work_a0, const int* work_a1);
+__global__ void k_grad_atm(BatchDev b, const int* work_mol, const int* work_i0, const int* work_i1,
+                           int lds);
 __global__ void k_grad_cn(BatchDev b, const int* work_mol, const int* work_a0, const int* work_a1);
 
 namespace {
@@ -1055,7 +1056,7 @@ int upload_plan(tblite_amd_context ctx, Calculator& c, const Plan& pl, DeviceBat
     b.g_d4 = carve<double>(ar, (int64_t)TA_GD4 * TA_D4_MAXREF * A);
     b.g_vsh = carve<double>(ar, Sh);
     b.g_w = carve<double>(ar, pl.tot_mat);
-    b.g_dc6 = carve<double>(ar, pl.tot_pair);
+    b.g_c6d = carve<double>(ar, pl.tot_pair);
     gradient_tiles(pl, db.h_wtiles);
     db.d_wtiles = carve<int>(ar, (int64_t)db.h_wtiles.size());
     if (ar.used <= ar.cap && !db.h_wtiles.empty())
@@ -1383,10 +1384,16 @@ int run_gradient(tblite_amd_context ctx)
                      db.d_pair_a1);
   const int natm = (int)pl.atm_mol.size();
   if (natm > 0) {
-    hipLaunchKernelGGL(k_grad_atm, dim3(natm), dim3(256), 0, st, b, db.d_atm_mol, db.d_atm_i0,
-                       db.d_atm_i1);
-    hipLaunchKernelGGL(k_grad_c6, dim3(npw), dim3(256), 0, st, b, db.d_pair_mol, db.d_pair_a0,
+    hipLaunchKernelGGL(k_grad_c6d, dim3(npw), dim3(256), 0, st, b, db.d_pair_mol, db.d_pair_a0,
                        db.d_pair_a1);
+    // per-workgroup LDS accumulation of the three-body terms when 4 natmax
+    // doubles fit (LDS per workgroup of the device)
+    int natmax = 0;
+    for (int m = 0; m < pl.nmol; ++m) natmax = std::max(natmax, pl.mol_nat[m]);
+    const size_t lds_bytes = sizeof(double) * 4 * (size_t)natmax;
+    const int use_lds = lds_bytes <= std::min<size_t>(ctx->lds_limit, 64 * 1024) ? 1 : 0;
+    hipLaunchKernelGGL(k_grad_atm, dim3(natm), dim3(256), use_lds ? lds_bytes : 0, st, b,
+                       db.d_atm_mol, db.d_atm_i0, db.d_atm_i1, use_lds);
   }
   hipLaunchKernelGGL(k_grad_cn, dim3(npw), dim3(256), 0, st, b, db.d_pair_mol, db.d_pair_a0,
                      db.d_pair_a1);

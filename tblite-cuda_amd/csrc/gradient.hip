@@ -22,8 +22,9 @@
 //   k_grad_h0     one thread per neighbour pair: integral derivatives
 //                 contracted on the fly (grad_pair.h), atomics per atom
 //   k_grad_pairs  one wave per atom: repulsion, ES2, AES2, D4 two-body
-//   k_grad_atm    three-body dispersion triples: geometry and dE3/dC6
-//   k_grad_c6     one wave per atom: dE3/dC6 -> dE/dCN(D4)
+//   k_grad_c6d    pair planes: CN derivatives of the three-body C6
+//   k_grad_atm    three-body dispersion triples: geometry and dE/dCN(D4),
+//                 accumulated in LDS per workgroup
 //   k_grad_cn     one wave per atom: both CN chains
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -80,11 +81,6 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_grad_pot(BatchDev b)
       d4_weights(d4, el[i], cn4, 0.0, w + 3 * K, tmp);
       d4_weights_dcn(d4, el[i], cn4, 0.0, w + 4 * K);
     }
-  }
-  // dE3/dC6 plane
-  if (d4.enabled) {
-    double* __restrict__ dc6 = b.g_dc6 + b.mol_pair_off[m];
-    for (int64_t x = tid; x < (int64_t)nat * nat; x += blockDim.x) dc6[x] = 0.0;
   }
   __syncthreads();
 
@@ -530,30 +526,80 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_grad_pairs(BatchDev b,
   }
 }
 
+// ------------------------------------------------------------ k_grad_c6d
+// Per atom pair the CN derivatives of the zero-charge C6 of the three-body
+// term: c6d(i, k) = dC6_ik(0)/dCN_i = sum_rs dg0_ir/dCN C6ref(ir, ks) g0_ks
+// (row i of the pair plane; column k's own derivative is c6d(k, i)).  Work
+// items of the pair planes, threads over the pairs of their rows.
+__global__ __launch_bounds__(GRAD_THREADS) void k_grad_c6d(BatchDev b, const int* __restrict__ work_mol,
+                                                           const int* __restrict__ work_a0,
+                                                           const int* __restrict__ work_a1)
+{
+  const int w = blockIdx.x, tid = threadIdx.x;
+  const int m = work_mol[w], ab = work_a0[w], ae = work_a1[w];
+  const int a0 = b.mol_atom_off[m], nat = b.mol_nat[m];
+  const int* __restrict__ el = b.at_el + a0;
+  const DevD4& d4 = b.par->d4;
+  const double* __restrict__ gd4 = b.g_d4 + (int64_t)TA_GD4 * TA_D4_MAXREF * a0;
+  double* __restrict__ c6d = b.g_c6d + b.mol_pair_off[m];
+  constexpr int K = TA_D4_MAXREF;
+  const int64_t p0 = (int64_t)ab * nat, p1 = (int64_t)ae * nat;
+  for (int64_t idx = p0 + tid; idx < p1; idx += GRAD_THREADS) {
+    const int i = (int)(idx / nat), k = (int)(idx - (int64_t)i * nat);
+    double v = 0.0;
+    if (i != k) {
+      double acc[K] = {0, 0, 0, 0, 0, 0, 0};
+      d4_accumulate(d4, el[i], el[k], 1.0, gd4 + (int64_t)TA_GD4 * K * k + 3 * K, acc);
+      const double* gc0 = gd4 + (int64_t)TA_GD4 * K * i + 4 * K;
+      for (int r = 0; r < d4.nref[el[i]]; ++r) v += gc0[r] * acc[r];
+    }
+    c6d[idx] = v;
+  }
+}
+
 // ------------------------------------------------------------ k_grad_atm
 // Three-body dispersion (zero-charge C6 of k_setup_pairs): for the triple
 // with squared distances x (ij), y (ik), z (jk) and
 //   E = c9 ang fdmp / (x y z)^(3/2),  ang = 3/8 A B C / (x y z) + 1,
 //   A = x + z - y, B = x - z + y, C = -x + z + y,
 //   fdmp = 1 / (1 + 6 (R0 / sqrt(x y z))^(alp/3)),
-// the gradient follows from dE/dx, dE/dy, dE/dz; dE/dC6 = E / (2 C6) for
-// each of the three pairs.  Work items and pair numbering as k_d4_atm; each
-// thread keeps its pair's sums in registers and adds the third atom's and
-// the other pairs' terms atomically.
+// the gradient follows from dE/dx, dE/dy, dE/dz, and dE/dC6 = E / (2 C6)
+// for each of the three pairs goes straight into dE/dCN(D4) through the
+// pair plane c6d of k_grad_c6d.  Work items and pair numbering as
+// k_d4_atm.  With lds != 0 the workgroup accumulates per-atom gradient and
+// CN terms in LDS (4 nat doubles of dynamic shared memory, LDS atomics) and
+// adds them to the global arrays once at the end; otherwise every term is a
+// global atomic.
 __device__ __forceinline__ int64_t tri_count_g(int64_t i) { return i * (i - 1) / 2; }
 
 __global__ __launch_bounds__(GRAD_THREADS) void k_grad_atm(BatchDev b, const int* __restrict__ work_mol,
                                                            const int* __restrict__ work_i0,
-                                                           const int* __restrict__ work_i1)
+                                                           const int* __restrict__ work_i1, int lds)
 {
+  extern __shared__ double smem[];
   const int w = blockIdx.x, tid = threadIdx.x;
   const int m = work_mol[w], i0 = work_i0[w], i1 = work_i1[w];
   const int a0 = b.mol_atom_off[m], nat = b.mol_nat[m];
   const double* __restrict__ xyz = b.xyz + 3 * (int64_t)a0;
   const int* __restrict__ el = b.at_el + a0;
   const double* __restrict__ c60 = b.d4_c60 + b.mol_pair_off[m];
-  double* __restrict__ dc6 = b.g_dc6 + b.mol_pair_off[m];
+  const double* __restrict__ c6d = b.g_c6d + b.mol_pair_off[m];
   double* __restrict__ grad = b.grad + 3 * (int64_t)a0;
+  double* __restrict__ dcn4 = b.g_dcn4 + a0;
+  lds_dbl* lg = TA_LDS(smem);          // 3 nat gradient terms
+  lds_dbl* lc = lg + 3 * nat;          // nat CN terms
+  if (lds) {
+    for (int x = tid; x < 4 * nat; x += GRAD_THREADS) lg[x] = 0.0;
+    __syncthreads();
+  }
+  auto add_g = [&](int a, int x, double v) {
+    if (lds) __hip_atomic_fetch_add(lg + 3 * a + x, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    else atomicAdd(&grad[3 * a + x], v);
+  };
+  auto add_c = [&](int a, double v) {
+    if (lds) __hip_atomic_fetch_add(lc + a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    else atomicAdd(&dcn4[a], v);
+  };
   const DevD4& d = b.par->d4;
   const double cut2 = d.disp3_cutoff * d.disp3_cutoff;
   const double p3 = d.alp / 3.0;
@@ -571,7 +617,7 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_grad_atm(BatchDev b, const int
     const double ri = d.r4r2[el[i]], rj = d.r4r2[el[j]];
     const double r0ij = d.a1 * sqrt(3.0 * ri * rj) + d.a2;
     const double cij = c60[(int64_t)i * nat + j];
-    double gi[3] = {0, 0, 0}, gj[3] = {0, 0, 0}, dcij = 0.0;
+    double gi[3] = {0, 0, 0}, gj[3] = {0, 0, 0}, fij = 0.0, ci = 0.0, cj = 0.0;
     for (int k = 0; k < j; ++k) {
       double rik[3], rjk[3];
       for (int x = 0; x < 3; ++x) {
@@ -605,52 +651,38 @@ __global__ __launch_bounds__(GRAD_THREADS) void k_grad_atm(BatchDev b, const int
       for (int x = 0; x < 3; ++x) {
         gi[x] -= 2.0 * (de[0] * rij[x] + de[1] * rik[x]);
         gj[x] += 2.0 * (de[0] * rij[x] - de[2] * rjk[x]);
-        atomicAdd(&grad[3 * k + x], 2.0 * (de[1] * rik[x] + de[2] * rjk[x]));
+        add_g(k, x, 2.0 * (de[1] * rik[x] + de[2] * rjk[x]));
       }
-      if (cij != 0.0) dcij += 0.5 * et / cij;
-      if (cik != 0.0) atomicAdd(&dc6[(int64_t)i * nat + k], 0.5 * et / cik);
-      if (cjk != 0.0) atomicAdd(&dc6[(int64_t)j * nat + k], 0.5 * et / cjk);
+      // dE/dC6 = E / (2 C6) of each pair, into the CN terms of its two atoms
+      if (cij != 0.0) fij += 0.5 * et / cij;
+      double ck = 0.0;
+      if (cik != 0.0) {
+        const double f = 0.5 * et / cik;
+        ci += f * c6d[(int64_t)i * nat + k];
+        ck += f * c6d[(int64_t)k * nat + i];
+      }
+      if (cjk != 0.0) {
+        const double f = 0.5 * et / cjk;
+        cj += f * c6d[(int64_t)j * nat + k];
+        ck += f * c6d[(int64_t)k * nat + j];
+      }
+      if (ck != 0.0) add_c(k, ck);
     }
+    ci += fij * c6d[(int64_t)i * nat + j];
+    cj += fij * c6d[(int64_t)j * nat + i];
     for (int x = 0; x < 3; ++x) {
-      atomicAdd(&grad[3 * i + x], gi[x]);
-      atomicAdd(&grad[3 * j + x], gj[x]);
+      add_g(i, x, gi[x]);
+      add_g(j, x, gj[x]);
     }
-    if (dcij != 0.0) atomicAdd(&dc6[(int64_t)i * nat + j], dcij);
+    if (ci != 0.0) add_c(i, ci);
+    if (cj != 0.0) add_c(j, cj);
   }
-}
-
-// ------------------------------------------------------------- k_grad_c6
-// dE/dCN(D4)_a += sum_j dE3/dC6_aj sum_rs dg0_ar/dCN C6ref(ar, js) g0_js
-__global__ __launch_bounds__(GRAD_THREADS) void k_grad_c6(BatchDev b, const int* __restrict__ work_mol,
-                                                          const int* __restrict__ work_a0,
-                                                          const int* __restrict__ work_a1)
-{
-  const int w = blockIdx.x;
-  const int m = work_mol[w], ab = work_a0[w], ae = work_a1[w];
-  const int lane = threadIdx.x & (TA_WAVE - 1), wv = wave_id();
-  const int nwv = GRAD_THREADS / TA_WAVE;
-  const int a0 = b.mol_atom_off[m], nat = b.mol_nat[m];
-  const int* __restrict__ el = b.at_el + a0;
-  const DevD4& d4 = b.par->d4;
-  const double* __restrict__ gd4 = b.g_d4 + (int64_t)TA_GD4 * TA_D4_MAXREF * a0;
-  const double* __restrict__ dc6 = b.g_dc6 + b.mol_pair_off[m];
-  constexpr int K = TA_D4_MAXREF;
-  for (int a = ab + wv; a < ae; a += nwv) {
-    const int ea = el[a];
-    const double* gc0 = gd4 + (int64_t)TA_GD4 * K * a + 4 * K;
-    double s = 0.0;
-    for (int j = lane; j < nat; j += TA_WAVE) {
-      if (j == a) continue;
-      const double wgt = a > j ? dc6[(int64_t)a * nat + j] : dc6[(int64_t)j * nat + a];
-      if (wgt == 0.0) continue;
-      double acc[K] = {0, 0, 0, 0, 0, 0, 0};
-      d4_accumulate(d4, ea, el[j], 1.0, gd4 + (int64_t)TA_GD4 * K * j + 3 * K, acc);
-      double t = 0.0;
-      for (int r = 0; r < d4.nref[ea]; ++r) t += gc0[r] * acc[r];
-      s += wgt * t;
-    }
-    s = wave_sum(s);
-    if (lane == 0) b.g_dcn4[a0 + a] += s;
+  if (lds) {
+    __syncthreads();
+    for (int x = tid; x < 3 * nat; x += GRAD_THREADS)
+      if (lg[x] != 0.0) atomicAdd(&grad[x], (double)lg[x]);
+    for (int a = tid; a < nat; a += GRAD_THREADS)
+      if (lc[a] != 0.0) atomicAdd(&dcn4[a], (double)lc[a]);
   }
 }
 

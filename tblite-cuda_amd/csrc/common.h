@@ -216,6 +216,10 @@ struct BatchDev {
   // eigensolver of the LDS-resident kernels: 1 Householder tridiagonalisation
   // + bisection + twisted-factorisation eigenvectors, 0 warm-started Jacobi
   int lds_eig;
+  // reduction to tridiagonal form of the global-memory kernel (ld <=
+  // TA_BAND_MAXLD): 1 two-stage (dense -> band -> tridiagonal), 0 one-stage
+  // Householder
+  int big_eig;
   // analytic gradient (all null unless a gradient is requested): per atom
   // dE/dR (3 each), dE/dCN of the GFN and of the D4 coordination number, the
   // atomic, dipole and quadrupole potentials of the final moments (1, 3, 6
@@ -270,7 +274,11 @@ struct BatchDev {
 #define TA_PROF_TRI_CORRECT 19
 #define TA_PROF_TRI_PANEL 20
 #define TA_PROF_CLUSTERS 21
-#define TA_PROF_SLOTS 22
+// two-stage reduction: dense -> band, band -> tridiagonal, stage-2 back-transform
+#define TA_PROF_BAND1 22
+#define TA_PROF_BAND2 23
+#define TA_PROF_Q2 24
+#define TA_PROF_SLOTS 25
 
 // Global-memory eigensolver constants: threads per workgroup, reflectors per
 // back-transformation block, columns per tridiagonalisation panel, largest
@@ -298,10 +306,33 @@ __host__ __device__ inline int64_t big_tri_save_offset(int ld)
   return 3 * (int64_t)ld * TA_BIG_NT + (int64_t)(TA_BIG_NB + 2 * TA_BIG_NP) * ld;
 }
 
-__host__ __device__ inline int64_t big_scratch_doubles(int ld)
+// Two-stage (band) reduction of the global-memory eigensolver, after the
+// saved tridiagonal form: stage-1 reflectors (ld x ld), stage-2 reflectors
+// (ld x (ld / 16 + 2) records of 16), two panel W blocks and the panel X
+// block (16 x ld each), stage-1 tau (ld).
+__host__ __device__ inline int64_t band_v1_offset(int ld)
 {
   return big_tri_save_offset(ld) + 3 * (int64_t)ld + 8;
 }
+__host__ __device__ inline int64_t band_v2_offset(int ld)
+{
+  return band_v1_offset(ld) + (int64_t)ld * ld;
+}
+__host__ __device__ inline int64_t band_w_offset(int ld)
+{
+  return band_v2_offset(ld) + (int64_t)ld * (ld / 16 + 2) * 16;
+}
+__host__ __device__ inline int64_t band_x_offset(int ld) { return band_w_offset(ld) + 32 * (int64_t)ld; }
+__host__ __device__ inline int64_t band_tau_offset(int ld) { return band_x_offset(ld) + 16 * (int64_t)ld; }
+
+__host__ __device__ inline int64_t big_scratch_doubles(int ld)
+{
+  return band_tau_offset(ld) + ld;
+}
+
+// largest padded AO dimension of the two-stage reduction (its panel strip
+// is held in registers: 4 blocks of 16 rows per wave)
+#define TA_BAND_MAXLD 512
 
 // LDS layout of the SCF kernel, in doubles (integer arrays in units of ints
 // from the start of the region).  ld is the padded AO dimension of the

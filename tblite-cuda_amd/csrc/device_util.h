@@ -40,6 +40,65 @@ __device__ __forceinline__ double seg16_sum(double v)
   return v;
 }
 
+// Lane exchanges on the VALU (no LDS crossbar): DPP within a 16-lane row,
+// v_permlane16/32_swap across rows.  Every lane of a reduction gets a
+// bit-identical sum (each step adds the same two operands in every lane).
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)b, CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, true);
+  return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// sum over the lane's 16-lane row (quad_perm xor 1, xor 2, row_half_mirror,
+// row_mirror)
+__device__ __forceinline__ double row16_sum(double v)
+{
+  v += dpp_f64<0xB1>(v);
+  v += dpp_f64<0x4E>(v);
+  v += dpp_f64<0x141>(v);
+  v += dpp_f64<0x140>(v);
+  return v;
+}
+
+// v(lane) + v(lane ^ 16)
+__device__ __forceinline__ double xor16_sum(double v)
+{
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)b, (unsigned)b, false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(b >> 32), (unsigned)(b >> 32), false,
+                                                   false);
+  const double a = __builtin_bit_cast(double, ((unsigned long long)hi[0] << 32) | lo[0]);
+  const double c = __builtin_bit_cast(double, ((unsigned long long)hi[1] << 32) | lo[1]);
+  return a + c;
+}
+
+// v(lane) + v(lane ^ 32)
+__device__ __forceinline__ double xor32_sum(double v)
+{
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)b, (unsigned)b, false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(b >> 32), (unsigned)(b >> 32), false,
+                                                   false);
+  const double a = __builtin_bit_cast(double, ((unsigned long long)hi[0] << 32) | lo[0]);
+  const double c = __builtin_bit_cast(double, ((unsigned long long)hi[1] << 32) | lo[1]);
+  return a + c;
+}
+
+// lane 0's value in every lane (scalar broadcast)
+__device__ __forceinline__ double bcast_lane0(double v)
+{
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// wave64 sum on the VALU
+__device__ __forceinline__ double wave_sum_dpp(double v) { return xor32_sum(xor16_sum(row16_sum(v))); }
+
 __device__ __forceinline__ double wave_max(double v)
 {
 #pragma unroll

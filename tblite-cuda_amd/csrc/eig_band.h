@@ -1,0 +1,854 @@
+// Two-stage reduction of a symmetric matrix to tridiagonal form, and the
+// matching back-transformations, for the global-memory eigensolver (one
+// workgroup of NT = 512 threads per molecule).  It replaces the one-stage
+// tridiagonalize_blocked() of eig_big.h in the reduction step of sygvd
+// (src/lapack/sygvd.cu:223-313) with far less memory traffic:
+//
+//   stage 1   dense -> band of half-width BW = 16, in panels of BW columns.
+//             The strip of a panel is held in registers; a Householder QR of
+//             its rows below the band gives Q = I - V T V^T, and the trailing
+//             matrix takes Q^T A2 Q = A2 - V W^T - W V^T with X = A2 V T,
+//             W = X - V (T^T V^T X) / 2.  Panel p's update is applied while
+//             the trailing lower triangle is read for panel p+1's product
+//             X = A2 V, so each panel reads and writes that triangle once
+//             (the one-stage reduction reads it once per column).  The
+//             products run on the FP64 matrix cores.
+//   stage 2   band -> tridiagonal by bulge chasing: sweep s annihilates
+//             column s with a reflector of length BW and chases the bulge it
+//             creates down the band, one BW x BW block step (task) at a time.
+//             A wave runs one task (three blocks in registers); sweeps are
+//             pipelined LAG = 3 tasks apart, which keeps every pair of tasks
+//             of one step on disjoint elements and every pair of tasks that
+//             share an element in sweep order; one barrier per step.  The
+//             band stays in A's lower triangle, with room for the bulges
+//             (2 BW - 1 below the diagonal; stage 1 leaves zeros there).
+//   vectors   Z = Q1 Q2 Y: the stage-2 reflectors applied in reverse order
+//             to 16-column chunks of Y held in LDS (one wave per column pair,
+//             no barriers), then the stage-1 reflectors in compact WY blocks
+//             of 32 (matrix-core products).
+//
+// Global layouts (ld = leading dimension, rows r global):
+//   V1[k * ld + r]           stage-1 reflector k = c0 + j (panel c0, column
+//                            j): zero above row k + BW, 1 there; a reflector
+//                            with tau1[k] = 0 is the identity (zero column)
+//   V2[(first(s) + t) * 16]  stage-2 reflector of sweep s, task t: entry 0 is
+//                            tau, entries 1..15 the vector below its leading 1
+//                            (first(s) = chase_first: records of consecutive
+//                            sweeps are contiguous)
+//   W[k * ld + r], X[k * ld + r]   panel work vectors (16 columns each)
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "device_util.h"
+#include "eig_big.h"
+
+namespace band {
+
+constexpr int BW = 16;             // half-bandwidth after stage 1
+constexpr int QB = 4;              // 16-row strip blocks per wave
+constexpr int MAXN = 8 * QB * BW;  // largest order served (512)
+constexpr int LAG = 3;             // tasks between consecutive sweeps
+
+using big::ta_double4;
+
+// stage-2 tasks of sweep s (s <= n - 3): task 0, then one per further block
+// that still holds at least two rows
+__device__ __forceinline__ int chase_tasks(int n, int s) { return 1 + (n - 3 - s) / BW; }
+
+// sum_{u=0}^{x} floor(u / BW) (0 for x < 0)
+__device__ __forceinline__ int floor_sum(int x)
+{
+  if (x < 0) return 0;
+  const int q = x / BW, r = x - q * BW;
+  return (BW / 2) * q * (q - 1) + q * (r + 1);
+}
+
+// index of the first record of sweep s in the compact stage-2 record array
+// (records of consecutive sweeps are contiguous; record (s, t) = first + t)
+__device__ __forceinline__ int chase_first(int n, int s)
+{
+  return s + floor_sum(n - 3) - floor_sum(n - 3 - s);
+}
+
+// number of stage-1 reflectors (16 per panel; panels while n - c0 - BW >= 2)
+__device__ __forceinline__ int band_reflectors(int n)
+{
+  return n - BW >= 2 ? ((n - BW - 2) / BW + 1) * BW : 0;
+}
+
+__device__ __forceinline__ ta_double4 mfma(double a, double b, ta_double4 c)
+{
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ double pick4(ta_double4 v, int i)
+{
+  return i == 0 ? v[0] : (i == 1 ? v[1] : (i == 2 ? v[2] : v[3]));
+}
+
+// 64 x 64 LDS tile, columns XOR-swizzled by the row so that both a row
+// fragment and a column fragment of 16 lanes spread over the banks
+__device__ __forceinline__ int tix(int r, int c) { return r * 64 + (c ^ ((r & 15) << 2)); }
+
+// ------------------------------------------------------ stage 1: one pass
+// Trailing matrix A2 = A[r2:, r2:] (m = n - r2), lower 64 x 64 tiles in row
+// order.  With upd: A2 -= Vp Wp^T + Wp Vp^T on the lower triangle (stored).
+// With want_x: X = A2 Vc (A2 after the update, 16 columns), from the lower
+// tiles only: tile (I, J) adds A_IJ V_J to X_I and, below the diagonal,
+// A_IJ^T V_I to X_J.  Waves 0-3 keep X_I in registers over a tile row and
+// store it at its end; waves 4-7 add the transposed contributions to rows
+// stored earlier.  tile: 4096 doubles of LDS.
+template <int NT>
+__device__ __noinline__ void band_pass(gbl_dbl* A, int n, int ld, int r2, const gbl_dbl* Vp,
+                                       const gbl_dbl* Wp, bool upd, const gbl_dbl* Vc,
+                                       gbl_dbl* X, bool want_x, lds_dbl* tile)
+{
+  static_assert(NT == 512, "tiling assumes 8 waves");
+  const int lane = threadIdx.x & 63, w = wave_id();
+  const int lr = lane & 15, lk = lane >> 4;
+  const int bi = w >> 1, bj = (w & 1) * 2;
+  const int m = n - r2, nt = (m + 63) >> 6;
+  // loads use clamped (in-range) indices and zero the value afterwards, so
+  // they are branch-free; row-major offsets fit in 32 bits (ld <= 1024)
+  gbl_dbl* A2 = A + (int64_t)r2 * ld + r2;
+  const gbl_dbl* Vr = want_x ? Vc + r2 + lr * ld : nullptr;  // V[row][lr] = Vr[row]
+  for (int I = 0; I < nt; ++I) {
+    const int I0 = I * 64;
+    double au[8], vb[16];
+    {
+      const int row = I0 + 16 * bi + lr, rc = min(row, m - 1);
+#pragma unroll
+      for (int kk = 0; kk < 8; ++kk) {
+        const int k = lk + 4 * kk;
+        double a = 0.0;
+        if (upd) a = k < BW ? Vp[k * ld + r2 + rc] : Wp[(k - BW) * ld + r2 + rc];
+        au[kk] = row < m ? -a : 0.0;
+      }
+    }
+    if (want_x && w >= 4) {
+#pragma unroll
+      for (int kk = 0; kk < 16; ++kk) {
+        const int row = I0 + 4 * kk + lk;
+        const double v = Vr[min(row, m - 1)];
+        vb[kk] = row < m ? v : 0.0;
+      }
+    }
+    ta_double4 xi = {0.0, 0.0, 0.0, 0.0};
+    for (int J = 0; J <= I; ++J) {
+      const int J0 = J * 64;
+      // each wave loads (and later stores) only the lower-triangle elements
+      // of its own blocks: the upper half of a diagonal tile is another
+      // wave's lower half, which that wave may already have updated and
+      // stored, so it is taken from the tile in LDS instead (below)
+      ta_double4 acc[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int row = I0 + 16 * bi + 4 * i + lk, col = J0 + 16 * (bj + h) + lr;
+          const int rc = min(row, m - 1), cc = min(min(col, row), m - 1);
+          const double v = A2[rc * ld + cc];
+          acc[h][i] = (row < m && col <= row) ? v : 0.0;
+        }
+      // waves 0-3: V rows of the tile column, issued with the tile loads
+      if (want_x && w < 4) {
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk) {
+          const int col = J0 + 4 * kk + lk;
+          const double v = Vr[min(col, m - 1)];
+          vb[kk] = col < m ? v : 0.0;
+        }
+      }
+      if (upd) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int col = J0 + 16 * (bj + h) + lr, cc = min(col, m - 1);
+          double bo[8];
+#pragma unroll
+          for (int kk = 0; kk < 8; ++kk) {
+            const int k = lk + 4 * kk;
+            const double b = k < BW ? Wp[k * ld + r2 + cc] : Vp[(k - BW) * ld + r2 + cc];
+            bo[kk] = col < m ? b : 0.0;
+          }
+#pragma unroll
+          for (int kk = 0; kk < 8; ++kk) acc[h] = mfma(au[kk], bo[kk], acc[h]);
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int row = I0 + 16 * bi + 4 * i + lk, col = J0 + 16 * (bj + h) + lr;
+            if (row < m && col <= row) A2[row * ld + col] = acc[h][i];
+          }
+      }
+      if (want_x) {
+        // the full symmetric tile: lower elements at their place and, in a
+        // diagonal tile, mirrored into the upper half
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int tr = 16 * bi + 4 * i + lk, tc = 16 * (bj + h) + lr;
+            if (I != J || tr >= tc) tile[tix(tr, tc)] = acc[h][i];
+            if (I == J && tr > tc) tile[tix(tc, tr)] = acc[h][i];
+          }
+        __syncthreads();
+        if (w < 4) {
+#pragma unroll
+          for (int kk = 0; kk < 16; ++kk)
+            xi = mfma(tile[tix(16 * w + lr, 4 * kk + lk)], vb[kk], xi);
+        } else if (I != J) {
+          const int wb = w - 4;
+          ta_double4 xj = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+          for (int kk = 0; kk < 16; ++kk)
+            xj = mfma(tile[tix(4 * kk + lk, 16 * wb + lr)], vb[kk], xj);
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int row = J0 + 16 * wb + 4 * i + lk;
+            if (row < m) X[lr * ld + r2 + row] += xj[i];
+          }
+        }
+        __syncthreads();
+      }
+    }
+    if (want_x && w < 4) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int row = I0 + 16 * w + 4 * i + lk;
+        if (row < m) X[lr * ld + r2 + row] = xi[i];
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// Sum of the eight waves' 16 x 16 partials (D layout) into out[16 * 16];
+// part: 2048 doubles of LDS.  Two barriers.
+__device__ __forceinline__ void reduce_partials(ta_double4 g, lds_dbl* part, lds_dbl* out)
+{
+  const int lane = threadIdx.x & 63, w = wave_id();
+  const int lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) part[w * 256 + (4 * i + lk) * 16 + lr] = g[i];
+  __syncthreads();
+  if (threadIdx.x < 256) {
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) s += part[q * 256 + threadIdx.x];
+    out[threadIdx.x] = s;
+  }
+  __syncthreads();
+}
+
+// ------------------------------------------------------ stage 1: panel
+// Panel at column c0 (m = n - c0 - BW >= 2 rows below the band): the strip
+// (rows c0.., columns c0..c0+15) is loaded into registers (MFMA D layout of
+// 16-row blocks, block b on wave b % 8) and brought up to date with the
+// previous panel's update (Vp, Wp; upd); its diagonal block is stored as band
+// data; a Householder QR (dlarfg per column) of its rows below the band gives
+// the reflectors (V1 at row c0, tau1) and R, which goes into A with zeros in
+// the bulge region below it; T of the block reflector I - V T V^T ends in
+// sm[0..256).  tile: 4096 doubles of LDS (reductions); sm: 560 doubles.
+template <int NT>
+__device__ __noinline__ void band_panel_qr(gbl_dbl* A, int n, int ld, int c0, gbl_dbl* V1,
+                                           gbl_dbl* tau1, const gbl_dbl* Vp, const gbl_dbl* Wp,
+                                           bool upd, lds_dbl* tile, lds_dbl* sm,
+                                           lds_dbl* tprof)
+{
+  const int tid = threadIdx.x, lane = tid & 63, w = wave_id();
+  const int lr = lane & 15, lk = lane >> 4;
+  const bool clk_on = tprof != nullptr && tid == 0;
+  long long clk_last = clk_on ? wall_clock64() : 0;
+  auto lap = [&](int slot) {
+    if (clk_on) {
+      const long long t = wall_clock64();
+      tprof[slot] += (double)(t - clk_last);
+      clk_last = t;
+    }
+  };
+  lds_dbl* Tm = sm;          // T of the panel, [k][c]
+  lds_dbl* Gs = sm + 256;    // reduced V^T V
+  lds_dbl* sc = sm + 512;    // pivot (0), tau of the panel's reflectors (1..16)
+  lds_dbl* red = tile;       // QR: partial sums of squares [8]
+  lds_dbl* red2 = tile + 8;  // QR: partial column dots [8][16]
+  const int ms = n - c0, m = ms - BW, r2 = c0 + BW;
+  gbl_dbl* Vc = V1 + (int64_t)c0 * ld;
+  // (a) strip into registers with the previous panel's update
+  ta_double4 x[QB];
+  double bop[8];
+#pragma unroll
+  for (int kk = 0; kk < 8; ++kk) {
+    const int k = lk + 4 * kk;
+    double b = 0.0;
+    if (upd) b = k < BW ? Wp[(int64_t)k * ld + c0 + lr] : Vp[(int64_t)(k - BW) * ld + c0 + lr];
+    bop[kk] = b;
+  }
+#pragma unroll
+  for (int q = 0; q < QB; ++q) {
+    const int rb = 16 * (w + 8 * q);
+    ta_double4 acc;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int rl = rb + 4 * i + lk;
+      double v = 0.0;
+      if (rl < ms) {
+        const int gr = c0 + rl, gc = c0 + lr;
+        v = rl >= lr ? A[(int64_t)gr * ld + gc] : A[(int64_t)gc * ld + gr];
+      }
+      acc[i] = v;
+    }
+    if (upd && rb < ms) {
+      const int gr = c0 + rb + lr;
+#pragma unroll
+      for (int kk = 0; kk < 8; ++kk) {
+        const int k = lk + 4 * kk;
+        double a = 0.0;
+        if (rb + lr < ms) a = k < BW ? Vp[(int64_t)k * ld + gr] : Wp[(int64_t)(k - BW) * ld + gr];
+        acc = mfma(-a, bop[kk], acc);
+      }
+    }
+    x[q] = acc;
+  }
+  if (w == 0) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int rl = 4 * i + lk;
+      if (rl >= lr) A[(int64_t)(c0 + rl) * ld + c0 + lr] = x[0][i];
+    }
+  }
+  lap(0);
+  // (b) QR.  The lane's rows are rb(q) + 4 i + lk with rb(q) = 16 (w + 8 q)
+  // wave-uniform; column j lives in the lanes lr == j of each 16-lane row.
+  const int kref = m < BW ? m : BW;
+  const int src_row = lane & 48;
+  for (int j = 0; j < kref; ++j) {
+    const int piv = BW + j;
+    double ss = 0.0;
+#pragma unroll
+    for (int q = 0; q < QB; ++q)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int rl = 16 * (w + 8 * q) + 4 * i + lk;
+        const double xv = x[q][i];
+        ss += (lr == j && rl > piv && rl < ms) ? xv * xv : 0.0;
+      }
+    ss = wave_sum_dpp(ss);
+    if (lane == 0) red[w] = ss;
+    if (w == 1 && lr == j && lk == (j & 3)) sc[0] = pick4(x[0], j >> 2);
+    __syncthreads();
+    double sst = 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) sst += red[q];
+    const double alpha = sc[0];
+    double beta, tau, scale;
+    if (sst == 0.0) {
+      beta = alpha;
+      tau = 0.0;
+      scale = 0.0;
+    } else {
+      beta = -copysign(sqrt(alpha * alpha + sst), alpha);
+      tau = (beta - alpha) / beta;
+      scale = 1.0 / (alpha - beta);
+    }
+    // column j <- (beta at the pivot, reflector below); partial dots v . x_c
+    // (all shuffles issued before the first use)
+    double xs[QB][4];
+#pragma unroll
+    for (int q = 0; q < QB; ++q)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) xs[q][i] = __shfl(x[q][i], src_row | j, 64);
+    double pd = 0.0;
+#pragma unroll
+    for (int q = 0; q < QB; ++q)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int rl = 16 * (w + 8 * q) + 4 * i + lk;
+        const double xj = xs[q][i];
+        const bool below = rl > piv && rl < ms;
+        const double v = rl == piv ? 1.0 : (below ? xj * scale : 0.0);
+        const double xo = x[q][i];
+        pd += v * xo;
+        x[q][i] = lr == j ? (rl == piv ? beta : (below ? v : xo)) : xo;
+      }
+    pd = xor32_sum(xor16_sum(pd));
+    if (lk == 0) red2[w * 16 + lr] = pd;
+    if (tid == 0) sc[1 + j] = tau;
+    __syncthreads();
+    double pc = 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) pc += red2[q * 16 + lr];
+    const double f = lr > j ? tau * pc : 0.0;
+    // the reflector again, now stored in column j
+#pragma unroll
+    for (int q = 0; q < QB; ++q)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) xs[q][i] = __shfl(x[q][i], src_row | j, 64);
+#pragma unroll
+    for (int q = 0; q < QB; ++q)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int rl = 16 * (w + 8 * q) + 4 * i + lk;
+        const double v = rl == piv ? 1.0 : ((rl > piv && rl < ms) ? xs[q][i] : 0.0);
+        x[q][i] -= f * v;
+      }
+  }
+  __syncthreads();
+  lap(1);
+  // (c) R and the zeroed bulge region into A, the reflectors into V1 (zero
+  // columns for identities); x keeps V for T
+  const double tcol = lr < kref ? sc[1 + lr] : 0.0;
+  if (tid < BW) tau1[c0 + tid] = tid < kref ? sc[1 + tid] : 0.0;
+#pragma unroll
+  for (int q = 0; q < QB; ++q)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int rl = 16 * (w + 8 * q) + 4 * i + lk;
+      double vv = 0.0;
+      if (rl >= BW && rl < ms) {
+        const int pr = rl - BW;
+        const double xv = x[q][i];
+        if (pr <= lr) A[(int64_t)(r2 + pr) * ld + c0 + lr] = xv;
+        else if (pr - lr < BW) A[(int64_t)(r2 + pr) * ld + c0 + lr] = 0.0;
+        if (tcol != 0.0 && pr >= lr) vv = pr == lr ? 1.0 : xv;
+        Vc[(int64_t)lr * ld + r2 + pr] = vv;
+      }
+      x[q][i] = vv;
+    }
+  // (d) T from T^-1 = diag(1/tau) + strict_upper(V^T V)
+  ta_double4 g = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int q = 0; q < QB; ++q)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) g = mfma(x[q][i], x[q][i], g);
+  reduce_partials(g, tile, Gs);
+  if (tid < BW) {
+    const int c = tid;
+    for (int i = BW - 1; i >= 0; --i) {
+      const double ti = sc[1 + i];
+      const double dii = (i < kref && ti != 0.0) ? 1.0 / ti : 1.0;
+      double t = 0.0;
+      if (i == c) {
+        t = 1.0 / dii;
+      } else if (i < c) {
+        double acc = 0.0;
+        for (int k = i + 1; k <= c; ++k) acc += Gs[i * BW + k] * Tm[k * BW + c];
+        t = -acc / dii;
+      }
+      Tm[i * BW + c] = t;
+    }
+  }
+  __syncthreads();
+  lap(2);
+}
+
+// W = X T - V (T^T V^T X T) / 2 = X T + V M, M = -(T^T Z T) / 2, Z = V^T X,
+// for the panel at c0 (rows c0 + BW .. n-1 of V, X and W).  T in sm[0..256);
+// tile: 4096 doubles of LDS; sm[256..512) scratch.
+template <int NT>
+__device__ __noinline__ void band_panel_w(int n, int ld, int c0, const gbl_dbl* Vc,
+                                          const gbl_dbl* X, gbl_dbl* Wc, lds_dbl* tile,
+                                          lds_dbl* sm)
+{
+  const int tid = threadIdx.x, lane = tid & 63, w = wave_id();
+  const int lr = lane & 15, lk = lane >> 4;
+  const int ms = n - c0;
+  lds_dbl* Tm = sm;
+  lds_dbl* Zs = sm + 256;
+  lds_dbl* Ps = tile + 2048;
+  lds_dbl* Ms = tile + 2304;
+  // Z = V^T X from D-layout blocks
+  ta_double4 z = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int q = 0; q < QB; ++q) {
+    const int rb = 16 * (w + 8 * q);
+    if (rb < BW || rb >= ms) continue;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int rl = rb + 4 * i + lk;
+      const int off = lr * ld + c0 + rl;
+      const double v = rl < ms ? Vc[off] : 0.0;
+      const double xv = rl < ms ? X[off] : 0.0;
+      z = mfma(v, xv, z);
+    }
+  }
+  reduce_partials(z, tile, Zs);
+  if (tid < 256) {
+    const int a = tid >> 4, c = tid & 15;
+    double t = 0.0;
+#pragma unroll
+    for (int k = 0; k < BW; ++k) t += Zs[a * BW + k] * Tm[k * BW + c];
+    Ps[tid] = t;
+  }
+  __syncthreads();
+  if (tid < 256) {
+    const int a = tid >> 4, c = tid & 15;
+    double t = 0.0;
+#pragma unroll
+    for (int k = 0; k < BW; ++k) t += Tm[k * BW + a] * Ps[k * BW + c];
+    Ms[tid] = -0.5 * t;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < QB; ++q) {
+    const int rb = 16 * (w + 8 * q);
+    if (rb < BW || rb >= ms) continue;
+    ta_double4 acc = {0.0, 0.0, 0.0, 0.0};
+    const bool in = rb + lr < ms;
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      const int k = 4 * kk + lk;
+      const int off = k * ld + c0 + rb + lr;
+      const double xa = in ? X[off] : 0.0;
+      const double va = in ? Vc[off] : 0.0;
+      acc = mfma(xa, Tm[k * BW + lr], acc);
+      acc = mfma(va, Ms[k * BW + lr], acc);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int rl = rb + 4 * i + lk;
+      if (rl < ms) Wc[lr * ld + c0 + rl] = acc[i];
+    }
+  }
+  __syncthreads();
+}
+
+// ------------------------------------------------------ stage 1: driver
+// A (n x n, lower triangle valid, n <= MAXN) -> band of half-width BW in its
+// lower triangle, zeros 17..31 below the diagonal.  V1, tau1: stage-1
+// reflectors (layout above); Wb: 2 x 16 x ld, X: 16 x ld doubles of global
+// scratch; tile: 4096 doubles of LDS; sm: 560 doubles of LDS; tprof: optional
+// step clocks (strip, QR, store + T, trailing pass, W).
+template <int NT>
+__device__ __noinline__ void band_reduce(gbl_dbl* A, int n, int ld, gbl_dbl* V1, gbl_dbl* tau1,
+                                         gbl_dbl* Wb, gbl_dbl* X, lds_dbl* tile, lds_dbl* sm,
+                                         lds_dbl* tprof)
+{
+  static_assert(NT == 512, "strip layout assumes 8 waves");
+  const bool clk_on = tprof != nullptr && threadIdx.x == 0;
+  long long clk_last = 0;
+  auto lap = [&](int slot) {
+    if (clk_on) {
+      const long long t = wall_clock64();
+      tprof[slot] += (double)(t - clk_last);
+      clk_last = t;
+    }
+  };
+  int p = 0;
+  for (int c0 = 0; n - c0 - BW >= 2; c0 += BW, ++p) {
+    const int r2 = c0 + BW;
+    const bool upd = c0 > 0;
+    const gbl_dbl* Vp = V1 + (int64_t)(upd ? c0 - BW : 0) * ld;
+    const gbl_dbl* Wp = Wb + (int64_t)((p + 1) & 1) * BW * ld;
+    gbl_dbl* Wc = Wb + (int64_t)(p & 1) * BW * ld;
+    const gbl_dbl* Vc = V1 + (int64_t)c0 * ld;
+    band_panel_qr<NT>(A, n, ld, c0, V1, tau1, Vp, Wp, upd, tile, sm, tprof);
+    if (clk_on) clk_last = wall_clock64();
+    band_pass<NT>(A, n, ld, r2, Vp, Wp, upd, Vc, X, true, tile);
+    lap(3);
+    band_panel_w<NT>(n, ld, c0, Vc, X, Wc, tile, sm);
+    lap(4);
+  }
+  // the last panel's update of the remaining trailing block
+  if (p > 0) {
+    const int c0 = (p - 1) * BW, r2 = c0 + BW;
+    if (clk_on) clk_last = wall_clock64();
+    band_pass<NT>(A, n, ld, r2, V1 + (int64_t)c0 * ld, Wb + (int64_t)((p - 1) & 1) * BW * ld,
+                  true, nullptr, nullptr, false, tile);
+    lap(3);
+  }
+}
+
+// ------------------------------------------------------ stage 2
+// One task (sweep s, task t) on the band in A, by one wave.  Lane (i, g):
+// row i = lane % 16 of each block, columns 4 g .. 4 g + 3.
+__device__ __forceinline__ void chase_task(gbl_dbl* A, int n, int ld, int s, int t,
+                                           gbl_dbl* V2)
+{
+  const int lane = threadIdx.x & 63;
+  const int i = lane & 15, g = lane >> 4;
+  const int lo = s + 1 + BW * t;
+  const int hi = min(s + BW * (t + 1), n - 1) + 1;
+  const int L = hi - lo;
+  const int c = t == 0 ? s : lo - BW;  // column annihilated by this task
+  const int Lc = t == 0 ? 1 : BW;
+  const int bh = min(hi + BW, n);
+  const int Lr = bh - hi;
+  double lb[4], dd[4], rr[4];
+#pragma unroll
+  for (int jj = 0; jj < 4; ++jj) {
+    const int j = 4 * g + jj;
+    lb[jj] = (i < L && j < Lc) ? A[(int64_t)(lo + i) * ld + c + j] : 0.0;
+    const int a = i > j ? i : j, b = i > j ? j : i;
+    dd[jj] = (i < L && j < L) ? A[(int64_t)(lo + a) * ld + lo + b] : 0.0;
+    rr[jj] = (i < Lr && j < L) ? A[(int64_t)(hi + i) * ld + lo + j] : 0.0;
+  }
+  // reflector from column 0 of the left block (held by lanes 0..15)
+  const double x = __shfl(lb[0], i, 64);
+  const double ss = row16_sum(i >= 1 ? x * x : 0.0);
+  const double alpha = bcast_lane0(x);
+  double beta, tau, scale;
+  if (ss == 0.0) {
+    beta = alpha;
+    tau = 0.0;
+    scale = 0.0;
+  } else {
+    beta = -copysign(sqrt(alpha * alpha + ss), alpha);
+    tau = (beta - alpha) / beta;
+    scale = 1.0 / (alpha - beta);
+  }
+  const double vi = i == 0 ? 1.0 : (i < L ? x * scale : 0.0);
+  double vj[4];
+#pragma unroll
+  for (int jj = 0; jj < 4; ++jj) vj[jj] = __shfl(vi, 4 * g + jj, 64);
+  // left block: H applied from the left; its first column becomes beta e_1
+  if (t > 0) {
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj) {
+      const double pcol = row16_sum(vi * lb[jj]);
+      lb[jj] -= tau * vi * pcol;
+    }
+  }
+  if (g == 0) lb[0] = i == 0 ? beta : 0.0;
+  // diagonal block: H D H = D - v w^T - w v^T, w = y - (tau/2)(y.v) v, y = tau D v
+  double part = 0.0;
+#pragma unroll
+  for (int jj = 0; jj < 4; ++jj) part += dd[jj] * vj[jj];
+  part = xor32_sum(xor16_sum(part));
+  const double y = tau * part;
+  const double yv = row16_sum(y * vi);
+  const double wv = y - 0.5 * tau * yv * vi;
+#pragma unroll
+  for (int jj = 0; jj < 4; ++jj) {
+    const double wj = __shfl(wv, 4 * g + jj, 64);
+    dd[jj] -= vi * wj + wv * vj[jj];
+  }
+  // block below: H applied from the right
+  double qr = 0.0;
+#pragma unroll
+  for (int jj = 0; jj < 4; ++jj) qr += rr[jj] * vj[jj];
+  qr = xor32_sum(xor16_sum(qr));
+#pragma unroll
+  for (int jj = 0; jj < 4; ++jj) rr[jj] -= tau * qr * vj[jj];
+#pragma unroll
+  for (int jj = 0; jj < 4; ++jj) {
+    const int j = 4 * g + jj;
+    if (i < L && j < Lc) A[(int64_t)(lo + i) * ld + c + j] = lb[jj];
+    if (i < L && j <= i) A[(int64_t)(lo + i) * ld + lo + j] = dd[jj];
+    if (i < Lr && j < L) A[(int64_t)(hi + i) * ld + lo + j] = rr[jj];
+  }
+  if (g == 0) V2[(chase_first(n, s) + t) * BW + i] = i == 0 ? tau : vi;
+}
+
+// Band (A's lower triangle, half-width BW, zeros below) -> tridiagonal:
+// d[k] = T[k][k], e[k] = T[k+1][k] in LDS.  Requires n >= 3.
+template <int NT>
+__device__ __noinline__ void band_chase(gbl_dbl* A, int n, int ld, gbl_dbl* V2, lds_dbl* d,
+                                        lds_dbl* e)
+{
+  constexpr int NW = NT / 64;
+  const int w = wave_id();
+  const int t0max = chase_tasks(n, 0);
+  const int nsteps = LAG * (n - 3) + 1;
+  for (int step = 0; step < nsteps; ++step) {
+    const int shi = min(step / LAG, n - 3);
+    for (int s = shi - w; s >= 0; s -= NW) {
+      const int t = step - LAG * s;
+      if (t >= t0max) break;
+      if (t < chase_tasks(n, s)) chase_task(A, n, ld, s, t, V2);
+    }
+    __syncthreads();
+  }
+  for (int k = threadIdx.x; k < n; k += NT) {
+    d[k] = A[(int64_t)k * ld + k];
+    if (k < n - 1) e[k] = A[(int64_t)(k + 1) * ld + k];
+  }
+  __syncthreads();
+}
+
+// ------------------------------------------------------ back-transforms
+// Y <- Q2 Y on the first ncol columns of Y (Y[k * ld + col]), the stage-2
+// reflectors in reverse order.  Chunks of CW = 16 columns (8 when LDS is
+// short) are held in LDS; wave w owns CW / 8 chunk columns, and its lanes
+// take four tasks of one sweep at once (tasks of a sweep touch disjoint
+// rows), so the chunk needs no barrier.  The reflector records stream
+// through an LDS staging area in batches of whole sweeps, the next batch
+// loaded into registers while the current one is applied.  region: LDS of
+// nreg doubles (chunk + staging: at least 9 n + 16 chase_tasks(n, 0)).
+constexpr int Q2_PF = 4;  // staged doubles per thread and batch
+
+template <int NT>
+__device__ __noinline__ void apply_q2(const gbl_dbl* V2, int n, int ld, int ncol, gbl_dbl* Y,
+                                      lds_dbl* region, int nreg)
+{
+  const int tid = threadIdx.x, lane = tid & 63, w = wave_id();
+  const int i = lane & 15, ts = lane >> 4;
+  // chunk rows padded by one double: the 16 rows a reflector touches then
+  // fall into different LDS banks
+  const int CW = (nreg - 17 * n) / BW >= chase_tasks(n, 0) ? 16 : 8;
+  const int CS = CW + 1;
+  lds_dbl* chunk = region;
+  lds_dbl* stage = region + CS * n;
+  int cap = (nreg - CS * n) / BW;  // records per batch
+  if (cap > Q2_PF * NT / BW) cap = Q2_PF * NT / BW;
+  const int ca = (CW / 8) * w, cb = ca + 1;
+  const bool two = CW == 16;
+  for (int c0 = 0; c0 < ncol; c0 += CW) {
+    const int cw = min(CW, ncol - c0);
+    for (int idx = tid; idx < n * CW; idx += NT) {
+      const int r = idx / CW, c = idx - r * CW;
+      chunk[r * CS + c] = c < cw ? Y[(int64_t)r * ld + c0 + c] : 0.0;
+    }
+    // batches of whole sweeps, from the last sweep down: sweeps [sb, se]
+    auto batch_start = [&](int se) {
+      int sb = se;
+      const int end = chase_first(n, se + 1);
+      while (sb > 0 && end - chase_first(n, sb - 1) <= cap) --sb;
+      return sb;
+    };
+    int se = n - 3;
+    int sb = batch_start(se);
+    double pf[Q2_PF];
+    {
+      const int r0 = chase_first(n, sb) * BW, r1 = chase_first(n, se + 1) * BW;
+#pragma unroll
+      for (int q = 0; q < Q2_PF; ++q) {
+        const int k = r0 + tid + q * NT;
+        pf[q] = k < r1 ? V2[k] : 0.0;
+      }
+    }
+    while (se >= 0) {
+      __syncthreads();  // previous batch applied (and the chunk loaded)
+      const int base = chase_first(n, sb);
+#pragma unroll
+      for (int q = 0; q < Q2_PF; ++q) {
+        const int k = tid + q * NT;
+        if (k < cap * BW) stage[k] = pf[q];
+      }
+      __syncthreads();
+      // next batch into registers while this one is applied
+      const int se2 = sb - 1;
+      const int sb2 = se2 >= 0 ? batch_start(se2) : 0;
+      if (se2 >= 0) {
+        const int r0 = chase_first(n, sb2) * BW, r1 = chase_first(n, se2 + 1) * BW;
+#pragma unroll
+        for (int q = 0; q < Q2_PF; ++q) {
+          const int k = r0 + tid + q * NT;
+          pf[q] = k < r1 ? V2[k] : 0.0;
+        }
+      }
+      if (ca < cw) {
+        for (int s = se; s >= sb; --s) {
+          const int ns = chase_tasks(n, s);
+          const int rs = chase_first(n, s) - base;
+          for (int tb = 0; tb < ns; tb += 4) {
+            const int t = tb + ts;
+            const int lo = s + 1 + BW * t;
+            const int L = t < ns ? min(s + BW * (t + 1), n - 1) + 1 - lo : 0;
+            const bool act = i < L;
+            const lds_dbl* rec = stage + (rs + (t < ns ? t : 0)) * BW;
+            const double tau = t < ns ? rec[0] : 0.0;
+            const double vi = act ? (i == 0 ? 1.0 : rec[i]) : 0.0;
+            const int row = act ? lo + i : 0;
+            const double ya = act ? chunk[row * CS + ca] : 0.0;
+            const double yb = (act && two) ? chunk[row * CS + cb] : 0.0;
+            const double pa = row16_sum(vi * ya);
+            const double pb = row16_sum(vi * yb);
+            if (act) {
+              chunk[row * CS + ca] = ya - tau * vi * pa;
+              if (two) chunk[row * CS + cb] = yb - tau * vi * pb;
+            }
+          }
+        }
+      }
+      se = se2;
+      sb = sb2;
+    }
+    __syncthreads();
+    for (int idx = tid; idx < n * CW; idx += NT) {
+      const int r = idx / CW, c = idx - r * CW;
+      if (c < cw) Y[(int64_t)r * ld + c0 + c] = chunk[r * CS + c];
+    }
+    __syncthreads();
+  }
+}
+
+// Y <- Q1 Y on the first ncol columns of Y, Q1 = H_0 H_1 ... (stage-1
+// reflectors, reflector k on rows k + BW..), in compact WY blocks of NB
+// reflectors, last block first, as big::back_transform: T^-1 =
+// diag(1/tau) + strict_upper(V^T V), W = V^T Y, W' = T W, Y -= V W'.
+// W: NB x ld doubles of global scratch; sinv, tm: NB x NB doubles of LDS;
+// tile: 2048 doubles of LDS.
+template <int NT, int NB>
+__device__ __noinline__ void back_transform_q1(const gbl_dbl* V1, const gbl_dbl* tau1, int n,
+                                               int ld, int ncol, gbl_dbl* Y, gbl_dbl* W,
+                                               lds_dbl* sinv, lds_dbl* tm, lds_dbl* tile)
+{
+  const int nr = band_reflectors(n);
+  const int nblk = (nr + NB - 1) / NB;
+  for (int bi = nblk - 1; bi >= 0; --bi) {
+    const int k0 = bi * NB;
+    const int nb = (k0 + NB <= nr) ? NB : nr - k0;
+    const int r0 = k0 + BW;
+    const int m = n - r0;
+    if (m <= 0) continue;
+    auto vel = [=](int rr, int j) -> double {
+      return (j < nb && rr >= j && tau1[k0 + j] != 0.0) ? V1[(int64_t)(k0 + j) * ld + r0 + rr]
+                                                         : 0.0;
+    };
+    auto all_k = [m](int, int, int* ka, int* kb) {
+      *ka = 0;
+      *kb = m;
+    };
+    auto no_skip = [](int, int) { return false; };
+    big::gemm<NT, false, true>(
+        NB, NB, [=](int i, int k) { return vel(k, i); }, [=](int k, int j) { return vel(k, j); },
+        all_k, no_skip,
+        [=](int i, int j, double val) {
+          double s = 0.0;
+          if (i < j) s = val;
+          if (i == j) s = (i < nb && tau1[k0 + i] != 0.0) ? 1.0 / tau1[k0 + i] : 1.0;
+          sinv[i * NB + j] = s;
+        },
+        tile);
+    big::gemm<NT, false, true>(
+        NB, ncol, [=](int i, int k) { return vel(k, i); },
+        [=](int k, int j) { return Y[(int64_t)(r0 + k) * ld + j]; }, all_k, no_skip,
+        [=](int i, int j, double val) { W[(int64_t)i * ld + j] = val; }, tile);
+    if (threadIdx.x < NB) {
+      const int j = threadIdx.x;
+      for (int i = NB - 1; i >= 0; --i) {
+        double t = 0.0;
+        if (i == j) {
+          t = 1.0 / sinv[i * NB + i];
+        } else if (i < j) {
+          double acc = 0.0;
+          for (int k = i + 1; k <= j; ++k) acc += sinv[i * NB + k] * tm[k * NB + j];
+          t = -acc / sinv[i * NB + i];
+        }
+        tm[i * NB + j] = t;
+      }
+    }
+    __syncthreads();
+    big::gemm<NT, true, true>(
+        NB, ncol, [=](int i, int k) { return tm[i * NB + k]; },
+        [=](int k, int j) { return W[(int64_t)k * ld + j]; },
+        [](int, int, int* ka, int* kb) {
+          *ka = 0;
+          *kb = NB;
+        },
+        no_skip, [=](int i, int j, double val) { W[(int64_t)i * ld + j] = val; }, tile);
+    big::gemm<NT, true, true>(
+        m, ncol, [=](int i, int k) { return vel(i, k); },
+        [=](int k, int j) { return W[(int64_t)k * ld + j]; },
+        [](int, int, int* ka, int* kb) {
+          *ka = 0;
+          *kb = NB;
+        },
+        no_skip, [=](int i, int j, double val) { Y[(int64_t)(r0 + i) * ld + j] -= val; }, tile);
+  }
+}
+
+}  // namespace band

@@ -433,6 +433,10 @@ struct tblite_amd_context_s {
   // at ld 65-96 against 368 + 156 + 134 + 12 us for tridiagonalisation,
   // bisection, eigenvectors and basis product (47.8 vs 60.8 ms per step)
   int lds_eig = 0;
+  // reduction to tridiagonal form of the global-memory SCF kernel (ld <= 512):
+  // 1 two-stage dense -> band -> tridiagonal, 0 one-stage Householder
+  // (TBLITE_AMD_BIG_EIG=band | householder)
+  int big_eig = 0;
   // minimum LDS request of the global-memory SCF kernel (bytes); above half a
   // CU's LDS it runs one molecule per CU (TBLITE_AMD_BIG_LDS_MIN_KB)
   size_t big_lds_min = 0;
@@ -1042,6 +1046,7 @@ int upload_plan(tblite_amd_context ctx, Calculator& c, const Plan& pl, DeviceBat
   b.jacobi_skip = ctx->jacobi_skip;
   b.jacobi_adapt = ctx->jacobi_adapt;
   b.lds_eig = ctx->lds_eig;
+  b.big_eig = ctx->big_eig;
   b.prof = ctx->profile ? carve<long long>(ar, (int64_t)nmol * TA_PROF_SLOTS) : nullptr;
   db.grad = grad;
   db.d_wtiles = nullptr;
@@ -1565,6 +1570,8 @@ static tblite_amd_context make_context(int device, bool single_stream)
   if (pj) ctx->jacobi_tol = std::max(1e-12, std::min(1e-2, std::atof(pj)));
   const char* pe2 = std::getenv("TBLITE_AMD_LDS_EIG");
   if (pe2) ctx->lds_eig = std::strcmp(pe2, "householder") == 0 ? 1 : 0;
+  const char* pbe = std::getenv("TBLITE_AMD_BIG_EIG");
+  if (pbe) ctx->big_eig = std::strcmp(pbe, "band") == 0 ? 1 : 0;
   const char* pl = std::getenv("TBLITE_AMD_LDS_MAX_LD");
   if (pl) ctx->lds_max_ld = std::max(16, std::min(kLdsMaxLd, std::atoi(pl) / 16 * 16));
   if (!ok) {

@@ -32,6 +32,7 @@
 #include "common.h"
 #include "d4.h"
 #include "device_util.h"
+#include "eig_band.h"
 #include "eig_big.h"
 #include "eig_lds.h"
 #include "fermi.h"
@@ -641,6 +642,30 @@ __device__ bool jacobi(double* A, double* CT, int ld, const JRot& R, const JPlan
 }
 
 
+// ------------------------------------------- two-stage reduction
+// A (lower triangle) -> tridiagonal d, e (LDS) through a band of half-width
+// 16; the stage-1 and stage-2 reflectors stay in the molecule's scratch for
+// big_eigenvectors.  tile: 4096 doubles of LDS, sm: 560 doubles of LDS.
+template <int NT>
+__device__ __noinline__ void big_band_tridiagonalize(gbl_dbl* A, gbl_dbl* scr, int n, int ld,
+                                                     lds_dbl* d, lds_dbl* e, lds_dbl* tile,
+                                                     lds_dbl* sm, lds_dbl* red, PhaseClock& clk)
+{
+  // stage-1 step clocks in red[16..20] -> the tri_* slots
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 5; ++q) red[16 + q] = 0.0;
+  __syncthreads();
+  band::band_reduce<NT>(A, n, ld, scr + band_v1_offset(ld), scr + band_tau_offset(ld),
+                        scr + band_w_offset(ld), scr + band_x_offset(ld), tile, sm,
+                        clk.on ? red + 16 : nullptr);
+  if (clk.on) {
+    for (int q = 0; q < 5; ++q) clk.acc[TA_PROF_TRI_COLUMN + q] += (long long)red[16 + q];
+  }
+  clk.lap(TA_PROF_BAND1);
+  band::band_chase<NT>(A, n, ld, scr + band_v2_offset(ld), d, e);
+  clk.lap(TA_PROF_BAND2);
+}
+
 // ------------------------------------------- global-memory eigensolve
 // F (in A, full) -> eigenvalues ev (ascending) of the generalized problem
 // F C = S C e, with X = L^-1 (lower triangular): A = X F X^T, reduced to
@@ -655,7 +680,8 @@ template <int NT>
 __device__ __noinline__ void big_eigenvalues(double* A_, const double* X_, double* Y_,
                                              double* scr_, int n, int ld, int npart, double* er_,
                                              double* ev_, double* red_, double* scal_,
-                                             const double* prev_, double* warm_, PhaseClock& clk)
+                                             const double* prev_, double* warm_, bool two_stage,
+                                             PhaseClock& clk)
 {
   gbl_dbl* A = TA_GBL(A_);
   const gbl_dbl* X = TA_GBL(X_);
@@ -691,17 +717,23 @@ __device__ __noinline__ void big_eigenvalues(double* A_, const double* X_, doubl
       [=](int k, int j) { return Y[(int64_t)k * ld + j]; }, k_lower, upper_tile,
       [=](int i, int j, double val) { A[(int64_t)i * ld + j] = val; }, tmp);
   clk.lap(TA_PROF_TRANSFORM);
-  // per-wave column partials of the tridiagonalisation share the GEMM tile region
-  gbl_dbl* wbt = scr + 3 * (int64_t)ld * TA_BIG_NT;
-  gbl_dbl* pvt = wbt + (int64_t)TA_BIG_NB * ld;
-  if (threadIdx.x == 0)
-    for (int q = 0; q < 5; ++q) red[16 + q] = 0.0;
-  big::tridiagonalize_blocked<NT, TA_BIG_NP>(A, n, ld, d, e, tau, vv, vn, yy, yrow, ww, tmp, pvt,
-                                             pvt + (int64_t)TA_BIG_NP * ld, npart, red,
-                                             scal, clk.on ? red + 16 : nullptr);
-  if (clk.on) {
-    for (int q = 0; q < 5; ++q) clk.acc[TA_PROF_TRI_COLUMN + q] += (long long)red[16 + q];
-    clk.last = wall_clock64();
+  if (two_stage) {
+    // dense -> band (panel QR, one pass over the trailing triangle per panel)
+    // -> tridiagonal (bulge chasing); reflectors kept in the scratch
+    big_band_tridiagonalize<NT>(A, scr, n, ld, d, e, tmp, er + 3 * ld, red, clk);
+  } else {
+    // per-wave column partials of the tridiagonalisation share the GEMM tile region
+    gbl_dbl* wbt = scr + 3 * (int64_t)ld * TA_BIG_NT;
+    gbl_dbl* pvt = wbt + (int64_t)TA_BIG_NB * ld;
+    if (threadIdx.x == 0)
+      for (int q = 0; q < 5; ++q) red[16 + q] = 0.0;
+    big::tridiagonalize_blocked<NT, TA_BIG_NP>(A, n, ld, d, e, tau, vv, vn, yy, yrow, ww, tmp, pvt,
+                                               pvt + (int64_t)TA_BIG_NP * ld, npart, red,
+                                               scal, clk.on ? red + 16 : nullptr);
+    if (clk.on) {
+      for (int q = 0; q < 5; ++q) clk.acc[TA_PROF_TRI_COLUMN + q] += (long long)red[16 + q];
+      clk.last = wall_clock64();
+    }
   }
   clk.lap(TA_PROF_TRIDIAG);
   big::bisect_all<NT>(d, e, e2, n, ev, TA_LDS(prev_), TA_LDS(warm_), red, scal);
@@ -717,7 +749,8 @@ template <int NT>
 __device__ __noinline__ void big_eigenvectors(const double* A_, const double* X_, double* Y_,
                                               double* CT_, double* scr_, int n, int ld, int nvec,
                                               double* er_, const double* ev_, double* scal_,
-                                              int* runs_, PhaseClock& clk)
+                                              int* runs_, bool two_stage, int eig_region,
+                                              PhaseClock& clk)
 {
   const gbl_dbl* A = TA_GBL(A_);
   const gbl_dbl* X = TA_GBL(X_);
@@ -739,8 +772,18 @@ __device__ __noinline__ void big_eigenvectors(const double* A_, const double* X_
   clk.lap(TA_PROF_INVIT);
   big::orthogonalize_clusters<NT>(ev, n, nvec, tnorm, Y, ld, runs, scal);
   clk.lap(TA_PROF_CLUSTERS);
-  big::back_transform<NT, TA_BIG_NB>(A, tau, n, ld, nvec, Y, wbt, tmp + 2048,
-                                     tmp + 2048 + TA_BIG_NB * TA_BIG_NB, tmp);
+  if (two_stage) {
+    // Z = Q1 Q2 Y: bulge-chasing reflectors on column chunks in LDS (the
+    // eigensolver region is free here), then the panel reflectors
+    band::apply_q2<NT>(scr + band_v2_offset(ld), n, ld, nvec, Y, er, eig_region);
+    clk.lap(TA_PROF_Q2);
+    band::back_transform_q1<NT, TA_BIG_NB>(scr + band_v1_offset(ld), scr + band_tau_offset(ld), n,
+                                           ld, nvec, Y, wbt, tmp + 2048,
+                                           tmp + 2048 + TA_BIG_NB * TA_BIG_NB, tmp);
+  } else {
+    big::back_transform<NT, TA_BIG_NB>(A, tau, n, ld, nvec, Y, wbt, tmp + 2048,
+                                       tmp + 2048 + TA_BIG_NB * TA_BIG_NB, tmp);
+  }
   clk.lap(TA_PROF_BACKTR);
   // CT = Z^T X; X[k][a] = 0 for a > k
   big::gemm<NT, false, true>(
@@ -909,6 +952,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
     CT = b.big_ct + b.mol_big_off[m];
     scr = b.big_s + b.mol_scr_off[m];
   }
+  // two-stage (band) reduction in the global-memory eigensolver
+  const bool two_stage = !MATS_LDS && b.big_eig != 0 && ld <= TA_BAND_MAXLD;
   // element (i, j) of the working matrix A / the density
   auto aix = [ld](int i, int j) { return MATS_LDS ? sk(i, j, ld) : i * ld + j; };
   double* Pm = MATS_LDS ? A : Y;
@@ -1408,7 +1453,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
       // levels of the previous step (emo_s) warm-start the bisection; the
       // eigenvectors follow once the occupations are known
       big_eigenvalues<NT>(A, C, Y, scr, n, ld, big_npart(ldmax), smem + L.eig, ev, red, scal,
-                          emo_s, red + 24, clk);
+                          emo_s, red + 24, two_stage, clk);
       if (b.coeff != nullptr) {
         // keep d, e, tau and |T| for the eigenvectors of a requested wavefunction
         double* sv = scr + big_tri_save_offset(ld);
@@ -1459,7 +1504,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
     if constexpr (!MATS_LDS) {
       int nvec = 0;
       if (last_nocc > 0) nvec = occl[last_nocc - 1] + 1;
-      big_eigenvectors<NT>(A, C, Y, CT, scr, n, ld, nvec, smem + L.eig, ev, scal, rank, clk);
+      big_eigenvectors<NT>(A, C, Y, CT, scr, n, ld, nvec, smem + L.eig, ev, scal, rank, two_stage,
+                           L.iint / 2 - L.eig, clk);
     }
 
     // ------------------------------------------------ density -> A
@@ -1645,7 +1691,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
         for (int k = tid; k < 3 * ld; k += NT) er[k] = sv[k];
         if (tid == 0) scal[0] = sv[3 * ld];
         __syncthreads();
-        big_eigenvectors<NT>(A, C, Y, CT, scr, n, ld, n, smem + L.eig, ev, scal, rank, clk);
+        big_eigenvectors<NT>(A, C, Y, CT, scr, n, ld, n, smem + L.eig, ev, scal, rank, two_stage,
+                             L.iint / 2 - L.eig, clk);
       }
       __syncthreads();
     }

@@ -154,7 +154,8 @@ _PROTOS = {
 PROFILE_SLOTS = ["init", "mix", "potential", "fock", "transform", "jacobi", "occupation",
                  "density", "mulliken", "energy", "sweeps", "iterations",
                  "tridiagonal", "bisection", "inverse_iteration", "back_transform",
-                 "tri_column", "tri_reflector", "tri_symv", "tri_correct", "tri_panel", "clusters"]
+                 "tri_column", "tri_reflector", "tri_symv", "tri_correct", "tri_panel", "clusters",
+                 "band_stage1", "band_stage2", "band_q2"]
 
 EXPORTED = sorted(_PROTOS)
 

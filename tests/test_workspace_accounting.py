@@ -46,7 +46,9 @@ def test_grid_molecule_brings_pair_planes():
     """One multi-workgroup molecule makes the engine allocate the AES2 pair
     planes of every molecule in the batch; the admission counts them."""
     rng = np.random.default_rng(92)
-    small = [workloads.water_cluster(4, workloads.liquid_radius(4), rng) for _ in range(20)]
+    # the planes (18 nat^2 doubles per molecule, 12 MB here) outweigh the
+    # difference between the large molecule's own path-2 and path-3 scratch
+    small = [workloads.water_cluster(8, workloads.liquid_radius(8), rng) for _ in range(150)]
     big = workloads.water_cluster(30, workloads.liquid_radius(30), rng)  # 180 AOs
     need_lo, _ = _estimate(small + [big], 1024)
     need_hi, admit_hi = _estimate(small + [big], 128)

@@ -24,6 +24,34 @@ __device__ __forceinline__ int wave_id()
   return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
 }
 
+// A wave-uniform pointer or integer moved into scalar registers.  Arguments
+// of non-inlined device functions arrive in vector registers, so the compiler
+// cannot see that they are uniform; re-reading them through readfirstlane
+// lets memory accesses use a scalar base plus a 32-bit vector offset instead
+// of a 64-bit address per lane.
+template <class T>
+__device__ __forceinline__ T* sgpr_ptr(T* p)
+{
+  const unsigned long long b = (unsigned long long)p;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+  return (T*)(((unsigned long long)hi << 32) | lo);
+}
+
+__device__ __forceinline__ int sgpr_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Element idx of a global array addressed as base + 32-bit byte offset (the
+// scalar-base addressing mode when base is uniform); idx < 2^29.
+typedef __attribute__((address_space(1))) char gbl_char;
+__device__ __forceinline__ double gld(const gbl_dbl* base, unsigned idx)
+{
+  return *(const gbl_dbl*)((const gbl_char*)base + (idx << 3));
+}
+__device__ __forceinline__ void gst(gbl_dbl* base, unsigned idx, double v)
+{
+  *(gbl_dbl*)((gbl_char*)base + (idx << 3)) = v;
+}
+
 __device__ __forceinline__ double wave_sum(double v)
 {
 #pragma unroll

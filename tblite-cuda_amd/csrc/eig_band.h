@@ -88,9 +88,12 @@ __device__ __forceinline__ double pick4(ta_double4 v, int i)
   return i == 0 ? v[0] : (i == 1 ? v[1] : (i == 2 ? v[2] : v[3]));
 }
 
-// 64 x 64 LDS tile, columns XOR-swizzled by the row so that both a row
-// fragment and a column fragment of 16 lanes spread over the banks
-__device__ __forceinline__ int tix(int r, int c) { return r * 64 + (c ^ ((r & 15) << 2)); }
+// 64 x 64 LDS tile with rows padded to 65 doubles: row and column fragments
+// of 16 lanes spread over the banks, and every access of a lane is one base
+// address plus a compile-time offset
+constexpr int TS = 65;
+constexpr int TILE_DOUBLES = 64 * TS;
+__device__ __forceinline__ int tix(int r, int c) { return r * TS + c; }
 
 // ------------------------------------------------------ stage 1: one pass
 // Trailing matrix A2 = A[r2:, r2:] (m = n - r2), lower 64 x 64 tiles in row
@@ -99,21 +102,29 @@ __device__ __forceinline__ int tix(int r, int c) { return r * 64 + (c ^ ((r & 15
 // tiles only: tile (I, J) adds A_IJ V_J to X_I and, below the diagonal,
 // A_IJ^T V_I to X_J.  Waves 0-3 keep X_I in registers over a tile row and
 // store it at its end; waves 4-7 add the transposed contributions to rows
-// stored earlier.  tile: 4096 doubles of LDS.
+// stored earlier.  tile: TILE_DOUBLES of LDS.
 template <int NT>
 __device__ __noinline__ void band_pass(gbl_dbl* A, int n, int ld, int r2, const gbl_dbl* Vp,
                                        const gbl_dbl* Wp, bool upd, const gbl_dbl* Vc,
                                        gbl_dbl* X, bool want_x, lds_dbl* tile)
 {
   static_assert(NT == 512, "tiling assumes 8 waves");
+  A = sgpr_ptr(A);
+  Vp = sgpr_ptr(Vp);
+  Wp = sgpr_ptr(Wp);
+  Vc = sgpr_ptr(Vc);
+  X = sgpr_ptr(X);
+  n = sgpr_int(n);
+  ld = sgpr_int(ld);
+  r2 = sgpr_int(r2);
   const int lane = threadIdx.x & 63, w = wave_id();
   const int lr = lane & 15, lk = lane >> 4;
   const int bi = w >> 1, bj = (w & 1) * 2;
   const int m = n - r2, nt = (m + 63) >> 6;
   // loads use clamped (in-range) indices and zero the value afterwards, so
   // they are branch-free; row-major offsets fit in 32 bits (ld <= 1024)
-  gbl_dbl* A2 = A + (int64_t)r2 * ld + r2;
-  const gbl_dbl* Vr = want_x ? Vc + r2 + lr * ld : nullptr;  // V[row][lr] = Vr[row]
+  gbl_dbl* A2 = sgpr_ptr(A + (int64_t)r2 * ld + r2);
+  const gbl_dbl* Vr = want_x ? Vc + r2 : nullptr;  // V[row][lr] = Vr[lr * ld + row]
   for (int I = 0; I < nt; ++I) {
     const int I0 = I * 64;
     double au[8], vb[16];
@@ -123,7 +134,7 @@ __device__ __noinline__ void band_pass(gbl_dbl* A, int n, int ld, int r2, const 
       for (int kk = 0; kk < 8; ++kk) {
         const int k = lk + 4 * kk;
         double a = 0.0;
-        if (upd) a = k < BW ? Vp[k * ld + r2 + rc] : Wp[(k - BW) * ld + r2 + rc];
+        if (upd) a = k < BW ? gld(Vp, k * ld + r2 + rc) : gld(Wp, (k - BW) * ld + r2 + rc);
         au[kk] = row < m ? -a : 0.0;
       }
     }
@@ -131,7 +142,7 @@ __device__ __noinline__ void band_pass(gbl_dbl* A, int n, int ld, int r2, const 
 #pragma unroll
       for (int kk = 0; kk < 16; ++kk) {
         const int row = I0 + 4 * kk + lk;
-        const double v = Vr[min(row, m - 1)];
+        const double v = gld(Vr, lr * ld + min(row, m - 1));
         vb[kk] = row < m ? v : 0.0;
       }
     }
@@ -149,18 +160,9 @@ __device__ __noinline__ void band_pass(gbl_dbl* A, int n, int ld, int r2, const 
         for (int i = 0; i < 4; ++i) {
           const int row = I0 + 16 * bi + 4 * i + lk, col = J0 + 16 * (bj + h) + lr;
           const int rc = min(row, m - 1), cc = min(min(col, row), m - 1);
-          const double v = A2[rc * ld + cc];
+          const double v = gld(A2, rc * ld + cc);
           acc[h][i] = (row < m && col <= row) ? v : 0.0;
         }
-      // waves 0-3: V rows of the tile column, issued with the tile loads
-      if (want_x && w < 4) {
-#pragma unroll
-        for (int kk = 0; kk < 16; ++kk) {
-          const int col = J0 + 4 * kk + lk;
-          const double v = Vr[min(col, m - 1)];
-          vb[kk] = col < m ? v : 0.0;
-        }
-      }
       if (upd) {
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
@@ -169,7 +171,7 @@ __device__ __noinline__ void band_pass(gbl_dbl* A, int n, int ld, int r2, const 
 #pragma unroll
           for (int kk = 0; kk < 8; ++kk) {
             const int k = lk + 4 * kk;
-            const double b = k < BW ? Wp[k * ld + r2 + cc] : Vp[(k - BW) * ld + r2 + cc];
+            const double b = k < BW ? gld(Wp, k * ld + r2 + cc) : gld(Vp, (k - BW) * ld + r2 + cc);
             bo[kk] = col < m ? b : 0.0;
           }
 #pragma unroll
@@ -180,7 +182,7 @@ __device__ __noinline__ void band_pass(gbl_dbl* A, int n, int ld, int r2, const 
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
             const int row = I0 + 16 * bi + 4 * i + lk, col = J0 + 16 * (bj + h) + lr;
-            if (row < m && col <= row) A2[row * ld + col] = acc[h][i];
+            if (row < m && col <= row) gst(A2, row * ld + col, acc[h][i]);
           }
       }
       if (want_x) {
@@ -196,6 +198,13 @@ __device__ __noinline__ void band_pass(gbl_dbl* A, int n, int ld, int r2, const 
           }
         __syncthreads();
         if (w < 4) {
+          // V rows of the tile column
+#pragma unroll
+          for (int kk = 0; kk < 16; ++kk) {
+            const int col = J0 + 4 * kk + lk;
+            const double v = gld(Vr, lr * ld + min(col, m - 1));
+            vb[kk] = col < m ? v : 0.0;
+          }
 #pragma unroll
           for (int kk = 0; kk < 16; ++kk)
             xi = mfma(tile[tix(16 * w + lr, 4 * kk + lk)], vb[kk], xi);
@@ -208,7 +217,7 @@ __device__ __noinline__ void band_pass(gbl_dbl* A, int n, int ld, int r2, const 
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
             const int row = J0 + 16 * wb + 4 * i + lk;
-            if (row < m) X[lr * ld + r2 + row] += xj[i];
+            if (row < m) gst(X, lr * ld + r2 + row, gld(X, lr * ld + r2 + row) + xj[i]);
           }
         }
         __syncthreads();
@@ -218,7 +227,7 @@ __device__ __noinline__ void band_pass(gbl_dbl* A, int n, int ld, int r2, const 
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int row = I0 + 16 * w + 4 * i + lk;
-        if (row < m) X[lr * ld + r2 + row] = xi[i];
+        if (row < m) gst(X, lr * ld + r2 + row, xi[i]);
       }
     }
   }
@@ -251,7 +260,7 @@ __device__ __forceinline__ void reduce_partials(ta_double4 g, lds_dbl* part, lds
 // data; a Householder QR (dlarfg per column) of its rows below the band gives
 // the reflectors (V1 at row c0, tau1) and R, which goes into A with zeros in
 // the bulge region below it; T of the block reflector I - V T V^T ends in
-// sm[0..256).  tile: 4096 doubles of LDS (reductions); sm: 560 doubles.
+// sm[0..256).  tile: TILE_DOUBLES of LDS (reductions); sm: 560 doubles.
 template <int NT>
 __device__ __noinline__ void band_panel_qr(gbl_dbl* A, int n, int ld, int c0, gbl_dbl* V1,
                                            gbl_dbl* tau1, const gbl_dbl* Vp, const gbl_dbl* Wp,
@@ -519,7 +528,7 @@ __device__ __noinline__ void band_panel_w(int n, int ld, int c0, const gbl_dbl* 
 // A (n x n, lower triangle valid, n <= MAXN) -> band of half-width BW in its
 // lower triangle, zeros 17..31 below the diagonal.  V1, tau1: stage-1
 // reflectors (layout above); Wb: 2 x 16 x ld, X: 16 x ld doubles of global
-// scratch; tile: 4096 doubles of LDS; sm: 560 doubles of LDS; tprof: optional
+// scratch; tile: TILE_DOUBLES of LDS; sm: 560 doubles of LDS; tprof: optional
 // step clocks (strip, QR, store + T, trailing pass, W).
 template <int NT>
 __device__ __noinline__ void band_reduce(gbl_dbl* A, int n, int ld, gbl_dbl* V1, gbl_dbl* tau1,
@@ -669,39 +678,55 @@ __device__ __noinline__ void band_chase(gbl_dbl* A, int n, int ld, gbl_dbl* V2, 
 }
 
 // ------------------------------------------------------ back-transforms
+// sum over the lane's 8-lane half-row (quad_perm xor 1, xor 2,
+// row_half_mirror); bit-identical in every lane of the group
+__device__ __forceinline__ double row8_sum(double v)
+{
+  v += dpp_f64<0xB1>(v);
+  v += dpp_f64<0x4E>(v);
+  v += dpp_f64<0x141>(v);
+  return v;
+}
+
 // Y <- Q2 Y on the first ncol columns of Y (Y[k * ld + col]), the stage-2
-// reflectors in reverse order.  Chunks of CW = 16 columns (8 when LDS is
-// short) are held in LDS; wave w owns CW / 8 chunk columns, and its lanes
-// take four tasks of one sweep at once (tasks of a sweep touch disjoint
-// rows), so the chunk needs no barrier.  The reflector records stream
-// through an LDS staging area in batches of whole sweeps, the next batch
-// loaded into registers while the current one is applied.  region: LDS of
-// nreg doubles (chunk + staging: at least 9 n + 16 chase_tasks(n, 0)).
+// reflectors one at a time in reverse order.  Chunks of 16 columns are held
+// in LDS (rows of 18 doubles: a wave's column pair is one 16-byte access, and
+// the 8 rows of a half-task fall into different banks); wave w owns chunk
+// columns 2w, 2w+1.  Its lanes take eight tasks of one sweep at once (tasks
+// of a sweep touch disjoint rows), two rows of a reflector per lane, so the
+// chunk needs no barrier.  The reflector records stream through an LDS
+// staging area in batches of whole sweeps, the next batch loaded into
+// registers while the current one is applied.  region: LDS of nreg doubles
+// (chunk 10 n + staging of at least 16 chase_tasks(n, 0)).
 constexpr int Q2_PF = 4;  // staged doubles per thread and batch
 
 template <int NT>
 __device__ __noinline__ void apply_q2(const gbl_dbl* V2, int n, int ld, int ncol, gbl_dbl* Y,
                                       lds_dbl* region, int nreg)
 {
+  V2 = sgpr_ptr(V2);
+  Y = sgpr_ptr(Y);
+  n = sgpr_int(n);
+  ld = sgpr_int(ld);
   const int tid = threadIdx.x, lane = tid & 63, w = wave_id();
-  const int i = lane & 15, ts = lane >> 4;
-  // chunk rows padded by one double: the 16 rows a reflector touches then
-  // fall into different LDS banks
-  const int CW = (nreg - 17 * n) / BW >= chase_tasks(n, 0) ? 16 : 8;
-  const int CS = CW + 1;
+  const int r8 = lane & 7, ts = lane >> 3;
+  // 16 columns per chunk where LDS allows (rows of 18 doubles), else 8
+  // (rows of 10; waves 4-7 idle)
+  const int CW = (nreg - 18 * n) / BW >= chase_tasks(n, 0) ? 16 : 8;
+  const int CS = CW + 2;
   lds_dbl* chunk = region;
   lds_dbl* stage = region + CS * n;
   int cap = (nreg - CS * n) / BW;  // records per batch
   if (cap > Q2_PF * NT / BW) cap = Q2_PF * NT / BW;
-  const int ca = (CW / 8) * w, cb = ca + 1;
-  const bool two = CW == 16;
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  typedef __attribute__((address_space(3))) d2 lds_d2;
+  lds_d2* pair = (lds_d2*)(chunk + 2 * w);
   for (int c0 = 0; c0 < ncol; c0 += CW) {
     const int cw = min(CW, ncol - c0);
     for (int idx = tid; idx < n * CW; idx += NT) {
       const int r = idx / CW, c = idx - r * CW;
       chunk[r * CS + c] = c < cw ? Y[(int64_t)r * ld + c0 + c] : 0.0;
     }
-    // batches of whole sweeps, from the last sweep down: sweeps [sb, se]
     auto batch_start = [&](int se) {
       int sb = se;
       const int end = chase_first(n, se + 1);
@@ -739,27 +764,26 @@ __device__ __noinline__ void apply_q2(const gbl_dbl* V2, int n, int ld, int ncol
           pf[q] = k < r1 ? V2[k] : 0.0;
         }
       }
-      if (ca < cw) {
+      if (2 * w < cw) {
         for (int s = se; s >= sb; --s) {
           const int ns = chase_tasks(n, s);
           const int rs = chase_first(n, s) - base;
-          for (int tb = 0; tb < ns; tb += 4) {
+          for (int tb = 0; tb < ns; tb += 8) {
             const int t = tb + ts;
             const int lo = s + 1 + BW * t;
             const int L = t < ns ? min(s + BW * (t + 1), n - 1) + 1 - lo : 0;
-            const bool act = i < L;
+            const bool a0 = r8 < L, a1 = r8 + 8 < L;
             const lds_dbl* rec = stage + (rs + (t < ns ? t : 0)) * BW;
             const double tau = t < ns ? rec[0] : 0.0;
-            const double vi = act ? (i == 0 ? 1.0 : rec[i]) : 0.0;
-            const int row = act ? lo + i : 0;
-            const double ya = act ? chunk[row * CS + ca] : 0.0;
-            const double yb = (act && two) ? chunk[row * CS + cb] : 0.0;
-            const double pa = row16_sum(vi * ya);
-            const double pb = row16_sum(vi * yb);
-            if (act) {
-              chunk[row * CS + ca] = ya - tau * vi * pa;
-              if (two) chunk[row * CS + cb] = yb - tau * vi * pb;
-            }
+            const double v0 = a0 ? (r8 == 0 ? 1.0 : rec[r8]) : 0.0;
+            const double v1 = a1 ? rec[r8 + 8] : 0.0;
+            const int row0 = a0 ? lo + r8 : 0, row1 = a1 ? lo + r8 + 8 : 0;
+            const d2 y0 = pair[row0 * (CS / 2)];
+            const d2 y1 = pair[row1 * (CS / 2)];
+            const double px = row8_sum(v0 * y0.x + v1 * y1.x);
+            const double py = row8_sum(v0 * y0.y + v1 * y1.y);
+            if (a0) pair[row0 * (CS / 2)] = d2{y0.x - tau * v0 * px, y0.y - tau * v0 * py};
+            if (a1) pair[row1 * (CS / 2)] = d2{y1.x - tau * v1 * px, y1.y - tau * v1 * py};
           }
         }
       }
@@ -769,6 +793,173 @@ __device__ __noinline__ void apply_q2(const gbl_dbl* V2, int n, int ld, int ncol
     __syncthreads();
     for (int idx = tid; idx < n * CW; idx += NT) {
       const int r = idx / CW, c = idx - r * CW;
+      if (c < cw) Y[(int64_t)r * ld + c0 + c] = chunk[r * CS + c];
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------- stage-2 back-transform (blocked)
+// Reflectors of 16 consecutive sweeps (group g: sweeps 16 g .. 16 g + 15) at
+// the same task t act on rows rb .. rb + 30 (rb = 16 g + 1 + 16 t), reflector
+// j of the block on rows rb + j .. rb + j + 15.  Together they are one block
+// reflector I - V T V^T (V: 31 x 16, column j the reflector of sweep 16 g + j
+// shifted down by j; T upper triangular from the forward recurrence
+// T[0:j, j] = -tau_j T[0:j, 0:j] (V[:, 0:j]^T v_j), T[j][j] = tau_j, so an
+// identity reflector gives a zero row and column).  Y <- Q2 Y takes the
+// groups last first and, within a group, the tasks in increasing order;
+// group g runs task t at step (G - 1 - g) + t.  Blocks of one step act on
+// disjoint rows, and every pair of blocks that share a row keeps that order.
+__device__ __forceinline__ int q2_first_block(int F, int g) { return g + g * F - g * (g - 1) / 2; }
+
+__device__ __forceinline__ bool q2_member(int n, int s, int t)
+{
+  return s <= n - 3 && t < chase_tasks(n, s);
+}
+
+// T of every block into Tg (256 doubles per block, row-major).  lds: 528
+// doubles per wave.
+template <int NT>
+__device__ __noinline__ void q2_prepare(const gbl_dbl* V2, int n, gbl_dbl* Tg, lds_dbl* lds)
+{
+  constexpr int NW = NT / 64;
+  V2 = sgpr_ptr(V2);
+  Tg = sgpr_ptr(Tg);
+  n = sgpr_int(n);
+  const int lane = threadIdx.x & 63, w = wave_id();
+  const int a = lane & 15;
+  const int F = (n - 3) / BW, G = F + 1;
+  const int nb = q2_first_block(F, G);
+  lds_dbl* Vl = lds + w * 528;  // [16][16] reflector vectors
+  lds_dbl* Gm = Vl + 256;       // [16][16] V^T V
+  lds_dbl* tl = Vl + 512;       // [16] tau
+  for (int blk = w; blk < nb; blk += NW) {
+    int g = 0;
+    while (q2_first_block(F, g + 1) <= blk) ++g;
+    const int t = blk - q2_first_block(F, g);
+    const int s = BW * g + a;
+    const bool mem = lane < 16 && q2_member(n, s, t);
+    const gbl_dbl* rec = V2 + (int64_t)(mem ? chase_first(n, s) + t : 0) * BW;
+    if (lane < 16) {
+#pragma unroll
+      for (int i = 0; i < BW; ++i) Vl[a * BW + i] = mem ? (i == 0 ? 1.0 : rec[i]) : 0.0;
+      tl[a] = mem ? rec[0] : 0.0;
+    }
+    __builtin_amdgcn_wave_barrier();
+    // G[a][c] = sum_r V[r][a] V[r][c], V[r][j] = v_j[r - j]
+    if (lane < 16) {
+#pragma unroll
+      for (int c = 0; c < BW; ++c) {
+        double acc = 0.0;
+#pragma unroll
+        for (int i = 0; i < BW; ++i) {
+          const int k = i + a - c;
+          if (k >= 0 && k < BW) acc += Vl[a * BW + i] * Vl[c * BW + k];
+        }
+        Gm[a * BW + c] = acc;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    // row a of T: T[a][j] = -tau_j sum_{k=a}^{j-1} T[a][k] G[k][j] (j > a)
+    double tr[BW];
+#pragma unroll
+    for (int j = 0; j < BW; ++j) {
+      double v = 0.0;
+      if (j == a) v = tl[j];
+      if (j > a) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < j; ++k)
+          if (k >= a) acc += tr[k] * Gm[k * BW + j];
+        v = -tl[j] * acc;
+      }
+      tr[j] = v;
+    }
+    if (lane < 16) {
+#pragma unroll
+      for (int j = 0; j < BW; ++j) Tg[(int64_t)blk * 256 + a * BW + j] = tr[j];
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  __syncthreads();
+}
+
+// V of block (g, t) at block row r, column j (0 outside the reflector)
+__device__ __forceinline__ double q2_v(const gbl_dbl* V2, int n, int g, int t, int r, int j)
+{
+  const int s = BW * g + j, k = r - j;
+  if (k < 0 || k >= BW || !q2_member(n, s, t)) return 0.0;
+  return k == 0 ? 1.0 : V2[(int64_t)(chase_first(n, s) + t) * BW + k];
+}
+
+// Y <- Q2 Y on the first ncol columns of Y (Y[k * ld + col]) with the block
+// reflectors (Tg from q2_prepare), 16-column chunks of Y in LDS (chunk:
+// 17 n doubles, rows padded against bank conflicts).  Each block is three
+// matrix-core products by one wave: W = V^T Y_rows, W' = T W,
+// Y_rows -= V W'.
+template <int NT>
+__device__ __noinline__ void apply_q2_wy(const gbl_dbl* V2, const gbl_dbl* Tg, int n, int ld,
+                                         int ncol, gbl_dbl* Y, lds_dbl* chunk)
+{
+  constexpr int NW = NT / 64;
+  constexpr int CS = 17;
+  V2 = sgpr_ptr(V2);
+  Tg = sgpr_ptr(Tg);
+  Y = sgpr_ptr(Y);
+  n = sgpr_int(n);
+  ld = sgpr_int(ld);
+  const int tid = threadIdx.x, lane = tid & 63, w = wave_id();
+  const int lr = lane & 15, lk = lane >> 4;
+  const int F = (n - 3) / BW, G = F + 1;
+  const int nsteps = G + F;
+  for (int c0 = 0; c0 < ncol; c0 += 16) {
+    const int cw = min(16, ncol - c0);
+    for (int idx = tid; idx < n * 16; idx += NT) {
+      const int r = idx >> 4, c = idx & 15;
+      chunk[r * CS + c] = c < cw ? Y[(int64_t)r * ld + c0 + c] : 0.0;
+    }
+    __syncthreads();
+    for (int step = 0; step < nsteps; ++step) {
+      int k = 0;
+      for (int g = G - 1; g >= 0; --g) {
+        const int t = step - (G - 1 - g);
+        if (t < 0) break;
+        if (t >= F + 1 - g) continue;
+        if ((k++ % NW) != w) continue;
+        const int rb = BW * g + 1 + BW * t;
+        const int blk = q2_first_block(F, g) + t;
+        // W = V^T Y_rows (rows 0..31 of the block; row 31 of V is zero)
+        ta_double4 wv = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int kk = 0; kk < 8; ++kk) {
+          const int r = 4 * kk + lk, row = rb + r;
+          const double av = q2_v(V2, n, g, t, r, lr);
+          const double bv = row < n ? chunk[row * CS + lr] : 0.0;
+          wv = mfma(av, bv, wv);
+        }
+        // W' = T W
+        ta_double4 w2 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk)
+          w2 = mfma(Tg[(int64_t)blk * 256 + lr * BW + 4 * kk + lk], wv[kk], w2);
+        // Y_rows -= V W', two halves of 16 rows
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          ta_double4 dv = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+          for (int kk = 0; kk < 4; ++kk)
+            dv = mfma(q2_v(V2, n, g, t, 16 * h + lr, 4 * kk + lk), w2[kk], dv);
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int r = 16 * h + 4 * i + lk, row = rb + r;
+            if (r < 31 && row < n) chunk[row * CS + lr] -= dv[i];
+          }
+        }
+      }
+      __syncthreads();
+    }
+    for (int idx = tid; idx < n * 16; idx += NT) {
+      const int r = idx >> 4, c = idx & 15;
       if (c < cw) Y[(int64_t)r * ld + c0 + c] = chunk[r * CS + c];
     }
     __syncthreads();

@@ -645,7 +645,7 @@ __device__ bool jacobi(double* A, double* CT, int ld, const JRot& R, const JPlan
 // ------------------------------------------- two-stage reduction
 // A (lower triangle) -> tridiagonal d, e (LDS) through a band of half-width
 // 16; the stage-1 and stage-2 reflectors stay in the molecule's scratch for
-// big_eigenvectors.  tile: 4096 doubles of LDS, sm: 560 doubles of LDS.
+// big_eigenvectors.  tile: band::TILE_DOUBLES of LDS, sm: 560 doubles of LDS.
 template <int NT>
 __device__ __noinline__ void big_band_tridiagonalize(gbl_dbl* A, gbl_dbl* scr, int n, int ld,
                                                      lds_dbl* d, lds_dbl* e, lds_dbl* tile,
@@ -720,7 +720,10 @@ __device__ __noinline__ void big_eigenvalues(double* A_, const double* X_, doubl
   if (two_stage) {
     // dense -> band (panel QR, one pass over the trailing triangle per panel)
     // -> tridiagonal (bulge chasing); reflectors kept in the scratch
-    big_band_tridiagonalize<NT>(A, scr, n, ld, d, e, tmp, er + 3 * ld, red, clk);
+    // LDS: the products' tile from er + 8 ld (the row-sum vector of the
+    // one-stage path and the tile region after it: ld + 4096 doubles), the
+    // panel scalars at er + 2 ld; d and e are written last
+    big_band_tridiagonalize<NT>(A, scr, n, ld, d, e, er + 8 * ld, er + 2 * ld, red, clk);
   } else {
     // per-wave column partials of the tridiagonalisation share the GEMM tile region
     gbl_dbl* wbt = scr + 3 * (int64_t)ld * TA_BIG_NT;

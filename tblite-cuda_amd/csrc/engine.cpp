@@ -434,9 +434,9 @@ struct tblite_amd_context_s {
   // bisection, eigenvectors and basis product (47.8 vs 60.8 ms per step)
   int lds_eig = 0;
   // reduction to tridiagonal form of the global-memory SCF kernel (ld <= 512):
-  // 1 two-stage dense -> band -> tridiagonal, 0 one-stage Householder
-  // (TBLITE_AMD_BIG_EIG=band | householder)
-  int big_eig = 0;
+  // 1 two-stage dense -> band -> tridiagonal (the default), 0 one-stage
+  // Householder (TBLITE_AMD_BIG_EIG=band | householder)
+  int big_eig = 1;
   // minimum LDS request of the global-memory SCF kernel (bytes); above half a
   // CU's LDS it runs one molecule per CU (TBLITE_AMD_BIG_LDS_MIN_KB)
   size_t big_lds_min = 0;

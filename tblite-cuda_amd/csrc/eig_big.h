@@ -441,17 +441,18 @@ __device__ __forceinline__ void sturm3(PD d, PE e2, int n, double x1,
 
 // All eigenvalues of T (d, e) in ascending order into lam, to an absolute
 // accuracy of 1e-12 |T| (far below what the SCF observables resolve); e2 =
-// e^2 scratch (LDS).  BTPL = 4 adjacent lanes share a level: each evaluates
-// three Sturm counts per step, so a step splits the bracket into 13 parts
-// (log13 instead of log4 steps) and a workgroup holds NT / 4 levels at once.
+// e^2 scratch (LDS).  BTPL adjacent lanes share a level (4 by default): each
+// evaluates three Sturm counts per step, so a step splits the bracket into
+// 3 BTPL + 1 parts (13 for four lanes: log13 instead of log4 steps) and a
+// workgroup holds NT / BTPL levels at once.  Fewer lanes per level do fewer
+// Sturm counts in total (one lane: 3 log4 against 12 log13 per level), which
+// is what counts once the levels fill the workgroup and every CU is busy.
 // Warm start: with *warm > 0 each level is first bracketed in prev[i] -/+
 // *warm (the previous SCF step's levels), checked by Sturm counts, and falls
 // back to the Gershgorin interval when the check fails.  On return *warm
 // holds the bracket for the next call (twice the largest level shift seen,
 // plus a margin) and scal[0] = |T| (Gershgorin).
-constexpr int BTPL = 4;
-
-template <int NT>
+template <int NT, int BTPL = 4>
 __device__ __noinline__ void bisect_all(const lds_dbl* d, const lds_dbl* e, lds_dbl* e2, int n,
                                        lds_dbl* lam, const lds_dbl* prev, lds_dbl* warm,
                                        lds_dbl* red, lds_dbl* scal)

@@ -739,7 +739,13 @@ __device__ __noinline__ void big_eigenvalues(double* A_, const double* X_, doubl
     }
   }
   clk.lap(TA_PROF_TRIDIAG);
-  big::bisect_all<NT>(d, e, e2, n, ev, TA_LDS(prev_), TA_LDS(warm_), red, scal);
+  // lanes per level: as few as still fill the workgroup (see bisect_all)
+  if (n > NT / 2)
+    big::bisect_all<NT, 1>(d, e, e2, n, ev, TA_LDS(prev_), TA_LDS(warm_), red, scal);
+  else if (n > NT / 4)
+    big::bisect_all<NT, 2>(d, e, e2, n, ev, TA_LDS(prev_), TA_LDS(warm_), red, scal);
+  else
+    big::bisect_all<NT, 4>(d, e, e2, n, ev, TA_LDS(prev_), TA_LDS(warm_), red, scal);
   __syncthreads();
   clk.lap(TA_PROF_BISECT);
 }

@@ -39,7 +39,7 @@ PEAK_FP64 = 78.6e12  # MI355X dense FP64 (matrix and vector), FLOP/s
 # commands: profiles/k_scf_config1.md, profiles/k_scf_config3.md.
 MEASURED_TRAFFIC = {
     ("config1", "k_scf<512, 2, 3, 6, true>"): 16.59e9 + 0.93e9,
-    ("config3", "k_scf<512, 1, 0, 0, false>"): 3001.9e9 + 672.4e9,
+    ("config3", "k_scf<512, 1, 0, 0, false>"): 1644.3e9 + 752.7e9,
 }
 
 WORKLOADS = {
@@ -68,7 +68,8 @@ def scf_kernel_name(path, nthreads):
     """Kernel instantiation of an SCF bucket from its molecules' path (0: LDS
     matrices, 1: global-memory matrices) and its workgroup size."""
     if path == 1:
-        return "k_scf<512, 1, 0, 0, false> (global-memory matrices, Householder eigensolver)"
+        return ("k_scf<512, 1, 0, 0, false> (global-memory matrices, two-stage band reduction "
+                "to 512 AOs)")
     maxt = 2 if nthreads == 512 else 1
     maxb, maxc = (2, 4) if nthreads == 128 else (3, 6)
     return f"k_scf<{nthreads}, {maxt}, {maxb}, {maxc}, true> (LDS matrices, Jacobi)"

@@ -325,21 +325,9 @@ __host__ __device__ inline int64_t band_w_offset(int ld)
 __host__ __device__ inline int64_t band_x_offset(int ld) { return band_w_offset(ld) + 32 * (int64_t)ld; }
 __host__ __device__ inline int64_t band_tau_offset(int ld) { return band_x_offset(ld) + 16 * (int64_t)ld; }
 
-// blocks of the stage-2 back-transformation (16 sweeps x one task each) of
-// an order-n band: sum over groups g of 1 + (n - 3 - 16 g) / 16
-__host__ __device__ inline int band_q2_blocks(int n)
-{
-  if (n < 3) return 0;
-  int c = 0;
-  for (int g = 0; 16 * g <= n - 3; ++g) c += 1 + (n - 3 - 16 * g) / 16;
-  return c;
-}
-// their T factors (16 x 16 each), after the stage-1 tau
-__host__ __device__ inline int64_t band_t_offset(int ld) { return band_tau_offset(ld) + ld; }
-
 __host__ __device__ inline int64_t big_scratch_doubles(int ld)
 {
-  return band_t_offset(ld) + 256 * (int64_t)band_q2_blocks(ld);
+  return band_tau_offset(ld) + ld;
 }
 
 // largest padded AO dimension of the two-stage reduction (its panel strip

@@ -799,173 +799,6 @@ __device__ __noinline__ void apply_q2(const gbl_dbl* V2, int n, int ld, int ncol
   }
 }
 
-// ------------------------------------------- stage-2 back-transform (blocked)
-// Reflectors of 16 consecutive sweeps (group g: sweeps 16 g .. 16 g + 15) at
-// the same task t act on rows rb .. rb + 30 (rb = 16 g + 1 + 16 t), reflector
-// j of the block on rows rb + j .. rb + j + 15.  Together they are one block
-// reflector I - V T V^T (V: 31 x 16, column j the reflector of sweep 16 g + j
-// shifted down by j; T upper triangular from the forward recurrence
-// T[0:j, j] = -tau_j T[0:j, 0:j] (V[:, 0:j]^T v_j), T[j][j] = tau_j, so an
-// identity reflector gives a zero row and column).  Y <- Q2 Y takes the
-// groups last first and, within a group, the tasks in increasing order;
-// group g runs task t at step (G - 1 - g) + t.  Blocks of one step act on
-// disjoint rows, and every pair of blocks that share a row keeps that order.
-__device__ __forceinline__ int q2_first_block(int F, int g) { return g + g * F - g * (g - 1) / 2; }
-
-__device__ __forceinline__ bool q2_member(int n, int s, int t)
-{
-  return s <= n - 3 && t < chase_tasks(n, s);
-}
-
-// T of every block into Tg (256 doubles per block, row-major).  lds: 528
-// doubles per wave.
-template <int NT>
-__device__ __noinline__ void q2_prepare(const gbl_dbl* V2, int n, gbl_dbl* Tg, lds_dbl* lds)
-{
-  constexpr int NW = NT / 64;
-  V2 = sgpr_ptr(V2);
-  Tg = sgpr_ptr(Tg);
-  n = sgpr_int(n);
-  const int lane = threadIdx.x & 63, w = wave_id();
-  const int a = lane & 15;
-  const int F = (n - 3) / BW, G = F + 1;
-  const int nb = q2_first_block(F, G);
-  lds_dbl* Vl = lds + w * 528;  // [16][16] reflector vectors
-  lds_dbl* Gm = Vl + 256;       // [16][16] V^T V
-  lds_dbl* tl = Vl + 512;       // [16] tau
-  for (int blk = w; blk < nb; blk += NW) {
-    int g = 0;
-    while (q2_first_block(F, g + 1) <= blk) ++g;
-    const int t = blk - q2_first_block(F, g);
-    const int s = BW * g + a;
-    const bool mem = lane < 16 && q2_member(n, s, t);
-    const gbl_dbl* rec = V2 + (int64_t)(mem ? chase_first(n, s) + t : 0) * BW;
-    if (lane < 16) {
-#pragma unroll
-      for (int i = 0; i < BW; ++i) Vl[a * BW + i] = mem ? (i == 0 ? 1.0 : rec[i]) : 0.0;
-      tl[a] = mem ? rec[0] : 0.0;
-    }
-    __builtin_amdgcn_wave_barrier();
-    // G[a][c] = sum_r V[r][a] V[r][c], V[r][j] = v_j[r - j]
-    if (lane < 16) {
-#pragma unroll
-      for (int c = 0; c < BW; ++c) {
-        double acc = 0.0;
-#pragma unroll
-        for (int i = 0; i < BW; ++i) {
-          const int k = i + a - c;
-          if (k >= 0 && k < BW) acc += Vl[a * BW + i] * Vl[c * BW + k];
-        }
-        Gm[a * BW + c] = acc;
-      }
-    }
-    __builtin_amdgcn_wave_barrier();
-    // row a of T: T[a][j] = -tau_j sum_{k=a}^{j-1} T[a][k] G[k][j] (j > a)
-    double tr[BW];
-#pragma unroll
-    for (int j = 0; j < BW; ++j) {
-      double v = 0.0;
-      if (j == a) v = tl[j];
-      if (j > a) {
-        double acc = 0.0;
-#pragma unroll
-        for (int k = 0; k < j; ++k)
-          if (k >= a) acc += tr[k] * Gm[k * BW + j];
-        v = -tl[j] * acc;
-      }
-      tr[j] = v;
-    }
-    if (lane < 16) {
-#pragma unroll
-      for (int j = 0; j < BW; ++j) Tg[(int64_t)blk * 256 + a * BW + j] = tr[j];
-    }
-    __builtin_amdgcn_wave_barrier();
-  }
-  __syncthreads();
-}
-
-// V of block (g, t) at block row r, column j (0 outside the reflector)
-__device__ __forceinline__ double q2_v(const gbl_dbl* V2, int n, int g, int t, int r, int j)
-{
-  const int s = BW * g + j, k = r - j;
-  if (k < 0 || k >= BW || !q2_member(n, s, t)) return 0.0;
-  return k == 0 ? 1.0 : V2[(int64_t)(chase_first(n, s) + t) * BW + k];
-}
-
-// Y <- Q2 Y on the first ncol columns of Y (Y[k * ld + col]) with the block
-// reflectors (Tg from q2_prepare), 16-column chunks of Y in LDS (chunk:
-// 17 n doubles, rows padded against bank conflicts).  Each block is three
-// matrix-core products by one wave: W = V^T Y_rows, W' = T W,
-// Y_rows -= V W'.
-template <int NT>
-__device__ __noinline__ void apply_q2_wy(const gbl_dbl* V2, const gbl_dbl* Tg, int n, int ld,
-                                         int ncol, gbl_dbl* Y, lds_dbl* chunk)
-{
-  constexpr int NW = NT / 64;
-  constexpr int CS = 17;
-  V2 = sgpr_ptr(V2);
-  Tg = sgpr_ptr(Tg);
-  Y = sgpr_ptr(Y);
-  n = sgpr_int(n);
-  ld = sgpr_int(ld);
-  const int tid = threadIdx.x, lane = tid & 63, w = wave_id();
-  const int lr = lane & 15, lk = lane >> 4;
-  const int F = (n - 3) / BW, G = F + 1;
-  const int nsteps = G + F;
-  for (int c0 = 0; c0 < ncol; c0 += 16) {
-    const int cw = min(16, ncol - c0);
-    for (int idx = tid; idx < n * 16; idx += NT) {
-      const int r = idx >> 4, c = idx & 15;
-      chunk[r * CS + c] = c < cw ? Y[(int64_t)r * ld + c0 + c] : 0.0;
-    }
-    __syncthreads();
-    for (int step = 0; step < nsteps; ++step) {
-      int k = 0;
-      for (int g = G - 1; g >= 0; --g) {
-        const int t = step - (G - 1 - g);
-        if (t < 0) break;
-        if (t >= F + 1 - g) continue;
-        if ((k++ % NW) != w) continue;
-        const int rb = BW * g + 1 + BW * t;
-        const int blk = q2_first_block(F, g) + t;
-        // W = V^T Y_rows (rows 0..31 of the block; row 31 of V is zero)
-        ta_double4 wv = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int kk = 0; kk < 8; ++kk) {
-          const int r = 4 * kk + lk, row = rb + r;
-          const double av = q2_v(V2, n, g, t, r, lr);
-          const double bv = row < n ? chunk[row * CS + lr] : 0.0;
-          wv = mfma(av, bv, wv);
-        }
-        // W' = T W
-        ta_double4 w2 = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int kk = 0; kk < 4; ++kk)
-          w2 = mfma(Tg[(int64_t)blk * 256 + lr * BW + 4 * kk + lk], wv[kk], w2);
-        // Y_rows -= V W', two halves of 16 rows
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          ta_double4 dv = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-          for (int kk = 0; kk < 4; ++kk)
-            dv = mfma(q2_v(V2, n, g, t, 16 * h + lr, 4 * kk + lk), w2[kk], dv);
-#pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const int r = 16 * h + 4 * i + lk, row = rb + r;
-            if (r < 31 && row < n) chunk[row * CS + lr] -= dv[i];
-          }
-        }
-      }
-      __syncthreads();
-    }
-    for (int idx = tid; idx < n * 16; idx += NT) {
-      const int r = idx >> 4, c = idx & 15;
-      if (c < cw) Y[(int64_t)r * ld + c0 + c] = chunk[r * CS + c];
-    }
-    __syncthreads();
-  }
-}
-
 // Y <- Q1 Y on the first ncol columns of Y, Q1 = H_0 H_1 ... (stage-1
 // reflectors, reflector k on rows k + BW..), in compact WY blocks of NB
 // reflectors, last block first, as big::back_transform: T^-1 =

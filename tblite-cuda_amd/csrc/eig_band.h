@@ -16,12 +16,13 @@
 //   stage 2   band -> tridiagonal by bulge chasing: sweep s annihilates
 //             column s with a reflector of length BW and chases the bulge it
 //             creates down the band, one BW x BW block step (task) at a time.
-//             A wave runs one task (three blocks in registers); sweeps are
-//             pipelined LAG = 3 tasks apart, which keeps every pair of tasks
-//             of one step on disjoint elements and every pair of tasks that
-//             share an element in sweep order; one barrier per step.  The
-//             band stays in A's lower triangle, with room for the bulges
-//             (2 BW - 1 below the diagonal; stage 1 leaves zeros there).
+//             A wave runs the tasks of one sweep (three blocks in registers);
+//             sweep s's task t waits for sweep s - 1's task t + 2 through a
+//             progress counter in LDS, which keeps every pair of tasks that
+//             share an element in sweep order and tasks running at the same
+//             time on disjoint elements.  The band stays in A's lower
+//             triangle, with room for the bulges (2 BW - 1 below the
+//             diagonal; stage 1 leaves zeros there).
 //   vectors   Z = Q1 Q2 Y: the stage-2 reflectors applied in reverse order
 //             to 16-column chunks of Y held in LDS (one wave per column pair,
 //             no barriers), then the stage-1 reflectors in compact WY blocks
@@ -49,7 +50,6 @@ namespace band {
 constexpr int BW = 16;             // half-bandwidth after stage 1
 constexpr int QB = 4;              // 16-row strip blocks per wave
 constexpr int MAXN = 8 * QB * BW;  // largest order served (512)
-constexpr int LAG = 3;             // tasks between consecutive sweeps
 
 using big::ta_double4;
 
@@ -571,30 +571,54 @@ __device__ __noinline__ void band_reduce(gbl_dbl* A, int n, int ld, gbl_dbl* V1,
 }
 
 // ------------------------------------------------------ stage 2
-// One task (sweep s, task t) on the band in A, by one wave.  Lane (i, g):
-// row i = lane % 16 of each block, columns 4 g .. 4 g + 3.
-__device__ __forceinline__ void chase_task(gbl_dbl* A, int n, int ld, int s, int t,
-                                           gbl_dbl* V2)
+// Task (sweep s, task t) on the band in A, by one wave.  Rows of the task
+// lo .. hi-1 (L of them), the annihilated column c (the left block's Lc
+// columns start there), the block below hi .. bh-1 (Lr rows).  Lane (i, g):
+// row i = lane % 16 of each 16 x 16 block, columns 4 g .. 4 g + 3.
+struct ChaseGeom {
+  int lo, hi, L, c, Lc, Lr;
+};
+
+__device__ __forceinline__ ChaseGeom chase_geom(int n, int s, int t)
+{
+  ChaseGeom q;
+  q.lo = s + 1 + BW * t;
+  q.hi = min(s + BW * (t + 1), n - 1) + 1;
+  q.L = q.hi - q.lo;
+  q.c = t == 0 ? s : q.lo - BW;
+  q.Lc = t == 0 ? 1 : BW;
+  q.Lr = min(q.hi + BW, n) - q.hi;
+  return q;
+}
+
+// the task's diagonal block and block below (and, with left, its left block)
+__device__ __forceinline__ void chase_load(const gbl_dbl* A, int n, int ld, int s, int t, bool left,
+                                           double* lb, double* dd, double* rr)
 {
   const int lane = threadIdx.x & 63;
   const int i = lane & 15, g = lane >> 4;
-  const int lo = s + 1 + BW * t;
-  const int hi = min(s + BW * (t + 1), n - 1) + 1;
-  const int L = hi - lo;
-  const int c = t == 0 ? s : lo - BW;  // column annihilated by this task
-  const int Lc = t == 0 ? 1 : BW;
-  const int bh = min(hi + BW, n);
-  const int Lr = bh - hi;
-  double lb[4], dd[4], rr[4];
+  const ChaseGeom q = chase_geom(n, s, t);
 #pragma unroll
   for (int jj = 0; jj < 4; ++jj) {
     const int j = 4 * g + jj;
-    lb[jj] = (i < L && j < Lc) ? A[(int64_t)(lo + i) * ld + c + j] : 0.0;
+    if (left) lb[jj] = (i < q.L && j < q.Lc) ? gld(A, (q.lo + i) * ld + q.c + j) : 0.0;
     const int a = i > j ? i : j, b = i > j ? j : i;
-    dd[jj] = (i < L && j < L) ? A[(int64_t)(lo + a) * ld + lo + b] : 0.0;
-    rr[jj] = (i < Lr && j < L) ? A[(int64_t)(hi + i) * ld + lo + j] : 0.0;
+    dd[jj] = (i < q.L && j < q.L) ? gld(A, (q.lo + a) * ld + q.lo + b) : 0.0;
+    rr[jj] = (i < q.Lr && j < q.L) ? gld(A, (q.hi + i) * ld + q.lo + j) : 0.0;
   }
-  // reflector from column 0 of the left block (held by lanes 0..15)
+}
+
+// The task on its three blocks in registers: the reflector from column 0 of
+// the left block, applied from the left to the left block, two-sided to the
+// diagonal block and from the right to the block below; the blocks and the
+// reflector record are stored, rr keeps the updated block below (the next
+// task's left block).
+__device__ __forceinline__ void chase_apply(gbl_dbl* A, int n, int ld, int s, int t, double* lb,
+                                            double* dd, double* rr, gbl_dbl* V2)
+{
+  const int lane = threadIdx.x & 63;
+  const int i = lane & 15, g = lane >> 4;
+  const ChaseGeom q = chase_geom(n, s, t);
   const double x = __shfl(lb[0], i, 64);
   const double ss = row16_sum(i >= 1 ? x * x : 0.0);
   const double alpha = bcast_lane0(x);
@@ -608,11 +632,10 @@ __device__ __forceinline__ void chase_task(gbl_dbl* A, int n, int ld, int s, int
     tau = (beta - alpha) / beta;
     scale = 1.0 / (alpha - beta);
   }
-  const double vi = i == 0 ? 1.0 : (i < L ? x * scale : 0.0);
+  const double vi = i == 0 ? 1.0 : (i < q.L ? x * scale : 0.0);
   double vj[4];
 #pragma unroll
   for (int jj = 0; jj < 4; ++jj) vj[jj] = __shfl(vi, 4 * g + jj, 64);
-  // left block: H applied from the left; its first column becomes beta e_1
   if (t > 0) {
 #pragma unroll
     for (int jj = 0; jj < 4; ++jj) {
@@ -621,7 +644,7 @@ __device__ __forceinline__ void chase_task(gbl_dbl* A, int n, int ld, int s, int
     }
   }
   if (g == 0) lb[0] = i == 0 ? beta : 0.0;
-  // diagonal block: H D H = D - v w^T - w v^T, w = y - (tau/2)(y.v) v, y = tau D v
+  // H D H = D - v w^T - w v^T, w = y - (tau/2)(y.v) v, y = tau D v
   double part = 0.0;
 #pragma unroll
   for (int jj = 0; jj < 4; ++jj) part += dd[jj] * vj[jj];
@@ -634,7 +657,6 @@ __device__ __forceinline__ void chase_task(gbl_dbl* A, int n, int ld, int s, int
     const double wj = __shfl(wv, 4 * g + jj, 64);
     dd[jj] -= vi * wj + wv * vj[jj];
   }
-  // block below: H applied from the right
   double qr = 0.0;
 #pragma unroll
   for (int jj = 0; jj < 4; ++jj) qr += rr[jj] * vj[jj];
@@ -644,35 +666,88 @@ __device__ __forceinline__ void chase_task(gbl_dbl* A, int n, int ld, int s, int
 #pragma unroll
   for (int jj = 0; jj < 4; ++jj) {
     const int j = 4 * g + jj;
-    if (i < L && j < Lc) A[(int64_t)(lo + i) * ld + c + j] = lb[jj];
-    if (i < L && j <= i) A[(int64_t)(lo + i) * ld + lo + j] = dd[jj];
-    if (i < Lr && j < L) A[(int64_t)(hi + i) * ld + lo + j] = rr[jj];
+    if (i < q.L && j < q.Lc) gst(A, (q.lo + i) * ld + q.c + j, lb[jj]);
+    if (i < q.L && j <= i) gst(A, (q.lo + i) * ld + q.lo + j, dd[jj]);
+    if (i < q.Lr && j < q.L) gst(A, (q.hi + i) * ld + q.lo + j, rr[jj]);
   }
-  if (g == 0) V2[(chase_first(n, s) + t) * BW + i] = i == 0 ? tau : vi;
+  if (g == 0) gst(V2, (chase_first(n, s) + t) * BW + i, i == 0 ? tau : vi);
 }
 
 // Band (A's lower triangle, half-width BW, zeros below) -> tridiagonal:
 // d[k] = T[k][k], e[k] = T[k+1][k] in LDS.  Requires n >= 3.
+//
+// Wave w runs sweeps w, w + 8, ... in order, its tasks one after another.
+// Sweep s's task t starts once sweep s - 1 has finished task t + 2: tasks
+// that share an element then keep sweep order, and those running at the
+// same time touch disjoint elements (the model of the pipelined order with a
+// lag of three tasks, checked element by element).  Progress is published
+// per wave in LDS (stat: NW ints, "sweep s has finished task t" as
+// s * 65536 + t + 1, after the task's stores have completed); sweeps only
+// wait for lower sweeps, and all waves of the workgroup are resident, so the
+// waits end; each is still bounded (a wait that runs out marks the result
+// non-finite).  The next task's diagonal block and block below are loaded
+// before the progress store when sweep s - 1 is already past them (its task
+// t + 3 finished; task t + 4 and later touch rows below them), so that load
+// overlaps the store's completion; its left block is the block below of the
+// task just done, kept in registers.
 template <int NT>
 __device__ __noinline__ void band_chase(gbl_dbl* A, int n, int ld, gbl_dbl* V2, lds_dbl* d,
-                                        lds_dbl* e)
+                                        lds_dbl* e, lds_int* stat)
 {
   constexpr int NW = NT / 64;
-  const int w = wave_id();
-  const int t0max = chase_tasks(n, 0);
-  const int nsteps = LAG * (n - 3) + 1;
-  for (int step = 0; step < nsteps; ++step) {
-    const int shi = min(step / LAG, n - 3);
-    for (int s = shi - w; s >= 0; s -= NW) {
-      const int t = step - LAG * s;
-      if (t >= t0max) break;
-      if (t < chase_tasks(n, s)) chase_task(A, n, ld, s, t, V2);
+  constexpr int SPIN_MAX = 1 << 26;
+  A = sgpr_ptr(A);
+  V2 = sgpr_ptr(V2);
+  n = sgpr_int(n);
+  ld = sgpr_int(ld);
+  const int lane = threadIdx.x & 63, w = wave_id();
+  if (threadIdx.x < NW + 1) stat[threadIdx.x] = threadIdx.x < NW ? -1 : 0;
+  __syncthreads();
+  bool ok = true;
+  auto progress = [&](int sw) {
+    return __hip_atomic_load(stat + (sw % NW), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+  };
+  auto passed = [&](int sw, int t) {  // sweep sw has finished task t (or all of its tasks)
+    return sw < 0 || progress(sw) >= sw * 65536 + min(t, chase_tasks(n, sw) - 1) + 1;
+  };
+  auto wait_passed = [&](int sw, int t) {
+    int spins = 0;
+    while (ok && !passed(sw, t)) {
+      __builtin_amdgcn_s_sleep(1);
+      if (++spins > SPIN_MAX) ok = false;
     }
-    __syncthreads();
+  };
+  for (int s = w; s <= n - 3 && ok; s += NW) {
+    const int ns = chase_tasks(n, s);
+    double lb[4], dd[4], rr[4];
+    wait_passed(s - 1, 2);
+    chase_load(A, n, ld, s, 0, true, lb, dd, rr);
+    for (int t = 0; t < ns && ok; ++t) {
+      chase_apply(A, n, ld, s, t, lb, dd, rr, V2);
+      bool pre = false;
+      if (t + 1 < ns) {
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) lb[jj] = rr[jj];
+        if (passed(s - 1, t + 3)) {
+          chase_load(A, n, ld, s, t + 1, false, lb, dd, rr);
+          pre = true;
+        }
+      }
+      if (lane == 0)
+        __hip_atomic_store(stat + w, s * 65536 + t + 1, __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (t + 1 < ns && !pre) {
+        wait_passed(s - 1, t + 3);
+        chase_load(A, n, ld, s, t + 1, false, lb, dd, rr);
+      }
+    }
   }
+  if (!ok && lane == 0) __hip_atomic_store(stat + NW, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  __syncthreads();
+  const bool failed = stat[NW] != 0;
   for (int k = threadIdx.x; k < n; k += NT) {
-    d[k] = A[(int64_t)k * ld + k];
-    if (k < n - 1) e[k] = A[(int64_t)(k + 1) * ld + k];
+    d[k] = failed ? NAN : gld(A, k * ld + k);
+    if (k < n - 1) e[k] = failed ? NAN : gld(A, (k + 1) * ld + k);
   }
   __syncthreads();
 }

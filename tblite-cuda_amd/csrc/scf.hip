@@ -662,7 +662,7 @@ __device__ __noinline__ void big_band_tridiagonalize(gbl_dbl* A, gbl_dbl* scr, i
     for (int q = 0; q < 5; ++q) clk.acc[TA_PROF_TRI_COLUMN + q] += (long long)red[16 + q];
   }
   clk.lap(TA_PROF_BAND1);
-  band::band_chase<NT>(A, n, ld, scr + band_v2_offset(ld), d, e);
+  band::band_chase<NT>(A, n, ld, scr + band_v2_offset(ld), d, e, (lds_int*)(sm + 532));
   clk.lap(TA_PROF_BAND2);
 }
 

@@ -309,7 +309,8 @@ __host__ __device__ inline int64_t big_tri_save_offset(int ld)
 // Two-stage (band) reduction of the global-memory eigensolver, after the
 // saved tridiagonal form: stage-1 reflectors (ld x ld), stage-2 reflectors
 // (ld x (ld / 16 + 2) records of 16), two panel W blocks and the panel X
-// block (16 x ld each), stage-1 tau (ld).
+// block (16 x ld each), stage-1 tau (ld), the T factor of each stage-1 panel
+// (16 x 16 per panel of 16 reflectors: 16 ld).
 __host__ __device__ inline int64_t band_v1_offset(int ld)
 {
   return big_tri_save_offset(ld) + 3 * (int64_t)ld + 8;
@@ -325,9 +326,11 @@ __host__ __device__ inline int64_t band_w_offset(int ld)
 __host__ __device__ inline int64_t band_x_offset(int ld) { return band_w_offset(ld) + 32 * (int64_t)ld; }
 __host__ __device__ inline int64_t band_tau_offset(int ld) { return band_x_offset(ld) + 16 * (int64_t)ld; }
 
+__host__ __device__ inline int64_t band_t1_offset(int ld) { return band_tau_offset(ld) + ld; }
+
 __host__ __device__ inline int64_t big_scratch_doubles(int ld)
 {
-  return band_tau_offset(ld) + ld;
+  return band_t1_offset(ld) + 16 * (int64_t)ld;
 }
 
 // largest padded AO dimension of the two-stage reduction (its panel strip

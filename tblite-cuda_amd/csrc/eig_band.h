@@ -25,8 +25,9 @@
 //             diagonal; stage 1 leaves zeros there).
 //   vectors   Z = Q1 Q2 Y: the stage-2 reflectors applied in reverse order
 //             to 16-column chunks of Y held in LDS (one wave per column pair,
-//             no barriers), then the stage-1 reflectors in compact WY blocks
-//             of 32 (matrix-core products).
+//             no barriers), then the stage-1 reflectors one panel of 16 at a
+//             time in compact WY form with the T factor stage 1 saved
+//             (matrix-core products).
 //
 // Global layouts (ld = leading dimension, rows r global):
 //   V1[k * ld + r]           stage-1 reflector k = c0 + j (panel c0, column
@@ -263,9 +264,9 @@ __device__ __forceinline__ void reduce_partials(ta_double4 g, lds_dbl* part, lds
 // sm[0..256).  tile: TILE_DOUBLES of LDS (reductions); sm: 560 doubles.
 template <int NT>
 __device__ __noinline__ void band_panel_qr(gbl_dbl* A, int n, int ld, int c0, gbl_dbl* V1,
-                                           gbl_dbl* tau1, const gbl_dbl* Vp, const gbl_dbl* Wp,
-                                           bool upd, lds_dbl* tile, lds_dbl* sm,
-                                           lds_dbl* tprof)
+                                           gbl_dbl* tau1, gbl_dbl* Tp, const gbl_dbl* Vp,
+                                           const gbl_dbl* Wp, bool upd, lds_dbl* tile,
+                                           lds_dbl* sm, lds_dbl* tprof)
 {
   const int tid = threadIdx.x, lane = tid & 63, w = wave_id();
   const int lr = lane & 15, lk = lane >> 4;
@@ -450,6 +451,7 @@ __device__ __noinline__ void band_panel_qr(gbl_dbl* A, int n, int ld, int c0, gb
     }
   }
   __syncthreads();
+  if (tid < BW * BW) Tp[tid] = Tm[tid];  // kept for the back-transformation
   lap(2);
 }
 
@@ -527,13 +529,14 @@ __device__ __noinline__ void band_panel_w(int n, int ld, int c0, const gbl_dbl* 
 // ------------------------------------------------------ stage 1: driver
 // A (n x n, lower triangle valid, n <= MAXN) -> band of half-width BW in its
 // lower triangle, zeros 17..31 below the diagonal.  V1, tau1: stage-1
-// reflectors (layout above); Wb: 2 x 16 x ld, X: 16 x ld doubles of global
+// reflectors (layout above); T1: the T factor of each panel (16 x 16, T[i][c]
+// at 256 p + 16 i + c); Wb: 2 x 16 x ld, X: 16 x ld doubles of global
 // scratch; tile: TILE_DOUBLES of LDS; sm: 560 doubles of LDS; tprof: optional
 // step clocks (strip, QR, store + T, trailing pass, W).
 template <int NT>
 __device__ __noinline__ void band_reduce(gbl_dbl* A, int n, int ld, gbl_dbl* V1, gbl_dbl* tau1,
-                                         gbl_dbl* Wb, gbl_dbl* X, lds_dbl* tile, lds_dbl* sm,
-                                         lds_dbl* tprof)
+                                         gbl_dbl* T1, gbl_dbl* Wb, gbl_dbl* X, lds_dbl* tile,
+                                         lds_dbl* sm, lds_dbl* tprof)
 {
   static_assert(NT == 512, "strip layout assumes 8 waves");
   const bool clk_on = tprof != nullptr && threadIdx.x == 0;
@@ -553,7 +556,7 @@ __device__ __noinline__ void band_reduce(gbl_dbl* A, int n, int ld, gbl_dbl* V1,
     const gbl_dbl* Wp = Wb + (int64_t)((p + 1) & 1) * BW * ld;
     gbl_dbl* Wc = Wb + (int64_t)(p & 1) * BW * ld;
     const gbl_dbl* Vc = V1 + (int64_t)c0 * ld;
-    band_panel_qr<NT>(A, n, ld, c0, V1, tau1, Vp, Wp, upd, tile, sm, tprof);
+    band_panel_qr<NT>(A, n, ld, c0, V1, tau1, T1 + BW * BW * p, Vp, Wp, upd, tile, sm, tprof);
     if (clk_on) clk_last = wall_clock64();
     band_pass<NT>(A, n, ld, r2, Vp, Wp, upd, Vc, X, true, tile);
     lap(3);
@@ -875,79 +878,124 @@ __device__ __noinline__ void apply_q2(const gbl_dbl* V2, int n, int ld, int ncol
 }
 
 // Y <- Q1 Y on the first ncol columns of Y, Q1 = H_0 H_1 ... (stage-1
-// reflectors, reflector k on rows k + BW..), in compact WY blocks of NB
-// reflectors, last block first, as big::back_transform: T^-1 =
-// diag(1/tau) + strict_upper(V^T V), W = V^T Y, W' = T W, Y -= V W'.
-// W: NB x ld doubles of global scratch; sinv, tm: NB x NB doubles of LDS;
-// tile: 2048 doubles of LDS.
-template <int NT, int NB>
-__device__ __noinline__ void back_transform_q1(const gbl_dbl* V1, const gbl_dbl* tau1, int n,
-                                               int ld, int ncol, gbl_dbl* Y, gbl_dbl* W,
-                                               lds_dbl* sinv, lds_dbl* tm, lds_dbl* tile)
+// reflectors, reflector k on rows k + BW..), one panel of 16 at a time, last
+// panel first: Y -= V (T (V^T Y)) with the panel's T saved by stage 1.
+//   W' = T V^T Y   one 16-column strip per wave: W = V^T Y on the matrix
+//                  cores over the panel's rows (two accumulators, the next
+//                  16 rows loaded while the current ones are multiplied),
+//                  then W' = T W from the accumulator registers (the D layout
+//                  of W is the B-operand layout of T W); W' into LDS.
+//   Y -= V W'      16 x 16 blocks of Y (row block, strip) spread over the
+//                  waves, each four matrix-core steps on the block loaded as
+//                  the accumulator; the next block's operands are loaded
+//                  before the current block is multiplied and stored.
+// ws: nws doubles of LDS (T, then W' rows for as many columns as fit).
+template <int NT>
+__device__ __noinline__ void back_transform_q1(const gbl_dbl* V1, const gbl_dbl* T1, int n,
+                                               int ld, int ncol, gbl_dbl* Y, lds_dbl* ws, int nws)
 {
-  const int nr = band_reflectors(n);
-  const int nblk = (nr + NB - 1) / NB;
-  for (int bi = nblk - 1; bi >= 0; --bi) {
-    const int k0 = bi * NB;
-    const int nb = (k0 + NB <= nr) ? NB : nr - k0;
-    const int r0 = k0 + BW;
-    const int m = n - r0;
-    if (m <= 0) continue;
-    auto vel = [=](int rr, int j) -> double {
-      return (j < nb && rr >= j && tau1[k0 + j] != 0.0) ? V1[(int64_t)(k0 + j) * ld + r0 + rr]
-                                                         : 0.0;
-    };
-    auto all_k = [m](int, int, int* ka, int* kb) {
-      *ka = 0;
-      *kb = m;
-    };
-    auto no_skip = [](int, int) { return false; };
-    big::gemm<NT, false, true>(
-        NB, NB, [=](int i, int k) { return vel(k, i); }, [=](int k, int j) { return vel(k, j); },
-        all_k, no_skip,
-        [=](int i, int j, double val) {
-          double s = 0.0;
-          if (i < j) s = val;
-          if (i == j) s = (i < nb && tau1[k0 + i] != 0.0) ? 1.0 / tau1[k0 + i] : 1.0;
-          sinv[i * NB + j] = s;
-        },
-        tile);
-    big::gemm<NT, false, true>(
-        NB, ncol, [=](int i, int k) { return vel(k, i); },
-        [=](int k, int j) { return Y[(int64_t)(r0 + k) * ld + j]; }, all_k, no_skip,
-        [=](int i, int j, double val) { W[(int64_t)i * ld + j] = val; }, tile);
-    if (threadIdx.x < NB) {
-      const int j = threadIdx.x;
-      for (int i = NB - 1; i >= 0; --i) {
-        double t = 0.0;
-        if (i == j) {
-          t = 1.0 / sinv[i * NB + i];
-        } else if (i < j) {
-          double acc = 0.0;
-          for (int k = i + 1; k <= j; ++k) acc += sinv[i * NB + k] * tm[k * NB + j];
-          t = -acc / sinv[i * NB + i];
+  constexpr int NW = NT / 64;
+  V1 = sgpr_ptr(V1);
+  T1 = sgpr_ptr(T1);
+  Y = sgpr_ptr(Y);
+  n = sgpr_int(n);
+  ld = sgpr_int(ld);
+  ncol = sgpr_int(ncol);
+  const int tid = threadIdx.x, lane = tid & 63, w = wave_id();
+  const int lr = lane & 15, lk = lane >> 4;
+  const int np = band_reflectors(n) / BW;
+  lds_dbl* Ts = ws;
+  lds_dbl* Wq = ws + BW * BW;
+  int cc = (((nws - BW * BW) / BW - 4) / 16) * 16;  // columns per chunk
+  if (cc > ((ncol + 15) & ~15)) cc = (ncol + 15) & ~15;
+  const int CSW = cc + 4;  // W' row stride
+  for (int c0 = 0; c0 < ncol; c0 += cc) {
+    const int cw = min(cc, ncol - c0);
+    const int nst = (cw + 15) >> 4;
+    for (int p = np - 1; p >= 0; --p) {
+      const int r0 = BW * p + BW, m = n - r0;
+      const gbl_dbl* V = V1 + (int64_t)(BW * p) * ld;  // reflector j: V[j * ld + r], r >= r0
+      __syncthreads();  // the previous panel's blocks of Y are written, T and W' are free
+      if (tid < BW * BW) Ts[tid] = T1[BW * BW * p + tid];
+      __syncthreads();
+      // ---- W' = T V^T Y, strip st = 16 columns
+      for (int st = w; st < nst; st += NW) {
+        const int cl = 16 * st + lr;
+        const unsigned col = c0 + (cl < cw ? cl : 0);
+        ta_double4 a0 = {0.0, 0.0, 0.0, 0.0}, a1 = {0.0, 0.0, 0.0, 0.0};
+        double va[4], ya[4];
+        auto fetch = [&](int k) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int r = k + 4 * u + lk;
+            const bool ok = r < m;
+            const unsigned rr = r0 + (ok ? r : 0);
+            const double v = gld(V, lr * ld + rr);
+            va[u] = ok ? v : 0.0;
+            ya[u] = gld(Y, rr * ld + col);
+          }
+        };
+        fetch(0);
+        for (int k = 0; k < m; k += 16) {
+          const double v0 = va[0], v1 = va[1], v2 = va[2], v3 = va[3];
+          const double y0 = ya[0], y1 = ya[1], y2 = ya[2], y3 = ya[3];
+          if (k + 16 < m) fetch(k + 16);
+          a0 = mfma(v0, y0, a0);
+          a1 = mfma(v1, y1, a1);
+          a0 = mfma(v2, y2, a0);
+          a1 = mfma(v3, y3, a1);
         }
-        tm[i * NB + j] = t;
+        ta_double4 wq = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk)
+          wq = mfma(Ts[lr * BW + 4 * kk + lk], a0[kk] + a1[kk], wq);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) Wq[(4 * q + lk) * CSW + 16 * st + lr] = wq[q];
+      }
+      __syncthreads();
+      // ---- Y -= V W' over (row block, strip) blocks
+      const int nrb = (m + 15) >> 4;
+      const int ntask = nrb * nst;
+      ta_double4 cy;
+      double av[4], bw[4];
+      auto load = [&](int tk) {
+        const int rb = tk / nst, st = tk - rb * nst;
+        const int cl = 16 * st + lr;
+        const unsigned col = c0 + (cl < cw ? cl : 0);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int row = r0 + 16 * rb + 4 * q + lk;
+          cy[q] = gld(Y, (row < n ? row : r0) * ld + col);
+        }
+        const int ra = r0 + 16 * rb + lr;
+        const unsigned rac = ra < n ? ra : r0;
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {
+          av[ks] = gld(V, (4 * ks + lk) * ld + rac);
+          bw[ks] = -Wq[(4 * ks + lk) * CSW + 16 * st + lr];
+        }
+      };
+      if (w < ntask) load(w);
+      for (int tk = w; tk < ntask; tk += NW) {
+        ta_double4 c = cy;
+        const double a0 = av[0], a1 = av[1], a2 = av[2], a3 = av[3];
+        const double b0 = bw[0], b1 = bw[1], b2 = bw[2], b3 = bw[3];
+        if (tk + NW < ntask) load(tk + NW);
+        c = mfma(a0, b0, c);
+        c = mfma(a1, b1, c);
+        c = mfma(a2, b2, c);
+        c = mfma(a3, b3, c);
+        const int rb = tk / nst, st = tk - rb * nst;
+        const int cl = 16 * st + lr;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int row = r0 + 16 * rb + 4 * q + lk;
+          if (row < n && cl < cw) gst(Y, row * ld + c0 + cl, c[q]);
+        }
       }
     }
-    __syncthreads();
-    big::gemm<NT, true, true>(
-        NB, ncol, [=](int i, int k) { return tm[i * NB + k]; },
-        [=](int k, int j) { return W[(int64_t)k * ld + j]; },
-        [](int, int, int* ka, int* kb) {
-          *ka = 0;
-          *kb = NB;
-        },
-        no_skip, [=](int i, int j, double val) { W[(int64_t)i * ld + j] = val; }, tile);
-    big::gemm<NT, true, true>(
-        m, ncol, [=](int i, int k) { return vel(i, k); },
-        [=](int k, int j) { return W[(int64_t)k * ld + j]; },
-        [](int, int, int* ka, int* kb) {
-          *ka = 0;
-          *kb = NB;
-        },
-        no_skip, [=](int i, int j, double val) { Y[(int64_t)(r0 + i) * ld + j] -= val; }, tile);
   }
+  __syncthreads();
 }
 
 }  // namespace band

@@ -656,7 +656,8 @@ __device__ __noinline__ void big_band_tridiagonalize(gbl_dbl* A, gbl_dbl* scr, i
     for (int q = 0; q < 5; ++q) red[16 + q] = 0.0;
   __syncthreads();
   band::band_reduce<NT>(A, n, ld, scr + band_v1_offset(ld), scr + band_tau_offset(ld),
-                        scr + band_w_offset(ld), scr + band_x_offset(ld), tile, sm,
+                        scr + band_t1_offset(ld), scr + band_w_offset(ld),
+                        scr + band_x_offset(ld), tile, sm,
                         clk.on ? red + 16 : nullptr);
   if (clk.on) {
     for (int q = 0; q < 5; ++q) clk.acc[TA_PROF_TRI_COLUMN + q] += (long long)red[16 + q];
@@ -786,9 +787,8 @@ __device__ __noinline__ void big_eigenvectors(const double* A_, const double* X_
     // eigensolver region is free here), then the panel reflectors
     band::apply_q2<NT>(scr + band_v2_offset(ld), n, ld, nvec, Y, er, eig_region);
     clk.lap(TA_PROF_Q2);
-    band::back_transform_q1<NT, TA_BIG_NB>(scr + band_v1_offset(ld), scr + band_tau_offset(ld), n,
-                                           ld, nvec, Y, wbt, tmp + 2048,
-                                           tmp + 2048 + TA_BIG_NB * TA_BIG_NB, tmp);
+    band::back_transform_q1<NT>(scr + band_v1_offset(ld), scr + band_t1_offset(ld), n, ld, nvec,
+                                Y, er, eig_region);
   } else {
     big::back_transform<NT, TA_BIG_NB>(A, tau, n, ld, nvec, Y, wbt, tmp + 2048,
                                        tmp + 2048 + TA_BIG_NB * TA_BIG_NB, tmp);

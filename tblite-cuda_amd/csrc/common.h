@@ -328,9 +328,20 @@ __host__ __device__ inline int64_t band_tau_offset(int ld) { return band_x_offse
 
 __host__ __device__ inline int64_t band_t1_offset(int ld) { return band_tau_offset(ld) + ld; }
 
+// blocks of the stage-2 back-transformation of an order-ld band (group g of
+// 16 sweeps has F + 1 - g blocks, F = (ld - 3) / 16), and their matrix-core
+// operands (1024 doubles per block) after the stage-1 T factors
+__host__ __device__ inline int64_t band_q2_blocks(int ld)
+{
+  if (ld < 3) return 0;
+  const int64_t f = (ld - 3) / 16;
+  return (f + 1) * (f + 2) / 2;
+}
+__host__ __device__ inline int64_t band_q2op_offset(int ld) { return band_t1_offset(ld) + 16 * (int64_t)ld; }
+
 __host__ __device__ inline int64_t big_scratch_doubles(int ld)
 {
-  return band_t1_offset(ld) + 16 * (int64_t)ld;
+  return band_q2op_offset(ld) + 1024 * band_q2_blocks(ld);
 }
 
 // largest padded AO dimension of the two-stage reduction (its panel strip

@@ -115,6 +115,15 @@ __device__ __forceinline__ double xor32_sum(double v)
   return a + c;
 }
 
+// Orders the LDS accesses of one wave around this point (the hardware runs a
+// wave's LDS instructions in order; this keeps the compiler from moving them
+// across), so that lanes read what other lanes of the wave wrote before it.
+__device__ __forceinline__ void wave_lds_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 // lane 0's value in every lane (scalar broadcast)
 __device__ __forceinline__ double bcast_lane0(double v)
 {

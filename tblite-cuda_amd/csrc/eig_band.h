@@ -23,11 +23,11 @@
 //             time on disjoint elements.  The band stays in A's lower
 //             triangle, with room for the bulges (2 BW - 1 below the
 //             diagonal; stage 1 leaves zeros there).
-//   vectors   Z = Q1 Q2 Y: the stage-2 reflectors applied in reverse order
-//             to 16-column chunks of Y held in LDS (one wave per column pair,
-//             no barriers), then the stage-1 reflectors one panel of 16 at a
-//             time in compact WY form with the T factor stage 1 saved
-//             (matrix-core products).
+//   vectors   Z = Q1 Q2 Y: the stage-2 reflectors in blocks of 16 sweeps at
+//             one task (block reflectors of 31 rows on the matrix cores, each
+//             wave carrying its column strips of Y through the blocks in
+//             order), then the stage-1 reflectors one panel of 16 at a time
+//             in compact WY form with the T factor stage 1 saved.
 //
 // Global layouts (ld = leading dimension, rows r global):
 //   V1[k * ld + r]           stage-1 reflector k = c0 + j (panel c0, column
@@ -756,124 +756,213 @@ __device__ __noinline__ void band_chase(gbl_dbl* A, int n, int ld, gbl_dbl* V2, 
 }
 
 // ------------------------------------------------------ back-transforms
-// sum over the lane's 8-lane half-row (quad_perm xor 1, xor 2,
-// row_half_mirror); bit-identical in every lane of the group
-__device__ __forceinline__ double row8_sum(double v)
+// Stage-2 reflectors in blocks: group g holds sweeps 16 g .. 16 g + 15, and
+// block (g, t) their reflectors of task t, on rows rb .. rb + 30 (rb =
+// 16 g + 1 + 16 t); reflector j of the block (sweep 16 g + j) acts on rows
+// rb + j .. rb + j + 15, and a sweep without a task t adds the identity.
+// Group g has F + 1 - g blocks (F = (n - 3) / 16), numbered from
+// q2_first_block(F, g).  Y <- Q2 Y takes the groups last first and the
+// blocks of a group in task order: two reflectors that share a row are
+// sweeps s > s' with tasks (t, t) or (t, t + 1), so the one of the later
+// sweep always comes first, as in the reverse of the chase.
+__device__ __forceinline__ int q2_first_block(int F, int g) { return g * (F + 1) - g * (g - 1) / 2; }
+
+// Matrix-core operands of every block into Op (1024 doubles per block; value
+// q of lane l at q * 64 + l).  With V (32 x 16, row 31 zero) the block's
+// reflectors and T its upper triangular factor (forward recurrence T[j][j] =
+// tau_j, T[0:j, j] = -tau_j T[0:j, 0:j] V[:, 0:j]^T v_j; the block's product
+// H_0 H_1 ... H_15 = I - V T V^T):
+//   q = 0..7          U = V T^T, rows 4q .. 4q + 3: U[4q + l/16][l%16], the A
+//                     operand of W' = T V^T Y = U^T Y
+//   q = 8 + 4h + k    V[16h + l%16][4k + l/16], the A operand of
+//                     Y_h -= V_h W' (rows 16h .. 16h + 15 of the block)
+// V^T V on the matrix cores (the A and B operands of a k-step coincide), T
+// row by row (row a depends only on its own earlier entries), U from T in
+// LDS.  One block per wave at a time; lds: 512 doubles per wave.
+template <int NT>
+__device__ __noinline__ void q2_operands(const gbl_dbl* V2, int n, gbl_dbl* Op, lds_dbl* lds)
 {
-  v += dpp_f64<0xB1>(v);
-  v += dpp_f64<0x4E>(v);
-  v += dpp_f64<0x141>(v);
-  return v;
+  constexpr int NW = NT / 64;
+  V2 = sgpr_ptr(V2);
+  Op = sgpr_ptr(Op);
+  n = sgpr_int(n);
+  const int lane = threadIdx.x & 63, w = wave_id();
+  const int lr = lane & 15, lk = lane >> 4;
+  const int F = (n - 3) / BW, G = F + 1;
+  const int nb = q2_first_block(F, G);
+  lds_dbl* Gs = lds + w * 512;  // V^T V, tau on the diagonal
+  lds_dbl* Ts = Gs + 256;       // T
+  for (int blk = w; blk < nb; blk += NW) {
+    int g = 0;
+    while (q2_first_block(F, g + 1) <= blk) ++g;
+    const int t = blk - q2_first_block(F, g), s0 = BW * g;
+    // V[r][j] (0 outside reflector j's rows, or for a sweep without task t)
+    auto vel = [&](int r, int j, bool tau) -> double {
+      const int s = s0 + j, e = tau ? 0 : r - j;
+      const bool in = e >= 0 && e < BW && s <= n - 3 && t < chase_tasks(n, s);
+      const unsigned rec = in ? chase_first(n, s) + t : 0;
+      const double x = gld(V2, rec * BW + (in ? e : 0));
+      return in ? ((e == 0 && !tau) ? 1.0 : x) : 0.0;
+    };
+    double x[8], vo[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) x[k] = vel(4 * k + lk, lr, false);
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) vo[4 * h + k] = vel(16 * h + lr, 4 * k + lk, false);
+    const double tau = vel(0, lr, true);
+    ta_double4 gm = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) gm = mfma(x[k], x[k], gm);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) Gs[(4 * q + lk) * BW + lr] = gm[q];
+    if (lk == 0) Gs[lr * BW + lr] = tau;
+    wave_lds_sync();
+    // row a = lr of T
+    {
+      const int a = lr;
+      double tr[BW];
+#pragma unroll
+      for (int j = 0; j < BW; ++j) {
+        double v = 0.0;
+        if (j == a) v = Gs[j * BW + j];
+        if (j > a) {
+          double acc = 0.0;
+#pragma unroll
+          for (int k = 0; k < j; ++k)
+            if (k >= a) acc += tr[k] * Gs[k * BW + j];
+          v = -Gs[j * BW + j] * acc;
+        }
+        tr[j] = v;
+      }
+      if (lk == 0) {
+#pragma unroll
+        for (int j = 0; j < BW; ++j) Ts[a * BW + j] = tr[j];
+      }
+    }
+    wave_lds_sync();
+    const unsigned ob = (unsigned)blk * 1024 + lane;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      ta_double4 du = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) du = mfma(vo[4 * h + k], Ts[lr * BW + 4 * k + lk], du);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) gst(Op, ob + (4 * h + q) * 64, du[q]);
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) gst(Op, ob + (8 + i) * 64, vo[i]);
+    wave_lds_sync();
+  }
 }
 
-// Y <- Q2 Y on the first ncol columns of Y (Y[k * ld + col]), the stage-2
-// reflectors one at a time in reverse order.  Chunks of 16 columns are held
-// in LDS (rows of 18 doubles: a wave's column pair is one 16-byte access, and
-// the 8 rows of a half-task fall into different banks); wave w owns chunk
-// columns 2w, 2w+1.  Its lanes take eight tasks of one sweep at once (tasks
-// of a sweep touch disjoint rows), two rows of a reflector per lane, so the
-// chunk needs no barrier.  The reflector records stream through an LDS
-// staging area in batches of whole sweeps, the next batch loaded into
-// registers while the current one is applied.  region: LDS of nreg doubles
-// (chunk 10 n + staging of at least 16 chase_tasks(n, 0)).
-constexpr int Q2_PF = 4;  // staged doubles per thread and batch
-
+// Y <- Q2 Y on the first ncol columns of Y (Y[k * ld + col]) with the block
+// operands of q2_operands.  Each wave carries one or two 16-column strips of
+// Y (strips w + 16 i and w + 16 i + 8) through every block in order, with no
+// barrier: per block W' = U^T Y_rows (eight matrix-core steps over the
+// block's 32 rows, the D layout of the rows being the B-operand layout),
+// then Y_rows -= V W' (the D layout of W' is again the B-operand layout).
+// The block's 32 rows stay in registers as two 16-row halves: after block t
+// the first half is final for the group and stored, the second becomes the
+// first half of block t + 1, and the next 16 rows are loaded while the
+// current block is multiplied (as are the next block's U operands).
 template <int NT>
-__device__ __noinline__ void apply_q2(const gbl_dbl* V2, int n, int ld, int ncol, gbl_dbl* Y,
-                                      lds_dbl* region, int nreg)
+__device__ __noinline__ void apply_q2(const gbl_dbl* Op, int n, int ld, int ncol, gbl_dbl* Y)
 {
-  V2 = sgpr_ptr(V2);
+  constexpr int NW = NT / 64;
+  Op = sgpr_ptr(Op);
   Y = sgpr_ptr(Y);
   n = sgpr_int(n);
   ld = sgpr_int(ld);
-  const int tid = threadIdx.x, lane = tid & 63, w = wave_id();
-  const int r8 = lane & 7, ts = lane >> 3;
-  // 16 columns per chunk where LDS allows (rows of 18 doubles), else 8
-  // (rows of 10; waves 4-7 idle)
-  const int CW = (nreg - 18 * n) / BW >= chase_tasks(n, 0) ? 16 : 8;
-  const int CS = CW + 2;
-  lds_dbl* chunk = region;
-  lds_dbl* stage = region + CS * n;
-  int cap = (nreg - CS * n) / BW;  // records per batch
-  if (cap > Q2_PF * NT / BW) cap = Q2_PF * NT / BW;
-  typedef double d2 __attribute__((ext_vector_type(2)));
-  typedef __attribute__((address_space(3))) d2 lds_d2;
-  lds_d2* pair = (lds_d2*)(chunk + 2 * w);
-  for (int c0 = 0; c0 < ncol; c0 += CW) {
-    const int cw = min(CW, ncol - c0);
-    for (int idx = tid; idx < n * CW; idx += NT) {
-      const int r = idx / CW, c = idx - r * CW;
-      chunk[r * CS + c] = c < cw ? Y[(int64_t)r * ld + c0 + c] : 0.0;
+  ncol = sgpr_int(ncol);
+  const int lane = threadIdx.x & 63, w = wave_id();
+  const int lr = lane & 15, lk = lane >> 4;
+  const int F = (n - 3) / BW, G = F + 1;
+  const int nst = (ncol + 15) >> 4;
+  auto yload = [&](int r0, unsigned col, ta_double4& y) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int row = r0 + 4 * q + lk;
+      y[q] = gld(Y, (row < n ? row : n - 1) * ld + col);
     }
-    auto batch_start = [&](int se) {
-      int sb = se;
-      const int end = chase_first(n, se + 1);
-      while (sb > 0 && end - chase_first(n, sb - 1) <= cap) --sb;
-      return sb;
-    };
-    int se = n - 3;
-    int sb = batch_start(se);
-    double pf[Q2_PF];
-    {
-      const int r0 = chase_first(n, sb) * BW, r1 = chase_first(n, se + 1) * BW;
+  };
+  auto ystore = [&](int r0, int col, const ta_double4& y) {
 #pragma unroll
-      for (int q = 0; q < Q2_PF; ++q) {
-        const int k = r0 + tid + q * NT;
-        pf[q] = k < r1 ? V2[k] : 0.0;
-      }
+    for (int q = 0; q < 4; ++q) {
+      const int row = r0 + 4 * q + lk;
+      if (row < n && col < ncol) gst(Y, row * ld + col, y[q]);
     }
-    while (se >= 0) {
-      __syncthreads();  // previous batch applied (and the chunk loaded)
-      const int base = chase_first(n, sb);
-#pragma unroll
-      for (int q = 0; q < Q2_PF; ++q) {
-        const int k = tid + q * NT;
-        if (k < cap * BW) stage[k] = pf[q];
+  };
+  for (int sa = w; sa < nst; sa += 2 * NW) {
+    const bool hb = sa + NW < nst;
+    const int ca = 16 * sa + lr, cb = 16 * (sa + NW) + lr;
+    const unsigned cla = ca < ncol ? ca : 0, clb = (hb && cb < ncol) ? cb : 0;
+    for (int g = G - 1; g >= 0; --g) {
+      const int nt = F + 1 - g;
+      const unsigned ob0 = (unsigned)q2_first_block(F, g) * 1024 + lane;
+      int rb = BW * g + 1;
+      ta_double4 a0, a1, b0, b1, na, nb;
+      yload(rb, cla, a0);
+      yload(rb + 16, cla, a1);
+      if (hb) {
+        yload(rb, clb, b0);
+        yload(rb + 16, clb, b1);
       }
-      __syncthreads();
-      // next batch into registers while this one is applied
-      const int se2 = sb - 1;
-      const int sb2 = se2 >= 0 ? batch_start(se2) : 0;
-      if (se2 >= 0) {
-        const int r0 = chase_first(n, sb2) * BW, r1 = chase_first(n, se2 + 1) * BW;
+      double u[8];
 #pragma unroll
-        for (int q = 0; q < Q2_PF; ++q) {
-          const int k = r0 + tid + q * NT;
-          pf[q] = k < r1 ? V2[k] : 0.0;
+      for (int q = 0; q < 8; ++q) u[q] = gld(Op, ob0 + q * 64);
+      for (int t = 0; t < nt; ++t) {
+        const unsigned ob = ob0 + t * 1024;
+        const bool more = t + 1 < nt;
+        double v[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = gld(Op, ob + (8 + i) * 64);
+        if (more) {
+          yload(rb + 32, cla, na);
+          if (hb) yload(rb + 32, clb, nb);
         }
-      }
-      if (2 * w < cw) {
-        for (int s = se; s >= sb; --s) {
-          const int ns = chase_tasks(n, s);
-          const int rs = chase_first(n, s) - base;
-          for (int tb = 0; tb < ns; tb += 8) {
-            const int t = tb + ts;
-            const int lo = s + 1 + BW * t;
-            const int L = t < ns ? min(s + BW * (t + 1), n - 1) + 1 - lo : 0;
-            const bool a0 = r8 < L, a1 = r8 + 8 < L;
-            const lds_dbl* rec = stage + (rs + (t < ns ? t : 0)) * BW;
-            const double tau = t < ns ? rec[0] : 0.0;
-            const double v0 = a0 ? (r8 == 0 ? 1.0 : rec[r8]) : 0.0;
-            const double v1 = a1 ? rec[r8 + 8] : 0.0;
-            const int row0 = a0 ? lo + r8 : 0, row1 = a1 ? lo + r8 + 8 : 0;
-            const d2 y0 = pair[row0 * (CS / 2)];
-            const d2 y1 = pair[row1 * (CS / 2)];
-            const double px = row8_sum(v0 * y0.x + v1 * y1.x);
-            const double py = row8_sum(v0 * y0.y + v1 * y1.y);
-            if (a0) pair[row0 * (CS / 2)] = d2{y0.x - tau * v0 * px, y0.y - tau * v0 * py};
-            if (a1) pair[row1 * (CS / 2)] = d2{y1.x - tau * v1 * px, y1.y - tau * v1 * py};
+        ta_double4 wa = {0.0, 0.0, 0.0, 0.0}, wb = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) wa = mfma(u[k], a0[k], wa);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) wa = mfma(u[4 + k], a1[k], wa);
+        if (hb) {
+#pragma unroll
+          for (int k = 0; k < 4; ++k) wb = mfma(u[k], b0[k], wb);
+#pragma unroll
+          for (int k = 0; k < 4; ++k) wb = mfma(u[4 + k], b1[k], wb);
+        }
+        if (more) {
+#pragma unroll
+          for (int q = 0; q < 8; ++q) u[q] = gld(Op, ob + 1024 + q * 64);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          a0 = mfma(v[k], -wa[k], a0);
+          a1 = mfma(v[4 + k], -wa[k], a1);
+        }
+        ystore(rb, ca, a0);
+        if (hb) {
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            b0 = mfma(v[k], -wb[k], b0);
+            b1 = mfma(v[4 + k], -wb[k], b1);
           }
+          ystore(rb, cb, b0);
+        }
+        if (more) {
+          a0 = a1;
+          a1 = na;
+          b0 = b1;
+          b1 = nb;
+          rb += 16;
+        } else {
+          ystore(rb + 16, ca, a1);
+          if (hb) ystore(rb + 16, cb, b1);
         }
       }
-      se = se2;
-      sb = sb2;
     }
-    __syncthreads();
-    for (int idx = tid; idx < n * CW; idx += NT) {
-      const int r = idx / CW, c = idx - r * CW;
-      if (c < cw) Y[(int64_t)r * ld + c0 + c] = chunk[r * CS + c];
-    }
-    __syncthreads();
   }
 }
 

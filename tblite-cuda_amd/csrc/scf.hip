@@ -783,9 +783,13 @@ __device__ __noinline__ void big_eigenvectors(const double* A_, const double* X_
   big::orthogonalize_clusters<NT>(ev, n, nvec, tnorm, Y, ld, runs, scal);
   clk.lap(TA_PROF_CLUSTERS);
   if (two_stage) {
-    // Z = Q1 Q2 Y: bulge-chasing reflectors on column chunks in LDS (the
-    // eigensolver region is free here), then the panel reflectors
-    band::apply_q2<NT>(scr + band_v2_offset(ld), n, ld, nvec, Y, er, eig_region);
+    // Z = Q1 Q2 Y: the bulge-chasing reflectors in blocks (operands formed
+    // first, the eigensolver region of LDS is free here), then the panel
+    // reflectors
+    band::q2_operands<NT>(scr + band_v2_offset(ld), n, scr + band_q2op_offset(ld), er);
+    __syncthreads();
+    band::apply_q2<NT>(scr + band_q2op_offset(ld), n, ld, nvec, Y);
+    __syncthreads();
     clk.lap(TA_PROF_Q2);
     band::back_transform_q1<NT>(scr + band_v1_offset(ld), scr + band_t1_offset(ld), n, ld, nvec,
                                 Y, er, eig_region);

@@ -536,6 +536,11 @@ __device__ __forceinline__ double start_vector(int i, int k)
 // Inverse iteration: column i of Y (Y[k * ld + i]) receives the normalised
 // eigenvector of T for lam[i].  scr: 3 * ld * NT doubles of global scratch
 // (pivots^-1, first and second superdiagonals of U, slot-fastest).
+//
+// Each sweep is a dependency chain along k whose loads do not depend on the
+// chain, but follow the stores of earlier steps (which the compiler must
+// assume may alias them); they are issued IB steps at a time ahead of those
+// steps' stores, so that one memory latency is paid per IB steps.
 template <int NT>
 __device__ __noinline__ void inverse_iteration(const lds_dbl* d, const lds_dbl* e,
                                               const lds_dbl* lam, int n, int nvec, double tnorm,
@@ -543,6 +548,11 @@ __device__ __noinline__ void inverse_iteration(const lds_dbl* d, const lds_dbl* 
 {
   const double pert = EPS * tnorm + 1e-300;
   constexpr int NIT = 3;
+  constexpr int IB = 8;
+  Y = sgpr_ptr(Y);
+  scr = sgpr_ptr(scr);
+  n = sgpr_int(n);
+  ld = sgpr_int(ld);
   for (int i0 = 0; i0 < nvec; i0 += NT) {
     const int i = i0 + threadIdx.x;
     if (i < nvec) {
@@ -555,53 +565,89 @@ __device__ __noinline__ void inverse_iteration(const lds_dbl* d, const lds_dbl* 
       for (int it = 0; it < NIT; ++it) {
         // forward elimination with partial pivoting, rhs transformed in place
         double p = d[0] - l, q = n > 1 ? e[0] : 0.0;
-        double bp = it == 0 ? start_vector(i, 0) : Y[i] * scale;
-        for (int k = 0; k < n - 1; ++k) {
-          const double sk = e[k], dk1 = d[k + 1] - l, ek1 = (k + 1 < n - 1) ? e[k + 1] : 0.0;
-          const double bk1 =
-              it == 0 ? start_vector(i, k + 1) : Y[(int64_t)(k + 1) * ld + i] * scale;
-          double U0, U1, U2, rk;
-          if (fabs(p) >= fabs(sk)) {
-            if (fabs(p) < pert) p = p < 0.0 ? -pert : pert;
-            const double m = sk / p;
-            U0 = p;
-            U1 = q;
-            U2 = 0.0;
-            rk = bp;
-            p = dk1 - m * q;
-            q = ek1;
-            bp = bk1 - m * bp;
-          } else {
-            const double m = p / sk;
-            U0 = sk;
-            U1 = dk1;
-            U2 = ek1;
-            rk = bk1;
-            p = q - m * dk1;
-            q = -m * ek1;
-            bp = bp - m * bk1;
+        double bp = it == 0 ? start_vector(i, 0) : gld(Y, i) * scale;
+        for (int k0 = 0; k0 < n - 1; k0 += IB) {
+          double bq[IB];
+#pragma unroll
+          for (int u = 0; u < IB; ++u) {
+            const int k = k0 + u;
+            const unsigned kc = k + 1 < n ? k + 1 : n - 1;
+            bq[u] = it == 0 ? start_vector(i, k + 1) : gld(Y, kc * ld + i) * scale;
           }
-          u0[(int64_t)k * NT + s] = 1.0 / U0;
-          u1[(int64_t)k * NT + s] = U1;
-          u2[(int64_t)k * NT + s] = U2;
-          Y[(int64_t)k * ld + i] = rk;
+#pragma unroll
+          for (int u = 0; u < IB; ++u) {
+            const int k = k0 + u;
+            if (k < n - 1) {
+              const double sk = e[k], dk1 = d[k + 1] - l, ek1 = (k + 1 < n - 1) ? e[k + 1] : 0.0;
+              const double bk1 = bq[u];
+              double U0, U1, U2, rk;
+              if (fabs(p) >= fabs(sk)) {
+                if (fabs(p) < pert) p = p < 0.0 ? -pert : pert;
+                const double m = sk / p;
+                U0 = p;
+                U1 = q;
+                U2 = 0.0;
+                rk = bp;
+                p = dk1 - m * q;
+                q = ek1;
+                bp = bk1 - m * bp;
+              } else {
+                const double m = p / sk;
+                U0 = sk;
+                U1 = dk1;
+                U2 = ek1;
+                rk = bk1;
+                p = q - m * dk1;
+                q = -m * ek1;
+                bp = bp - m * bk1;
+              }
+              gst(u0, k * NT + s, 1.0 / U0);
+              gst(u1, k * NT + s, U1);
+              gst(u2, k * NT + s, U2);
+              gst(Y, k * ld + i, rk);
+            }
+          }
         }
         if (fabs(p) < pert) p = p < 0.0 ? -pert : pert;
         // back substitution
         double x2 = 0.0, x1 = bp / p, nrm = x1 * x1;
-        Y[(int64_t)(n - 1) * ld + i] = x1;
-        for (int k = n - 2; k >= 0; --k) {
-          const double x = (Y[(int64_t)k * ld + i] - u1[(int64_t)k * NT + s] * x1 -
-                            u2[(int64_t)k * NT + s] * x2) *
-                           u0[(int64_t)k * NT + s];
-          Y[(int64_t)k * ld + i] = x;
-          nrm += x * x;
-          x2 = x1;
-          x1 = x;
+        gst(Y, (n - 1) * ld + i, x1);
+        for (int k1 = n - 2; k1 >= 0; k1 -= IB) {
+          double yq[IB], a0[IB], a1[IB], a2[IB];
+#pragma unroll
+          for (int u = 0; u < IB; ++u) {
+            const int k = k1 - u;
+            const unsigned kc = k >= 0 ? k : 0;
+            yq[u] = gld(Y, kc * ld + i);
+            a0[u] = gld(u0, kc * NT + s);
+            a1[u] = gld(u1, kc * NT + s);
+            a2[u] = gld(u2, kc * NT + s);
+          }
+#pragma unroll
+          for (int u = 0; u < IB; ++u) {
+            const int k = k1 - u;
+            if (k >= 0) {
+              const double x = (yq[u] - a1[u] * x1 - a2[u] * x2) * a0[u];
+              gst(Y, k * ld + i, x);
+              nrm += x * x;
+              x2 = x1;
+              x1 = x;
+            }
+          }
         }
         scale = 1.0 / sqrt(nrm);
       }
-      for (int k = 0; k < n; ++k) Y[(int64_t)k * ld + i] *= scale;
+      for (int k0 = 0; k0 < n; k0 += IB) {
+        double yq[IB];
+#pragma unroll
+        for (int u = 0; u < IB; ++u) {
+          const unsigned kc = k0 + u < n ? k0 + u : n - 1;
+          yq[u] = gld(Y, kc * ld + i);
+        }
+#pragma unroll
+        for (int u = 0; u < IB; ++u)
+          if (k0 + u < n) gst(Y, (k0 + u) * ld + i, yq[u] * scale);
+      }
     }
   }
   __syncthreads();

@@ -368,6 +368,123 @@ __device__ bool orthonormalizer(double* A, double* C, int n, int ld, double* col
   return true;
 }
 
+// ------------------------------------------------ blocked orthonormalizer
+// Global-memory path: S (in A, row-major, lower triangle read) = L L^T with
+// the blocks of L below the diagonal left in A, and C = L^-1 (row-major, zero
+// above the diagonal), in panels of 64 columns:
+//   diagonal block  Cholesky in LDS (one barrier per column: the block's
+//                   trailing triangle takes a[i][c] -= a[i][j] a[c][j] / d_j,
+//                   then L = (unit lower part) D^1/2), then its inverse in
+//                   place, columns last to first (x[j][j] = 1 / l[j][j],
+//                   x[i][j] = -x[j][j] sum_{k=j+1}^{i} x[i][k] l[k][j], eight
+//                   lanes per row), into the diagonal block of C;
+//   below           L21 = A21 X11^T, then the trailing update
+//                   A22 -= L21 L21^T (lower tiles), on the matrix cores.
+// Then the blocks of L^-1 below the diagonal, subdiagonal d = 1, 2, .. at a
+// time: X_IJ = -X_II (sum_{K=J}^{I-1} L_IK X_KJ), I = J + d (the sum into
+// C's block first), on the matrix cores.  The GEMM tiles are the 64 x 64
+// blocks.  Returns false if S is not positive definite.  lds: 4224 doubles
+// (diagonal block and its pivots; the products' tile overlays them).
+template <int NT>
+__device__ __noinline__ bool orthonormalizer_blocked(gbl_dbl* A, gbl_dbl* C, int n, int ld,
+                                                     lds_dbl* lds)
+{
+  constexpr int NB = 64, TS = 65;
+  const int tid = threadIdx.x;
+  const int nb = (n + NB - 1) / NB;
+  lds_dbl* T = lds;
+  lds_dbl* dg = lds + NB * TS;
+  lds_dbl* tile = lds;
+  auto no_skip = [](int, int) { return false; };
+  for (int idx = tid; idx < ld * ld; idx += NT) C[idx] = 0.0;
+  for (int p = 0; p < nb; ++p) {
+    const int c0 = NB * p, w = min(NB, n - c0), r0 = c0 + w, m = n - r0;
+    __syncthreads();  // the previous panel's products are done with the tile
+    for (int idx = tid; idx < NB * NB; idx += NT) {
+      const int i = idx >> 6, j = idx & 63;
+      T[i * TS + j] = (i < w && j <= i) ? A[(int64_t)(c0 + i) * ld + c0 + j] : 0.0;
+    }
+    __syncthreads();
+    for (int j = 0; j < w; ++j) {
+      const double dj = T[j * TS + j];
+      if (!(dj > 0.0)) return false;  // the same value in every thread
+      const double rdj = 1.0 / dj;
+      for (int idx = tid; idx < (w - j - 1) * NB; idx += NT) {
+        const int i = j + 1 + (idx >> 6), c = idx & 63;
+        if (c > j && c <= i) T[i * TS + c] -= T[i * TS + j] * T[c * TS + j] * rdj;
+      }
+      __syncthreads();
+    }
+    if (tid < w) dg[tid] = sqrt(T[tid * TS + tid]);
+    __syncthreads();
+    for (int idx = tid; idx < NB * NB; idx += NT) {
+      const int i = idx >> 6, j = idx & 63;
+      if (i < w && j < i) T[i * TS + j] /= dg[j];
+      else if (i < w && j == i) T[i * TS + i] = dg[i];
+    }
+    __syncthreads();
+    {
+      const int g = tid >> 3, q = tid & 7;
+      for (int j = w - 1; j >= 0; --j) {
+        const int i = j + 1 + g;
+        double s = 0.0;
+        if (i < w)
+          for (int k = j + 1 + q; k <= i; k += 8) s += T[i * TS + k] * T[k * TS + j];
+        s += dpp_f64<0xB1>(s);
+        s += dpp_f64<0x4E>(s);
+        s += dpp_f64<0x141>(s);
+        const double xjj = 1.0 / T[j * TS + j];
+        __syncthreads();  // column j (still L) read by every row
+        if (q == 0 && i < w) T[i * TS + j] = -xjj * s;
+        if (tid == 0) T[j * TS + j] = xjj;
+        __syncthreads();
+      }
+    }
+    for (int idx = tid; idx < NB * NB; idx += NT) {
+      const int i = idx >> 6, j = idx & 63;
+      if (i < w && j <= i) C[(int64_t)(c0 + i) * ld + c0 + j] = T[i * TS + j];
+    }
+    __syncthreads();
+    if (m <= 0) continue;
+    auto kpanel = [w](int, int, int* ka, int* kb) {
+      *ka = 0;
+      *kb = w;
+    };
+    // L21 = A21 X11^T in place (a tile reads all of its rows before it stores)
+    big::gemm<NT, true, false>(
+        m, w, [=](int i, int k) { return A[(int64_t)(r0 + i) * ld + c0 + k]; },
+        [=](int k, int j) { return C[(int64_t)(c0 + j) * ld + c0 + k]; }, kpanel, no_skip,
+        [=](int i, int j, double v) { A[(int64_t)(r0 + i) * ld + c0 + j] = v; }, tile);
+    // A22 -= L21 L21^T (tiles on and below the diagonal)
+    big::gemm<NT, true, false>(
+        m, m, [=](int i, int k) { return A[(int64_t)(r0 + i) * ld + c0 + k]; },
+        [=](int k, int j) { return A[(int64_t)(r0 + j) * ld + c0 + k]; }, kpanel,
+        [](int ti, int tj) { return tj > ti; },
+        [=](int i, int j, double v) { A[(int64_t)(r0 + i) * ld + r0 + j] -= v; }, tile);
+  }
+  for (int d = 1; d < nb; ++d) {
+    auto sub = [d](int ti, int tj) { return ti - tj != d; };
+    big::gemm<NT, true, true>(
+        n, n, [=](int i, int k) { return A[(int64_t)i * ld + k]; },
+        [=](int k, int j) { return C[(int64_t)k * ld + j]; },
+        [](int ti, int tj, int* ka, int* kb) {
+          *ka = NB * tj;
+          *kb = NB * ti;
+        },
+        sub, [=](int i, int j, double v) { C[(int64_t)i * ld + j] = v; }, tile);
+    big::gemm<NT, true, true>(
+        n, n, [=](int i, int k) { return C[(int64_t)i * ld + k]; },
+        [=](int k, int j) { return C[(int64_t)k * ld + j]; },
+        [n](int ti, int, int* ka, int* kb) {
+          *ka = NB * ti;
+          *kb = min(NB * ti + NB, n);
+        },
+        sub, [=](int i, int j, double v) { C[(int64_t)i * ld + j] = -v; }, tile);
+  }
+  __syncthreads();
+  return true;
+}
+
 // ---------------------------------------------------------------- Jacobi
 // Cyclic parallel Jacobi with round-robin pairing on the symmetric matrix A
 // (upper triangle kept during the sweeps), rotations accumulated into the
@@ -1059,12 +1176,17 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
 #pragma unroll
       for (int u = 0; u < SB; ++u) {
         const int idx = base + u * NT;
-        if (idx < ld * ld) A[sk(idx / ld, idx - (idx / ld) * ld, ld)] = sv[u];
+        if (idx < ld * ld) A[aix(idx / ld, idx - (idx / ld) * ld)] = sv[u];
       }
     }
   }
   __syncthreads();
-  if (!orthonormalizer<NT, MATS_LDS>(A, C, n, ld, work)) status |= TA_STATUS_CHOLESKY;
+  bool chol_ok;
+  if constexpr (MATS_LDS)
+    chol_ok = orthonormalizer<NT, MATS_LDS>(A, C, n, ld, work);
+  else
+    chol_ok = orthonormalizer_blocked<NT>(TA_GBL(A), TA_GBL(C), n, ld, TA_LDS(work));
+  if (!chol_ok) status |= TA_STATUS_CHOLESKY;
   clk.lap(TA_PROF_INIT);
 
   if (tid == 0) red[24] = 0.0;  // bisection warm-start bracket (global-memory path)

@@ -1437,9 +1437,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
     // A1[i][j] = (H_ij - S_ij (v_i + v_j)/2)/2 - (D(.,j,i).vdp(at i) + Q(.,j,i).vqp(at i))/2
     // LDS path: four rows per wave (16-lane segments, lanes over j), the row's
     // potentials read once per row, the integral loads of CH column steps
-    // issued together (as in the Mulliken loop).  Global-memory path: four
-    // elements per thread and step, their eleven integral loads each issued
-    // before any store.
+    // issued together (as in the Mulliken loop), then F = A1 + A1^T.
+    // Global-memory path: F directly from the lower triangle (below).
     if constexpr (MATS_LDS) {
       constexpr int JS = NT >= 512 ? 6 : (NT >= 256 ? 4 : 2), CH = NT >= 512 ? 3 : 2;
       for (int i0 = 4 * wid; i0 < ld; i0 += 4 * NW) {
@@ -1475,50 +1474,100 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
         }
       }
     } else {
-      constexpr int FB = 4;
-      const int tot = ld * ld;
-      for (int base = tid; base < tot; base += FB * NT) {
-        double fv[FB];
+      // Global-memory path: the lower triangle only.  The integral kernel
+      // stores the dipole and quadrupole integrals of an atom pair on the
+      // row atom's center below the diagonal (the half neighbour list has
+      // jat <= iat) and writes the element above the diagonal as the same
+      // operator moved to the other center (r - R_j = (r - R_i) + vec); that
+      // element is formed here from the one below in the same way, so each
+      // integral is read once and F_ij = f(i,j) + f(j,i) is written to both
+      // triangles (f(i,i) twice on the diagonal).  Two 64-column chunks of a
+      // row per wave and step.
+      const gbl_dbl* gH = TA_GBL(v.H);
+      const gbl_dbl* gS = TA_GBL(v.S);
+      const gbl_dbl* gD = TA_GBL(v.D);
+      const gbl_dbl* gQ = TA_GBL(v.Q);
+      const double* xyz = v.xyz;
+      auto felem = [&](int i, int j, int ai, const double* xi, double vi, const double* vdi,
+                       const double* vqi) {
+        const int64_t e = (int64_t)i * n + j;
+        const double sv = gS[e], hv = gH[e];
+        double di[3], qi[6];
 #pragma unroll
-        for (int u = 0; u < FB; ++u) {
-          const int idx = base + u * NT;
-          const int i = idx / ld, j = idx - (idx / ld) * ld;
-          double f = 0.0;
-          if (idx < tot && i < n && j < n) {
-            const int64_t e = (int64_t)i * n + j;
-            const int ai = v.ao_at[i];
-            const double* vd = vdp + 3 * ai;
-            const double* vq = vqp + 6 * ai;
-            f = 0.5 * (v.H[e] - v.S[e] * 0.5 * (vao[i] + vao[j]));
-            f -= 0.5 * (v.D[e] * vd[0] + v.D[n2 + e] * vd[1] + v.D[2 * n2 + e] * vd[2]);
-            f -= 0.5 * (v.Q[e] * vq[0] + v.Q[n2 + e] * vq[1] + v.Q[2 * n2 + e] * vq[2] +
-                        v.Q[3 * n2 + e] * vq[3] + v.Q[4 * n2 + e] * vq[4] +
-                        v.Q[5 * n2 + e] * vq[5]);
+        for (int k = 0; k < 3; ++k) di[k] = gD[k * n2 + e];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) qi[k] = gQ[k * n2 + e];
+        const double base = 0.5 * (hv - sv * 0.5 * (vi + vao[j]));
+        const double fij = base - 0.5 * (di[0] * vdi[0] + di[1] * vdi[1] + di[2] * vdi[2]) -
+                           0.5 * (qi[0] * vqi[0] + qi[1] * vqi[1] + qi[2] * vqi[2] +
+                                  qi[3] * vqi[3] + qi[4] * vqi[4] + qi[5] * vqi[5]);
+        if (j == i) return 2.0 * fij;
+        const int aj = v.ao_at[j];
+        const double* vdj = vdp + 3 * aj;
+        const double* vqj = vqp + 6 * aj;
+        double vec[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) vec[k] = xi[k] - xyz[3 * aj + k];
+        double dj[3], qj[6];
+        dj[0] = di[0] + vec[0] * sv;
+        dj[1] = di[1] + vec[1] * sv;
+        dj[2] = di[2] + vec[2] * sv;
+        const double t0 = 2 * vec[0] * di[0] + vec[0] * vec[0] * sv;
+        const double t2 = 2 * vec[1] * di[1] + vec[1] * vec[1] * sv;
+        const double t5 = 2 * vec[2] * di[2] + vec[2] * vec[2] * sv;
+        const double t1 = vec[0] * di[1] + vec[1] * di[0] + vec[0] * vec[1] * sv;
+        const double t3 = vec[0] * di[2] + vec[2] * di[0] + vec[0] * vec[2] * sv;
+        const double t4 = vec[1] * di[2] + vec[2] * di[1] + vec[1] * vec[2] * sv;
+        const double tr = 0.5 * (t0 + t2 + t5);
+        qj[0] = qi[0] + 1.5 * t0 - tr;
+        qj[1] = qi[1] + 1.5 * t1;
+        qj[2] = qi[2] + 1.5 * t2 - tr;
+        qj[3] = qi[3] + 1.5 * t3;
+        qj[4] = qi[4] + 1.5 * t4;
+        qj[5] = qi[5] + 1.5 * t5 - tr;
+        const double fji = base - 0.5 * (dj[0] * vdj[0] + dj[1] * vdj[1] + dj[2] * vdj[2]) -
+                           0.5 * (qj[0] * vqj[0] + qj[1] * vqj[1] + qj[2] * vqj[2] +
+                                  qj[3] * vqj[3] + qj[4] * vqj[4] + qj[5] * vqj[5]);
+        return fij + fji;
+      };
+      for (int i = wid; i < n; i += NW) {
+        const int ai = v.ao_at[i];
+        const double xi[3] = {xyz[3 * ai], xyz[3 * ai + 1], xyz[3 * ai + 2]};
+        const double vi = vao[i];
+        const double vdi[3] = {vdp[3 * ai], vdp[3 * ai + 1], vdp[3 * ai + 2]};
+        const double vqi[6] = {vqp[6 * ai],     vqp[6 * ai + 1], vqp[6 * ai + 2],
+                               vqp[6 * ai + 3], vqp[6 * ai + 4], vqp[6 * ai + 5]};
+        for (int j0 = 0; j0 <= i; j0 += 128) {
+          const int ja = j0 + lane, jb = ja + 64;
+          const bool ina = ja <= i, inb = jb <= i;
+          const double fa = felem(i, ina ? ja : i, ai, xi, vi, vdi, vqi);
+          const double fb = felem(i, inb ? jb : i, ai, xi, vi, vdi, vqi);
+          if (ina) {
+            A[(int64_t)i * ld + ja] = fa;
+            A[(int64_t)ja * ld + i] = fa;
           }
-          fv[u] = f;
-        }
-#pragma unroll
-        for (int u = 0; u < FB; ++u) {
-          const int idx = base + u * NT;
-          if (idx < tot) {
-            const int i = idx / ld, j = idx - (idx / ld) * ld;
-            A[aix(i, j)] = fv[u];
+          if (inb) {
+            A[(int64_t)i * ld + jb] = fb;
+            A[(int64_t)jb * ld + i] = fb;
           }
         }
       }
+      __syncthreads();
     }
-    __syncthreads();
-    for (int idx = tid; idx < ld * ld; idx += NT) {
-      const int i = idx / ld, j = idx - (idx / ld) * ld;
-      if (i < j) {
-        const double t = A[aix(i, j)] + A[aix(j, i)];
-        A[aix(i, j)] = t;
-        A[aix(j, i)] = t;
-      } else if (i == j) {
-        A[aix(i, i)] *= 2.0;
+    if constexpr (MATS_LDS) {
+      __syncthreads();
+      for (int idx = tid; idx < ld * ld; idx += NT) {
+        const int i = idx / ld, j = idx - (idx / ld) * ld;
+        if (i < j) {
+          const double t = A[aix(i, j)] + A[aix(j, i)];
+          A[aix(i, j)] = t;
+          A[aix(j, i)] = t;
+        } else if (i == j) {
+          A[aix(i, i)] *= 2.0;
+        }
       }
+      __syncthreads();
     }
-    __syncthreads();
 
     clk.lap(TA_PROF_FOCK);
     // ------------------------------------------------ eigensolve

@@ -43,8 +43,10 @@ constexpr double ORTOL = 1.0e-5;  // relative level gap below which vectors are 
 // The product runs on the FP64 matrix cores (v_mfma_f64_16x16x4f64: lane l
 // supplies a[l%16][l/16] and b[l/16][l%16] and holds d[4i + l/16][l%16] of
 // the 16 x 16 result).  Each of the 8 waves owns a 16 x 32 strip of the tile;
-// 16-deep k slices are staged through LDS, the next slice's global loads are
-// issued before the current slice is multiplied.
+// 16-deep k slices are staged through LDS.  The global loads run GQ slices
+// ahead of the multiplication (four registers per thread and slice), so that
+// the memory latency of a slice is covered by the products of the slices
+// before it rather than by one.
 typedef double ta_double4 __attribute__((ext_vector_type(4)));
 
 // One 64 x 64 output tile (rows i0.., columns j0..) over k in [k0, k1): the
@@ -72,35 +74,44 @@ __device__ __noinline__ void gemm_tile(int M, int N, int i0, int j0, int k0, int
     bk[r] = BJ ? idx >> 6 : idx & 15;
   }
   ta_double4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
-  double ra[2], rb[2];
-  auto fetch = [&](int kb) {
+  constexpr int GQ = 4;  // slices in flight
+  double ra[GQ][2], rb[GQ][2];
+  auto fetch = [&](int q, int kb) {
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
       const int gi = i0 + ai[r], gk = kb + ak[r];
-      ra[r] = (gi < M && gk < k1) ? la(gi, gk) : 0.0;
+      ra[q][r] = (gi < M && gk < k1) ? la(gi, gk) : 0.0;
       const int gj = j0 + bjj[r], gk2 = kb + bk[r];
-      rb[r] = (gj < N && gk2 < k1) ? lb(gk2, gj) : 0.0;
+      rb[q][r] = (gj < N && gk2 < k1) ? lb(gk2, gj) : 0.0;
     }
   };
-  if (k0 < k1) fetch(k0);
-  for (int kb = k0; kb < k1; kb += 16) {
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
-      as[ak[r] * 64 + ai[r]] = ra[r];
-      bs[bk[r] * 64 + bjj[r]] = rb[r];
-    }
-    __syncthreads();
-    if (kb + 16 < k1) fetch(kb + 16);
+  for (int q = 0; q < GQ; ++q)
+    if (k0 + 16 * q < k1) fetch(q, k0 + 16 * q);
+  for (int kq = k0; kq < k1; kq += 16 * GQ) {
 #pragma unroll
-    for (int kk = 0; kk < 4; ++kk) {
-      const int kr_ = (4 * kk + lk) * 64;
-      const double a = as[kr_ + 16 * bi + lr];
-      const double b0 = bs[kr_ + 16 * bj + lr];
-      const double b1 = bs[kr_ + 16 * bj + 16 + lr];
-      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, acc1, 0, 0, 0);
+    for (int q = 0; q < GQ; ++q) {
+      const int kb = kq + 16 * q;
+      if (kb < k1) {
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+          as[ak[r] * 64 + ai[r]] = ra[q][r];
+          bs[bk[r] * 64 + bjj[r]] = rb[q][r];
+        }
+        __syncthreads();
+        if (kb + 16 * GQ < k1) fetch(q, kb + 16 * GQ);
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+          const int kr_ = (4 * kk + lk) * 64;
+          const double a = as[kr_ + 16 * bi + lr];
+          const double b0 = bs[kr_ + 16 * bj + lr];
+          const double b1 = bs[kr_ + 16 * bj + 16 + lr];
+          acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, acc0, 0, 0, 0);
+          acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, acc1, 0, 0, 0);
+        }
+        __syncthreads();
+      }
     }
-    __syncthreads();
   }
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
@@ -124,6 +135,77 @@ __device__ __noinline__ void gemm(int M, int N, LA la, LB lb, KR kr, SK skip, ST
       kr(ti, tj, &k0, &k1);
       gemm_tile<NT, AK, BJ>(M, N, ti * 64, tj * 64, k0, k1, la, lb, st, tile);
     }
+  __syncthreads();
+}
+
+// out(i, j) = sum_{k in [k0, k1)} a(i, k) b(k, j) over 32 x 32 output blocks
+// without LDS or barriers: each wave takes blocks w, w + NW, .. (skip(bi,
+// bj) drops a block, kr(bi, bj, &k0, &k1) gives its k range) and holds the
+// block as four 16 x 16 accumulators on the FP64 matrix cores, its operands
+// read straight from memory in the operand layout (lane l supplies
+// a[i0 + l%16][k + l/16] and b[k + l/16][j0 + l%16] per k-step of 4) and the
+// next 16 k loaded while the current 16 are multiplied.  Out-of-range indices
+// are clamped for the loads (the loaders see only i < M, j < N and k in
+// [k0, k1)); k past k1 contributes zero.  st(i, j, value) for i < M, j < N.
+template <int NT, class LA, class LB, class KR, class SK, class ST>
+__device__ __noinline__ void gemm_reg(int M, int N, LA la, LB lb, KR kr, SK skip, ST st)
+{
+  constexpr int NW = NT / 64;
+  const int lane = threadIdx.x & 63, w = wave_id();
+  const int lr = lane & 15, lk = lane >> 4;
+  const int nbi = (M + 31) >> 5, nbj = (N + 31) >> 5;
+  for (int b = w; b < nbi * nbj; b += NW) {
+    const int bi = b / nbj, bj = b - bi * nbj;
+    if (skip(bi, bj)) continue;
+    int k0, k1;
+    kr(bi, bj, &k0, &k1);
+    const int i0 = 32 * bi, j0 = 32 * bj;
+    const int ia = min(i0 + lr, M - 1), ib = min(i0 + 16 + lr, M - 1);
+    const int ja = min(j0 + lr, N - 1), jb = min(j0 + 16 + lr, N - 1);
+    ta_double4 c00 = {0.0, 0.0, 0.0, 0.0}, c01 = c00, c10 = c00, c11 = c00;
+    double pa[4], pb[4], qa[4], qb[4];
+    auto fetch = [&](int k) {
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int kk = k + 4 * s + lk;
+        const bool in = kk < k1;
+        const int kc = in ? kk : k1 - 1;
+        const double x0 = la(ia, kc), x1 = la(ib, kc);
+        pa[s] = in ? x0 : 0.0;
+        pb[s] = in ? x1 : 0.0;
+        qa[s] = lb(kc, ja);
+        qb[s] = lb(kc, jb);
+      }
+    };
+    if (k0 < k1) fetch(k0);
+    for (int k = k0; k < k1; k += 16) {
+      double ca[4], cb[4], da[4], db[4];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        ca[s] = pa[s];
+        cb[s] = pb[s];
+        da[s] = qa[s];
+        db[s] = qb[s];
+      }
+      if (k + 16 < k1) fetch(k + 16);
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        c00 = __builtin_amdgcn_mfma_f64_16x16x4f64(ca[s], da[s], c00, 0, 0, 0);
+        c01 = __builtin_amdgcn_mfma_f64_16x16x4f64(ca[s], db[s], c01, 0, 0, 0);
+        c10 = __builtin_amdgcn_mfma_f64_16x16x4f64(cb[s], da[s], c10, 0, 0, 0);
+        c11 = __builtin_amdgcn_mfma_f64_16x16x4f64(cb[s], db[s], c11, 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int r0 = i0 + 4 * q + lk, r1 = r0 + 16;
+      const int s0 = j0 + lr, s1 = s0 + 16;
+      if (r0 < M && s0 < N) st(r0, s0, c00[q]);
+      if (r0 < M && s1 < N) st(r0, s1, c01[q]);
+      if (r1 < M && s0 < N) st(r1, s0, c10[q]);
+      if (r1 < M && s1 < N) st(r1, s1, c11[q]);
+    }
+  }
   __syncthreads();
 }
 

@@ -938,8 +938,7 @@ __device__ __noinline__ void big_density(const double* CT_, const double* f_, co
   gbl_dbl* P = TA_GBL(P_);
   const lds_dbl* f = TA_LDS(f_);
   const lds_int* occ = TA_LDSI(occ_);
-  lds_dbl* tmp = TA_LDS(er_) + 9 * ld;
-  big::gemm<NT, false, true>(
+  big::gemm_reg<NT>(
       n, n,
       [=](int i, int t) {
         const int k = occ[t];
@@ -950,14 +949,13 @@ __device__ __noinline__ void big_density(const double* CT_, const double* f_, co
         *k0 = 0;
         *k1 = nocc;
       },
-      [](int ti, int tj) { return tj > ti; },
+      [](int bi, int bj) { return bj > bi; },
       [=](int i, int j, double val) {
         if (i >= j) {
           P[(int64_t)i * ld + j] = val;
           P[(int64_t)j * ld + i] = val;
         }
-      },
-      tmp);
+      });
 }
 
 // ------------------------------------------------------------- D4 phases

@@ -282,8 +282,7 @@ __device__ __noinline__ void band_panel_qr(gbl_dbl* A, int n, int ld, int c0, gb
   lds_dbl* Tm = sm;          // T of the panel, [k][c]
   lds_dbl* Gs = sm + 256;    // reduced V^T V
   lds_dbl* sc = sm + 512;    // pivot (0), tau of the panel's reflectors (1..16)
-  lds_dbl* red = tile;       // QR: partial sums of squares [8]
-  lds_dbl* red2 = tile + 8;  // QR: partial column dots [8][16]
+  lds_dbl* red2 = tile;  // QR: partial Gram rows [2][8][16], pivot rows [2][16]
   const int ms = n - c0, m = ms - BW, r2 = c0 + BW;
   gbl_dbl* Vc = V1 + (int64_t)c0 * ld;
   // (a) strip into registers with the previous panel's update
@@ -334,75 +333,64 @@ __device__ __noinline__ void band_panel_qr(gbl_dbl* A, int n, int ld, int c0, gb
   // wave-uniform; column j lives in the lanes lr == j of each 16-lane row.
   const int kref = m < BW ? m : BW;
   const int src_row = lane & 48;
+  // One barrier per column: the waves' partial Gram row g_c = sum over the
+  // rows below the pivot of x_j x_c (g_j = the squared norm below the pivot)
+  // and the pivot row go through LDS, double-buffered by column parity; the
+  // reflector v (1 at the pivot, x_j * scale below) then has
+  // v . x_c = x_c[piv] + scale g_c.  Column j's broadcast along each 16-lane
+  // row (all shuffles issued before the first use) serves both the Gram row
+  // and the update.
   for (int j = 0; j < kref; ++j) {
     const int piv = BW + j;
-    double ss = 0.0;
-#pragma unroll
-    for (int q = 0; q < QB; ++q)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int rl = 16 * (w + 8 * q) + 4 * i + lk;
-        const double xv = x[q][i];
-        ss += (lr == j && rl > piv && rl < ms) ? xv * xv : 0.0;
-      }
-    ss = wave_sum_dpp(ss);
-    if (lane == 0) red[w] = ss;
-    if (w == 1 && lr == j && lk == (j & 3)) sc[0] = pick4(x[0], j >> 2);
-    __syncthreads();
-    double sst = 0.0;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) sst += red[q];
-    const double alpha = sc[0];
-    double beta, tau, scale;
-    if (sst == 0.0) {
-      beta = alpha;
-      tau = 0.0;
-      scale = 0.0;
-    } else {
-      beta = -copysign(sqrt(alpha * alpha + sst), alpha);
-      tau = (beta - alpha) / beta;
-      scale = 1.0 / (alpha - beta);
-    }
-    // column j <- (beta at the pivot, reflector below); partial dots v . x_c
-    // (all shuffles issued before the first use)
+    lds_dbl* R = red2 + (j & 1) * 8 * 16;
+    lds_dbl* P = red2 + 2 * 8 * 16 + (j & 1) * 16;
     double xs[QB][4];
 #pragma unroll
     for (int q = 0; q < QB; ++q)
 #pragma unroll
       for (int i = 0; i < 4; ++i) xs[q][i] = __shfl(x[q][i], src_row | j, 64);
-    double pd = 0.0;
+    double g = 0.0;
 #pragma unroll
     for (int q = 0; q < QB; ++q)
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int rl = 16 * (w + 8 * q) + 4 * i + lk;
-        const double xj = xs[q][i];
-        const bool below = rl > piv && rl < ms;
-        const double v = rl == piv ? 1.0 : (below ? xj * scale : 0.0);
-        const double xo = x[q][i];
-        pd += v * xo;
-        x[q][i] = lr == j ? (rl == piv ? beta : (below ? v : xo)) : xo;
+        g += (rl > piv && rl < ms) ? xs[q][i] * x[q][i] : 0.0;
       }
-    pd = xor32_sum(xor16_sum(pd));
-    if (lk == 0) red2[w * 16 + lr] = pd;
-    if (tid == 0) sc[1 + j] = tau;
+    g = xor32_sum(xor16_sum(g));
+    if (lk == 0) R[w * 16 + lr] = g;
+    if (w == 1 && lk == (j & 3)) P[lr] = pick4(x[0], j >> 2);
     __syncthreads();
-    double pc = 0.0;
+    double gc = 0.0, gj = 0.0;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) pc += red2[q * 16 + lr];
-    const double f = lr > j ? tau * pc : 0.0;
-    // the reflector again, now stored in column j
-#pragma unroll
-    for (int q = 0; q < QB; ++q)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) xs[q][i] = __shfl(x[q][i], src_row | j, 64);
+    for (int q = 0; q < 8; ++q) {
+      gc += R[q * 16 + lr];
+      gj += R[q * 16 + j];
+    }
+    const double alpha = P[j];
+    double beta, tau, scale;
+    if (gj == 0.0) {
+      beta = alpha;
+      tau = 0.0;
+      scale = 0.0;
+    } else {
+      beta = -copysign(sqrt(alpha * alpha + gj), alpha);
+      tau = (beta - alpha) / beta;
+      scale = 1.0 / (alpha - beta);
+    }
+    const double f = lr > j ? tau * (P[lr] + scale * gc) : 0.0;
+    if (tid == 0) sc[1 + j] = tau;
+    // column j <- (beta at the pivot, reflector below); columns c > j take
+    // x_c -= f v
 #pragma unroll
     for (int q = 0; q < QB; ++q)
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int rl = 16 * (w + 8 * q) + 4 * i + lk;
-        const double v = rl == piv ? 1.0 : ((rl > piv && rl < ms) ? xs[q][i] : 0.0);
-        x[q][i] -= f * v;
+        const bool below = rl > piv && rl < ms;
+        const double v = rl == piv ? 1.0 : (below ? xs[q][i] * scale : 0.0);
+        const double xo = x[q][i];
+        x[q][i] = lr == j ? (rl == piv ? beta : (below ? v : xo)) : xo - f * v;
       }
   }
   __syncthreads();

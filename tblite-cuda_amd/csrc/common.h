@@ -345,8 +345,8 @@ __host__ __device__ inline int64_t big_scratch_doubles(int ld)
 }
 
 // largest padded AO dimension of the two-stage reduction (its panel strip
-// is held in registers: 4 blocks of 16 rows per wave)
-#define TA_BAND_MAXLD 512
+// is held in registers: up to 8 blocks of 16 rows per wave)
+#define TA_BAND_MAXLD 1024
 
 // LDS layout of the SCF kernel, in doubles (integer arrays in units of ints
 // from the start of the region).  ld is the padded AO dimension of the

@@ -48,9 +48,10 @@
 
 namespace band {
 
-constexpr int BW = 16;             // half-bandwidth after stage 1
-constexpr int QB = 4;              // 16-row strip blocks per wave
-constexpr int MAXN = 8 * QB * BW;  // largest order served (512)
+constexpr int BW = 16;  // half-bandwidth after stage 1
+// The panel strip of stage 1 is held in registers, QB blocks of 16 rows per
+// wave (8 QB BW rows): QB = 4 up to order 512, QB = 8 up to 1024.
+constexpr int band_qb(int n) { return n <= 8 * 4 * BW ? 4 : 8; }
 
 using big::ta_double4;
 
@@ -262,7 +263,7 @@ __device__ __forceinline__ void reduce_partials(ta_double4 g, lds_dbl* part, lds
 // the reflectors (V1 at row c0, tau1) and R, which goes into A with zeros in
 // the bulge region below it; T of the block reflector I - V T V^T ends in
 // sm[0..256).  tile: TILE_DOUBLES of LDS (reductions); sm: 560 doubles.
-template <int NT>
+template <int NT, int QB>
 __device__ __noinline__ void band_panel_qr(gbl_dbl* A, int n, int ld, int c0, gbl_dbl* V1,
                                            gbl_dbl* tau1, gbl_dbl* Tp, const gbl_dbl* Vp,
                                            const gbl_dbl* Wp, bool upd, lds_dbl* tile,
@@ -340,23 +341,31 @@ __device__ __noinline__ void band_panel_qr(gbl_dbl* A, int n, int ld, int c0, gb
   // v . x_c = x_c[piv] + scale g_c.  Column j's broadcast along each 16-lane
   // row (all shuffles issued before the first use) serves both the Gram row
   // and the update.
+  // The broadcast is issued QH blocks at a time; with QB > QH (the deep
+  // strip) it is issued again for the update instead of being held across
+  // the barrier, which keeps the strip and the broadcast within the
+  // register budget.
+  constexpr int QH = QB > 4 ? 4 : QB;
   for (int j = 0; j < kref; ++j) {
     const int piv = BW + j;
     lds_dbl* R = red2 + (j & 1) * 8 * 16;
     lds_dbl* P = red2 + 2 * 8 * 16 + (j & 1) * 16;
-    double xs[QB][4];
-#pragma unroll
-    for (int q = 0; q < QB; ++q)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) xs[q][i] = __shfl(x[q][i], src_row | j, 64);
+    double xs[QH][4];
     double g = 0.0;
 #pragma unroll
-    for (int q = 0; q < QB; ++q)
+    for (int h = 0; h < QB; h += QH) {
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int rl = 16 * (w + 8 * q) + 4 * i + lk;
-        g += (rl > piv && rl < ms) ? xs[q][i] * x[q][i] : 0.0;
-      }
+      for (int q = 0; q < QH; ++q)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) xs[q][i] = __shfl(x[h + q][i], src_row | j, 64);
+#pragma unroll
+      for (int q = 0; q < QH; ++q)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int rl = 16 * (w + 8 * (h + q)) + 4 * i + lk;
+          g += (rl > piv && rl < ms) ? xs[q][i] * x[h + q][i] : 0.0;
+        }
+    }
     g = xor32_sum(xor16_sum(g));
     if (lk == 0) R[w * 16 + lr] = g;
     if (w == 1 && lk == (j & 3)) P[lr] = pick4(x[0], j >> 2);
@@ -383,15 +392,24 @@ __device__ __noinline__ void band_panel_qr(gbl_dbl* A, int n, int ld, int c0, gb
     // column j <- (beta at the pivot, reflector below); columns c > j take
     // x_c -= f v
 #pragma unroll
-    for (int q = 0; q < QB; ++q)
+    for (int h = 0; h < QB; h += QH) {
+      if (QB > QH) {
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int rl = 16 * (w + 8 * q) + 4 * i + lk;
-        const bool below = rl > piv && rl < ms;
-        const double v = rl == piv ? 1.0 : (below ? xs[q][i] * scale : 0.0);
-        const double xo = x[q][i];
-        x[q][i] = lr == j ? (rl == piv ? beta : (below ? v : xo)) : xo - f * v;
+        for (int q = 0; q < QH; ++q)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) xs[q][i] = __shfl(x[h + q][i], src_row | j, 64);
       }
+#pragma unroll
+      for (int q = 0; q < QH; ++q)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int rl = 16 * (w + 8 * (h + q)) + 4 * i + lk;
+          const bool below = rl > piv && rl < ms;
+          const double v = rl == piv ? 1.0 : (below ? xs[q][i] * scale : 0.0);
+          const double xo = x[h + q][i];
+          x[h + q][i] = lr == j ? (rl == piv ? beta : (below ? v : xo)) : xo - f * v;
+        }
+    }
   }
   __syncthreads();
   lap(1);
@@ -446,7 +464,7 @@ __device__ __noinline__ void band_panel_qr(gbl_dbl* A, int n, int ld, int c0, gb
 // W = X T - V (T^T V^T X T) / 2 = X T + V M, M = -(T^T Z T) / 2, Z = V^T X,
 // for the panel at c0 (rows c0 + BW .. n-1 of V, X and W).  T in sm[0..256);
 // tile: 4096 doubles of LDS; sm[256..512) scratch.
-template <int NT>
+template <int NT, int QB>
 __device__ __noinline__ void band_panel_w(int n, int ld, int c0, const gbl_dbl* Vc,
                                           const gbl_dbl* X, gbl_dbl* Wc, lds_dbl* tile,
                                           lds_dbl* sm)
@@ -515,13 +533,13 @@ __device__ __noinline__ void band_panel_w(int n, int ld, int c0, const gbl_dbl* 
 }
 
 // ------------------------------------------------------ stage 1: driver
-// A (n x n, lower triangle valid, n <= MAXN) -> band of half-width BW in its
+// A (n x n, lower triangle valid, n <= 8 QB BW) -> band of half-width BW in its
 // lower triangle, zeros 17..31 below the diagonal.  V1, tau1: stage-1
 // reflectors (layout above); T1: the T factor of each panel (16 x 16, T[i][c]
 // at 256 p + 16 i + c); Wb: 2 x 16 x ld, X: 16 x ld doubles of global
 // scratch; tile: TILE_DOUBLES of LDS; sm: 560 doubles of LDS; tprof: optional
 // step clocks (strip, QR, store + T, trailing pass, W).
-template <int NT>
+template <int NT, int QB>
 __device__ __noinline__ void band_reduce(gbl_dbl* A, int n, int ld, gbl_dbl* V1, gbl_dbl* tau1,
                                          gbl_dbl* T1, gbl_dbl* Wb, gbl_dbl* X, lds_dbl* tile,
                                          lds_dbl* sm, lds_dbl* tprof)
@@ -544,11 +562,11 @@ __device__ __noinline__ void band_reduce(gbl_dbl* A, int n, int ld, gbl_dbl* V1,
     const gbl_dbl* Wp = Wb + (int64_t)((p + 1) & 1) * BW * ld;
     gbl_dbl* Wc = Wb + (int64_t)(p & 1) * BW * ld;
     const gbl_dbl* Vc = V1 + (int64_t)c0 * ld;
-    band_panel_qr<NT>(A, n, ld, c0, V1, tau1, T1 + BW * BW * p, Vp, Wp, upd, tile, sm, tprof);
+    band_panel_qr<NT, QB>(A, n, ld, c0, V1, tau1, T1 + BW * BW * p, Vp, Wp, upd, tile, sm, tprof);
     if (clk_on) clk_last = wall_clock64();
     band_pass<NT>(A, n, ld, r2, Vp, Wp, upd, Vc, X, true, tile);
     lap(3);
-    band_panel_w<NT>(n, ld, c0, Vc, X, Wc, tile, sm);
+    band_panel_w<NT, QB>(n, ld, c0, Vc, X, Wc, tile, sm);
     lap(4);
   }
   // the last panel's update of the remaining trailing block

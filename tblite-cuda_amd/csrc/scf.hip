@@ -772,10 +772,14 @@ __device__ __noinline__ void big_band_tridiagonalize(gbl_dbl* A, gbl_dbl* scr, i
   if (threadIdx.x == 0)
     for (int q = 0; q < 5; ++q) red[16 + q] = 0.0;
   __syncthreads();
-  band::band_reduce<NT>(A, n, ld, scr + band_v1_offset(ld), scr + band_tau_offset(ld),
-                        scr + band_t1_offset(ld), scr + band_w_offset(ld),
-                        scr + band_x_offset(ld), tile, sm,
-                        clk.on ? red + 16 : nullptr);
+  if (band::band_qb(n) == 4)
+    band::band_reduce<NT, 4>(A, n, ld, scr + band_v1_offset(ld), scr + band_tau_offset(ld),
+                             scr + band_t1_offset(ld), scr + band_w_offset(ld),
+                             scr + band_x_offset(ld), tile, sm, clk.on ? red + 16 : nullptr);
+  else
+    band::band_reduce<NT, 8>(A, n, ld, scr + band_v1_offset(ld), scr + band_tau_offset(ld),
+                             scr + band_t1_offset(ld), scr + band_w_offset(ld),
+                             scr + band_x_offset(ld), tile, sm, clk.on ? red + 16 : nullptr);
   if (clk.on) {
     for (int q = 0; q < 5; ++q) clk.acc[TA_PROF_TRI_COLUMN + q] += (long long)red[16 + q];
   }
@@ -1021,7 +1025,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
 {
   extern __shared__ double smem[];
   const int tid = threadIdx.x;
-  const int lane = tid & 63, wid = tid >> 6;
+  // the wave index as a scalar: loops over a wave's rows, shells or atoms
+  // then run under scalar control instead of per-lane exit masks
+  const int lane = tid & 63, wid = wave_id();
   constexpr int NW = NT / 64;
   const ScfLds L = scf_lds_layout(ldmax, natmax, nshmax, MATS_LDS);
   int* ismem = reinterpret_cast<int*>(smem);
@@ -1137,15 +1143,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
   const double diff = fmin(nuhf, nocc);
   const double nalp = (nocc - diff) / 2.0 + diff, nbet = (nocc - diff) / 2.0;
 
-  // repulsion total; three-body dispersion total (zero charges, k_d4_atm)
   const DevD4& d4 = P.d4;
-  double erep = 0.0, e3 = 0.0;
-  for (int i = tid; i < nat; i += NT) {
-    erep += b.erep[v.a0 + i];
-    if (d4.enabled) e3 += b.d4_e3[v.a0 + i];
-  }
-  erep = block_sum(erep, red);
-  e3 = block_sum(e3, red);
   double* d4gw = b.d4_gw + TA_D4_MAXREF * (int64_t)v.a0;
   double* d4dgw = b.d4_dgw + TA_D4_MAXREF * (int64_t)v.a0;
   const double* d4cn = b.d4_cn + v.a0;
@@ -1161,9 +1159,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
 
   // --- orthonormal start basis C = L^-T
   {
-    // six loads per thread in flight
+    // six loads per thread in flight; the trip count is the same for every
+    // thread (a loop whose threads leave it at different times had the
+    // values live across it spilled under the exit mask, i.e. not at all)
     constexpr int SB = 6;
-    for (int base = tid; base < ld * ld; base += SB * NT) {
+    for (int base0 = 0; base0 < ld * ld; base0 += SB * NT) {
+      const int base = base0 + tid;
       double sv[SB];
 #pragma unroll
       for (int u = 0; u < SB; ++u) {
@@ -1845,6 +1846,16 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
   if (!converged && status == 0) status |= TA_STATUS_NOT_CONVERGED;
 
   // ------------------------------------------------ results
+  // repulsion total; three-body dispersion total (zero charges, k_d4_atm),
+  // summed here where they are used, so that nothing holds them across the
+  // SCF
+  double erep = 0.0, e3 = 0.0;
+  for (int i = tid; i < nat; i += NT) {
+    erep += b.erep[v.a0 + i];
+    if (d4.enabled) e3 += b.d4_e3[v.a0 + i];
+  }
+  erep = block_sum(erep, red);
+  e3 = block_sum(e3, red);
   for (int r = tid; r < n; r += NT) {
     b.emo[v.o0 + r] = have_levels ? emo_s[r] : 0.0;
     b.focc[v.o0 + r] = have_levels ? focc_u[inv[r]] : 0.0;

@@ -38,47 +38,51 @@ constexpr double ORTOL = 1.0e-5;  // relative level gap below which vectors are 
 // a and b are loader functors, kr(ti, tj, &k0, &k1) restricts the k range of
 // a tile (triangular operands), skip(ti, tj) drops tiles, st(i, j, value)
 // stores.  AK: a is read k-fastest (row-major a), else i-fastest.  BJ: b is
-// read j-fastest (row-major b), else k-fastest.  tile: 2048 doubles of LDS.
+// read j-fastest (row-major b), else k-fastest.  tile: 128 KS doubles of LDS
+// (KS = 16 by default: 2048).
 //
 // The product runs on the FP64 matrix cores (v_mfma_f64_16x16x4f64: lane l
 // supplies a[l%16][l/16] and b[l/16][l%16] and holds d[4i + l/16][l%16] of
 // the 16 x 16 result).  Each of the 8 waves owns a 16 x 32 strip of the tile;
-// 16-deep k slices are staged through LDS.  The global loads run GQ slices
-// ahead of the multiplication (four registers per thread and slice), so that
-// the memory latency of a slice is covered by the products of the slices
-// before it rather than by one.
+// k slices are staged through LDS.  The global loads run GQ slices ahead of
+// the multiplication, so that the memory latency of a slice is covered by
+// the products of the slices before it rather than by one.
 typedef double ta_double4 __attribute__((ext_vector_type(4)));
 
 // One 64 x 64 output tile (rows i0.., columns j0..) over k in [k0, k1): the
 // body shared by the single-workgroup product below and the grid-wide one of
 // the multi-workgroup path.  All NT = 512 threads of the workgroup take part.
-template <int NT, bool AK, bool BJ, class LA, class LB, class ST>
+// KS: depth of the k slices staged through LDS (16 or 32; tile holds
+// 128 KS doubles), two barriers per slice.
+template <int NT, bool AK, bool BJ, class LA, class LB, class ST, int KS = 16>
 __device__ __noinline__ void gemm_tile(int M, int N, int i0, int j0, int k0, int k1, LA la,
                                           LB lb, ST st, lds_dbl* tile)
 {
   static_assert(NT == 512, "gemm tiling assumes 512 threads");
-  lds_dbl* as = tile;         // [16][64]: as[k * 64 + i]
-  lds_dbl* bs = tile + 1024;  // [16][64]: bs[k * 64 + j]
+  static_assert(KS == 16 || KS == 32, "k slices of 16 or 32");
+  constexpr int R = 64 * KS / NT;        // staged elements per thread and operand
+  constexpr int GQ = KS == 16 ? 4 : 2;   // slices in flight
+  lds_dbl* as = tile;            // [KS][64]: as[k * 64 + i]
+  lds_dbl* bs = tile + 64 * KS;  // [KS][64]: bs[k * 64 + j]
   const int tid = threadIdx.x;
   const int lane = tid & 63, w = wave_id();
   const int bi = w >> 1, bj = (w & 1) * 2;
   const int lr = lane & 15, lk = lane >> 4;
-  // staging coordinates of this thread's two elements of each operand slice
-  int ai[2], ak[2], bjj[2], bk[2];
+  // staging coordinates of this thread's elements of each operand slice
+  int ai[R], ak[R], bjj[R], bk[R];
 #pragma unroll
-  for (int r = 0; r < 2; ++r) {
+  for (int r = 0; r < R; ++r) {
     const int idx = tid + r * NT;
-    ai[r] = AK ? idx >> 4 : idx & 63;
-    ak[r] = AK ? idx & 15 : idx >> 6;
-    bjj[r] = BJ ? idx & 63 : idx >> 4;
-    bk[r] = BJ ? idx >> 6 : idx & 15;
+    ai[r] = AK ? idx / KS : idx & 63;
+    ak[r] = AK ? idx % KS : idx >> 6;
+    bjj[r] = BJ ? idx & 63 : idx / KS;
+    bk[r] = BJ ? idx >> 6 : idx % KS;
   }
   ta_double4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
-  constexpr int GQ = 4;  // slices in flight
-  double ra[GQ][2], rb[GQ][2];
+  double ra[GQ][R], rb[GQ][R];
   auto fetch = [&](int q, int kb) {
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
+    for (int r = 0; r < R; ++r) {
       const int gi = i0 + ai[r], gk = kb + ak[r];
       ra[q][r] = (gi < M && gk < k1) ? la(gi, gk) : 0.0;
       const int gj = j0 + bjj[r], gk2 = kb + bk[r];
@@ -87,21 +91,21 @@ __device__ __noinline__ void gemm_tile(int M, int N, int i0, int j0, int k0, int
   };
 #pragma unroll
   for (int q = 0; q < GQ; ++q)
-    if (k0 + 16 * q < k1) fetch(q, k0 + 16 * q);
-  for (int kq = k0; kq < k1; kq += 16 * GQ) {
+    if (k0 + KS * q < k1) fetch(q, k0 + KS * q);
+  for (int kq = k0; kq < k1; kq += KS * GQ) {
 #pragma unroll
     for (int q = 0; q < GQ; ++q) {
-      const int kb = kq + 16 * q;
+      const int kb = kq + KS * q;
       if (kb < k1) {
 #pragma unroll
-        for (int r = 0; r < 2; ++r) {
+        for (int r = 0; r < R; ++r) {
           as[ak[r] * 64 + ai[r]] = ra[q][r];
           bs[bk[r] * 64 + bjj[r]] = rb[q][r];
         }
         __syncthreads();
-        if (kb + 16 * GQ < k1) fetch(q, kb + 16 * GQ);
+        if (kb + KS * GQ < k1) fetch(q, kb + KS * GQ);
 #pragma unroll
-        for (int kk = 0; kk < 4; ++kk) {
+        for (int kk = 0; kk < KS / 4; ++kk) {
           const int kr_ = (4 * kk + lk) * 64;
           const double a = as[kr_ + 16 * bi + lr];
           const double b0 = bs[kr_ + 16 * bj + lr];
@@ -124,7 +128,7 @@ __device__ __noinline__ void gemm_tile(int M, int N, int i0, int j0, int k0, int
   }
 }
 
-template <int NT, bool AK, bool BJ, class LA, class LB, class KR, class SK, class ST>
+template <int NT, bool AK, bool BJ, int KS = 16, class LA, class LB, class KR, class SK, class ST>
 __device__ __noinline__ void gemm(int M, int N, LA la, LB lb, KR kr, SK skip, ST st, lds_dbl* tile)
 {
   const int mt = (M + 63) >> 6, nt = (N + 63) >> 6;
@@ -133,7 +137,7 @@ __device__ __noinline__ void gemm(int M, int N, LA la, LB lb, KR kr, SK skip, ST
       if (skip(ti, tj)) continue;
       int k0, k1;
       kr(ti, tj, &k0, &k1);
-      gemm_tile<NT, AK, BJ>(M, N, ti * 64, tj * 64, k0, k1, la, lb, st, tile);
+      gemm_tile<NT, AK, BJ, LA, LB, ST, KS>(M, N, ti * 64, tj * 64, k0, k1, la, lb, st, tile);
     }
   __syncthreads();
 }

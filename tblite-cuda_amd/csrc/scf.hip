@@ -451,12 +451,12 @@ __device__ __noinline__ bool orthonormalizer_blocked(gbl_dbl* A, gbl_dbl* C, int
       *kb = w;
     };
     // L21 = A21 X11^T in place (a tile reads all of its rows before it stores)
-    big::gemm<NT, true, false>(
+    big::gemm<NT, true, false, 32>(
         m, w, [=](int i, int k) { return A[(int64_t)(r0 + i) * ld + c0 + k]; },
         [=](int k, int j) { return C[(int64_t)(c0 + j) * ld + c0 + k]; }, kpanel, no_skip,
         [=](int i, int j, double v) { A[(int64_t)(r0 + i) * ld + c0 + j] = v; }, tile);
     // A22 -= L21 L21^T (tiles on and below the diagonal)
-    big::gemm<NT, true, false>(
+    big::gemm<NT, true, false, 32>(
         m, m, [=](int i, int k) { return A[(int64_t)(r0 + i) * ld + c0 + k]; },
         [=](int k, int j) { return A[(int64_t)(r0 + j) * ld + c0 + k]; }, kpanel,
         [](int ti, int tj) { return tj > ti; },
@@ -464,7 +464,7 @@ __device__ __noinline__ bool orthonormalizer_blocked(gbl_dbl* A, gbl_dbl* C, int
   }
   for (int d = 1; d < nb; ++d) {
     auto sub = [d](int ti, int tj) { return ti - tj != d; };
-    big::gemm<NT, true, true>(
+    big::gemm<NT, true, true, 32>(
         n, n, [=](int i, int k) { return A[(int64_t)i * ld + k]; },
         [=](int k, int j) { return C[(int64_t)k * ld + j]; },
         [](int ti, int tj, int* ka, int* kb) {
@@ -472,7 +472,7 @@ __device__ __noinline__ bool orthonormalizer_blocked(gbl_dbl* A, gbl_dbl* C, int
           *kb = NB * ti;
         },
         sub, [=](int i, int j, double v) { C[(int64_t)i * ld + j] = v; }, tile);
-    big::gemm<NT, true, true>(
+    big::gemm<NT, true, true, 32>(
         n, n, [=](int i, int k) { return C[(int64_t)i * ld + k]; },
         [=](int k, int j) { return C[(int64_t)k * ld + j]; },
         [n](int ti, int, int* ka, int* kb) {
@@ -829,12 +829,13 @@ __device__ __noinline__ void big_eigenvalues(double* A_, const double* X_, doubl
   };
   auto no_skip = [](int, int) { return false; };
   auto upper_tile = [](int ti, int tj) { return tj > ti; };
-  // U = X F, stored transposed in Y; A = X U^T = X F X^T (lower tiles)
-  big::gemm<NT, true, true>(
+  // U = X F, stored transposed in Y; A = X U^T = X F X^T (lower tiles); 32-deep
+  // k slices (tmp holds 4096 doubles)
+  big::gemm<NT, true, true, 32>(
       n, n, [=](int i, int k) { return X[(int64_t)i * ld + k]; },
       [=](int k, int j) { return A[(int64_t)k * ld + j]; }, k_lower, no_skip,
       [=](int i, int j, double val) { Y[(int64_t)j * ld + i] = val; }, tmp);
-  big::gemm<NT, true, true>(
+  big::gemm<NT, true, true, 32>(
       n, n, [=](int i, int k) { return X[(int64_t)i * ld + k]; },
       [=](int k, int j) { return Y[(int64_t)k * ld + j]; }, k_lower, upper_tile,
       [=](int i, int j, double val) { A[(int64_t)i * ld + j] = val; }, tmp);
@@ -920,7 +921,7 @@ __device__ __noinline__ void big_eigenvectors(const double* A_, const double* X_
   }
   clk.lap(TA_PROF_BACKTR);
   // CT = Z^T X; X[k][a] = 0 for a > k
-  big::gemm<NT, false, true>(
+  big::gemm<NT, false, true, 32>(
       nvec, n, [=](int i, int k) { return Y[(int64_t)k * ld + i]; },
       [=](int k, int j) { return X[(int64_t)k * ld + j]; },
       [n](int, int tj, int* k0, int* k1) {

@@ -464,9 +464,14 @@ static double hscale_of(const or_calc* c, int ei, int ish, int ej, int jsh)
   return zij * (1.0 + c->par.enscale * den) * c->par.kshell[lj][li];
 }
 
+/* tr / nltr: lattice translation of each neighbour-list entry for periodic
+ * cells (vec = R_i - R_j - T, generate's trans(:, itr)); NULL for molecules.
+ * For a periodic image of the atom itself (iat == jat, T != 0) the block is
+ * added once, without the mirrored copy: the list holds both T and -T. */
 static void hamiltonian(const or_calc* c, const basis_t* b, const double* xyz, const int* inl,
-                        const int* nnl, const int* nlat, double intcut, const double* se,
-                        double* S, double* D, double* Q, double* H)
+                        const int* nnl, const int* nlat, const double* tr, const int* nltr,
+                        double intcut, const double* se, double* S, double* D, double* Q,
+                        double* H)
 {
   int nao = b->nao;
   memset(S, 0, sizeof(double) * nao * nao);
@@ -481,6 +486,10 @@ static void hamiltonian(const or_calc* c, const basis_t* b, const double* xyz, c
       int ej = b->el[jat], js = b->ish_at[jat];
       double vec[3];
       for (int k = 0; k < 3; ++k) vec[k] = xyz[3 * iat + k] - xyz[3 * jat + k];
+      if (nltr) {
+        const double* t = tr + 3 * nltr[inl[iat] + img];
+        for (int k = 0; k < 3; ++k) vec[k] -= t[k];
+      }
       double r2 = vec[0] * vec[0] + vec[1] * vec[1] + vec[2] * vec[2];
       double rr = sqrt(sqrt(r2) / (c->par.elem[ej].atomic_rad + c->par.elem[ei].atomic_rad));
       for (int ish = 0; ish < b->nsh_at[iat]; ++ish) {
@@ -1659,7 +1668,7 @@ int or_hamiltonian(const or_calc* calc, int nat, const int* num, const double* x
   int* nnl = (int*)malloc(sizeof(int) * nat);
   int* nlat = (int*)malloc(sizeof(int) * cap);
   or_adjlist(nat, xyz, cutoff, inl, nnl, nlat, cap);
-  hamiltonian(calc, &b, xyz, inl, nnl, nlat, integral_cutoff(accuracy), se, S, D, Q, H);
+  hamiltonian(calc, &b, xyz, inl, nnl, nlat, NULL, NULL, integral_cutoff(accuracy), se, S, D, Q, H);
   free(cn); free(se); free(inl); free(nnl); free(nlat);
   free_basis(&b);
   return 0;
@@ -2375,7 +2384,7 @@ int or_integral_gradient(const or_calc* calc, int nat, const int* num, const dou
   int* nnl = (int*)malloc(sizeof(int) * nat);
   int* nlat = (int*)malloc(sizeof(int) * cap);
   or_adjlist(nat, xyz, cutoff, inl, nnl, nlat, cap);
-  hamiltonian(calc, &b, xyz, inl, nnl, nlat, integral_cutoff(accuracy), se, S, D, Q, H);
+  hamiltonian(calc, &b, xyz, inl, nnl, nlat, NULL, NULL, integral_cutoff(accuracy), se, S, D, Q, H);
   double acc = 0.0;
   for (int iao = 0; iao < nao; ++iao) {
     int iat = b.ao2at[iao];
@@ -2398,12 +2407,573 @@ int or_integral_gradient(const or_calc* calc, int nat, const int* num, const dou
   return 0;
 }
 
-int or_singlepoint(const or_calc* calc, int nat, const int* num, const double* xyz, double charge,
-                   int uhf, double accuracy, double kt, int max_iter, double damping,
-                   or_result* res)
+/* ------------------------------------------------------- periodic cells
+ * Three-dimensional periodic cells at the Gamma point.  The reference carries
+ * the lattice only as comments: the translations trans(:, itr) of generate
+ * (src/adjlist.cu:26-30), get_hamiltonian (src/h0.cu:359) and ncoord_dexp
+ * (src/coulomb.cu:212-213), and the periodic branches of the ES2 and AES2
+ * updates (src/coulomb.cu:94-96, :554-557), which it leaves unimplemented.
+ * Restated here from the published method (tblite's Ewald-summed Coulomb
+ * terms, lattice sums of the short-range terms over the images within each
+ * cutoff):
+ *
+ *   lattice      a_k = lat[3k .. 3k+2] (bohr), V = |a_1 . (a_2 x a_3)|,
+ *                b_k with a_j . b_k = 2 pi delta_jk
+ *   translations T = n_1 a_1 + n_2 a_2 + n_3 a_3, |n_k| <= N_k with
+ *                N_k = ceil(R / d_k + s_k), d_k = V / |a_l x a_m| the spacing
+ *                of the lattice planes and s_k the spread of the atoms'
+ *                fractional coordinates (so every image within R of every
+ *                atom pair is present); n_1 outermost, n_3 innermost;
+ *                T_x = (n_1 a_1x + n_2 a_2x) + n_3 a_3x.  R is the largest
+ *                real-space cutoff of any term (PBC_RMIN or the Ewald one).
+ *   Ewald split  alpha = sqrt(pi) / V^(1/3); real space out to 6 / alpha
+ *                (erfc(6) = 2e-17), reciprocal space G != 0 with
+ *                |G| <= 12 alpha (exp(-G^2 / 4 alpha^2) = exp(-36)).
+ *   ES2          A_ab = sum'_T [f(r)(gamma_ab(r) - 1/r) + erfc(alpha r)/r]
+ *                     + (4 pi / V) sum_G exp(-G^2/4 alpha^2)/G^2 cos(G.R)
+ *                     - pi / (V alpha^2) + delta_AB (eta_ab - 2 alpha/sqrt(pi))
+ *                with r = |R - T|, R = R_A - R_B, the sum skipping r = 0.
+ *                The Klopman-Ohno deviation gamma - 1/r ~ -1/(2 eta^2 r^3)
+ *                would make the lattice sum diverge logarithmically, so it is
+ *                switched off smoothly between PBC_ES2_RON and PBC_ES2_ROFF
+ *                (f = 1 - 10x^3 + 15x^4 - 6x^5); the result is independent
+ *                of alpha element by element, charged cells included
+ *                (uniform neutralising background).
+ *   AES2         the undamped kernels are derivatives of the Ewald sum of
+ *                1/r (tin-foil boundary): sd = -grad phi, dd = -hess phi,
+ *                sq = hess phi / 3 (the trace part drops against traceless
+ *                quadrupoles), plus the damping corrections
+ *                (f3 - 1), (f5 - 1) of the molecular kernels summed over the
+ *                images within PBC_AES_CUT; an atom interacts with its own
+ *                images (self term (4 alpha^3 / 3 sqrt(pi)) I of the Hessian).
+ *   CN, repulsion, D4 CN and two-body: sums over (j, T) != (i, 0) within
+ *                their cutoffs.  D4 three-body: (1/3) sum_i sum_{a<b}
+ *                over pairs of distinct sites a, b (atom images) within the
+ *                cutoff of i and of each other.
+ *   H0, S, D, Q  neighbour list (i, j <= i, T) as in generate; blocks of the
+ *                images of a pair summed into one Gamma-point block.
+ */
+#define PBC_RMIN 60.0
+#define PBC_ES2_RON 20.0
+#define PBC_ES2_ROFF 25.0
+#define PBC_AES_CUT 40.0
+#define PBC_EWALD_X 6.0
+
+typedef struct {
+  double lat[9], rec[9];
+  double vol, alpha, rreal, gmax;
+  int ntr, ng;
+  double* tr;
+  double* g;
+} pbc_t;
+
+static double pbc_alpha_scale = 1.0; /* test hook: the results must not depend on it */
+
+void or_set_ewald_alpha_scale(double s) { pbc_alpha_scale = s; }
+
+static void cross3(const double* a, const double* b, double* c)
+{
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+static void pbc_free(pbc_t* p)
+{
+  free(p->tr);
+  free(p->g);
+  p->tr = p->g = NULL;
+}
+
+/* returns 0, or -1 for a degenerate lattice */
+static int pbc_init(pbc_t* p, const double* lat, int nat, const double* xyz)
+{
+  memset(p, 0, sizeof(*p));
+  memcpy(p->lat, lat, sizeof(double) * 9);
+  const double *a1 = lat, *a2 = lat + 3, *a3 = lat + 6;
+  double c[9];
+  cross3(a2, a3, c);
+  cross3(a3, a1, c + 3);
+  cross3(a1, a2, c + 6);
+  const double det = a1[0] * c[0] + a1[1] * c[1] + a1[2] * c[2];
+  if (!(fabs(det) > 1.0e-6)) return -1;
+  p->vol = fabs(det);
+  for (int k = 0; k < 9; ++k) p->rec[k] = 2.0 * PI * c[k] / det;
+  p->alpha = pbc_alpha_scale * sqrt(PI) / cbrt(p->vol);
+  p->rreal = PBC_EWALD_X / p->alpha;
+  p->gmax = 2.0 * PBC_EWALD_X * p->alpha;
+  const double rmax = fmax(PBC_RMIN, p->rreal);
+  int nrep[3];
+  for (int k = 0; k < 3; ++k) {
+    const double* ck = c + 3 * k;
+    const double dk = p->vol / sqrt(ck[0] * ck[0] + ck[1] * ck[1] + ck[2] * ck[2]);
+    double fmn = 0.0, fmx = 0.0;
+    for (int i = 0; i < nat; ++i) {
+      const double f = (xyz[3 * i] * p->rec[3 * k] + xyz[3 * i + 1] * p->rec[3 * k + 1] +
+                        xyz[3 * i + 2] * p->rec[3 * k + 2]) / (2.0 * PI);
+      if (i == 0 || f < fmn) fmn = f;
+      if (i == 0 || f > fmx) fmx = f;
+    }
+    nrep[k] = (int)ceil(rmax / dk + (fmx - fmn));
+  }
+  p->ntr = (2 * nrep[0] + 1) * (2 * nrep[1] + 1) * (2 * nrep[2] + 1);
+  p->tr = (double*)malloc(sizeof(double) * 3 * p->ntr);
+  int t = 0;
+  for (int n1 = -nrep[0]; n1 <= nrep[0]; ++n1)
+    for (int n2 = -nrep[1]; n2 <= nrep[1]; ++n2)
+      for (int n3 = -nrep[2]; n3 <= nrep[2]; ++n3) {
+        for (int x = 0; x < 3; ++x)
+          p->tr[3 * t + x] = ((double)n1 * a1[x] + (double)n2 * a2[x]) + (double)n3 * a3[x];
+        ++t;
+      }
+  /* reciprocal vectors: |m_k| <= gmax |a_k| / (2 pi) */
+  int mrep[3];
+  for (int k = 0; k < 3; ++k) {
+    const double* ak = lat + 3 * k;
+    mrep[k] = (int)ceil(p->gmax * sqrt(ak[0] * ak[0] + ak[1] * ak[1] + ak[2] * ak[2]) / (2.0 * PI));
+  }
+  const int gcap = (2 * mrep[0] + 1) * (2 * mrep[1] + 1) * (2 * mrep[2] + 1);
+  p->g = (double*)malloc(sizeof(double) * 3 * gcap);
+  const double g2max = p->gmax * p->gmax;
+  int ng = 0;
+  for (int m1 = -mrep[0]; m1 <= mrep[0]; ++m1)
+    for (int m2 = -mrep[1]; m2 <= mrep[1]; ++m2)
+      for (int m3 = -mrep[2]; m3 <= mrep[2]; ++m3) {
+        if (m1 == 0 && m2 == 0 && m3 == 0) continue;
+        double gv[3];
+        for (int x = 0; x < 3; ++x)
+          gv[x] = ((double)m1 * p->rec[x] + (double)m2 * p->rec[3 + x]) + (double)m3 * p->rec[6 + x];
+        const double g2 = gv[0] * gv[0] + gv[1] * gv[1] + gv[2] * gv[2];
+        if (g2 > g2max) continue;
+        memcpy(p->g + 3 * ng, gv, sizeof(gv));
+        ++ng;
+      }
+  p->ng = ng;
+  return 0;
+}
+
+int or_pbc_tables(const double* lat, int nat, const double* xyz, double* info, double* tr,
+                  int tr_cap, double* g, int g_cap)
+{
+  pbc_t p;
+  if (pbc_init(&p, lat, nat, xyz)) return -1;
+  if (info) {
+    info[0] = p.vol;
+    info[1] = p.alpha;
+    info[2] = p.rreal;
+    info[3] = p.gmax;
+    info[4] = p.ntr;
+    info[5] = p.ng;
+  }
+  if (tr && tr_cap >= p.ntr) memcpy(tr, p.tr, sizeof(double) * 3 * p.ntr);
+  if (g && g_cap >= p.ng) memcpy(g, p.g, sizeof(double) * 3 * p.ng);
+  pbc_free(&p);
+  return 0;
+}
+
+/* generate with translations (src/adjlist.cu:15-44 text): entries (jat, itr)
+ * for jat <= iat, every itr, skipping r2 < epsilon and r2 > cutoff^2; the
+ * vector is (R_i - R_j) - T, squared norm without contraction */
+static int adjlist_pbc(int nat, const double* xyz, const pbc_t* p, double cutoff, int* inl,
+                       int* nnl, int* nlat, int* nltr, int capacity)
+{
+  const double cutoff2 = cutoff * cutoff;
+  const double eps = 2.220446049250313e-16;
+  int img = 0;
+  for (int i = 0; i < nat; ++i) {
+    inl[i] = img;
+    for (int j = 0; j <= i; ++j)
+      for (int t = 0; t < p->ntr; ++t) {
+        volatile double dx = (xyz[3 * i] - xyz[3 * j]) - p->tr[3 * t];
+        volatile double dy = (xyz[3 * i + 1] - xyz[3 * j + 1]) - p->tr[3 * t + 1];
+        volatile double dz = (xyz[3 * i + 2] - xyz[3 * j + 2]) - p->tr[3 * t + 2];
+        volatile double xx = dx * dx, yy = dy * dy, zz = dz * dz;
+        volatile double s = xx + yy;
+        double r2 = s + zz;
+        if (r2 < eps || r2 > cutoff2) continue;
+        if (img < capacity) {
+          nlat[img] = j;
+          nltr[img] = t;
+        }
+        ++img;
+      }
+    nnl[i] = img - inl[i];
+  }
+  if (img > capacity) return -img;
+  return img;
+}
+
+int or_adjlist_pbc(int nat, const double* xyz, const double* lat, double cutoff, int* inl,
+                   int* nnl, int* nlat, int* nltr, int capacity)
+{
+  pbc_t p;
+  if (pbc_init(&p, lat, nat, xyz)) return -1;
+  int n = adjlist_pbc(nat, xyz, &p, cutoff, inl, nnl, nlat, nltr, capacity);
+  pbc_free(&p);
+  return n;
+}
+
+/* image vector (R_i - R_j) - T and its squared norm */
+static double pbc_vec(const double* xyz, int i, int j, const double* t, double v[3])
+{
+  for (int x = 0; x < 3; ++x) v[x] = (xyz[3 * i + x] - xyz[3 * j + x]) - t[x];
+  return v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+}
+
+/* ncoord_dexp with translations (src/coulomb.cu:205-233 text) */
+static void ncoord_pbc(const or_calc* c, const basis_t* b, const double* xyz, const pbc_t* p,
+                       double* cn)
+{
+  const tblite_amd_param* pr = &c->par;
+  double cutoff2 = pr->cn_cutoff * pr->cn_cutoff;
+  for (int i = 0; i < b->nat; ++i) cn[i] = 0.0;
+  for (int i = 0; i < b->nat; ++i)
+    for (int j = 0; j <= i; ++j)
+      for (int t = 0; t < p->ntr; ++t) {
+        double v[3];
+        double r2 = pbc_vec(xyz, i, j, p->tr + 3 * t, v);
+        if (r2 > cutoff2 || r2 < 1.0e-12) continue;
+        double r1 = sqrt(r2);
+        double rc = pr->elem[b->el[i]].rcov + pr->elem[b->el[j]].rcov;
+        double countf = exp_count(pr->cn_ka, r1, rc) * exp_count(pr->cn_kb, r1, rc + pr->cn_rshift);
+        cn[i] += countf;
+        if (i != j) cn[j] += countf;
+      }
+}
+
+/* repulsion with translations: half of each image pair to each atom */
+static void repulsion_pbc(const or_calc* c, const basis_t* b, const double* xyz, const pbc_t* p,
+                          double* e)
+{
+  const tblite_amd_param* pr = &c->par;
+  double cutoff2 = pr->rep_cutoff * pr->rep_cutoff;
+  for (int i = 0; i < b->nat; ++i) e[i] = 0.0;
+  for (int i = 0; i < b->nat; ++i) {
+    const tblite_amd_element* a = &pr->elem[b->el[i]];
+    for (int j = 0; j <= i; ++j) {
+      const tblite_amd_element* bb = &pr->elem[b->el[j]];
+      for (int t = 0; t < p->ntr; ++t) {
+        double v[3];
+        double r2 = pbc_vec(xyz, i, j, p->tr + 3 * t, v);
+        if (r2 > cutoff2 || r2 < 1.0e-12) continue;
+        double r1 = sqrt(r2);
+        double alpha = sqrt(a->arep * bb->arep);
+        double zeff = a->zeff * bb->zeff;
+        double kexp = (a->z <= 2 && bb->z <= 2) ? pr->rep_kexp_light : pr->rep_kexp;
+        double dE = zeff * exp(-alpha * pow(r1, kexp)) / pow(r1, pr->rep_rexp);
+        e[i] += 0.5 * dE;
+        if (i != j) e[j] += 0.5 * dE;
+      }
+    }
+  }
+}
+
+/* reciprocal-space part of the Ewald sum of 1/r at R: value (with the
+ * -pi/(V alpha^2) background term), gradient and Hessian (any may be NULL) */
+static void pbc_recip(const pbc_t* p, const double R[3], double* phi, double grad[3],
+                      double hess[9])
+{
+  const double fac = 4.0 * PI / p->vol, a2 = 4.0 * p->alpha * p->alpha;
+  double v = 0.0, gr[3] = {0, 0, 0}, h[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int k = 0; k < p->ng; ++k) {
+    const double* gv = p->g + 3 * k;
+    const double g2 = gv[0] * gv[0] + gv[1] * gv[1] + gv[2] * gv[2];
+    const double w = fac * exp(-g2 / a2) / g2;
+    const double arg = gv[0] * R[0] + gv[1] * R[1] + gv[2] * R[2];
+    const double cs = cos(arg), sn = sin(arg);
+    v += w * cs;
+    for (int x = 0; x < 3; ++x) {
+      gr[x] -= w * sn * gv[x];
+      for (int y = 0; y < 3; ++y) h[3 * x + y] -= w * cs * gv[x] * gv[y];
+    }
+  }
+  if (phi) *phi = v - PI / (p->vol * p->alpha * p->alpha);
+  if (grad) memcpy(grad, gr, sizeof(gr));
+  if (hess) memcpy(hess, h, sizeof(h));
+}
+
+static double es2_switch(double r)
+{
+  if (r <= PBC_ES2_RON) return 1.0;
+  if (r >= PBC_ES2_ROFF) return 0.0;
+  double x = (r - PBC_ES2_RON) / (PBC_ES2_ROFF - PBC_ES2_RON);
+  return 1.0 - x * x * x * (10.0 + x * (-15.0 + 6.0 * x));
+}
+
+static void es2_matrix_pbc(const or_calc* c, const basis_t* b, const double* xyz, const pbc_t* p,
+                           double* amat)
+{
+  const int nsh = b->nsh, nat = b->nat;
+  const double gexp = c->par.gexp, al = p->alpha;
+  const double rcut = fmax(PBC_ES2_ROFF, p->rreal);
+  const double rcut2 = rcut * rcut;
+  for (int iat = 0; iat < nat; ++iat) {
+    const int ei = b->el[iat], ii = b->ish_at[iat];
+    for (int jat = 0; jat < nat; ++jat) {
+      const int ej = b->el[jat], jj = b->ish_at[jat];
+      double R[3];
+      for (int x = 0; x < 3; ++x) R[x] = xyz[3 * iat + x] - xyz[3 * jat + x];
+      double rec;
+      pbc_recip(p, R, &rec, NULL, NULL);
+      for (int ish = 0; ish < b->nsh_at[iat]; ++ish)
+        for (int jsh = 0; jsh < b->nsh_at[jat]; ++jsh) {
+          const double eta = 0.5 * (c->hardness[ei][ish] + c->hardness[ej][jsh]);
+          const double etag = pow(eta, -gexp);
+          double real = 0.0;
+          for (int t = 0; t < p->ntr; ++t) {
+            double v[3];
+            const double r2 = pbc_vec(xyz, iat, jat, p->tr + 3 * t, v);
+            if (r2 < 1.0e-12 || r2 > rcut2) continue;
+            const double r1 = sqrt(r2);
+            const double f = es2_switch(r1);
+            double term = erfc(al * r1) / r1;
+            if (f > 0.0) term += f * (1.0 / pow(pow(r1, gexp) + etag, 1.0 / gexp) - 1.0 / r1);
+            real += term;
+          }
+          double a = real + rec;
+          if (iat == jat) a += eta - 2.0 * al / sqrt(PI);
+          amat[(jj + jsh) + nsh * (ii + ish)] = a;
+        }
+    }
+  }
+}
+
+static void aes2_matrices_pbc(const or_calc* c, const basis_t* b, const double* xyz,
+                              const double* mrad, const pbc_t* p, double* sd, double* dd,
+                              double* sq)
+{
+  const int nat = b->nat;
+  const double kdmp3 = c->par.dmp3, kdmp5 = c->par.dmp5, al = p->alpha;
+  const double rr2 = p->rreal * p->rreal, ra2 = PBC_AES_CUT * PBC_AES_CUT;
+  const double tpi = 2.0 * al / sqrt(PI);
+  for (int iat = 0; iat < nat; ++iat)
+    for (int jat = 0; jat < nat; ++jat) {
+      double R[3], gr[3], h[9];
+      for (int x = 0; x < 3; ++x) R[x] = xyz[3 * iat + x] - xyz[3 * jat + x];
+      pbc_recip(p, R, NULL, gr, h);
+      double csd[3] = {0, 0, 0}, cdd[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, csq[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      for (int t = 0; t < p->ntr; ++t) {
+        double v[3];
+        const double r2 = pbc_vec(xyz, iat, jat, p->tr + 3 * t, v);
+        if (r2 < 1.0e-12) continue;
+        const double r1 = sqrt(r2);
+        if (r2 <= rr2) {
+          const double ex = exp(-al * al * r2), ec = erfc(al * r1) / r1;
+          const double B = (ec + tpi * ex) / r2;
+          const double C = (3.0 * ec + tpi * (3.0 + 2.0 * al * al * r2) * ex) / (r2 * r2);
+          for (int x = 0; x < 3; ++x) {
+            gr[x] -= v[x] * B;
+            for (int y = 0; y < 3; ++y) h[3 * x + y] += v[x] * v[y] * C - (x == y ? B : 0.0);
+          }
+        }
+        if (r2 <= ra2) {
+          const double g1 = 1.0 / r1, g3 = g1 * g1 * g1, g5 = g3 * g1 * g1;
+          const double rr = 0.5 * (mrad[jat] + mrad[iat]) * g1;
+          const double f3 = 1.0 / (1.0 + 6.0 * pow(rr, kdmp3)) - 1.0;
+          const double f5 = 1.0 / (1.0 + 6.0 * pow(rr, kdmp5)) - 1.0;
+          for (int x = 0; x < 3; ++x) {
+            csd[x] += v[x] * g3 * f3;
+            for (int y = 0; y < 3; ++y) {
+              cdd[3 * x + y] += ((x == y ? g3 : 0.0) - 3.0 * v[x] * v[y] * g5) * f5;
+              csq[3 * x + y] += v[x] * v[y] * g5 * f5;
+            }
+          }
+        }
+      }
+      if (iat == jat)
+        for (int x = 0; x < 3; ++x) h[4 * x] += 2.0 * al * al * tpi / 3.0;
+      double* s = sd + 3 * (jat + nat * iat);
+      for (int x = 0; x < 3; ++x) s[x] = -gr[x] + csd[x];
+      for (int l = 0; l < 3; ++l)
+        for (int k = 0; k < 3; ++k) dd[k + 3 * (jat + nat * (l + 3 * iat))] = -h[3 * k + l] + cdd[3 * k + l];
+      double m[9];
+      for (int k = 0; k < 9; ++k) m[k] = h[k] / 3.0 + csq[k];
+      double* q = sq + 6 * (jat + nat * iat);
+      q[0] = m[0];
+      q[1] = 2.0 * m[1];
+      q[2] = m[4];
+      q[3] = 2.0 * m[2];
+      q[4] = 2.0 * m[5];
+      q[5] = m[8];
+    }
+}
+
+/* D4 CN with translations */
+static void d4_ncoord_pbc(const or_calc* c, const basis_t* b, const double* xyz, const pbc_t* p,
+                          double* cn)
+{
+  const tblite_amd_d4_param* d = &c->par.d4;
+  int nat = b->nat;
+  for (int i = 0; i < nat; ++i) cn[i] = 0.0;
+  for (int i = 0; i < nat; ++i)
+    for (int j = 0; j <= i; ++j)
+      for (int t = 0; t < p->ntr; ++t) {
+        double v[3];
+        double r2 = pbc_vec(xyz, i, j, p->tr + 3 * t, v);
+        if (r2 > d->cn_cutoff * d->cn_cutoff || r2 < 1.0e-12) continue;
+        const tblite_amd_element* ei = &c->par.elem[b->el[i]];
+        const tblite_amd_element* ej = &c->par.elem[b->el[j]];
+        double r = sqrt(r2), rc = ei->rcov + ej->rcov;
+        double den = d->cn_k4 * exp(-pow(fabs(ei->en - ej->en) + d->cn_k5, 2) / d->cn_k6);
+        double cnt = den * 0.5 * (1.0 + erf(-d->cn_kn * (r - rc) / rc));
+        cn[i] += cnt;
+        if (i != j) cn[j] += cnt;
+      }
+}
+
+/* two-body damping matrix summed over the images (diagonal: the atom's own
+ * images) */
+static void d4_dispmat_pbc(const or_calc* c, const basis_t* b, const double* xyz, const pbc_t* p,
+                           double* dmat)
+{
+  const tblite_amd_d4_param* d = &c->par.d4;
+  int nat = b->nat;
+  const double cut2 = d->disp2_cutoff * d->disp2_cutoff;
+  for (int i = 0; i < nat; ++i)
+    for (int j = 0; j < nat; ++j) {
+      double rr = 3.0 * d->elem[b->el[i]].r4r2 * d->elem[b->el[j]].r4r2;
+      double r0 = d->a1 * sqrt(rr) + d->a2;
+      double r06 = pow(r0, 6), r08 = r06 * r0 * r0;
+      double acc = 0.0;
+      for (int t = 0; t < p->ntr; ++t) {
+        double v[3];
+        double r2 = pbc_vec(xyz, i, j, p->tr + 3 * t, v);
+        if (r2 < 1.0e-12 || r2 > cut2) continue;
+        acc += -(d->s6 / (r2 * r2 * r2 + r06) + d->s8 * rr / (r2 * r2 * r2 * r2 + r08));
+      }
+      dmat[i * nat + j] = acc;
+    }
+}
+
+/* ATM term over the sites within the three-body cutoff of each atom */
+static void d4_threebody_pbc(const or_calc* c, const basis_t* b, const double* cn,
+                             const double* xyz, const pbc_t* p, double* e)
+{
+  const tblite_amd_d4_param* d = &c->par.d4;
+  int nat = b->nat, K = TBLITE_AMD_D4_MAXREF;
+  double* gw = (double*)calloc((size_t)nat * K, sizeof(double));
+  double* dgw = (double*)calloc((size_t)nat * K, sizeof(double));
+  double* c6 = (double*)calloc((size_t)nat * nat, sizeof(double));
+  for (int i = 0; i < nat; ++i) d4_weights(c, b->el[i], cn[i], 0.0, gw + K * i, dgw + K * i);
+  for (int i = 0; i < nat; ++i)
+    for (int j = 0; j < nat; ++j) {
+      int ei = b->el[i], ej = b->el[j];
+      double acc = 0.0;
+      for (int r = 0; r < d->elem[ei].nref; ++r)
+        for (int s = 0; s < d->elem[ej].nref; ++s)
+          acc += gw[K * i + r] * c->d4_c6[ei][ej][r][s] * gw[K * j + s];
+      c6[i * nat + j] = acc;
+    }
+  const double cut2 = d->disp3_cutoff * d->disp3_cutoff;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int i = 0; i < nat; ++i) {
+    /* sites (j, T) within the cutoff of atom i: position R_j + T relative to R_i */
+    int cap = 64, n = 0;
+    double* pos = (double*)malloc(sizeof(double) * 3 * cap);
+    int* at = (int*)malloc(sizeof(int) * cap);
+    for (int j = 0; j < nat; ++j)
+      for (int t = 0; t < p->ntr; ++t) {
+        double v[3];
+        double r2 = pbc_vec(xyz, j, i, p->tr + 3 * t, v);
+        /* v = (R_j - R_i) - T: the image of j at R_j - T */
+        if (r2 < 1.0e-12 || r2 > cut2) continue;
+        if (n == cap) {
+          cap *= 2;
+          pos = (double*)realloc(pos, sizeof(double) * 3 * cap);
+          at = (int*)realloc(at, sizeof(int) * cap);
+        }
+        memcpy(pos + 3 * n, v, sizeof(v));
+        at[n++] = j;
+      }
+    const double ri = d->elem[b->el[i]].r4r2;
+    double ei3 = 0.0;
+    for (int a = 0; a < n; ++a) {
+      const int j = at[a];
+      const double* rij = pos + 3 * a;
+      const double r2ij = rij[0] * rij[0] + rij[1] * rij[1] + rij[2] * rij[2];
+      const double rj = d->elem[b->el[j]].r4r2;
+      const double r0ij = d->a1 * sqrt(3.0 * ri * rj) + d->a2;
+      for (int bb = a + 1; bb < n; ++bb) {
+        const int k = at[bb];
+        const double* rik = pos + 3 * bb;
+        double rjk[3];
+        for (int x = 0; x < 3; ++x) rjk[x] = rik[x] - rij[x];
+        const double r2jk = rjk[0] * rjk[0] + rjk[1] * rjk[1] + rjk[2] * rjk[2];
+        if (r2jk < 1.0e-12 || r2jk > cut2) continue;
+        const double r2ik = rik[0] * rik[0] + rik[1] * rik[1] + rik[2] * rik[2];
+        const double rk = d->elem[b->el[k]].r4r2;
+        const double r0ik = d->a1 * sqrt(3.0 * ri * rk) + d->a2;
+        const double r0jk = d->a1 * sqrt(3.0 * rj * rk) + d->a2;
+        const double c9 = d->s9 * sqrt(fabs(c6[i * nat + j] * c6[i * nat + k] * c6[j * nat + k]));
+        const double r2ijk = r2ij * r2ik * r2jk, rijk = sqrt(r2ijk);
+        const double fdmp = 1.0 / (1.0 + 6.0 * pow(r0ij * r0ik * r0jk / rijk, d->alp / 3.0));
+        const double ang = 0.375 * (r2ij + r2jk - r2ik) * (r2ij - r2jk + r2ik) *
+                               (-r2ij + r2jk + r2ik) / r2ijk + 1.0;
+        ei3 += c9 * ang * fdmp / (rijk * r2ijk);
+      }
+    }
+    e[i] = ei3 / 3.0;
+    free(pos);
+    free(at);
+  }
+  free(gw);
+  free(dgw);
+  free(c6);
+}
+
+int or_pbc_terms(const or_calc* calc, int nat, const int* num, const double* xyz,
+                 const double* lat, double* cn, double* erep, double* cn4, double* amat,
+                 double* sd, double* dd, double* sq, double* mrad, double* dmat, double* e3)
 {
   basis_t b;
+  pbc_t p;
+  if (make_basis(calc, nat, num, &b)) return -1;
+  if (pbc_init(&p, lat, nat, xyz)) {
+    free_basis(&b);
+    return -1;
+  }
+  double* cnl = (double*)malloc(sizeof(double) * nat);
+  double* mr = (double*)malloc(sizeof(double) * nat);
+  double* c4 = (double*)malloc(sizeof(double) * nat);
+  ncoord_pbc(calc, &b, xyz, &p, cnl);
+  mp_radii(calc, &b, cnl, mr);
+  if (cn) memcpy(cn, cnl, sizeof(double) * nat);
+  if (mrad) memcpy(mrad, mr, sizeof(double) * nat);
+  if (erep) repulsion_pbc(calc, &b, xyz, &p, erep);
+  if (amat) es2_matrix_pbc(calc, &b, xyz, &p, amat);
+  if (sd && dd && sq) aes2_matrices_pbc(calc, &b, xyz, mr, &p, sd, dd, sq);
+  if (calc->par.d4.enabled) {
+    d4_ncoord_pbc(calc, &b, xyz, &p, c4);
+    if (cn4) memcpy(cn4, c4, sizeof(double) * nat);
+    if (dmat) d4_dispmat_pbc(calc, &b, xyz, &p, dmat);
+    if (e3) d4_threebody_pbc(calc, &b, c4, xyz, &p, e3);
+  }
+  free(cnl);
+  free(mr);
+  free(c4);
+  pbc_free(&p);
+  free_basis(&b);
+  return 0;
+}
+
+/* lat: lattice vectors of a periodic cell (NULL for a molecule) */
+static int singlepoint_impl(const or_calc* calc, int nat, const int* num, const double* xyz,
+                            const double* lat, double charge, int uhf, double accuracy, double kt,
+                            int max_iter, double damping, or_result* res)
+{
+  basis_t b;
+  pbc_t pbc;
+  const pbc_t* pb = NULL;
+  /* analytic gradients and the strain derivative of a cell are not built */
+  if (lat && res && (res->gradient || res->sigma)) return TBLITE_AMD_ERR_ARGUMENT;
   if (make_basis(calc, nat, num, &b)) return TBLITE_AMD_ERR_ARGUMENT;
+  if (lat) {
+    if (pbc_init(&pbc, lat, nat, xyz)) {
+      free_basis(&b);
+      return TBLITE_AMD_ERR_ARGUMENT;
+    }
+    pb = &pbc;
+  }
   int nsh = b.nsh, nao = b.nao;
   size_t n2 = (size_t)nao * nao;
   double* erep = (double*)calloc(nat, sizeof(double));
@@ -2433,19 +3003,30 @@ int or_singlepoint(const or_calc* calc, int nat, const int* num, const double* x
   int status = 0, rc = 0;
 
   /* repulsion */
-  repulsion(calc, &b, xyz, erep);
+  if (pb) repulsion_pbc(calc, &b, xyz, pb, erep);
+  else repulsion(calc, &b, xyz, erep);
   /* coulomb update (CN of the multipole damping radii = GFN CN) */
-  ncoord(calc, &b, xyz, cn);
+  if (pb) ncoord_pbc(calc, &b, xyz, pb, cn);
+  else ncoord(calc, &b, xyz, cn);
   mp_radii(calc, &b, cn, cc.mrad);
-  es2_matrix(calc, &b, xyz, cc.amat);
-  aes2_matrices(calc, &b, xyz, cc.mrad, cc.sd, cc.dd, cc.sq);
+  if (pb) {
+    es2_matrix_pbc(calc, &b, xyz, pb, cc.amat);
+    aes2_matrices_pbc(calc, &b, xyz, cc.mrad, pb, cc.sd, cc.dd, cc.sq);
+  } else {
+    es2_matrix(calc, &b, xyz, cc.amat);
+    aes2_matrices(calc, &b, xyz, cc.mrad, cc.sd, cc.dd, cc.sq);
+  }
   /* dispersion: D4 CN, two-body damping matrix, three-body energy (zero charges) */
   const int d4on = calc->par.d4.enabled;
   double* cn4 = (double*)calloc(nat, sizeof(double));
   double* dmat = (double*)calloc(d4on ? (size_t)nat * nat : 1, sizeof(double));
   double* e3 = (double*)calloc(nat, sizeof(double));
   double* e2 = (double*)calloc(nat, sizeof(double));
-  if (d4on) {
+  if (d4on && pb) {
+    d4_ncoord_pbc(calc, &b, xyz, pb, cn4);
+    d4_dispmat_pbc(calc, &b, xyz, pb, dmat);
+    d4_threebody_pbc(calc, &b, cn4, xyz, pb, e3);
+  } else if (d4on) {
     d4_ncoord(calc, &b, xyz, cn4);
     d4_dispmat(calc, &b, xyz, dmat);
     d4_threebody(calc, &b, cn4, xyz, e3);
@@ -2480,9 +3061,18 @@ int or_singlepoint(const or_calc* calc, int nat, const int* num, const double* x
   int cap = nat * (nat + 1) / 2 + 1;
   int* inl = (int*)malloc(sizeof(int) * nat);
   int* nnl = (int*)malloc(sizeof(int) * nat);
+  int* nltr = NULL;
+  if (pb) cap = -adjlist_pbc(nat, xyz, pb, cutoff, inl, nnl, NULL, NULL, 0) + 1;
   int* nlat = (int*)malloc(sizeof(int) * cap);
-  or_adjlist(nat, xyz, cutoff, inl, nnl, nlat, cap);
-  hamiltonian(calc, &b, xyz, inl, nnl, nlat, integral_cutoff(accuracy), se, S, D, Q, H);
+  if (pb) {
+    nltr = (int*)malloc(sizeof(int) * cap);
+    adjlist_pbc(nat, xyz, pb, cutoff, inl, nnl, nlat, nltr, cap);
+  } else {
+    or_adjlist(nat, xyz, cutoff, inl, nnl, nlat, cap);
+  }
+  hamiltonian(calc, &b, xyz, inl, nnl, nlat, pb ? pb->tr : NULL, nltr, integral_cutoff(accuracy),
+              se, S, D, Q, H);
+  free(nltr);
 
   /* SCF */
   int ndim = nsh + 9 * nat;
@@ -2708,8 +3298,26 @@ int or_singlepoint(const or_calc* calc, int nat, const int* num, const double* x
   free(n0at); free(vsh); free(vat); free(vdp); free(vqp); free(vao); free(eelec); free(eao);
   free(cc.amat); free(cc.sd); free(cc.dd); free(cc.sq); free(cc.mrad);
   free(cn4); free(dmat); free(e3); free(e2);
+  if (pb) pbc_free(&pbc);
   free_basis(&b);
   return rc;
+}
+
+int or_singlepoint(const or_calc* calc, int nat, const int* num, const double* xyz, double charge,
+                   int uhf, double accuracy, double kt, int max_iter, double damping,
+                   or_result* res)
+{
+  return singlepoint_impl(calc, nat, num, xyz, NULL, charge, uhf, accuracy, kt, max_iter, damping,
+                          res);
+}
+
+int or_singlepoint_pbc(const or_calc* calc, int nat, const int* num, const double* xyz,
+                       const double* lat, double charge, int uhf, double accuracy, double kt,
+                       int max_iter, double damping, or_result* res)
+{
+  if (!lat) return TBLITE_AMD_ERR_ARGUMENT;
+  return singlepoint_impl(calc, nat, num, xyz, lat, charge, uhf, accuracy, kt, max_iter, damping,
+                          res);
 }
 
 int or_singlepoint_many(const or_calc* calc, int nmol, const int* mol_nat, const int* num,

@@ -51,6 +51,31 @@ int or_singlepoint(const or_calc* calc, int nat, const int* num, const double* x
                    int uhf, double accuracy, double kt, int max_iter, double damping,
                    or_result* res);
 
+/* Singlepoint of a three-dimensional periodic cell at the Gamma point.
+ * lat(9): lattice vectors a_k = lat[3k .. 3k+2] in bohr.  Gradients are not
+ * built for cells (res->gradient / sigma must be NULL). */
+int or_singlepoint_pbc(const or_calc* calc, int nat, const int* num, const double* xyz,
+                       const double* lat, double charge, int uhf, double accuracy, double kt,
+                       int max_iter, double damping, or_result* res);
+/* Lattice tables of a cell: info = (V, alpha, real-space cutoff, |G|max,
+ * number of translations, number of reciprocal vectors); the translations
+ * tr(3, ntr) and reciprocal vectors g(3, ng) when the capacities suffice. */
+int or_pbc_tables(const double* lat, int nat, const double* xyz, double* info, double* tr,
+                  int tr_cap, double* g, int g_cap);
+/* Neighbour list of a cell: entries (nlat, nltr) = (atom, translation index). */
+int or_adjlist_pbc(int nat, const double* xyz, const double* lat, double cutoff, int* inl,
+                   int* nnl, int* nlat, int* nltr, int capacity);
+/* Classical terms of a cell (any output may be NULL): GFN CN, atom-resolved
+ * repulsion, D4 CN, ES2 matrix (nsh, nsh), AES2 matrices (layout of
+ * or_coulomb_matrices), multipole radii, D4 two-body damping matrix
+ * dmat[i*nat + j] (image sums) and atom-resolved three-body energy. */
+int or_pbc_terms(const or_calc* calc, int nat, const int* num, const double* xyz,
+                 const double* lat, double* cn, double* erep, double* cn4, double* amat,
+                 double* sd, double* dd, double* sq, double* mrad, double* dmat, double* e3);
+/* Test hook: scale factor of the Ewald splitting parameter (results must not
+ * depend on it). */
+void or_set_ewald_alpha_scale(double s);
+
 /* Independent molecules over an OpenMP pool of nthreads (0: all cores).
  * Molecule m has mol_nat[m] atoms starting at atom offset sum(mol_nat[:m]). */
 int or_singlepoint_many(const or_calc* calc, int nmol, const int* mol_nat, const int* num,

@@ -61,6 +61,18 @@ def lib():
         L.or_singlepoint.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
                                      C.c_double, C.c_int, C.c_double, C.c_double, C.c_int, C.c_double,
                                      C.POINTER(Result)]
+        L.or_singlepoint_pbc.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                         C.POINTER(C.c_double), C.c_double, C.c_int, C.c_double,
+                                         C.c_double, C.c_int, C.c_double, C.POINTER(Result)]
+        L.or_pbc_tables.argtypes = [C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int,
+                                    C.POINTER(C.c_double), C.c_int]
+        L.or_adjlist_pbc.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_double,
+                                     C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                     C.POINTER(C.c_int), C.c_int]
+        L.or_pbc_terms.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                   C.POINTER(C.c_double)] + [C.POINTER(C.c_double)] * 10
+        L.or_set_ewald_alpha_scale.argtypes = [C.c_double]
         L.or_singlepoint_many.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                           C.POINTER(C.c_double), C.c_double, C.c_double, C.c_int,
                                           C.c_double, C.c_int, C.POINTER(C.c_double),
@@ -139,10 +151,12 @@ class Oracle:
         return len(num), nsh.value, nao.value
 
     def singlepoint(self, num, xyz, charge=0.0, uhf=0, accuracy=1.0, temperature=300.0,
-                    max_iter=250, damping=0.4, wavefunction=False, gradient=False):
+                    max_iter=250, damping=0.4, wavefunction=False, gradient=False, lattice=None):
         """wavefunction=True also returns density and coeff (nao x nao numpy
         arrays indexed [row, column]; coeff[:, k] is level k); gradient=True
-        returns gradient (nat, 3) = dE/dR in Eh/bohr and sigma (3, 3)."""
+        returns gradient (nat, 3) = dE/dR in Eh/bohr and sigma (3, 3).
+        lattice (3, 3), rows = lattice vectors in bohr: a periodic cell at the
+        Gamma point (no gradient)."""
         num, xyz = self._mol(num, xyz)
         nat, nsh, nao = self.dims(num)
         out = dict(qsh=np.zeros(nsh), qat=np.zeros(nat), dpat=np.zeros((nat, 3)),
@@ -157,8 +171,14 @@ class Oracle:
         r = Result()
         for k in out:
             setattr(r, k, _dp(out[k]))
-        rc = self.L.or_singlepoint(self.calc, nat, _ip(num), _dp(xyz), charge, uhf, accuracy,
-                                   temperature * KB_EH, max_iter, damping, C.byref(r))
+        if lattice is None:
+            rc = self.L.or_singlepoint(self.calc, nat, _ip(num), _dp(xyz), charge, uhf, accuracy,
+                                       temperature * KB_EH, max_iter, damping, C.byref(r))
+        else:
+            lat = np.ascontiguousarray(lattice, dtype=np.float64).reshape(9)
+            rc = self.L.or_singlepoint_pbc(self.calc, nat, _ip(num), _dp(xyz), _dp(lat), charge,
+                                           uhf, accuracy, temperature * KB_EH, max_iter, damping,
+                                           C.byref(r))
         out.update(energy=r.energy, erep=r.erep, eelec=r.eelec, edisp2=r.edisp2,
                    edisp3=r.edisp3, iterations=r.iterations,
                    converged=bool(r.converged), status=r.status, rc=rc)
@@ -182,6 +202,52 @@ class Oracle:
                                    temperature * KB_EH, max_iter, damping, nthreads, _dp(e),
                                    _ip(it), _ip(st))
         return e, it, st
+
+    # ---- periodic cells (lattice: (3, 3), rows = lattice vectors, bohr)
+    def set_ewald_alpha_scale(self, s):
+        self.L.or_set_ewald_alpha_scale(float(s))
+
+    def pbc_tables(self, lattice, xyz):
+        lat = np.ascontiguousarray(lattice, dtype=np.float64).reshape(9)
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        info = np.zeros(6)
+        if self.L.or_pbc_tables(_dp(lat), len(xyz), _dp(xyz), _dp(info), None, 0, None, 0):
+            raise ValueError("degenerate lattice")
+        ntr, ng = int(info[4]), int(info[5])
+        tr = np.zeros((ntr, 3))
+        g = np.zeros((max(ng, 1), 3))
+        self.L.or_pbc_tables(_dp(lat), len(xyz), _dp(xyz), _dp(info), _dp(tr), ntr, _dp(g), ng)
+        return dict(volume=info[0], alpha=info[1], rreal=info[2], gmax=info[3], trans=tr, recip=g[:ng])
+
+    def adjlist_pbc(self, xyz, lattice, cutoff):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        lat = np.ascontiguousarray(lattice, dtype=np.float64).reshape(9)
+        nat = len(xyz)
+        inl = np.zeros(nat, dtype=np.int32)
+        nnl = np.zeros(nat, dtype=np.int32)
+        n = -self.L.or_adjlist_pbc(nat, _dp(xyz), _dp(lat), cutoff, _ip(inl), _ip(nnl), None, None, 0)
+        nlat = np.zeros(max(n, 1), dtype=np.int32)
+        nltr = np.zeros(max(n, 1), dtype=np.int32)
+        self.L.or_adjlist_pbc(nat, _dp(xyz), _dp(lat), cutoff, _ip(inl), _ip(nnl), _ip(nlat),
+                              _ip(nltr), max(n, 1))
+        return inl, nnl, nlat[:n], nltr[:n]
+
+    def pbc_terms(self, num, xyz, lattice):
+        """Classical terms of a cell: cn, erep (per atom), cn4, amat (nsh, nsh),
+        sd (nat, nat, 3) [i, j] = amat_sd(:, j, i), dd (nat, 3, nat, 3),
+        sq (nat, nat, 6), mrad, dmat (nat, nat), e3 (per atom)."""
+        num, xyz = self._mol(num, xyz)
+        nat, nsh, nao = self.dims(num)
+        lat = np.ascontiguousarray(lattice, dtype=np.float64).reshape(9)
+        o = dict(cn=np.zeros(nat), erep=np.zeros(nat), cn4=np.zeros(nat), amat=np.zeros((nsh, nsh)),
+                 sd=np.zeros((nat, nat, 3)), dd=np.zeros((nat, 3, nat, 3)), sq=np.zeros((nat, nat, 6)),
+                 mrad=np.zeros(nat), dmat=np.zeros((nat, nat)), e3=np.zeros(nat))
+        rc = self.L.or_pbc_terms(self.calc, nat, _ip(num), _dp(xyz), _dp(lat),
+                                 *[_dp(o[k]) for k in ("cn", "erep", "cn4", "amat", "sd", "dd", "sq",
+                                                       "mrad", "dmat", "e3")])
+        if rc:
+            raise ValueError("or_pbc_terms failed")
+        return o
 
     def broyden_linear(self, A, b, q0, damping=0.4, memory=250, nstep=10, omega0=0.01):
         """Mixer iterates on F(q) = A q + b: visited q's (nstep+1, n) and

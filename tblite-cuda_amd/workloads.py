@@ -241,3 +241,48 @@ def flatten(mols):
     num = np.concatenate([m[0] for m in mols]).astype(np.int32)
     xyz = np.ascontiguousarray(np.concatenate([m[1] for m in mols]), dtype=np.float64)
     return mol_nat, num, xyz
+
+
+def water_box(n, rng, density=LIQUID_DENSITY, shear=0.0):
+    """Periodic cell of n rigid waters at the given number density
+    (molecules / A^3): O uniform in the cell with the O-O minimum image at
+    least 2.6 A, uniform orientations.  The cell is cubic, or sheared by
+    `shear` (a_2 gains shear * L along x, a_3 gains shear * L along y) for a
+    triclinic case.  Returns (num, xyz_bohr, lattice_bohr) with
+    lattice[k] = k-th lattice vector."""
+    L = (n / density) ** (1.0 / 3.0)
+    lat = np.array([[L, 0.0, 0.0], [shear * L, L, 0.0], [0.0, shear * L, L]])
+    inv = np.linalg.inv(lat)
+    pts = []
+    for _ in range(200000):
+        if len(pts) == n:
+            break
+        f = rng.uniform(0.0, 1.0, 3)
+        p = f @ lat
+        ok = True
+        for q in pts:
+            d = (p - q) @ inv
+            d -= np.round(d)
+            d = d @ lat
+            if d @ d < 2.6 * 2.6:
+                ok = False
+                break
+        if ok:
+            pts.append(p)
+    if len(pts) < n:
+        raise RuntimeError("water_box: could not place the molecules")
+    frame = _water_frame()
+    num, xyz = [], []
+    for p in pts:
+        R = _random_rotation(rng)
+        for k, z in enumerate((8, 1, 1)):
+            num.append(z)
+            xyz.append(p + frame[k] @ R.T)
+    return (np.array(num, dtype=np.int32), np.array(xyz) * AATOAU, lat * AATOAU)
+
+
+def periodic_cells(count=4, seed=5):
+    """Seeded water cells: 4, 6, 8, ... molecules at liquid density, every
+    other one triclinic (shear 0.15)."""
+    rng = np.random.default_rng(seed)
+    return [water_box(4 + 2 * k, rng, shear=0.15 if k % 2 else 0.0) for k in range(count)]

@@ -167,6 +167,23 @@ int tblite_amd_get_error(tblite_amd_context ctx, char* buffer, int buffer_size);
  * numbers, xyz(3,nat) in bohr. */
 tblite_amd_structure tblite_amd_new_structure(tblite_amd_context ctx, int nat, const int* num,
                                               const double* xyz, double charge, int uhf);
+/* Periodic structure: tblite's tblite_new_structure(error, natoms, numbers,
+ * positions, charge, uhf, lattice, periodic) with structure_type's `periodic`
+ * flag (src/structure.h:9, never set there).  lattice(3,3) holds the three
+ * lattice vectors in bohr, lattice(:, k) = a_k contiguous (tblite's
+ * mol%lattice); periodic(3) all nonzero for a three-dimensional cell, all
+ * zero (or NULL) for a molecule; other combinations are an error.  Cells are
+ * computed at the Gamma point: lattice sums over the images within each
+ * cutoff (neighbour list, CN, repulsion, H0 and the multipole integrals, D4
+ * two- and three-body) and Ewald sums for ES2 and AES2 (the periodic
+ * branches the reference leaves as comments: src/adjlist.cu:26-30,
+ * src/h0.cu:359, src/coulomb.cu:94-96, :212-213, :554-557).  Analytic
+ * gradients of cells are not available (ERR_ARGUMENT). */
+tblite_amd_structure tblite_amd_new_structure_periodic(tblite_amd_context ctx, int nat,
+                                                       const int* num, const double* xyz,
+                                                       double charge, int uhf,
+                                                       const double* lattice,
+                                                       const int* periodic);
 void tblite_amd_delete_structure(tblite_amd_structure* mol);
 /* Atom, shell and orbital counts of a structure under calc's parameter set
  * (calc may be NULL: the built-in GFN2 data). */
@@ -225,6 +242,16 @@ int tblite_amd_singlepoint_flat(tblite_amd_context ctx, tblite_amd_calculator ca
                                 const int* mol_nat, const int* num, const double* xyz,
                                 const double* charge, const int* uhf, double accuracy,
                                 double* energies, int* iterations, int* status);
+
+/* Flat batch of periodic cells: as tblite_amd_singlepoint_flat, with
+ * lattice(9, nmol) the lattice vectors of every cell (layout of
+ * tblite_amd_new_structure_periodic).  Cells run one at a time across the
+ * device. */
+int tblite_amd_singlepoint_flat_periodic(tblite_amd_context ctx, tblite_amd_calculator calc,
+                                         int nmol, const int* mol_nat, const int* num,
+                                         const double* xyz, const double* lattice,
+                                         const double* charge, const int* uhf, double accuracy,
+                                         double* energies, int* iterations, int* status);
 
 /* Analytic gradients.  The reference's xtb_singlepoint takes gradient(3,nat)
  * and sigma(3,3) arguments (src/xtb.cu:188-197) and sketches the FP64
@@ -375,6 +402,12 @@ int tblite_amd_batch_get_cn(tblite_amd_context ctx, int imol, double* cn);
  * NULL to query nnz. */
 int tblite_amd_batch_get_adjlist(tblite_amd_context ctx, int imol, int* inl, int* nnl, int* nlat,
                                  int capacity);
+/* Lattice images of cell imol (generate's trans(:, itr), src/adjlist.cu:26-30
+ * text): the translation index of every neighbour-list entry, nltr(nnz), and
+ * the translation table trans(3, ntr); either may be NULL.  Returns ntr (0
+ * for a molecule) or < 0. */
+int tblite_amd_batch_get_images(tblite_amd_context ctx, int imol, int* nltr, int nltr_capacity,
+                                double* trans, int trans_capacity);
 /* Integrals, get_hamiltonian (src/h0.cu:292-502): overlap(nao,nao),
  * dpint(3,nao,nao), qpint(6,nao,nao), hamiltonian(nao,nao) in tblite's
  * layout; any pointer may be NULL. */

@@ -2956,6 +2956,38 @@ int or_pbc_terms(const or_calc* calc, int nat, const int* num, const double* xyz
   return 0;
 }
 
+int or_hamiltonian_pbc(const or_calc* calc, int nat, const int* num, const double* xyz,
+                       const double* lat, double accuracy, double* S, double* D, double* Q,
+                       double* H)
+{
+  basis_t b;
+  pbc_t p;
+  if (make_basis(calc, nat, num, &b)) return -1;
+  if (pbc_init(&p, lat, nat, xyz)) {
+    free_basis(&b);
+    return -1;
+  }
+  double* cn = (double*)malloc(sizeof(double) * nat);
+  double* se = (double*)malloc(sizeof(double) * b.nsh);
+  ncoord_pbc(calc, &b, xyz, &p, cn);
+  for (int i = 0; i < nat; ++i) {
+    int e = b.el[i], ii = b.ish_at[i];
+    for (int s = 0; s < b.nsh_at[i]; ++s) se[ii + s] = calc->selfenergy[e][s] - calc->kcn[e][s] * cn[i];
+  }
+  double cutoff = get_cutoff(b.min_alpha, accuracy);
+  int* inl = (int*)malloc(sizeof(int) * nat);
+  int* nnl = (int*)malloc(sizeof(int) * nat);
+  int cap = -adjlist_pbc(nat, xyz, &p, cutoff, inl, nnl, NULL, NULL, 0) + 1;
+  int* nlat = (int*)malloc(sizeof(int) * cap);
+  int* nltr = (int*)malloc(sizeof(int) * cap);
+  adjlist_pbc(nat, xyz, &p, cutoff, inl, nnl, nlat, nltr, cap);
+  hamiltonian(calc, &b, xyz, inl, nnl, nlat, p.tr, nltr, integral_cutoff(accuracy), se, S, D, Q, H);
+  free(cn); free(se); free(inl); free(nnl); free(nlat); free(nltr);
+  pbc_free(&p);
+  free_basis(&b);
+  return 0;
+}
+
 /* lat: lattice vectors of a periodic cell (NULL for a molecule) */
 static int singlepoint_impl(const or_calc* calc, int nat, const int* num, const double* xyz,
                             const double* lat, double charge, int uhf, double accuracy, double kt,

@@ -72,6 +72,10 @@ int or_adjlist_pbc(int nat, const double* xyz, const double* lat, double cutoff,
 int or_pbc_terms(const or_calc* calc, int nat, const int* num, const double* xyz,
                  const double* lat, double* cn, double* erep, double* cn4, double* amat,
                  double* sd, double* dd, double* sq, double* mrad, double* dmat, double* e3);
+/* Gamma-point S, D, Q, H of a cell (layouts of or_hamiltonian). */
+int or_hamiltonian_pbc(const or_calc* calc, int nat, const int* num, const double* xyz,
+                       const double* lat, double accuracy, double* S, double* D, double* Q,
+                       double* H);
 /* Test hook: scale factor of the Ewald splitting parameter (results must not
  * depend on it). */
 void or_set_ewald_alpha_scale(double s);

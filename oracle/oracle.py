@@ -100,6 +100,8 @@ def lib():
                                     C.POINTER(C.c_double)]
         L.or_hamiltonian.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
                                      C.c_double] + [C.POINTER(C.c_double)] * 4
+        L.or_hamiltonian_pbc.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                         C.POINTER(C.c_double), C.c_double] + [C.POINTER(C.c_double)] * 4
         L.or_coulomb_matrices.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int),
                                           C.POINTER(C.c_double)] + [C.POINTER(C.c_double)] * 5
         L.or_sygvd.argtypes = [C.c_int] + [C.POINTER(C.c_double)] * 4
@@ -307,6 +309,18 @@ class Oracle:
         Q = np.zeros((nao, nao, 6))
         H = np.zeros((nao, nao))
         self.L.or_hamiltonian(self.calc, len(num), _ip(num), _dp(xyz), accuracy, _dp(S), _dp(D), _dp(Q), _dp(H))
+        return S, D, Q, H
+
+    def hamiltonian_pbc(self, num, xyz, lattice, accuracy=1.0):
+        num, xyz = self._mol(num, xyz)
+        _, _, nao = self.dims(num)
+        lat = np.ascontiguousarray(lattice, dtype=np.float64).reshape(9)
+        S = np.zeros((nao, nao))
+        D = np.zeros((nao, nao, 3))
+        Q = np.zeros((nao, nao, 6))
+        H = np.zeros((nao, nao))
+        self.L.or_hamiltonian_pbc(self.calc, len(num), _ip(num), _dp(xyz), _dp(lat), accuracy,
+                                  _dp(S), _dp(D), _dp(Q), _dp(H))
         return S, D, Q, H
 
     def coulomb_matrices(self, num, xyz):

@@ -17,7 +17,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtblite_amd.so")
-SOURCES = ["setup.hip", "scf.hip", "scf_grid.hip", "gradient.hip", "engine.cpp", "io.cpp"]
+SOURCES = ["setup.hip", "scf.hip", "scf_grid.hip", "gradient.hip", "periodic.hip", "engine.cpp", "io.cpp"]
 ARCH = os.environ.get("TBLITE_AMD_ARCH", "gfx950")
 
 FLAGS = [

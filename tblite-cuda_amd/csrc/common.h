@@ -239,11 +239,48 @@ struct BatchDev {
   double* g_vsh;
   double* g_w;
   double* g_c6d;
+  // periodic cells (all null for a batch of molecules): per molecule the
+  // number of lattice translations (0 for a molecule), their offset into
+  // trans (3 doubles each, in the order of the neighbour list's translation
+  // index), the reciprocal vectors G != 0 of the Ewald sums (mol_ng,
+  // mol_g_off into recip) and the Ewald constants (TA_PBC_EWALD doubles:
+  // cell volume, splitting parameter, real-space cutoff); per neighbour-list
+  // entry its translation index; per atom of a cell the three-body sites,
+  // the images of atoms within the three-body cutoff (capacity-based offset
+  // atm_site_off, count atm_site_n, 4 doubles each: R_site - R_atom and the
+  // atom index; capacity atm_site_cap)
+  const int* mol_ntr;
+  const int64_t* mol_tr_off;
+  const double* trans;
+  const int* mol_ng;
+  const int64_t* mol_g_off;
+  const double* recip;
+  const double* mol_ewald;
+  int* nl_itr;
+  const int64_t* atm_site_off;
+  const int* atm_site_cap;
+  int* atm_site_n;
+  double* atm_site;
   // optional per-molecule phase clocks of the SCF kernel (TA_PROF_SLOTS each), may be null
   long long* prof;
   // optional completion records (one per molecule, pinned host memory), may be null
   TaDoneRecord* done;
 };
+
+// Periodic cells at the Gamma point.  Every real-space lattice sum runs over
+// one translation table per cell, n_k in [-N_k, N_k] with
+// N_k = ceil(max(TA_PBC_RMIN, Ewald cutoff) / d_k + fractional spread),
+// n_1 outermost; the Ewald splitting parameter is sqrt(pi) / V^(1/3), its
+// real-space cutoff TA_PBC_EWALD_X / alpha and its reciprocal cutoff
+// 2 TA_PBC_EWALD_X alpha; the ES2 kernel is switched to 1/r between
+// TA_PBC_ES2_RON and TA_PBC_ES2_ROFF (quintic), the AES2 damping corrections
+// are summed out to TA_PBC_AES_CUT.
+#define TA_PBC_RMIN 60.0
+#define TA_PBC_ES2_RON 20.0
+#define TA_PBC_ES2_ROFF 25.0
+#define TA_PBC_AES_CUT 40.0
+#define TA_PBC_EWALD_X 6.0
+#define TA_PBC_EWALD 4
 
 // D4 weight blocks per atom of the gradient (BatchDev::g_d4)
 #define TA_GD4 5

@@ -31,9 +31,14 @@ __global__ void k_setup(BatchDev b);
 __global__ void k_adjlist(BatchDev b);
 __global__ void k_d4_atm(BatchDev b, const int* work_mol, const int* work_i0, const int* work_i1);
 __global__ void k_setup_pairs(BatchDev b, const int* work_mol, const int* work_a0, const int* work_a1);
-template <int LMAX>
+template <int LMAX, bool PBC>
 __global__ void k_integrals(BatchDev b, const int* work_mol, const int* work_kind,
                             const int* work_start);
+// periodic cells (periodic.hip)
+__global__ void k_pbc_pairs(BatchDev b, const int* work_mol, const int* work_a0, const int* work_a1);
+__global__ void k_pbc_atm_sites(BatchDev b, const int* work_mol, const int* work_atom);
+__global__ void k_pbc_atm(BatchDev b, const int* work_mol, const int* work_atom, const int* work_r0,
+                          const int* work_r1);
 template <int NT, int MAXT, int MAXB, int MAXC, bool MATS_LDS>
 __global__ void k_scf(BatchDev b, const int* mols, int ldmax, int natmax, int nshmax);
 // analytic gradient (gradient.hip)
@@ -245,6 +250,104 @@ double get_cutoff(double min_alpha, double acc)
   return std::fmin(std::sqrt(2.0 * integral_cutoff(acc) / min_alpha), 40.0);
 }
 
+// ------------------------------------------------------- periodic cells
+// Lattice tables of one cell (a_k = lat[3k .. 3k+2], bohr): the translations
+// of every real-space sum and the reciprocal vectors of the Ewald sums, with
+// the constants listed at TA_PBC_RMIN in common.h.  The translations are
+// formed with a fixed operation order, T_x = (n1 a1x + n2 a2x) + n3 a3x, so
+// the neighbour list's translation indices and distances are reproducible
+// on any host.  Returns false for a degenerate lattice.
+struct CellTables {
+  double vol = 0.0, alpha = 0.0, rreal = 0.0;
+  std::vector<double> tr, g;
+};
+
+bool cell_tables(const double* lat, int nat, const double* xyz, CellTables& ct)
+{
+#pragma clang fp contract(off)
+  const double *a1 = lat, *a2 = lat + 3, *a3 = lat + 6;
+  auto cross = [](const double* a, const double* b, double* c) {
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+  };
+  double c[9], rec[9];
+  cross(a2, a3, c);
+  cross(a3, a1, c + 3);
+  cross(a1, a2, c + 6);
+  const double det = a1[0] * c[0] + a1[1] * c[1] + a1[2] * c[2];
+  if (!(std::fabs(det) > 1.0e-6)) return false;
+  const double pi = 3.14159265358979323846;
+  ct.vol = std::fabs(det);
+  for (int k = 0; k < 9; ++k) rec[k] = 2.0 * pi * c[k] / det;
+  ct.alpha = std::sqrt(pi) / std::cbrt(ct.vol);
+  ct.rreal = TA_PBC_EWALD_X / ct.alpha;
+  const double gmax = 2.0 * TA_PBC_EWALD_X * ct.alpha;
+  const double rmax = std::fmax(TA_PBC_RMIN, ct.rreal);
+  int nrep[3];
+  for (int k = 0; k < 3; ++k) {
+    const double* ck = c + 3 * k;
+    const double dk = ct.vol / std::sqrt(ck[0] * ck[0] + ck[1] * ck[1] + ck[2] * ck[2]);
+    double fmn = 0.0, fmx = 0.0;
+    for (int i = 0; i < nat; ++i) {
+      const double f = (xyz[3 * i] * rec[3 * k] + xyz[3 * i + 1] * rec[3 * k + 1] +
+                        xyz[3 * i + 2] * rec[3 * k + 2]) / (2.0 * pi);
+      if (i == 0 || f < fmn) fmn = f;
+      if (i == 0 || f > fmx) fmx = f;
+    }
+    nrep[k] = (int)std::ceil(rmax / dk + (fmx - fmn));
+  }
+  ct.tr.clear();
+  for (int n1 = -nrep[0]; n1 <= nrep[0]; ++n1)
+    for (int n2 = -nrep[1]; n2 <= nrep[1]; ++n2)
+      for (int n3 = -nrep[2]; n3 <= nrep[2]; ++n3)
+        for (int x = 0; x < 3; ++x)
+          ct.tr.push_back(((double)n1 * a1[x] + (double)n2 * a2[x]) + (double)n3 * a3[x]);
+  int mrep[3];
+  for (int k = 0; k < 3; ++k) {
+    const double* ak = lat + 3 * k;
+    mrep[k] = (int)std::ceil(gmax * std::sqrt(ak[0] * ak[0] + ak[1] * ak[1] + ak[2] * ak[2]) /
+                             (2.0 * pi));
+  }
+  ct.g.clear();
+  for (int m1 = -mrep[0]; m1 <= mrep[0]; ++m1)
+    for (int m2 = -mrep[1]; m2 <= mrep[1]; ++m2)
+      for (int m3 = -mrep[2]; m3 <= mrep[2]; ++m3) {
+        if (m1 == 0 && m2 == 0 && m3 == 0) continue;
+        double gv[3];
+        for (int x = 0; x < 3; ++x)
+          gv[x] = ((double)m1 * rec[x] + (double)m2 * rec[3 + x]) + (double)m3 * rec[6 + x];
+        if (gv[0] * gv[0] + gv[1] * gv[1] + gv[2] * gv[2] > gmax * gmax) continue;
+        ct.g.insert(ct.g.end(), gv, gv + 3);
+      }
+  return true;
+}
+
+// Upper bounds of a cell's neighbour-list length (pairs j <= i with an image
+// within cutoff) and of each atom's three-body site count (images within
+// cut3), with a relative slack of 1e-9 so that rounding on the device can
+// only give fewer entries.
+void cell_counts(int nat, const double* xyz, const std::vector<double>& tr, double cutoff,
+                 double cut3, int64_t& nl, std::vector<int>& sites)
+{
+  const int ntr = (int)(tr.size() / 3);
+  const double c2 = cutoff * cutoff * (1.0 + 1.0e-9) + 1.0e-9;
+  const double s2 = cut3 * cut3 * (1.0 + 1.0e-9) + 1.0e-9;
+  nl = 0;
+  sites.assign(nat, 0);
+  for (int i = 0; i < nat; ++i)
+    for (int j = 0; j < nat; ++j) {
+      const double dx = xyz[3 * i] - xyz[3 * j], dy = xyz[3 * i + 1] - xyz[3 * j + 1];
+      const double dz = xyz[3 * i + 2] - xyz[3 * j + 2];
+      for (int t = 0; t < ntr; ++t) {
+        const double x = dx - tr[3 * t], y = dy - tr[3 * t + 1], z = dz - tr[3 * t + 2];
+        const double r2 = x * x + y * y + z * z;
+        if (j <= i && r2 <= c2) ++nl;
+        if (r2 <= s2) ++sites[i];
+      }
+    }
+}
+
 // -------------------------------------------------------------- workspace
 struct Arena {
   char* base = nullptr;
@@ -281,6 +384,24 @@ struct Plan {
   // pair-plane work: atom rows [pair_a0, pair_a1) of one molecule
   std::vector<int> pair_mol, pair_a0, pair_a1;
   std::vector<int> grid_mols;  // molecules run by the multi-workgroup SCF path
+  // periodic cells (empty for a batch of molecules): per molecule the
+  // translation and reciprocal-vector counts and offsets and the Ewald
+  // constants; the tables; integral work of cells (s/p-only items first,
+  // nwork_pbc_sp of them); pair-plane items (atom rows); three-body site
+  // items (one per atom) and pair items (site rows of one anchor atom); per
+  // atom the site capacity and offset
+  bool any_pbc = false;
+  std::vector<int> mol_ntr, mol_ng;
+  std::vector<int64_t> mol_tr_off, mol_g_off;
+  std::vector<double> trans, recip, mol_ewald;
+  std::vector<int> pw_mol, pw_kind, pw_start;
+  int nwork_pbc_sp = 0;
+  std::vector<int> pp_mol, pp_a0, pp_a1;
+  std::vector<int> ps_mol, ps_atom;
+  std::vector<int> pa_mol, pa_atom, pa_r0, pa_r1;
+  std::vector<int64_t> site_off;
+  std::vector<int> site_cap;
+  int64_t tot_site = 0;
   struct Bucket {
     int nt, maxt;
     bool lds;
@@ -306,6 +427,10 @@ struct DeviceBatch {
   int* d_atm_mol = nullptr;
   int* d_atm_i0 = nullptr;
   int* d_atm_i1 = nullptr;
+  int *d_pw_mol = nullptr, *d_pw_kind = nullptr, *d_pw_start = nullptr;
+  int *d_pp_mol = nullptr, *d_pp_a0 = nullptr, *d_pp_a1 = nullptr;
+  int *d_ps_mol = nullptr, *d_ps_atom = nullptr;
+  int *d_pa_mol = nullptr, *d_pa_atom = nullptr, *d_pa_r0 = nullptr, *d_pa_r1 = nullptr;
   std::vector<int*> d_bucket_mols;
   // energy-weighted density tiles of the gradient (molecule, row tile,
   // column tile); the host copy stays alive while the upload is in flight
@@ -460,6 +585,8 @@ struct tblite_amd_structure_s {
   std::vector<double> xyz;
   double charge;
   int uhf;
+  bool periodic = false;  // three-dimensional cell with lattice vectors lat
+  double lat[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 struct tblite_amd_calculator_s {
@@ -548,8 +675,14 @@ void route_molecules(const tblite_amd_context ctx, const std::vector<int>& ld,
   }
 }
 
+// lattice (9 per molecule) with xyz: periodic cells; is_cell (optional,
+// one flag per molecule) marks which molecules of the batch are cells (all
+// of them when null).  Cells run on the multi-workgroup SCF path, which
+// reads the interaction planes their Ewald sums fill.
 int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* mol_nat,
-              const int* num, const double* charge, const int* uhf, double acc, Plan& pl)
+              const int* num, const double* charge, const int* uhf, double acc, Plan& pl,
+              const double* xyz = nullptr, const double* lattice = nullptr,
+              const char* is_cell = nullptr)
 {
   pl = Plan();
   pl.nmol = nmol;
@@ -590,10 +723,26 @@ int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* 
       lds[m] = (nao + 3) & ~3;
     }
     route_molecules(ctx, lds, pl.mol_path);
+    if (lattice)
+      for (int m = 0; m < nmol; ++m)
+        if (!is_cell || is_cell[m]) pl.mol_path[m] = 2;
+  }
+  if (lattice && !xyz) {
+    set_error(ctx, "internal: periodic plan without coordinates");
+    return TBLITE_AMD_ERR_ARGUMENT;
+  }
+  pl.any_pbc = lattice != nullptr;
+  if (pl.any_pbc) {
+    pl.mol_ntr.assign(nmol, 0);
+    pl.mol_ng.assign(nmol, 0);
+    pl.mol_tr_off.assign(nmol, 0);
+    pl.mol_g_off.assign(nmol, 0);
+    pl.mol_ewald.assign((size_t)TA_PBC_EWALD * nmol, 0.0);
   }
   int64_t atom = 0, shell = 0, ao = 0, mat = 0, nl = 0, gam = 0, pair = 0, mix = 0, mixmem = 0,
           big = 0, scr = 0;
   std::vector<int> wd_mol, wd_kind, wd_start;  // integral work of molecules with d shells
+  std::vector<int> pwd_mol, pwd_kind, pwd_start;  // the same for periodic cells
   for (int m = 0; m < nmol; ++m) {
     const int nat = mol_nat[m];
     bool has_d = false;
@@ -638,6 +787,44 @@ int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* 
     pl.mol_uhf[m] = uhf ? uhf[m] : 0;
     pl.mol_charge[m] = charge ? charge[m] : 0.0;
     pl.mol_cutoff[m] = get_cutoff(min_alpha, acc);
+    const bool cell = lattice && (!is_cell || is_cell[m]);
+    int64_t nl_cap = (int64_t)nat * (nat + 1) / 2;
+    std::vector<int> site_cnt;
+    if (cell) {
+      CellTables ct;
+      if (!cell_tables(lattice + 9 * (size_t)m, nat, xyz + 3 * atom, ct)) {
+        set_error(ctx, "molecule %d: degenerate lattice", m);
+        return TBLITE_AMD_ERR_ARGUMENT;
+      }
+      const int64_t ntr = (int64_t)ct.tr.size() / 3;
+      if (ntr > (1 << 20)) {
+        set_error(ctx, "molecule %d: %lld lattice translations (cell too thin for the cutoffs)", m,
+                  (long long)ntr);
+        return TBLITE_AMD_ERR_TOO_LARGE;
+      }
+      pl.mol_ntr[m] = (int)ntr;
+      pl.mol_tr_off[m] = (int64_t)pl.trans.size() / 3;
+      pl.trans.insert(pl.trans.end(), ct.tr.begin(), ct.tr.end());
+      pl.mol_ng[m] = (int)(ct.g.size() / 3);
+      pl.mol_g_off[m] = (int64_t)pl.recip.size() / 3;
+      pl.recip.insert(pl.recip.end(), ct.g.begin(), ct.g.end());
+      pl.mol_ewald[TA_PBC_EWALD * m] = ct.vol;
+      pl.mol_ewald[TA_PBC_EWALD * m + 1] = ct.alpha;
+      pl.mol_ewald[TA_PBC_EWALD * m + 2] = ct.rreal;
+      cell_counts(nat, xyz + 3 * atom, ct.tr, pl.mol_cutoff[m],
+                  c.dev.d4.enabled ? c.dev.d4.disp3_cutoff : 0.0, nl_cap, site_cnt);
+      if (nl_cap > (int64_t)1 << 30) {
+        set_error(ctx, "molecule %d: neighbour list of %lld entries", m, (long long)nl_cap);
+        return TBLITE_AMD_ERR_TOO_LARGE;
+      }
+    }
+    if (pl.any_pbc)
+      for (int i = 0; i < nat; ++i) {
+        pl.site_off.push_back(pl.tot_site);
+        const int cap = cell && c.dev.d4.enabled ? site_cnt[i] : 0;
+        pl.site_cap.push_back(cap);
+        pl.tot_site += cap;
+      }
     pl.mol_mat_off[m] = mat;
     pl.mol_nl_off[m] = nl;
     pl.mol_gam_off[m] = gam;
@@ -671,13 +858,56 @@ int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* 
     shell += nsh;
     ao += nao;
     mat += (int64_t)nao * nao;
-    nl += (int64_t)nat * (nat + 1) / 2;
+    nl += nl_cap;
     gam += (int64_t)nsh * nsh;
     pair += (int64_t)nat * nat;
     mix += ndim;
     mixmem += (int64_t)ndim * pl.mem;
     // integral work: neighbour chunks over the list capacity, on-site chunks
-    {
+    // (cells: their own lists, s/p-only molecules first)
+    if (cell) {
+      std::vector<int>& wm = has_d ? pwd_mol : pl.pw_mol;
+      std::vector<int>& wk = has_d ? pwd_kind : pl.pw_kind;
+      std::vector<int>& ws = has_d ? pwd_start : pl.pw_start;
+      for (int64_t s = 0; s < nl_cap; s += ctx->int_chunk) {
+        wm.push_back(m);
+        wk.push_back(0);
+        ws.push_back((int)s);
+      }
+      for (int s = 0; s < nat; s += ctx->int_chunk) {
+        wm.push_back(m);
+        wk.push_back(1);
+        ws.push_back(s);
+      }
+      // pair planes: one thread per atom pair, rows of up to 256 pairs
+      const int rows = std::max(1, 256 / nat);
+      for (int a = 0; a < nat; a += rows) {
+        pl.pp_mol.push_back(m);
+        pl.pp_a0.push_back(a);
+        pl.pp_a1.push_back(std::min(nat, a + rows));
+      }
+      // three-body: site lists per atom, then pair items of about 2^16
+      // site pairs (rows of one anchor's sites)
+      if (c.dev.d4.enabled)
+        for (int i = 0; i < nat; ++i) {
+          pl.ps_mol.push_back(m);
+          pl.ps_atom.push_back(i);
+          const int n = site_cnt[i];
+          int r0 = 0;
+          int64_t accp = 0;
+          for (int a = 0; a < n; ++a) {
+            accp += n - a - 1;
+            if (accp >= (1 << 16) || a == n - 1) {
+              pl.pa_mol.push_back(m);
+              pl.pa_atom.push_back(i);
+              pl.pa_r0.push_back(r0);
+              pl.pa_r1.push_back(a + 1);
+              r0 = a + 1;
+              accp = 0;
+            }
+          }
+        }
+    } else {
       std::vector<int>& wm = has_d ? wd_mol : pl.work_mol;
       std::vector<int>& wk = has_d ? wd_kind : pl.work_kind;
       std::vector<int>& ws = has_d ? wd_start : pl.work_start;
@@ -694,7 +924,7 @@ int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* 
       }
     }
     // pair planes: rows of about kPairChunk pairs per work item
-    {
+    if (!cell) {
       const int rows = (int)std::max<int64_t>(1, ((int64_t)1 << ctx->pair_chunk_log2) / nat);
       for (int a = 0; a < nat; a += rows) {
         pl.pair_mol.push_back(m);
@@ -704,7 +934,7 @@ int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* 
     }
     // three-body work: anchor atom i owns the triples k < j < i; chunks of
     // consecutive anchors hold about kAtmChunk triples each
-    if (c.dev.d4.enabled) {
+    if (c.dev.d4.enabled && !cell) {
       int i0 = 0;
       int64_t acc = 0;
       for (int i = 0; i < nat; ++i) {
@@ -728,6 +958,10 @@ int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* 
   pl.work_mol.insert(pl.work_mol.end(), wd_mol.begin(), wd_mol.end());
   pl.work_kind.insert(pl.work_kind.end(), wd_kind.begin(), wd_kind.end());
   pl.work_start.insert(pl.work_start.end(), wd_start.begin(), wd_start.end());
+  pl.nwork_pbc_sp = (int)pl.pw_mol.size();
+  pl.pw_mol.insert(pl.pw_mol.end(), pwd_mol.begin(), pwd_mol.end());
+  pl.pw_kind.insert(pl.pw_kind.end(), pwd_kind.begin(), pwd_kind.end());
+  pl.pw_start.insert(pl.pw_start.end(), pwd_start.begin(), pwd_start.end());
   pl.tot_gam = gam;
   pl.tot_pair = pair;
   pl.tot_mix = mix;
@@ -848,6 +1082,27 @@ void pack_plan(const Plan& pl, std::vector<char>& img, std::vector<std::pair<siz
   host_pack(img, seg, pl.pair_mol);       // 32
   host_pack(img, seg, pl.pair_a0);        // 33
   host_pack(img, seg, pl.pair_a1);        // 34
+  host_pack(img, seg, pl.mol_ntr);        // 35
+  host_pack(img, seg, pl.mol_tr_off);     // 36
+  host_pack(img, seg, pl.trans);          // 37
+  host_pack(img, seg, pl.mol_ng);         // 38
+  host_pack(img, seg, pl.mol_g_off);      // 39
+  host_pack(img, seg, pl.recip);          // 40
+  host_pack(img, seg, pl.mol_ewald);      // 41
+  host_pack(img, seg, pl.pw_mol);         // 42
+  host_pack(img, seg, pl.pw_kind);        // 43
+  host_pack(img, seg, pl.pw_start);       // 44
+  host_pack(img, seg, pl.pp_mol);         // 45
+  host_pack(img, seg, pl.pp_a0);          // 46
+  host_pack(img, seg, pl.pp_a1);          // 47
+  host_pack(img, seg, pl.ps_mol);         // 48
+  host_pack(img, seg, pl.ps_atom);        // 49
+  host_pack(img, seg, pl.pa_mol);         // 50
+  host_pack(img, seg, pl.pa_atom);        // 51
+  host_pack(img, seg, pl.pa_r0);          // 52
+  host_pack(img, seg, pl.pa_r1);          // 53
+  host_pack(img, seg, pl.site_off);       // 54
+  host_pack(img, seg, pl.site_cap);       // 55
   for (const auto& bk : pl.buckets) host_pack(img, seg, bk.mols);
 
 }
@@ -879,6 +1134,7 @@ size_t workspace_need(const Plan& pl, size_t img_bytes, bool wfn_arrays, bool gr
   need += dbytes(std::max<int64_t>(pl.tot_scr, 1));
   need += dbytes((int64_t)nmol * TA_PROF_SLOTS);
   need += 2 * dbytes(A) + 2 * dbytes(TA_D4_MAXREF * A) + 2 * dbytes(pl.tot_pair) + 2 * dbytes(nmol); // D4
+  if (pl.any_pbc) need += ibytes(pl.tot_nl) + ibytes(A) + dbytes(4 * std::max<int64_t>(pl.tot_site, 1));
   if (grad) {
     int64_t ntiles = 0;
     for (int m = 0; m < nmol; ++m) {
@@ -984,6 +1240,29 @@ int upload_plan(tblite_amd_context ctx, Calculator& c, const Plan& pl, DeviceBat
   db.d_pair_mol = I(32);
   db.d_pair_a0 = I(33);
   db.d_pair_a1 = I(34);
+  if (pl.any_pbc) {
+    b.mol_ntr = I(35);
+    b.mol_tr_off = L(36);
+    b.trans = Dd(37);
+    b.mol_ng = I(38);
+    b.mol_g_off = L(39);
+    b.recip = Dd(40);
+    b.mol_ewald = Dd(41);
+    b.atm_site_off = L(54);
+    b.atm_site_cap = I(55);
+  }
+  db.d_pw_mol = I(42);
+  db.d_pw_kind = I(43);
+  db.d_pw_start = I(44);
+  db.d_pp_mol = I(45);
+  db.d_pp_a0 = I(46);
+  db.d_pp_a1 = I(47);
+  db.d_ps_mol = I(48);
+  db.d_ps_atom = I(49);
+  db.d_pa_mol = I(50);
+  db.d_pa_atom = I(51);
+  db.d_pa_r0 = I(52);
+  db.d_pa_r1 = I(53);
   db.d_bucket_mols.clear();
   for (size_t k = 0; k < pl.buckets.size(); ++k) db.d_bucket_mols.push_back(I(nfixed + k));
 
@@ -1041,6 +1320,11 @@ int upload_plan(tblite_amd_context ctx, Calculator& c, const Plan& pl, DeviceBat
   b.d4_c60 = carve<double>(ar, pl.tot_pair);
   b.edisp2 = carve<double>(ar, nmol);
   b.edisp3 = carve<double>(ar, nmol);
+  if (pl.any_pbc) {
+    b.nl_itr = carve<int>(ar, std::max<int64_t>(pl.tot_nl, 1));
+    b.atm_site_n = carve<int>(ar, A);
+    b.atm_site = carve<double>(ar, 4 * std::max<int64_t>(pl.tot_site, 1));
+  }
   b.intcut = 0.0;
   b.jacobi_tol = ctx->jacobi_tol;
   b.jacobi_skip = ctx->jacobi_skip;
@@ -1258,8 +1542,21 @@ int run_batch(tblite_amd_context ctx, Calculator& c, const double* xyz, double a
     HIP_TRY(ctx, hipEventRecord(ctx->ev[5], st));
     HIP_TRY(ctx, hipStreamWaitEvent(ps, ctx->ev[5], 0));
   }
-  hipLaunchKernelGGL(k_setup_pairs, dim3((int)pl.pair_mol.size()), dim3(256), 0, ps, b,
-                     db.d_pair_mol, db.d_pair_a0, db.d_pair_a1);
+  if (!pl.pair_mol.empty())
+    hipLaunchKernelGGL(k_setup_pairs, dim3((int)pl.pair_mol.size()), dim3(256), 0, ps, b,
+                       db.d_pair_mol, db.d_pair_a0, db.d_pair_a1);
+  HIP_TRY(ctx, hipGetLastError());
+  // periodic cells: Ewald-summed ES2/AES2 and image-summed D4 planes, then
+  // the three-body sites and energies (these need the zero-charge C6)
+  if (!pl.pp_mol.empty())
+    hipLaunchKernelGGL(k_pbc_pairs, dim3((int)pl.pp_mol.size()), dim3(256), 0, ps, b,
+                       db.d_pp_mol, db.d_pp_a0, db.d_pp_a1);
+  if (!pl.ps_mol.empty())
+    hipLaunchKernelGGL(k_pbc_atm_sites, dim3((int)pl.ps_mol.size()), dim3(256), 0, ps, b,
+                       db.d_ps_mol, db.d_ps_atom);
+  if (!pl.pa_mol.empty())
+    hipLaunchKernelGGL(k_pbc_atm, dim3((int)pl.pa_mol.size()), dim3(256), 0, ps, b, db.d_pa_mol,
+                       db.d_pa_atom, db.d_pa_r0, db.d_pa_r1);
   HIP_TRY(ctx, hipGetLastError());
   // three-body dispersion (writes every atom's d4_e3 entry; needs the D4 CN
   // of k_setup and the zero-charge C6 planes of k_setup_pairs)
@@ -1276,11 +1573,18 @@ int run_batch(tblite_amd_context ctx, Calculator& c, const double* xyz, double a
   HIP_TRY(ctx, hipEventRecord(ctx->ev[2], st));
   const int nwork = (int)pl.work_mol.size(), nsp = pl.nwork_sp;
   if (nsp > 0)
-    hipLaunchKernelGGL(k_integrals<1>, dim3(nsp), dim3(pl.int_chunk), 0, st, b, db.d_work_mol,
-                       db.d_work_kind, db.d_work_start);
+    hipLaunchKernelGGL((k_integrals<1, false>), dim3(nsp), dim3(pl.int_chunk), 0, st, b,
+                       db.d_work_mol, db.d_work_kind, db.d_work_start);
   if (nwork > nsp)
-    hipLaunchKernelGGL(k_integrals<2>, dim3(nwork - nsp), dim3(pl.int_chunk), 0, st, b,
+    hipLaunchKernelGGL((k_integrals<2, false>), dim3(nwork - nsp), dim3(pl.int_chunk), 0, st, b,
                        db.d_work_mol + nsp, db.d_work_kind + nsp, db.d_work_start + nsp);
+  const int npw = (int)pl.pw_mol.size(), npsp = pl.nwork_pbc_sp;
+  if (npsp > 0)
+    hipLaunchKernelGGL((k_integrals<1, true>), dim3(npsp), dim3(pl.int_chunk), 0, st, b,
+                       db.d_pw_mol, db.d_pw_kind, db.d_pw_start);
+  if (npw > npsp)
+    hipLaunchKernelGGL((k_integrals<2, true>), dim3(npw - npsp), dim3(pl.int_chunk), 0, st, b,
+                       db.d_pw_mol + npsp, db.d_pw_kind + npsp, db.d_pw_start + npsp);
   HIP_TRY(ctx, hipGetLastError());
   HIP_TRY(ctx, hipEventRecord(ctx->ev[3], st));
   // SCF buckets on the auxiliary streams, largest bucket first; on a
@@ -1443,11 +1747,16 @@ int copy_gradients(tblite_amd_context ctx, int imol, double* gradients, double* 
 int flat_singlepoint(tblite_amd_context ctx, Calculator& c, int nmol, const int* mol_nat,
                      const int* num, const double* xyz, const double* charge, const int* uhf,
                      double acc, double* energies, int* iterations, int* status,
-                     bool wfn_arrays = false, bool grad = false)
+                     bool wfn_arrays = false, bool grad = false, const double* lattice = nullptr,
+                     const char* is_cell = nullptr)
 {
   if (!ctx) return TBLITE_AMD_ERR_ARGUMENT;
   if (nmol < 1 || !mol_nat || !num || !xyz || !energies) {
     set_error(ctx, "invalid arguments to batched singlepoint");
+    return TBLITE_AMD_ERR_ARGUMENT;
+  }
+  if (lattice && grad) {
+    set_error(ctx, "analytic gradients of periodic cells are not available");
     return TBLITE_AMD_ERR_ARGUMENT;
   }
   HIP_TRY(ctx, hipSetDevice(ctx->device));
@@ -1455,7 +1764,7 @@ int flat_singlepoint(tblite_amd_context ctx, Calculator& c, int nmol, const int*
   if (rc) return rc;
   ctx->have_batch = false;
   ctx->prepared = nullptr;
-  rc = make_plan(ctx, c, nmol, mol_nat, num, charge, uhf, acc, ctx->plan);
+  rc = make_plan(ctx, c, nmol, mol_nat, num, charge, uhf, acc, ctx->plan, xyz, lattice, is_cell);
   if (rc) return rc;
   rc = upload_plan(ctx, c, ctx->plan, ctx->batch, wfn_arrays, nullptr, nullptr, grad);
   if (rc) return rc;
@@ -1634,6 +1943,46 @@ tblite_amd_structure tblite_amd_new_structure(tblite_amd_context ctx, int nat, c
   s->xyz.assign(xyz, xyz + 3 * nat);
   s->charge = charge;
   s->uhf = uhf;
+  return s;
+}
+
+tblite_amd_structure tblite_amd_new_structure_periodic(tblite_amd_context ctx, int nat,
+                                                       const int* num, const double* xyz,
+                                                       double charge, int uhf,
+                                                       const double* lattice,
+                                                       const int* periodic)
+{
+  if (!ctx) return nullptr;
+  int np = 0;
+  for (int k = 0; periodic && k < 3; ++k) np += periodic[k] ? 1 : 0;
+  if (np != 0 && np != 3) {
+    set_error(ctx, "only three-dimensional periodicity is supported (periodic = %d %d %d)",
+              periodic[0], periodic[1], periodic[2]);
+    return nullptr;
+  }
+  if (np == 3 && !lattice) {
+    set_error(ctx, "periodic structure without lattice vectors");
+    return nullptr;
+  }
+  if (np == 3) {
+    for (int k = 0; k < 9; ++k)
+      if (!std::isfinite(lattice[k])) {
+        set_error(ctx, "non-finite lattice vector");
+        return nullptr;
+      }
+    CellTables ct;
+    std::vector<double> probe(3, 0.0);
+    if (!cell_tables(lattice, 1, probe.data(), ct)) {
+      set_error(ctx, "degenerate lattice (zero cell volume)");
+      return nullptr;
+    }
+  }
+  tblite_amd_structure s = tblite_amd_new_structure(ctx, nat, num, xyz, charge, uhf);
+  if (!s) return nullptr;
+  if (np == 3) {
+    s->periodic = true;
+    std::memcpy(s->lat, lattice, sizeof(double) * 9);
+  }
   return s;
 }
 
@@ -1820,7 +2169,8 @@ int tblite_amd_xtb_singlepoint(tblite_amd_context ctx, tblite_amd_structure mol,
   const double acc = accuracy > 0.0 ? accuracy : calc->c.accuracy;
   int st = 0, it = 0;
   int rc = flat_singlepoint(ctx, calc->c, 1, &mol->nat, mol->num.data(), mol->xyz.data(),
-                            &mol->charge, &mol->uhf, acc, energy, &it, &st, wfn != nullptr);
+                            &mol->charge, &mol->uhf, acc, energy, &it, &st, wfn != nullptr, false,
+                            mol->periodic ? mol->lat : nullptr);
   if (iterations) *iterations = it;
   if (ctx->have_batch && wfn) {
     int rc2 = fill_wavefunction(ctx, 0, wfn);
@@ -1845,11 +2195,18 @@ int tblite_amd_xtb_singlepoint_batch(tblite_amd_context ctx, int nmol,
 {
   if (!ctx || nmol < 1 || !mols || !calc || !energies) return TBLITE_AMD_ERR_ARGUMENT;
   std::vector<int> mol_nat(nmol), num, uhf(nmol);
-  std::vector<double> xyz, charge(nmol);
+  std::vector<double> xyz, charge(nmol), lat(9 * (size_t)nmol, 0.0);
+  std::vector<char> is_cell(nmol, 0);
+  bool any_cell = false;
   for (int m = 0; m < nmol; ++m) {
     if (!mols[m]) {
       set_error(ctx, "structure %d is NULL", m);
       return TBLITE_AMD_ERR_ARGUMENT;
+    }
+    if (mols[m]->periodic) {
+      any_cell = true;
+      is_cell[m] = 1;
+      std::memcpy(lat.data() + 9 * (size_t)m, mols[m]->lat, sizeof(double) * 9);
     }
     mol_nat[m] = mols[m]->nat;
     num.insert(num.end(), mols[m]->num.begin(), mols[m]->num.end());
@@ -1861,7 +2218,9 @@ int tblite_amd_xtb_singlepoint_batch(tblite_amd_context ctx, int nmol,
   bool any_wfn = false;
   for (int m = 0; wfns && m < nmol; ++m) any_wfn = any_wfn || wfns[m] != nullptr;
   int rc = flat_singlepoint(ctx, calc->c, nmol, mol_nat.data(), num.data(), xyz.data(),
-                            charge.data(), uhf.data(), acc, energies, iterations, status, any_wfn);
+                            charge.data(), uhf.data(), acc, energies, iterations, status, any_wfn,
+                            false, any_cell ? lat.data() : nullptr,
+                            any_cell ? is_cell.data() : nullptr);
   if (ctx->have_batch && wfns)
     for (int m = 0; m < nmol; ++m)
       if (wfns[m]) {
@@ -1880,6 +2239,27 @@ int tblite_amd_singlepoint_flat(tblite_amd_context ctx, tblite_amd_calculator ca
   const double acc = accuracy > 0.0 ? accuracy : calc->c.accuracy;
   return flat_singlepoint(ctx, calc->c, nmol, mol_nat, num, xyz, charge, uhf, acc, energies,
                           iterations, status);
+}
+
+int tblite_amd_singlepoint_flat_periodic(tblite_amd_context ctx, tblite_amd_calculator calc,
+                                         int nmol, const int* mol_nat, const int* num,
+                                         const double* xyz, const double* lattice,
+                                         const double* charge, const int* uhf, double accuracy,
+                                         double* energies, int* iterations, int* status)
+{
+  if (!ctx || !calc) return TBLITE_AMD_ERR_ARGUMENT;
+  if (!lattice) {
+    set_error(ctx, "periodic batch without lattice vectors");
+    return TBLITE_AMD_ERR_ARGUMENT;
+  }
+  for (int k = 0; k < 9 * nmol; ++k)
+    if (!std::isfinite(lattice[k])) {
+      set_error(ctx, "non-finite lattice vector");
+      return TBLITE_AMD_ERR_ARGUMENT;
+    }
+  const double acc = accuracy > 0.0 ? accuracy : calc->c.accuracy;
+  return flat_singlepoint(ctx, calc->c, nmol, mol_nat, num, xyz, charge, uhf, acc, energies,
+                          iterations, status, false, false, lattice);
 }
 
 int tblite_amd_singlepoint_flat_gradient(tblite_amd_context ctx, tblite_amd_calculator calc,
@@ -1907,6 +2287,10 @@ int tblite_amd_xtb_singlepoint_gradient(tblite_amd_context ctx, tblite_amd_struc
                                         double* sigma, int* iterations, int verbosity)
 {
   if (!ctx || !mol || !calc || !energy || !gradient) return TBLITE_AMD_ERR_ARGUMENT;
+  if (mol->periodic) {
+    set_error(ctx, "analytic gradients of periodic cells are not available");
+    return TBLITE_AMD_ERR_ARGUMENT;
+  }
   const double acc = accuracy > 0.0 ? accuracy : calc->c.accuracy;
   int st = 0, it = 0;
   int rc = flat_singlepoint(ctx, calc->c, 1, &mol->nat, mol->num.data(), mol->xyz.data(),
@@ -2166,6 +2550,36 @@ int tblite_amd_batch_get_adjlist(tblite_amd_context ctx, int imol, int* inl, int
                              hipMemcpyDeviceToHost));
   }
   return nnz;
+}
+
+int tblite_amd_batch_get_images(tblite_amd_context ctx, int imol, int* nltr, int nltr_capacity,
+                                double* trans, int trans_capacity)
+{
+  int rc = check_batch(ctx, imol);
+  if (rc) return rc;
+  const Plan& pl = ctx->plan;
+  if (!pl.any_pbc || pl.mol_ntr[imol] == 0) return 0;
+  const BatchDev& b = ctx->batch.b;
+  const int ntr = pl.mol_ntr[imol];
+  if (trans) {
+    if (trans_capacity < ntr) {
+      set_error(ctx, "translation capacity %d < %d", trans_capacity, ntr);
+      return TBLITE_AMD_ERR_ARGUMENT;
+    }
+    std::memcpy(trans, pl.trans.data() + 3 * pl.mol_tr_off[imol], sizeof(double) * 3 * ntr);
+  }
+  if (nltr) {
+    int nnz = 0;
+    HIP_TRY(ctx, hipMemcpy(&nnz, b.mol_nnz + imol, sizeof(int), hipMemcpyDeviceToHost));
+    if (nltr_capacity < nnz) {
+      set_error(ctx, "neighbour-list capacity %d < %d", nltr_capacity, nnz);
+      return TBLITE_AMD_ERR_ARGUMENT;
+    }
+    if (nnz > 0)
+      HIP_TRY(ctx, hipMemcpy(nltr, b.nl_itr + pl.mol_nl_off[imol], 4 * (size_t)nnz,
+                             hipMemcpyDeviceToHost));
+  }
+  return ntr;
 }
 
 int tblite_amd_batch_get_integrals(tblite_amd_context ctx, int imol, double* overlap,

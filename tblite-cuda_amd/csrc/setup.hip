@@ -37,6 +37,38 @@ __device__ __forceinline__ double exp_count(double k, double r, double r0)
   return 1.0 / (1.0 + exp(-k * (r0 / r - 1.0)));
 }
 
+// CN, D4 CN and repulsion contributions of one partner at squared distance r2
+__device__ __forceinline__ void setup_pair_terms(const DevParams& P, const DevElem& ei,
+                                                 const DevElem& ej, double r2, double cn_cut2,
+                                                 double d4_cut2, double rep_cut2, double& cn,
+                                                 double& cn4, double& er)
+{
+  const DevD4& d4 = P.d4;
+  double r1 = sqrt(r2);
+  if (r2 <= cn_cut2) {
+    double rc = ei.rcov + ej.rcov;
+    cn += exp_count(P.cn_ka, r1, rc) * exp_count(P.cn_kb, r1, rc + P.cn_rshift);
+  }
+  if (d4.enabled && r2 <= d4_cut2) {
+    // D4 CN: electronegativity-weighted error-function count
+    const double rc = ei.rcov + ej.rcov;
+    const double den = fabs(ei.en - ej.en) + d4.k5;
+    cn4 += d4.k4 * exp(-den * den / d4.k6) * 0.5 * (1.0 + erf(-d4.kn * (r1 - rc) / rc));
+  }
+  if (r2 <= rep_cut2) {
+    double alpha = sqrt(ei.arep * ej.arep);
+    double zeff = ei.zeff * ej.zeff;
+    double kexp = (ei.z <= 2 && ej.z <= 2) ? P.rep_kexp_light : P.rep_kexp;
+    double r1k = pow(r1, kexp);
+    double r1r = pow(r1, P.rep_rexp);
+    er += 0.5 * zeff * exp(-alpha * r1k) / r1r;
+  }
+}
+
+// For a periodic cell (mol_ntr > 0) the partners are every image (j, T) other
+// than the atom itself within each cutoff (ncoord_dexp's trans(:, itr) loop,
+// src/coulomb.cu:212-213 text); its three-body energies are summed later by
+// k_pbc_atm and start from zero here.
 __global__ __launch_bounds__(SETUP_THREADS) void k_setup(BatchDev b)
 {
   const int m = blockIdx.x;
@@ -55,31 +87,30 @@ __global__ __launch_bounds__(SETUP_THREADS) void k_setup(BatchDev b)
     const DevElem& ei = P.el[el[i]];
     const double xi = xyz[3 * i], yi = xyz[3 * i + 1], zi = xyz[3 * i + 2];
     double cn = 0.0, er = 0.0, cn4 = 0.0;
-    for (int j = 0; j < nat; ++j) {
-      if (j == i) continue;
-      const DevElem& ej = P.el[el[j]];
-      double dx = xi - xyz[3 * j], dy = yi - xyz[3 * j + 1], dz = zi - xyz[3 * j + 2];
-      double r2 = dx * dx + dy * dy + dz * dz;
-      if (r2 < 1.0e-12) continue;
-      double r1 = sqrt(r2);
-      if (r2 <= cn_cut2) {
-        double rc = ei.rcov + ej.rcov;
-        cn += exp_count(P.cn_ka, r1, rc) * exp_count(P.cn_kb, r1, rc + P.cn_rshift);
+    const int ntr = b.mol_ntr ? b.mol_ntr[m] : 0;
+    if (ntr == 0) {
+      for (int j = 0; j < nat; ++j) {
+        if (j == i) continue;
+        const DevElem& ej = P.el[el[j]];
+        double dx = xi - xyz[3 * j], dy = yi - xyz[3 * j + 1], dz = zi - xyz[3 * j + 2];
+        double r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 < 1.0e-12) continue;
+        setup_pair_terms(P, ei, ej, r2, cn_cut2, d4_cut2, rep_cut2, cn, cn4, er);
       }
-      if (d4.enabled && r2 <= d4_cut2) {
-        // D4 CN: electronegativity-weighted error-function count
-        const double rc = ei.rcov + ej.rcov;
-        const double den = fabs(ei.en - ej.en) + d4.k5;
-        cn4 += d4.k4 * exp(-den * den / d4.k6) * 0.5 * (1.0 + erf(-d4.kn * (r1 - rc) / rc));
+    } else {
+      const double* __restrict__ tr = b.trans + 3 * b.mol_tr_off[m];
+      const double cut2 = fmax(cn_cut2, fmax(rep_cut2, d4.enabled ? d4_cut2 : 0.0));
+      for (int j = 0; j < nat; ++j) {
+        const DevElem& ej = P.el[el[j]];
+        const double dx0 = xi - xyz[3 * j], dy0 = yi - xyz[3 * j + 1], dz0 = zi - xyz[3 * j + 2];
+        for (int t = 0; t < ntr; ++t) {
+          const double dx = dx0 - tr[3 * t], dy = dy0 - tr[3 * t + 1], dz = dz0 - tr[3 * t + 2];
+          const double r2 = dx * dx + dy * dy + dz * dz;
+          if (r2 < 1.0e-12 || r2 > cut2) continue;
+          setup_pair_terms(P, ei, ej, r2, cn_cut2, d4_cut2, rep_cut2, cn, cn4, er);
+        }
       }
-      if (r2 <= rep_cut2) {
-        double alpha = sqrt(ei.arep * ej.arep);
-        double zeff = ei.zeff * ej.zeff;
-        double kexp = (ei.z <= 2 && ej.z <= 2) ? P.rep_kexp_light : P.rep_kexp;
-        double r1k = pow(r1, kexp);
-        double r1r = pow(r1, P.rep_rexp);
-        er += 0.5 * zeff * exp(-alpha * r1k) / r1r;
-      }
+      b.d4_e3[a0 + i] = 0.0;
     }
     b.cn[a0 + i] = cn;
     b.erep[a0 + i] = er;
@@ -276,10 +307,75 @@ __device__ __forceinline__ double pair_r2_exact(const double* __restrict__ xyz, 
   return s + zz;
 }
 
+// (R_i - R_j) - T with the same operation sequence as pair_r2_exact
+__device__ __forceinline__ double pair_r2_image_exact(const double* __restrict__ xyz, int i, int j,
+                                                      const double* __restrict__ t)
+{
+#pragma clang fp contract(off)
+  double dx = (xyz[3 * i] - xyz[3 * j]) - t[0];
+  double dy = (xyz[3 * i + 1] - xyz[3 * j + 1]) - t[1];
+  double dz = (xyz[3 * i + 2] - xyz[3 * j + 2]) - t[2];
+  double xx = dx * dx;
+  double yy = dy * dy;
+  double zz = dz * dz;
+  double s = xx + yy;
+  return s + zz;
+}
+
+// Neighbour list of a periodic cell: entries (jat, itr), jat <= iat, every
+// translation itr in table order (generate, src/adjlist.cu:26-30 text).  One
+// thread per atom, block scan per chunk of atoms as for molecules.
+__device__ void adjlist_cell(const BatchDev& b, int m, int* red)
+{
+  const int tid = threadIdx.x;
+  const int a0 = b.mol_atom_off[m], nat = b.mol_nat[m];
+  const int ntr = b.mol_ntr[m];
+  const double* __restrict__ xyz = b.xyz + 3 * (int64_t)a0;
+  const double* __restrict__ tr = b.trans + 3 * b.mol_tr_off[m];
+  const double cutoff = b.mol_cutoff[m];
+  const double cutoff2 = cutoff * cutoff;
+  const double eps = 2.220446049250313e-16;
+  int* __restrict__ jat_out = b.nl_jat + b.mol_nl_off[m];
+  int* __restrict__ itr_out = b.nl_itr + b.mol_nl_off[m];
+  int base = 0;
+  for (int chunk = 0; chunk < nat; chunk += blockDim.x) {
+    const int i = chunk + tid;
+    int cnt = 0;
+    if (i < nat)
+      for (int j = 0; j <= i; ++j)
+        for (int t = 0; t < ntr; ++t) {
+          double r2 = pair_r2_image_exact(xyz, i, j, tr + 3 * t);
+          if (!(r2 < eps || r2 > cutoff2)) ++cnt;
+        }
+    int tot;
+    int excl = block_exclusive_scan(cnt, red, &tot);
+    if (i < nat) {
+      b.nl_inl[a0 + i] = base + excl;
+      b.nl_nnl[a0 + i] = cnt;
+      int pos = base + excl;
+      for (int j = 0; j <= i; ++j)
+        for (int t = 0; t < ntr; ++t) {
+          double r2 = pair_r2_image_exact(xyz, i, j, tr + 3 * t);
+          if (!(r2 < eps || r2 > cutoff2)) {
+            jat_out[pos] = j;
+            itr_out[pos] = t;
+            ++pos;
+          }
+        }
+    }
+    base += tot;
+  }
+  if (tid == 0) b.mol_nnz[m] = base;
+}
+
 __global__ __launch_bounds__(SETUP_THREADS) void k_adjlist(BatchDev b)
 {
   __shared__ int red[SETUP_THREADS / TA_WAVE];
   const int m = blockIdx.x;
+  if (b.mol_ntr && b.mol_ntr[m] > 0) {
+    adjlist_cell(b, m, red);
+    return;
+  }
   const int tid = threadIdx.x;
   const int a0 = b.mol_atom_off[m], nat = b.mol_nat[m];
   const double* __restrict__ xyz = b.xyz + 3 * (int64_t)a0;
@@ -555,11 +651,25 @@ __device__ void shell_block(int li, int lj, const DevElem& ej, int jsh, const De
 #define WORK_PAIRS 0
 #define WORK_ONSITE 1
 
+// element store of k_integrals: plain for molecules (one thread per matrix
+// element), atomic accumulation for periodic cells (the images of one atom
+// pair, and an atom's own images next to its on-site block, share elements)
+template <bool ADD>
+__device__ __forceinline__ void put_elem(double* p, double v)
+{
+  if constexpr (ADD) atomicAdd(p, v);
+  else *p = v;
+}
+
 // get_hamiltonian (src/h0.cu:306-477 text) plus the on-site blocks.  One
-// thread per atom pair (neighbour-list entry) or per atom (on-site); each
-// thread writes a disjoint set of matrix elements, so no atomics.  LMAX = 1
-// serves molecules without d shells (the host splits the work list by it).
-template <int LMAX>
+// thread per atom pair (neighbour-list entry) or per atom (on-site); for a
+// molecule each thread writes a disjoint set of matrix elements, so no
+// atomics.  LMAX = 1 serves molecules without d shells (the host splits the
+// work list by it).  PBC: periodic cells, whose list entries carry a
+// translation (vec = R_i - R_j - T, src/h0.cu:359 text); the blocks of all
+// images of a pair add into one Gamma-point block, and an image of the atom
+// itself adds its block once, without the mirrored copy.
+template <int LMAX, bool PBC>
 __global__ __launch_bounds__(128) void k_integrals(BatchDev b, const int* __restrict__ work_mol,
                                                  const int* __restrict__ work_kind,
                                                  const int* __restrict__ work_start)
@@ -583,6 +693,7 @@ __global__ __launch_bounds__(128) void k_integrals(BatchDev b, const int* __rest
   const double intcut = b.intcut;
 
   int iat, jat;
+  const double* tvec = nullptr;
   if (kind == WORK_PAIRS) {
     if (item >= b.mol_nnz[m]) return;
     // locate the owning atom: largest iat with inl(iat) <= item
@@ -597,6 +708,8 @@ __global__ __launch_bounds__(128) void k_integrals(BatchDev b, const int* __rest
     while (lo < nat - 1 && inl[lo + 1] <= item) ++lo;
     iat = lo;
     jat = b.nl_jat[b.mol_nl_off[m] + item];
+    if constexpr (PBC)
+      tvec = b.trans + 3 * (b.mol_tr_off[m] + b.nl_itr[b.mol_nl_off[m] + item]);
   } else {
     if (item >= nat) return;
     iat = jat = item;
@@ -633,8 +746,13 @@ __global__ __launch_bounds__(128) void k_integrals(BatchDev b, const int* __rest
   }
   double vec[3];
   for (int k = 0; k < 3; ++k) vec[k] = xyz[3 * iat + k] - xyz[3 * jat + k];
+  if constexpr (PBC)
+    if (tvec)
+      for (int k = 0; k < 3; ++k) vec[k] -= tvec[k];
   const double r2 = vec[0] * vec[0] + vec[1] * vec[1] + vec[2] * vec[2];
   const double rr = sqrt(sqrt(r2) / (ej.atomic_rad + ei.atomic_rad));
+  const bool pair = PBC ? (kind == WORK_PAIRS) : (iat != jat);
+  const bool mirror = iat != jat;
 
   constexpr int NB = (2 * LMAX + 1) * (2 * LMAX + 1);  // largest shell block
   double st[NB], dt[3 * NB], qt[6 * NB];
@@ -646,7 +764,7 @@ __global__ __launch_bounds__(128) void k_integrals(BatchDev b, const int* __rest
       const int jj = b.sh_iao[s0 + js + jsh];
       shell_block<LMAX>(li, lj, ej, jsh, ei, ish, r2, vec, intcut, st, dt, qt);
       const int si = 2 * li + 1, sj = 2 * lj + 1;
-      if (iat != jat) {
+      if (pair) {
         const double shp = (1.0 + ei.shpoly[ish] * rr) * (1.0 + ej.shpoly[jsh] * rr);
         const double hij =
             0.5 * (se[is + ish] + se[js + jsh]) * P.hscale[ei_][ej_][ish][jsh] * shp;
@@ -676,17 +794,15 @@ __global__ __launch_bounds__(128) void k_integrals(BatchDev b, const int* __rest
             qj[5] = qi[5] + 1.5 * t5 - tr;
             const int64_t a_ij = (int64_t)(ii + iao) * n + (jj + jao); // fortran (jj+jao, ii+iao)
             const int64_t a_ji = (int64_t)(jj + jao) * n + (ii + iao); // fortran (ii+iao, jj+jao)
-            S[a_ij] = sv;
-            S[a_ji] = sv;
-            H[a_ij] = sv * hij;
-            H[a_ji] = sv * hij;
-            for (int k = 0; k < 3; ++k) {
-              D[k * n2 + a_ij] = di[k];
-              D[k * n2 + a_ji] = dj[k];
-            }
-            for (int k = 0; k < 6; ++k) {
-              Q[k * n2 + a_ij] = qi[k];
-              Q[k * n2 + a_ji] = qj[k];
+            put_elem<PBC>(S + a_ij, sv);
+            put_elem<PBC>(H + a_ij, sv * hij);
+            for (int k = 0; k < 3; ++k) put_elem<PBC>(D + k * n2 + a_ij, di[k]);
+            for (int k = 0; k < 6; ++k) put_elem<PBC>(Q + k * n2 + a_ij, qi[k]);
+            if (!PBC || mirror) {
+              put_elem<PBC>(S + a_ji, sv);
+              put_elem<PBC>(H + a_ji, sv * hij);
+              for (int k = 0; k < 3; ++k) put_elem<PBC>(D + k * n2 + a_ji, dj[k]);
+              for (int k = 0; k < 6; ++k) put_elem<PBC>(Q + k * n2 + a_ji, qj[k]);
             }
           }
       } else {
@@ -694,15 +810,18 @@ __global__ __launch_bounds__(128) void k_integrals(BatchDev b, const int* __rest
           for (int jao = 0; jao < sj; ++jao) {
             const int ij = jao + sj * iao;
             const int64_t a_ij = (int64_t)(ii + iao) * n + (jj + jao);
-            S[a_ij] = st[ij];
-            H[a_ij] = (ish == jsh && iao == jao) ? se[is + ish] : 0.0;
-            for (int k = 0; k < 3; ++k) D[k * n2 + a_ij] = dt[3 * ij + k];
-            for (int k = 0; k < 6; ++k) Q[k * n2 + a_ij] = qt[6 * ij + k];
+            put_elem<PBC>(S + a_ij, st[ij]);
+            const double hv = (ish == jsh && iao == jao) ? se[is + ish] : 0.0;
+            if (!PBC || hv != 0.0) put_elem<PBC>(H + a_ij, hv);
+            for (int k = 0; k < 3; ++k) put_elem<PBC>(D + k * n2 + a_ij, dt[3 * ij + k]);
+            for (int k = 0; k < 6; ++k) put_elem<PBC>(Q + k * n2 + a_ij, qt[6 * ij + k]);
           }
       }
     }
   }
 }
 
-template __global__ void k_integrals<1>(BatchDev, const int*, const int*, const int*);
-template __global__ void k_integrals<2>(BatchDev, const int*, const int*, const int*);
+template __global__ void k_integrals<1, false>(BatchDev, const int*, const int*, const int*);
+template __global__ void k_integrals<2, false>(BatchDev, const int*, const int*, const int*);
+template __global__ void k_integrals<1, true>(BatchDev, const int*, const int*, const int*);
+template __global__ void k_integrals<2, true>(BatchDev, const int*, const int*, const int*);

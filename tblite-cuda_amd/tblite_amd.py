@@ -98,6 +98,10 @@ _PROTOS = {
     "tblite_amd_delete_context": (None, [C.POINTER(_P)]),
     "tblite_amd_get_error": (_I, [_P, C.c_char_p, _I]),
     "tblite_amd_new_structure": (_P, [_P, _I, _IP, _DP, _D, _I]),
+    "tblite_amd_new_structure_periodic": (_P, [_P, _I, _IP, _DP, _D, _I, _DP, _IP]),
+    "tblite_amd_singlepoint_flat_periodic": (_I, [_P, _P, _I, _IP, _IP, _DP, _DP, _DP, _IP, _D, _DP,
+                                                  _IP, _IP]),
+    "tblite_amd_batch_get_images": (_I, [_P, _I, _IP, _I, _DP, _I]),
     "tblite_amd_delete_structure": (None, [C.POINTER(_P)]),
     "tblite_amd_get_structure_dims": (_I, [_P, _P, _IP, _IP, _IP]),
     "tblite_amd_new_gfn2_calculator": (_P, [_P, C.POINTER(Param)]),
@@ -301,6 +305,36 @@ class Engine:
         self._check(rc, allow=(-3,))
         return energies, iters, status
 
+    def singlepoint_flat_periodic(self, mol_nat, num, xyz, lattice, charge=None, uhf=None,
+                                  accuracy=None):
+        """Batch of periodic cells; lattice (nmol, 3, 3), rows = lattice vectors (bohr)."""
+        mol_nat = np.ascontiguousarray(mol_nat, dtype=np.int32)
+        num = np.ascontiguousarray(num, dtype=np.int32)
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        lat = np.ascontiguousarray(lattice, dtype=np.float64).reshape(-1)
+        nmol = len(mol_nat)
+        self._nmol = nmol
+        energies = np.zeros(nmol)
+        iters = np.zeros(nmol, dtype=np.int32)
+        status = np.zeros(nmol, dtype=np.int32)
+        ch = None if charge is None else np.ascontiguousarray(charge, dtype=np.float64)
+        uh = None if uhf is None else np.ascontiguousarray(uhf, dtype=np.int32)
+        rc = self.lib.tblite_amd_singlepoint_flat_periodic(
+            self.ctx, self.calc, nmol, _ip(mol_nat), _ip(num), _dp(xyz), _dp(lat), _dp(ch), _ip(uh),
+            self.accuracy if accuracy is None else accuracy, _dp(energies), _ip(iters), _ip(status))
+        self._check(rc, allow=(-3,))
+        return energies, iters, status
+
+    def images(self, imol):
+        """Translation index of every neighbour-list entry and the translation table of cell imol."""
+        ntr = self._check_nonneg(self.lib.tblite_amd_batch_get_images(self.ctx, imol, None, 0, None, 0))
+        nnz = self._check_nonneg(self.lib.tblite_amd_batch_get_adjlist(self.ctx, imol, None, None, None, 0))
+        nltr = np.zeros(max(nnz, 1), dtype=np.int32)
+        tr = np.zeros((max(ntr, 1), 3))
+        self._check_nonneg(self.lib.tblite_amd_batch_get_images(self.ctx, imol, _ip(nltr), len(nltr),
+                                                                _dp(tr), len(tr)))
+        return nltr[:nnz], tr[:ntr]
+
     def singlepoint_flat_gradient(self, mol_nat, num, xyz, charge=None, uhf=None, accuracy=None):
         """Batched singlepoint with analytic gradients: energies, gradients
         (total atoms, 3) dE/dR in Eh/bohr, sigmas (nmol, 3, 3) strain
@@ -372,7 +406,7 @@ class Engine:
         return e, it, st
 
     def singlepoint(self, num, xyz, charge=0.0, uhf=0, accuracy=None, wavefunction=True,
-                    gradient=False):
+                    gradient=False, lattice=None):
         """One molecule through tblite_amd_xtb_singlepoint (the per-molecule path).
 
         With wavefunction=True a wavefunction handle is passed and every array
@@ -381,10 +415,17 @@ class Engine:
         with False the call is energy-only (no handle, as a caller that wants
         the energy alone would make it).  gradient=True calls
         tblite_amd_xtb_singlepoint_gradient instead and also returns gradient
-        (nat, 3) in Eh/bohr and sigma (3, 3)."""
+        (nat, 3) in Eh/bohr and sigma (3, 3).  lattice (3, 3), rows = lattice
+        vectors in bohr, makes the structure a periodic cell."""
         num = np.ascontiguousarray(num, dtype=np.int32)
         xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
-        mol = self.lib.tblite_amd_new_structure(self.ctx, len(num), _ip(num), _dp(xyz), charge, uhf)
+        if lattice is None:
+            mol = self.lib.tblite_amd_new_structure(self.ctx, len(num), _ip(num), _dp(xyz), charge, uhf)
+        else:
+            lat = np.ascontiguousarray(lattice, dtype=np.float64).reshape(9)
+            per = np.ones(3, dtype=np.int32)
+            mol = self.lib.tblite_amd_new_structure_periodic(self.ctx, len(num), _ip(num), _dp(xyz),
+                                                             charge, uhf, _dp(lat), _ip(per))
         if not mol:
             raise EngineError(self.error())
         wfn = self.lib.tblite_amd_new_wavefunction(self.ctx) if wavefunction else None

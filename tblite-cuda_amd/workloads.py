@@ -281,8 +281,8 @@ def water_box(n, rng, density=LIQUID_DENSITY, shear=0.0):
     return (np.array(num, dtype=np.int32), np.array(xyz) * AATOAU, lat * AATOAU)
 
 
-def periodic_cells(count=4, seed=5):
-    """Seeded water cells: 4, 6, 8, ... molecules at liquid density, every
-    other one triclinic (shear 0.15)."""
+def periodic_cells(count=4, seed=5, first=4):
+    """Seeded water cells: first, first + 2, ... molecules at liquid density,
+    every other one triclinic (shear 0.15)."""
     rng = np.random.default_rng(seed)
-    return [water_box(4 + 2 * k, rng, shear=0.15 if k % 2 else 0.0) for k in range(count)]
+    return [water_box(first + 2 * k, rng, shear=0.15 if k % 2 else 0.0) for k in range(count)]

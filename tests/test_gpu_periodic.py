@@ -1,0 +1,110 @@
+"""Periodic cells on the GPU against the oracle (Gamma point).
+
+Cells of 12, 14 and 16 rigid waters at liquid density (cubic and triclinic)
+run through tblite_amd_singlepoint_flat_periodic; every stage the oracle
+exposes is compared: the translation table and the neighbour list with its
+images bit for bit, CN, the Gamma-point S/D/Q/H, the D4 two- and three-body
+energies, charges and the total energy (tolerances of the molecular tests:
+1e-12 for integer-free setup quantities, 1e-11 for integrals, 1e-6 Eh for
+energies, 1e-5 e for charges, iterations within one).  The periodic
+structure handle goes through tblite_amd_xtb_singlepoint.  Cells run on the
+multi-workgroup SCF path (it reads the Ewald-summed interaction planes).
+"""
+import numpy as np
+import pytest
+
+import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def cells():
+    return workloads.periodic_cells(count=3, seed=5, first=12)
+
+
+@pytest.fixture(scope="module")
+def run(engine, cells):
+    mol_nat = [len(c[0]) for c in cells]
+    num = np.concatenate([c[0] for c in cells])
+    xyz = np.concatenate([c[1] for c in cells])
+    lat = np.stack([c[2] for c in cells])
+    print(f"[gpu] periodic cells nat={mol_nat}", flush=True)
+    e, it, st = engine.singlepoint_flat_periodic(mol_nat, num, xyz, lat)
+    return e, it, st
+
+
+@pytest.fixture(scope="module")
+def refs(orc, cells):
+    return [orc.singlepoint(n, x, lattice=l) for n, x, l in cells]
+
+
+def test_cells_energy_and_charges(engine, cells, run, refs):
+    e, it, st = run
+    assert list(st) == [0] * len(cells)
+    for k, (ref, (num, xyz, lat)) in enumerate(zip(refs, cells)):
+        assert ref["rc"] == 0
+        assert abs(e[k] - ref["energy"]) < 1e-6, (k, e[k], ref["energy"])
+        assert abs(int(it[k]) - ref["iterations"]) <= 1
+        assert engine.molecule_path(k) == 2
+        ch = engine.charges(k)
+        assert np.abs(ch["qsh"] - ref["qsh"]).max() < 1e-5
+        assert np.abs(ch["qat"] - ref["qat"]).max() < 1e-5
+        assert np.abs(ch["dpat"] - ref["dpat"]).max() < 1e-5
+        assert np.abs(ch["qpat"] - ref["qpat"]).max() < 1e-5
+        assert abs(ch["focc"].sum() - (ch["qsh"].sum() * 0 + np.sum(ref["focc"]))) < 1e-8
+        e2, e3 = engine.dispersion(k)
+        assert abs(e2 - ref["edisp2"]) < 1e-8
+        assert abs(e3 - ref["edisp3"]) < 1e-10
+
+
+def test_cell_images_bitexact(engine, orc, cells, run):
+    for k, (num, xyz, lat) in enumerate(cells):
+        inl, nnl, nlat = engine.adjlist(k)
+        nltr, tr = engine.images(k)
+        ref = orc.pbc_tables(lat, xyz)
+        assert tr.shape == ref["trans"].shape
+        assert np.array_equal(tr, ref["trans"])
+        rinl, rnnl, rnlat, rnltr = orc.adjlist_pbc(xyz, lat, orc.cutoff(num, engine.accuracy))
+        assert np.array_equal(inl, rinl) and np.array_equal(nnl, rnnl)
+        assert np.array_equal(nlat, rnlat) and np.array_equal(nltr, rnltr)
+
+
+def test_cell_setup_and_integrals(engine, orc, cells, run):
+    for k, (num, xyz, lat) in enumerate(cells):
+        t = orc.pbc_terms(num, xyz, lat)
+        assert np.abs(engine.cn(k) - t["cn"]).max() < 1e-12
+        S, D, Q, H = engine.integrals(k)
+        rS, rD, rQ, rH = orc.hamiltonian_pbc(num, xyz, lat, engine.accuracy)
+        assert np.abs(S - rS).max() < 1e-11
+        assert np.abs(H - rH).max() < 1e-11
+        assert np.abs(D - rD).max() < 1e-11
+        assert np.abs(Q - rQ).max() < 1e-11
+
+
+def test_periodic_structure_singlepoint(engine, orc, cells, refs):
+    num, xyz, lat = cells[0]
+    out = engine.singlepoint(num, xyz, lattice=lat)
+    assert out["rc"] == 0
+    assert abs(out["energy"] - refs[0]["energy"]) < 1e-6
+    assert np.abs(out["qat"] - refs[0]["qat"]).max() < 1e-5
+    # the density reproduces the electron count through the Gamma-point overlap
+    S, _, _, _ = orc.hamiltonian_pbc(num, xyz, lat)
+    nel = float(np.sum(out["density"] * S))
+    assert abs(nel - (out["n0at"].sum())) < 1e-8
+
+
+def test_cell_gradient_and_bad_flags_rejected(engine, cells):
+    from tblite_amd import _dp, _ip
+    num, xyz, lat = cells[0]
+    out = engine.singlepoint(num, xyz, lattice=lat, gradient=True, wavefunction=False)
+    assert out["rc"] == -1
+    assert "periodic" in engine.error()
+    per = np.array([1, 1, 0], dtype=np.int32)
+    latf = np.ascontiguousarray(lat, dtype=np.float64).reshape(9)
+    xyzf = np.ascontiguousarray(xyz, dtype=np.float64)
+    numi = np.ascontiguousarray(num, dtype=np.int32)
+    h = engine.lib.tblite_amd_new_structure_periodic(engine.ctx, len(num), _ip(numi), _dp(xyzf), 0.0,
+                                                     0, _dp(latf), _ip(per))
+    assert not h
+    assert "three-dimensional" in engine.error()

@@ -39,7 +39,7 @@ __global__ void k_pbc_pairs(BatchDev b, const int* work_mol, const int* work_a0,
 __global__ void k_pbc_atm_sites(BatchDev b, const int* work_mol, const int* work_atom);
 __global__ void k_pbc_atm(BatchDev b, const int* work_mol, const int* work_atom, const int* work_r0,
                           const int* work_r1);
-template <int NT, int MAXT, int MAXB, int MAXC, bool MATS_LDS>
+template <int NT, int MAXT, int MAXB, int MAXC, bool MATS_LDS, bool CELL>
 __global__ void k_scf(BatchDev b, const int* mols, int ldmax, int natmax, int nshmax);
 // analytic gradient (gradient.hip)
 __global__ void k_grad_pot(BatchDev b);
@@ -64,7 +64,7 @@ constexpr int kAtmChunkLog2 = 18;
 constexpr int kPairChunkLog2 = 15;
 constexpr int kLdsMaxLd = 96;
 constexpr int kNumStreams = 4;
-constexpr int kMaxBuckets = 8;
+constexpr int kMaxBuckets = 16;  // size buckets of molecules plus those of periodic cells
 constexpr int kGridMaxLd = 12288;  // largest padded AO dimension of the multi-workgroup path
 
 // ------------------------------------------------------------ calculator
@@ -405,6 +405,7 @@ struct Plan {
   struct Bucket {
     int nt, maxt;
     bool lds;
+    bool cell = false;  // periodic cells (AES2 from the Ewald-summed planes)
     int ldmax, natmax, nshmax;
     std::vector<int> mols;
   };
@@ -723,9 +724,14 @@ int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* 
       lds[m] = (nao + 3) & ~3;
     }
     route_molecules(ctx, lds, pl.mol_path);
+    // cells: the LDS-resident kernels (their CELL instantiation reads the
+    // Ewald-summed AES2 planes) up to lds_max_ld, else the multi-workgroup
+    // path; the one-workgroup global-memory kernel derives transposed
+    // multipole elements by the molecular origin shift, which the image sums
+    // of a cell do not obey
     if (lattice)
       for (int m = 0; m < nmol; ++m)
-        if (!is_cell || is_cell[m]) pl.mol_path[m] = 2;
+        if (!is_cell || is_cell[m]) pl.mol_path[m] = lds[m] <= ctx->lds_max_ld ? 0 : 2;
   }
   if (lattice && !xyz) {
     set_error(ctx, "internal: periodic plan without coordinates");
@@ -990,16 +996,20 @@ int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* 
   for (int m = 0; m < nmol; ++m) order[m] = m;
   std::stable_sort(order.begin(), order.end(),
                    [&](int a, int b) { return pl.mol_nao[a] > pl.mol_nao[b]; });
-  for (int k = (int)bounds.size() - 1; k >= 0; --k) {
+  std::vector<char> cellf(nmol, 0);
+  for (int m = 0; m < nmol && pl.any_pbc; ++m) cellf[m] = pl.mol_ntr[m] > 0;
+  for (int k = (int)bounds.size() - 1; k >= 0; --k)
+  for (int want_cell = 0; want_cell < (pl.any_pbc ? 2 : 1); ++want_cell) {
     Plan::Bucket bk;
     const int lo = (k == 0) ? 0 : bounds[k - 1];
     const int hi = bounds[k];
     bk.ldmax = 0;
     bk.natmax = 0;
     bk.nshmax = 0;
+    bk.cell = want_cell != 0;
     for (int m : order) {
       const int ld = (pl.mol_nao[m] + 3) & ~3;
-      if (ld > lo && ld <= hi && pl.mol_path[m] != 2) {
+      if (ld > lo && ld <= hi && pl.mol_path[m] != 2 && (cellf[m] != 0) == bk.cell) {
         bk.mols.push_back(m);
         bk.ldmax = std::max(bk.ldmax, ld);
         bk.natmax = std::max(bk.natmax, pl.mol_nat[m]);
@@ -1123,7 +1133,7 @@ size_t workspace_need(const Plan& pl, size_t img_bytes, bool wfn_arrays, bool gr
   if (wfn_arrays) need += 2 * dbytes(pl.tot_mat);        // coeff, density
   // gam; sd, dd, sq planes only for the multi-workgroup SCF (the
   // one-workgroup kernels evaluate the AES2 pair kernels where they use them)
-  const bool aes_planes = !pl.grid_mols.empty();
+  const bool aes_planes = !pl.grid_mols.empty() || pl.any_pbc;
   need += dbytes(pl.tot_gam) + (aes_planes ? 18 * dbytes(pl.tot_pair) : 0);
   need += dbytes(Sh) + dbytes(A) + dbytes(3 * A) + dbytes(6 * A) + 2 * dbytes(AO); // wfn
   need += 4 * dbytes(pl.tot_mix) + 2 * dbytes(pl.tot_mixmem);
@@ -1170,7 +1180,7 @@ int upload_plan(tblite_amd_context ctx, Calculator& c, const Plan& pl, DeviceBat
   const size_t nfixed = seg.size() - pl.buckets.size();  // fixed arrays before the bucket lists
   const int64_t A = pl.tot_atoms, Sh = pl.tot_sh, AO = pl.tot_ao;
   const size_t need = workspace_need(pl, img.size(), wfn_arrays, grad);
-  const bool aes_planes = !pl.grid_mols.empty();
+  const bool aes_planes = !pl.grid_mols.empty() || pl.any_pbc;
 
   Arena& ar = fixed ? *fixed : ctx->arena;
   if (fixed && ar.cap < need) {
@@ -1360,7 +1370,7 @@ int upload_plan(tblite_amd_context ctx, Calculator& c, const Plan& pl, DeviceBat
   return TBLITE_AMD_OK;
 }
 
-template <int NT, int MAXT, int MAXB, int MAXC, bool LDS>
+template <int NT, int MAXT, int MAXB, int MAXC, bool LDS, bool CELL = false>
 int launch_scf(tblite_amd_context ctx, hipStream_t st, const BatchDev& b, const int* mols,
                int count, int ldmax, int natmax, int nshmax)
 {
@@ -1372,9 +1382,9 @@ int launch_scf(tblite_amd_context ctx, hipStream_t st, const BatchDev& b, const 
     set_error(ctx, "SCF bucket needs %zu bytes of LDS (limit %zu)", bytes, ctx->lds_limit);
     return TBLITE_AMD_ERR_TOO_LARGE;
   }
-  HIP_TRY(ctx, hipFuncSetAttribute((const void*)k_scf<NT, MAXT, MAXB, MAXC, LDS>,
+  HIP_TRY(ctx, hipFuncSetAttribute((const void*)k_scf<NT, MAXT, MAXB, MAXC, LDS, CELL>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-  hipLaunchKernelGGL((k_scf<NT, MAXT, MAXB, MAXC, LDS>), dim3(count), dim3(NT), bytes, st, b,
+  hipLaunchKernelGGL((k_scf<NT, MAXT, MAXB, MAXC, LDS, CELL>), dim3(count), dim3(NT), bytes, st, b,
                      mols, ldmax, natmax, nshmax);
   HIP_TRY(ctx, hipGetLastError());
   return TBLITE_AMD_OK;
@@ -1611,7 +1621,18 @@ int run_batch(tblite_amd_context ctx, Calculator& c, const double* xyz, double a
     // blocks of A (ld <= 32: 120 over 64; ld <= 64: 496 over 192; ld <= 96:
     // 1128 over 448) and up to MAXC two-AO pieces of one slot's CT rows (4, 6
     // and 8 threads per slot)
-    if (!bk.lds)
+    if (bk.cell && !bk.lds)
+      rc = TBLITE_AMD_ERR_ARGUMENT;  // cells above the LDS sizes run on the multi-workgroup path
+    else if (bk.cell && bk.nt == 128)
+      rc = launch_scf<128, 1, 2, 4, true, true>(ctx, as, b, mols, cnt, bk.ldmax, bk.natmax,
+                                                bk.nshmax);
+    else if (bk.cell && bk.nt == 256)
+      rc = launch_scf<256, 1, 3, 6, true, true>(ctx, as, b, mols, cnt, bk.ldmax, bk.natmax,
+                                                bk.nshmax);
+    else if (bk.cell)
+      rc = launch_scf<512, 2, 3, 6, true, true>(ctx, as, b, mols, cnt, bk.ldmax, bk.natmax,
+                                                bk.nshmax);
+    else if (!bk.lds)
       rc = launch_scf<TA_BIG_NT, 1, 0, 0, false>(ctx, as, b, mols, cnt, bk.ldmax, bk.natmax,
                                                  bk.nshmax);
     else if (bk.nt == 128)

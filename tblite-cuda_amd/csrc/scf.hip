@@ -67,6 +67,10 @@ struct MolView {
   const double* gam;
   const double* xyz;   // atom positions and multipole radii (AES2 pair kernels)
   const double* mrad;
+  // periodic cell: the Ewald-summed AES2 planes (row a, column i), nat^2 each
+  const double* aes_sd;
+  const double* aes_dd;
+  const double* aes_sq;
   double* qsh;
   double* qat;
   double* dpat;
@@ -1019,8 +1023,12 @@ __device__ __noinline__ double d4_energy_part(const DevD4& d4, const int* __rest
 
 // ---------------------------------------------------------------- kernel
 // The global-memory variant runs two workgroups per CU (128 VGPRs): its phases
-// are bound by memory latency, not by arithmetic.
-template <int NT, int MAXT, int MAXB, int MAXC, bool MATS_LDS>
+// are bound by memory latency, not by arithmetic.  CELL: periodic cells (LDS
+// instantiations only), whose AES2 interactions are read from the
+// Ewald-summed planes of k_pbc_pairs, diagonal included (an atom interacts
+// with its own images), instead of the molecular pair kernels evaluated in
+// place.
+template <int NT, int MAXT, int MAXB, int MAXC, bool MATS_LDS, bool CELL>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (NT >= 512 ? 1 : 2) : 4))) void k_scf(BatchDev b, const int* __restrict__ mols, int ldmax,
                                             int natmax, int nshmax)
 {
@@ -1063,6 +1071,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
   v.gam = b.gam + b.mol_gam_off[m];
   v.xyz = b.xyz + 3 * (int64_t)v.a0;
   v.mrad = b.mrad + v.a0;
+  if constexpr (CELL) {
+    v.aes_sd = b.aes_sd + 3 * b.mol_pair_off[m];
+    v.aes_dd = b.aes_dd + 9 * b.mol_pair_off[m];
+    v.aes_sq = b.aes_sq + 6 * b.mol_pair_off[m];
+  }
   v.qsh = b.qsh + v.s0;
   v.qat = b.qat + v.a0;
   v.dpat = b.dpat + 3 * (int64_t)v.a0;
@@ -1386,6 +1399,28 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
     for (int a0 = 4 * wid; a0 < nat; a0 += 4 * NW) {
       const int a = a0 + seg;
       double va = 0.0, d[3] = {0.0, 0.0, 0.0}, q[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      if constexpr (CELL) {
+        const int64_t p2 = (int64_t)nat * nat;
+        for (int i = sl; i < nat && a < nat; i += 16) {
+          const int64_t ai = (int64_t)a * nat + i;
+          const double qi = v.qat[i];
+          const double dpi[3] = {v.dpat[3 * i], v.dpat[3 * i + 1], v.dpat[3 * i + 2]};
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            const double sdk = v.aes_sd[k * p2 + ai];
+            va += sdk * dpi[k];
+            d[k] -= sdk * qi;
+#pragma unroll
+            for (int l = 0; l < 3; ++l) d[k] += v.aes_dd[(3 * l + k) * p2 + ai] * dpi[l];
+          }
+#pragma unroll
+          for (int k = 0; k < 6; ++k) {
+            const double sqk = v.aes_sq[k * p2 + ai];
+            va += sqk * v.qpat[6 * i + k];
+            q[k] += sqk * qi;
+          }
+        }
+      } else {
       for (int i = sl; i < nat && a < nat; i += 16) {
         const AesPair pr = aes_pair(v.xyz, v.mrad, a, i, P.dmp3, P.dmp5);
         const double qi = v.qat[i];
@@ -1404,6 +1439,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
           va += sqk * v.qpat[6 * i + k];
           q[k] += sqk * qi;
         }
+      }
       }
       va = seg16_sum(va);
 #pragma unroll
@@ -1807,7 +1843,24 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
       const double qs = v.qsh[sh];
       ec += qs * qs * qs * P.el[v.el[v.sh_at[sh]]].hubbard_derivs[v.sh_ish[sh]] / 3.0;
     }
-    for (int ai = tid; ai < nat * nat; ai += NT) {
+    if constexpr (CELL) {
+      const int64_t p2 = (int64_t)nat * nat;
+      for (int ai = tid; ai < nat * nat; ai += NT) {
+        const int a = ai / nat, i = ai - a * nat;
+        const double qi = v.qat[i];
+        const double dpi[3] = {v.dpat[3 * i], v.dpat[3 * i + 1], v.dpat[3 * i + 2]};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          double t = -v.aes_sd[k * p2 + ai] * qi;
+#pragma unroll
+          for (int l = 0; l < 3; ++l) t += 0.5 * v.aes_dd[(3 * l + k) * p2 + ai] * dpi[l];
+          ec += v.dpat[3 * a + k] * t;
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) ec += v.qpat[6 * a + k] * v.aes_sq[k * p2 + ai] * qi;
+      }
+    }
+    for (int ai = tid; ai < nat * nat && !CELL; ai += NT) {
       const int a = ai / nat, i = ai - a * nat;
       if (a == i) continue;  // the pair kernels vanish on the diagonal
       const AesPair pr = aes_pair(v.xyz, v.mrad, a, i, P.dmp3, P.dmp5);
@@ -1918,7 +1971,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
 }
 
 // explicit instantiations, launched by the host per size bucket
-template __global__ void k_scf<128, 1, 2, 4, true>(BatchDev, const int*, int, int, int);
-template __global__ void k_scf<256, 1, 3, 6, true>(BatchDev, const int*, int, int, int);
-template __global__ void k_scf<512, 2, 3, 6, true>(BatchDev, const int*, int, int, int);
-template __global__ void k_scf<512, 1, 0, 0, false>(BatchDev, const int*, int, int, int);
+template __global__ void k_scf<128, 1, 2, 4, true, false>(BatchDev, const int*, int, int, int);
+template __global__ void k_scf<256, 1, 3, 6, true, false>(BatchDev, const int*, int, int, int);
+template __global__ void k_scf<512, 2, 3, 6, true, false>(BatchDev, const int*, int, int, int);
+template __global__ void k_scf<512, 1, 0, 0, false, false>(BatchDev, const int*, int, int, int);
+template __global__ void k_scf<128, 1, 2, 4, true, true>(BatchDev, const int*, int, int, int);
+template __global__ void k_scf<256, 1, 3, 6, true, true>(BatchDev, const int*, int, int, int);
+template __global__ void k_scf<512, 2, 3, 6, true, true>(BatchDev, const int*, int, int, int);

@@ -1,14 +1,17 @@
 """Periodic cells on the GPU against the oracle (Gamma point).
 
-Cells of 12, 14 and 16 rigid waters at liquid density (cubic and triclinic)
-run through tblite_amd_singlepoint_flat_periodic; every stage the oracle
+Cells of 4, 12, 14, 16 and 18 rigid waters at liquid density (cubic and
+triclinic) run through tblite_amd_singlepoint_flat_periodic in one batch;
+every stage the oracle
 exposes is compared: the translation table and the neighbour list with its
 images bit for bit, CN, the Gamma-point S/D/Q/H, the D4 two- and three-body
 energies, charges and the total energy (tolerances of the molecular tests:
 1e-12 for integer-free setup quantities, 1e-11 for integrals, 1e-6 Eh for
 energies, 1e-5 e for charges, iterations within one).  The periodic
-structure handle goes through tblite_amd_xtb_singlepoint.  Cells run on the
-multi-workgroup SCF path (it reads the Ewald-summed interaction planes).
+structure handle goes through tblite_amd_xtb_singlepoint.  Cells up to 96
+AOs run batched on the LDS-resident SCF kernels (their cell instantiation
+reads the Ewald-summed AES2 planes), larger ones on the multi-workgroup path;
+both are checked, and a cell gives the same energy on either.
 """
 import numpy as np
 import pytest
@@ -20,7 +23,10 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(scope="module")
 def cells():
-    return workloads.periodic_cells(count=3, seed=5, first=12)
+    small = workloads.periodic_cells(count=1, seed=6, first=4)
+    mid = workloads.periodic_cells(count=3, seed=5, first=12)
+    big = workloads.periodic_cells(count=1, seed=7, first=18)
+    return small + mid + big
 
 
 @pytest.fixture(scope="module")
@@ -39,14 +45,15 @@ def refs(orc, cells):
     return [orc.singlepoint(n, x, lattice=l) for n, x, l in cells]
 
 
-def test_cells_energy_and_charges(engine, cells, run, refs):
+def test_cells_energy_and_charges(engine, orc, cells, run, refs):
     e, it, st = run
     assert list(st) == [0] * len(cells)
     for k, (ref, (num, xyz, lat)) in enumerate(zip(refs, cells)):
         assert ref["rc"] == 0
         assert abs(e[k] - ref["energy"]) < 1e-6, (k, e[k], ref["energy"])
         assert abs(int(it[k]) - ref["iterations"]) <= 1
-        assert engine.molecule_path(k) == 2
+        nao = orc.dims(num)[2]
+        assert engine.molecule_path(k) == (0 if nao <= 96 else 2), (k, nao)
         ch = engine.charges(k)
         assert np.abs(ch["qsh"] - ref["qsh"]).max() < 1e-5
         assert np.abs(ch["qat"] - ref["qat"]).max() < 1e-5
@@ -108,3 +115,16 @@ def test_cell_gradient_and_bad_flags_rejected(engine, cells):
                                                      0, _dp(latf), _ip(per))
     assert not h
     assert "three-dimensional" in engine.error()
+
+
+def test_cell_same_on_lds_and_multi_workgroup_paths(cells, refs):
+    import tblite_amd
+    num, xyz, lat = cells[1]
+    eng = tblite_amd.Engine(tuning={"auto_grid": 0, "lds_max_ld": 64})
+    try:
+        e, it, st = eng.singlepoint_flat_periodic([len(num)], num, xyz, lat[None])
+        assert eng.molecule_path(0) == 2
+    finally:
+        eng.close()
+    assert st[0] == 0
+    assert abs(e[0] - refs[1]["energy"]) < 1e-6

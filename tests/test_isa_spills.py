@@ -23,7 +23,7 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(ROOT, "tblite-cuda_amd", "csrc")
-SOURCES = ["scf.hip", "scf_grid.hip", "gradient.hip", "setup.hip"]
+SOURCES = ["scf.hip", "scf_grid.hip", "gradient.hip", "setup.hip", "periodic.hip"]
 
 
 def _hipcc():

@@ -310,6 +310,8 @@ def main():
                          "the closed-loop measurement")
     ap.add_argument("--gradient-steps", type=int, default=2,
                     help="timed steps of singlepoint + analytic gradient (0: skip)")
+    ap.add_argument("--cell-steps", type=int, default=2,
+                    help="timed steps of the periodic-cell leg (512 water cells; 0: skip)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device copy of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -456,6 +458,9 @@ def main():
         print(f"[bench] singlepoint + gradient: {g_ms:.2f} ms/step "
               f"({nmol / (g_ms / 1e3):.1f}/s)", file=sys.stderr, flush=True)
 
+    if args.cell_steps > 0:
+        extra["periodic"] = cell_leg(eng, args.cell_steps, not args.no_cpu)
+
     cpu = None
     if args.workload == "config2":
         # one 4200-AO molecule: the CPU oracle needs minutes per SCF iteration,
@@ -491,6 +496,51 @@ def main():
         "extra": extra,
     }
     print(json.dumps(line))
+
+
+def cell_leg(eng, steps, with_cpu):
+    """Periodic water cells (Gamma point): one flat call per step, timed on
+    the host around the call (plan, uploads and result copies included) and
+    on the device (phase events); a parity and CPU sample against the oracle."""
+    cl = workloads.cells()
+    mol_nat = [len(c[0]) for c in cl]
+    num = np.concatenate([c[0] for c in cl])
+    xyz = np.concatenate([c[1] for c in cl])
+    lat = np.stack([c[2] for c in cl])
+    e, it, st = eng.singlepoint_flat_periodic(mol_nat, num, xyz, lat)
+    ms, dev = [], []
+    for _ in range(steps):
+        t = time.perf_counter()
+        e, it, st = eng.singlepoint_flat_periodic(mol_nat, num, xyz, lat)
+        ms.append((time.perf_counter() - t) * 1e3)
+        dev.append(float(eng.timings()[4]))
+    paths = [eng.molecule_path(k) for k in range(len(cl))]
+    out = {"value": round(len(cl) / (np.mean(ms) / 1e3), 2), "unit": "cells/s",
+           "workload": f"{len(cl)} water cells, 4..16 molecules at liquid density, "
+                       "half triclinic, seed 8",
+           "ms_per_step": round(float(np.mean(ms)), 3),
+           "device_ms": round(float(np.mean(dev)), 3),
+           "lds_path_cells": int(paths.count(0)), "multi_workgroup_cells": int(paths.count(2)),
+           "status_nonzero": int(np.count_nonzero(st))}
+    print(f"[bench] periodic cells: {out['ms_per_step']:.2f} ms/step "
+          f"({out['value']:.1f} cells/s, device {out['device_ms']:.2f} ms)",
+          file=sys.stderr, flush=True)
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        o = oracle.Oracle()
+        pick = np.linspace(0, len(cl) - 1, 6).astype(int)
+        t = time.perf_counter()
+        de = []
+        for k in pick:
+            r = o.singlepoint(cl[k][0], cl[k][1], lattice=cl[k][2])
+            de.append(abs(r["energy"] - e[k]))
+        dt = time.perf_counter() - t
+        out["parity_sample"] = {"cells": len(pick), "max_abs_dE": float(max(de))}
+        out["cpu_baseline"] = {"value": round(len(pick) / dt, 3), "unit": "cells/s",
+                               "kind": "port", "sample": f"{len(pick)} evenly spaced cells, "
+                               "CPU oracle one cell at a time (its three-body term on OpenMP)"}
+    return out
 
 
 def _cpu_model():

@@ -31,9 +31,11 @@ __global__ void k_setup(BatchDev b);
 __global__ void k_adjlist(BatchDev b);
 __global__ void k_d4_atm(BatchDev b, const int* work_mol, const int* work_i0, const int* work_i1);
 __global__ void k_setup_pairs(BatchDev b, const int* work_mol, const int* work_a0, const int* work_a1);
-template <int LMAX, bool PBC>
+template <int LMAX>
 __global__ void k_integrals(BatchDev b, const int* work_mol, const int* work_kind,
                             const int* work_start);
+template <int LMAX>
+__global__ void k_pbc_integrals(BatchDev b, const int* work_mol, const int* work_i, const int* work_j);
 // periodic cells (periodic.hip)
 __global__ void k_pbc_pairs(BatchDev b, const int* work_mol, const int* work_a0, const int* work_a1);
 __global__ void k_pbc_atm_sites(BatchDev b, const int* work_mol, const int* work_atom);
@@ -259,6 +261,8 @@ double get_cutoff(double min_alpha, double acc)
 // on any host.  Returns false for a degenerate lattice.
 struct CellTables {
   double vol = 0.0, alpha = 0.0, rreal = 0.0;
+  int nrep[3] = {0, 0, 0};          // translation index ranges [-N_k, N_k]
+  double rec[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, dk[3] = {0, 0, 0};  // b_k, plane spacings
   std::vector<double> tr, g;
 };
 
@@ -284,10 +288,12 @@ bool cell_tables(const double* lat, int nat, const double* xyz, CellTables& ct)
   ct.rreal = TA_PBC_EWALD_X / ct.alpha;
   const double gmax = 2.0 * TA_PBC_EWALD_X * ct.alpha;
   const double rmax = std::fmax(TA_PBC_RMIN, ct.rreal);
-  int nrep[3];
+  int* nrep = ct.nrep;
+  for (int k = 0; k < 9; ++k) ct.rec[k] = rec[k];
   for (int k = 0; k < 3; ++k) {
     const double* ck = c + 3 * k;
     const double dk = ct.vol / std::sqrt(ck[0] * ck[0] + ck[1] * ck[1] + ck[2] * ck[2]);
+    ct.dk[k] = dk;
     double fmn = 0.0, fmx = 0.0;
     for (int i = 0; i < nat; ++i) {
       const double f = (xyz[3 * i] * rec[3 * k] + xyz[3 * i + 1] * rec[3 * k + 1] +
@@ -324,27 +330,44 @@ bool cell_tables(const double* lat, int nat, const double* xyz, CellTables& ct)
 }
 
 // Upper bounds of a cell's neighbour-list length (pairs j <= i with an image
-// within cutoff) and of each atom's three-body site count (images within
-// cut3), with a relative slack of 1e-9 so that rounding on the device can
-// only give fewer entries.
-void cell_counts(int nat, const double* xyz, const std::vector<double>& tr, double cutoff,
-                 double cut3, int64_t& nl, std::vector<int>& sites)
+// within cutoff) and of each atom's three-body site count (images of atoms
+// j >= i within cut3), with a relative slack of 1e-9 so that rounding on the
+// device can only give fewer entries.
+// Only the translations of the box n_k in [f_k - c / d_k, f_k + c / d_k]
+// around a pair's fractional separation f can lie within c of it
+// (|R - T| >= d_k |f_k - n_k|), so each pair scans that box, not the table.
+void cell_counts(int nat, const double* xyz, const CellTables& ct, double cutoff, double cut3,
+                 int64_t& nl, std::vector<int>& sites)
 {
-  const int ntr = (int)(tr.size() / 3);
+  const std::vector<double>& tr = ct.tr;
   const double c2 = cutoff * cutoff * (1.0 + 1.0e-9) + 1.0e-9;
   const double s2 = cut3 * cut3 * (1.0 + 1.0e-9) + 1.0e-9;
+  const double cmax = std::max(cutoff, cut3) * (1.0 + 1.0e-9) + 1.0e-9;
+  const double pi2 = 2.0 * 3.14159265358979323846;
+  const int w2 = 2 * ct.nrep[2] + 1, w1 = 2 * ct.nrep[1] + 1;
   nl = 0;
   sites.assign(nat, 0);
+  // pairs j <= i once: the list entries of i, and the sites of anchor j
+  // (atom i >= j; the table holds -T with every T, so the count is symmetric)
   for (int i = 0; i < nat; ++i)
-    for (int j = 0; j < nat; ++j) {
+    for (int j = 0; j <= i; ++j) {
       const double dx = xyz[3 * i] - xyz[3 * j], dy = xyz[3 * i + 1] - xyz[3 * j + 1];
       const double dz = xyz[3 * i + 2] - xyz[3 * j + 2];
-      for (int t = 0; t < ntr; ++t) {
-        const double x = dx - tr[3 * t], y = dy - tr[3 * t + 1], z = dz - tr[3 * t + 2];
-        const double r2 = x * x + y * y + z * z;
-        if (j <= i && r2 <= c2) ++nl;
-        if (r2 <= s2) ++sites[i];
+      int lo[3], hi[3];
+      for (int k = 0; k < 3; ++k) {
+        const double f = (dx * ct.rec[3 * k] + dy * ct.rec[3 * k + 1] + dz * ct.rec[3 * k + 2]) / pi2;
+        lo[k] = std::max(-ct.nrep[k], (int)std::floor(f - cmax / ct.dk[k]) - 1);
+        hi[k] = std::min(ct.nrep[k], (int)std::ceil(f + cmax / ct.dk[k]) + 1);
       }
+      for (int n1 = lo[0]; n1 <= hi[0]; ++n1)
+        for (int n2 = lo[1]; n2 <= hi[1]; ++n2)
+          for (int n3 = lo[2]; n3 <= hi[2]; ++n3) {
+            const int t = ((n1 + ct.nrep[0]) * w1 + (n2 + ct.nrep[1])) * w2 + (n3 + ct.nrep[2]);
+            const double x = dx - tr[3 * t], y = dy - tr[3 * t + 1], z = dz - tr[3 * t + 2];
+            const double r2 = x * x + y * y + z * z;
+            if (r2 <= c2) ++nl;
+            if (r2 <= s2) ++sites[j];
+          }
     }
 }
 
@@ -386,15 +409,16 @@ struct Plan {
   std::vector<int> grid_mols;  // molecules run by the multi-workgroup SCF path
   // periodic cells (empty for a batch of molecules): per molecule the
   // translation and reciprocal-vector counts and offsets and the Ewald
-  // constants; the tables; integral work of cells (s/p-only items first,
-  // nwork_pbc_sp of them); pair-plane items (atom rows); three-body site
+  // constants; the tables; integral work of cells (one item per atom pair
+  // (i, j <= i), s/p-only cells first, nwork_pbc_sp of them); pair-plane
+  // items (atom rows); three-body site
   // items (one per atom) and pair items (site rows of one anchor atom); per
   // atom the site capacity and offset
   bool any_pbc = false;
   std::vector<int> mol_ntr, mol_ng;
   std::vector<int64_t> mol_tr_off, mol_g_off;
   std::vector<double> trans, recip, mol_ewald;
-  std::vector<int> pw_mol, pw_kind, pw_start;
+  std::vector<int> pw_mol, pw_i, pw_j;
   int nwork_pbc_sp = 0;
   std::vector<int> pp_mol, pp_a0, pp_a1;
   std::vector<int> ps_mol, ps_atom;
@@ -428,7 +452,7 @@ struct DeviceBatch {
   int* d_atm_mol = nullptr;
   int* d_atm_i0 = nullptr;
   int* d_atm_i1 = nullptr;
-  int *d_pw_mol = nullptr, *d_pw_kind = nullptr, *d_pw_start = nullptr;
+  int *d_pw_mol = nullptr, *d_pw_i = nullptr, *d_pw_j = nullptr;
   int *d_pp_mol = nullptr, *d_pp_a0 = nullptr, *d_pp_a1 = nullptr;
   int *d_ps_mol = nullptr, *d_ps_atom = nullptr;
   int *d_pa_mol = nullptr, *d_pa_atom = nullptr, *d_pa_r0 = nullptr, *d_pa_r1 = nullptr;
@@ -748,7 +772,7 @@ int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* 
   int64_t atom = 0, shell = 0, ao = 0, mat = 0, nl = 0, gam = 0, pair = 0, mix = 0, mixmem = 0,
           big = 0, scr = 0;
   std::vector<int> wd_mol, wd_kind, wd_start;  // integral work of molecules with d shells
-  std::vector<int> pwd_mol, pwd_kind, pwd_start;  // the same for periodic cells
+  std::vector<int> pwd_mol, pwd_i, pwd_j;  // periodic cells with d shells
   for (int m = 0; m < nmol; ++m) {
     const int nat = mol_nat[m];
     bool has_d = false;
@@ -817,7 +841,7 @@ int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* 
       pl.mol_ewald[TA_PBC_EWALD * m] = ct.vol;
       pl.mol_ewald[TA_PBC_EWALD * m + 1] = ct.alpha;
       pl.mol_ewald[TA_PBC_EWALD * m + 2] = ct.rreal;
-      cell_counts(nat, xyz + 3 * atom, ct.tr, pl.mol_cutoff[m],
+      cell_counts(nat, xyz + 3 * atom, ct, pl.mol_cutoff[m],
                   c.dev.d4.enabled ? c.dev.d4.disp3_cutoff : 0.0, nl_cap, site_cnt);
       if (nl_cap > (int64_t)1 << 30) {
         set_error(ctx, "molecule %d: neighbour list of %lld entries", m, (long long)nl_cap);
@@ -873,18 +897,14 @@ int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* 
     // (cells: their own lists, s/p-only molecules first)
     if (cell) {
       std::vector<int>& wm = has_d ? pwd_mol : pl.pw_mol;
-      std::vector<int>& wk = has_d ? pwd_kind : pl.pw_kind;
-      std::vector<int>& ws = has_d ? pwd_start : pl.pw_start;
-      for (int64_t s = 0; s < nl_cap; s += ctx->int_chunk) {
-        wm.push_back(m);
-        wk.push_back(0);
-        ws.push_back((int)s);
-      }
-      for (int s = 0; s < nat; s += ctx->int_chunk) {
-        wm.push_back(m);
-        wk.push_back(1);
-        ws.push_back(s);
-      }
+      std::vector<int>& wi = has_d ? pwd_i : pl.pw_i;
+      std::vector<int>& wj = has_d ? pwd_j : pl.pw_j;
+      for (int i = 0; i < nat; ++i)
+        for (int j = 0; j <= i; ++j) {
+          wm.push_back(m);
+          wi.push_back(i);
+          wj.push_back(j);
+        }
       // pair planes: one thread per atom pair, rows of up to 256 pairs
       const int rows = std::max(1, 256 / nat);
       for (int a = 0; a < nat; a += rows) {
@@ -966,8 +986,8 @@ int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* 
   pl.work_start.insert(pl.work_start.end(), wd_start.begin(), wd_start.end());
   pl.nwork_pbc_sp = (int)pl.pw_mol.size();
   pl.pw_mol.insert(pl.pw_mol.end(), pwd_mol.begin(), pwd_mol.end());
-  pl.pw_kind.insert(pl.pw_kind.end(), pwd_kind.begin(), pwd_kind.end());
-  pl.pw_start.insert(pl.pw_start.end(), pwd_start.begin(), pwd_start.end());
+  pl.pw_i.insert(pl.pw_i.end(), pwd_i.begin(), pwd_i.end());
+  pl.pw_j.insert(pl.pw_j.end(), pwd_j.begin(), pwd_j.end());
   pl.tot_gam = gam;
   pl.tot_pair = pair;
   pl.tot_mix = mix;
@@ -1100,8 +1120,8 @@ void pack_plan(const Plan& pl, std::vector<char>& img, std::vector<std::pair<siz
   host_pack(img, seg, pl.recip);          // 40
   host_pack(img, seg, pl.mol_ewald);      // 41
   host_pack(img, seg, pl.pw_mol);         // 42
-  host_pack(img, seg, pl.pw_kind);        // 43
-  host_pack(img, seg, pl.pw_start);       // 44
+  host_pack(img, seg, pl.pw_i);           // 43
+  host_pack(img, seg, pl.pw_j);           // 44
   host_pack(img, seg, pl.pp_mol);         // 45
   host_pack(img, seg, pl.pp_a0);          // 46
   host_pack(img, seg, pl.pp_a1);          // 47
@@ -1262,8 +1282,8 @@ int upload_plan(tblite_amd_context ctx, Calculator& c, const Plan& pl, DeviceBat
     b.atm_site_cap = I(55);
   }
   db.d_pw_mol = I(42);
-  db.d_pw_kind = I(43);
-  db.d_pw_start = I(44);
+  db.d_pw_i = I(43);
+  db.d_pw_j = I(44);
   db.d_pp_mol = I(45);
   db.d_pp_a0 = I(46);
   db.d_pp_a1 = I(47);
@@ -1583,18 +1603,19 @@ int run_batch(tblite_amd_context ctx, Calculator& c, const double* xyz, double a
   HIP_TRY(ctx, hipEventRecord(ctx->ev[2], st));
   const int nwork = (int)pl.work_mol.size(), nsp = pl.nwork_sp;
   if (nsp > 0)
-    hipLaunchKernelGGL((k_integrals<1, false>), dim3(nsp), dim3(pl.int_chunk), 0, st, b,
-                       db.d_work_mol, db.d_work_kind, db.d_work_start);
+    hipLaunchKernelGGL(k_integrals<1>, dim3(nsp), dim3(pl.int_chunk), 0, st, b, db.d_work_mol,
+                       db.d_work_kind, db.d_work_start);
   if (nwork > nsp)
-    hipLaunchKernelGGL((k_integrals<2, false>), dim3(nwork - nsp), dim3(pl.int_chunk), 0, st, b,
+    hipLaunchKernelGGL(k_integrals<2>, dim3(nwork - nsp), dim3(pl.int_chunk), 0, st, b,
                        db.d_work_mol + nsp, db.d_work_kind + nsp, db.d_work_start + nsp);
+  // periodic cells: one workgroup per atom pair sums the pair's images in LDS
   const int npw = (int)pl.pw_mol.size(), npsp = pl.nwork_pbc_sp;
   if (npsp > 0)
-    hipLaunchKernelGGL((k_integrals<1, true>), dim3(npsp), dim3(pl.int_chunk), 0, st, b,
-                       db.d_pw_mol, db.d_pw_kind, db.d_pw_start);
+    hipLaunchKernelGGL(k_pbc_integrals<1>, dim3(npsp), dim3(64), 0, st, b, db.d_pw_mol,
+                       db.d_pw_i, db.d_pw_j);
   if (npw > npsp)
-    hipLaunchKernelGGL((k_integrals<2, true>), dim3(npw - npsp), dim3(pl.int_chunk), 0, st, b,
-                       db.d_pw_mol + npsp, db.d_pw_kind + npsp, db.d_pw_start + npsp);
+    hipLaunchKernelGGL(k_pbc_integrals<2>, dim3(npw - npsp), dim3(64), 0, st, b,
+                       db.d_pw_mol + npsp, db.d_pw_i + npsp, db.d_pw_j + npsp);
   HIP_TRY(ctx, hipGetLastError());
   HIP_TRY(ctx, hipEventRecord(ctx->ev[3], st));
   // SCF buckets on the auxiliary streams, largest bucket first; on a

@@ -16,8 +16,10 @@
 //         TA_PBC_AES_CUT; an atom and its own images interact (self term
 //         4 alpha^3 / (3 sqrt(pi)) of the Hessian)
 //   D4    two-body damping factors summed over the images; three-body term
-//         (1/3) sum_i sum_{a<b} over pairs of sites (atom images) within the
-//         three-body cutoff of atom i and of each other
+//         over the triangles of sites (atom images) within the three-body
+//         cutoff of each other, each class of translated triangles counted
+//         once: anchored at a site of its lowest atom index i (sites of atoms
+//         >= i), weighted by 1 / (number of its sites with atom index i)
 //
 // Launch shapes: k_pbc_pairs one thread per atom pair, work items of atom
 // rows of one cell; k_pbc_atm_sites one workgroup per atom of a cell (block
@@ -196,9 +198,9 @@ __global__ __launch_bounds__(PBC_THREADS) void k_pbc_pairs(BatchDev b, const int
   }
 }
 
-// Three-body sites of atom i of cell m (work item w): every image (j, T) with
-// 1e-12 < |R_j - T - R_i|^2 <= cutoff^2, in (j, T) order, stored as
-// (R_j - T - R_i, j) up to the host-computed capacity.
+// Three-body sites of atom i of cell m (work item w): every image (j, T) of
+// an atom j >= i with 1e-12 < |R_j - T - R_i|^2 <= cutoff^2, in (j, T)
+// order, stored as (R_j - T - R_i, j) up to the host-computed capacity.
 __global__ __launch_bounds__(PBC_THREADS) void k_pbc_atm_sites(BatchDev b, const int* __restrict__ work_mol,
                                                                const int* __restrict__ work_atom)
 {
@@ -214,7 +216,7 @@ __global__ __launch_bounds__(PBC_THREADS) void k_pbc_atm_sites(BatchDev b, const
   double* __restrict__ site = b.atm_site + 4 * off;
   const int64_t ncand = (int64_t)nat * ntr;
   int base = 0;
-  for (int64_t c0 = 0; c0 < ncand; c0 += PBC_THREADS) {
+  for (int64_t c0 = (int64_t)i * ntr; c0 < ncand; c0 += PBC_THREADS) {
     const int64_t c = c0 + tid;
     double v[3] = {0.0, 0.0, 0.0};
     int j = 0, keep = 0;
@@ -239,7 +241,18 @@ __global__ __launch_bounds__(PBC_THREADS) void k_pbc_atm_sites(BatchDev b, const
 }
 
 // Three-body energy of anchor i of cell m over the site pairs (a, b > a) with
-// a in [r0, r1) (work item w), a third of it added to d4_e3 of atom i.
+// a in [r0, r1) (work item w), each triangle weighted by one over its number
+// of sites of atom i (the anchor and images of it), added to d4_e3 of atom i.
+// The damping exponent alp / 3 = 16 / 3 of GFN2's D4 is formed as x^5 cbrt(x).
+__device__ __forceinline__ double atm_damp_pow(double x, double p3, bool p16)
+{
+  if (p16) {
+    const double x2 = x * x;
+    return x2 * x2 * x * cbrt(x);
+  }
+  return pow(x, p3);
+}
+
 __global__ __launch_bounds__(PBC_THREADS) void k_pbc_atm(BatchDev b, const int* __restrict__ work_mol,
                                                          const int* __restrict__ work_atom,
                                                          const int* __restrict__ work_r0,
@@ -257,6 +270,7 @@ __global__ __launch_bounds__(PBC_THREADS) void k_pbc_atm(BatchDev b, const int* 
   const DevD4& d = b.par->d4;
   const double cut2 = d.disp3_cutoff * d.disp3_cutoff;
   const double p3 = d.alp / 3.0;
+  const bool p16 = d.alp == 16.0;
   const double ri = d.r4r2[el[i]];
   double e = 0.0;
   for (int a = r0; a < r1; ++a) {
@@ -266,6 +280,7 @@ __global__ __launch_bounds__(PBC_THREADS) void k_pbc_atm(BatchDev b, const int* 
     const double rj = d.r4r2[el[j]];
     const double r0ij = d.a1 * sqrt(3.0 * ri * rj) + d.a2;
     const double c6ij = c60[(int64_t)i * nat + j];
+    const double wa = (j == i) ? 1.0 : 0.0;
     for (int bb = a + 1 + tid; bb < n; bb += PBC_THREADS) {
       const double* sb = site + 4 * bb;
       const int k = (int)sb[3];
@@ -278,12 +293,13 @@ __global__ __launch_bounds__(PBC_THREADS) void k_pbc_atm(BatchDev b, const int* 
       const double r0jk = d.a1 * sqrt(3.0 * rj * rk) + d.a2;
       const double c9 = d.s9 * sqrt(fabs(c6ij * c60[(int64_t)i * nat + k] * c60[(int64_t)j * nat + k]));
       const double r2ijk = r2ij * r2ik * r2jk, rijk = sqrt(r2ijk);
-      const double fdmp = 1.0 / (1.0 + 6.0 * pow(r0ij * r0ik * r0jk / rijk, p3));
+      const double fdmp = 1.0 / (1.0 + 6.0 * atm_damp_pow(r0ij * r0ik * r0jk / rijk, p3, p16));
       const double ang = 0.375 * (r2ij + r2jk - r2ik) * (r2ij - r2jk + r2ik) *
                              (-r2ij + r2jk + r2ik) / r2ijk + 1.0;
-      e += c9 * ang * fdmp / (rijk * r2ijk);
+      const double w = 1.0 / (1.0 + wa + (k == i ? 1.0 : 0.0));
+      e += w * c9 * ang * fdmp / (rijk * r2ijk);
     }
   }
   e = block_sum(e, red);
-  if (tid == 0 && e != 0.0) atomicAdd(b.d4_e3 + a0 + i, e / 3.0);
+  if (tid == 0 && e != 0.0) atomicAdd(b.d4_e3 + a0 + i, e);
 }

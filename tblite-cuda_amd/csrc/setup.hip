@@ -651,25 +651,11 @@ __device__ void shell_block(int li, int lj, const DevElem& ej, int jsh, const De
 #define WORK_PAIRS 0
 #define WORK_ONSITE 1
 
-// element store of k_integrals: plain for molecules (one thread per matrix
-// element), atomic accumulation for periodic cells (the images of one atom
-// pair, and an atom's own images next to its on-site block, share elements)
-template <bool ADD>
-__device__ __forceinline__ void put_elem(double* p, double v)
-{
-  if constexpr (ADD) atomicAdd(p, v);
-  else *p = v;
-}
-
 // get_hamiltonian (src/h0.cu:306-477 text) plus the on-site blocks.  One
-// thread per atom pair (neighbour-list entry) or per atom (on-site); for a
-// molecule each thread writes a disjoint set of matrix elements, so no
-// atomics.  LMAX = 1 serves molecules without d shells (the host splits the
-// work list by it).  PBC: periodic cells, whose list entries carry a
-// translation (vec = R_i - R_j - T, src/h0.cu:359 text); the blocks of all
-// images of a pair add into one Gamma-point block, and an image of the atom
-// itself adds its block once, without the mirrored copy.
-template <int LMAX, bool PBC>
+// thread per atom pair (neighbour-list entry) or per atom (on-site); each
+// thread writes a disjoint set of matrix elements, so no atomics.  LMAX = 1
+// serves molecules without d shells (the host splits the work list by it).
+template <int LMAX>
 __global__ __launch_bounds__(128) void k_integrals(BatchDev b, const int* __restrict__ work_mol,
                                                  const int* __restrict__ work_kind,
                                                  const int* __restrict__ work_start)
@@ -693,7 +679,6 @@ __global__ __launch_bounds__(128) void k_integrals(BatchDev b, const int* __rest
   const double intcut = b.intcut;
 
   int iat, jat;
-  const double* tvec = nullptr;
   if (kind == WORK_PAIRS) {
     if (item >= b.mol_nnz[m]) return;
     // locate the owning atom: largest iat with inl(iat) <= item
@@ -708,8 +693,6 @@ __global__ __launch_bounds__(128) void k_integrals(BatchDev b, const int* __rest
     while (lo < nat - 1 && inl[lo + 1] <= item) ++lo;
     iat = lo;
     jat = b.nl_jat[b.mol_nl_off[m] + item];
-    if constexpr (PBC)
-      tvec = b.trans + 3 * (b.mol_tr_off[m] + b.nl_itr[b.mol_nl_off[m] + item]);
   } else {
     if (item >= nat) return;
     iat = jat = item;
@@ -746,13 +729,8 @@ __global__ __launch_bounds__(128) void k_integrals(BatchDev b, const int* __rest
   }
   double vec[3];
   for (int k = 0; k < 3; ++k) vec[k] = xyz[3 * iat + k] - xyz[3 * jat + k];
-  if constexpr (PBC)
-    if (tvec)
-      for (int k = 0; k < 3; ++k) vec[k] -= tvec[k];
   const double r2 = vec[0] * vec[0] + vec[1] * vec[1] + vec[2] * vec[2];
   const double rr = sqrt(sqrt(r2) / (ej.atomic_rad + ei.atomic_rad));
-  const bool pair = PBC ? (kind == WORK_PAIRS) : (iat != jat);
-  const bool mirror = iat != jat;
 
   constexpr int NB = (2 * LMAX + 1) * (2 * LMAX + 1);  // largest shell block
   double st[NB], dt[3 * NB], qt[6 * NB];
@@ -764,7 +742,7 @@ __global__ __launch_bounds__(128) void k_integrals(BatchDev b, const int* __rest
       const int jj = b.sh_iao[s0 + js + jsh];
       shell_block<LMAX>(li, lj, ej, jsh, ei, ish, r2, vec, intcut, st, dt, qt);
       const int si = 2 * li + 1, sj = 2 * lj + 1;
-      if (pair) {
+      if (iat != jat) {
         const double shp = (1.0 + ei.shpoly[ish] * rr) * (1.0 + ej.shpoly[jsh] * rr);
         const double hij =
             0.5 * (se[is + ish] + se[js + jsh]) * P.hscale[ei_][ej_][ish][jsh] * shp;
@@ -794,15 +772,17 @@ __global__ __launch_bounds__(128) void k_integrals(BatchDev b, const int* __rest
             qj[5] = qi[5] + 1.5 * t5 - tr;
             const int64_t a_ij = (int64_t)(ii + iao) * n + (jj + jao); // fortran (jj+jao, ii+iao)
             const int64_t a_ji = (int64_t)(jj + jao) * n + (ii + iao); // fortran (ii+iao, jj+jao)
-            put_elem<PBC>(S + a_ij, sv);
-            put_elem<PBC>(H + a_ij, sv * hij);
-            for (int k = 0; k < 3; ++k) put_elem<PBC>(D + k * n2 + a_ij, di[k]);
-            for (int k = 0; k < 6; ++k) put_elem<PBC>(Q + k * n2 + a_ij, qi[k]);
-            if (!PBC || mirror) {
-              put_elem<PBC>(S + a_ji, sv);
-              put_elem<PBC>(H + a_ji, sv * hij);
-              for (int k = 0; k < 3; ++k) put_elem<PBC>(D + k * n2 + a_ji, dj[k]);
-              for (int k = 0; k < 6; ++k) put_elem<PBC>(Q + k * n2 + a_ji, qj[k]);
+            S[a_ij] = sv;
+            S[a_ji] = sv;
+            H[a_ij] = sv * hij;
+            H[a_ji] = sv * hij;
+            for (int k = 0; k < 3; ++k) {
+              D[k * n2 + a_ij] = di[k];
+              D[k * n2 + a_ji] = dj[k];
+            }
+            for (int k = 0; k < 6; ++k) {
+              Q[k * n2 + a_ij] = qi[k];
+              Q[k * n2 + a_ji] = qj[k];
             }
           }
       } else {
@@ -810,18 +790,175 @@ __global__ __launch_bounds__(128) void k_integrals(BatchDev b, const int* __rest
           for (int jao = 0; jao < sj; ++jao) {
             const int ij = jao + sj * iao;
             const int64_t a_ij = (int64_t)(ii + iao) * n + (jj + jao);
-            put_elem<PBC>(S + a_ij, st[ij]);
-            const double hv = (ish == jsh && iao == jao) ? se[is + ish] : 0.0;
-            if (!PBC || hv != 0.0) put_elem<PBC>(H + a_ij, hv);
-            for (int k = 0; k < 3; ++k) put_elem<PBC>(D + k * n2 + a_ij, dt[3 * ij + k]);
-            for (int k = 0; k < 6; ++k) put_elem<PBC>(Q + k * n2 + a_ij, qt[6 * ij + k]);
+            S[a_ij] = st[ij];
+            H[a_ij] = (ish == jsh && iao == jao) ? se[is + ish] : 0.0;
+            for (int k = 0; k < 3; ++k) D[k * n2 + a_ij] = dt[3 * ij + k];
+            for (int k = 0; k < 6; ++k) Q[k * n2 + a_ij] = qt[6 * ij + k];
           }
       }
     }
   }
 }
 
-template __global__ void k_integrals<1, false>(BatchDev, const int*, const int*, const int*);
-template __global__ void k_integrals<2, false>(BatchDev, const int*, const int*, const int*);
-template __global__ void k_integrals<1, true>(BatchDev, const int*, const int*, const int*);
-template __global__ void k_integrals<2, true>(BatchDev, const int*, const int*, const int*);
+// Gamma-point integrals of periodic cells, one workgroup per atom pair
+// (i, j <= i) of a cell (work item w): its neighbour-list entries (all images
+// of the pair, contiguous in i's list) are spread over the threads, each
+// image's shell blocks (vec = R_i - R_j - T, src/h0.cu:359 text) are summed
+// into an LDS copy of the pair block with LDS atomics, and the block is
+// stored once with plain stores (no global atomics, each element written by
+// one workgroup).  The pair (i, i) adds the on-site blocks and the atom's
+// images without a mirrored copy; every other pair also stores the mirrored
+// block, whose dipole and quadrupole elements are summed per image with the
+// operator moved to the other centre.
+#define PBC_INT_THREADS 64
+template <int LMAX>
+__global__ __launch_bounds__(PBC_INT_THREADS) void k_pbc_integrals(BatchDev b, const int* __restrict__ work_mol,
+                                                                  const int* __restrict__ work_i,
+                                                                  const int* __restrict__ work_j)
+{
+  constexpr int MA = TA_MAX_SHELL * (2 * LMAX + 1);  // AOs per atom, at most
+  constexpr int NE = MA * MA;
+  // block: c = 0 S, 1 H, 2..4 D, 5..10 Q (row i, column j); 11..13 D and
+  // 14..19 Q of the mirrored block
+  __shared__ double blk[20 * NE];
+  const int w = blockIdx.x, tid = threadIdx.x;
+  const int m = work_mol[w], iat = work_i[w], jat = work_j[w];
+  const int a0 = b.mol_atom_off[m], nat = b.mol_nat[m];
+  const int s0 = b.mol_sh_off[m];
+  const int n = b.mol_nao[m];
+  const int64_t n2 = (int64_t)n * n;
+  const double* __restrict__ xyz = b.xyz + 3 * (int64_t)a0;
+  const int* __restrict__ el = b.at_el + a0;
+  const DevParams& P = *b.par;
+  const double* __restrict__ se = b.selfenergy + s0;
+  const double intcut = b.intcut;
+  const int ei_ = el[iat], ej_ = el[jat];
+  const DevElem& ei = P.el[ei_];
+  const DevElem& ej = P.el[ej_];
+  const int is = b.at_ish[a0 + iat], js = b.at_ish[a0 + jat];
+  const int ii0 = b.at_iao[a0 + iat], jj0 = b.at_iao[a0 + jat];
+  int nai = 0, naj = 0;
+  for (int s = 0; s < ei.nsh; ++s) nai += 2 * ei.ang[s] + 1;
+  for (int s = 0; s < ej.nsh; ++s) naj += 2 * ej.ang[s] + 1;
+  const int ne = nai * naj;
+  for (int x = tid; x < 20 * ne; x += PBC_INT_THREADS) blk[x] = 0.0;
+  // entries of the pair: j's run inside i's list (ascending j, then T)
+  const int64_t nl0 = b.mol_nl_off[m];
+  const int* __restrict__ jl = b.nl_jat + nl0;
+  const int* __restrict__ tl = b.nl_itr + nl0;
+  const double* __restrict__ tr = b.trans + 3 * b.mol_tr_off[m];
+  const int l0 = b.nl_inl[a0 + iat], l1 = l0 + b.nl_nnl[a0 + iat];
+  int lo = l0, hi = l1;  // first entry with jat >= j
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (jl[mid] < jat) lo = mid + 1;
+    else hi = mid;
+  }
+  int e1 = lo, e2 = l1;  // first entry with jat > j
+  while (e1 < e2) {
+    const int mid = (e1 + e2) >> 1;
+    if (jl[mid] <= jat) e1 = mid + 1;
+    else e2 = mid;
+  }
+  __syncthreads();
+  constexpr int NB = (2 * LMAX + 1) * (2 * LMAX + 1);
+  double st[NB], dt[3 * NB], qt[6 * NB];
+  const bool mirror = iat != jat;
+  // one item per image, and for the pair (i, i) one more for the on-site block
+  const int nitem = (e1 - lo) + (mirror ? 0 : 1);
+  for (int it = tid; it < nitem; it += PBC_INT_THREADS) {
+    const bool onsite = it == e1 - lo;
+    double vec[3];
+    for (int k = 0; k < 3; ++k) vec[k] = xyz[3 * iat + k] - xyz[3 * jat + k];
+    if (!onsite) {
+      const double* t = tr + 3 * tl[lo + it];
+      for (int k = 0; k < 3; ++k) vec[k] -= t[k];
+    }
+    const double r2 = vec[0] * vec[0] + vec[1] * vec[1] + vec[2] * vec[2];
+    const double rr = sqrt(sqrt(r2) / (ej.atomic_rad + ei.atomic_rad));
+    int ioff = 0;
+    for (int ish = 0; ish < ei.nsh; ++ish) {
+      const int li = ei.ang[ish], si = 2 * li + 1;
+      int joff = 0;
+      for (int jsh = 0; jsh < ej.nsh; ++jsh) {
+        const int lj = ej.ang[jsh], sj = 2 * lj + 1;
+        shell_block<LMAX>(li, lj, ej, jsh, ei, ish, r2, vec, intcut, st, dt, qt);
+        double hij = 0.0;
+        if (!onsite) {
+          const double shp = (1.0 + ei.shpoly[ish] * rr) * (1.0 + ej.shpoly[jsh] * rr);
+          hij = 0.5 * (se[is + ish] + se[js + jsh]) * P.hscale[ei_][ej_][ish][jsh] * shp;
+        }
+        for (int iao = 0; iao < si; ++iao)
+          for (int jao = 0; jao < sj; ++jao) {
+            const int ij = jao + sj * iao;
+            const double sv = st[ij];
+            const double* di = dt + 3 * ij;
+            const double* qi = qt + 6 * ij;
+            const int e = (ioff + iao) * naj + (joff + jao);
+            if (sv == 0.0 && di[0] == 0.0 && di[1] == 0.0 && di[2] == 0.0 && qi[0] == 0.0 &&
+                qi[1] == 0.0 && qi[2] == 0.0 && qi[3] == 0.0 && qi[4] == 0.0 && qi[5] == 0.0)
+              continue;  // every primitive pair screened
+            atomicAdd(blk + e, sv);
+            if (!onsite) atomicAdd(blk + ne + e, sv * hij);
+            for (int k = 0; k < 3; ++k) atomicAdd(blk + (2 + k) * ne + e, di[k]);
+            for (int k = 0; k < 6; ++k) atomicAdd(blk + (5 + k) * ne + e, qi[k]);
+            if (mirror) {
+              double dj[3], qj[6];
+              dj[0] = di[0] + vec[0] * sv;
+              dj[1] = di[1] + vec[1] * sv;
+              dj[2] = di[2] + vec[2] * sv;
+              double t0 = 2 * vec[0] * di[0] + vec[0] * vec[0] * sv;
+              double t2 = 2 * vec[1] * di[1] + vec[1] * vec[1] * sv;
+              double t5 = 2 * vec[2] * di[2] + vec[2] * vec[2] * sv;
+              double t1 = vec[0] * di[1] + vec[1] * di[0] + vec[0] * vec[1] * sv;
+              double t3 = vec[0] * di[2] + vec[2] * di[0] + vec[0] * vec[2] * sv;
+              double t4 = vec[1] * di[2] + vec[2] * di[1] + vec[1] * vec[2] * sv;
+              double trc = 0.5 * (t0 + t2 + t5);
+              qj[0] = qi[0] + 1.5 * t0 - trc;
+              qj[1] = qi[1] + 1.5 * t1;
+              qj[2] = qi[2] + 1.5 * t2 - trc;
+              qj[3] = qi[3] + 1.5 * t3;
+              qj[4] = qi[4] + 1.5 * t4;
+              qj[5] = qi[5] + 1.5 * t5 - trc;
+              for (int k = 0; k < 3; ++k) atomicAdd(blk + (11 + k) * ne + e, dj[k]);
+              for (int k = 0; k < 6; ++k) atomicAdd(blk + (14 + k) * ne + e, qj[k]);
+            }
+          }
+        joff += sj;
+      }
+      ioff += si;
+    }
+  }
+  __syncthreads();
+  double* __restrict__ S = b.S + b.mol_mat_off[m];
+  double* __restrict__ H = b.H + b.mol_mat_off[m];
+  double* __restrict__ D = b.D + 3 * b.mol_mat_off[m];
+  double* __restrict__ Q = b.Q + 6 * b.mol_mat_off[m];
+  for (int e = tid; e < ne; e += PBC_INT_THREADS) {
+    const int a = e / naj, c = e - (e / naj) * naj;
+    const int64_t a_ij = (int64_t)(ii0 + a) * n + (jj0 + c);  // fortran (jj+c, ii+a)
+    double h = blk[ne + e];
+    if (!mirror && a == c) {
+      // self energy on the diagonal of the on-site block
+      int off = 0, sh = 0;
+      while (off + 2 * ei.ang[sh] + 1 <= a) off += 2 * ei.ang[sh++] + 1;
+      h += se[is + sh];
+    }
+    S[a_ij] = blk[e];
+    H[a_ij] = h;
+    for (int k = 0; k < 3; ++k) D[k * n2 + a_ij] = blk[(2 + k) * ne + e];
+    for (int k = 0; k < 6; ++k) Q[k * n2 + a_ij] = blk[(5 + k) * ne + e];
+    if (mirror) {
+      const int64_t a_ji = (int64_t)(jj0 + c) * n + (ii0 + a);
+      S[a_ji] = blk[e];
+      H[a_ji] = h;
+      for (int k = 0; k < 3; ++k) D[k * n2 + a_ji] = blk[(11 + k) * ne + e];
+      for (int k = 0; k < 6; ++k) Q[k * n2 + a_ji] = blk[(14 + k) * ne + e];
+    }
+  }
+}
+
+template __global__ void k_pbc_integrals<1>(BatchDev, const int*, const int*, const int*);
+template __global__ void k_pbc_integrals<2>(BatchDev, const int*, const int*, const int*);
+template __global__ void k_integrals<1>(BatchDev, const int*, const int*, const int*);
+template __global__ void k_integrals<2>(BatchDev, const int*, const int*, const int*);

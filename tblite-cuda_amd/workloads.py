@@ -11,6 +11,8 @@ Workloads (sizes as used by the tests and the benchmark):
   config1(count=4096)            (H2O)n, n ~ U{2..16}, liquid-density radius, seed 1
   config2()                      one (H2O)700 at liquid density, seed 2
   config3(count=2048)            tree-grown C/N/O/S organics, 4..60 heavy atoms, seed 3
+  cells(count=512)               periodic water cells, n ~ U{4..16} molecules
+                                 at liquid density, half of them triclinic, seed 8
   config4(count=20000)           request stream, 50 % water / 50 % organic,
                                  atoms log-uniform in 6..400, request k from
                                  child k of SeedSequence(4)
@@ -286,3 +288,11 @@ def periodic_cells(count=4, seed=5, first=4):
     every other one triclinic (shear 0.15)."""
     rng = np.random.default_rng(seed)
     return [water_box(first + 2 * k, rng, shear=0.15 if k % 2 else 0.0) for k in range(count)]
+
+
+def cells(count=512, seed=8):
+    """Seeded periodic water cells: n ~ U{4..16} molecules at liquid density,
+    every other cell triclinic (shear 0.15)."""
+    rng = np.random.default_rng(seed)
+    return [water_box(int(rng.integers(4, 17)), rng, shear=0.15 if k % 2 else 0.0)
+            for k in range(count)]

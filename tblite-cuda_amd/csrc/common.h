@@ -243,8 +243,8 @@ struct BatchDev {
   // number of lattice translations (0 for a molecule), their offset into
   // trans (3 doubles each, in the order of the neighbour list's translation
   // index), the reciprocal vectors G != 0 of the Ewald sums (mol_ng,
-  // mol_g_off into recip) and the Ewald constants (TA_PBC_EWALD doubles:
-  // cell volume, splitting parameter, real-space cutoff); per neighbour-list
+  // mol_g_off into recip) and the cell constants (TA_PBC_EWALD doubles, see
+  // TA_PBC_VOL ...); per neighbour-list
   // entry its translation index; per atom of a cell the three-body sites,
   // the images of atoms within the three-body cutoff (capacity-based offset
   // atm_site_off, count atm_site_n, 4 doubles each: R_site - R_atom and the
@@ -280,7 +280,42 @@ struct BatchDev {
 #define TA_PBC_ES2_ROFF 25.0
 #define TA_PBC_AES_CUT 40.0
 #define TA_PBC_EWALD_X 6.0
-#define TA_PBC_EWALD 4
+// per-cell constants: volume, splitting parameter, Ewald real-space cutoff,
+// the reciprocal vectors b_k (a_j . b_k = 2 pi delta_jk), the spacings d_k of
+// the lattice planes and the translation index ranges N_k
+#define TA_PBC_VOL 0
+#define TA_PBC_ALPHA 1
+#define TA_PBC_RREAL 2
+#define TA_PBC_REC 4
+#define TA_PBC_DK 13
+#define TA_PBC_NREP 16
+#define TA_PBC_EWALD 20
+
+// Translations of a cell's table that can lie within c of the separation d
+// (R_i - R_j): the box n_k in [f_k - c / d_k, f_k + c / d_k] around the
+// fractional separation f (|d - T| >= d_k |f_k - n_k|), widened by one and
+// clipped to the table.  Table index of (n1, n2, n3):
+// ((n1 + N1)(2 N2 + 1) + n2 + N2)(2 N3 + 1) + n3 + N3, ascending in the
+// nested order n1, n2, n3, so a box walk visits translations in table order.
+__host__ __device__ inline void cell_box(const double* cst, const double d[3], double c, int lo[3],
+                                         int hi[3])
+{
+  for (int k = 0; k < 3; ++k) {
+    const double* bk = cst + TA_PBC_REC + 3 * k;
+    const double f = (d[0] * bk[0] + d[1] * bk[1] + d[2] * bk[2]) * (1.0 / (2.0 * M_PI));
+    const double w = c / cst[TA_PBC_DK + k];
+    const int nk = (int)cst[TA_PBC_NREP + k];
+    const int l = (int)floor(f - w) - 1, h = (int)ceil(f + w) + 1;
+    lo[k] = l > -nk ? l : -nk;
+    hi[k] = h < nk ? h : nk;
+  }
+}
+
+__host__ __device__ inline int cell_tindex(const double* cst, int n1, int n2, int n3)
+{
+  const int N1 = (int)cst[TA_PBC_NREP], N2 = (int)cst[TA_PBC_NREP + 1], N3 = (int)cst[TA_PBC_NREP + 2];
+  return ((n1 + N1) * (2 * N2 + 1) + (n2 + N2)) * (2 * N3 + 1) + (n3 + N3);
+}
 
 // D4 weight blocks per atom of the gradient (BatchDev::g_d4)
 #define TA_GD4 5

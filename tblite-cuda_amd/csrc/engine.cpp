@@ -264,6 +264,19 @@ struct CellTables {
   int nrep[3] = {0, 0, 0};          // translation index ranges [-N_k, N_k]
   double rec[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, dk[3] = {0, 0, 0};  // b_k, plane spacings
   std::vector<double> tr, g;
+  // the per-cell constant block of the device (TA_PBC_EWALD doubles)
+  void constants(double* cst) const
+  {
+    for (int k = 0; k < TA_PBC_EWALD; ++k) cst[k] = 0.0;
+    cst[TA_PBC_VOL] = vol;
+    cst[TA_PBC_ALPHA] = alpha;
+    cst[TA_PBC_RREAL] = rreal;
+    for (int k = 0; k < 9; ++k) cst[TA_PBC_REC + k] = rec[k];
+    for (int k = 0; k < 3; ++k) {
+      cst[TA_PBC_DK + k] = dk[k];
+      cst[TA_PBC_NREP + k] = nrep[k];
+    }
+  }
 };
 
 bool cell_tables(const double* lat, int nat, const double* xyz, CellTables& ct)
@@ -343,8 +356,8 @@ void cell_counts(int nat, const double* xyz, const CellTables& ct, double cutoff
   const double c2 = cutoff * cutoff * (1.0 + 1.0e-9) + 1.0e-9;
   const double s2 = cut3 * cut3 * (1.0 + 1.0e-9) + 1.0e-9;
   const double cmax = std::max(cutoff, cut3) * (1.0 + 1.0e-9) + 1.0e-9;
-  const double pi2 = 2.0 * 3.14159265358979323846;
-  const int w2 = 2 * ct.nrep[2] + 1, w1 = 2 * ct.nrep[1] + 1;
+  double cst[TA_PBC_EWALD];
+  ct.constants(cst);
   nl = 0;
   sites.assign(nat, 0);
   // pairs j <= i once: the list entries of i, and the sites of anchor j
@@ -353,16 +366,13 @@ void cell_counts(int nat, const double* xyz, const CellTables& ct, double cutoff
     for (int j = 0; j <= i; ++j) {
       const double dx = xyz[3 * i] - xyz[3 * j], dy = xyz[3 * i + 1] - xyz[3 * j + 1];
       const double dz = xyz[3 * i + 2] - xyz[3 * j + 2];
+      const double d[3] = {dx, dy, dz};
       int lo[3], hi[3];
-      for (int k = 0; k < 3; ++k) {
-        const double f = (dx * ct.rec[3 * k] + dy * ct.rec[3 * k + 1] + dz * ct.rec[3 * k + 2]) / pi2;
-        lo[k] = std::max(-ct.nrep[k], (int)std::floor(f - cmax / ct.dk[k]) - 1);
-        hi[k] = std::min(ct.nrep[k], (int)std::ceil(f + cmax / ct.dk[k]) + 1);
-      }
+      cell_box(cst, d, cmax, lo, hi);
       for (int n1 = lo[0]; n1 <= hi[0]; ++n1)
         for (int n2 = lo[1]; n2 <= hi[1]; ++n2)
           for (int n3 = lo[2]; n3 <= hi[2]; ++n3) {
-            const int t = ((n1 + ct.nrep[0]) * w1 + (n2 + ct.nrep[1])) * w2 + (n3 + ct.nrep[2]);
+            const int t = cell_tindex(cst, n1, n2, n3);
             const double x = dx - tr[3 * t], y = dy - tr[3 * t + 1], z = dz - tr[3 * t + 2];
             const double r2 = x * x + y * y + z * z;
             if (r2 <= c2) ++nl;
@@ -769,6 +779,46 @@ int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* 
     pl.mol_g_off.assign(nmol, 0);
     pl.mol_ewald.assign((size_t)TA_PBC_EWALD * nmol, 0.0);
   }
+  // cells: lattice tables and list capacities of every cell first, spread
+  // over host threads (the capacity counts are O(nat^2) box walks per cell)
+  struct CellPre {
+    bool ok = false;
+    CellTables ct;
+    int64_t nl = 0;
+    std::vector<int> sites;
+  };
+  std::vector<CellPre> pre;
+  if (lattice) {
+    pre.resize(nmol);
+    std::vector<int64_t> aoff(nmol + 1, 0);
+    for (int m = 0; m < nmol; ++m) aoff[m + 1] = aoff[m] + std::max(0, mol_nat[m]);
+    auto work = [&](int m) {
+      const int nat = mol_nat[m];
+      if ((is_cell && !is_cell[m]) || nat < 1) return;
+      double min_alpha = INFINITY;
+      for (int i = 0; i < nat; ++i) {
+        const int z = num[aoff[m] + i];
+        const int e = (z > 0 && z < 120) ? c.zmap[z] : -1;
+        if (e < 0) return;  // reported by the molecule loop
+        min_alpha = std::min(min_alpha, c.min_alpha_el[e]);
+      }
+      CellPre& p = pre[m];
+      if (!cell_tables(lattice + 9 * (size_t)m, nat, xyz + 3 * aoff[m], p.ct)) return;
+      p.ok = true;
+      if ((int64_t)p.ct.tr.size() / 3 > (1 << 20)) return;  // rejected by the molecule loop
+      cell_counts(nat, xyz + 3 * aoff[m], p.ct, get_cutoff(min_alpha, acc),
+                  c.dev.d4.enabled ? c.dev.d4.disp3_cutoff : 0.0, p.nl, p.sites);
+    };
+    const unsigned hc = std::thread::hardware_concurrency();
+    const int nth = std::max(1, std::min(nmol, std::min(16, (int)(hc ? hc : 1))));
+    std::atomic<int> next{0};
+    std::vector<std::thread> th;
+    for (int t = 0; t < nth; ++t)
+      th.emplace_back([&]() {
+        for (int m = next++; m < nmol; m = next++) work(m);
+      });
+    for (auto& t : th) t.join();
+  }
   int64_t atom = 0, shell = 0, ao = 0, mat = 0, nl = 0, gam = 0, pair = 0, mix = 0, mixmem = 0,
           big = 0, scr = 0;
   std::vector<int> wd_mol, wd_kind, wd_start;  // integral work of molecules with d shells
@@ -821,11 +871,11 @@ int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* 
     int64_t nl_cap = (int64_t)nat * (nat + 1) / 2;
     std::vector<int> site_cnt;
     if (cell) {
-      CellTables ct;
-      if (!cell_tables(lattice + 9 * (size_t)m, nat, xyz + 3 * atom, ct)) {
+      if (!pre[m].ok) {
         set_error(ctx, "molecule %d: degenerate lattice", m);
         return TBLITE_AMD_ERR_ARGUMENT;
       }
+      const CellTables& ct = pre[m].ct;
       const int64_t ntr = (int64_t)ct.tr.size() / 3;
       if (ntr > (1 << 20)) {
         set_error(ctx, "molecule %d: %lld lattice translations (cell too thin for the cutoffs)", m,
@@ -838,11 +888,9 @@ int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* 
       pl.mol_ng[m] = (int)(ct.g.size() / 3);
       pl.mol_g_off[m] = (int64_t)pl.recip.size() / 3;
       pl.recip.insert(pl.recip.end(), ct.g.begin(), ct.g.end());
-      pl.mol_ewald[TA_PBC_EWALD * m] = ct.vol;
-      pl.mol_ewald[TA_PBC_EWALD * m + 1] = ct.alpha;
-      pl.mol_ewald[TA_PBC_EWALD * m + 2] = ct.rreal;
-      cell_counts(nat, xyz + 3 * atom, ct, pl.mol_cutoff[m],
-                  c.dev.d4.enabled ? c.dev.d4.disp3_cutoff : 0.0, nl_cap, site_cnt);
+      ct.constants(pl.mol_ewald.data() + TA_PBC_EWALD * (size_t)m);
+      nl_cap = pre[m].nl;
+      site_cnt.swap(pre[m].sites);
       if (nl_cap > (int64_t)1 << 30) {
         set_error(ctx, "molecule %d: neighbour list of %lld entries", m, (long long)nl_cap);
         return TBLITE_AMD_ERR_TOO_LARGE;

@@ -42,8 +42,9 @@ __device__ __forceinline__ double es2_switch(double r)
   return 1.0 - x * x * x * (10.0 + x * (-15.0 + 6.0 * x));
 }
 
-// ES2 row elements, AES2 planes, D4 damping and zero-charge C6 of the atom
-// pairs (i, j) with i in [a_begin, a_end) of cell m (work item w).
+// ES2 row elements, AES2 planes, D4 damping factors and sqrt(|C6_ij(q = 0)|)
+// (in d4_c60: the only reader for a cell is k_pbc_atm) of the atom pairs
+// (i, j) with i in [a_begin, a_end) of cell m (work item w).
 __global__ __launch_bounds__(PBC_THREADS) void k_pbc_pairs(BatchDev b, const int* __restrict__ work_mol,
                                                            const int* __restrict__ work_a0,
                                                            const int* __restrict__ work_a1)
@@ -59,14 +60,16 @@ __global__ __launch_bounds__(PBC_THREADS) void k_pbc_pairs(BatchDev b, const int
   const int ntr = b.mol_ntr[m], ng = b.mol_ng[m];
   const double* __restrict__ tr = b.trans + 3 * b.mol_tr_off[m];
   const double* __restrict__ gv = b.recip + 3 * b.mol_g_off[m];
-  const double* ew = b.mol_ewald + TA_PBC_EWALD * m;
-  const double vol = ew[0], al = ew[1], rreal = ew[2];
+  const double* cst = b.mol_ewald + TA_PBC_EWALD * m;
+  const double vol = cst[TA_PBC_VOL], al = cst[TA_PBC_ALPHA], rreal = cst[TA_PBC_RREAL];
   const double fac = 4.0 * M_PI / vol, a4 = 4.0 * al * al;
   const double tpi = 2.0 * al / sqrt(M_PI);
   const double rcut_es = fmax(TA_PBC_ES2_ROFF, rreal);
   const double es_cut2 = rcut_es * rcut_es, rr2 = rreal * rreal;
   const double ra2 = TA_PBC_AES_CUT * TA_PBC_AES_CUT;
   const double d2_cut2 = d4.disp2_cutoff * d4.disp2_cutoff;
+  // every term vanishes beyond the largest of its cutoffs
+  const double rmax = fmax(fmax(rcut_es, rreal), fmax(TA_PBC_AES_CUT, d4.enabled ? d4.disp2_cutoff : 0.0));
   const double gexp = P.gexp;
   const double* __restrict__ mrad = b.mrad + a0;
   const int64_t n2 = (int64_t)nat * nat;
@@ -118,7 +121,12 @@ __global__ __launch_bounds__(PBC_THREADS) void k_pbc_pairs(BatchDev b, const int
     double csq[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     double dv = 0.0;
     const double mr = 0.5 * (mrad[j] + mrad[i]);
-    for (int t = 0; t < ntr; ++t) {
+    int blo[3], bhi[3];
+    cell_box(cst, R, rmax, blo, bhi);
+    for (int n1 = blo[0]; n1 <= bhi[0]; ++n1)
+    for (int n2 = blo[1]; n2 <= bhi[1]; ++n2)
+    for (int n3 = blo[2]; n3 <= bhi[2]; ++n3) {
+      const int t = cell_tindex(cst, n1, n2, n3);
       double v[3];
       for (int x = 0; x < 3; ++x) v[x] = R[x] - tr[3 * t + x];
       const double r2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
@@ -193,7 +201,8 @@ __global__ __launch_bounds__(PBC_THREADS) void k_pbc_pairs(BatchDev b, const int
       double cv = 0.0;
       for (int r = 0; r < d4.nref[el[i]]; ++r) cv += gw0[TA_D4_MAXREF * i + r] * acc[r];
       b.d4_disp[b.mol_pair_off[m] + idx] = d4.qref_uniform ? dv * cv : dv;
-      b.d4_c60[b.mol_pair_off[m] + idx] = cv;
+      // a cell's three-body term takes the product of three square roots
+      b.d4_c60[b.mol_pair_off[m] + idx] = sqrt(fabs(cv));
     }
   }
 }
@@ -207,35 +216,45 @@ __global__ __launch_bounds__(PBC_THREADS) void k_pbc_atm_sites(BatchDev b, const
   __shared__ int red[PBC_THREADS / TA_WAVE];
   const int w = blockIdx.x, tid = threadIdx.x;
   const int m = work_mol[w], i = work_atom[w];
-  const int a0 = b.mol_atom_off[m], nat = b.mol_nat[m], ntr = b.mol_ntr[m];
+  const int a0 = b.mol_atom_off[m], nat = b.mol_nat[m];
   const double* __restrict__ xyz = b.xyz + 3 * (int64_t)a0;
   const double* __restrict__ tr = b.trans + 3 * b.mol_tr_off[m];
-  const double cut2 = b.par->d4.disp3_cutoff * b.par->d4.disp3_cutoff;
+  const double* cst = b.mol_ewald + TA_PBC_EWALD * m;
+  const double cut = b.par->d4.disp3_cutoff, cut2 = cut * cut;
   const int64_t off = b.atm_site_off[a0 + i];
   const int cap = b.atm_site_cap[a0 + i];
   double* __restrict__ site = b.atm_site + 4 * off;
-  const int64_t ncand = (int64_t)nat * ntr;
   int base = 0;
-  for (int64_t c0 = (int64_t)i * ntr; c0 < ncand; c0 += PBC_THREADS) {
-    const int64_t c = c0 + tid;
-    double v[3] = {0.0, 0.0, 0.0};
-    int j = 0, keep = 0;
-    if (c < ncand) {
-      j = (int)(c / ntr);
-      const int t = (int)(c - (int64_t)j * ntr);
-      for (int x = 0; x < 3; ++x) v[x] = (xyz[3 * j + x] - xyz[3 * i + x]) - tr[3 * t + x];
-      const double r2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-      keep = (r2 >= 1.0e-12 && r2 <= cut2) ? 1 : 0;
+  for (int j = i; j < nat; ++j) {
+    // the box of translations around R_j - R_i, walked in table order
+    const double d[3] = {xyz[3 * j] - xyz[3 * i], xyz[3 * j + 1] - xyz[3 * i + 1],
+                         xyz[3 * j + 2] - xyz[3 * i + 2]};
+    int lo[3], hi[3];
+    cell_box(cst, d, cut, lo, hi);
+    const int w1 = hi[1] - lo[1] + 1, w2 = hi[2] - lo[2] + 1;
+    const int nb = (hi[0] - lo[0] + 1) * w1 * w2;
+    for (int c0 = 0; c0 < nb; c0 += PBC_THREADS) {
+      const int c = c0 + tid;
+      double v[3] = {0.0, 0.0, 0.0};
+      int keep = 0;
+      if (c < nb) {
+        const int q = c / w2;
+        const int n1 = lo[0] + q / w1, n2 = lo[1] + q % w1, n3 = lo[2] + c % w2;
+        const int t = cell_tindex(cst, n1, n2, n3);
+        for (int x = 0; x < 3; ++x) v[x] = d[x] - tr[3 * t + x];
+        const double r2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+        keep = (r2 >= 1.0e-12 && r2 <= cut2) ? 1 : 0;
+      }
+      int tot;
+      const int pos = base + block_exclusive_scan(keep, red, &tot);
+      if (keep && pos < cap) {
+        site[4 * pos] = v[0];
+        site[4 * pos + 1] = v[1];
+        site[4 * pos + 2] = v[2];
+        site[4 * pos + 3] = (double)j;
+      }
+      base += tot;
     }
-    int tot;
-    const int pos = base + block_exclusive_scan(keep, red, &tot);
-    if (keep && pos < cap) {
-      site[4 * pos] = v[0];
-      site[4 * pos + 1] = v[1];
-      site[4 * pos + 2] = v[2];
-      site[4 * pos + 3] = (double)j;
-    }
-    base += tot;
   }
   if (tid == 0) b.atm_site_n[a0 + i] = base < cap ? base : cap;
 }
@@ -259,6 +278,7 @@ __global__ __launch_bounds__(PBC_THREADS) void k_pbc_atm(BatchDev b, const int* 
                                                          const int* __restrict__ work_r1)
 {
   __shared__ double red[PBC_THREADS / TA_WAVE];
+  __shared__ double r0t[TA_MAX_ELEM * TA_MAX_ELEM];  // R0 of every element pair
   const int w = blockIdx.x, tid = threadIdx.x;
   const int m = work_mol[w], i = work_atom[w];
   const int a0 = b.mol_atom_off[m], nat = b.mol_nat[m];
@@ -266,38 +286,43 @@ __global__ __launch_bounds__(PBC_THREADS) void k_pbc_atm(BatchDev b, const int* 
   const int r0 = work_r0[w], r1 = min(work_r1[w], n);
   const double* __restrict__ site = b.atm_site + 4 * b.atm_site_off[a0 + i];
   const int* __restrict__ el = b.at_el + a0;
-  const double* __restrict__ c60 = b.d4_c60 + b.mol_pair_off[m];
+  const double* __restrict__ sc6 = b.d4_c60 + b.mol_pair_off[m];  // sqrt(|C6(0)|)
   const DevD4& d = b.par->d4;
+  const int ne = b.par->nelem;
+  for (int x = tid; x < ne * ne; x += PBC_THREADS) {
+    const int e1 = x / ne, e2 = x - (x / ne) * ne;
+    r0t[e1 * TA_MAX_ELEM + e2] = d.a1 * sqrt(3.0 * d.r4r2[e1] * d.r4r2[e2]) + d.a2;
+  }
+  __syncthreads();
   const double cut2 = d.disp3_cutoff * d.disp3_cutoff;
   const double p3 = d.alp / 3.0;
   const bool p16 = d.alp == 16.0;
-  const double ri = d.r4r2[el[i]];
+  const int ei = el[i];
+  const double* __restrict__ sci = sc6 + (int64_t)i * nat;
   double e = 0.0;
   for (int a = r0; a < r1; ++a) {
     const double* sa = site + 4 * a;
-    const int j = (int)sa[3];
+    const int j = (int)sa[3], ej = el[j];
     const double r2ij = sa[0] * sa[0] + sa[1] * sa[1] + sa[2] * sa[2];
-    const double rj = d.r4r2[el[j]];
-    const double r0ij = d.a1 * sqrt(3.0 * ri * rj) + d.a2;
-    const double c6ij = c60[(int64_t)i * nat + j];
+    const double r0ij = r0t[ei * TA_MAX_ELEM + ej];
+    const double cij = d.s9 * sci[j];
+    const double* __restrict__ scj = sc6 + (int64_t)j * nat;
     const double wa = (j == i) ? 1.0 : 0.0;
     for (int bb = a + 1 + tid; bb < n; bb += PBC_THREADS) {
       const double* sb = site + 4 * bb;
-      const int k = (int)sb[3];
       const double dx = sb[0] - sa[0], dy = sb[1] - sa[1], dz = sb[2] - sa[2];
       const double r2jk = dx * dx + dy * dy + dz * dz;
       if (r2jk < 1.0e-12 || r2jk > cut2) continue;
+      const int k = (int)sb[3], ek = el[k];
       const double r2ik = sb[0] * sb[0] + sb[1] * sb[1] + sb[2] * sb[2];
-      const double rk = d.r4r2[el[k]];
-      const double r0ik = d.a1 * sqrt(3.0 * ri * rk) + d.a2;
-      const double r0jk = d.a1 * sqrt(3.0 * rj * rk) + d.a2;
-      const double c9 = d.s9 * sqrt(fabs(c6ij * c60[(int64_t)i * nat + k] * c60[(int64_t)j * nat + k]));
-      const double r2ijk = r2ij * r2ik * r2jk, rijk = sqrt(r2ijk);
-      const double fdmp = 1.0 / (1.0 + 6.0 * atm_damp_pow(r0ij * r0ik * r0jk / rijk, p3, p16));
+      const double c9 = cij * sci[k] * scj[k];
+      const double r2ijk = r2ij * r2ik * r2jk;
+      const double irijk = 1.0 / sqrt(r2ijk), ir2 = irijk * irijk;
+      const double x = r0ij * r0t[ei * TA_MAX_ELEM + ek] * r0t[ej * TA_MAX_ELEM + ek] * irijk;
       const double ang = 0.375 * (r2ij + r2jk - r2ik) * (r2ij - r2jk + r2ik) *
-                             (-r2ij + r2jk + r2ik) / r2ijk + 1.0;
-      const double w = 1.0 / (1.0 + wa + (k == i ? 1.0 : 0.0));
-      e += w * c9 * ang * fdmp / (rijk * r2ijk);
+                             (-r2ij + r2jk + r2ik) * ir2 + 1.0;
+      const double wt = (k == i ? 2.0 : 1.0) + wa;  // sites of atom i in the triangle
+      e += c9 * ang * irijk * ir2 / (wt * (1.0 + 6.0 * atm_damp_pow(x, p3, p16)));
     }
   }
   e = block_sum(e, red);

@@ -99,16 +99,22 @@ __global__ __launch_bounds__(SETUP_THREADS) void k_setup(BatchDev b)
       }
     } else {
       const double* __restrict__ tr = b.trans + 3 * b.mol_tr_off[m];
+      const double* cst = b.mol_ewald + TA_PBC_EWALD * m;
       const double cut2 = fmax(cn_cut2, fmax(rep_cut2, d4.enabled ? d4_cut2 : 0.0));
       for (int j = 0; j < nat; ++j) {
         const DevElem& ej = P.el[el[j]];
-        const double dx0 = xi - xyz[3 * j], dy0 = yi - xyz[3 * j + 1], dz0 = zi - xyz[3 * j + 2];
-        for (int t = 0; t < ntr; ++t) {
-          const double dx = dx0 - tr[3 * t], dy = dy0 - tr[3 * t + 1], dz = dz0 - tr[3 * t + 2];
-          const double r2 = dx * dx + dy * dy + dz * dz;
-          if (r2 < 1.0e-12 || r2 > cut2) continue;
-          setup_pair_terms(P, ei, ej, r2, cn_cut2, d4_cut2, rep_cut2, cn, cn4, er);
-        }
+        const double d0[3] = {xi - xyz[3 * j], yi - xyz[3 * j + 1], zi - xyz[3 * j + 2]};
+        int lo[3], hi[3];
+        cell_box(cst, d0, sqrt(cut2), lo, hi);
+        for (int n1 = lo[0]; n1 <= hi[0]; ++n1)
+          for (int n2 = lo[1]; n2 <= hi[1]; ++n2)
+            for (int n3 = lo[2]; n3 <= hi[2]; ++n3) {
+              const int t = cell_tindex(cst, n1, n2, n3);
+              const double dx = d0[0] - tr[3 * t], dy = d0[1] - tr[3 * t + 1], dz = d0[2] - tr[3 * t + 2];
+              const double r2 = dx * dx + dy * dy + dz * dz;
+              if (r2 < 1.0e-12 || r2 > cut2) continue;
+              setup_pair_terms(P, ei, ej, r2, cn_cut2, d4_cut2, rep_cut2, cn, cn4, er);
+            }
       }
       b.d4_e3[a0 + i] = 0.0;
     }
@@ -329,9 +335,9 @@ __device__ void adjlist_cell(const BatchDev& b, int m, int* red)
 {
   const int tid = threadIdx.x;
   const int a0 = b.mol_atom_off[m], nat = b.mol_nat[m];
-  const int ntr = b.mol_ntr[m];
   const double* __restrict__ xyz = b.xyz + 3 * (int64_t)a0;
   const double* __restrict__ tr = b.trans + 3 * b.mol_tr_off[m];
+  const double* cst = b.mol_ewald + TA_PBC_EWALD * m;
   const double cutoff = b.mol_cutoff[m];
   const double cutoff2 = cutoff * cutoff;
   const double eps = 2.220446049250313e-16;
@@ -340,13 +346,24 @@ __device__ void adjlist_cell(const BatchDev& b, int m, int* red)
   int base = 0;
   for (int chunk = 0; chunk < nat; chunk += blockDim.x) {
     const int i = chunk + tid;
+    // translations of each pair's box in table order (the rest lie beyond
+    // the cutoff)
+    auto walk = [&](int j, auto&& visit) {
+      const double d[3] = {xyz[3 * i] - xyz[3 * j], xyz[3 * i + 1] - xyz[3 * j + 1],
+                           xyz[3 * i + 2] - xyz[3 * j + 2]};
+      int lo[3], hi[3];
+      cell_box(cst, d, cutoff, lo, hi);
+      for (int n1 = lo[0]; n1 <= hi[0]; ++n1)
+        for (int n2 = lo[1]; n2 <= hi[1]; ++n2)
+          for (int n3 = lo[2]; n3 <= hi[2]; ++n3) {
+            const int t = cell_tindex(cst, n1, n2, n3);
+            double r2 = pair_r2_image_exact(xyz, i, j, tr + 3 * t);
+            if (!(r2 < eps || r2 > cutoff2)) visit(t);
+          }
+    };
     int cnt = 0;
     if (i < nat)
-      for (int j = 0; j <= i; ++j)
-        for (int t = 0; t < ntr; ++t) {
-          double r2 = pair_r2_image_exact(xyz, i, j, tr + 3 * t);
-          if (!(r2 < eps || r2 > cutoff2)) ++cnt;
-        }
+      for (int j = 0; j <= i; ++j) walk(j, [&](int) { ++cnt; });
     int tot;
     int excl = block_exclusive_scan(cnt, red, &tot);
     if (i < nat) {
@@ -354,14 +371,11 @@ __device__ void adjlist_cell(const BatchDev& b, int m, int* red)
       b.nl_nnl[a0 + i] = cnt;
       int pos = base + excl;
       for (int j = 0; j <= i; ++j)
-        for (int t = 0; t < ntr; ++t) {
-          double r2 = pair_r2_image_exact(xyz, i, j, tr + 3 * t);
-          if (!(r2 < eps || r2 > cutoff2)) {
-            jat_out[pos] = j;
-            itr_out[pos] = t;
-            ++pos;
-          }
-        }
+        walk(j, [&](int t) {
+          jat_out[pos] = j;
+          itr_out[pos] = t;
+          ++pos;
+        });
     }
     base += tot;
   }

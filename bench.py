@@ -312,6 +312,8 @@ def main():
                     help="timed steps of singlepoint + analytic gradient (0: skip)")
     ap.add_argument("--cell-steps", type=int, default=2,
                     help="timed steps of the periodic-cell leg (512 water cells; 0: skip)")
+    ap.add_argument("--cells-only", action="store_true",
+                    help="run only the periodic-cell leg and print its JSON")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device copy of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -323,6 +325,11 @@ def main():
               "so a multi-GPU run is a split of the batch with no exchange", file=sys.stderr)
         sys.exit(2)
 
+    if args.cells_only:
+        eng = tblite_amd.Engine(device=0)
+        print(json.dumps(cell_leg(eng, max(1, args.cell_steps), not args.no_cpu)))
+        eng.close()
+        return
     if args.workload == "config4":
         if args.hw_queues > 0:
             os.environ["GPU_MAX_HW_QUEUES"] = str(min(32, args.hw_queues))

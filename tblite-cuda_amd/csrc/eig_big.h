@@ -620,8 +620,14 @@ __device__ __forceinline__ double start_vector(int i, int k)
 }
 
 // Inverse iteration: column i of Y (Y[k * ld + i]) receives the normalised
-// eigenvector of T for lam[i].  scr: 3 * ld * NT doubles of global scratch
-// (pivots^-1, first and second superdiagonals of U, slot-fastest).
+// eigenvector of T for lam[i].  scr: 3 * ld * NT doubles of global scratch,
+// slot-fastest: per elimination step k the pivot reciprocal 1 / U_kk (and
+// 1 / U_{n-1,n-1} at k = n - 1), the first superdiagonal U_k,k+1, and the
+// row interchange: 2 when rows k and k+1 were not exchanged (the multiplier
+// is then e_k / U_kk, and U_k,k+2 = 0), else the multiplier itself (|m| < 1;
+// U_k,k+2 = e_k+1).  The factorisation of T - lam is formed in the first
+// sweep only; the later sweeps reuse it, so their forward chain is one FMA
+// per step instead of a division.
 //
 // Each sweep is a dependency chain along k whose loads do not depend on the
 // chain, but follow the stores of earlier steps (which the compiler must
@@ -652,6 +658,37 @@ __device__ __noinline__ void inverse_iteration(const lds_dbl* d, const lds_dbl* 
         // forward elimination with partial pivoting, rhs transformed in place
         double p = d[0] - l, q = n > 1 ? e[0] : 0.0;
         double bp = it == 0 ? start_vector(i, 0) : gld(Y, i) * scale;
+        if (it > 0) {
+          // stored factors: the rhs only
+          for (int k0 = 0; k0 < n - 1; k0 += IB) {
+            double bq[IB], a0[IB], a2[IB];
+#pragma unroll
+            for (int u = 0; u < IB; ++u) {
+              const int k = k0 + u;
+              const unsigned kc = k + 1 < n ? k + 1 : n - 1;
+              const unsigned kf = k < n - 1 ? k : n - 2;
+              bq[u] = gld(Y, kc * ld + i) * scale;
+              a0[u] = gld(u0, kf * NT + s);
+              a2[u] = gld(u2, kf * NT + s);
+            }
+#pragma unroll
+            for (int u = 0; u < IB; ++u) {
+              const int k = k0 + u;
+              if (k < n - 1) {
+                double rk;
+                if (a2[u] == 2.0) {
+                  const double m = e[k] * a0[u];
+                  rk = bp;
+                  bp = bq[u] - m * bp;
+                } else {
+                  rk = bq[u];
+                  bp = bp - a2[u] * bq[u];
+                }
+                gst(Y, k * ld + i, rk);
+              }
+            }
+          }
+        } else
         for (int k0 = 0; k0 < n - 1; k0 += IB) {
           double bq[IB];
 #pragma unroll
@@ -666,13 +703,13 @@ __device__ __noinline__ void inverse_iteration(const lds_dbl* d, const lds_dbl* 
             if (k < n - 1) {
               const double sk = e[k], dk1 = d[k + 1] - l, ek1 = (k + 1 < n - 1) ? e[k + 1] : 0.0;
               const double bk1 = bq[u];
-              double U0, U1, U2, rk;
+              double U0, U1, X, rk;
               if (fabs(p) >= fabs(sk)) {
                 if (fabs(p) < pert) p = p < 0.0 ? -pert : pert;
                 const double m = sk / p;
                 U0 = p;
                 U1 = q;
-                U2 = 0.0;
+                X = 2.0;
                 rk = bp;
                 p = dk1 - m * q;
                 q = ek1;
@@ -681,7 +718,7 @@ __device__ __noinline__ void inverse_iteration(const lds_dbl* d, const lds_dbl* 
                 const double m = p / sk;
                 U0 = sk;
                 U1 = dk1;
-                U2 = ek1;
+                X = m;
                 rk = bk1;
                 p = q - m * dk1;
                 q = -m * ek1;
@@ -689,14 +726,22 @@ __device__ __noinline__ void inverse_iteration(const lds_dbl* d, const lds_dbl* 
               }
               gst(u0, k * NT + s, 1.0 / U0);
               gst(u1, k * NT + s, U1);
-              gst(u2, k * NT + s, U2);
+              gst(u2, k * NT + s, X);
               gst(Y, k * ld + i, rk);
             }
           }
         }
-        if (fabs(p) < pert) p = p < 0.0 ? -pert : pert;
+        // last pivot: formed in the first sweep, reused after
+        double ip;
+        if (it == 0) {
+          if (fabs(p) < pert) p = p < 0.0 ? -pert : pert;
+          ip = 1.0 / p;
+          gst(u0, (n - 1) * NT + s, ip);
+        } else {
+          ip = gld(u0, (n - 1) * NT + s);
+        }
         // back substitution
-        double x2 = 0.0, x1 = bp / p, nrm = x1 * x1;
+        double x2 = 0.0, x1 = bp * ip, nrm = x1 * x1;
         gst(Y, (n - 1) * ld + i, x1);
         for (int k1 = n - 2; k1 >= 0; k1 -= IB) {
           double yq[IB], a0[IB], a1[IB], a2[IB];
@@ -713,7 +758,9 @@ __device__ __noinline__ void inverse_iteration(const lds_dbl* d, const lds_dbl* 
           for (int u = 0; u < IB; ++u) {
             const int k = k1 - u;
             if (k >= 0) {
-              const double x = (yq[u] - a1[u] * x1 - a2[u] * x2) * a0[u];
+              // U_k,k+2: e_k+1 after a row interchange, else 0
+              const double u2k = a2[u] == 2.0 ? 0.0 : (k + 1 < n - 1 ? e[k + 1] : 0.0);
+              const double x = (yq[u] - a1[u] * x1 - u2k * x2) * a0[u];
               gst(Y, k * ld + i, x);
               nrm += x * x;
               x2 = x1;

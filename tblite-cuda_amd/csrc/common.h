@@ -459,6 +459,10 @@ __host__ __device__ inline ScfLds scf_lds_layout(int ld, int natmax, int nshmax,
   int w1 = ld + nshmax + 10 * natmax, w2 = 11 * ld;
   int t3 = 2048 + 2 * TA_BIG_NB * TA_BIG_NB, t4 = big_npart(ld) * ld;
   int w3 = mats ? 0 : 9 * ld + (t3 > t4 ? t3 : t4);
+  // two-stage reduction of molecules below 96 AOs: the 64 x 65 product tile
+  // at 8 ld, then the 560 panel scalars (above, they sit at 2 ld)
+  const int wb = 8 * (ld < 92 ? ld : 92) + 64 * 65 + 560;
+  if (!mats && w3 < wb) w3 = wb;
   L.work = off;
   L.eig = off;
   int wmax = w1 > w2 ? w1 : w2;

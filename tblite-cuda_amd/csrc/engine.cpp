@@ -758,14 +758,15 @@ int make_plan(tblite_amd_context ctx, const Calculator& c, int nmol, const int* 
       lds[m] = (nao + 3) & ~3;
     }
     route_molecules(ctx, lds, pl.mol_path);
-    // cells: the LDS-resident kernels (their CELL instantiation reads the
-    // Ewald-summed AES2 planes) up to lds_max_ld, else the multi-workgroup
-    // path; the one-workgroup global-memory kernel derives transposed
-    // multipole elements by the molecular origin shift, which the image sums
-    // of a cell do not obey
+    // cells by size, like molecules (no cost-model routing): the CELL
+    // instantiations of the one-workgroup kernels read the Ewald-summed AES2
+    // planes and, in the global-memory kernel's Fock build, the stored
+    // transposed multipole elements (a cell's image sums do not obey the
+    // molecular origin shift it derives them by)
     if (lattice)
       for (int m = 0; m < nmol; ++m)
-        if (!is_cell || is_cell[m]) pl.mol_path[m] = lds[m] <= ctx->lds_max_ld ? 0 : 2;
+        if (!is_cell || is_cell[m])
+          pl.mol_path[m] = lds[m] <= ctx->lds_max_ld ? 0 : (lds[m] <= ctx->grid_min_ld ? 1 : 2);
   }
   if (lattice && !xyz) {
     set_error(ctx, "internal: periodic plan without coordinates");
@@ -1691,7 +1692,8 @@ int run_batch(tblite_amd_context ctx, Calculator& c, const double* xyz, double a
     // 1128 over 448) and up to MAXC two-AO pieces of one slot's CT rows (4, 6
     // and 8 threads per slot)
     if (bk.cell && !bk.lds)
-      rc = TBLITE_AMD_ERR_ARGUMENT;  // cells above the LDS sizes run on the multi-workgroup path
+      rc = launch_scf<TA_BIG_NT, 1, 0, 0, false, true>(ctx, as, b, mols, cnt, bk.ldmax, bk.natmax,
+                                                       bk.nshmax);
     else if (bk.cell && bk.nt == 128)
       rc = launch_scf<128, 1, 2, 4, true, true>(ctx, as, b, mols, cnt, bk.ldmax, bk.natmax,
                                                 bk.nshmax);

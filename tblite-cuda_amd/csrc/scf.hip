@@ -850,7 +850,11 @@ __device__ __noinline__ void big_eigenvalues(double* A_, const double* X_, doubl
     // LDS: the products' tile from er + 8 ld (the row-sum vector of the
     // one-stage path and the tile region after it: ld + 4096 doubles), the
     // panel scalars at er + 2 ld; d and e are written last
-    big_band_tridiagonalize<NT>(A, scr, n, ld, d, e, er + 8 * ld, er + 2 * ld, red, clk);
+    // panel scalars (560 doubles) at er + 2 ld clear the tile at er + 8 ld
+    // from ld = 96 on; below, they follow the tile (the LDS layout reserves
+    // the room, scf_lds_layout)
+    lds_dbl* band_sm = ld >= 96 ? er + 2 * ld : er + 8 * ld + band::TILE_DOUBLES;
+    big_band_tridiagonalize<NT>(A, scr, n, ld, d, e, er + 8 * ld, band_sm, red, clk);
   } else {
     // per-wave column partials of the tridiagonalisation share the GEMM tile region
     gbl_dbl* wbt = scr + 3 * (int64_t)ld * TA_BIG_NT;
@@ -1541,10 +1545,19 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
         const int aj = v.ao_at[j];
         const double* vdj = vdp + 3 * aj;
         const double* vqj = vqp + 6 * aj;
+        double dj[3], qj[6];
+        if constexpr (CELL) {
+          // a cell's elements are sums over lattice images, each with its own
+          // shift: the element above the diagonal is read as stored
+          const int64_t et = (int64_t)j * n + i;
+#pragma unroll
+          for (int k = 0; k < 3; ++k) dj[k] = gD[k * n2 + et];
+#pragma unroll
+          for (int k = 0; k < 6; ++k) qj[k] = gQ[k * n2 + et];
+        } else {
         double vec[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) vec[k] = xi[k] - xyz[3 * aj + k];
-        double dj[3], qj[6];
         dj[0] = di[0] + vec[0] * sv;
         dj[1] = di[1] + vec[1] * sv;
         dj[2] = di[2] + vec[2] * sv;
@@ -1561,6 +1574,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
         qj[3] = qi[3] + 1.5 * t3;
         qj[4] = qi[4] + 1.5 * t4;
         qj[5] = qi[5] + 1.5 * t5 - tr;
+        }
         const double fji = base - 0.5 * (dj[0] * vdj[0] + dj[1] * vdj[1] + dj[2] * vdj[2]) -
                            0.5 * (qj[0] * vqj[0] + qj[1] * vqj[1] + qj[2] * vqj[2] +
                                   qj[3] * vqj[3] + qj[4] * vqj[4] + qj[5] * vqj[5]);
@@ -1978,3 +1992,4 @@ template __global__ void k_scf<512, 1, 0, 0, false, false>(BatchDev, const int*,
 template __global__ void k_scf<128, 1, 2, 4, true, true>(BatchDev, const int*, int, int, int);
 template __global__ void k_scf<256, 1, 3, 6, true, true>(BatchDev, const int*, int, int, int);
 template __global__ void k_scf<512, 2, 3, 6, true, true>(BatchDev, const int*, int, int, int);
+template __global__ void k_scf<512, 1, 0, 0, false, true>(BatchDev, const int*, int, int, int);

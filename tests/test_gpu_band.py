@@ -133,3 +133,24 @@ def test_band_vs_one_stage_full_device():
     assert np.all(st1 == 0) and np.all(st0 == 0)
     assert np.max(np.abs(e1 - e0)) < 1e-9, np.max(np.abs(e1 - e0))
     assert np.array_equal(it1, it0), np.nonzero(it1 != it0)[0][:10]
+
+
+def test_band_path_small_orders(orc):
+    """Molecules of 18-108 AOs forced onto the global-memory kernel (LDS limit
+    16): below 96 AOs the panel scalars of the band reduction must not share
+    LDS with its product tile (they did below 89 AOs: every such SCF failed)."""
+    rng = np.random.default_rng(3)
+    mols = [workloads.water_cluster(k, workloads.liquid_radius(k), rng) for k in range(3, 19)]
+    eng = tblite_amd.Engine(tuning={"auto_grid": 0, "lds_max_ld": 16})
+    try:
+        e, it, st = eng.singlepoint_flat([len(m[0]) for m in mols], np.concatenate([m[0] for m in mols]),
+                                         np.concatenate([m[1] for m in mols]))
+        paths = [eng.molecule_path(k) for k in range(len(mols))]
+    finally:
+        eng.close()
+    assert paths == [1] * len(mols)
+    for k, (num, xyz) in enumerate(mols):
+        r = orc.singlepoint(num, xyz)
+        assert st[k] == 0, (k, orc.dims(num)[2])
+        assert abs(e[k] - r["energy"]) < 1e-6, (k, orc.dims(num)[2], e[k] - r["energy"])
+        assert abs(int(it[k]) - r["iterations"]) <= 1

@@ -1,6 +1,6 @@
 """Periodic cells on the GPU against the oracle (Gamma point).
 
-Cells of 4, 12, 14, 16 and 18 rigid waters at liquid density (cubic and
+Cells of 4, 12, 14, 16, 18 and 40 rigid waters at liquid density (cubic and
 triclinic) run through tblite_amd_singlepoint_flat_periodic in one batch;
 every stage the oracle
 exposes is compared: the translation table and the neighbour list with its
@@ -9,9 +9,10 @@ energies, charges and the total energy (tolerances of the molecular tests:
 1e-12 for integer-free setup quantities, 1e-11 for integrals, 1e-6 Eh for
 energies, 1e-5 e for charges, iterations within one).  The periodic
 structure handle goes through tblite_amd_xtb_singlepoint.  Cells up to 96
-AOs run batched on the LDS-resident SCF kernels (their cell instantiation
-reads the Ewald-summed AES2 planes), larger ones on the multi-workgroup path;
-both are checked, and a cell gives the same energy on either.
+AOs run batched on the LDS-resident SCF kernels and up to 1024 AOs on the
+one-workgroup global-memory kernel (their cell instantiations read the
+Ewald-summed AES2 planes), larger ones on the multi-workgroup path; a cell
+gives the same energy on all three.
 """
 import numpy as np
 import pytest
@@ -26,7 +27,8 @@ def cells():
     small = workloads.periodic_cells(count=1, seed=6, first=4)
     mid = workloads.periodic_cells(count=3, seed=5, first=12)
     big = workloads.periodic_cells(count=1, seed=7, first=18)
-    return small + mid + big
+    large = [workloads.water_box(40, np.random.default_rng(12), shear=0.1)]
+    return small + mid + big + large
 
 
 @pytest.fixture(scope="module")
@@ -53,7 +55,7 @@ def test_cells_energy_and_charges(engine, orc, cells, run, refs):
         assert abs(e[k] - ref["energy"]) < 1e-6, (k, e[k], ref["energy"])
         assert abs(int(it[k]) - ref["iterations"]) <= 1
         nao = orc.dims(num)[2]
-        assert engine.molecule_path(k) == (0 if nao <= 96 else 2), (k, nao)
+        assert engine.molecule_path(k) == (0 if nao <= 96 else 1), (k, nao)
         ch = engine.charges(k)
         assert np.abs(ch["qsh"] - ref["qsh"]).max() < 1e-5
         assert np.abs(ch["qat"] - ref["qat"]).max() < 1e-5
@@ -117,14 +119,17 @@ def test_cell_gradient_and_bad_flags_rejected(engine, cells):
     assert "three-dimensional" in engine.error()
 
 
-def test_cell_same_on_lds_and_multi_workgroup_paths(cells, refs):
+@pytest.mark.parametrize("tuning,path", [({"lds_max_ld": 64}, 1),
+                                         ({"lds_max_ld": 64, "grid_min_ld": 64}, 2)])
+def test_cell_same_on_every_scf_path(cells, refs, tuning, path):
     import tblite_amd
-    num, xyz, lat = cells[1]
-    eng = tblite_amd.Engine(tuning={"auto_grid": 0, "lds_max_ld": 64})
+    num, xyz, lat = cells[1]  # 72 AOs: LDS-resident by default
+    eng = tblite_amd.Engine(tuning=dict(auto_grid=0, **tuning))
     try:
         e, it, st = eng.singlepoint_flat_periodic([len(num)], num, xyz, lat[None])
-        assert eng.molecule_path(0) == 2
+        assert eng.molecule_path(0) == path
     finally:
         eng.close()
     assert st[0] == 0
     assert abs(e[0] - refs[1]["energy"]) < 1e-6
+    assert abs(int(it[0]) - refs[1]["iterations"]) <= 1

@@ -600,8 +600,25 @@ __device__ __forceinline__ ChaseGeom chase_geom(int n, int s, int t)
   return q;
 }
 
+// Storage of the band during the chase: A's lower triangle in global memory
+// (row r, column c at r * ld + c), or an LDS copy of its 32 lower diagonals
+// (the band of half-width 16 and the bulge room below it; element (r, c),
+// 0 <= r - c < 32, at r * 32 + r - c).
+struct BandGbl {
+  gbl_dbl* A;
+  int ld;
+  __device__ __forceinline__ double get(int r, int c) const { return gld(A, r * ld + c); }
+  __device__ __forceinline__ void put(int r, int c, double v) const { gst(A, r * ld + c, v); }
+};
+struct BandLds {
+  lds_dbl* B;
+  __device__ __forceinline__ double get(int r, int c) const { return B[r * 32 + (r - c)]; }
+  __device__ __forceinline__ void put(int r, int c, double v) const { B[r * 32 + (r - c)] = v; }
+};
+
 // the task's diagonal block and block below (and, with left, its left block)
-__device__ __forceinline__ void chase_load(const gbl_dbl* A, int n, int ld, int s, int t, bool left,
+template <class BS>
+__device__ __forceinline__ void chase_load(const BS& A, int n, int s, int t, bool left,
                                            double* lb, double* dd, double* rr)
 {
   const int lane = threadIdx.x & 63;
@@ -610,10 +627,10 @@ __device__ __forceinline__ void chase_load(const gbl_dbl* A, int n, int ld, int 
 #pragma unroll
   for (int jj = 0; jj < 4; ++jj) {
     const int j = 4 * g + jj;
-    if (left) lb[jj] = (i < q.L && j < q.Lc) ? gld(A, (q.lo + i) * ld + q.c + j) : 0.0;
+    if (left) lb[jj] = (i < q.L && j < q.Lc) ? A.get(q.lo + i, q.c + j) : 0.0;
     const int a = i > j ? i : j, b = i > j ? j : i;
-    dd[jj] = (i < q.L && j < q.L) ? gld(A, (q.lo + a) * ld + q.lo + b) : 0.0;
-    rr[jj] = (i < q.Lr && j < q.L) ? gld(A, (q.hi + i) * ld + q.lo + j) : 0.0;
+    dd[jj] = (i < q.L && j < q.L) ? A.get(q.lo + a, q.lo + b) : 0.0;
+    rr[jj] = (i < q.Lr && j < q.L) ? A.get(q.hi + i, q.lo + j) : 0.0;
   }
 }
 
@@ -622,7 +639,8 @@ __device__ __forceinline__ void chase_load(const gbl_dbl* A, int n, int ld, int 
 // diagonal block and from the right to the block below; the blocks and the
 // reflector record are stored, rr keeps the updated block below (the next
 // task's left block).
-__device__ __forceinline__ void chase_apply(gbl_dbl* A, int n, int ld, int s, int t, double* lb,
+template <class BS>
+__device__ __forceinline__ void chase_apply(const BS& A, int n, int s, int t, double* lb,
                                             double* dd, double* rr, gbl_dbl* V2)
 {
   const int lane = threadIdx.x & 63;
@@ -675,9 +693,9 @@ __device__ __forceinline__ void chase_apply(gbl_dbl* A, int n, int ld, int s, in
 #pragma unroll
   for (int jj = 0; jj < 4; ++jj) {
     const int j = 4 * g + jj;
-    if (i < q.L && j < q.Lc) gst(A, (q.lo + i) * ld + q.c + j, lb[jj]);
-    if (i < q.L && j <= i) gst(A, (q.lo + i) * ld + q.lo + j, dd[jj]);
-    if (i < q.Lr && j < q.L) gst(A, (q.hi + i) * ld + q.lo + j, rr[jj]);
+    if (i < q.L && j < q.Lc) A.put(q.lo + i, q.c + j, lb[jj]);
+    if (i < q.L && j <= i) A.put(q.lo + i, q.lo + j, dd[jj]);
+    if (i < q.Lr && j < q.L) A.put(q.hi + i, q.lo + j, rr[jj]);
   }
   if (g == 0) gst(V2, (chase_first(n, s) + t) * BW + i, i == 0 ? tau : vi);
 }
@@ -699,16 +717,19 @@ __device__ __forceinline__ void chase_apply(gbl_dbl* A, int n, int ld, int s, in
 // t + 3 finished; task t + 4 and later touch rows below them), so that load
 // overlaps the store's completion; its left block is the block below of the
 // task just done, kept in registers.
-template <int NT>
-__device__ __noinline__ void band_chase(gbl_dbl* A, int n, int ld, gbl_dbl* V2, lds_dbl* d,
-                                        lds_dbl* e, lds_int* stat)
+//
+// band (optional, LDS, 32 n doubles): the chase runs on an LDS copy of the
+// band's 32 lower diagonals (copied in first; stage 1 leaves zeros in the
+// bulge room), so each task's blocks cost LDS round trips instead of memory
+// round trips; A is then only read.
+template <int NT, class BS>
+__device__ __forceinline__ void band_chase_on(const BS& A, int n, gbl_dbl* V2, lds_dbl* d,
+                                              lds_dbl* e, lds_int* stat)
 {
   constexpr int NW = NT / 64;
   constexpr int SPIN_MAX = 1 << 26;
-  A = sgpr_ptr(A);
   V2 = sgpr_ptr(V2);
   n = sgpr_int(n);
-  ld = sgpr_int(ld);
   const int lane = threadIdx.x & 63, w = wave_id();
   if (threadIdx.x < NW + 1) stat[threadIdx.x] = threadIdx.x < NW ? -1 : 0;
   __syncthreads();
@@ -730,15 +751,15 @@ __device__ __noinline__ void band_chase(gbl_dbl* A, int n, int ld, gbl_dbl* V2, 
     const int ns = chase_tasks(n, s);
     double lb[4], dd[4], rr[4];
     wait_passed(s - 1, 2);
-    chase_load(A, n, ld, s, 0, true, lb, dd, rr);
+    chase_load(A, n, s, 0, true, lb, dd, rr);
     for (int t = 0; t < ns && ok; ++t) {
-      chase_apply(A, n, ld, s, t, lb, dd, rr, V2);
+      chase_apply(A, n, s, t, lb, dd, rr, V2);
       bool pre = false;
       if (t + 1 < ns) {
 #pragma unroll
         for (int jj = 0; jj < 4; ++jj) lb[jj] = rr[jj];
         if (passed(s - 1, t + 3)) {
-          chase_load(A, n, ld, s, t + 1, false, lb, dd, rr);
+          chase_load(A, n, s, t + 1, false, lb, dd, rr);
           pre = true;
         }
       }
@@ -747,7 +768,7 @@ __device__ __noinline__ void band_chase(gbl_dbl* A, int n, int ld, gbl_dbl* V2, 
                            __HIP_MEMORY_SCOPE_WORKGROUP);
       if (t + 1 < ns && !pre) {
         wait_passed(s - 1, t + 3);
-        chase_load(A, n, ld, s, t + 1, false, lb, dd, rr);
+        chase_load(A, n, s, t + 1, false, lb, dd, rr);
       }
     }
   }
@@ -755,10 +776,26 @@ __device__ __noinline__ void band_chase(gbl_dbl* A, int n, int ld, gbl_dbl* V2, 
   __syncthreads();
   const bool failed = stat[NW] != 0;
   for (int k = threadIdx.x; k < n; k += NT) {
-    d[k] = failed ? NAN : gld(A, k * ld + k);
-    if (k < n - 1) e[k] = failed ? NAN : gld(A, (k + 1) * ld + k);
+    d[k] = failed ? NAN : A.get(k, k);
+    if (k < n - 1) e[k] = failed ? NAN : A.get(k + 1, k);
   }
   __syncthreads();
+}
+
+template <int NT>
+__device__ __noinline__ void band_chase(gbl_dbl* A, int n, int ld, gbl_dbl* V2, lds_dbl* d,
+                                        lds_dbl* e, lds_int* stat, lds_dbl* band = nullptr)
+{
+  if (band) {
+    for (int x = threadIdx.x; x < 32 * n; x += NT) {
+      const int r = x >> 5, o = x & 31;
+      band[x] = r - o >= 0 ? gld(A, (int64_t)r * ld + (r - o)) : 0.0;
+    }
+    __syncthreads();
+    band_chase_on<NT>(BandLds{band}, n, V2, d, e, stat);
+  } else {
+    band_chase_on<NT>(BandGbl{sgpr_ptr(A), sgpr_int(ld)}, n, V2, d, e, stat);
+  }
 }
 
 // ------------------------------------------------------ back-transforms

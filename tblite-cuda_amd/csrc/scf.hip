@@ -770,7 +770,8 @@ __device__ bool jacobi(double* A, double* CT, int ld, const JRot& R, const JPlan
 template <int NT>
 __device__ __noinline__ void big_band_tridiagonalize(gbl_dbl* A, gbl_dbl* scr, int n, int ld,
                                                      lds_dbl* d, lds_dbl* e, lds_dbl* tile,
-                                                     lds_dbl* sm, lds_dbl* red, PhaseClock& clk)
+                                                     lds_dbl* sm, lds_dbl* red, int region,
+                                                     PhaseClock& clk)
 {
   // stage-1 step clocks in red[16..20] -> the tri_* slots
   if (threadIdx.x == 0)
@@ -788,7 +789,15 @@ __device__ __noinline__ void big_band_tridiagonalize(gbl_dbl* A, gbl_dbl* scr, i
     for (int q = 0; q < 5; ++q) clk.acc[TA_PROF_TRI_COLUMN + q] += (long long)red[16 + q];
   }
   clk.lap(TA_PROF_BAND1);
-  band::band_chase<NT>(A, n, ld, scr + band_v2_offset(ld), d, e, (lds_int*)(sm + 532));
+  // the chase on an LDS copy of the band (32 n doubles after d and e, then
+  // its progress counters) when it fits the eigensolver region (region
+  // doubles from d), else on A
+  if (2 * ld + 32 * n + 8 <= region) {
+    lds_dbl* band = d + 2 * ld;
+    band::band_chase<NT>(A, n, ld, scr + band_v2_offset(ld), d, e, (lds_int*)(band + 32 * n), band);
+  } else {
+    band::band_chase<NT>(A, n, ld, scr + band_v2_offset(ld), d, e, (lds_int*)(sm + 532));
+  }
   clk.lap(TA_PROF_BAND2);
 }
 
@@ -807,7 +816,7 @@ __device__ __noinline__ void big_eigenvalues(double* A_, const double* X_, doubl
                                              double* scr_, int n, int ld, int npart, double* er_,
                                              double* ev_, double* red_, double* scal_,
                                              const double* prev_, double* warm_, bool two_stage,
-                                             PhaseClock& clk)
+                                             int eig_region, PhaseClock& clk)
 {
   gbl_dbl* A = TA_GBL(A_);
   const gbl_dbl* X = TA_GBL(X_);
@@ -854,7 +863,7 @@ __device__ __noinline__ void big_eigenvalues(double* A_, const double* X_, doubl
     // from ld = 96 on; below, they follow the tile (the LDS layout reserves
     // the room, scf_lds_layout)
     lds_dbl* band_sm = ld >= 96 ? er + 2 * ld : er + 8 * ld + band::TILE_DOUBLES;
-    big_band_tridiagonalize<NT>(A, scr, n, ld, d, e, er + 8 * ld, band_sm, red, clk);
+    big_band_tridiagonalize<NT>(A, scr, n, ld, d, e, er + 8 * ld, band_sm, red, eig_region, clk);
   } else {
     // per-wave column partials of the tridiagonalisation share the GEMM tile region
     gbl_dbl* wbt = scr + 3 * (int64_t)ld * TA_BIG_NT;
@@ -1687,7 +1696,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(MATS_LDS ? (
       // levels of the previous step (emo_s) warm-start the bisection; the
       // eigenvectors follow once the occupations are known
       big_eigenvalues<NT>(A, C, Y, scr, n, ld, big_npart(ldmax), smem + L.eig, ev, red, scal,
-                          emo_s, red + 24, two_stage, clk);
+                          emo_s, red + 24, two_stage, L.iint / 2 - L.eig, clk);
       if (b.coeff != nullptr) {
         // keep d, e, tau and |T| for the eigenvectors of a requested wavefunction
         double* sv = scr + big_tri_save_offset(ld);

@@ -39,7 +39,7 @@ PEAK_FP64 = 78.6e12  # MI355X dense FP64 (matrix and vector), FLOP/s
 # commands: profiles/k_scf_config1.md, profiles/k_scf_config3.md.
 MEASURED_TRAFFIC = {
     ("config1", "k_scf<512, 2, 3, 6, true>"): 16.59e9 + 0.93e9,
-    ("config3", "k_scf<512, 1, 0, 0, false>"): 1780.6e9 + 960.8e9,
+    ("config3", "k_scf<512, 1, 0, 0, false>"): 1873.0e9 + 946.8e9,
 }
 
 WORKLOADS = {

@@ -245,8 +245,7 @@ int tblite_amd_singlepoint_flat(tblite_amd_context ctx, tblite_amd_calculator ca
 
 /* Flat batch of periodic cells: as tblite_amd_singlepoint_flat, with
  * lattice(9, nmol) the lattice vectors of every cell (layout of
- * tblite_amd_new_structure_periodic).  Cells run one at a time across the
- * device. */
+ * tblite_amd_new_structure_periodic).  Cells are batched like molecules. */
 int tblite_amd_singlepoint_flat_periodic(tblite_amd_context ctx, tblite_amd_calculator calc,
                                          int nmol, const int* mol_nat, const int* num,
                                          const double* xyz, const double* lattice,

@@ -133,3 +133,60 @@ def test_cell_same_on_every_scf_path(cells, refs, tuning, path):
     assert st[0] == 0
     assert abs(e[0] - refs[1]["energy"]) < 1e-6
     assert abs(int(it[0]) - refs[1]["iterations"]) <= 1
+
+
+def test_mixed_batch_of_cells_and_molecules(engine, orc, cells, refs):
+    """tblite_amd_xtb_singlepoint_batch over periodic and molecular structure
+    handles in one call: each result equals its own reference (cells on the
+    cell kernels, molecules on the molecular ones), wavefunctions included."""
+    import ctypes as C
+    import tblite_amd
+    lib = engine.lib
+    mol = workloads.config0()[0]
+    items = [("cell", cells[0]), ("mol", mol), ("cell", cells[4]), ("mol", mol)]
+    handles, wfns = [], []
+    try:
+        for kind, it in items:
+            num = np.ascontiguousarray(it[0], dtype=np.int32)
+            xyz = np.ascontiguousarray(it[1], dtype=np.float64)
+            if kind == "cell":
+                lat = np.ascontiguousarray(it[2], dtype=np.float64).reshape(9)
+                per = np.ones(3, dtype=np.int32)
+                h = lib.tblite_amd_new_structure_periodic(engine.ctx, len(num), tblite_amd._ip(num),
+                                                          tblite_amd._dp(xyz), 0.0, 0,
+                                                          tblite_amd._dp(lat), tblite_amd._ip(per))
+            else:
+                h = lib.tblite_amd_new_structure(engine.ctx, len(num), tblite_amd._ip(num),
+                                                 tblite_amd._dp(xyz), 0.0, 0)
+            assert h, engine.error()
+            handles.append(h)
+            wfns.append(lib.tblite_amd_new_wavefunction(engine.ctx))
+        n = len(items)
+        marr = (C.c_void_p * n)(*handles)
+        warr = (C.c_void_p * n)(*wfns)
+        e = np.zeros(n)
+        it = np.zeros(n, dtype=np.int32)
+        st = np.zeros(n, dtype=np.int32)
+        print("[gpu] mixed batch of cells and molecules", flush=True)
+        rc = lib.tblite_amd_xtb_singlepoint_batch(engine.ctx, n, marr, engine.calc, warr, 1.0,
+                                                  tblite_amd._dp(e), tblite_amd._ip(it),
+                                                  tblite_amd._ip(st))
+        assert rc == 0, engine.error()
+        ref_mol = orc.singlepoint(*mol)
+        expect = [refs[0], ref_mol, refs[4], ref_mol]
+        for k, r in enumerate(expect):
+            assert st[k] == 0
+            assert abs(e[k] - r["energy"]) < 1e-6, (k, e[k], r["energy"])
+            cnt = C.c_int()
+            assert lib.tblite_amd_get_wavefunction_array(wfns[k], b"qat", None, C.byref(cnt)) == 0
+            q = np.zeros(cnt.value)
+            lib.tblite_amd_get_wavefunction_array(wfns[k], b"qat", tblite_amd._dp(q), C.byref(cnt))
+            assert np.abs(q - r["qat"]).max() < 1e-5
+    finally:
+        for h in handles:
+            hh = C.c_void_p(h)
+            lib.tblite_amd_delete_structure(C.byref(hh))
+        for w in wfns:
+            if w:
+                ww = C.c_void_p(w)
+                lib.tblite_amd_delete_wavefunction(C.byref(ww))

@@ -190,3 +190,21 @@ def test_mixed_batch_of_cells_and_molecules(engine, orc, cells, refs):
             if w:
                 ww = C.c_void_p(w)
                 lib.tblite_amd_delete_wavefunction(C.byref(ww))
+
+
+def test_diamond_cell(engine, orc):
+    """A dense cell of another element: cubic diamond (8 C, a = 3.567 A,
+    9261 translations within the 60 bohr table), against the oracle."""
+    a = 3.567 * workloads.AATOAU
+    frac = np.array([[0, 0, 0], [0, .5, .5], [.5, 0, .5], [.5, .5, 0], [.25, .25, .25],
+                     [.25, .75, .75], [.75, .25, .75], [.75, .75, .25]])
+    xyz = frac * a
+    lat = np.eye(3) * a
+    num = np.full(8, 6, dtype=np.int32)
+    ref = orc.singlepoint(num, xyz, lattice=lat)
+    e, it, st = engine.singlepoint_flat_periodic([8], num, xyz, lat[None])
+    assert st[0] == 0 and ref["rc"] == 0
+    assert abs(e[0] - ref["energy"]) < 1e-6
+    assert abs(int(it[0]) - ref["iterations"]) <= 1
+    ch = engine.charges(0)
+    assert np.abs(ch["qsh"] - ref["qsh"]).max() < 1e-5
